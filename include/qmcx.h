@@ -1,0 +1,134 @@
+/* qmcx.h — C ABI of the MI355X-native VMC particle-move engine (libqmcx.so).
+ *
+ * Plain C types only: opaque handles, host pointers and sizes.  Every call
+ * returns 0 on success or a negative qmcx_status; qmcx_last_error() gives the
+ * message of the last failure on the calling thread.  Host buffers are
+ * borrowed for the duration of the call.  A batch is driven by one HIP stream;
+ * calls on one batch must not overlap.
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the miniqmc source tree).
+ */
+#ifndef QMCX_H
+#define QMCX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qmcx_system_s* qmcx_system_t;
+typedef struct qmcx_batch_s* qmcx_batch_t;
+
+enum qmcx_status {
+  QMCX_OK = 0,
+  QMCX_ERR_ARG = -1,     /* invalid argument or size */
+  QMCX_ERR_HIP = -2,     /* HIP runtime error */
+  QMCX_ERR_STATE = -3,   /* call out of order (e.g. no spline table yet) */
+  QMCX_ERR_NOMEM = -4
+};
+
+enum qmcx_precision {
+  QMCX_FP64 = 0,   /* all-FP64: spline table, distances, Jastrows, determinant */
+  QMCX_FP32 = 1    /* FP32 spline/distance/Jastrow/Woodbury, FP64 LU recompute, log-det and acceptance */
+};
+
+enum qmcx_accept_mode {
+  QMCX_ACCEPT_REFERENCE = 0,  /* accept when u < 0.5 (src/Drivers/miniqmc.cpp:445) */
+  QMCX_ACCEPT_METROPOLIS = 1  /* accept when u < |Psi'/Psi|^2 */
+};
+
+const char* qmcx_last_error(void);
+int qmcx_device_count(int* count);
+
+/* ---- system: lattice, ions, Jastrow functors, SPO table --------------------
+ * replaces build_ions/tile_cell (src/Particle/ParticleSet_builder.cpp:23,
+ * src/Input/nio.hpp:23), buildJ1/buildJ2 (src/Input/nio.hpp:82,127) and
+ * build_SPOSet + einspline_spo::set (src/QMCWaveFunctions/SPOSet_builder.cpp:19,
+ * src/QMCWaveFunctions/einspline_spo.hpp:128-160).
+ * lattice9: rows are the cell vectors (bohr).  ion_pos: [nions][3] Cartesian,
+ * grouped by species.  nup == ndn is required. */
+int qmcx_system_create(qmcx_system_t* out, const double* lattice9, int nions, const double* ion_pos,
+                       const int* ion_species, int nspecies, int nup, int ndn);
+int qmcx_system_destroy(qmcx_system_t sys);
+
+/* BsplineFunctor::setupParameters (src/QMCWaveFunctions/Jastrow/BsplineFunctor.h:120-130)
+ * for the J1 functor of one ion species (OneBodyJastrow::addFunc, OneBodyJastrow.h:101-106). */
+int qmcx_system_set_j1(qmcx_system_t sys, int species, int nparams, double rcut, double cusp, const double* params);
+/* J2 functor of a spin pair: 0 = same spin (uu/dd), 1 = opposite spin (ud/du)
+ * (TwoBodyJastrow::addFunc, TwoBodyJastrow.h:204-240). */
+int qmcx_system_set_j2(qmcx_system_t sys, int pair, int nparams, double rcut, double cusp, const double* params);
+
+/* Spline table upload: coefs is [ng0+3][ng1+3][ng2+3][npad] FP32, orbitals
+ * innermost (BsplineAllocator::allocateMultiBspline layout,
+ * src/Numerics/Spline2/BsplineAllocator.hpp:84-136).  norb must equal nup.
+ * spo_lattice9 is the lattice the SPO maps positions with (einspline_spo::Lattice).
+ * precision_mask: bit0 keeps an FP32 device table, bit1 an FP64 copy. */
+int qmcx_system_set_spline(qmcx_system_t sys, const int* ng, int norb, int npad, const float* coefs,
+                           const double* spo_lattice9, int precision_mask);
+
+/* ---- stand-alone kernels (parity surface) ----------------------------------
+ * einspline_spo::evaluate_vgh -> MultiBsplineEval::evaluate_vgh
+ * (src/QMCWaveFunctions/einspline_spo.hpp:194-202, src/Numerics/Spline2/MultiBspline.hpp:163-280):
+ * npts Cartesian points -> out[npts][10][norb] = v, gx, gy, gz, hxx, hxy, hxz, hyy, hyz, hzz. */
+int qmcx_spline_vgh(qmcx_system_t sys, int precision, int npts, const double* r, double* out);
+
+/* DTD_BConds::computeDistances (src/Particle/Lattice/ParticleBConds.h:154-217): rows from
+ * npts sources to the nt targets, flip index per source; out r[npts][nt], dr[npts][nt][3]. */
+int qmcx_distance_rows(qmcx_system_t sys, int precision, int npts, const double* src, const int* flip_ind,
+                       int nt, const double* targets, double* r, double* dr);
+
+/* DiracMatrix::invert_transpose (src/QMCWaveFunctions/DiracMatrix.h:93-121), batched FP64 LU with
+ * partial pivoting: a[nmat][n][n] (rows = electrons) -> psiM[nmat][n][n] = (a^-1)^T,
+ * logdet[nmat] = log|det a|, phase[nmat] in {0, pi}. */
+int qmcx_invert_transpose(int n, int nmat, const double* a, double* psiM, double* logdet, double* phase);
+
+/* ---- walker batch -----------------------------------------------------------
+ * Device-resident walker state for nwalkers walkers (ParticleSet + WaveFunction
+ * per Mover, src/Drivers/Mover.hpp:46-64; DiracDeterminant<DU_TYPE>,
+ * src/QMCWaveFunctions/DiracDeterminant.h:35-165, with the DelayedUpdate engine
+ * src/QMCWaveFunctions/DelayedUpdate.h:29-232 as its DU_TYPE).
+ * delay_rank k >= 1 (k = 1 is Sherman-Morrison). */
+int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t sys, int nwalkers, int delay_rank, int precision);
+int qmcx_batch_destroy(qmcx_batch_t b);
+int qmcx_batch_set_positions(qmcx_batch_t b, const double* R /* [nw][N][3] */);
+int qmcx_batch_get_positions(qmcx_batch_t b, double* R);
+/* global walker index of walker 0 of this batch (selects the random streams) */
+int qmcx_batch_set_walker_offset(qmcx_batch_t b, int walker0);
+/* spline stencil slices per workgroup (autotuner knob; 0 = default) */
+int qmcx_batch_set_tile(qmcx_batch_t b, int spline_slices);
+
+/* WaveFunction::evaluateLog (src/QMCWaveFunctions/WaveFunction.cpp:178-195): SPO for all
+ * electrons, FP64 LU, J1/J2 from scratch.  log_out[nw] (may be NULL). */
+int qmcx_evaluate_log(qmcx_batch_t b, double* log_out);
+
+/* The VMC sweep (src/Drivers/miniqmc.cpp:422-467 and the flex_* batched form,
+ * src/Drivers/miniqmc_sync_move.cpp:459-513): for every electron setActive,
+ * evalGrad, makeMove(sqrt(tau) * N(0,1)), ratioGrad, accept/reject, then
+ * completeUpdates and evaluateGL.  Random numbers: Philox4x64-10 keyed by seed,
+ * counter (electron, sweep, walker).  recompute_every > 0 refreshes every
+ * inverse with FP64 LU after every that many sweeps.
+ * log_out [nsweeps][nw] LogValue after each sweep (may be NULL). */
+int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int accept_mode, int recompute_every,
+                    double* log_out);
+
+/* Per-move record of the most recent sweep, [nw][N][9]:
+ * ratio, det ratio, accepted, grad_now[3], grad_new[3].  enable must be set before the sweep. */
+int qmcx_set_trace(qmcx_batch_t b, int enable);
+int qmcx_get_trace(qmcx_batch_t b, double* trace);
+
+/* Outputs of the last evaluateGL: G[nw][N][3], L[nw][N], log[nw]; per-walker accept counts. */
+int qmcx_get_gl(qmcx_batch_t b, double* G, double* L, double* log_out);
+int qmcx_get_accepts(qmcx_batch_t b, long long* naccept);
+
+/* Inverse matrix psiM of one determinant of one walker as FP64, [n][n] (DiracDeterminant::psiM). */
+int qmcx_get_inverse(qmcx_batch_t b, int walker, int det, double* psiM);
+
+/* Device time of the last qmcx_run_sweeps (HIP events on the batch stream), milliseconds. */
+int qmcx_last_sweep_ms(qmcx_batch_t b, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,794 @@
+// C ABI of the VMC particle-move engine: host-side orchestration of the
+// kernels in qmcx_kernels.hpp.  See include/qmcx.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/qmcx.h"
+#include "qmcx_kernels.hpp"
+
+using namespace qmcx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) return fail(QMCX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ------------------------------------------------------------------ lattice (host)
+void inv3(const double a[9], double o[9]) {
+  const double c00 = a[4] * a[8] - a[5] * a[7];
+  const double c01 = a[5] * a[6] - a[3] * a[8];
+  const double c02 = a[3] * a[7] - a[4] * a[6];
+  const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+  const double id = 1.0 / det;
+  o[0] = c00 * id;
+  o[3] = c01 * id;
+  o[6] = c02 * id;
+  o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+  o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+  o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+  o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+  o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+  o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Replace the longest basis vector by a shorter +-combination when one exists
+// (LatticeAnalyzer.h:134-158); repeated until no change (:160-184).
+bool shorten(double rb[9]) {
+  const double eps = 10.0 * std::numeric_limits<float>::epsilon();
+  int imax = 0;
+  double r2max = dot3(rb, rb);
+  for (int i = 1; i < 3; ++i) {
+    const double r2 = dot3(rb + 3 * i, rb + 3 * i);
+    if (r2 - r2max > eps) {
+      r2max = r2;
+      imax = i;
+    }
+  }
+  double cand[4][3];
+  for (int d = 0; d < 3; ++d) {
+    cand[0][d] = rb[d] + rb[3 + d] - rb[6 + d];
+    cand[1][d] = rb[d] + rb[6 + d] - rb[3 + d];
+    cand[2][d] = rb[3 + d] + rb[6 + d] - rb[d];
+    cand[3][d] = rb[d] + rb[3 + d] + rb[6 + d];
+  }
+  for (int i = 0; i < 4; ++i)
+    if (dot3(cand[i], cand[i]) - r2max < -eps) {
+      for (int d = 0; d < 3; ++d) rb[3 * imax + d] = cand[i][d];
+      return true;
+    }
+  return false;
+}
+void reduce_basis(double rb[9]) {
+  for (;;) {
+    double saved[9];
+    std::memcpy(saved, rb, sizeof(saved));
+    bool changed = false;
+    for (int i = 0; i < 3; ++i) {
+      rb[3 * i] = rb[3 * i + 1] = rb[3 * i + 2] = 0.0;
+      changed = shorten(rb);
+      for (int d = 0; d < 3; ++d) rb[3 * i + d] = saved[3 * i + d];
+      if (changed) break;
+    }
+    if (!changed && !shorten(rb)) return;
+  }
+}
+
+template <typename T>
+Cell<T> make_cell(const double lat[9]) {
+  double rb[9], g[9];
+  std::memcpy(rb, lat, sizeof(rb));
+  reduce_basis(rb);
+  inv3(rb, g);
+  Cell<T> C;
+  for (int i = 0; i < 9; ++i) {
+    C.g[i] = (T)g[i];
+    C.rb[i] = (T)rb[i];
+  }
+  const double* b0 = rb;
+  const double* b1 = rb + 3;
+  const double* b2 = rb + 6;
+  for (int d = 0; d < 3; ++d) {
+    C.corner[0 + d] = T(0);
+    C.corner[3 + d] = (T)(-b0[d]);
+    C.corner[6 + d] = (T)(-b1[d]);
+    C.corner[9 + d] = (T)(-b2[d]);
+    C.corner[12 + d] = (T)(-(b0[d] + b1[d]));
+    C.corner[15 + d] = (T)(-(b0[d] + b2[d]));
+    C.corner[18 + d] = (T)(-(b1[d] + b2[d]));
+    C.corner[21 + d] = (T)(-(b0[d] + b1[d] + b2[d]));
+  }
+  return C;
+}
+
+// radial functor coefficients (BsplineFunctor.h:93-130)
+template <typename T>
+Functor<T> make_functor(int np, double rcut, double cusp, const double* p) {
+  Functor<T> f;
+  std::memset(&f, 0, sizeof(f));
+  const double dr = rcut / (double)(np + 1);
+  std::vector<double> c(np + 4, 0.0);
+  c[1] = p[0];
+  c[2] = p[1];
+  c[0] = p[1] - 2.0 * dr * cusp;
+  for (int i = 2; i < np; ++i) c[i + 1] = p[i];
+  f.rcut = (T)rcut;
+  f.drinv = (T)(1.0 / dr);
+  f.ncoef = np + 4;
+  for (int i = 0; i < np + 4; ++i) f.coef[i] = (T)c[i];
+  return f;
+}
+
+}  // namespace
+
+// ====================================================================== handles
+struct qmcx_system_s {
+  int dev = 0;
+  double lattice[9];
+  int nions = 0, nspecies = 0, nup = 0, ndn = 0, N = 0, n = 0;
+  double* d_ions = nullptr;
+  int* d_species = nullptr;
+  Functor<double> fnd[kMaxSpecies + 2];
+  Functor<float> fnf[kMaxSpecies + 2];
+  Functor<double>* d_fnd = nullptr;
+  Functor<float>* d_fnf = nullptr;
+  bool fn_dirty = true;
+  int ng[3] = {0, 0, 0};
+  int norb = 0, npad = 0;
+  double spoG[9];
+  float* d_coef_f = nullptr;
+  double* d_coef_d = nullptr;
+
+  template <typename T>
+  SysP<T> params() const {
+    SysP<T> S;
+    S.cell = make_cell<T>(lattice);
+    std::memcpy(S.spoG, spoG, sizeof(spoG));
+    S.ng[0] = ng[0];
+    S.ng[1] = ng[1];
+    S.ng[2] = ng[2];
+    S.N = N;
+    S.n = n;
+    S.M = nions;
+    S.nspecies = nspecies;
+    S.npad = npad;
+    S.zs = npad;
+    S.ys = (long long)(ng[2] + 3) * npad;
+    S.xs = (long long)(ng[1] + 3) * S.ys;
+    S.ions = d_ions;
+    S.species = d_species;
+    if constexpr (sizeof(T) == 4) {
+      S.coefs = d_coef_f;
+      S.fn = d_fnf;
+    } else {
+      S.coefs = d_coef_d;
+      S.fn = d_fnd;
+    }
+    return S;
+  }
+  int sync_functors() {
+    if (!fn_dirty) return 0;
+    HIPCHK(hipMemcpy(d_fnd, fnd, sizeof(fnd), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_fnf, fnf, sizeof(fnf), hipMemcpyHostToDevice));
+    fn_dirty = false;
+    return 0;
+  }
+};
+
+struct qmcx_batch_s {
+  qmcx_system_s* sys = nullptr;
+  int W = 0, K = 1, prec = QMCX_FP64, walker0 = 0, slices = 0, nt = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_ms = 0.f;
+  std::vector<void*> allocs;
+  double* R = nullptr;
+  void* A = nullptr;
+  void* Ainv = nullptr;
+  void* dpsiM = nullptr;
+  void* d2psiM = nullptr;
+  void* J2 = nullptr;
+  void* J1 = nullptr;
+  void* Vbuf = nullptr;
+  void* Binv = nullptr;
+  int* rows = nullptr;
+  int* ndelay = nullptr;
+  double* logdet = nullptr;
+  double* phase = nullptr;
+  double* jlog = nullptr;
+  unsigned long long* nacc = nullptr;
+  double* trace = nullptr;
+  int* sweep = nullptr;
+  double* G = nullptr;
+  double* L = nullptr;
+  double* logv = nullptr;
+  double* hist = nullptr;
+  int hist_cap = 0;
+  double* lu_ws = nullptr;
+  bool trace_on = false;
+
+  template <typename T>
+  BatchP<T> params() const {
+    BatchP<T> B;
+    B.W = W;
+    B.K = K;
+    B.R = R;
+    B.A = (T*)A;
+    B.Ainv = (T*)Ainv;
+    B.dpsiM = (T*)dpsiM;
+    B.d2psiM = (T*)d2psiM;
+    B.J2 = (T*)J2;
+    B.J1 = (T*)J1;
+    B.Vbuf = (T*)Vbuf;
+    B.Binv = (T*)Binv;
+    B.rows = rows;
+    B.ndelay = ndelay;
+    B.logdet = logdet;
+    B.phase = phase;
+    B.jlog = jlog;
+    B.nacc = nacc;
+    B.trace = trace_on ? trace : nullptr;
+    B.sweep = sweep;
+    return B;
+  }
+  template <typename P>
+  int alloc(P*& p, size_t count) {
+    void* q = nullptr;
+    HIPCHK(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(P)));
+    HIPCHK(hipMemset(q, 0, std::max<size_t>(count, 1) * sizeof(P)));
+    allocs.push_back(q);
+    p = (P*)q;
+    return 0;
+  }
+};
+
+namespace {
+
+// workgroup size and spline slices for the move kernel
+template <typename T>
+int choose_shape(const qmcx_system_s* s, int want_slices, int& nt, int& slices) {
+  const int vw = 16 / (int)sizeof(T);
+  const int nq = s->npad / vw;
+  // at least one lane per electron (two electrons per lane at most)
+  int base = std::max(256, ((s->N + 1) / 2 + 63) / 64 * 64);
+  slices = want_slices;
+  if (slices <= 0) {
+    slices = 1;
+    while (slices < 16 && nq * slices * 2 <= 1024) slices *= 2;
+  }
+  if (16 % slices != 0 || nq * slices > 1024) return fail(QMCX_ERR_ARG, "spline tile does not fit a workgroup");
+  nt = std::max(base, (nq * slices + 63) / 64 * 64);
+  if (nt > 1024) return fail(QMCX_ERR_ARG, "workgroup too large for this system");
+  return 0;
+}
+
+template <typename T>
+int spline_slices_for(const qmcx_system_s* s, int nt) {
+  const int vw = 16 / (int)sizeof(T);
+  const int nq = s->npad / vw;
+  int S = 1;
+  while (S < 16 && nq * S * 2 <= nt) S *= 2;
+  return S;
+}
+
+template <typename T>
+int launch_spo_rows(qmcx_batch_t b) {
+  const qmcx_system_s* s = b->sys;
+  const int vw = 16 / (int)sizeof(T);
+  const int nq = s->npad / vw;
+  const int slices = spline_slices_for<T>(s, 1024);
+  const int nt = std::max(64, (nq * slices + 63) / 64 * 64);
+  const size_t lds = sizeof(T) * ((size_t)slices * 5 * s->npad + 5 * (size_t)s->n);
+  hipLaunchKernelGGL(k_spo_rows<T>, dim3(s->N, b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(),
+                     slices);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_lu_batch(qmcx_batch_t b) {
+  const qmcx_system_s* s = b->sys;
+  const int n = s->n;
+  const size_t lds = ((n * 4 + 15) & ~15) + 16 * 8 + 16 * 4;
+  hipLaunchKernelGGL((k_lu_invert<T, T>), dim3(b->W * 2), dim3(1024), lds, b->stream, n, (const T*)b->A, (T*)b->Ainv,
+                     b->lu_ws, b->logdet, b->phase, 1);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_gl(qmcx_batch_t b, bool record) {
+  const qmcx_system_s* s = b->sys;
+  hipLaunchKernelGGL(k_det_gl<T>, dim3((s->N + 3) / 4, b->W), dim3(256), 0, b->stream, s->params<T>(),
+                     b->params<T>(), b->G, b->L);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_finish_gl<T>, dim3(b->W), dim3(256), 0, b->stream, s->params<T>(), b->params<T>(), b->G, b->L,
+                     b->logv, record ? b->hist : nullptr, b->hist_cap);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int evaluate_log_impl(qmcx_batch_t b) {
+  const qmcx_system_s* s = b->sys;
+  if (launch_spo_rows<T>(b)) return QMCX_ERR_HIP;
+  if (launch_lu_batch<T>(b)) return QMCX_ERR_HIP;
+  const size_t lds = sizeof(double) * (3 * (size_t)s->N + 32) + sizeof(Functor<T>) * (kMaxSpecies + 2);
+  hipLaunchKernelGGL(k_jastrow_init<T>, dim3(b->W), dim3(1024), lds, b->stream, s->params<T>(), b->params<T>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemsetAsync(b->ndelay, 0, sizeof(int) * b->W, b->stream));
+  return launch_gl<T>(b, false);
+}
+
+template <typename T>
+int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode) {
+  const qmcx_system_s* s = b->sys;
+  int nt = 0, slices = 0;
+  if (choose_shape<T>(s, b->slices, nt, slices)) return QMCX_ERR_ARG;
+  const MoveLds L = move_lds<T>(s->N, s->nions, s->n, b->K, slices, s->npad);
+  MoveArgs a;
+  a.mode = mode;
+  a.slices = slices;
+  a.walker0 = b->walker0;
+  a.sqrt_tau = std::sqrt(tau);
+  a.seed = seed;
+  const int chunk = b->K == 1 ? s->n : b->K;
+  const size_t flds = sizeof(T) * 2 * kFlushRows * b->K + sizeof(int) * b->K;
+  for (int det = 0; det < 2; ++det) {
+    for (int e = 0; e < s->n; e += chunk) {
+      a.det = det;
+      a.e_begin = det * s->n + e;
+      a.e_end = det * s->n + std::min(s->n, e + chunk);
+      hipLaunchKernelGGL(k_moves<T>, dim3(b->W), dim3(nt), L.total, b->stream, s->params<T>(), b->params<T>(), a);
+      HIPCHK(hipGetLastError());
+      if (b->K > 1) {
+        hipLaunchKernelGGL(k_flush_generic<T>, dim3((s->n + kFlushRows - 1) / kFlushRows, b->W), dim3(256), flds,
+                           b->stream, s->params<T>(), b->params<T>(), det);
+        HIPCHK(hipGetLastError());
+      }
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* qmcx_last_error(void) { return g_err.c_str(); }
+
+int qmcx_device_count(int* count) {
+  if (!count) return fail(QMCX_ERR_ARG, "count is null");
+  HIPCHK(hipGetDeviceCount(count));
+  return 0;
+}
+
+int qmcx_system_create(qmcx_system_t* out, const double* lattice9, int nions, const double* ion_pos,
+                       const int* ion_species, int nspecies, int nup, int ndn) {
+  if (!out || !lattice9 || nions <= 0 || !ion_pos || !ion_species) return fail(QMCX_ERR_ARG, "null argument");
+  if (nspecies <= 0 || nspecies > kMaxSpecies) return fail(QMCX_ERR_ARG, "nspecies out of range");
+  if (nup != ndn || nup <= 0) return fail(QMCX_ERR_ARG, "nup must equal ndn");
+  for (int i = 0; i < nions; ++i)
+    if (ion_species[i] < 0 || ion_species[i] >= nspecies) return fail(QMCX_ERR_ARG, "ion species out of range");
+  auto* s = new qmcx_system_s;
+  HIPCHK(hipGetDevice(&s->dev));
+  std::memcpy(s->lattice, lattice9, sizeof(s->lattice));
+  s->nions = nions;
+  s->nspecies = nspecies;
+  s->nup = nup;
+  s->ndn = ndn;
+  s->N = nup + ndn;
+  s->n = nup;
+  std::vector<double> soa(3 * (size_t)nions);
+  for (int i = 0; i < nions; ++i)
+    for (int d = 0; d < 3; ++d) soa[(size_t)d * nions + i] = ion_pos[3 * i + d];
+  std::memset(s->fnd, 0, sizeof(s->fnd));
+  std::memset(s->fnf, 0, sizeof(s->fnf));
+  HIPCHK(hipMalloc(&s->d_ions, sizeof(double) * soa.size()));
+  HIPCHK(hipMemcpy(s->d_ions, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&s->d_species, sizeof(int) * nions));
+  HIPCHK(hipMemcpy(s->d_species, ion_species, sizeof(int) * nions, hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&s->d_fnd, sizeof(s->fnd)));
+  HIPCHK(hipMalloc(&s->d_fnf, sizeof(s->fnf)));
+  *out = s;
+  return 0;
+}
+
+int qmcx_system_destroy(qmcx_system_t s) {
+  if (!s) return 0;
+  hipFree(s->d_ions);
+  hipFree(s->d_species);
+  hipFree(s->d_fnd);
+  hipFree(s->d_fnf);
+  hipFree(s->d_coef_f);
+  hipFree(s->d_coef_d);
+  delete s;
+  return 0;
+}
+
+static int set_functor(qmcx_system_t s, int slot, int nparams, double rcut, double cusp, const double* params) {
+  if (!s || !params) return fail(QMCX_ERR_ARG, "null argument");
+  if (nparams < 2 || nparams + 4 > kMaxFunctorCoefs) return fail(QMCX_ERR_ARG, "nparams out of range");
+  if (!(rcut > 0)) return fail(QMCX_ERR_ARG, "rcut must be positive");
+  s->fnd[slot] = make_functor<double>(nparams, rcut, cusp, params);
+  s->fnf[slot] = make_functor<float>(nparams, rcut, cusp, params);
+  s->fn_dirty = true;
+  return 0;
+}
+
+int qmcx_system_set_j1(qmcx_system_t s, int species, int nparams, double rcut, double cusp, const double* params) {
+  if (!s || species < 0 || species >= s->nspecies) return fail(QMCX_ERR_ARG, "species out of range");
+  return set_functor(s, species, nparams, rcut, cusp, params);
+}
+
+int qmcx_system_set_j2(qmcx_system_t s, int pair, int nparams, double rcut, double cusp, const double* params) {
+  if (!s || pair < 0 || pair > 1) return fail(QMCX_ERR_ARG, "pair must be 0 (same spin) or 1 (opposite spin)");
+  return set_functor(s, kMaxSpecies + pair, nparams, rcut, cusp, params);
+}
+
+int qmcx_system_set_spline(qmcx_system_t s, const int* ng, int norb, int npad, const float* coefs,
+                           const double* spo_lattice9, int precision_mask) {
+  if (!s || !ng || !coefs || !spo_lattice9) return fail(QMCX_ERR_ARG, "null argument");
+  if (norb != s->n) return fail(QMCX_ERR_ARG, "norb must equal the number of electrons per spin");
+  if (npad < norb || npad % 4 != 0) return fail(QMCX_ERR_ARG, "npad must be >= norb and a multiple of 4");
+  if (ng[0] < 1 || ng[1] < 1 || ng[2] < 1) return fail(QMCX_ERR_ARG, "grid must be positive");
+  if ((precision_mask & 3) == 0) return fail(QMCX_ERR_ARG, "precision_mask selects no table");
+  s->ng[0] = ng[0];
+  s->ng[1] = ng[1];
+  s->ng[2] = ng[2];
+  s->norb = norb;
+  s->npad = npad;
+  inv3(spo_lattice9, s->spoG);
+  const size_t cnt = (size_t)(ng[0] + 3) * (ng[1] + 3) * (ng[2] + 3) * npad;
+  hipFree(s->d_coef_f);
+  hipFree(s->d_coef_d);
+  s->d_coef_f = nullptr;
+  s->d_coef_d = nullptr;
+  HIPCHK(hipMalloc(&s->d_coef_f, sizeof(float) * cnt));
+  HIPCHK(hipMemcpy(s->d_coef_f, coefs, sizeof(float) * cnt, hipMemcpyHostToDevice));
+  if (precision_mask & 2) {
+    HIPCHK(hipMalloc(&s->d_coef_d, sizeof(double) * cnt));
+    // widen on the device: FP32 values are exact in FP64
+    const size_t chunk = 1 << 24;
+    std::vector<double> tmp(std::min(cnt, chunk));
+    for (size_t o = 0; o < cnt; o += chunk) {
+      const size_t m = std::min(chunk, cnt - o);
+      for (size_t i = 0; i < m; ++i) tmp[i] = (double)coefs[o + i];
+      HIPCHK(hipMemcpy(s->d_coef_d + o, tmp.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+    }
+  }
+  if (!(precision_mask & 1)) {
+    hipFree(s->d_coef_f);
+    s->d_coef_f = nullptr;
+  }
+  return 0;
+}
+
+int qmcx_spline_vgh(qmcx_system_t s, int precision, int npts, const double* r, double* out) {
+  if (!s || !r || !out || npts <= 0) return fail(QMCX_ERR_ARG, "bad argument");
+  const bool f32 = precision == QMCX_FP32;
+  if ((f32 && !s->d_coef_f) || (!f32 && !s->d_coef_d)) return fail(QMCX_ERR_STATE, "no spline table of that precision");
+  double *dr = nullptr, *dout = nullptr;
+  HIPCHK(hipMalloc(&dr, sizeof(double) * 3 * npts));
+  HIPCHK(hipMalloc(&dout, sizeof(double) * 10 * (size_t)npts * s->norb));
+  HIPCHK(hipMemcpy(dr, r, sizeof(double) * 3 * npts, hipMemcpyHostToDevice));
+  if (f32) {
+    const int slices = spline_slices_for<float>(s, 1024);
+    const int nt = std::max(64, (s->npad / 4 * slices + 63) / 64 * 64);
+    hipLaunchKernelGGL(k_spline_vgh<float>, dim3(npts), dim3(nt), sizeof(float) * slices * 10 * s->npad, 0,
+                       s->params<float>(), dr, dout, s->norb, slices);
+  } else {
+    const int slices = spline_slices_for<double>(s, 1024);
+    const int nt = std::max(64, (s->npad / 2 * slices + 63) / 64 * 64);
+    hipLaunchKernelGGL(k_spline_vgh<double>, dim3(npts), dim3(nt), sizeof(double) * slices * 10 * s->npad, 0,
+                       s->params<double>(), dr, dout, s->norb, slices);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(out, dout, sizeof(double) * 10 * (size_t)npts * s->norb, hipMemcpyDeviceToHost));
+  hipFree(dr);
+  hipFree(dout);
+  return 0;
+}
+
+int qmcx_distance_rows(qmcx_system_t s, int precision, int npts, const double* src, const int* flip_ind, int nt,
+                       const double* targets, double* r, double* dr) {
+  if (!s || !src || !flip_ind || !targets || !r || !dr || npts <= 0 || nt <= 0) return fail(QMCX_ERR_ARG, "bad argument");
+  double *d_src, *d_tgt, *d_r, *d_dr;
+  int* d_flip;
+  const size_t np = (size_t)npts * nt;
+  HIPCHK(hipMalloc(&d_src, sizeof(double) * 3 * npts));
+  HIPCHK(hipMalloc(&d_tgt, sizeof(double) * 3 * nt));
+  HIPCHK(hipMalloc(&d_r, sizeof(double) * np));
+  HIPCHK(hipMalloc(&d_dr, sizeof(double) * 3 * np));
+  HIPCHK(hipMalloc(&d_flip, sizeof(int) * npts));
+  HIPCHK(hipMemcpy(d_src, src, sizeof(double) * 3 * npts, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_tgt, targets, sizeof(double) * 3 * nt, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_flip, flip_ind, sizeof(int) * npts, hipMemcpyHostToDevice));
+  const int blocks = (int)((np + 255) / 256);
+  if (precision == QMCX_FP32)
+    hipLaunchKernelGGL(k_distance_rows<float>, dim3(blocks), dim3(256), 0, 0, make_cell<float>(s->lattice), npts, d_src,
+                       d_flip, nt, d_tgt, d_r, d_dr);
+  else
+    hipLaunchKernelGGL(k_distance_rows<double>, dim3(blocks), dim3(256), 0, 0, make_cell<double>(s->lattice), npts,
+                       d_src, d_flip, nt, d_tgt, d_r, d_dr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(r, d_r, sizeof(double) * np, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(dr, d_dr, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
+  hipFree(d_src);
+  hipFree(d_tgt);
+  hipFree(d_r);
+  hipFree(d_dr);
+  hipFree(d_flip);
+  return 0;
+}
+
+int qmcx_invert_transpose(int n, int nmat, const double* a, double* psiM, double* logdet, double* phase) {
+  if (n <= 0 || n > 1024 || nmat <= 0 || !a || !psiM || !logdet || !phase) return fail(QMCX_ERR_ARG, "bad argument");
+  const size_t nn = (size_t)n * n * nmat;
+  double *d_a, *d_p, *d_ws, *d_ld, *d_ph;
+  HIPCHK(hipMalloc(&d_a, sizeof(double) * nn));
+  HIPCHK(hipMalloc(&d_p, sizeof(double) * nn));
+  HIPCHK(hipMalloc(&d_ws, sizeof(double) * 2 * nn));
+  HIPCHK(hipMalloc(&d_ld, sizeof(double) * nmat));
+  HIPCHK(hipMalloc(&d_ph, sizeof(double) * nmat));
+  HIPCHK(hipMemcpy(d_a, a, sizeof(double) * nn, hipMemcpyHostToDevice));
+  const size_t lds = ((n * 4 + 15) & ~15) + 16 * 8 + 16 * 4;
+  hipLaunchKernelGGL((k_lu_invert<double, double>), dim3(nmat), dim3(1024), lds, 0, n, d_a, d_p, d_ws, d_ld, d_ph, 1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(psiM, d_p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(logdet, d_ld, sizeof(double) * nmat, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(phase, d_ph, sizeof(double) * nmat, hipMemcpyDeviceToHost));
+  hipFree(d_a);
+  hipFree(d_p);
+  hipFree(d_ws);
+  hipFree(d_ld);
+  hipFree(d_ph);
+  return 0;
+}
+
+int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int delay_rank, int precision) {
+  if (!out || !s || nwalkers <= 0) return fail(QMCX_ERR_ARG, "bad argument");
+  if (precision != QMCX_FP32 && precision != QMCX_FP64) return fail(QMCX_ERR_ARG, "unknown precision");
+  if (delay_rank < 1 || delay_rank > 128 || delay_rank > s->n) return fail(QMCX_ERR_ARG, "delay rank out of range");
+  if (s->npad == 0) return fail(QMCX_ERR_STATE, "spline table not set");
+  if (precision == QMCX_FP32 && !s->d_coef_f) return fail(QMCX_ERR_STATE, "no FP32 spline table");
+  if (precision == QMCX_FP64 && !s->d_coef_d) return fail(QMCX_ERR_STATE, "no FP64 spline table");
+  if (s->N > 2048) return fail(QMCX_ERR_ARG, "at most 2048 electrons per walker");
+  if (int rc = s->sync_functors()) return rc;
+  auto* b = new qmcx_batch_s;
+  b->sys = s;
+  b->W = nwalkers;
+  b->K = delay_rank;
+  b->prec = precision;
+  const size_t es = precision == QMCX_FP32 ? 4 : 8;
+  const size_t W = nwalkers, n = s->n, N = s->N, K = delay_rank;
+  // LDS check for the move kernel
+  {
+    int nt, slices;
+    int rc = precision == QMCX_FP32 ? choose_shape<float>(s, 0, nt, slices) : choose_shape<double>(s, 0, nt, slices);
+    if (rc) {
+      delete b;
+      return rc;
+    }
+    const size_t lds = precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, slices, s->npad).total
+                                              : move_lds<double>(s->N, s->nions, s->n, delay_rank, slices, s->npad).total;
+    if (lds > 160 * 1024) {
+      delete b;
+      return fail(QMCX_ERR_ARG, "walker state does not fit in LDS (reduce delay rank)");
+    }
+  }
+  int rc = 0;
+  unsigned char* p8 = nullptr;
+  rc |= b->alloc(b->R, 3 * W * N);
+  rc |= b->alloc(p8, es * W * 2 * n * n);
+  b->A = p8;
+  rc |= b->alloc(p8, es * W * 2 * n * n);
+  b->Ainv = p8;
+  rc |= b->alloc(p8, es * W * 2 * n * 3 * n);
+  b->dpsiM = p8;
+  rc |= b->alloc(p8, es * W * 2 * n * n);
+  b->d2psiM = p8;
+  rc |= b->alloc(p8, es * W * 5 * N);
+  b->J2 = p8;
+  rc |= b->alloc(p8, es * W * 5 * N);
+  b->J1 = p8;
+  rc |= b->alloc(p8, es * W * K * n);
+  b->Vbuf = p8;
+  rc |= b->alloc(p8, es * W * K * K);
+  b->Binv = p8;
+  rc |= b->alloc(b->rows, W * K);
+  rc |= b->alloc(b->ndelay, W);
+  rc |= b->alloc(b->logdet, W * 2);
+  rc |= b->alloc(b->phase, W * 2);
+  rc |= b->alloc(b->jlog, W * 2);
+  rc |= b->alloc(b->nacc, W);
+  rc |= b->alloc(b->sweep, 2);
+  rc |= b->alloc(b->G, W * N * 3);
+  rc |= b->alloc(b->L, W * N);
+  rc |= b->alloc(b->logv, W);
+  rc |= b->alloc(b->lu_ws, W * 2 * 2 * n * n);
+  if (rc) {
+    qmcx_batch_destroy(b);
+    return QMCX_ERR_NOMEM;
+  }
+  HIPCHK(hipStreamCreate(&b->stream));
+  HIPCHK(hipEventCreate(&b->ev0));
+  HIPCHK(hipEventCreate(&b->ev1));
+  *out = b;
+  return 0;
+}
+
+int qmcx_batch_destroy(qmcx_batch_t b) {
+  if (!b) return 0;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  for (void* p : b->allocs) hipFree(p);
+  if (b->trace) hipFree(b->trace);
+  if (b->hist) hipFree(b->hist);
+  if (b->stream) hipStreamDestroy(b->stream);
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  delete b;
+  return 0;
+}
+
+int qmcx_batch_set_positions(qmcx_batch_t b, const double* R) {
+  if (!b || !R) return fail(QMCX_ERR_ARG, "null argument");
+  const int N = b->sys->N;
+  std::vector<double> soa(3 * (size_t)N * b->W);
+  for (int w = 0; w < b->W; ++w)
+    for (int i = 0; i < N; ++i)
+      for (int d = 0; d < 3; ++d) soa[((size_t)w * 3 + d) * N + i] = R[((size_t)w * N + i) * 3 + d];
+  HIPCHK(hipMemcpy(b->R, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int qmcx_batch_get_positions(qmcx_batch_t b, double* R) {
+  if (!b || !R) return fail(QMCX_ERR_ARG, "null argument");
+  const int N = b->sys->N;
+  std::vector<double> soa(3 * (size_t)N * b->W);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(soa.data(), b->R, sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
+  for (int w = 0; w < b->W; ++w)
+    for (int i = 0; i < N; ++i)
+      for (int d = 0; d < 3; ++d) R[((size_t)w * N + i) * 3 + d] = soa[((size_t)w * 3 + d) * N + i];
+  return 0;
+}
+
+int qmcx_batch_set_walker_offset(qmcx_batch_t b, int walker0) {
+  if (!b || walker0 < 0) return fail(QMCX_ERR_ARG, "bad argument");
+  b->walker0 = walker0;
+  return 0;
+}
+
+int qmcx_batch_set_tile(qmcx_batch_t b, int spline_slices) {
+  if (!b || spline_slices < 0 || spline_slices > 16 || (spline_slices && 16 % spline_slices)) return fail(QMCX_ERR_ARG, "bad tile");
+  int nt, sl;
+  int rc = b->prec == QMCX_FP32 ? choose_shape<float>(b->sys, spline_slices, nt, sl)
+                                : choose_shape<double>(b->sys, spline_slices, nt, sl);
+  if (rc) return rc;
+  b->slices = spline_slices;
+  return 0;
+}
+
+int qmcx_evaluate_log(qmcx_batch_t b, double* log_out) {
+  if (!b) return fail(QMCX_ERR_ARG, "null batch");
+  int rc = b->prec == QMCX_FP32 ? evaluate_log_impl<float>(b) : evaluate_log_impl<double>(b);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (log_out) HIPCHK(hipMemcpy(log_out, b->logv, sizeof(double) * b->W, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int qmcx_set_trace(qmcx_batch_t b, int enable) {
+  if (!b) return fail(QMCX_ERR_ARG, "null batch");
+  if (enable && !b->trace) HIPCHK(hipMalloc(&b->trace, sizeof(double) * 9 * (size_t)b->W * b->sys->N));
+  b->trace_on = enable != 0;
+  return 0;
+}
+
+int qmcx_get_trace(qmcx_batch_t b, double* trace) {
+  if (!b || !trace || !b->trace) return fail(QMCX_ERR_STATE, "trace not enabled");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(trace, b->trace, sizeof(double) * 9 * (size_t)b->W * b->sys->N, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int accept_mode, int recompute_every,
+                    double* log_out) {
+  if (!b || nsweeps < 0 || !(tau > 0)) return fail(QMCX_ERR_ARG, "bad argument");
+  if (accept_mode != QMCX_ACCEPT_REFERENCE && accept_mode != QMCX_ACCEPT_METROPOLIS)
+    return fail(QMCX_ERR_ARG, "unknown accept mode");
+  if (nsweeps > b->hist_cap) {
+    if (b->hist) hipFree(b->hist);
+    b->hist = nullptr;
+    HIPCHK(hipMalloc(&b->hist, sizeof(double) * (size_t)nsweeps * b->W));
+    b->hist_cap = nsweeps;
+  }
+  // mark the first sweep of this run so evaluateGL writes its history from slot 0
+  {
+    int cur[2];
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(cur, b->sweep, sizeof(cur), hipMemcpyDeviceToHost));
+    cur[1] = cur[0];
+    HIPCHK(hipMemcpy(b->sweep, cur, sizeof(cur), hipMemcpyHostToDevice));
+  }
+  const bool f32 = b->prec == QMCX_FP32;
+  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  for (int sw = 0; sw < nsweeps; ++sw) {
+    int rc = f32 ? sweep_impl<float>(b, seed, tau, accept_mode) : sweep_impl<double>(b, seed, tau, accept_mode);
+    if (rc) return rc;
+    if (recompute_every > 0 && (sw + 1) % recompute_every == 0) {
+      rc = f32 ? launch_lu_batch<float>(b) : launch_lu_batch<double>(b);
+      if (rc) return rc;
+    }
+    rc = f32 ? launch_gl<float>(b, true) : launch_gl<double>(b, true);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_next_sweep, dim3(1), dim3(64), 0, b->stream, b->sweep);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(b->ev1, b->stream));
+  HIPCHK(hipEventSynchronize(b->ev1));
+  HIPCHK(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+  if (log_out && nsweeps > 0)
+    HIPCHK(hipMemcpy(log_out, b->hist, sizeof(double) * (size_t)nsweeps * b->W, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int qmcx_get_gl(qmcx_batch_t b, double* G, double* L, double* log_out) {
+  if (!b) return fail(QMCX_ERR_ARG, "null batch");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const size_t WN = (size_t)b->W * b->sys->N;
+  if (G) HIPCHK(hipMemcpy(G, b->G, sizeof(double) * 3 * WN, hipMemcpyDeviceToHost));
+  if (L) HIPCHK(hipMemcpy(L, b->L, sizeof(double) * WN, hipMemcpyDeviceToHost));
+  if (log_out) HIPCHK(hipMemcpy(log_out, b->logv, sizeof(double) * b->W, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int qmcx_get_accepts(qmcx_batch_t b, long long* naccept) {
+  if (!b || !naccept) return fail(QMCX_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  std::vector<unsigned long long> tmp(b->W);
+  HIPCHK(hipMemcpy(tmp.data(), b->nacc, sizeof(unsigned long long) * b->W, hipMemcpyDeviceToHost));
+  for (int i = 0; i < b->W; ++i) naccept[i] = (long long)tmp[i];
+  return 0;
+}
+
+int qmcx_get_inverse(qmcx_batch_t b, int walker, int det, double* psiM) {
+  if (!b || !psiM || walker < 0 || walker >= b->W || det < 0 || det > 1) return fail(QMCX_ERR_ARG, "bad argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const size_t n = b->sys->n, nn = n * n, off = ((size_t)walker * 2 + det) * nn;
+  if (b->prec == QMCX_FP32) {
+    std::vector<float> tmp(nn);
+    HIPCHK(hipMemcpy(tmp.data(), (float*)b->Ainv + off, sizeof(float) * nn, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nn; ++i) psiM[i] = tmp[i];
+  } else {
+    HIPCHK(hipMemcpy(psiM, (double*)b->Ainv + off, sizeof(double) * nn, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int qmcx_last_sweep_ms(qmcx_batch_t b, double* ms) {
+  if (!b || !ms) return fail(QMCX_ERR_ARG, "null argument");
+  *ms = b->last_ms;
+  return 0;
+}
+
+}  // extern "C"

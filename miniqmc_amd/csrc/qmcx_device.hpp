@@ -1,0 +1,225 @@
+// Device-side building blocks of the VMC particle-move engine (gfx950).
+//
+// Everything here runs inside one workgroup that owns one walker, or is a
+// per-lane helper.  Wavefronts are 64 lanes; block reductions combine the
+// per-wave partials through LDS in a fixed order so every lane of the
+// workgroup sees bit-identical sums (the accept/reject decision is evaluated
+// redundantly by all lanes).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qmcx {
+
+constexpr int kMaxFunctorCoefs = 24;
+constexpr int kMaxSpecies = 4;
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------------ RNG
+// Philox4x64-10 (Salmon et al., SC'11); the walker move stream draws one block
+// for the three Gaussian components and one for the acceptance uniform.
+__device__ inline void philox4x64_10(uint64_t c[4], uint64_t k0, uint64_t k1) {
+  const uint64_t M0 = 0xD2E7470EE14C6C93ULL, M1 = 0xCA5A826395121157ULL;
+  const uint64_t W0 = 0x9E3779B97F4A7C15ULL, W1 = 0xBB67AE8584CAA73BULL;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) { k0 += W0; k1 += W1; }
+    const uint64_t hi0 = __umul64hi(M0, c[0]), lo0 = M0 * c[0];
+    const uint64_t hi1 = __umul64hi(M1, c[2]), lo1 = M1 * c[2];
+    const uint64_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+  }
+}
+
+__device__ inline double u01(uint64_t x) { return (double)(x >> 11) * (1.0 / 9007199254740992.0); }
+
+// Gaussian displacement components and acceptance uniform of move
+// (walker, sweep, electron): counter {iel, sweep, walker, 0|1}, key {seed, 0x51A7}.
+__device__ inline void move_randoms(uint64_t seed, uint64_t walker, uint64_t sweep, uint64_t iel, double nrm[3],
+                                    double& uacc) {
+  const double two_pi = 6.283185307179586476925286766559;
+  uint64_t c[4] = {iel, sweep, walker, 0};
+  philox4x64_10(c, seed, 0x51A7ULL);
+  const double r0 = sqrt(-2.0 * log(1.0 - u01(c[0])));
+  const double r1 = sqrt(-2.0 * log(1.0 - u01(c[2])));
+  double s1, c1, s3, c3;
+  sincos(two_pi * u01(c[1]), &s1, &c1);
+  sincos(two_pi * u01(c[3]), &s3, &c3);
+  nrm[0] = r0 * c1;
+  nrm[1] = r0 * s1;
+  nrm[2] = r1 * c3;
+  uint64_t d[4] = {iel, sweep, walker, 1};
+  philox4x64_10(d, seed, 0x51A7ULL);
+  uacc = u01(d[0]);
+}
+
+// ------------------------------------------------------------------ reductions
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Sum NV values over the workgroup; result is broadcast to every lane.
+// scratch: LDS of at least NV*16 elements.  Ends with a barrier so scratch can
+// be reused immediately.
+template <typename T, int NV>
+__device__ inline void block_sum(T (&v)[NV], T* scratch) {
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const int nw = (blockDim.x + kWave - 1) / kWave;
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) scratch[i * 16 + wid] = v[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    T s = 0;
+    for (int q = 0; q < nw; ++q) s += scratch[i * 16 + q];
+    v[i] = s;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------ radial functor
+// Uniform cubic B-spline u(r) with cutoff (BsplineFunctor.h:93-339):
+// coef[0] = p1 - 2 dr cusp, coef[1..] = p0, p1, ..., three trailing zeros.
+template <typename T>
+struct Functor {
+  T rcut, drinv;
+  int ncoef;
+  int pad;
+  T coef[kMaxFunctorCoefs];
+};
+
+// u, (du/dr)/r and d2u/dr2; caller guarantees r < rcut.
+template <typename T>
+__device__ inline void functor_vgl(const Functor<T>& f, T r, T& u, T& du, T& d2u) {
+  const T x = r * f.drinv;
+  const int i = (int)x;
+  const T t = x - (T)i;
+  const T t2 = t * t, t3 = t2 * t;
+  const T c0 = f.coef[i], c1 = f.coef[i + 1], c2 = f.coef[i + 2], c3 = f.coef[i + 3];
+  const T s6 = T(1) / T(6);
+  // power-form rows of the cubic B-spline basis
+  u = c0 * ((-t3 + T(3) * t2 - T(3) * t + T(1)) * s6) + c1 * ((T(3) * t3 - T(6) * t2 + T(4)) * s6) +
+      c2 * ((-T(3) * t3 + T(3) * t2 + T(3) * t + T(1)) * s6) + c3 * (t3 * s6);
+  const T d = c0 * (T(-0.5) * t2 + t - T(0.5)) + c1 * (T(1.5) * t2 - T(2) * t) +
+              c2 * (T(-1.5) * t2 + t + T(0.5)) + c3 * (T(0.5) * t2);
+  const T dd = c0 * (T(1) - t) + c1 * (T(3) * t - T(2)) + c2 * (T(1) - T(3) * t) + c3 * t;
+  du = f.drinv * d / r;
+  d2u = f.drinv * f.drinv * dd;
+}
+
+template <typename T>
+__device__ inline T functor_v(const Functor<T>& f, T r) {
+  const T x = r * f.drinv;
+  const int i = (int)x;
+  const T t = x - (T)i;
+  const T t2 = t * t, t3 = t2 * t;
+  const T s6 = T(1) / T(6);
+  return f.coef[i] * ((-t3 + T(3) * t2 - T(3) * t + T(1)) * s6) +
+         f.coef[i + 1] * ((T(3) * t3 - T(6) * t2 + T(4)) * s6) +
+         f.coef[i + 2] * ((-T(3) * t3 + T(3) * t2 + T(3) * t + T(1)) * s6) + f.coef[i + 3] * (t3 * s6);
+}
+
+// ------------------------------------------------------------------ minimum image
+// Periodic general cell: reduced basis rb (rows), its inverse g, eight corner
+// shifts (ParticleBConds.h:99-217).  d is (target - source) before the flip.
+template <typename T>
+struct Cell {
+  T g[9];        // g[3*i+j] = g(i,j)
+  T rb[9];       // rb[3*i+j] = rb[i][j]
+  T corner[24];
+};
+
+template <typename T>
+__device__ inline T min_image(const Cell<T>& C, T d0, T d1, T d2, T flip, T& ox, T& oy, T& oz) {
+  d0 *= flip;
+  d1 *= flip;
+  d2 *= flip;
+  const T s0 = -floor(d0 * C.g[0] + d1 * C.g[3] + d2 * C.g[6]);
+  const T s1 = -floor(d0 * C.g[1] + d1 * C.g[4] + d2 * C.g[7]);
+  const T s2 = -floor(d0 * C.g[2] + d1 * C.g[5] + d2 * C.g[8]);
+  const T ex = d0 + s0 * C.rb[0] + s1 * C.rb[3] + s2 * C.rb[6];
+  const T ey = d1 + s0 * C.rb[1] + s1 * C.rb[4] + s2 * C.rb[7];
+  const T ez = d2 + s0 * C.rb[2] + s1 * C.rb[5] + s2 * C.rb[8];
+  T best = ex * ex + ey * ey + ez * ez;
+  int ic = 0;
+#pragma unroll
+  for (int c = 1; c < 8; ++c) {
+    const T x = ex + C.corner[3 * c], y = ey + C.corner[3 * c + 1], z = ez + C.corner[3 * c + 2];
+    const T q = x * x + y * y + z * z;
+    ic = q < best ? c : ic;
+    best = q < best ? q : best;
+  }
+  ox = flip * (ex + C.corner[3 * ic]);
+  oy = flip * (ey + C.corner[3 * ic + 1]);
+  oz = flip * (ez + C.corner[3 * ic + 2]);
+  return sqrt(best);
+}
+
+// ------------------------------------------------------------------ spline location
+// toUnit_floor with the SPO lattice, then the grid cell and fraction per axis
+// (CrystalLattice.h:185-196, MultiBsplineEvalHelper.hpp:43-64); kept in FP64.
+__device__ inline void spline_locate(const double* Ginv, const int* ng, double x, double y, double z, int ix[3],
+                                     double t[3]) {
+  const double r[3] = {x, y, z};
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double v = r[0] * Ginv[j] + r[1] * Ginv[3 + j] + r[2] * Ginv[6 + j];
+    if (-2.220446049250313e-16 < v && v < 0.0) v = 0.0;
+    else v -= floor(v);
+    const double xs = v * (double)ng[j];
+    if (xs < 0) {
+      ix[j] = 0;
+      t[j] = 0.0;
+    } else {
+      const double ip = floor(xs);
+      t[j] = xs - ip;
+      ix[j] = (int)ip;
+      if (ix[j] > ng[j] - 1) {
+        ix[j] = ng[j] - 1;
+        t[j] = 1.0 - 2.220446049250313e-16;
+      }
+    }
+  }
+}
+
+// cubic B-spline weights a, da, d2a at fraction t (MultiBsplineData.hpp:52-80)
+template <typename T>
+__device__ inline void bspline_weights(T t, T a[4], T da[4], T d2a[4]) {
+  const T s6 = T(1) / T(6);
+  a[0] = ((T(-1) * t + T(3)) * t - T(3)) * t * s6 + s6;
+  a[1] = ((T(3) * t - T(6)) * t) * t * s6 + T(4) * s6;
+  a[2] = ((T(-3) * t + T(3)) * t + T(3)) * t * s6 + s6;
+  a[3] = t * t * t * s6;
+  da[0] = (T(-0.5) * t + T(1)) * t - T(0.5);
+  da[1] = (T(1.5) * t - T(2)) * t;
+  da[2] = (T(-1.5) * t + T(1)) * t + T(0.5);
+  da[3] = T(0.5) * t * t;
+  d2a[0] = T(1) - t;
+  d2a[1] = T(3) * t - T(2);
+  d2a[2] = T(1) - T(3) * t;
+  d2a[3] = t;
+}
+
+// 16-byte vector of T (clang ext_vector_type so the load can be non-temporal)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <typename T>
+struct V16;
+template <>
+struct V16<float> {
+  using type = f32x4;
+  static constexpr int n = 4;
+};
+template <>
+struct V16<double> {
+  using type = f64x2;
+  static constexpr int n = 2;
+};
+
+}  // namespace qmcx

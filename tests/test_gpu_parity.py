@@ -1,0 +1,141 @@
+"""HIP path vs the FP64 CPU oracle, through the C ABI (libqmcx.so).
+
+Every comparison uses the same seeded inputs on both sides.  Tolerances:
+  * FP64 kernels: relative 1e-10 or tighter (only summation order differs);
+  * FP32 kernels: stated per test (FP32 spline sums, FP32 Woodbury) and reported;
+  * accept/reject sequences: identical.
+"""
+import numpy as np
+import pytest
+
+from miniqmc_amd import qmcx, systems
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def cubic():
+    return systems.make_system("nio8-cubic")
+
+
+@pytest.fixture(scope="module")
+def cubic_dev(cubic):
+    return qmcx.System(cubic)
+
+
+@pytest.fixture(scope="module")
+def cubic_orc(cubic):
+    return po.OracleSystem(cubic)
+
+
+def _rand_points(sysd, npts, seed):
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-0.3, 1.3, size=(npts, 3))   # includes points outside the cell
+    return u @ sysd.lattice
+
+
+@pytest.mark.parametrize("prec,tol", [(qmcx.FP64, 1e-12), (qmcx.FP32, 2e-5)])
+def test_spline_vgh_vs_oracle(cubic, cubic_dev, prec, tol):
+    _say("spline VGH", prec)
+    r = _rand_points(cubic, 64, 3)
+    gpu = cubic_dev.spline_vgh(r, prec)
+    u = po.to_unit_floor(cubic.spo_lattice, r)
+    ref = po.spline_vgh_unit(cubic.ng, cubic.norb, cubic.npad, cubic.coefs, u)
+    for p in range(10):
+        scale = np.abs(ref[:, p]).max()
+        err = np.abs(gpu[:, p] - ref[:, p]).max() / scale
+        assert err < tol, (p, err)
+
+
+@pytest.mark.parametrize("prec,tol", [(qmcx.FP64, 1e-12), (qmcx.FP32, 2e-5)])
+def test_distance_rows_vs_oracle(cubic, cubic_dev, prec, tol):
+    _say("distance rows", prec)
+    rng = np.random.default_rng(5)
+    src = rng.uniform(-0.5, 1.5, size=(8, 3)) @ cubic.lattice
+    tgt = rng.uniform(0, 1, size=(40, 3)) @ cubic.lattice
+    flips = np.array([0, 3, 7, 40, 12, 0, 39, 20], dtype=np.int32)
+    r, dr = cubic_dev.distance_rows(src, flips, tgt, prec)
+    for p in range(len(src)):
+        rr, dd = po.distance_row(cubic.lattice, src[p], tgt, int(flips[p]))
+        assert np.abs(r[p] - rr).max() < tol * 10
+        assert np.abs(dr[p] - dd).max() < tol * 10
+
+
+def test_invert_transpose_kat():
+    # test_dirac_matrix.cpp:69-107: log|det| 3.78518913425 and the known inverse
+    _say("LU KAT")
+    a = np.array([[2.3, 4.5, 2.6], [0.5, 8.5, 3.3], [1.8, 4.4, 4.9]])
+    psiM, ld, ph = qmcx.invert_transpose(a.T.copy())
+    b = np.array([[0.6159749342, -0.2408954682, -0.1646081192],
+                  [0.07923894288, 0.1496231042, -0.1428117337],
+                  [-0.2974298429, -0.04586322768, 0.3927890292]])
+    assert abs(ld[0] - 3.78518913425) < 1e-10
+    assert np.abs(psiM[0] - b).max() < 1e-9
+
+
+def test_invert_transpose_batched_vs_oracle():
+    _say("LU batched n=96")
+    rng = np.random.default_rng(7)
+    a = rng.uniform(-1, 1, size=(5, 96, 96))
+    psiM, ld, ph = qmcx.invert_transpose(a)
+    for m in range(5):
+        ref, rld, rph = po.invert_transpose(a[m])
+        assert abs(ld[m] - rld) < 1e-10 * abs(rld) + 1e-10
+        assert ph[m] == rph
+        assert np.abs(psiM[m] - ref).max() < 1e-9 * np.abs(ref).max()
+
+
+def _compare_run(sysd, dev, orc, nw, nsweeps, delay, prec, mode, tau, tol_ratio, tol_log, seed=11):
+    R0 = systems.initial_positions(sysd, nw, seed=seed)
+    ref = orc.run_vmc(R0, nsweeps, seed, tau, mode=mode, delay=delay, trace=True)
+    wb = qmcx.WalkerBatch(dev, nw, delay_rank=delay, precision=prec)
+    wb.set_positions(R0)
+    log0 = wb.evaluate_log()
+    np.testing.assert_allclose(log0, ref["log_init"], rtol=tol_log)
+    wb.set_trace(True)
+    flips = 0
+    for s in range(nsweeps):
+        hist = wb.run(1, seed, tau, mode)
+        # the counter-based stream is indexed by the absolute sweep number
+        tr = wb.trace()
+        rt = ref["trace"][:, s]
+        flips += int((tr[:, :, 2] != rt[:, :, 2]).sum())
+        if flips == 0:
+            # scaled by max(|ratio|, 1): near-zero ratios carry the FP32 cancellation error of the dot product
+            rel = np.abs(tr[:, :, 0] - rt[:, :, 0]) / np.maximum(np.abs(rt[:, :, 0]), 1.0)
+            _say(f"  sweep {s}: max ratio err {rel.max():.3e}, median {np.median(rel):.3e}")
+            assert rel.max() < tol_ratio, ("ratio", s, rel.max())
+            g_err = np.abs(tr[:, :, 3:9] - rt[:, :, 3:9]) / (1 + np.abs(rt[:, :, 3:9]))
+            assert g_err.max() < tol_ratio * 100, ("grad", s, g_err.max())
+        lerr = np.abs(hist[0] - ref["log"][:, s]) / np.abs(ref["log"][:, s])
+        _say(f"  sweep {s}: max |dlog|/|log| {lerr.max():.3e}")
+        np.testing.assert_allclose(hist[0], ref["log"][:, s], rtol=tol_log)
+    G, L, lv = wb.gl()
+    return dict(flips=flips, ref=ref, G=G, L=L, log=lv, acc=wb.accepts(), R=wb.positions())
+
+
+@pytest.mark.parametrize("delay", [1, 4])
+def test_sweep_fp64_vs_oracle(cubic, cubic_dev, cubic_orc, delay):
+    _say("sweep FP64 delay", delay)
+    out = _compare_run(cubic, cubic_dev, cubic_orc, nw=3, nsweeps=2, delay=delay, prec=qmcx.FP64,
+                       mode=qmcx.ACCEPT_METROPOLIS, tau=0.5, tol_ratio=1e-9, tol_log=1e-11)
+    assert out["flips"] == 0
+    ref = out["ref"]
+    assert np.array_equal(out["acc"], ref["naccept"])
+    np.testing.assert_allclose(out["R"], ref["R"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(out["G"], ref["G"], rtol=1e-8, atol=1e-8)
+    np.testing.assert_allclose(out["L"], ref["L"], rtol=1e-8, atol=1e-7)
+
+
+@pytest.mark.parametrize("delay", [1, 8])
+def test_sweep_fp32_vs_fp64_oracle(cubic, cubic_dev, cubic_orc, delay):
+    _say("sweep FP32 delay", delay)
+    # reference acceptance (u < 0.5) makes the accept sequence independent of precision
+    out = _compare_run(cubic, cubic_dev, cubic_orc, nw=3, nsweeps=2, delay=delay, prec=qmcx.FP32,
+                       mode=qmcx.ACCEPT_REFERENCE, tau=0.5, tol_ratio=2e-3, tol_log=2e-6)
+    assert out["flips"] == 0
