@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Walker-moves/s of the VMC particle-move sweep at NiO-64 on one MI355X.
+
+One step = one sweep of every electron of every walker (W walkers x 768
+single-electron moves), followed by evaluateGL, exactly the scope of the
+reference's "Diffusion" timer (src/Drivers/miniqmc_sync_move.cpp:437-513).
+Inputs (spline table, walker state) are resident in HBM before the timed
+region.  The headline is the batched FP32 form (FP32 spline/distance/Jastrow/
+Woodbury, FP64 log-det and acceptance product); the FP64 delay-1 GPU path and
+the oracle's CPU path are timed beside it in the same run.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from miniqmc_amd import qmcx, systems  # noqa: E402
+from miniqmc_amd import autotune  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "walker-moves/s at NiO-64 on 1 MI355X, batched FP32 vs FP64 delay-1; log|Psi| err"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def per_move_bytes(sysd, k, acc, s=4):
+    """Algorithmic HBM bytes per walker-move of the move kernel.
+
+    Spline stencil 64*Np*s, positions 3(N+M)s, Jastrow state a*2*5*N*s, and the
+    determinant row work s*n*(1+2c) + a*2*n*s with c the mean number of delayed
+    rows seen by a move.  Moves inside a chunk of k electrons see on average
+    a*(k-1)/2 delayed rows (the flush's 2*n^2*s/k per move is in the flush kernel).
+    """
+    n, N, M, Np = sysd.nup, sysd.nel, sysd.nions, sysd.npad
+    cbar = acc * (k - 1) / 2.0 if k > 1 else 0.0
+    b_spline = 64 * Np * s
+    b_pos = 3 * (N + M) * s
+    b_j = acc * 2 * 5 * N * s
+    b_det = s * n * (1 + 2 * cbar) + acc * 2 * n * s
+    return b_spline + b_pos + b_j + b_det, dict(spline=b_spline, pos=b_pos, jastrow=b_j, det=b_det, cbar=cbar)
+
+
+def cpu_baseline(sysd, R0, seed, tau, nwalk, nthreads):
+    from oracle import pyoracle as po
+    orc = po.OracleSystem(sysd, spline_prec=8)
+    t0 = time.time()
+    res = orc.run_vmc(R0[:nwalk], 1, seed, tau, mode=qmcx.ACCEPT_METROPOLIS, delay=1, nthreads=nthreads)
+    wall = time.time() - t0
+    moves = nwalk * sysd.nel
+    return dict(value=moves / res["seconds"], unit="walker-moves/s", cores=nthreads, kind="port",
+                sample=f"{nwalk} walkers x 1 sweep of {sysd.name} ({moves} moves), all-FP64 oracle, delay 1, "
+                       f"OpenMP over walkers; sweep time {res['seconds']:.2f} s (wall incl. evaluateLog {wall:.2f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--system", default="nio64")
+    ap.add_argument("--walkers", type=int, default=128)
+    ap.add_argument("--delay", type=int, default=0, help="delay rank (0: autotune over {16,32,64})")
+    ap.add_argument("--tau", type=float, default=0.3)
+    ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--fp64-steps", type=int, default=4)
+    ap.add_argument("--cpu-walkers", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on exactly one GPU: walkers are independent, so a multi-GPU run is one "
+              "replica of this process per GPU with no exchange inside a sweep; --gpus must be 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    t_setup = time.time()
+    sysd = systems.make_system(args.system, seed=args.seed)
+    dev = qmcx.System(sysd, precisions=(qmcx.FP32, qmcx.FP64))
+    W, N = args.walkers, sysd.nel
+    R0 = systems.initial_positions(sysd, W, seed=args.seed)
+    log(f"setup {time.time() - t_setup:.1f}s: {sysd.name} N={N} norb={sysd.norb} ng={sysd.ng} "
+        f"table={sysd.table_bytes() / 1e6:.0f} MB W={W}")
+
+    # ---- delay rank / spline tile (project autotuner)
+    if args.delay > 0:
+        k, tile = args.delay, 0
+        tune = None
+    else:
+        k, tile, tune = autotune.tune(dev, W, R0, seed=args.seed, tau=args.tau)
+        log(f"autotune: delay {k}, tile {tile}: {tune}")
+
+    # ---- error leg: FP32 batched vs FP64 delay-1 from the same start, same random streams
+    b32 = qmcx.WalkerBatch(dev, W, delay_rank=k, precision=qmcx.FP32, tile=tile)
+    b64 = qmcx.WalkerBatch(dev, W, delay_rank=1, precision=qmcx.FP64)
+    for b in (b32, b64):
+        b.set_positions(R0)
+        b.evaluate_log()
+        b.set_trace(True)
+    h32 = b32.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
+    h64 = b64.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
+    t32, t64 = b32.trace(), b64.trace()
+    flips = int((t32[:, :, 2] != t64[:, :, 2]).sum())
+    lerr = np.abs(h32[0] - h64[0]) / np.abs(h64[0])
+    acc_rate = float(t64[:, :, 2].mean())
+    for b in (b32, b64):
+        b.set_trace(False)
+    log(f"error leg: max rel |dlog| {lerr.max():.2e}, accept flips {flips}, acceptance {acc_rate:.3f}")
+
+    # ---- headline: batched FP32
+    b32.run(args.warmup, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
+    b32.set_kernel_timing(True)
+    a0 = b32.accepts().sum()
+    t0 = time.perf_counter()
+    b32.run(args.steps, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
+    wall = time.perf_counter() - t0
+    a1 = b32.accepts().sum()
+    mv_ms, mv_n, fl_ms, fl_n = b32.kernel_timing()
+    b32.set_kernel_timing(False)
+    moves = W * N * args.steps
+    value = moves / wall
+    acc_timed = float(a1 - a0) / moves
+    log(f"FP32 k={k}: {value:.4e} moves/s, {1e3 * wall / args.steps:.2f} ms/sweep; "
+        f"move kernel {mv_ms / max(mv_n, 1):.3f} ms x {mv_n}, flush {fl_ms / max(fl_n, 1):.3f} ms x {fl_n}")
+
+    # ---- FP64 delay-1 GPU path (same walkers, continuing its own chain)
+    b64.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
+    t0 = time.perf_counter()
+    b64.run(args.fp64_steps, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
+    wall64 = time.perf_counter() - t0
+    value64 = W * N * args.fp64_steps / wall64
+    log(f"FP64 k=1: {value64:.4e} moves/s, {1e3 * wall64 / args.fp64_steps:.2f} ms/sweep")
+
+    # ---- roofline of the dominant kernel (the fused move kernel)
+    bpm, parts = per_move_bytes(sysd, k, acc_timed)
+    chunk = k if k > 1 else sysd.nup
+    launch_ms = mv_ms / max(mv_n, 1)
+    achieved = bpm * W * chunk / (launch_ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_moves<float>", bytes_per_move=round(bpm, 1),
+                    moves_per_launch=W * chunk, avg_launch_ms=round(launch_ms, 4))
+
+    cpu = None
+    if not args.no_cpu:
+        nthreads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(sysd, R0, args.seed, args.tau, min(args.cpu_walkers, W), nthreads)
+        log(f"CPU oracle: {cpu['value']:.4e} moves/s on {nthreads} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "walker-moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{sysd.name} VMC sweep: {N} e-, {sysd.norb} orbitals, {sysd.ng[0]}^3 FP32 spline "
+                        f"({sysd.table_bytes() / 1e6:.0f} MB), {W} walkers, tau {args.tau}, Metropolis; "
+                        f"batched FP32, delay rank {k}",
+            "walkers": W, "electrons": N, "orbitals": sysd.norb, "grid": list(sysd.ng), "delay_rank": k,
+            "spline_tile": tile, "moves_per_step": W * N,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "fp64_delay1": {"value": value64, "unit": "walker-moves/s", "ms_per_step": 1e3 * wall64 / args.fp64_steps,
+                        "steps": args.fp64_steps, "form": "batched, 1 workgroup per walker, in-kernel Sherman-Morrison"},
+        "speedup_vs_fp64_delay1": value / value64,
+        "log_psi_err": {"max_rel": float(lerr.max()), "mean_rel": float(lerr.mean()),
+                        "vs": "FP64 delay-1 GPU path (parity-pinned to the FP64 oracle), same start and streams, 1 sweep"},
+        "accept_flips": flips,
+        "acceptance": acc_timed,
+        "flush_kernel_ms_per_launch": fl_ms / max(fl_n, 1),
+        "bytes_per_move_parts": parts,
+        "autotune": tune,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

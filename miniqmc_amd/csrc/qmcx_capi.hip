@@ -223,6 +223,44 @@ struct qmcx_batch_s {
   int hist_cap = 0;
   double* lu_ws = nullptr;
   bool trace_on = false;
+  // optional per-kernel timing: HIP events bracket every move / flush launch
+  bool prof_on = false;
+  std::vector<hipEvent_t> prof_ev;
+  std::vector<int> prof_kind;   // 0 = move kernel, 1 = flush kernel
+  size_t prof_used = 0;
+  double prof_ms[2] = {0, 0};
+  long long prof_n[2] = {0, 0};
+  int prof_record(int kind) {
+    if (!prof_on) return 0;
+    if (prof_used + 2 > prof_ev.size()) {
+      for (int i = 0; i < 64; ++i) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        prof_ev.push_back(e);
+        prof_kind.push_back(0);
+      }
+    }
+    prof_kind[prof_used] = kind;
+    HIPCHK(hipEventRecord(prof_ev[prof_used++], stream));
+    return 0;
+  }
+  int prof_close() {
+    if (!prof_on) return 0;
+    HIPCHK(hipEventRecord(prof_ev[prof_used++], stream));
+    return 0;
+  }
+  int prof_collect() {
+    if (!prof_on) return 0;
+    HIPCHK(hipStreamSynchronize(stream));
+    for (size_t i = 0; i + 1 < prof_used; i += 2) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, prof_ev[i], prof_ev[i + 1]));
+      prof_ms[prof_kind[i]] += ms;
+      prof_n[prof_kind[i]] += 1;
+    }
+    prof_used = 0;
+    return 0;
+  }
 
   template <typename T>
   BatchP<T> params() const {
@@ -356,12 +394,16 @@ int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode) {
       a.det = det;
       a.e_begin = det * s->n + e;
       a.e_end = det * s->n + std::min(s->n, e + chunk);
+      if (b->prof_record(0)) return QMCX_ERR_HIP;
       hipLaunchKernelGGL(k_moves<T>, dim3(b->W), dim3(nt), L.total, b->stream, s->params<T>(), b->params<T>(), a);
       HIPCHK(hipGetLastError());
+      if (b->prof_close()) return QMCX_ERR_HIP;
       if (b->K > 1) {
+        if (b->prof_record(1)) return QMCX_ERR_HIP;
         hipLaunchKernelGGL(k_flush_generic<T>, dim3((s->n + kFlushRows - 1) / kFlushRows, b->W), dim3(256), flds,
                            b->stream, s->params<T>(), b->params<T>(), det);
         HIPCHK(hipGetLastError());
+        if (b->prof_close()) return QMCX_ERR_HIP;
       }
     }
   }
@@ -645,6 +687,7 @@ int qmcx_batch_destroy(qmcx_batch_t b) {
   if (b->stream) hipStreamDestroy(b->stream);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
+  for (hipEvent_t e : b->prof_ev) hipEventDestroy(e);
   delete b;
   return 0;
 }
@@ -741,12 +784,14 @@ int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int 
     }
     rc = f32 ? launch_gl<float>(b, true) : launch_gl<double>(b, true);
     if (rc) return rc;
+    if (b->prof_on && b->prof_used > 4096 && b->prof_collect()) return QMCX_ERR_HIP;
     hipLaunchKernelGGL(k_next_sweep, dim3(1), dim3(64), 0, b->stream, b->sweep);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(b->ev1, b->stream));
   HIPCHK(hipEventSynchronize(b->ev1));
   HIPCHK(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+  if (b->prof_collect()) return QMCX_ERR_HIP;
   if (log_out && nsweeps > 0)
     HIPCHK(hipMemcpy(log_out, b->hist, sizeof(double) * (size_t)nsweeps * b->W, hipMemcpyDeviceToHost));
   return 0;
@@ -781,6 +826,24 @@ int qmcx_get_inverse(qmcx_batch_t b, int walker, int det, double* psiM) {
     for (size_t i = 0; i < nn; ++i) psiM[i] = tmp[i];
   } else {
     HIPCHK(hipMemcpy(psiM, (double*)b->Ainv + off, sizeof(double) * nn, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int qmcx_set_kernel_timing(qmcx_batch_t b, int enable) {
+  if (!b) return fail(QMCX_ERR_ARG, "null batch");
+  b->prof_on = enable != 0;
+  b->prof_ms[0] = b->prof_ms[1] = 0;
+  b->prof_n[0] = b->prof_n[1] = 0;
+  b->prof_used = 0;
+  return 0;
+}
+
+int qmcx_get_kernel_timing(qmcx_batch_t b, double* ms2, long long* launches2) {
+  if (!b || !ms2 || !launches2) return fail(QMCX_ERR_ARG, "null argument");
+  for (int i = 0; i < 2; ++i) {
+    ms2[i] = b->prof_ms[i];
+    launches2[i] = b->prof_n[i];
   }
   return 0;
 }

@@ -57,6 +57,8 @@ _SIGS = {
     "qmcx_get_accepts": [_vp, _lp],
     "qmcx_get_inverse": [_vp, _i, _i, _dp],
     "qmcx_last_sweep_ms": [_vp, C.POINTER(C.c_double)],
+    "qmcx_set_kernel_timing": [_vp, _i],
+    "qmcx_get_kernel_timing": [_vp, _dp, _lp],
 }
 
 
@@ -224,6 +226,15 @@ class WalkerBatch:
         p = np.zeros((self.n, self.n))
         _check(load().qmcx_get_inverse(self.h, walker, det, p.reshape(-1)))
         return p
+
+    def set_kernel_timing(self, on=True):
+        _check(load().qmcx_set_kernel_timing(self.h, 1 if on else 0))
+
+    def kernel_timing(self):
+        """(move_ms, move_launches, flush_ms, flush_launches) accumulated since timing was enabled."""
+        ms, n = np.zeros(2), np.zeros(2, dtype=np.int64)
+        _check(load().qmcx_get_kernel_timing(self.h, ms, n))
+        return float(ms[0]), int(n[0]), float(ms[1]), int(n[1])
 
     def last_sweep_ms(self):
         ms = C.c_double(0)
