@@ -155,12 +155,14 @@ struct qmcx_system_s {
   double spoG[9];
   float* d_coef_f = nullptr;
   double* d_coef_d = nullptr;
+  Cell<float>* d_cell_f = nullptr;
+  Cell<double>* d_cell_d = nullptr;
+  double* d_spoG = nullptr;
 
   template <typename T>
   SysP<T> params() const {
     SysP<T> S;
-    S.cell = make_cell<T>(lattice);
-    std::memcpy(S.spoG, spoG, sizeof(spoG));
+    S.spoG = d_spoG;
     S.ng[0] = ng[0];
     S.ng[1] = ng[1];
     S.ng[2] = ng[2];
@@ -177,9 +179,11 @@ struct qmcx_system_s {
     if constexpr (sizeof(T) == 4) {
       S.coefs = d_coef_f;
       S.fn = d_fnf;
+      S.cell = d_cell_f;
     } else {
       S.coefs = d_coef_d;
       S.fn = d_fnd;
+      S.cell = d_cell_d;
     }
     return S;
   }
@@ -195,6 +199,7 @@ struct qmcx_system_s {
 struct qmcx_batch_s {
   qmcx_system_s* sys = nullptr;
   int W = 0, K = 1, prec = QMCX_FP64, walker0 = 0, slices = 0, nt = 0;
+  int flush_kind = 0;   // 0: matrix cores when possible, 1: vector FMA
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
@@ -275,7 +280,7 @@ struct qmcx_batch_s {
     B.J2 = (T*)J2;
     B.J1 = (T*)J1;
     B.Vbuf = (T*)Vbuf;
-    B.Binv = (T*)Binv;
+    B.Binv = (double*)Binv;
     B.rows = rows;
     B.ndelay = ndelay;
     B.logdet = logdet;
@@ -299,22 +304,41 @@ struct qmcx_batch_s {
 
 namespace {
 
-// workgroup size and spline slices for the move kernel
+// workgroup size and spline slices for the move kernel: at most kMoveThreads
+// lanes, at most two electrons per lane, 16/slices stencil columns per lane.
 template <typename T>
 int choose_shape(const qmcx_system_s* s, int want_slices, int& nt, int& slices) {
   const int vw = 16 / (int)sizeof(T);
   const int nq = s->npad / vw;
-  // at least one lane per electron (two electrons per lane at most)
-  int base = std::max(256, ((s->N + 1) / 2 + 63) / 64 * 64);
+  const int base = std::max(256, ((s->N + 1) / 2 + 63) / 64 * 64);
   slices = want_slices;
   if (slices <= 0) {
     slices = 1;
-    while (slices < 16 && nq * slices * 2 <= 1024) slices *= 2;
+    while (slices < 16 && nq * slices * 2 <= kMoveThreads) slices *= 2;
   }
-  if (16 % slices != 0 || nq * slices > 1024) return fail(QMCX_ERR_ARG, "spline tile does not fit a workgroup");
+  if (16 % slices != 0 || nq * slices > kMoveThreads) return fail(QMCX_ERR_ARG, "spline tile does not fit a workgroup");
   nt = std::max(base, (nq * slices + 63) / 64 * 64);
-  if (nt > 1024) return fail(QMCX_ERR_ARG, "workgroup too large for this system");
+  if (nt > kMoveThreads || s->N > 2 * nt) return fail(QMCX_ERR_ARG, "workgroup too large for this system");
   return 0;
+}
+
+template <typename T, int JPT>
+int launch_moves_jpt(qmcx_batch_t b, int nt, int slices, size_t lds, const MoveArgs& a) {
+  const qmcx_system_s* s = b->sys;
+  switch (slices) {
+    case 16: hipLaunchKernelGGL((k_moves<T, JPT, 1>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(), a); break;
+    case 8: hipLaunchKernelGGL((k_moves<T, JPT, 2>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(), a); break;
+    case 4: hipLaunchKernelGGL((k_moves<T, JPT, 4>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(), a); break;
+    case 2: hipLaunchKernelGGL((k_moves<T, JPT, 8>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(), a); break;
+    default: hipLaunchKernelGGL((k_moves<T, JPT, 16>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(), a); break;
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_moves(qmcx_batch_t b, int nt, int slices, size_t lds, const MoveArgs& a) {
+  return (b->sys->N <= nt) ? launch_moves_jpt<T, 1>(b, nt, slices, lds, a) : launch_moves_jpt<T, 2>(b, nt, slices, lds, a);
 }
 
 template <typename T>
@@ -375,6 +399,13 @@ int evaluate_log_impl(qmcx_batch_t b) {
   return launch_gl<T>(b, false);
 }
 
+// the matrix-core flush needs FP32 and n a multiple of 16 (row blocks of 32 are
+// clipped); it can be switched off for comparisons
+template <typename T>
+bool use_mfma_flush(const qmcx_batch_t b) {
+  return sizeof(T) == 4 && b->sys->n % 16 == 0 && b->flush_kind == 0;
+}
+
 template <typename T>
 int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode) {
   const qmcx_system_s* s = b->sys;
@@ -395,13 +426,19 @@ int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode) {
       a.e_begin = det * s->n + e;
       a.e_end = det * s->n + std::min(s->n, e + chunk);
       if (b->prof_record(0)) return QMCX_ERR_HIP;
-      hipLaunchKernelGGL(k_moves<T>, dim3(b->W), dim3(nt), L.total, b->stream, s->params<T>(), b->params<T>(), a);
-      HIPCHK(hipGetLastError());
+      if (launch_moves<T>(b, nt, slices, L.total, a)) return QMCX_ERR_HIP;
       if (b->prof_close()) return QMCX_ERR_HIP;
       if (b->K > 1) {
         if (b->prof_record(1)) return QMCX_ERR_HIP;
-        hipLaunchKernelGGL(k_flush_generic<T>, dim3((s->n + kFlushRows - 1) / kFlushRows, b->W), dim3(256), flds,
-                           b->stream, s->params<T>(), b->params<T>(), det);
+        if (use_mfma_flush<T>(b)) {
+          const int cp = (b->K + 15) & ~15;
+          const size_t lds = sizeof(float) * 2 * kFlushBM * cp + sizeof(int) * b->K;
+          hipLaunchKernelGGL(k_flush_mfma, dim3((s->n + kFlushBM - 1) / kFlushBM, b->W), dim3(256), lds, b->stream,
+                             s->params<float>(), b->params<float>(), det);
+        } else {
+          hipLaunchKernelGGL(k_flush_generic<T>, dim3((s->n + kFlushRows - 1) / kFlushRows, b->W), dim3(256), flds,
+                             b->stream, s->params<T>(), b->params<T>(), det);
+        }
         HIPCHK(hipGetLastError());
         if (b->prof_close()) return QMCX_ERR_HIP;
       }
@@ -450,6 +487,15 @@ int qmcx_system_create(qmcx_system_t* out, const double* lattice9, int nions, co
   HIPCHK(hipMemcpy(s->d_species, ion_species, sizeof(int) * nions, hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&s->d_fnd, sizeof(s->fnd)));
   HIPCHK(hipMalloc(&s->d_fnf, sizeof(s->fnf)));
+  {
+    const Cell<float> cf = make_cell<float>(lattice9);
+    const Cell<double> cd = make_cell<double>(lattice9);
+    HIPCHK(hipMalloc(&s->d_cell_f, sizeof(cf)));
+    HIPCHK(hipMalloc(&s->d_cell_d, sizeof(cd)));
+    HIPCHK(hipMemcpy(s->d_cell_f, &cf, sizeof(cf), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->d_cell_d, &cd, sizeof(cd), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&s->d_spoG, sizeof(double) * 9));
+  }
   *out = s;
   return 0;
 }
@@ -462,6 +508,9 @@ int qmcx_system_destroy(qmcx_system_t s) {
   hipFree(s->d_fnf);
   hipFree(s->d_coef_f);
   hipFree(s->d_coef_d);
+  hipFree(s->d_cell_f);
+  hipFree(s->d_cell_d);
+  hipFree(s->d_spoG);
   delete s;
   return 0;
 }
@@ -499,6 +548,7 @@ int qmcx_system_set_spline(qmcx_system_t s, const int* ng, int norb, int npad, c
   s->norb = norb;
   s->npad = npad;
   inv3(spo_lattice9, s->spoG);
+  HIPCHK(hipMemcpy(s->d_spoG, s->spoG, sizeof(double) * 9, hipMemcpyHostToDevice));
   const size_t cnt = (size_t)(ng[0] + 3) * (ng[1] + 3) * (ng[2] + 3) * npad;
   hipFree(s->d_coef_f);
   hipFree(s->d_coef_d);
@@ -654,7 +704,7 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
   b->J1 = p8;
   rc |= b->alloc(p8, es * W * K * n);
   b->Vbuf = p8;
-  rc |= b->alloc(p8, es * W * K * K);
+  rc |= b->alloc(p8, sizeof(double) * W * K * K);
   b->Binv = p8;
   rc |= b->alloc(b->rows, W * K);
   rc |= b->alloc(b->ndelay, W);
@@ -718,6 +768,12 @@ int qmcx_batch_get_positions(qmcx_batch_t b, double* R) {
 int qmcx_batch_set_walker_offset(qmcx_batch_t b, int walker0) {
   if (!b || walker0 < 0) return fail(QMCX_ERR_ARG, "bad argument");
   b->walker0 = walker0;
+  return 0;
+}
+
+int qmcx_batch_set_flush(qmcx_batch_t b, int kind) {
+  if (!b || kind < 0 || kind > 1) return fail(QMCX_ERR_ARG, "flush kind must be 0 (matrix cores) or 1 (vector)");
+  b->flush_kind = kind;
   return 0;
 }
 
@@ -855,3 +911,18 @@ int qmcx_last_sweep_ms(qmcx_batch_t b, double* ms) {
 }
 
 }  // extern "C"
+
+#ifdef QMCX_PHASE_TIMING
+// Profiling builds only: per-phase wall_clock64 ticks of the move kernel (walker 0, lane 0).
+extern "C" int qmcx_debug_phase_ticks(double* out8, int reset) {
+  unsigned long long t[8];
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpyFromSymbol(t, HIP_SYMBOL(qmcx_phase_ticks), sizeof(t)));
+  for (int i = 0; i < 8; ++i) out8[i] = (double)t[i];
+  if (reset) {
+    unsigned long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(qmcx_phase_ticks), z, sizeof(z)));
+  }
+  return 0;
+}
+#endif

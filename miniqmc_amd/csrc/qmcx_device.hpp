@@ -54,12 +54,42 @@ __device__ inline void move_randoms(uint64_t seed, uint64_t walker, uint64_t swe
 }
 
 // ------------------------------------------------------------------ reductions
+// Wave-wide sum through DPP lane moves (quad_perm, row mirrors, row broadcasts);
+// the total is read from lane 63 into a scalar register, so every lane holds
+// the same bits.
+template <int CTRL, int ROWMASK>
+__device__ inline float dpp_add(float x) {
+  const int y = __builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWMASK, 0xf, false);
+  return x + __int_as_float(y);
+}
+template <int CTRL, int ROWMASK>
+__device__ inline double dpp_add(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xf, false);
+  return x + __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline float lane63(float x) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63)); }
+__device__ inline double lane63(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 template <typename T>
 __device__ inline T wave_sum(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
+  v = dpp_add<0xB1, 0xf>(v);    // quad_perm [1,0,3,2]
+  v = dpp_add<0x4E, 0xf>(v);    // quad_perm [2,3,0,1]
+  v = dpp_add<0x141, 0xf>(v);   // row_half_mirror
+  v = dpp_add<0x140, 0xf>(v);   // row_mirror: every lane holds its row (16 lanes) sum
+  v = dpp_add<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+  v = dpp_add<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3: lane 63 holds the total
+  return lane63(v);
 }
+
+// Barrier for data exchanged through LDS only: waits for this wave's LDS
+// operations, not for its outstanding global loads.
+__device__ inline void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // Sum NV values over the workgroup; result is broadcast to every lane.
 // scratch: LDS of at least NV*16 elements.  Ends with a barrier so scratch can

@@ -19,8 +19,8 @@ namespace qmcx {
 
 template <typename T>
 struct SysP {
-  Cell<T> cell;
-  double spoG[9];
+  const Cell<T>* cell;        // device copy of the minimum-image cell
+  const double* spoG;         // [9] inverse of the SPO lattice
   int ng[3];
   int N, n, M, nspecies;
   int npad;
@@ -42,7 +42,7 @@ struct BatchP {
   T* J2;
   T* J1;
   T* Vbuf;
-  T* Binv;
+  double* Binv;    // capacitance-matrix inverse kept in FP64 in both precisions
   int* rows;
   int* ndelay;
   double* logdet;
@@ -61,17 +61,17 @@ struct MoveArgs {
 
 // ------------------------------------------------------------------ LDS plan
 struct MoveLds {
-  size_t oR, oIon, oRng, oRedD, oInv0, oInv, oPsi, oBinv, oP, oRedT, oPart, oRows, oSpec, oFn, total;
+  unsigned oR, oIon, oRng, oRedD, oInv0, oInv, oPsi, oBinv, oP, oRedT, oPart, oRows, oSpec, oFn, oCell, oSpoG, total;
 };
-__host__ __device__ inline size_t lds_take(size_t& o, size_t bytes) {
-  size_t r = o;
-  o = (o + bytes + 15) & ~size_t(15);
+__host__ __device__ inline unsigned lds_take(unsigned& o, size_t bytes) {
+  unsigned r = o;
+  o = (unsigned)((o + bytes + 15) & ~size_t(15));
   return r;
 }
 template <typename T>
 __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int S, int npad) {
   MoveLds L;
-  size_t o = 0;
+  unsigned o = 0;
   L.oR = lds_take(o, sizeof(double) * 3 * N);
   L.oIon = lds_take(o, sizeof(double) * 3 * M);
   L.oRng = lds_take(o, sizeof(double) * 64 * 4);
@@ -79,13 +79,15 @@ __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int S, i
   L.oInv0 = lds_take(o, sizeof(T) * n);
   L.oInv = lds_take(o, sizeof(T) * n);
   L.oPsi = lds_take(o, sizeof(T) * 5 * n);
-  L.oBinv = lds_take(o, sizeof(T) * K * K);
-  L.oP = lds_take(o, sizeof(T) * (K + 4));
-  L.oRedT = lds_take(o, sizeof(T) * 18 * 16);
+  L.oBinv = lds_take(o, sizeof(double) * K * K);
+  L.oP = lds_take(o, sizeof(double) * (K + 4));
+  L.oRedT = lds_take(o, sizeof(double) * 17 * 16);
   L.oPart = lds_take(o, sizeof(T) * S * 5 * npad);
   L.oRows = lds_take(o, sizeof(int) * K);
   L.oSpec = lds_take(o, sizeof(int) * M);
   L.oFn = lds_take(o, sizeof(Functor<T>) * (kMaxSpecies + 2));
+  L.oCell = lds_take(o, sizeof(Cell<T>));
+  L.oSpoG = lds_take(o, sizeof(double) * 12);
   L.total = o;
   return L;
 }
@@ -96,51 +98,74 @@ __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int S, i
 // stencil slice owning 16/S of the (i,j) columns of the 4x4x4 stencil; each
 // lane issues 4*16/S coalesced 16-byte loads along the orbital axis.  Slice
 // partials go to LDS `part` [S][NO][npad] and are summed by `spline_finish`.
-template <typename T, int NO>
-__device__ inline void spline_gather(const SysP<T>& S, double x, double y, double z, int slices, T* part) {
+// weight, first and second derivative of cubic B-spline knot i (0..3) at fraction t
+template <typename T>
+__device__ inline void knot_weight(int i, T t, T& w, T& dw, T& d2w) {
+  const T s6 = T(1) / T(6);
+  // w = ((p3 t + p2) t + p1) t + p0, dw = (q2 t + q1) t + q0, d2w = r1 t + r0
+  const T p3 = i == 0 ? T(-1) : i == 1 ? T(3) : i == 2 ? T(-3) : T(1);
+  const T p2 = i == 0 ? T(3) : i == 1 ? T(-6) : i == 2 ? T(3) : T(0);
+  const T p1 = i == 0 ? T(-3) : i == 1 ? T(0) : i == 2 ? T(3) : T(0);
+  const T p0 = i == 0 ? T(1) : i == 1 ? T(4) : i == 2 ? T(1) : T(0);
+  w = (((p3 * t + p2) * t + p1) * t + p0) * s6;
+  dw = ((T(3) * p3 * t + T(2) * p2) * t + p1) * s6;
+  d2w = (T(6) * p3 * t + T(2) * p2) * s6;
+}
+
+template <typename T, int NO, int PER>
+__device__ inline void spline_gather(const SysP<T>& S, const double* spoG, double x, double y, double z, T* part) {
   using VT = typename V16<T>::type;
   constexpr int VW = V16<T>::n;
+  constexpr int SL = 16 / PER;
+  // loads kept in flight per batch: all of them in FP32, two stencil columns in FP64
+  constexpr int G = (sizeof(T) == 8 && PER > 2) ? 2 : PER;
   const int nq = S.npad / VW;
   const int tid = threadIdx.x;
   const int q = tid % nq, s = tid / nq;
-  if (s >= slices) return;
+  if (s >= SL) return;
   int ix[3];
   double tt[3];
-  spline_locate(S.spoG, S.ng, x, y, z, ix, tt);
-  T a[4], da[4], d2a[4], b[4], db[4], d2b[4], c[4], dc[4], d2c[4];
-  bspline_weights<T>((T)tt[0], a, da, d2a);
-  bspline_weights<T>((T)tt[1], b, db, d2b);
+  spline_locate(spoG, S.ng, x, y, z, ix, tt);
+  const T* base0 = S.coefs + (size_t)ix[0] * S.xs + (size_t)ix[1] * S.ys + (size_t)ix[2] * S.zs + (size_t)q * VW;
+  const size_t zs = S.zs / VW;
+  T c[4], dc[4], d2c[4];
   bspline_weights<T>((T)tt[2], c, dc, d2c);
+  const T tx = (T)tt[0], ty = (T)tt[1];
   T acc[NO][VW];
 #pragma unroll
   for (int o = 0; o < NO; ++o)
 #pragma unroll
     for (int e = 0; e < VW; ++e) acc[o][e] = 0;
-  const int per = 16 / slices;
-  const T* base0 = S.coefs + (size_t)ix[0] * S.xs + (size_t)ix[1] * S.ys + (size_t)ix[2] * S.zs + (size_t)q * VW;
-  for (int pp = 0; pp < per; ++pp) {
-    const int p = s * per + pp;
-    const int i = p >> 2, j = p & 3;
-    const VT* src = reinterpret_cast<const VT*>(base0 + (size_t)i * S.xs + (size_t)j * S.ys);
-    VT v4[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v4[k] = __builtin_nontemporal_load(src + (size_t)k * (S.zs / VW));
-    const T w00 = a[i] * b[j], w10 = da[i] * b[j], w01 = a[i] * db[j], w20 = d2a[i] * b[j];
-    const T w11 = da[i] * db[j], w02 = a[i] * d2b[j];
+  for (int g0 = 0; g0 < PER; g0 += G) {
+  VT v4[G][4];
+#pragma unroll
+  for (int pp = 0; pp < G; ++pp) {
+    const int p = s * PER + g0 + pp;
+    const VT* src = reinterpret_cast<const VT*>(base0 + (size_t)(p >> 2) * S.xs + (size_t)(p & 3) * S.ys);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v4[pp][k] = __builtin_nontemporal_load(src + k * zs);
+  }
+#pragma unroll
+  for (int pp = 0; pp < G; ++pp) {
+    const int p = s * PER + g0 + pp;
+    T ai, dai, d2ai, bj, dbj, d2bj;
+    knot_weight<T>(p >> 2, tx, ai, dai, d2ai);
+    knot_weight<T>(p & 3, ty, bj, dbj, d2bj);
+    const T w00 = ai * bj, w10 = dai * bj, w01 = ai * dbj, w20 = d2ai * bj;
+    const T w11 = dai * dbj, w02 = ai * d2bj;
 #pragma unroll
     for (int e = 0; e < VW; ++e) {
-      T qv[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) qv[k] = v4[k][e];
-      const T s0 = c[0] * qv[0] + c[1] * qv[1] + c[2] * qv[2] + c[3] * qv[3];
-      const T s1 = dc[0] * qv[0] + dc[1] * qv[1] + dc[2] * qv[2] + dc[3] * qv[3];
+      const T q0 = v4[pp][0][e], q1 = v4[pp][1][e], q2 = v4[pp][2][e], q3 = v4[pp][3][e];
+      const T s0 = c[0] * q0 + c[1] * q1 + c[2] * q2 + c[3] * q3;
+      const T s1 = dc[0] * q0 + dc[1] * q1 + dc[2] * q2 + dc[3] * q3;
       acc[0][e] += w00 * s0;
       acc[1][e] += w10 * s0;
       acc[2][e] += w01 * s0;
       acc[3][e] += w00 * s1;
       acc[4][e] += w20 * s0;
       if constexpr (NO == 10) {
-        const T s2 = d2c[0] * qv[0] + d2c[1] * qv[1] + d2c[2] * qv[2] + d2c[3] * qv[3];
+        const T s2 = d2c[0] * q0 + d2c[1] * q1 + d2c[2] * q2 + d2c[3] * q3;
         acc[5][e] += w11 * s0;
         acc[6][e] += w10 * s1;
         acc[7][e] += w02 * s0;
@@ -149,10 +174,30 @@ __device__ inline void spline_gather(const SysP<T>& S, double x, double y, doubl
       }
     }
   }
+  }
 #pragma unroll
   for (int o = 0; o < NO; ++o)
 #pragma unroll
     for (int e = 0; e < VW; ++e) part[((size_t)s * NO + o) * S.npad + q * VW + e] = acc[o][e];
+}
+
+template <typename T, int NO>
+__device__ inline void spline_gather_rt(const SysP<T>& S, double x, double y, double z, int slices, T* part) {
+  switch (slices) {
+    case 16: spline_gather<T, NO, 1>(S, S.spoG, x, y, z, part); break;
+    case 8: spline_gather<T, NO, 2>(S, S.spoG, x, y, z, part); break;
+    case 4: spline_gather<T, NO, 4>(S, S.spoG, x, y, z, part); break;
+    case 2: spline_gather<T, NO, 8>(S, S.spoG, x, y, z, part); break;
+    default: spline_gather<T, NO, 16>(S, S.spoG, x, y, z, part); break;
+  }
+}
+
+// delta_inv powers of output o: 1, gx, gy, gz, gx^2, gx gy, gx gz, gy^2, gy gz, gz^2
+template <typename T>
+__device__ inline T vgh_scale(int o, T gx, T gy, T gz) {
+  const T f1 = (o == 1 || o == 4 || o == 5 || o == 6) ? gx : (o == 2 || o == 7 || o == 8) ? gy : (o == 3 || o == 9) ? gz : T(1);
+  const T f2 = (o == 4) ? gx : (o == 5 || o == 7) ? gy : (o == 6 || o == 8 || o == 9) ? gz : T(1);
+  return f1 * f2;
 }
 
 // Sum the slice partials and scale by the grid's delta_inv powers.
@@ -160,39 +205,89 @@ __device__ inline void spline_gather(const SysP<T>& S, double x, double y, doubl
 template <typename T, int NO, typename TO>
 __device__ inline void spline_finish(const SysP<T>& S, int slices, const T* part, TO* out, int ostride, int norb) {
   const T gx = (T)S.ng[0], gy = (T)S.ng[1], gz = (T)S.ng[2];
-  const T scale[10] = {T(1), gx, gy, gz, gx * gx, gx * gy, gx * gz, gy * gy, gy * gz, gz * gz};
   for (int idx = threadIdx.x; idx < NO * norb; idx += blockDim.x) {
     const int o = idx / norb, j = idx - o * norb;
     T sum = 0;
     for (int s = 0; s < slices; ++s) sum += part[((size_t)s * NO + o) * S.npad + j];
-    out[(size_t)o * ostride + j] = (TO)(sum * scale[o]);
+    out[(size_t)o * ostride + j] = (TO)(sum * vgh_scale<T>(o, gx, gy, gz));
   }
 }
 
 // ------------------------------------------------------------------ fused moves
+// Sum the wave partials red[v*16 + q] (q < nw) of kRed values; every lane of every
+// wave gets the same totals (same order), broadcast from lane 16*g of its wave.
+constexpr int kRed = 17;
+__device__ inline void combine_partials(const double* red, int nw, double tot[kRed]) {
+  const int lane = threadIdx.x & 63, q = lane & 15;
+#pragma unroll
+  for (int base = 0; base < kRed; base += 4) {
+    const int v = base + (lane >> 4);
+    double x = (v < kRed && q < nw) ? red[v * 16 + q] : 0.0;
+    x += __shfl_xor(x, 8, 64);
+    x += __shfl_xor(x, 4, 64);
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 1, 64);
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+      if (base + g < kRed) tot[base + g] = __shfl(x, g * 16, 64);
+  }
+}
+
+#ifdef QMCX_PHASE_TIMING
+// wall_clock64 ticks per phase, accumulated by lane 0 of workgroup 0
+__device__ unsigned long long qmcx_phase_ticks[8];
+#define QMCX_TICK(slot)                                                  \
+  if (w == 0 && tid == 0) {                                              \
+    const unsigned long long now_ = wall_clock64();                      \
+    qmcx_phase_ticks[slot] += now_ - tick_;                              \
+    tick_ = now_;                                                        \
+  }
+#else
+#define QMCX_TICK(slot)
+#endif
+
+// J2 pair terms of electron j against the moving electron iel at (x, y, z):
+// u, du/r, d2u and the minimum-image displacement (flip index iel).
+template <typename T>
+__device__ inline void j2_pair(const Cell<T>& C, const Functor<T>& f, double xj, double yj, double zj, double x, double y,
+                               double z, T flip, T& u, T& du, T& d2u, T& dx, T& dy, T& dz) {
+  const T r = min_image(C, (T)(xj - x), (T)(yj - y), (T)(zj - z), flip, dx, dy, dz);
+  u = du = d2u = 0;
+  if (r < f.rcut) functor_vgl(f, r, u, du, d2u);
+}
+
 // One workgroup = one walker; electrons [e_begin, e_end) of determinant `det`
 // are moved in order.  With K = 1 every accepted move updates Ainv in place
 // (Sherman-Morrison, DelayedUpdate.h:166-173); with K > 1 accepted rows are
-// delayed (DelayedUpdate.h:124-153) and k_flush applies them afterwards.
-template <typename T>
-__global__ __launch_bounds__(1024) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs a) {
+// delayed (DelayedUpdate.h:124-153) and the flush kernel applies them afterwards.
+// Reductions: each wave sums its lanes, lane 0 parks the partial in sRedT[v][wave],
+// and after one barrier every lane adds the wave partials in the same order.
+constexpr int kMoveThreads = 512;
+template <typename T, int JPT, int PER>
+__global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
   const int N = S.N, n = S.n, M = S.M, K = B.K;
-  const MoveLds L = move_lds<T>(N, M, n, K, a.slices, S.npad);
+  const MoveLds L = move_lds<T>(N, M, n, K, 16 / PER, S.npad);
   double* sR = (double*)(smem + L.oR);
   double* sIon = (double*)(smem + L.oIon);
   double* sRng = (double*)(smem + L.oRng);
   T* sInv0 = (T*)(smem + L.oInv0);
   T* sInv = (T*)(smem + L.oInv);
   T* sPsi = (T*)(smem + L.oPsi);
-  T* sBinv = (T*)(smem + L.oBinv);
-  T* sP = (T*)(smem + L.oP);
-  T* sRedT = (T*)(smem + L.oRedT);
+  double* sBinv = (double*)(smem + L.oBinv);
+  double* sP = (double*)(smem + L.oP);
+  double* sRed = (double*)(smem + L.oRedT);
   T* sPart = (T*)(smem + L.oPart);
   int* sRows = (int*)(smem + L.oRows);
   int* sSpec = (int*)(smem + L.oSpec);
   Functor<T>* sFn = (Functor<T>*)(smem + L.oFn);
+  Cell<T>* sCell = (Cell<T>*)(smem + L.oCell);
+  double* sSpoG = (double*)(smem + L.oSpoG);
+#ifdef QMCX_PHASE_TIMING
+  unsigned long long tick_ = wall_clock64();
+#endif
 
   for (int i = tid; i < 3 * N; i += nt) sR[i] = B.R[(size_t)w * 3 * N + i];
   for (int i = tid; i < 3 * M; i += nt) sIon[i] = S.ions[i];
@@ -203,6 +298,8 @@ __global__ __launch_bounds__(1024) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs
     int* dst = (int*)sFn;
     for (int i = tid; i < words; i += nt) dst[i] = src[i];
   }
+  if (tid == 0) *sCell = *S.cell;
+  if (tid < 9) sSpoG[tid] = S.spoG[tid];
   const size_t moff = ((size_t)w * 2 + a.det) * n * n;
   T* Ainv = B.Ainv + moff;
   T* Amat = B.A + moff;
@@ -215,8 +312,10 @@ __global__ __launch_bounds__(1024) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs
   int c = 0;
   double logd = 0, ph = 0, jl1 = 0, jl2 = 0;
   unsigned long long nacc = 0;
-  const int JPT = (N + nt - 1) / nt;   // electrons owned per lane (<= 2, checked on the host)
   __syncthreads();
+  const Cell<T>& C = *sCell;
+  // Barriers inside a move only publish LDS data (lds_sync); the one at the end of
+  // a move also publishes the global writes of an accepted move (Ainv, rows, J).
 
   for (int iel = a.e_begin; iel < a.e_end; ++iel) {
     const int row = iel - a.det * n;
@@ -230,36 +329,44 @@ __global__ __launch_bounds__(1024) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs
         sRng[tid * 4 + 2] = nr[2];
         sRng[tid * 4 + 3] = u;
       }
-      __syncthreads();
+      lds_sync();
     }
+    QMCX_TICK(0)
     // stored state of the moving electron, read before any lane can modify it
     const T Vat_old = J1[iel], Uat_old = J2[iel];
-    const T g1x = J1[N + iel], g1y = J1[2 * N + iel], g1z = J1[3 * N + iel];
-    const T g2x = J2[N + iel], g2y = J2[2 * N + iel], g2z = J2[3 * N + iel];
 
-    // ---- getInvRow (DelayedUpdate.h:96-114): p = U Ainv[row]
+    // ---- getInvRow (DelayedUpdate.h:96-114): p = U Ainv[row]; small algebra in FP64
     for (int j = tid; j < n; j += nt) sInv0[j] = Ainv[(size_t)row * n + j];
     if (c > 0) {
-      const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
       const T* ar = Ainv + (size_t)row * n;
-      for (int d = wid; d < c; d += nw) {
-        const T* u = Amat + (size_t)sRows[d] * n;
-        T s = 0;
-        for (int j = lane; j < n; j += 64) s += u[j] * ar[j];
-        s = wave_sum(s);
-        if (lane == 0) sP[d] = s;
+      for (int d0 = wid * 2; d0 < c; d0 += nw * 2) {
+        const T* u0 = Amat + (size_t)sRows[d0] * n;
+        const T* u1 = Amat + (size_t)sRows[d0 + 1 < c ? d0 + 1 : d0] * n;
+        double s0 = 0, s1 = 0;
+        for (int j = lane; j < n; j += 64) {
+          const double av = (double)ar[j];
+          s0 += (double)u0[j] * av;
+          s1 += (double)u1[j] * av;
+        }
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        if (lane == 0) {
+          sP[d0] = s0;
+          if (d0 + 1 < c) sP[d0 + 1] = s1;
+        }
       }
     }
-    __syncthreads();
+    lds_sync();
     if (c > 0) {
       // y = Binv^T p, parked in row c of Binv (acceptRow reads it)
       for (int r = tid; r < c; r += nt) {
-        T s = 0;
+        double s = 0;
         for (int q = 0; q < c; ++q) s += sBinv[q * K + r] * sP[q];
         sBinv[c * K + r] = s;
       }
-      __syncthreads();
+      lds_sync();
     }
+    QMCX_TICK(1)
 
     // ---- proposal
     const double x0 = sR[iel], y0 = sR[N + iel], z0 = sR[2 * N + iel];
@@ -267,98 +374,132 @@ __global__ __launch_bounds__(1024) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs
     const double ny = y0 + a.sqrt_tau * sRng[slot * 4 + 1];
     const double nz = z0 + a.sqrt_tau * sRng[slot * 4 + 2];
 
-    T red[17];
-#pragma unroll
-    for (int i = 0; i < 17; ++i) red[i] = 0;
-
     // invRow = Ainv[row] - V^T y ; evalGrad of the determinant (DiracDeterminant.cpp:80-89)
-    for (int j = tid; j < n; j += nt) {
-      T v = sInv0[j];
-      for (int d = 0; d < c; ++d) v -= Ainv[(size_t)sRows[d] * n + j] * sBinv[c * K + d];
-      sInv[j] = v;
+    {
+      double g0 = 0, g1 = 0, g2 = 0;
       const T* dp = dpsiM + (size_t)row * 3 * n;
-      red[0] += v * dp[j];
-      red[1] += v * dp[n + j];
-      red[2] += v * dp[2 * n + j];
-    }
-
-    // distance rows (old and proposed position) + J2 terms, one electron per lane slot
-    T ou[2], odu[2], od2u[2], odx[2], ody[2], odz[2], cu[2], cdu[2], cd2u[2], cdx[2], cdy[2], cdz[2];
-    const bool up = iel < n;
-    for (int q = 0; q < JPT; ++q) {
-      const int j = tid + q * nt;
-      ou[q] = odu[q] = od2u[q] = odx[q] = ody[q] = odz[q] = 0;
-      cu[q] = cdu[q] = cd2u[q] = cdx[q] = cdy[q] = cdz[q] = 0;
-      if (j < N && j != iel) {
-        const T flip = j < iel ? T(1) : T(-1);
-        const Functor<T>& f = sFn[kMaxSpecies + (((j < n) == up) ? 0 : 1)];
-        const double xj = sR[j], yj = sR[N + j], zj = sR[2 * N + j];
-        T r = min_image(S.cell, (T)(xj - x0), (T)(yj - y0), (T)(zj - z0), flip, odx[q], ody[q], odz[q]);
-        if (r < f.rcut) functor_vgl(f, r, ou[q], odu[q], od2u[q]);
-        r = min_image(S.cell, (T)(xj - nx), (T)(yj - ny), (T)(zj - nz), flip, cdx[q], cdy[q], cdz[q]);
-        if (r < f.rcut) functor_vgl(f, r, cu[q], cdu[q], cd2u[q]);
-        red[7] += cu[q];
-        red[8] += cdu[q] * cdx[q];
-        red[9] += cdu[q] * cdy[q];
-        red[10] += cdu[q] * cdz[q];
-        red[11] += cd2u[q] + T(2) * cdu[q];
+      const double* y = sBinv + c * K;
+      for (int j = tid; j < n; j += nt) {
+        double v = (double)sInv0[j];
+        int d = 0;
+        for (; d + 4 <= c; d += 4) {
+          const T a0 = Ainv[(size_t)sRows[d] * n + j], a1 = Ainv[(size_t)sRows[d + 1] * n + j];
+          const T a2 = Ainv[(size_t)sRows[d + 2] * n + j], a3 = Ainv[(size_t)sRows[d + 3] * n + j];
+          v -= (double)a0 * y[d] + (double)a1 * y[d + 1] + (double)a2 * y[d + 2] + (double)a3 * y[d + 3];
+        }
+        for (; d < c; ++d) v -= (double)Ainv[(size_t)sRows[d] * n + j] * y[d];
+        const T vt = (T)v;
+        sInv[j] = vt;
+        g0 += (double)vt * (double)dp[j];
+        g1 += (double)vt * (double)dp[n + j];
+        g2 += (double)vt * (double)dp[2 * n + j];
+      }
+      g0 = wave_sum(g0);
+      g1 = wave_sum(g1);
+      g2 = wave_sum(g2);
+      if (lane == 0) {
+        sRed[0 * 16 + wid] = g0;
+        sRed[1 * 16 + wid] = g1;
+        sRed[2 * 16 + wid] = g2;
       }
     }
-    // J1 against the ions at the proposed position (OneBodyJastrow.h:259-268)
-    for (int ci = tid; ci < M; ci += nt) {
-      const Functor<T>& f = sFn[sSpec[ci]];
-      if (f.ncoef > 0) {
-        T dx, dy, dz;
-        const T r = min_image(S.cell, (T)(sIon[ci] - nx), (T)(sIon[M + ci] - ny), (T)(sIon[2 * M + ci] - nz), T(-1),
-                              dx, dy, dz);
-        if (r < f.rcut) {
-          T u, du, d2u;
-          functor_vgl(f, r, u, du, d2u);
-          red[12] += u;
-          red[13] += d2u + T(2) * du;
-          red[14] += du * dx;
-          red[15] += du * dy;
-          red[16] += du * dz;
+    // J2 over the proposed row (TwoBodyJastrow.h:294-304) and J1 (OneBodyJastrow.h:259-268)
+    {
+      T r7 = 0, r8 = 0, r9 = 0, r10 = 0, r11 = 0, r12 = 0, r13 = 0, r14 = 0, r15 = 0, r16 = 0;
+      const bool up = iel < n;
+#pragma unroll
+      for (int q = 0; q < JPT; ++q) {
+        const int j = tid + q * nt;
+        if (j < N && j != iel) {
+          const Functor<T>& f = sFn[kMaxSpecies + (((j < n) == up) ? 0 : 1)];
+          T u, du, d2u, dx, dy, dz;
+          j2_pair(C, f, sR[j], sR[N + j], sR[2 * N + j], nx, ny, nz, j < iel ? T(1) : T(-1), u, du, d2u, dx, dy, dz);
+          r7 += u;
+          r8 += du * dx;
+          r9 += du * dy;
+          r10 += du * dz;
+          r11 += d2u + T(2) * du;
         }
       }
+      for (int ci = tid; ci < M; ci += nt) {
+        const Functor<T>& f = sFn[sSpec[ci]];
+        if (f.ncoef > 0) {
+          T dx, dy, dz;
+          const T r = min_image(C, (T)(sIon[ci] - nx), (T)(sIon[M + ci] - ny), (T)(sIon[2 * M + ci] - nz),
+                                T(-1), dx, dy, dz);
+          if (r < f.rcut) {
+            T u, du, d2u;
+            functor_vgl(f, r, u, du, d2u);
+            r12 += u;
+            r13 += d2u + T(2) * du;
+            r14 += du * dx;
+            r15 += du * dy;
+            r16 += du * dz;
+          }
+        }
+      }
+      double v[10] = {r7, r8, r9, r10, r11, r12, r13, r14, r15, r16};
+#pragma unroll
+      for (int i = 0; i < 10; ++i) v[i] = wave_sum(v[i]);
+      if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < 10; ++i) sRed[(7 + i) * 16 + wid] = v[i];
     }
     // SPO at the proposed position (einspline_spo::evaluate, einspline_spo.hpp:204-223)
-    spline_gather<T, 5>(S, nx, ny, nz, a.slices, sPart);
-    __syncthreads();
-    spline_finish<T, 5, T>(S, a.slices, sPart, sPsi, n, n);
-    __syncthreads();
-    // ratioGrad_compute (DiracDeterminant.cpp:103-124)
-    for (int j = tid; j < n; j += nt) {
-      const T v = sInv[j];
-      red[3] += v * sPsi[j];
-      red[4] += v * sPsi[n + j];
-      red[5] += v * sPsi[2 * n + j];
-      red[6] += v * sPsi[3 * n + j];
+    spline_gather<T, 5, PER>(S, sSpoG, nx, ny, nz, sPart);
+    lds_sync();
+    QMCX_TICK(2)
+    spline_finish<T, 5, T>(S, 16 / PER, sPart, sPsi, n, n);
+    lds_sync();
+    QMCX_TICK(3)
+    // ratioGrad_compute (DiracDeterminant.cpp:103-124), FP64 accumulation
+    {
+      double d0 = 0, d1 = 0, d2 = 0, d3 = 0;
+      for (int j = tid; j < n; j += nt) {
+        const double v = (double)sInv[j];
+        d0 += v * (double)sPsi[j];
+        d1 += v * (double)sPsi[n + j];
+        d2 += v * (double)sPsi[2 * n + j];
+        d3 += v * (double)sPsi[3 * n + j];
+      }
+      d0 = wave_sum(d0);
+      d1 = wave_sum(d1);
+      d2 = wave_sum(d2);
+      d3 = wave_sum(d3);
+      if (lane == 0) {
+        sRed[3 * 16 + wid] = d0;
+        sRed[4 * 16 + wid] = d1;
+        sRed[5 * 16 + wid] = d2;
+        sRed[6 * 16 + wid] = d3;
+      }
     }
-    block_sum<T, 17>(red, sRedT);
+    lds_sync();
+    double red[kRed];
+    combine_partials(sRed, nw, red);
+    QMCX_TICK(4)
 
-    const T rdet = red[3];
-    const double ratio = (double)rdet * exp((double)(Vat_old - red[12])) * exp((double)(Uat_old - red[7]));
+    const double rdet = red[3];
+    const double ratio = rdet * exp((double)(Vat_old - (T)red[12])) * exp((double)(Uat_old - (T)red[7]));
     const double uacc = sRng[slot * 4 + 3];
     const bool accept = a.mode == 0 ? (uacc < 0.5) : (uacc < ratio * ratio);
     if (B.trace != nullptr && tid == 0) {
       double* t = B.trace + ((size_t)w * N + iel) * 9;
       t[0] = ratio;
-      t[1] = (double)rdet;
+      t[1] = rdet;
       t[2] = accept ? 1.0 : 0.0;
-      t[3] = (double)(red[0] + g1x + g2x);
-      t[4] = (double)(red[1] + g1y + g2y);
-      t[5] = (double)(red[2] + g1z + g2z);
-      t[6] = (double)(red[4] / rdet + red[14] + red[8]);
-      t[7] = (double)(red[5] / rdet + red[15] + red[9]);
-      t[8] = (double)(red[6] / rdet + red[16] + red[10]);
+      t[3] = red[0] + (double)J1[N + iel] + (double)J2[N + iel];
+      t[4] = red[1] + (double)J1[2 * N + iel] + (double)J2[2 * N + iel];
+      t[5] = red[2] + (double)J1[3 * N + iel] + (double)J2[3 * N + iel];
+      t[6] = red[4] / rdet + red[14] + red[8];
+      t[7] = red[5] / rdet + red[15] + red[9];
+      t[8] = red[6] / rdet + red[16] + red[10];
     }
     if (accept) {
       if (tid == 0) {
-        logd += log(fabs((double)rdet));
-        ph += rdet > T(0) ? 0.0 : M_PI;
-        jl1 += (double)(Vat_old - red[12]);
-        jl2 += (double)(Uat_old - red[7]);
+        logd += log(fabs(rdet));
+        ph += rdet > 0.0 ? 0.0 : M_PI;
+        jl1 += (double)(Vat_old - (T)red[12]);
+        jl2 += (double)(Uat_old - (T)red[7]);
         ++nacc;
       }
       // new Slater row and its derivatives (DiracDeterminant.cpp:135-141)
@@ -371,44 +512,67 @@ __global__ __launch_bounds__(1024) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs
       }
       if (K == 1) {
         // Fahy/Sherman-Morrison: Ainv[m] -= ((Ainv[m].psiV - delta_m,row) / ratio) * Ainv_old[row]
-        const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
-        const T rinv = T(1) / rdet;
+        // pass 1: t[m] for every row, four rows in flight per wave
+        double* tm = (double*)sPart;
+        for (int m0 = wid * 4; m0 < n; m0 += nw * 4) {
+          double sm[4] = {0, 0, 0, 0};
+          const T* am = Ainv + (size_t)m0 * n;
+          const int nr = min(4, n - m0);
+          for (int j = lane; j < n; j += 64) {
+            const double pv = (double)sPsi[j];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              if (r < nr) sm[r] += (double)am[(size_t)r * n + j] * pv;
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) sm[r] = wave_sum(sm[r]);
+          if (lane < nr) {
+            double mine = lane == 0 ? sm[0] : lane == 1 ? sm[1] : lane == 2 ? sm[2] : sm[3];
+            tm[m0 + lane] = (mine - (m0 + lane == row ? 1.0 : 0.0)) / rdet;
+          }
+        }
+        lds_sync();
+        // pass 2: rank-1 update, every element independent (wave per row, lanes over columns)
         for (int m = wid; m < n; m += nw) {
+          const T t = (T)tm[m];
           T* am = Ainv + (size_t)m * n;
-          T s = 0;
-          for (int j = lane; j < n; j += 64) s += am[j] * sPsi[j];
-          s = wave_sum(s);
-          const T t = (s - (m == row ? T(1) : T(0))) * rinv;
           for (int j = lane; j < n; j += 64) am[j] -= t * sInv0[j];
         }
       } else {
-        // acceptRow (DelayedUpdate.h:124-153)
+        // acceptRow (DelayedUpdate.h:124-153), FP64 capacitance algebra
         for (int j = tid; j < n; j += nt) Vb[(size_t)c * n + j] = sInv0[j];
-        {
-          const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
-          for (int d = wid; d <= c; d += nw) {
-            const T* v = (d == c) ? sInv0 : Vb + (size_t)d * n;
-            T s = 0;
-            for (int j = lane; j < n; j += 64) s += v[j] * sPsi[j];
-            s = wave_sum(s);
-            if (lane == 0) sP[d] = -s;
+        for (int d0 = wid * 2; d0 <= c; d0 += nw * 2) {
+          const int d1 = d0 + 1 <= c ? d0 + 1 : d0;
+          const T* v0 = (d0 == c) ? sInv0 : Vb + (size_t)d0 * n;
+          const T* v1 = (d1 == c) ? sInv0 : Vb + (size_t)d1 * n;
+          double s0 = 0, s1 = 0;
+          for (int j = lane; j < n; j += 64) {
+            const double pv = (double)sPsi[j];
+            s0 += (double)v0[j] * pv;
+            s1 += (double)v1[j] * pv;
+          }
+          s0 = wave_sum(s0);
+          s1 = wave_sum(s1);
+          if (lane == 0) {
+            sP[d0] = -s0;
+            if (d1 != d0) sP[d1] = -s1;
           }
         }
-        __syncthreads();
-        T y = -sP[c];
+        lds_sync();
+        double y = -sP[c];
         for (int i = 0; i < c; ++i) y += sBinv[c * K + i] * sP[i];
-        const T yi = T(1) / y;
+        const double yi = 1.0 / y;
         for (int s2 = tid; s2 < c; s2 += nt) {
-          T t = 0;
+          double t = 0;
           for (int r = 0; r < c; ++r) t += sBinv[s2 * K + r] * sP[r];
           sBinv[s2 * K + c] = yi * t;
         }
-        __syncthreads();
+        lds_sync();
         for (int idx = tid; idx < c * c; idx += nt) {
           const int s2 = idx / c, r = idx - s2 * c;
           sBinv[s2 * K + r] -= sBinv[c * K + r] * sBinv[s2 * K + c];
         }
-        __syncthreads();
+        lds_sync();
         for (int i = tid; i < c; i += nt) sBinv[c * K + i] *= -yi;
         if (tid == 0) {
           sBinv[c * K + c] = yi;
@@ -416,34 +580,47 @@ __global__ __launch_bounds__(1024) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs
         }
         ++c;
       }
-      // J1 (OneBodyJastrow.h:271-283) and J2 (TwoBodyJastrow.h:307-353)
+      // J1 (OneBodyJastrow.h:271-283) and J2 (TwoBodyJastrow.h:307-353): the pair
+      // terms at the old and the accepted position are recomputed here.
+      const bool up = iel < n;
+#pragma unroll
       for (int q = 0; q < JPT; ++q) {
         const int j = tid + q * nt;
         if (j < N && j != iel) {
-          const T newl = cd2u[q] + T(2) * cdu[q];
-          J2[j] += cu[q] - ou[q];
-          J2[4 * N + j] += od2u[q] + T(2) * odu[q] - newl;
-          J2[N + j] -= cdu[q] * cdx[q] - odu[q] * odx[q];
-          J2[2 * N + j] -= cdu[q] * cdy[q] - odu[q] * ody[q];
-          J2[3 * N + j] -= cdu[q] * cdz[q] - odu[q] * odz[q];
+          const Functor<T>& f = sFn[kMaxSpecies + (((j < n) == up) ? 0 : 1)];
+          const T flip = j < iel ? T(1) : T(-1);
+          const double xj = sR[j], yj = sR[N + j], zj = sR[2 * N + j];
+          T ou, odu, od2u, odx, ody, odz, cu, cdu, cd2u, cdx, cdy, cdz;
+          j2_pair(C, f, xj, yj, zj, x0, y0, z0, flip, ou, odu, od2u, odx, ody, odz);
+          j2_pair(C, f, xj, yj, zj, nx, ny, nz, flip, cu, cdu, cd2u, cdx, cdy, cdz);
+          const T newl = cd2u + T(2) * cdu;
+          J2[j] += cu - ou;
+          J2[4 * N + j] += od2u + T(2) * odu - newl;
+          J2[N + j] -= cdu * cdx - odu * odx;
+          J2[2 * N + j] -= cdu * cdy - odu * ody;
+          J2[3 * N + j] -= cdu * cdz - odu * odz;
         } else if (j == iel) {
-          J2[iel] = red[7];
-          J2[N + iel] = red[8];
-          J2[2 * N + iel] = red[9];
-          J2[3 * N + iel] = red[10];
-          J2[4 * N + iel] = -red[11];
-          J1[iel] = red[12];
-          J1[N + iel] = red[14];
-          J1[2 * N + iel] = red[15];
-          J1[3 * N + iel] = red[16];
-          J1[4 * N + iel] = red[13];
-          sR[iel] = nx;
-          sR[N + iel] = ny;
-          sR[2 * N + iel] = nz;
+          J2[iel] = (T)red[7];
+          J2[N + iel] = (T)red[8];
+          J2[2 * N + iel] = (T)red[9];
+          J2[3 * N + iel] = (T)red[10];
+          J2[4 * N + iel] = (T)(-red[11]);
+          J1[iel] = (T)red[12];
+          J1[N + iel] = (T)red[14];
+          J1[2 * N + iel] = (T)red[15];
+          J1[3 * N + iel] = (T)red[16];
+          J1[4 * N + iel] = (T)red[13];
         }
+      }
+      lds_sync();   // every lane has read the old position
+      if (tid == 0) {
+        sR[iel] = nx;
+        sR[N + iel] = ny;
+        sR[2 * N + iel] = nz;
       }
     }
     __syncthreads();
+    QMCX_TICK(5)
   }
 
   for (int i = tid; i < 3 * N; i += nt) B.R[(size_t)w * 3 * N + i] = sR[i];
@@ -481,7 +658,7 @@ __global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, i
   T* Ainv = B.Ainv + moff;
   const T* Amat = B.A + moff;
   const T* Vb = B.Vbuf + (size_t)w * K * n;
-  const T* Bi = B.Binv + (size_t)w * K * K;
+  const double* Bi = B.Binv + (size_t)w * K * K;
   for (int i = tid; i < c; i += nt) sRows[i] = B.rows[(size_t)w * K + i];
   __syncthreads();
   const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
@@ -499,9 +676,9 @@ __global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, i
   __syncthreads();
   for (int idx = tid; idx < kFlushRows * c; idx += nt) {
     const int m = idx / c, d = idx - m * c;
-    T s = 0;
-    for (int e = 0; e < c; ++e) s += sT[m * K + e] * Bi[e * K + d];
-    sT2[m * K + d] = s;
+    double s = 0;
+    for (int e = 0; e < c; ++e) s += (double)sT[m * K + e] * Bi[e * K + d];
+    sT2[m * K + d] = (T)s;
   }
   __syncthreads();
   for (int j = tid; j < n; j += nt) {
@@ -516,6 +693,99 @@ __global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, i
 #pragma unroll
     for (int m = 0; m < kFlushRows; ++m)
       if (m0 + m < n) Ainv[(size_t)(m0 + m) * n + j] -= acc[m];
+  }
+}
+
+// FP32 flush on the matrix cores (v_mfma_f32_16x16x4f32), 32 rows per workgroup,
+// 4 waves.  16x16x4 operand layout: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; accumulator register r of lane l is D[4*(l/16)+r][l%16].
+// The K index of GEMM 1 (orbitals) is permuted per lane group g = l/16: group g
+// walks columns [g*n/4, (g+1)*n/4) with float4 loads, four MFMAs per load, which
+// sums the same products in another order.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+constexpr int kFlushBM = 32;
+__global__ __launch_bounds__(256) void k_flush_mfma(SysP<float> S, BatchP<float> B, int det) {
+  const int w = blockIdx.y, m0 = blockIdx.x * kFlushBM;
+  const int c = B.ndelay[w];
+  if (c == 0) return;
+  const int n = S.n, K = B.K, tid = threadIdx.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  const int cp = (c + 15) & ~15;           // delayed rows padded to a 16-wide tile
+  extern __shared__ __align__(16) unsigned char smem[];
+  float* sT = (float*)smem;                // [BM][cp]   T = Ainv U^T - E
+  float* sT2 = sT + kFlushBM * cp;         // [BM][cp]  -T Binv
+  int* sRows = (int*)(sT2 + kFlushBM * cp);
+  const size_t moff = ((size_t)w * 2 + det) * n * n;
+  float* Ainv = B.Ainv + moff;
+  const float* Amat = B.A + moff;
+  const float* Vb = B.Vbuf + (size_t)w * K * n;
+  const double* Bi = B.Binv + (size_t)w * K * K;
+  for (int i = tid; i < c; i += 256) sRows[i] = B.rows[(size_t)w * K + i];
+  __syncthreads();
+  const int li = lane & 15, g = lane >> 4;
+  const int q4 = n / 4;                    // columns per lane group (n % 16 == 0)
+  // ---- GEMM 1: T[32][cp] = Ainv[m0:m0+32][:] . U^T
+  const int ntile1 = 2 * (cp / 16);
+  for (int t = wid; t < ntile1; t += 4) {
+    const int rt = t & 1, ct = t >> 1;
+    const int arow = m0 + rt * 16 + li;
+    const bool aok = arow < n;               // the last block may run past row n-1
+    const int d = ct * 16 + li;
+    const bool dok = d < c;
+    const float* ap = Ainv + (size_t)(aok ? arow : 0) * n + g * q4;
+    const float* bp = Amat + (size_t)(dok ? sRows[d] : 0) * n + g * q4;
+    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < q4; k += 4) {
+      f32x4_t av = *reinterpret_cast<const f32x4_t*>(ap + k);
+      f32x4_t bv = *reinterpret_cast<const f32x4_t*>(bp + k);
+      if (!aok) av = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      if (!dok) bv = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[0], bv[0], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[1], bv[1], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[2], bv[2], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[3], bv[3], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = rt * 16 + 4 * g + r;      // row within the block
+      const int dd = ct * 16 + li;            // delayed-row index
+      float v = acc[r];
+      if (dd < c && sRows[dd] == m0 + i) v -= 1.f;
+      sT[i * cp + dd] = dd < c ? v : 0.f;
+    }
+  }
+  __syncthreads();
+  // ---- T2 = -(T Binv), FP64 products of the small capacitance inverse
+  for (int idx = tid; idx < kFlushBM * cp; idx += 256) {
+    const int i = idx / cp, dd = idx - i * cp;
+    double sum = 0;
+    if (dd < c)
+      for (int e = 0; e < c; ++e) sum += (double)sT[i * cp + e] * Bi[e * K + dd];
+    sT2[i * cp + dd] = (float)(-sum);
+  }
+  __syncthreads();
+  // ---- GEMM 2: Ainv[m0:m0+32][:] += T2 . V
+  const int ntile2 = 2 * (n / 16);
+  for (int t = wid; t < ntile2; t += 4) {
+    const int rt = t & 1, ct = t >> 1;
+    const int col = ct * 16 + li;
+    f32x4_t acc;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = m0 + rt * 16 + 4 * g + r;
+      acc[r] = row < n ? Ainv[(size_t)row * n + col] : 0.f;
+    }
+    for (int k0 = 0; k0 < cp; k0 += 4) {
+      const int kk = k0 + g;
+      const float av = sT2[(rt * 16 + li) * cp + kk];
+      const float bv = kk < c ? Vb[(size_t)kk * n + col] : 0.f;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = m0 + rt * 16 + 4 * g + r;
+      if (row < n) Ainv[(size_t)row * n + col] = acc[r];
+    }
   }
 }
 
@@ -619,7 +889,7 @@ __global__ __launch_bounds__(1024) void k_spo_rows(SysP<T> S, BatchP<T> B, int s
   T* out = part + (size_t)slices * 5 * S.npad;  // [5][n]
   const int e = blockIdx.x, w = blockIdx.y, N = S.N, n = S.n;
   const double* R = B.R + (size_t)w * 3 * N;
-  spline_gather<T, 5>(S, R[e], R[N + e], R[2 * N + e], slices, part);
+  spline_gather_rt<T, 5>(S, R[e], R[N + e], R[2 * N + e], slices, part);
   __syncthreads();
   spline_finish<T, 5, T>(S, slices, part, out, n, n);
   __syncthreads();
@@ -643,7 +913,7 @@ __global__ __launch_bounds__(1024) void k_spline_vgh(SysP<T> S, const double* r,
   extern __shared__ __align__(16) unsigned char smem[];
   T* part = (T*)smem;
   const int p = blockIdx.x;
-  spline_gather<T, 10>(S, r[3 * p], r[3 * p + 1], r[3 * p + 2], slices, part);
+  spline_gather_rt<T, 10>(S, r[3 * p], r[3 * p + 1], r[3 * p + 2], slices, part);
   __syncthreads();
   spline_finish<T, 10, double>(S, slices, part, out + (size_t)p * 10 * norb, norb, norb);
 }
@@ -675,6 +945,8 @@ __global__ __launch_bounds__(1024) void k_jastrow_init(SysP<T> S, BatchP<T> B) {
   double* sR = (double*)smem;
   double* sRed = sR + 3 * N;
   Functor<T>* sFn = (Functor<T>*)(sRed + 32);
+  Cell<T>* sCell = (Cell<T>*)(sFn + kMaxSpecies + 2);
+  if (tid == 0) *sCell = *S.cell;
   for (int i = tid; i < 3 * N; i += nt) sR[i] = B.R[(size_t)w * 3 * N + i];
   {
     const int words = (int)(sizeof(Functor<T>) * (kMaxSpecies + 2) / 4);
@@ -691,7 +963,7 @@ __global__ __launch_bounds__(1024) void k_jastrow_init(SysP<T> S, BatchP<T> B) {
       const Functor<T>& f = sFn[S.species[ci]];
       if (f.ncoef == 0) continue;
       T dx, dy, dz;
-      const T r = min_image(S.cell, (T)(S.ions[ci] - x), (T)(S.ions[M + ci] - y), (T)(S.ions[2 * M + ci] - z), T(-1),
+      const T r = min_image(*sCell, (T)(S.ions[ci] - x), (T)(S.ions[M + ci] - y), (T)(S.ions[2 * M + ci] - z), T(-1),
                             dx, dy, dz);
       if (r < f.rcut) {
         T u, du, d2u;
@@ -715,7 +987,7 @@ __global__ __launch_bounds__(1024) void k_jastrow_init(SysP<T> S, BatchP<T> B) {
       if (j == i) continue;
       const Functor<T>& f = sFn[kMaxSpecies + (((j < n) == up) ? 0 : 1)];
       T dx, dy, dz;
-      const T r = min_image(S.cell, (T)(sR[j] - x), (T)(sR[N + j] - y), (T)(sR[2 * N + j] - z),
+      const T r = min_image(*sCell, (T)(sR[j] - x), (T)(sR[N + j] - y), (T)(sR[2 * N + j] - z),
                             j < i ? T(1) : T(-1), dx, dy, dz);
       if (r < f.rcut) {
         T u, du, d2u;

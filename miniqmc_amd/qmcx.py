@@ -49,6 +49,7 @@ _SIGS = {
     "qmcx_batch_get_positions": [_vp, _dp],
     "qmcx_batch_set_walker_offset": [_vp, _i],
     "qmcx_batch_set_tile": [_vp, _i],
+    "qmcx_batch_set_flush": [_vp, _i],
     "qmcx_evaluate_log": [_vp, _vp],
     "qmcx_run_sweeps": [_vp, _i, C.c_uint64, C.c_double, _i, _i, _vp],
     "qmcx_set_trace": [_vp, _i],
@@ -184,6 +185,10 @@ class WalkerBatch:
 
     def __del__(self):
         self.close()
+
+    def set_flush(self, kind):
+        """0: matrix-core (MFMA) flush when possible, 1: vector-FMA flush."""
+        _check(load().qmcx_batch_set_flush(self.h, kind))
 
     def set_positions(self, R):
         R = _c(R).reshape(self.nw, self.N, 3)
