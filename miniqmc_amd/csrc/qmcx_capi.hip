@@ -411,14 +411,14 @@ int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode) {
   const qmcx_system_s* s = b->sys;
   int nt = 0, slices = 0;
   if (choose_shape<T>(s, b->slices, nt, slices)) return QMCX_ERR_ARG;
-  const MoveLds L = move_lds<T>(s->N, s->nions, s->n, b->K, slices, s->npad);
+  const int chunk = b->K == 1 ? s->n : b->K;
+  const MoveLds L = move_lds<T>(s->N, s->nions, s->n, b->K, slices, s->npad, chunk);
   MoveArgs a;
   a.mode = mode;
   a.slices = slices;
   a.walker0 = b->walker0;
   a.sqrt_tau = std::sqrt(tau);
   a.seed = seed;
-  const int chunk = b->K == 1 ? s->n : b->K;
   const size_t flds = sizeof(T) * 2 * kFlushRows * b->K + sizeof(int) * b->K;
   for (int det = 0; det < 2; ++det) {
     for (int e = 0; e < s->n; e += chunk) {
@@ -680,11 +680,14 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
       delete b;
       return rc;
     }
-    const size_t lds = precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, slices, s->npad).total
-                                              : move_lds<double>(s->N, s->nions, s->n, delay_rank, slices, s->npad).total;
+    const int chunk = delay_rank == 1 ? s->n : delay_rank;
+    const size_t lds = precision == QMCX_FP32
+                           ? move_lds<float>(s->N, s->nions, s->n, delay_rank, slices, s->npad, chunk).total
+                           : move_lds<double>(s->N, s->nions, s->n, delay_rank, slices, s->npad, chunk).total;
     if (lds > 160 * 1024) {
       delete b;
-      return fail(QMCX_ERR_ARG, "walker state does not fit in LDS (reduce delay rank)");
+      return fail(QMCX_ERR_ARG, "walker state (" + std::to_string(lds / 1024) +
+                                    " KB) does not fit in the 160 KB of LDS; use a smaller delay rank");
     }
   }
   int rc = 0;

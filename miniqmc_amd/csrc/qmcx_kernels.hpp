@@ -59,39 +59,6 @@ struct MoveArgs {
   uint64_t seed;
 };
 
-// ------------------------------------------------------------------ LDS plan
-struct MoveLds {
-  unsigned oR, oIon, oRng, oRedD, oInv0, oInv, oPsi, oBinv, oP, oRedT, oPart, oRows, oSpec, oFn, oCell, oSpoG, total;
-};
-__host__ __device__ inline unsigned lds_take(unsigned& o, size_t bytes) {
-  unsigned r = o;
-  o = (unsigned)((o + bytes + 15) & ~size_t(15));
-  return r;
-}
-template <typename T>
-__host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int S, int npad) {
-  MoveLds L;
-  unsigned o = 0;
-  L.oR = lds_take(o, sizeof(double) * 3 * N);
-  L.oIon = lds_take(o, sizeof(double) * 3 * M);
-  L.oRng = lds_take(o, sizeof(double) * 64 * 4);
-  L.oRedD = lds_take(o, sizeof(double) * 16 * 16);
-  L.oInv0 = lds_take(o, sizeof(T) * n);
-  L.oInv = lds_take(o, sizeof(T) * n);
-  L.oPsi = lds_take(o, sizeof(T) * 5 * n);
-  L.oBinv = lds_take(o, sizeof(double) * K * K);
-  L.oP = lds_take(o, sizeof(double) * (K + 4));
-  L.oRedT = lds_take(o, sizeof(double) * 17 * 16);
-  L.oPart = lds_take(o, sizeof(T) * S * 5 * npad);
-  L.oRows = lds_take(o, sizeof(int) * K);
-  L.oSpec = lds_take(o, sizeof(int) * M);
-  L.oFn = lds_take(o, sizeof(Functor<T>) * (kMaxSpecies + 2));
-  L.oCell = lds_take(o, sizeof(Cell<T>));
-  L.oSpoG = lds_take(o, sizeof(double) * 12);
-  L.total = o;
-  return L;
-}
-
 // ------------------------------------------------------------------ spline gather
 // One workgroup evaluates NO outputs (5: v, gx, gy, gz, hxx; 10: full VGH) of
 // every orbital at one point.  Lane (q, s): q = 16-byte orbital group, s =
@@ -213,430 +180,11 @@ __device__ inline void spline_finish(const SysP<T>& S, int slices, const T* part
   }
 }
 
-// ------------------------------------------------------------------ fused moves
-// Sum the wave partials red[v*16 + q] (q < nw) of kRed values; every lane of every
-// wave gets the same totals (same order), broadcast from lane 16*g of its wave.
-constexpr int kRed = 17;
-__device__ inline void combine_partials(const double* red, int nw, double tot[kRed]) {
-  const int lane = threadIdx.x & 63, q = lane & 15;
-#pragma unroll
-  for (int base = 0; base < kRed; base += 4) {
-    const int v = base + (lane >> 4);
-    double x = (v < kRed && q < nw) ? red[v * 16 + q] : 0.0;
-    x += __shfl_xor(x, 8, 64);
-    x += __shfl_xor(x, 4, 64);
-    x += __shfl_xor(x, 2, 64);
-    x += __shfl_xor(x, 1, 64);
-#pragma unroll
-    for (int g = 0; g < 4; ++g)
-      if (base + g < kRed) tot[base + g] = __shfl(x, g * 16, 64);
-  }
-}
+}  // namespace qmcx
 
-#ifdef QMCX_PHASE_TIMING
-// wall_clock64 ticks per phase, accumulated by lane 0 of workgroup 0
-__device__ unsigned long long qmcx_phase_ticks[8];
-#define QMCX_TICK(slot)                                                  \
-  if (w == 0 && tid == 0) {                                              \
-    const unsigned long long now_ = wall_clock64();                      \
-    qmcx_phase_ticks[slot] += now_ - tick_;                              \
-    tick_ = now_;                                                        \
-  }
-#else
-#define QMCX_TICK(slot)
-#endif
+#include "qmcx_moves.hpp"
 
-// J2 pair terms of electron j against the moving electron iel at (x, y, z):
-// u, du/r, d2u and the minimum-image displacement (flip index iel).
-template <typename T>
-__device__ inline void j2_pair(const Cell<T>& C, const Functor<T>& f, double xj, double yj, double zj, double x, double y,
-                               double z, T flip, T& u, T& du, T& d2u, T& dx, T& dy, T& dz) {
-  const T r = min_image(C, (T)(xj - x), (T)(yj - y), (T)(zj - z), flip, dx, dy, dz);
-  u = du = d2u = 0;
-  if (r < f.rcut) functor_vgl(f, r, u, du, d2u);
-}
-
-// One workgroup = one walker; electrons [e_begin, e_end) of determinant `det`
-// are moved in order.  With K = 1 every accepted move updates Ainv in place
-// (Sherman-Morrison, DelayedUpdate.h:166-173); with K > 1 accepted rows are
-// delayed (DelayedUpdate.h:124-153) and the flush kernel applies them afterwards.
-// Reductions: each wave sums its lanes, lane 0 parks the partial in sRedT[v][wave],
-// and after one barrier every lane adds the wave partials in the same order.
-constexpr int kMoveThreads = 512;
-template <typename T, int JPT, int PER>
-__global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
-  const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
-  const int N = S.N, n = S.n, M = S.M, K = B.K;
-  const MoveLds L = move_lds<T>(N, M, n, K, 16 / PER, S.npad);
-  double* sR = (double*)(smem + L.oR);
-  double* sIon = (double*)(smem + L.oIon);
-  double* sRng = (double*)(smem + L.oRng);
-  T* sInv0 = (T*)(smem + L.oInv0);
-  T* sInv = (T*)(smem + L.oInv);
-  T* sPsi = (T*)(smem + L.oPsi);
-  double* sBinv = (double*)(smem + L.oBinv);
-  double* sP = (double*)(smem + L.oP);
-  double* sRed = (double*)(smem + L.oRedT);
-  T* sPart = (T*)(smem + L.oPart);
-  int* sRows = (int*)(smem + L.oRows);
-  int* sSpec = (int*)(smem + L.oSpec);
-  Functor<T>* sFn = (Functor<T>*)(smem + L.oFn);
-  Cell<T>* sCell = (Cell<T>*)(smem + L.oCell);
-  double* sSpoG = (double*)(smem + L.oSpoG);
-#ifdef QMCX_PHASE_TIMING
-  unsigned long long tick_ = wall_clock64();
-#endif
-
-  for (int i = tid; i < 3 * N; i += nt) sR[i] = B.R[(size_t)w * 3 * N + i];
-  for (int i = tid; i < 3 * M; i += nt) sIon[i] = S.ions[i];
-  for (int i = tid; i < M; i += nt) sSpec[i] = S.species[i];
-  {
-    const int words = (int)(sizeof(Functor<T>) * (kMaxSpecies + 2) / 4);
-    const int* src = (const int*)S.fn;
-    int* dst = (int*)sFn;
-    for (int i = tid; i < words; i += nt) dst[i] = src[i];
-  }
-  if (tid == 0) *sCell = *S.cell;
-  if (tid < 9) sSpoG[tid] = S.spoG[tid];
-  const size_t moff = ((size_t)w * 2 + a.det) * n * n;
-  T* Ainv = B.Ainv + moff;
-  T* Amat = B.A + moff;
-  T* dpsiM = B.dpsiM + ((size_t)w * 2 + a.det) * (size_t)n * 3 * n;
-  T* d2psiM = B.d2psiM + moff;
-  T* J2 = B.J2 + (size_t)w * 5 * N;
-  T* J1 = B.J1 + (size_t)w * 5 * N;
-  T* Vb = B.Vbuf + (size_t)w * K * n;
-  const int sweep = B.sweep[0];
-  int c = 0;
-  double logd = 0, ph = 0, jl1 = 0, jl2 = 0;
-  unsigned long long nacc = 0;
-  __syncthreads();
-  const Cell<T>& C = *sCell;
-  // Barriers inside a move only publish LDS data (lds_sync); the one at the end of
-  // a move also publishes the global writes of an accepted move (Ainv, rows, J).
-
-  for (int iel = a.e_begin; iel < a.e_end; ++iel) {
-    const int row = iel - a.det * n;
-    const int slot = (iel - a.e_begin) & 63;
-    if (slot == 0) {
-      if (tid < 64 && iel + tid < a.e_end) {
-        double nr[3], u;
-        move_randoms(a.seed, (uint64_t)(a.walker0 + w), (uint64_t)sweep, (uint64_t)(iel + tid), nr, u);
-        sRng[tid * 4 + 0] = nr[0];
-        sRng[tid * 4 + 1] = nr[1];
-        sRng[tid * 4 + 2] = nr[2];
-        sRng[tid * 4 + 3] = u;
-      }
-      lds_sync();
-    }
-    QMCX_TICK(0)
-    // stored state of the moving electron, read before any lane can modify it
-    const T Vat_old = J1[iel], Uat_old = J2[iel];
-
-    // ---- getInvRow (DelayedUpdate.h:96-114): p = U Ainv[row]; small algebra in FP64
-    for (int j = tid; j < n; j += nt) sInv0[j] = Ainv[(size_t)row * n + j];
-    if (c > 0) {
-      const T* ar = Ainv + (size_t)row * n;
-      for (int d0 = wid * 2; d0 < c; d0 += nw * 2) {
-        const T* u0 = Amat + (size_t)sRows[d0] * n;
-        const T* u1 = Amat + (size_t)sRows[d0 + 1 < c ? d0 + 1 : d0] * n;
-        double s0 = 0, s1 = 0;
-        for (int j = lane; j < n; j += 64) {
-          const double av = (double)ar[j];
-          s0 += (double)u0[j] * av;
-          s1 += (double)u1[j] * av;
-        }
-        s0 = wave_sum(s0);
-        s1 = wave_sum(s1);
-        if (lane == 0) {
-          sP[d0] = s0;
-          if (d0 + 1 < c) sP[d0 + 1] = s1;
-        }
-      }
-    }
-    lds_sync();
-    if (c > 0) {
-      // y = Binv^T p, parked in row c of Binv (acceptRow reads it)
-      for (int r = tid; r < c; r += nt) {
-        double s = 0;
-        for (int q = 0; q < c; ++q) s += sBinv[q * K + r] * sP[q];
-        sBinv[c * K + r] = s;
-      }
-      lds_sync();
-    }
-    QMCX_TICK(1)
-
-    // ---- proposal
-    const double x0 = sR[iel], y0 = sR[N + iel], z0 = sR[2 * N + iel];
-    const double nx = x0 + a.sqrt_tau * sRng[slot * 4 + 0];
-    const double ny = y0 + a.sqrt_tau * sRng[slot * 4 + 1];
-    const double nz = z0 + a.sqrt_tau * sRng[slot * 4 + 2];
-
-    // invRow = Ainv[row] - V^T y ; evalGrad of the determinant (DiracDeterminant.cpp:80-89)
-    {
-      double g0 = 0, g1 = 0, g2 = 0;
-      const T* dp = dpsiM + (size_t)row * 3 * n;
-      const double* y = sBinv + c * K;
-      for (int j = tid; j < n; j += nt) {
-        double v = (double)sInv0[j];
-        int d = 0;
-        for (; d + 4 <= c; d += 4) {
-          const T a0 = Ainv[(size_t)sRows[d] * n + j], a1 = Ainv[(size_t)sRows[d + 1] * n + j];
-          const T a2 = Ainv[(size_t)sRows[d + 2] * n + j], a3 = Ainv[(size_t)sRows[d + 3] * n + j];
-          v -= (double)a0 * y[d] + (double)a1 * y[d + 1] + (double)a2 * y[d + 2] + (double)a3 * y[d + 3];
-        }
-        for (; d < c; ++d) v -= (double)Ainv[(size_t)sRows[d] * n + j] * y[d];
-        const T vt = (T)v;
-        sInv[j] = vt;
-        g0 += (double)vt * (double)dp[j];
-        g1 += (double)vt * (double)dp[n + j];
-        g2 += (double)vt * (double)dp[2 * n + j];
-      }
-      g0 = wave_sum(g0);
-      g1 = wave_sum(g1);
-      g2 = wave_sum(g2);
-      if (lane == 0) {
-        sRed[0 * 16 + wid] = g0;
-        sRed[1 * 16 + wid] = g1;
-        sRed[2 * 16 + wid] = g2;
-      }
-    }
-    // J2 over the proposed row (TwoBodyJastrow.h:294-304) and J1 (OneBodyJastrow.h:259-268)
-    {
-      T r7 = 0, r8 = 0, r9 = 0, r10 = 0, r11 = 0, r12 = 0, r13 = 0, r14 = 0, r15 = 0, r16 = 0;
-      const bool up = iel < n;
-#pragma unroll
-      for (int q = 0; q < JPT; ++q) {
-        const int j = tid + q * nt;
-        if (j < N && j != iel) {
-          const Functor<T>& f = sFn[kMaxSpecies + (((j < n) == up) ? 0 : 1)];
-          T u, du, d2u, dx, dy, dz;
-          j2_pair(C, f, sR[j], sR[N + j], sR[2 * N + j], nx, ny, nz, j < iel ? T(1) : T(-1), u, du, d2u, dx, dy, dz);
-          r7 += u;
-          r8 += du * dx;
-          r9 += du * dy;
-          r10 += du * dz;
-          r11 += d2u + T(2) * du;
-        }
-      }
-      for (int ci = tid; ci < M; ci += nt) {
-        const Functor<T>& f = sFn[sSpec[ci]];
-        if (f.ncoef > 0) {
-          T dx, dy, dz;
-          const T r = min_image(C, (T)(sIon[ci] - nx), (T)(sIon[M + ci] - ny), (T)(sIon[2 * M + ci] - nz),
-                                T(-1), dx, dy, dz);
-          if (r < f.rcut) {
-            T u, du, d2u;
-            functor_vgl(f, r, u, du, d2u);
-            r12 += u;
-            r13 += d2u + T(2) * du;
-            r14 += du * dx;
-            r15 += du * dy;
-            r16 += du * dz;
-          }
-        }
-      }
-      double v[10] = {r7, r8, r9, r10, r11, r12, r13, r14, r15, r16};
-#pragma unroll
-      for (int i = 0; i < 10; ++i) v[i] = wave_sum(v[i]);
-      if (lane == 0)
-#pragma unroll
-        for (int i = 0; i < 10; ++i) sRed[(7 + i) * 16 + wid] = v[i];
-    }
-    // SPO at the proposed position (einspline_spo::evaluate, einspline_spo.hpp:204-223)
-    spline_gather<T, 5, PER>(S, sSpoG, nx, ny, nz, sPart);
-    lds_sync();
-    QMCX_TICK(2)
-    spline_finish<T, 5, T>(S, 16 / PER, sPart, sPsi, n, n);
-    lds_sync();
-    QMCX_TICK(3)
-    // ratioGrad_compute (DiracDeterminant.cpp:103-124), FP64 accumulation
-    {
-      double d0 = 0, d1 = 0, d2 = 0, d3 = 0;
-      for (int j = tid; j < n; j += nt) {
-        const double v = (double)sInv[j];
-        d0 += v * (double)sPsi[j];
-        d1 += v * (double)sPsi[n + j];
-        d2 += v * (double)sPsi[2 * n + j];
-        d3 += v * (double)sPsi[3 * n + j];
-      }
-      d0 = wave_sum(d0);
-      d1 = wave_sum(d1);
-      d2 = wave_sum(d2);
-      d3 = wave_sum(d3);
-      if (lane == 0) {
-        sRed[3 * 16 + wid] = d0;
-        sRed[4 * 16 + wid] = d1;
-        sRed[5 * 16 + wid] = d2;
-        sRed[6 * 16 + wid] = d3;
-      }
-    }
-    lds_sync();
-    double red[kRed];
-    combine_partials(sRed, nw, red);
-    QMCX_TICK(4)
-
-    const double rdet = red[3];
-    const double ratio = rdet * exp((double)(Vat_old - (T)red[12])) * exp((double)(Uat_old - (T)red[7]));
-    const double uacc = sRng[slot * 4 + 3];
-    const bool accept = a.mode == 0 ? (uacc < 0.5) : (uacc < ratio * ratio);
-    if (B.trace != nullptr && tid == 0) {
-      double* t = B.trace + ((size_t)w * N + iel) * 9;
-      t[0] = ratio;
-      t[1] = rdet;
-      t[2] = accept ? 1.0 : 0.0;
-      t[3] = red[0] + (double)J1[N + iel] + (double)J2[N + iel];
-      t[4] = red[1] + (double)J1[2 * N + iel] + (double)J2[2 * N + iel];
-      t[5] = red[2] + (double)J1[3 * N + iel] + (double)J2[3 * N + iel];
-      t[6] = red[4] / rdet + red[14] + red[8];
-      t[7] = red[5] / rdet + red[15] + red[9];
-      t[8] = red[6] / rdet + red[16] + red[10];
-    }
-    if (accept) {
-      if (tid == 0) {
-        logd += log(fabs(rdet));
-        ph += rdet > 0.0 ? 0.0 : M_PI;
-        jl1 += (double)(Vat_old - (T)red[12]);
-        jl2 += (double)(Uat_old - (T)red[7]);
-        ++nacc;
-      }
-      // new Slater row and its derivatives (DiracDeterminant.cpp:135-141)
-      for (int j = tid; j < n; j += nt) {
-        Amat[(size_t)row * n + j] = sPsi[j];
-        dpsiM[(size_t)row * 3 * n + j] = sPsi[n + j];
-        dpsiM[(size_t)row * 3 * n + n + j] = sPsi[2 * n + j];
-        dpsiM[(size_t)row * 3 * n + 2 * n + j] = sPsi[3 * n + j];
-        d2psiM[(size_t)row * n + j] = sPsi[4 * n + j];
-      }
-      if (K == 1) {
-        // Fahy/Sherman-Morrison: Ainv[m] -= ((Ainv[m].psiV - delta_m,row) / ratio) * Ainv_old[row]
-        // pass 1: t[m] for every row, four rows in flight per wave
-        double* tm = (double*)sPart;
-        for (int m0 = wid * 4; m0 < n; m0 += nw * 4) {
-          double sm[4] = {0, 0, 0, 0};
-          const T* am = Ainv + (size_t)m0 * n;
-          const int nr = min(4, n - m0);
-          for (int j = lane; j < n; j += 64) {
-            const double pv = (double)sPsi[j];
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-              if (r < nr) sm[r] += (double)am[(size_t)r * n + j] * pv;
-          }
-#pragma unroll
-          for (int r = 0; r < 4; ++r) sm[r] = wave_sum(sm[r]);
-          if (lane < nr) {
-            double mine = lane == 0 ? sm[0] : lane == 1 ? sm[1] : lane == 2 ? sm[2] : sm[3];
-            tm[m0 + lane] = (mine - (m0 + lane == row ? 1.0 : 0.0)) / rdet;
-          }
-        }
-        lds_sync();
-        // pass 2: rank-1 update, every element independent (wave per row, lanes over columns)
-        for (int m = wid; m < n; m += nw) {
-          const T t = (T)tm[m];
-          T* am = Ainv + (size_t)m * n;
-          for (int j = lane; j < n; j += 64) am[j] -= t * sInv0[j];
-        }
-      } else {
-        // acceptRow (DelayedUpdate.h:124-153), FP64 capacitance algebra
-        for (int j = tid; j < n; j += nt) Vb[(size_t)c * n + j] = sInv0[j];
-        for (int d0 = wid * 2; d0 <= c; d0 += nw * 2) {
-          const int d1 = d0 + 1 <= c ? d0 + 1 : d0;
-          const T* v0 = (d0 == c) ? sInv0 : Vb + (size_t)d0 * n;
-          const T* v1 = (d1 == c) ? sInv0 : Vb + (size_t)d1 * n;
-          double s0 = 0, s1 = 0;
-          for (int j = lane; j < n; j += 64) {
-            const double pv = (double)sPsi[j];
-            s0 += (double)v0[j] * pv;
-            s1 += (double)v1[j] * pv;
-          }
-          s0 = wave_sum(s0);
-          s1 = wave_sum(s1);
-          if (lane == 0) {
-            sP[d0] = -s0;
-            if (d1 != d0) sP[d1] = -s1;
-          }
-        }
-        lds_sync();
-        double y = -sP[c];
-        for (int i = 0; i < c; ++i) y += sBinv[c * K + i] * sP[i];
-        const double yi = 1.0 / y;
-        for (int s2 = tid; s2 < c; s2 += nt) {
-          double t = 0;
-          for (int r = 0; r < c; ++r) t += sBinv[s2 * K + r] * sP[r];
-          sBinv[s2 * K + c] = yi * t;
-        }
-        lds_sync();
-        for (int idx = tid; idx < c * c; idx += nt) {
-          const int s2 = idx / c, r = idx - s2 * c;
-          sBinv[s2 * K + r] -= sBinv[c * K + r] * sBinv[s2 * K + c];
-        }
-        lds_sync();
-        for (int i = tid; i < c; i += nt) sBinv[c * K + i] *= -yi;
-        if (tid == 0) {
-          sBinv[c * K + c] = yi;
-          sRows[c] = row;
-        }
-        ++c;
-      }
-      // J1 (OneBodyJastrow.h:271-283) and J2 (TwoBodyJastrow.h:307-353): the pair
-      // terms at the old and the accepted position are recomputed here.
-      const bool up = iel < n;
-#pragma unroll
-      for (int q = 0; q < JPT; ++q) {
-        const int j = tid + q * nt;
-        if (j < N && j != iel) {
-          const Functor<T>& f = sFn[kMaxSpecies + (((j < n) == up) ? 0 : 1)];
-          const T flip = j < iel ? T(1) : T(-1);
-          const double xj = sR[j], yj = sR[N + j], zj = sR[2 * N + j];
-          T ou, odu, od2u, odx, ody, odz, cu, cdu, cd2u, cdx, cdy, cdz;
-          j2_pair(C, f, xj, yj, zj, x0, y0, z0, flip, ou, odu, od2u, odx, ody, odz);
-          j2_pair(C, f, xj, yj, zj, nx, ny, nz, flip, cu, cdu, cd2u, cdx, cdy, cdz);
-          const T newl = cd2u + T(2) * cdu;
-          J2[j] += cu - ou;
-          J2[4 * N + j] += od2u + T(2) * odu - newl;
-          J2[N + j] -= cdu * cdx - odu * odx;
-          J2[2 * N + j] -= cdu * cdy - odu * ody;
-          J2[3 * N + j] -= cdu * cdz - odu * odz;
-        } else if (j == iel) {
-          J2[iel] = (T)red[7];
-          J2[N + iel] = (T)red[8];
-          J2[2 * N + iel] = (T)red[9];
-          J2[3 * N + iel] = (T)red[10];
-          J2[4 * N + iel] = (T)(-red[11]);
-          J1[iel] = (T)red[12];
-          J1[N + iel] = (T)red[14];
-          J1[2 * N + iel] = (T)red[15];
-          J1[3 * N + iel] = (T)red[16];
-          J1[4 * N + iel] = (T)red[13];
-        }
-      }
-      lds_sync();   // every lane has read the old position
-      if (tid == 0) {
-        sR[iel] = nx;
-        sR[N + iel] = ny;
-        sR[2 * N + iel] = nz;
-      }
-    }
-    __syncthreads();
-    QMCX_TICK(5)
-  }
-
-  for (int i = tid; i < 3 * N; i += nt) B.R[(size_t)w * 3 * N + i] = sR[i];
-  if (tid == 0) {
-    B.logdet[w * 2 + a.det] += logd;
-    B.phase[w * 2 + a.det] += ph;
-    B.jlog[w * 2 + 0] += jl1;
-    B.jlog[w * 2 + 1] += jl2;
-    B.nacc[w] += nacc;
-    B.ndelay[w] = K > 1 ? c : 0;
-  }
-  if (K > 1) {
-    for (int i = tid; i < c * K; i += nt) B.Binv[(size_t)w * K * K + i] = sBinv[i];
-    for (int i = tid; i < c; i += nt) B.rows[(size_t)w * K + i] = sRows[i];
-  }
-}
+namespace qmcx {
 
 // ------------------------------------------------------------------ delayed flush
 // updateInvMat for K > 1 (DelayedUpdate.h:176-193), rows [m0, m0+FB) of one walker:

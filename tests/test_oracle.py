@@ -1,0 +1,309 @@
+"""CPU oracle pinned against the reference's known answers, library routines,
+closed forms and brute force.  Which check pins which oracle function:
+
+  philox4x64_10 / move_randoms   numpy.random.Philox raw stream (the same published generator)
+  invert_transpose (LU)          test_dirac_matrix.cpp KATs; numpy slogdet / inverse on random matrices
+  DelayedUpdate                  test_dirac_matrix.cpp update_row KAT; test_dirac_det.cpp sequences
+                                 (SM, delay 2) against fresh inversion; Woodbury = fresh LU for k = 1..16
+  distance rows                  test_particle.cpp PBC KAT; brute force over 125 images
+  to_unit_floor                  closed form (wrap to [0,1), tiny negatives snap to 0)
+  radial functor                 closed forms: u(0) = (c0+4c1+c2)/6, u'(0) = cusp, u = 0 past rcut;
+                                 finite differences of u, u' (du/r times r), u''
+  spline VGH                     scipy.interpolate.NdBSpline on the same coefficients (values and all
+                                 first/second derivatives); constant coefficients -> v = c, derivatives 0
+  SPO gradient convention        finite differences in Cartesian space times the cell length
+  whole sweep                    incremental log|Psi|, G, L after sweeps = fresh evaluateLog
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from miniqmc_amd import systems
+from oracle import pyoracle as po
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+APPROX = GOLD["catch_approx_epsilon"]["value"]
+
+
+def approx(a, b):
+    """Catch2 Approx as configured in the reference: |a-b| <= eps (1 + |b|)."""
+    return np.all(np.abs(np.asarray(a) - np.asarray(b)) <= APPROX * (1 + np.abs(np.asarray(b))))
+
+
+# ------------------------------------------------------------------ RNG
+def test_philox_matches_numpy():
+    for key in ([1, 2], [11, 0x51A7], [2**63 + 5, 7]):
+        for ctr in ([0, 0, 0, 0], [5, 9, 3, 0], [2**64 - 1, 0, 0, 0]):
+            g = np.random.Philox(key=np.array(key, dtype=np.uint64), counter=np.array(ctr, dtype=np.uint64))
+            raw = g.random_raw(4)
+            nxt = list(ctr)
+            nxt[0] = (nxt[0] + 1) % 2**64
+            if ctr[0] == 2**64 - 1:
+                nxt[1] += 1
+            assert np.array_equal(po.philox4x64_10(nxt, key), raw)
+
+
+def test_move_randoms_distribution():
+    n = np.array([po.move_randoms(11, w, 0, e)[0] for w in range(20) for e in range(100)])
+    u = np.array([po.move_randoms(11, w, 0, e)[1] for w in range(20) for e in range(100)])
+    assert abs(n.mean()) < 0.1 and abs(n.std() - 1) < 0.1
+    assert u.min() >= 0 and u.max() < 1 and abs(u.mean() - 0.5) < 0.05
+    # counter-based: the same (seed, walker, sweep, electron) gives the same numbers
+    assert po.move_randoms(3, 4, 5, 6)[1] == po.move_randoms(3, 4, 5, 6)[1]
+
+
+# ------------------------------------------------------------------ determinant
+def test_dirac_matrix_identity():
+    psiM, ld, ph = po.invert_transpose(np.eye(3))
+    assert approx(ld, 0.0) and ph == 0.0
+    assert np.allclose(psiM, np.eye(3))
+
+
+def test_dirac_matrix_inverse_kat():
+    k = GOLD["dirac_matrix_inverse"]
+    a = np.array(k["a"])
+    psiM, ld, ph = po.invert_transpose(a.T.copy())   # invert_transpose(a^T) -> a^-1
+    assert approx(ld, k["log_det"])
+    assert approx(psiM, np.array(k["inverse"]))
+
+
+def test_dirac_matrix_update_row_kat():
+    k = GOLD["dirac_matrix_update_row"]
+    a = np.array(GOLD["dirac_matrix_inverse"]["a"])
+    psiM, _, _ = po.invert_transpose(a.T.copy())
+    du = po.DelayedUpdate(3, 1)
+    inv_row = du.get_inv_row(psiM, 0)
+    ratio = float(np.dot(k["v"], inv_row))
+    assert approx(ratio, k["ratio"])
+    du.accept_row(psiM, 0, np.array(k["v"]))
+    assert approx(psiM, np.array(k["updated_inverse"]))
+
+
+@pytest.mark.parametrize("delay", [1, 2])
+def test_dirac_det_sequence_kat(delay):
+    """test_dirac_det.cpp:252-515: rows 0,1,2 of a2^T replaced by v2 rows, ratios and the final
+    inverse checked against fresh LU; delay 1 (Sherman-Morrison) and 2 (Woodbury)."""
+    a2 = np.array(GOLD["fake_spo"]["a2"])
+    v2 = np.array(GOLD["fake_spo"]["v2"])
+    A = a2.T.copy()                          # logdet(j,i) = a2(i,j)
+    psiM, ld0, _ = po.invert_transpose(A)
+    du = po.DelayedUpdate(4, delay)
+    cur = A.copy()
+    for r in range(3):
+        inv_row = du.get_inv_row(psiM, r)
+        ratio = float(np.dot(v2[r], inv_row))
+        new = cur.copy()
+        new[r] = v2[r]
+        s0, l0 = np.linalg.slogdet(cur)
+        s1, l1 = np.linalg.slogdet(new)
+        assert approx(ratio, s0 * s1 * np.exp(l1 - l0))
+        du.accept_row(psiM, r, v2[r])
+        cur = new
+    du.update_inv(psiM)
+    ref, _, _ = po.invert_transpose(cur)
+    assert approx(psiM, ref)
+
+
+@pytest.mark.parametrize("n,delay", [(16, 1), (16, 3), (24, 8), (32, 16)])
+def test_woodbury_equals_fresh_inverse(n, delay):
+    rng = np.random.default_rng(n + delay)
+    A = rng.uniform(-1, 1, (n, n)) + 3 * np.eye(n)
+    psiM, ld, ph = po.invert_transpose(A)
+    s, l = np.linalg.slogdet(A)
+    assert abs(l - ld) < 1e-12 * max(1, abs(l)) and ph == (0.0 if s > 0 else np.pi)
+    assert np.allclose(psiM.T @ A, np.eye(n), atol=1e-11) or np.allclose(A @ psiM.T, np.eye(n), atol=1e-11)
+    du = po.DelayedUpdate(n, delay)
+    cur = A.copy()
+    logs = 0.0
+    for r in rng.permutation(n)[: n - 2]:
+        v = rng.uniform(-1, 1, n) + 3 * np.eye(n)[r]
+        ratio = float(np.dot(v, du.get_inv_row(psiM, int(r))))
+        new = cur.copy()
+        new[r] = v
+        s0, l0 = np.linalg.slogdet(cur)
+        s1, l1 = np.linalg.slogdet(new)
+        assert abs(ratio - s0 * s1 * np.exp(l1 - l0)) < 1e-9 * max(1, abs(ratio))
+        du.accept_row(psiM, int(r), v)
+        cur = new
+        logs += np.log(abs(ratio))
+    du.update_inv(psiM)
+    ref, ldr, _ = po.invert_transpose(cur)
+    assert np.abs(psiM - ref).max() < 1e-9 * np.abs(ref).max()
+    assert abs(ld + logs - ldr) < 1e-9 * abs(ldr)
+
+
+# ------------------------------------------------------------------ lattice and distances
+def test_pbc_distance_kat():
+    k = GOLD["pbc_distance"]
+    R = np.array(k["lattice"])
+    pos = np.array(k["positions"])
+    r1, dr1 = po.distance_row(R, pos[1], pos, flip_ind=1)
+    r2, dr2 = po.distance_row(R, pos[2], pos, flip_ind=2)
+    assert approx(r1[2], k["distance_1_2"]) and approx(r2[1], k["distance_1_2"])
+    assert approx(dr1[2], k["displacement_1_2"])
+    assert approx(dr2[1], k["displacement_2_1"])
+    # exact antisymmetry from the flip index (ParticleBConds.h:186-212)
+    assert np.array_equal(dr1[2], -dr2[1])
+
+
+@pytest.mark.parametrize("name", ["nio8-cubic", "nio32", "nio64", "nio16", "tiny"])
+def test_min_image_brute_force(name):
+    sysd = systems.make_system(name) if name != "nio64" else None
+    R = sysd.lattice if sysd is not None else np.diag([1, 1, 2]) @ systems.NIO_CELL
+    info = po.lattice_info(R)
+    rng = np.random.default_rng(1)
+    src = rng.uniform(-1, 2, (6, 3)) @ R
+    tgt = rng.uniform(-1, 2, (50, 3)) @ R
+    shifts = np.array([[i, j, k] for i in range(-2, 3) for j in range(-2, 3) for k in range(-2, 3)]) @ R
+    # simulation cell radius: half the smallest distance between opposite faces
+    vol = abs(np.linalg.det(R))
+    scr = min(0.5 * vol / np.linalg.norm(np.cross(R[(i + 1) % 3], R[(i + 2) % 3])) for i in range(3))
+    for p in range(len(src)):
+        r, dr = po.distance_row(R, src[p], tgt, flip_ind=25)
+        for t in range(len(tgt)):
+            cand = tgt[t] - src[p] + shifts
+            d = np.linalg.norm(cand, axis=1)
+            best = d.min()
+            if best < scr * 0.999:
+                assert abs(r[t] - best) < 1e-9
+                assert np.allclose(dr[t], cand[np.argmin(d)], atol=1e-9)
+            assert abs(np.linalg.norm(dr[t]) - r[t]) < 1e-9
+            # the returned vector is a lattice image of the displacement
+            frac = (dr[t] - (tgt[t] - src[p])) @ np.linalg.inv(R)
+            assert np.allclose(frac, np.round(frac), atol=1e-9)
+    assert info["ws"] > 0
+
+
+def test_to_unit_floor():
+    R = systems.NIO_CELL
+    rng = np.random.default_rng(2)
+    r = rng.uniform(-3, 3, (200, 3)) @ R
+    u = po.to_unit_floor(R, r)
+    assert np.all(u >= 0) and np.all(u < 1)
+    ref = r @ np.linalg.inv(R)
+    assert np.allclose(u, ref - np.floor(ref), atol=1e-12)
+    tiny = np.array([[-1e-17, 0.3, 0.5]]) @ R
+    assert po.to_unit_floor(R, tiny)[0, 0] == 0.0
+
+
+def test_wigner_seitz_radius():
+    assert abs(po.lattice_info(systems.NIO_CELL)["ws"] - 5.5727792532) < 1e-6   # buildJ2's cap, nio.hpp:88
+    assert abs(po.lattice_info(np.eye(3) * 7.9)["ws"] - 3.95) < 1e-12
+
+
+# ------------------------------------------------------------------ radial functor
+@pytest.mark.parametrize("pair", [0, 1])
+def test_functor_closed_forms(pair):
+    p = np.array(systems.NIO_J2_PARAMS[pair][:10])
+    for rcut, cusp in ((5.5727792532, 0.0), (4.8, -0.25), (3.9, -0.5)):
+        r = np.array([0.0, 1e-7, rcut * 0.5, rcut - 1e-9, rcut, rcut + 1.0])
+        u, du, d2u, c = po.functor_vgl(10, rcut, cusp, p, np.maximum(r, 1e-300))
+        dr = rcut / 11
+        assert c[0] == pytest.approx(p[1] - 2 * dr * cusp) and c[1] == p[0] and c[2] == p[1]
+        assert u[0] == pytest.approx((c[0] + 4 * c[1] + c[2]) / 6, rel=1e-12)
+        # u'(0) = cusp: du is (du/dr)/r, so r*du at tiny r gives the slope
+        assert du[1] * r[1] == pytest.approx(cusp, abs=1e-6)
+        assert u[4] == 0 and u[5] == 0 and du[5] == 0
+
+
+def test_functor_finite_differences():
+    p = np.array(systems.NIO_J1_PARAMS[1][:10])
+    rcut, cusp = 4.8261684030, -0.25
+    r = np.linspace(0.2, rcut - 0.2, 57)
+    h = 1e-5
+    u, du, d2u, _ = po.functor_vgl(10, rcut, cusp, p, r)
+    up, dup, _, _ = po.functor_vgl(10, rcut, cusp, p, r + h)
+    um, dum, _, _ = po.functor_vgl(10, rcut, cusp, p, r - h)
+    assert np.allclose((up - um) / (2 * h), du * r, rtol=1e-6, atol=1e-8)
+    assert np.allclose((dup * (r + h) - dum * (r - h)) / (2 * h), d2u, rtol=1e-5, atol=1e-7)
+
+
+# ------------------------------------------------------------------ 3D spline
+def _ndbspline(sysd, prec64=True):
+    from scipy.interpolate import NdBSpline
+    nx, ny, nz = sysd.ng
+    c = sysd.coefs.reshape(nx + 3, ny + 3, nz + 3, sysd.npad)[..., : sysd.norb].astype(np.float64)
+    knots = tuple((np.arange(m + 3 + 4) - 3) / m for m in (nx, ny, nz))
+    return NdBSpline(knots, c, 3)
+
+
+def test_spline_vgh_vs_scipy():
+    sysd = systems.make_system("tiny")
+    rng = np.random.default_rng(4)
+    u = rng.uniform(0, 1, (40, 3))
+    u[0] = [0.0, 0.0, 0.0]
+    u[1] = [1 - 1e-12, 0.5, 0.99999]
+    out = po.spline_vgh_unit(sysd.ng, sysd.norb, sysd.npad, sysd.coefs, u)
+    spl = _ndbspline(sysd)
+    ng = np.array(sysd.ng, dtype=float)
+    orders = {0: (0, 0, 0), 1: (1, 0, 0), 2: (0, 1, 0), 3: (0, 0, 1), 4: (2, 0, 0), 5: (1, 1, 0), 6: (1, 0, 1),
+              7: (0, 2, 0), 8: (0, 1, 1), 9: (0, 0, 2)}
+    for o, nu in orders.items():
+        # the oracle returns derivatives with respect to the unit-cube coordinate
+        ref = spl(u, nu=nu)
+        assert np.allclose(out[:, o], ref, rtol=1e-10, atol=1e-10 * np.abs(ref).max()), o
+    del ng
+
+
+def test_spline_constant_coefficients():
+    sysd = systems.make_system("tiny")
+    coefs = np.full_like(sysd.coefs, 0.75)
+    u = np.random.default_rng(0).uniform(0, 1, (10, 3))
+    out = po.spline_vgh_unit(sysd.ng, sysd.norb, sysd.npad, coefs, u)
+    assert np.allclose(out[:, 0], 0.75, atol=1e-14)
+    assert np.allclose(out[:, 1:], 0.0, atol=1e-11)
+
+
+def test_spo_gradient_is_fractional_times_cell():
+    """einspline_spo::evaluate returns gradients in unit-cell coordinates (MultiBspline.hpp:263-279
+    scales by delta_inv of a unit-cube grid): for a cubic cell of side a, dpsi = a * Cartesian gradient."""
+    sysd = systems.make_system("nio8-cubic")
+    orc = po.OracleSystem(sysd)
+    r = np.array([[1.3, 2.2, 3.1], [6.0, 0.4, 7.5]])
+    psi, dpsi, d2 = orc.spo_evaluate(r)
+    h = 1e-5
+    a = sysd.lattice[0, 0]
+    for d in range(3):
+        e = np.zeros(3)
+        e[d] = h
+        fd = (orc.spo_evaluate(r + e)[0] - orc.spo_evaluate(r - e)[0]) / (2 * h)
+        assert np.allclose(dpsi[:, :, d], a * fd, rtol=1e-6, atol=1e-6)
+    fdd = (orc.spo_evaluate(r + [h, 0, 0])[1][:, :, 0] - orc.spo_evaluate(r - [h, 0, 0])[1][:, :, 0]) / (2 * h)
+    assert np.allclose(d2, a * fdd, rtol=1e-5, atol=1e-4)   # d2psi = h_xx only (einspline_spo.hpp:220)
+
+
+# ------------------------------------------------------------------ whole sweep
+@pytest.mark.parametrize("delay,mode", [(1, 1), (4, 1), (1, 0)])
+def test_incremental_state_equals_fresh(delay, mode):
+    sysd = systems.make_system("nio8-cubic")
+    orc = po.OracleSystem(sysd)
+    R0 = systems.initial_positions(sysd, 2)
+    res = orc.run_vmc(R0, 2, 11, 0.5, mode=mode, delay=delay, trace=True)
+    for w in range(2):
+        v, G, L = orc.evaluate_log(res["R"][w])
+        assert abs(v - res["log"][w, -1]) < 1e-9 * abs(v)
+        assert np.allclose(G, res["G"][w], atol=1e-8)
+        assert np.allclose(L, res["L"][w], atol=1e-6)
+    # each accepted move changes log|Psi| by log|ratio| (Metropolis ratios of the trace)
+    tr = res["trace"]
+    assert set(np.unique(tr[..., 2])) <= {0.0, 1.0}
+
+
+def test_delay_ranks_agree():
+    sysd = systems.make_system("nio8-cubic")
+    orc = po.OracleSystem(sysd)
+    R0 = systems.initial_positions(sysd, 2)
+    a = orc.run_vmc(R0, 1, 11, 0.5, mode=1, delay=1, trace=True)
+    for k in (2, 7, 16, 48):
+        b = orc.run_vmc(R0, 1, 11, 0.5, mode=1, delay=k, trace=True)
+        assert np.array_equal(a["trace"][..., 2], b["trace"][..., 2])
+        assert np.allclose(a["log"], b["log"], rtol=1e-11)
+
+
+def test_prime_seed_kat():
+    """PrimeNumberSet (PrimeNumberSet.h:61-102): odd primes from 3; the reference seeds walker i with
+    primes[i].  The build's walker streams use Philox keys instead; the KAT pins the reference layout."""
+    primes = [p for p in range(3, 30) if all(p % d for d in range(2, p))]
+    assert primes[:8] == GOLD["prime_seeds"]["first"]
