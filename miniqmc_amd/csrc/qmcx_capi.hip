@@ -392,7 +392,9 @@ int evaluate_log_impl(qmcx_batch_t b) {
   const qmcx_system_s* s = b->sys;
   if (launch_spo_rows<T>(b)) return QMCX_ERR_HIP;
   if (launch_lu_batch<T>(b)) return QMCX_ERR_HIP;
-  const size_t lds = sizeof(double) * (3 * (size_t)s->N + 32) + sizeof(Functor<T>) * (kMaxSpecies + 2);
+  // positions, reduction scratch, functors and the minimum-image cell (k_jastrow_init's layout)
+  const size_t lds =
+      sizeof(double) * (3 * (size_t)s->N + 32) + sizeof(Functor<T>) * (kMaxSpecies + 2) + sizeof(Cell<T>);
   hipLaunchKernelGGL(k_jastrow_init<T>, dim3(b->W), dim3(1024), lds, b->stream, s->params<T>(), b->params<T>());
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemsetAsync(b->ndelay, 0, sizeof(int) * b->W, b->stream));

@@ -151,6 +151,22 @@ def nio32():
     return sysd, qmcx.System(sysd), po.OracleSystem(sysd)
 
 
+@pytest.mark.parametrize("prec,tol", [(qmcx.FP64, 1e-12), (qmcx.FP32, 1e-6)])
+def test_evaluate_log_nio32_vs_oracle(nio32, prec, tol):
+    """evaluateLog from scratch (SPO rows, FP64 LU, J1/J2 recompute) at NiO-32, both precisions.
+    Regression: the J recompute kernel's LDS must hold the minimum-image cell."""
+    sysd, dev, orc = nio32
+    R0 = systems.initial_positions(sysd, 3, seed=9)
+    wb = qmcx.WalkerBatch(dev, 3, delay_rank=1, precision=prec)
+    wb.set_positions(R0)
+    got = wb.evaluate_log()
+    G, L, _ = wb.gl()
+    for w in range(3):
+        v, Gr, Lr = orc.evaluate_log(R0[w])
+        assert abs(got[w] - v) < tol * abs(v)
+        assert np.abs(G[w] - Gr).max() < 1e3 * tol * (1 + np.abs(Gr).max())
+
+
 def test_sweep_nio32_fp64_vs_oracle(nio32):
     """NiO-32 plain drop-in form: 1 walker, FP64, delay 1 (384 e-, 192 orbitals padded to 208)."""
     _say("NiO-32 FP64 delay 1")
