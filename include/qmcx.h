@@ -135,6 +135,30 @@ int qmcx_get_inverse(qmcx_batch_t b, int walker, int det, double* psiM);
 int qmcx_set_kernel_timing(qmcx_batch_t b, int enable);
 int qmcx_get_kernel_timing(qmcx_batch_t b, double* ms2, long long* launches2);
 
+/* ---- delayed-update engine of one determinant (DU_TYPE) --------------------
+ * The plug point DiracDeterminant<DU_TYPE> (src/QMCWaveFunctions/DiracDeterminant.h:35,127,
+ * instantiated for a GPU engine at DiracDeterminant.cpp:339-341) with the DelayedUpdate interface
+ * (src/QMCWaveFunctions/DelayedUpdate.h:60-231).  The inverse psiM lives on the device; host
+ * matrices are row-major n x n doubles.
+ *   create        DelayedUpdate::resize(norb, delay)                         DelayedUpdate.h:65-74
+ *   invert_transpose  DelayedUpdate::invert_transpose -> DiracMatrix      DelayedUpdate.h:75-82, DiracMatrix.h:93-121
+ *                 logdetT[i][j] = phi_j(r_i); Ainv receives (logdetT^-1)^T (may be NULL)
+ *   set_inverse   DelayedUpdate::initializeInv                               DelayedUpdate.h:87-91
+ *   get_inv_row   DelayedUpdate::getInvRow                                   DelayedUpdate.h:96-114
+ *   accept_row    DelayedUpdate::acceptRow (flushes when delay is reached)   DelayedUpdate.h:124-153
+ *   update_inv_mat DelayedUpdate::updateInvMat                               DelayedUpdate.h:158-231
+ * As in the reference, acceptRow expects getInvRow of the same row just before it. */
+typedef struct qmcx_du_s* qmcx_du_t;
+int qmcx_du_create(qmcx_du_t* out, int norb, int delay, int precision);
+int qmcx_du_destroy(qmcx_du_t h);
+int qmcx_du_invert_transpose(qmcx_du_t h, const double* logdetT, double* Ainv, double* log_value, double* phase);
+int qmcx_du_set_inverse(qmcx_du_t h, const double* Ainv);
+int qmcx_du_get_inverse(qmcx_du_t h, double* Ainv);
+int qmcx_du_get_inv_row(qmcx_du_t h, int row, double* inv_row);
+int qmcx_du_accept_row(qmcx_du_t h, int row, const double* psiV);
+int qmcx_du_update_inv_mat(qmcx_du_t h);
+int qmcx_du_delay_count(qmcx_du_t h, int* count);
+
 /* Device time of the last qmcx_run_sweeps (HIP events on the batch stream), milliseconds. */
 int qmcx_last_sweep_ms(qmcx_batch_t b, double* ms);
 

@@ -931,3 +931,213 @@ extern "C" int qmcx_debug_phase_ticks(double* out8, int reset) {
   return 0;
 }
 #endif
+
+// ====================================================================== delayed-update engine (DU_TYPE)
+struct qmcx_du_s {
+  int n = 0, K = 1, prec = QMCX_FP64, count = 0;
+  hipStream_t stream = nullptr;
+  void* Ainv = nullptr;
+  void* A = nullptr;
+  void* Vbuf = nullptr;
+  double* Binv = nullptr;
+  int* rows = nullptr;
+  int* ndelay = nullptr;
+  double* vec = nullptr;    // n doubles: invRow out / psiV in
+  double* mat = nullptr;    // n x n doubles: LU input
+  double* lu_ws = nullptr;
+  double* ldph = nullptr;   // log|det|, phase
+  std::vector<void*> allocs;
+
+  template <typename T>
+  BatchP<T> params() const {
+    BatchP<T> B;
+    std::memset(&B, 0, sizeof(B));
+    B.W = 1;
+    B.K = K;
+    B.A = (T*)A;
+    B.Ainv = (T*)Ainv;
+    B.Vbuf = (T*)Vbuf;
+    B.Binv = Binv;
+    B.rows = rows;
+    B.ndelay = ndelay;
+    return B;
+  }
+  template <typename T>
+  SysP<T> sys() const {
+    SysP<T> S;
+    std::memset(&S, 0, sizeof(S));
+    S.n = n;
+    return S;
+  }
+};
+typedef qmcx_du_s* qmcx_du_t;
+
+namespace {
+template <typename T>
+int du_flush(qmcx_du_t h) {
+  if (h->count == 0) return 0;
+  if (sizeof(T) == 4 && h->n % 16 == 0) {
+    const int cp = (h->K + 15) & ~15;
+    const size_t lds = sizeof(float) * 2 * kFlushBM * cp + sizeof(int) * h->K;
+    hipLaunchKernelGGL(k_flush_mfma, dim3((h->n + kFlushBM - 1) / kFlushBM, 1), dim3(256), lds, h->stream,
+                       h->sys<float>(), h->params<float>(), 0);
+  } else {
+    const size_t lds = sizeof(T) * 2 * kFlushRows * h->K + sizeof(int) * h->K;
+    hipLaunchKernelGGL(k_flush_generic<T>, dim3((h->n + kFlushRows - 1) / kFlushRows, 1), dim3(256), lds, h->stream,
+                       h->sys<T>(), h->params<T>(), 0);
+  }
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_du_reset, dim3(1), dim3(64), 0, h->stream, h->ndelay);
+  HIPCHK(hipGetLastError());
+  h->count = 0;
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int qmcx_du_create(qmcx_du_t* out, int norb, int delay, int precision) {
+  if (!out || norb <= 0 || norb > 1024 || delay < 1 || delay > 128 || delay > norb)
+    return fail(QMCX_ERR_ARG, "norb in [1,1024] and delay in [1, min(128, norb)] required");
+  if (precision != QMCX_FP32 && precision != QMCX_FP64) return fail(QMCX_ERR_ARG, "unknown precision");
+  auto* h = new qmcx_du_s;
+  h->n = norb;
+  h->K = delay;
+  h->prec = precision;
+  const size_t es = precision == QMCX_FP32 ? 4 : 8, n = norb, K = delay;
+  auto grab = [&](void** p, size_t bytes) -> int {
+    HIPCHK(hipMalloc(p, std::max<size_t>(bytes, 16)));
+    HIPCHK(hipMemset(*p, 0, std::max<size_t>(bytes, 16)));
+    h->allocs.push_back(*p);
+    return 0;
+  };
+  int rc = 0;
+  rc |= grab(&h->Ainv, es * n * n);
+  rc |= grab(&h->A, es * n * n);
+  rc |= grab(&h->Vbuf, es * K * n);
+  rc |= grab((void**)&h->Binv, sizeof(double) * K * K);
+  rc |= grab((void**)&h->rows, sizeof(int) * K);
+  rc |= grab((void**)&h->ndelay, sizeof(int));
+  rc |= grab((void**)&h->vec, sizeof(double) * n);
+  rc |= grab((void**)&h->mat, sizeof(double) * n * n);
+  rc |= grab((void**)&h->lu_ws, sizeof(double) * 2 * n * n);
+  rc |= grab((void**)&h->ldph, sizeof(double) * 2);
+  if (rc) {
+    for (void* p : h->allocs) hipFree(p);
+    delete h;
+    return QMCX_ERR_NOMEM;
+  }
+  HIPCHK(hipStreamCreate(&h->stream));
+  *out = h;
+  return 0;
+}
+
+int qmcx_du_destroy(qmcx_du_t h) {
+  if (!h) return 0;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (void* p : h->allocs) hipFree(p);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+int qmcx_du_invert_transpose(qmcx_du_t h, const double* logdetT, double* Ainv, double* log_value, double* phase) {
+  if (!h || !logdetT) return fail(QMCX_ERR_ARG, "null argument");
+  const size_t nn = (size_t)h->n * h->n;
+  HIPCHK(hipMemcpyAsync(h->mat, logdetT, sizeof(double) * nn, hipMemcpyHostToDevice, h->stream));
+  const size_t lds = ((h->n * 4 + 15) & ~15) + 16 * 8 + 16 * 4;
+  if (h->prec == QMCX_FP32)
+    hipLaunchKernelGGL((k_lu_invert<double, float>), dim3(1), dim3(1024), lds, h->stream, h->n, h->mat,
+                       (float*)h->Ainv, h->lu_ws, h->ldph, h->ldph + 1, 1);
+  else
+    hipLaunchKernelGGL((k_lu_invert<double, double>), dim3(1), dim3(1024), lds, h->stream, h->n, h->mat,
+                       (double*)h->Ainv, h->lu_ws, h->ldph, h->ldph + 1, 1);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_du_reset, dim3(1), dim3(64), 0, h->stream, h->ndelay);
+  HIPCHK(hipGetLastError());
+  h->count = 0;
+  double lp[2];
+  HIPCHK(hipMemcpyAsync(lp, h->ldph, sizeof(lp), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (log_value) *log_value = lp[0];
+  if (phase) *phase = lp[1];
+  if (Ainv) return qmcx_du_get_inverse(h, Ainv);
+  return 0;
+}
+
+int qmcx_du_set_inverse(qmcx_du_t h, const double* Ainv) {
+  if (!h || !Ainv) return fail(QMCX_ERR_ARG, "null argument");
+  const size_t nn = (size_t)h->n * h->n;
+  if (h->prec == QMCX_FP32) {
+    std::vector<float> tmp(Ainv, Ainv + nn);
+    HIPCHK(hipMemcpy(h->Ainv, tmp.data(), sizeof(float) * nn, hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(hipMemcpy(h->Ainv, Ainv, sizeof(double) * nn, hipMemcpyHostToDevice));
+  }
+  hipLaunchKernelGGL(k_du_reset, dim3(1), dim3(64), 0, h->stream, h->ndelay);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->count = 0;
+  return 0;
+}
+
+int qmcx_du_get_inverse(qmcx_du_t h, double* Ainv) {
+  if (!h || !Ainv) return fail(QMCX_ERR_ARG, "null argument");
+  const size_t nn = (size_t)h->n * h->n;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->prec == QMCX_FP32) {
+    std::vector<float> tmp(nn);
+    HIPCHK(hipMemcpy(tmp.data(), h->Ainv, sizeof(float) * nn, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nn; ++i) Ainv[i] = tmp[i];
+  } else {
+    HIPCHK(hipMemcpy(Ainv, h->Ainv, sizeof(double) * nn, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int qmcx_du_get_inv_row(qmcx_du_t h, int row, double* inv_row) {
+  if (!h || !inv_row || row < 0 || row >= h->n) return fail(QMCX_ERR_ARG, "bad argument");
+  if (h->prec == QMCX_FP32)
+    hipLaunchKernelGGL(k_du_get_inv_row<float>, dim3(1), dim3(256), 0, h->stream, h->params<float>(), h->n, row, h->vec);
+  else
+    hipLaunchKernelGGL(k_du_get_inv_row<double>, dim3(1), dim3(256), 0, h->stream, h->params<double>(), h->n, row,
+                       h->vec);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(inv_row, h->vec, sizeof(double) * h->n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int qmcx_du_accept_row(qmcx_du_t h, int row, const double* psiV) {
+  if (!h || !psiV || row < 0 || row >= h->n) return fail(QMCX_ERR_ARG, "bad argument");
+  HIPCHK(hipMemcpyAsync(h->vec, psiV, sizeof(double) * h->n, hipMemcpyHostToDevice, h->stream));
+  if (h->prec == QMCX_FP32)
+    hipLaunchKernelGGL(k_du_accept_row<float>, dim3(1), dim3(256), 0, h->stream, h->params<float>(), h->n, row, h->vec);
+  else
+    hipLaunchKernelGGL(k_du_accept_row<double>, dim3(1), dim3(256), 0, h->stream, h->params<double>(), h->n, row,
+                       h->vec);
+  HIPCHK(hipGetLastError());
+  h->count += 1;
+  if (h->count == h->K) {
+    int rc = h->prec == QMCX_FP32 ? du_flush<float>(h) : du_flush<double>(h);
+    if (rc) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int qmcx_du_update_inv_mat(qmcx_du_t h) {
+  if (!h) return fail(QMCX_ERR_ARG, "null argument");
+  int rc = h->prec == QMCX_FP32 ? du_flush<float>(h) : du_flush<double>(h);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int qmcx_du_delay_count(qmcx_du_t h, int* count) {
+  if (!h || !count) return fail(QMCX_ERR_ARG, "null argument");
+  *count = h->count;
+  return 0;
+}
+
+}  // extern "C"

@@ -634,3 +634,85 @@ __global__ void k_next_sweep(int* sweep) {
 }
 
 }  // namespace qmcx
+
+namespace qmcx {
+
+// ------------------------------------------------------------------ stand-alone delayed-update engine
+// One determinant, device-resident inverse (the DU_TYPE plug point of
+// DiracDeterminant<DU_TYPE>, DiracDeterminant.cpp:339-341).  The flush reuses the
+// batch flush kernels with one walker.
+//
+// getInvRow (DelayedUpdate.h:96-114): invRow = Ainv[row] - V^T Binv^T U Ainv[row];
+// Binv^T U Ainv[row] is parked in row c of Binv, where acceptRow reads it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_du_get_inv_row(BatchP<T> B, int n, int row, double* out) {
+  __shared__ double sp[128];
+  const int c = B.ndelay[0], K = B.K, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const T* ar = B.Ainv + (size_t)row * n;
+  for (int d = wid; d < c; d += 4) {
+    const T* u = B.A + (size_t)B.rows[d] * n;
+    double s = 0;
+    for (int j = lane; j < n; j += 64) s += (double)u[j] * (double)ar[j];
+    s = wave_sum(s);
+    if (lane == 0) sp[d] = s;
+  }
+  __syncthreads();
+  for (int r = tid; r < c; r += 256) {
+    double s = 0;
+    for (int q = 0; q < c; ++q) s += B.Binv[q * K + r] * sp[q];
+    B.Binv[c * K + r] = s;
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += 256) {
+    double v = (double)ar[j];
+    for (int d = 0; d < c; ++d) v -= (double)B.Vbuf[(size_t)d * n + j] * B.Binv[c * K + d];
+    out[j] = v;
+  }
+}
+
+// acceptRow (DelayedUpdate.h:124-153): V_c = Ainv[row], U_c = psiV (kept as row `row` of A),
+// the capacitance inverse grows by one row and column (FP64).
+template <typename T>
+__global__ __launch_bounds__(256) void k_du_accept_row(BatchP<T> B, int n, int row, const double* psiv) {
+  __shared__ double sp[130];
+  const int c = B.ndelay[0], K = B.K, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const T* ar = B.Ainv + (size_t)row * n;
+  for (int d = wid; d <= c; d += 4) {
+    const T* v = d == c ? ar : B.Vbuf + (size_t)d * n;
+    double s = 0;
+    for (int j = lane; j < n; j += 64) s += (double)v[j] * psiv[j];
+    s = wave_sum(s);
+    if (lane == 0) sp[d] = -s;
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += 256) {
+    B.Vbuf[(size_t)c * n + j] = ar[j];
+    B.A[(size_t)row * n + j] = (T)psiv[j];
+  }
+  double y = -sp[c];
+  for (int i = 0; i < c; ++i) y += B.Binv[c * K + i] * sp[i];
+  const double yi = 1.0 / y;
+  for (int s2 = tid; s2 < c; s2 += 256) {
+    double t = 0;
+    for (int r = 0; r < c; ++r) t += B.Binv[s2 * K + r] * sp[r];
+    B.Binv[s2 * K + c] = yi * t;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < c * c; idx += 256) {
+    const int s2 = idx / c, r = idx - s2 * c;
+    B.Binv[s2 * K + r] -= B.Binv[c * K + r] * B.Binv[s2 * K + c];
+  }
+  __syncthreads();
+  for (int i = tid; i < c; i += 256) B.Binv[c * K + i] *= -yi;
+  if (tid == 0) {
+    B.Binv[c * K + c] = yi;
+    B.rows[c] = row;
+    B.ndelay[0] = c + 1;
+  }
+}
+
+__global__ void k_du_reset(int* ndelay) {
+  if (threadIdx.x == 0) ndelay[0] = 0;
+}
+
+}  // namespace qmcx

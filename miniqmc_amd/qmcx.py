@@ -59,6 +59,15 @@ _SIGS = {
     "qmcx_get_inverse": [_vp, _i, _i, _dp],
     "qmcx_last_sweep_ms": [_vp, C.POINTER(C.c_double)],
     "qmcx_set_kernel_timing": [_vp, _i],
+    "qmcx_du_create": [C.POINTER(_vp), _i, _i, _i],
+    "qmcx_du_destroy": [_vp],
+    "qmcx_du_invert_transpose": [_vp, _dp, _vp, _vp, _vp],
+    "qmcx_du_set_inverse": [_vp, _dp],
+    "qmcx_du_get_inverse": [_vp, _dp],
+    "qmcx_du_get_inv_row": [_vp, _i, _dp],
+    "qmcx_du_accept_row": [_vp, _i, _dp],
+    "qmcx_du_update_inv_mat": [_vp],
+    "qmcx_du_delay_count": [_vp, C.POINTER(C.c_int)],
     "qmcx_get_kernel_timing": [_vp, _dp, _lp],
 }
 
@@ -113,6 +122,56 @@ def invert_transpose(a: np.ndarray):
     ld, ph = np.zeros(nmat), np.zeros(nmat)
     _check(load().qmcx_invert_transpose(n, nmat, a, psiM, ld, ph))
     return psiM, ld, ph
+
+
+class DelayedUpdate:
+    """Device-resident DU_TYPE engine of one determinant (DelayedUpdate.h:29-231 interface)."""
+
+    def __init__(self, norb: int, delay: int, precision: int = FP64):
+        h = _vp()
+        _check(load().qmcx_du_create(C.byref(h), norb, delay, precision))
+        self.h, self.n = h, norb
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().qmcx_du_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def invert_transpose(self, logdetT):
+        """logdetT[i][j] = phi_j(r_i) -> (psiM = (logdetT^-1)^T, log|det|, phase)."""
+        a = _c(logdetT).reshape(self.n, self.n)
+        out = np.zeros_like(a)
+        lv, ph = C.c_double(0), C.c_double(0)
+        _check(load().qmcx_du_invert_transpose(self.h, a.reshape(-1), out.ctypes.data, C.byref(lv), C.byref(ph)))
+        return out, lv.value, ph.value
+
+    def set_inverse(self, Ainv):
+        _check(load().qmcx_du_set_inverse(self.h, _c(Ainv).reshape(-1)))
+
+    def inverse(self):
+        out = np.zeros((self.n, self.n))
+        _check(load().qmcx_du_get_inverse(self.h, out.reshape(-1)))
+        return out
+
+    def get_inv_row(self, row):
+        out = np.zeros(self.n)
+        _check(load().qmcx_du_get_inv_row(self.h, row, out))
+        return out
+
+    def accept_row(self, row, psiV):
+        _check(load().qmcx_du_accept_row(self.h, row, _c(psiV)))
+
+    def update_inv_mat(self):
+        _check(load().qmcx_du_update_inv_mat(self.h))
+
+    @property
+    def delay_count(self):
+        c = C.c_int(0)
+        _check(load().qmcx_du_delay_count(self.h, C.byref(c)))
+        return c.value
 
 
 class System:

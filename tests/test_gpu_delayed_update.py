@@ -1,0 +1,92 @@
+"""The device DU_TYPE engine through the C ABI, written like the reference's own unit tests
+(src/QMCWaveFunctions/tests/test_dirac_matrix.cpp, test_dirac_det.cpp) with their known answers,
+plus larger sequences against the FP64 oracle.  Tolerance: Catch2 Approx as the reference
+configures it (100 * FLT_EPSILON relative) in FP64; stated per test in FP32."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from miniqmc_amd import qmcx
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+EPS = GOLD["catch_approx_epsilon"]["value"]
+
+
+def approx(a, b, eps=EPS):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.all(np.abs(a - b) <= eps * (1 + np.abs(b)))
+
+
+@pytest.mark.parametrize("prec", [qmcx.FP64, qmcx.FP32])
+def test_dirac_matrix_identity_and_inverse(prec):
+    du = qmcx.DelayedUpdate(3, 1, prec)
+    psiM, lv, ph = du.invert_transpose(np.eye(3))
+    assert approx(lv, 0.0) and approx(psiM, np.eye(3))
+    k = GOLD["dirac_matrix_inverse"]
+    a = np.array(k["a"])
+    psiM, lv, ph = du.invert_transpose(a.T.copy())
+    assert approx(lv, k["log_det"])
+    assert approx(psiM, np.array(k["inverse"]), EPS if prec == qmcx.FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("prec", [qmcx.FP64, qmcx.FP32])
+def test_dirac_matrix_update_row(prec):
+    # test_dirac_matrix.cpp:110-162
+    k = GOLD["dirac_matrix_update_row"]
+    a = np.array(GOLD["dirac_matrix_inverse"]["a"])
+    du = qmcx.DelayedUpdate(3, 1, prec)
+    du.invert_transpose(a.T.copy())
+    inv_row = du.get_inv_row(0)
+    assert approx(np.dot(k["v"], inv_row), k["ratio"], EPS if prec == qmcx.FP64 else 1e-6)
+    du.accept_row(0, np.array(k["v"]))
+    assert approx(du.inverse(), np.array(k["updated_inverse"]), EPS if prec == qmcx.FP64 else 1e-5)
+
+
+@pytest.mark.parametrize("delay", [1, 2])
+def test_dirac_determinant_sequence(delay):
+    # test_dirac_det.cpp:252-515 (DiracDeterminant_second, DiracDeterminant_delayed_update)
+    a2 = np.array(GOLD["fake_spo"]["a2"])
+    v2 = np.array(GOLD["fake_spo"]["v2"])
+    du = qmcx.DelayedUpdate(4, delay, qmcx.FP64)
+    cur = a2.T.copy()
+    du.invert_transpose(cur)
+    for r in range(3):
+        ratio = float(np.dot(v2[r], du.get_inv_row(r)))
+        new = cur.copy()
+        new[r] = v2[r]
+        s0, l0 = np.linalg.slogdet(cur)
+        s1, l1 = np.linalg.slogdet(new)
+        assert approx(ratio, s0 * s1 * np.exp(l1 - l0))
+        du.accept_row(r, v2[r])
+        cur = new
+    du.update_inv_mat()
+    ref, _, _ = po.invert_transpose(cur)
+    assert approx(du.inverse(), ref)
+
+
+@pytest.mark.parametrize("prec,n,delay", [(qmcx.FP64, 48, 1), (qmcx.FP64, 48, 8), (qmcx.FP64, 96, 32),
+                                          (qmcx.FP32, 64, 16), (qmcx.FP32, 96, 32)])
+def test_engine_sequence_vs_oracle(prec, n, delay):
+    """A run of accepted rows through the device engine and the oracle's engine in lockstep."""
+    rng = np.random.default_rng(n + delay)
+    A = rng.uniform(-1, 1, (n, n)) + 4 * np.eye(n)
+    dev = qmcx.DelayedUpdate(n, delay, prec)
+    dev.invert_transpose(A)
+    ref = po.DelayedUpdate(n, delay)
+    psiM, _, _ = po.invert_transpose(A)
+    tol = 1e-10 if prec == qmcx.FP64 else 2e-4
+    for step, r in enumerate(rng.permutation(n)[: int(1.5 * delay) + 3]):
+        v = rng.uniform(-1, 1, n) + 4 * np.eye(n)[r]
+        g = dev.get_inv_row(int(r))
+        h = ref.get_inv_row(psiM, int(r))
+        assert np.abs(g - h).max() <= tol * np.abs(h).max(), step
+        dev.accept_row(int(r), v)
+        ref.accept_row(psiM, int(r), v)
+        assert dev.delay_count == ref.delay_count
+    dev.update_inv_mat()
+    ref.update_inv(psiM)
+    assert np.abs(dev.inverse() - psiM).max() <= tol * 10 * np.abs(psiM).max()
