@@ -117,6 +117,21 @@ int qmcx_evaluate_log(qmcx_batch_t b, double* log_out);
 int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int accept_mode, int recompute_every,
                     double* log_out);
 
+/* ---- flex_* stage calls (the batched boundary of src/Drivers/miniqmc_sync_move.cpp:459-513) ----
+ * One electron of every walker per call.
+ * qmcx_flex_ratio_grad: ParticleSet::flex_setActive + WaveFunction::flex_evalGrad + ParticleSet::flex_makeMove
+ *   + WaveFunction::flex_ratioGrad (src/Particle/ParticleSet.cpp:296-346, src/QMCWaveFunctions/WaveFunction.cpp:343-417):
+ *   displ[nw][3] -> ratio[nw], grad_now[nw][3], grad_new[nw][3] (outputs may be NULL); no state changes.
+ * qmcx_flex_accept_restore: WaveFunction::flex_acceptrestoreMove + ParticleSet::acceptMove/rejectMove
+ *   (WaveFunction.cpp:418-485, miniqmc_sync_move.cpp:496-505) for the same electron, accepted[nw] != 0.
+ * qmcx_flex_complete_updates: WaveFunction::flex_completeUpdates (WaveFunction.cpp:486-499).
+ * qmcx_flex_evaluate_gl: WaveFunction::flex_evaluateGL; log_out[nw] (G, L via qmcx_get_gl). */
+int qmcx_flex_ratio_grad(qmcx_batch_t b, int iel, const double* displ, double* ratio, double* grad_now,
+                         double* grad_new);
+int qmcx_flex_accept_restore(qmcx_batch_t b, int iel, const int* accepted);
+int qmcx_flex_complete_updates(qmcx_batch_t b);
+int qmcx_flex_evaluate_gl(qmcx_batch_t b, double* log_out);
+
 /* Per-move record of the most recent sweep, [nw][N][9]:
  * ratio, det ratio, accepted, grad_now[3], grad_new[3].  enable must be set before the sweep. */
 int qmcx_set_trace(qmcx_batch_t b, int enable);

@@ -200,6 +200,9 @@ struct qmcx_batch_s {
   qmcx_system_s* sys = nullptr;
   int W = 0, K = 1, prec = QMCX_FP64, walker0 = 0, slices = 0, nt = 0;
   int flush_kind = 0;   // 0: matrix cores when possible, 1: vector FMA
+  double* flex_displ = nullptr;   // [W][3] proposals of the pending flex move
+  int* flex_decide = nullptr;     // [W]
+  int flex_iel = -1;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
@@ -421,6 +424,8 @@ int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode) {
   a.walker0 = b->walker0;
   a.sqrt_tau = std::sqrt(tau);
   a.seed = seed;
+  a.displ = nullptr;
+  a.decide = nullptr;
   const size_t flds = sizeof(T) * 2 * kFlushRows * b->K + sizeof(int) * b->K;
   for (int det = 0; det < 2; ++det) {
     for (int e = 0; e < s->n; e += chunk) {
@@ -739,6 +744,8 @@ int qmcx_batch_destroy(qmcx_batch_t b) {
   for (void* p : b->allocs) hipFree(p);
   if (b->trace) hipFree(b->trace);
   if (b->hist) hipFree(b->hist);
+  if (b->flex_displ) hipFree(b->flex_displ);
+  if (b->flex_decide) hipFree(b->flex_decide);
   if (b->stream) hipStreamDestroy(b->stream);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
@@ -888,6 +895,107 @@ int qmcx_get_inverse(qmcx_batch_t b, int walker, int det, double* psiM) {
   } else {
     HIPCHK(hipMemcpy(psiM, (double*)b->Ainv + off, sizeof(double) * nn, hipMemcpyDeviceToHost));
   }
+  return 0;
+}
+
+// ---- flex_* stage calls: one electron of every walker per call, through the move kernel
+static int flex_launch(qmcx_batch_t b, int iel, bool apply) {
+  const qmcx_system_s* s = b->sys;
+  const bool f32 = b->prec == QMCX_FP32;
+  int nt = 0, slices = 0;
+  int rc = f32 ? choose_shape<float>(s, b->slices, nt, slices) : choose_shape<double>(s, b->slices, nt, slices);
+  if (rc) return rc;
+  MoveArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.det = iel < s->n ? 0 : 1;
+  a.e_begin = iel;
+  a.e_end = iel + 1;
+  a.mode = QMCX_ACCEPT_METROPOLIS;
+  a.slices = slices;
+  a.walker0 = b->walker0;
+  a.displ = b->flex_displ;
+  a.decide = b->flex_decide;
+  const bool trace_was = b->trace_on;
+  b->trace_on = true;
+  if (f32) {
+    const MoveLds L = move_lds<float>(s->N, s->nions, s->n, b->K, slices, s->npad, 1);
+    rc = launch_moves<float>(b, nt, slices, L.total, a);
+  } else {
+    const MoveLds L = move_lds<double>(s->N, s->nions, s->n, b->K, slices, s->npad, 1);
+    rc = launch_moves<double>(b, nt, slices, L.total, a);
+  }
+  b->trace_on = trace_was;
+  if (rc) return rc;
+  if (apply && b->K > 1) {
+    // every flex move is its own chunk: its (at most one) delayed row is applied now
+    if (f32 && use_mfma_flush<float>(b)) {
+      const int cp = (b->K + 15) & ~15;
+      const size_t lds = sizeof(float) * 2 * kFlushBM * cp + sizeof(int) * b->K;
+      hipLaunchKernelGGL(k_flush_mfma, dim3((s->n + kFlushBM - 1) / kFlushBM, b->W), dim3(256), lds, b->stream,
+                         s->params<float>(), b->params<float>(), a.det);
+    } else if (f32) {
+      const size_t lds = sizeof(float) * 2 * kFlushRows * b->K + sizeof(int) * b->K;
+      hipLaunchKernelGGL(k_flush_generic<float>, dim3((s->n + kFlushRows - 1) / kFlushRows, b->W), dim3(256), lds,
+                         b->stream, s->params<float>(), b->params<float>(), a.det);
+    } else {
+      const size_t lds = sizeof(double) * 2 * kFlushRows * b->K + sizeof(int) * b->K;
+      hipLaunchKernelGGL(k_flush_generic<double>, dim3((s->n + kFlushRows - 1) / kFlushRows, b->W), dim3(256), lds,
+                         b->stream, s->params<double>(), b->params<double>(), a.det);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+int qmcx_flex_ratio_grad(qmcx_batch_t b, int iel, const double* displ, double* ratio, double* grad_now,
+                         double* grad_new) {
+  if (!b || !displ || iel < 0 || iel >= b->sys->N) return fail(QMCX_ERR_ARG, "bad argument");
+  if (!b->flex_displ) {
+    HIPCHK(hipMalloc(&b->flex_displ, sizeof(double) * 3 * b->W));
+    HIPCHK(hipMalloc(&b->flex_decide, sizeof(int) * b->W));
+  }
+  if (!b->trace) HIPCHK(hipMalloc(&b->trace, sizeof(double) * 9 * (size_t)b->W * b->sys->N));
+  std::vector<int> none(b->W, 0);
+  HIPCHK(hipMemcpyAsync(b->flex_displ, displ, sizeof(double) * 3 * b->W, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(b->flex_decide, none.data(), sizeof(int) * b->W, hipMemcpyHostToDevice, b->stream));
+  if (int rc = flex_launch(b, iel, false)) return rc;
+  std::vector<double> rec(9);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (int w = 0; w < b->W; ++w) {
+    HIPCHK(hipMemcpy(rec.data(), b->trace + ((size_t)w * b->sys->N + iel) * 9, sizeof(double) * 9,
+                     hipMemcpyDeviceToHost));
+    if (ratio) ratio[w] = rec[0];
+    for (int d = 0; d < 3; ++d) {
+      if (grad_now) grad_now[3 * w + d] = rec[3 + d];
+      if (grad_new) grad_new[3 * w + d] = rec[6 + d];
+    }
+  }
+  b->flex_iel = iel;
+  return 0;
+}
+
+int qmcx_flex_accept_restore(qmcx_batch_t b, int iel, const int* accepted) {
+  if (!b || !accepted) return fail(QMCX_ERR_ARG, "null argument");
+  if (iel != b->flex_iel) return fail(QMCX_ERR_STATE, "accept_restore must follow ratio_grad of the same electron");
+  HIPCHK(hipMemcpyAsync(b->flex_decide, accepted, sizeof(int) * b->W, hipMemcpyHostToDevice, b->stream));
+  if (int rc = flex_launch(b, iel, true)) return rc;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->flex_iel = -1;
+  return 0;
+}
+
+int qmcx_flex_complete_updates(qmcx_batch_t b) {
+  if (!b) return fail(QMCX_ERR_ARG, "null batch");
+  HIPCHK(hipStreamSynchronize(b->stream));   // flex moves leave no pending delayed rows
+  return 0;
+}
+
+int qmcx_flex_evaluate_gl(qmcx_batch_t b, double* log_out) {
+  if (!b) return fail(QMCX_ERR_ARG, "null batch");
+  int rc = b->prec == QMCX_FP32 ? launch_gl<float>(b, false) : launch_gl<double>(b, false);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (log_out) HIPCHK(hipMemcpy(log_out, b->logv, sizeof(double) * b->W, hipMemcpyDeviceToHost));
   return 0;
 }
 

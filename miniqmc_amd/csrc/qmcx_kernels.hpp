@@ -57,6 +57,8 @@ struct MoveArgs {
   int e_begin, e_end, det, mode, slices, walker0;
   double sqrt_tau;
   uint64_t seed;
+  const double* displ;   // [W][3] caller proposals (flex_makeMove) or null: Philox stream
+  const int* decide;     // [W] caller accept decisions (flex_acceptrestoreMove) or null
 };
 
 // ------------------------------------------------------------------ spline gather

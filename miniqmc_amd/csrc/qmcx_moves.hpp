@@ -295,9 +295,10 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     QMCX_TICK(0)
     // ---- phase 1: issue this move's global loads, then getInvRow's small algebra
     const double x0 = sR[iel], y0 = sR[N + iel], z0 = sR[2 * N + iel];
-    const double nx = x0 + a.sqrt_tau * sRng[e * 4 + 0];
-    const double ny = y0 + a.sqrt_tau * sRng[e * 4 + 1];
-    const double nz = z0 + a.sqrt_tau * sRng[e * 4 + 2];
+    const bool given = a.displ != nullptr;
+    const double nx = x0 + (given ? a.displ[3 * w + 0] : a.sqrt_tau * sRng[e * 4 + 0]);
+    const double ny = y0 + (given ? a.displ[3 * w + 1] : a.sqrt_tau * sRng[e * 4 + 1]);
+    const double nz = z0 + (given ? a.displ[3 * w + 2] : a.sqrt_tau * sRng[e * 4 + 2]);
     StencilLoads<T, kEarly ? PER : 1> st;
     if constexpr (kEarly)
       if (spline_lane) stencil_issue<T, PER>(S, sSpoG, nx, ny, nz, sq, ss, st);
@@ -449,7 +450,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     const double rdet = red[3];
     const double ratio = rdet * exp((double)(Vat_old - (T)red[12])) * exp((double)(Uat_old - (T)red[7]));
     const double uacc = sRng[e * 4 + 3];
-    const bool accept = a.mode == 0 ? (uacc < 0.5) : (uacc < ratio * ratio);
+    const bool accept = a.decide != nullptr ? (a.decide[w] != 0) : a.mode == 0 ? (uacc < 0.5) : (uacc < ratio * ratio);
     if (B.trace != nullptr && tid == 0) {
       double* t = B.trace + ((size_t)w * N + iel) * 9;
       t[0] = ratio;

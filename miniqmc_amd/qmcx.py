@@ -59,6 +59,10 @@ _SIGS = {
     "qmcx_get_inverse": [_vp, _i, _i, _dp],
     "qmcx_last_sweep_ms": [_vp, C.POINTER(C.c_double)],
     "qmcx_set_kernel_timing": [_vp, _i],
+    "qmcx_flex_ratio_grad": [_vp, _i, _dp, _vp, _vp, _vp],
+    "qmcx_flex_accept_restore": [_vp, _i, _ip],
+    "qmcx_flex_complete_updates": [_vp],
+    "qmcx_flex_evaluate_gl": [_vp, _vp],
     "qmcx_du_create": [C.POINTER(_vp), _i, _i, _i],
     "qmcx_du_destroy": [_vp],
     "qmcx_du_invert_transpose": [_vp, _dp, _vp, _vp, _vp],
@@ -290,6 +294,25 @@ class WalkerBatch:
         p = np.zeros((self.n, self.n))
         _check(load().qmcx_get_inverse(self.h, walker, det, p.reshape(-1)))
         return p
+
+    # ---- flex_* stage calls (miniqmc_sync_move.cpp:459-513)
+    def flex_ratio_grad(self, iel, displ):
+        """setActive + evalGrad + makeMove + ratioGrad for electron iel of every walker."""
+        ratio, gnow, gnew = np.zeros(self.nw), np.zeros((self.nw, 3)), np.zeros((self.nw, 3))
+        _check(load().qmcx_flex_ratio_grad(self.h, iel, _c(displ).reshape(-1), ratio.ctypes.data, gnow.ctypes.data,
+                                           gnew.ctypes.data))
+        return ratio, gnow, gnew
+
+    def flex_accept_restore(self, iel, accepted):
+        _check(load().qmcx_flex_accept_restore(self.h, iel, _c(accepted, np.int32)))
+
+    def flex_complete_updates(self):
+        _check(load().qmcx_flex_complete_updates(self.h))
+
+    def flex_evaluate_gl(self):
+        out = np.zeros(self.nw)
+        _check(load().qmcx_flex_evaluate_gl(self.h, out.ctypes.data))
+        return out
 
     def set_kernel_timing(self, on=True):
         _check(load().qmcx_set_kernel_timing(self.h, 1 if on else 0))

@@ -43,6 +43,12 @@
 
 namespace orc {
 
+// Hot loops get an AVX2 clone beside the baseline x86-64 one; the loader picks
+// the clone by the host CPU's feature bits.  No FMA contraction and each output
+// element keeps its own sequential summation order (vectorised across
+// elements), so both clones give bit-identical results.
+#define ORC_HOT __attribute__((target_clones("avx2", "default")))
+
 // ---------------------------------------------------------------- RNG
 // Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11), published constants.
 static inline void mulhilo64(uint64_t a, uint64_t b, uint64_t& hi, uint64_t& lo) {
@@ -201,36 +207,58 @@ static void to_unit_floor(const double G[3][3], const double r[3], double u[3]) 
   }
 }
 
+// One target range [t0, t1) with a fixed flip sign; branch-free so that the
+// corner search vectorises across targets.
+ORC_HOT static void min_image_range(const Lattice& L, const double pos[3], const double* __restrict px,
+                                    const double* __restrict py, const double* __restrict pz, int t0, int t1,
+                                    double flip, double* __restrict r, double* __restrict dx,
+                                    double* __restrict dy, double* __restrict dz) {
+  const double g00 = L.g[0][0], g01 = L.g[0][1], g02 = L.g[0][2];
+  const double g10 = L.g[1][0], g11 = L.g[1][1], g12 = L.g[1][2];
+  const double g20 = L.g[2][0], g21 = L.g[2][1], g22 = L.g[2][2];
+  const double r00 = L.rb[0][0], r01 = L.rb[0][1], r02 = L.rb[0][2];
+  const double r10 = L.rb[1][0], r11 = L.rb[1][1], r12 = L.rb[1][2];
+  const double r20 = L.rb[2][0], r21 = L.rb[2][1], r22 = L.rb[2][2];
+  double cx[8], cy[8], cz[8];
+  for (int c = 0; c < 8; ++c) { cx[c] = L.corner[c][0]; cy[c] = L.corner[c][1]; cz[c] = L.corner[c][2]; }
+  const double p0 = pos[0], p1 = pos[1], p2 = pos[2];
+  for (int t = t0; t < t1; ++t) {
+    const double d0 = (px[t] - p0) * flip;
+    const double d1 = (py[t] - p1) * flip;
+    const double d2 = (pz[t] - p2) * flip;
+    const double s0 = -std::floor(d0 * g00 + d1 * g10 + d2 * g20);
+    const double s1 = -std::floor(d0 * g01 + d1 * g11 + d2 * g21);
+    const double s2 = -std::floor(d0 * g02 + d1 * g12 + d2 * g22);
+    const double ex = d0 + s0 * r00 + s1 * r10 + s2 * r20;
+    const double ey = d1 + s0 * r01 + s1 * r11 + s2 * r21;
+    const double ez = d2 + s0 * r02 + s1 * r12 + s2 * r22;
+    double best = ex * ex + ey * ey + ez * ez, bx = ex, by = ey, bz = ez;
+#pragma GCC unroll 8
+    for (int c = 1; c < 8; ++c) {   // the first strictly smaller corner wins
+      const double x = ex + cx[c], y = ey + cy[c], z = ez + cz[c];
+      const double q = x * x + y * y + z * z;
+      const bool lt = q < best;
+      best = lt ? q : best;
+      bx = lt ? x : bx;
+      by = lt ? y : by;
+      bz = lt ? z : bz;
+    }
+    r[t] = std::sqrt(best);
+    dx[t] = flip * bx;
+    dy[t] = flip * by;
+    dz[t] = flip * bz;
+  }
+}
+
 // Minimum-image distances from `pos` to targets [0,n) stored SoA (x[], y[], z[]).
 // flip_ind: targets with index < flip_ind use (target - pos), the others
 // use -(pos - target) so that row/column pairs are exact negatives.
 // ParticleBConds.h:154-217.
 static void min_image_row(const Lattice& L, const double pos[3], const double* px, const double* py,
-                          const double* pz, int n, int flip_ind, double* r, double* dx, double* dy,
-                          double* dz) {
-  for (int t = 0; t < n; ++t) {
-    const double flip = t < flip_ind ? 1.0 : -1.0;
-    const double d0 = (px[t] - pos[0]) * flip;
-    const double d1 = (py[t] - pos[1]) * flip;
-    const double d2 = (pz[t] - pos[2]) * flip;
-    const double s0 = -std::floor(d0 * L.g[0][0] + d1 * L.g[1][0] + d2 * L.g[2][0]);
-    const double s1 = -std::floor(d0 * L.g[0][1] + d1 * L.g[1][1] + d2 * L.g[2][1]);
-    const double s2 = -std::floor(d0 * L.g[0][2] + d1 * L.g[1][2] + d2 * L.g[2][2]);
-    const double ex = d0 + s0 * L.rb[0][0] + s1 * L.rb[1][0] + s2 * L.rb[2][0];
-    const double ey = d1 + s0 * L.rb[0][1] + s1 * L.rb[1][1] + s2 * L.rb[2][1];
-    const double ez = d2 + s0 * L.rb[0][2] + s1 * L.rb[1][2] + s2 * L.rb[2][2];
-    double best = ex * ex + ey * ey + ez * ez;
-    int ic = 0;
-    for (int c = 1; c < 8; ++c) {
-      const double x = ex + L.corner[c][0], y = ey + L.corner[c][1], z = ez + L.corner[c][2];
-      const double q = x * x + y * y + z * z;
-      if (q < best) { best = q; ic = c; }
-    }
-    r[t] = std::sqrt(best);
-    dx[t] = flip * (ex + L.corner[ic][0]);
-    dy[t] = flip * (ey + L.corner[ic][1]);
-    dz[t] = flip * (ez + L.corner[ic][2]);
-  }
+                          const double* pz, int n, int flip_ind, double* r, double* dx, double* dy, double* dz) {
+  const int f = std::min(std::max(flip_ind, 0), n);
+  min_image_range(L, pos, px, py, pz, 0, f, 1.0, r, dx, dy, dz);
+  min_image_range(L, pos, px, py, pz, f, n, -1.0, r, dx, dy, dz);
 }
 
 // ---------------------------------------------------------------- radial B-spline functor
@@ -334,6 +362,50 @@ static inline void weights(double t, double a[4], double da[4], double d2a[4]) {
   }
 }
 
+// 64-term stencil sum over every orbital for one (i, j) column of the stencil;
+// each orbital's ten accumulators keep the same summation order whatever the
+// vector width.  One body, instantiated for float and double tables.
+#define ORC_SPLINE_COLUMN(NAME, C)                                                                              \
+  ORC_HOT static void NAME(const C* __restrict q, size_t zs, int n, const double* c, const double* dc,          \
+                           const double* d2c, const double* w, double* __restrict acc) {                        \
+    double* __restrict a0 = acc;                                                                                 \
+    double* __restrict a1 = acc + n;                                                                             \
+    double* __restrict a2 = acc + 2 * n;                                                                         \
+    double* __restrict a3 = acc + 3 * n;                                                                         \
+    double* __restrict a4 = acc + 4 * n;                                                                         \
+    double* __restrict a5 = acc + 5 * n;                                                                         \
+    double* __restrict a6 = acc + 6 * n;                                                                         \
+    double* __restrict a7 = acc + 7 * n;                                                                         \
+    double* __restrict a8 = acc + 8 * n;                                                                         \
+    double* __restrict a9 = acc + 9 * n;                                                                         \
+    const C* __restrict q0p = q;                                                                                 \
+    const C* __restrict q1p = q + zs;                                                                            \
+    const C* __restrict q2p = q + 2 * zs;                                                                        \
+    const C* __restrict q3p = q + 3 * zs;                                                                        \
+    const double c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];                                                     \
+    const double e0 = dc[0], e1 = dc[1], e2 = dc[2], e3 = dc[3];                                                 \
+    const double f0 = d2c[0], f1 = d2c[1], f2 = d2c[2], f3 = d2c[3];                                             \
+    const double w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4], w5 = w[5];                               \
+    for (int o = 0; o < n; ++o) {                                                                                \
+      const double q0 = (double)q0p[o], q1 = (double)q1p[o], q2 = (double)q2p[o], q3 = (double)q3p[o];          \
+      const double s0 = ((c0 * q0 + c1 * q1) + c2 * q2) + c3 * q3;                                              \
+      const double s1 = ((e0 * q0 + e1 * q1) + e2 * q2) + e3 * q3;                                              \
+      const double s2 = ((f0 * q0 + f1 * q1) + f2 * q2) + f3 * q3;                                              \
+      a0[o] += w0 * s0;                                                                                          \
+      a1[o] += w1 * s0;                                                                                          \
+      a2[o] += w2 * s0;                                                                                          \
+      a3[o] += w0 * s1;                                                                                          \
+      a4[o] += w3 * s0;                                                                                          \
+      a5[o] += w4 * s0;                                                                                          \
+      a6[o] += w1 * s1;                                                                                          \
+      a7[o] += w5 * s0;                                                                                          \
+      a8[o] += w2 * s1;                                                                                          \
+      a9[o] += w0 * s2;                                                                                          \
+    }                                                                                                            \
+  }
+ORC_SPLINE_COLUMN(spline_column_f, float)
+ORC_SPLINE_COLUMN(spline_column_d, double)
+
 // value, gradient (3), hessian (xx,xy,xz,yy,yz,zz) for every orbital at the
 // fractional point u; out has 10 planes of `norb` (MultiBspline.hpp:163-280).
 static void spline_vgh_unit(const Spline& s, const double u[3], double* out) {
@@ -348,31 +420,15 @@ static void spline_vgh_unit(const Spline& s, const double u[3], double* out) {
   weights(tz, c, dc, d2c);
   const size_t zs = s.npad, ys = (size_t)(s.ng[2] + 3) * zs, xs = (size_t)(s.ng[1] + 3) * ys;
   const int n = s.norb;
-  std::vector<double> acc(10 * (size_t)n, 0.0);
+  double* acc = out;
+  std::fill(acc, acc + 10 * (size_t)n, 0.0);
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j) {
-      const double w00 = a[i] * b[j], w10 = da[i] * b[j], w01 = a[i] * db[j];
-      const double w20 = d2a[i] * b[j], w11 = da[i] * db[j], w02 = a[i] * d2b[j];
+      // weights of (v, d/dx, d/dy, d2/dx2, d2/dxdy, d2/dy2) for this column
+      const double w[6] = {a[i] * b[j], da[i] * b[j], a[i] * db[j], d2a[i] * b[j], da[i] * db[j], a[i] * d2b[j]};
       const size_t base = (ix + i) * xs + (iy + j) * ys + iz * zs;
-      for (int o = 0; o < n; ++o) {
-        double s0 = 0, s1 = 0, s2 = 0;
-        for (int k = 0; k < 4; ++k) {
-          const double q = s.coef(base + k * zs + o);
-          s0 += c[k] * q;
-          s1 += dc[k] * q;
-          s2 += d2c[k] * q;
-        }
-        acc[0 * n + o] += w00 * s0;
-        acc[1 * n + o] += w10 * s0;
-        acc[2 * n + o] += w01 * s0;
-        acc[3 * n + o] += w00 * s1;
-        acc[4 * n + o] += w20 * s0;
-        acc[5 * n + o] += w11 * s0;
-        acc[6 * n + o] += w10 * s1;
-        acc[7 * n + o] += w02 * s0;
-        acc[8 * n + o] += w01 * s1;
-        acc[9 * n + o] += w00 * s2;
-      }
+      if (s.prec == 4) spline_column_f(s.cf + base, zs, n, c, dc, d2c, w, acc);
+      else spline_column_d(s.cd + base, zs, n, c, dc, d2c, w, acc);
     }
   const double gx = s.ng[0], gy = s.ng[1], gz = s.ng[2];  // delta_inv of a unit-cube grid
   const double sc[10] = {1.0, gx, gy, gz, gx * gx, gx * gy, gx * gz, gy * gy, gy * gz, gz * gz};
@@ -444,6 +500,55 @@ static double invert_transpose(int n, const double* A, double* psiM, double& pha
   return ld;
 }
 
+// Fahy's Sherman-Morrison in one pass over A (DelayedUpdate.h:138-147, k = 1):
+// t_m = A[m].u - [m == row], A[m] -= b t_m v.  Row m's update needs only t_m,
+// so eight rows are dotted together (each summed in j order) and updated at once.
+ORC_HOT static void sm_update(int n, double* __restrict A, const double* __restrict u, int row, double b,
+                              const double* __restrict v) {
+  int m = 0;
+  for (; m + 8 <= n; m += 8) {
+    double* a0 = A + (size_t)m * n;
+    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < n; ++j) {
+      const double uj = u[j];
+#pragma GCC unroll 8
+      for (int i = 0; i < 8; ++i) t[i] += a0[(size_t)i * n + j] * uj;
+    }
+    for (int i = 0; i < 8; ++i) {
+      const double bt = b * (t[i] - (m + i == row ? 1.0 : 0.0));
+      double* __restrict ar = a0 + (size_t)i * n;
+      for (int j = 0; j < n; ++j) ar[j] -= bt * v[j];
+    }
+  }
+  for (; m < n; ++m) {
+    double* __restrict ar = A + (size_t)m * n;
+    double t = 0;
+    for (int j = 0; j < n; ++j) t += ar[j] * u[j];
+    const double bt = b * (t - (m == row ? 1.0 : 0.0));
+    for (int j = 0; j < n; ++j) ar[j] -= bt * v[j];
+  }
+}
+static double dot_n(int n, const double* a, const double* b) {
+  double s = 0;
+  for (int j = 0; j < n; ++j) s += a[j] * b[j];
+  return s;
+}
+// A[m][j] -= sum_e T[m][e] W[e][j], vectorised over j, e summed in order
+ORC_HOT static void woodbury_apply(int n, int k, int c, double* __restrict A, const double* __restrict T,
+                                   const double* __restrict W) {
+  std::vector<double> row(n);
+  for (int m = 0; m < n; ++m) {
+    std::fill(row.begin(), row.end(), 0.0);
+    for (int e = 0; e < c; ++e) {
+      const double t = T[(size_t)m * k + e];
+      const double* w = W + (size_t)e * n;
+      for (int j = 0; j < n; ++j) row[j] += t * w[j];
+    }
+    double* a = A + (size_t)m * n;
+    for (int j = 0; j < n; ++j) a[j] -= row[j];
+  }
+}
+
 // The rank-k delayed (Woodbury) inverse update; DelayedUpdate.h:29-231.
 struct DelayedUpdate {
   int n = 0, k = 1, c = 0;
@@ -509,23 +614,11 @@ struct DelayedUpdate {
     if (c == 0) return;
     if (c == 1) {
       // Fahy's variant of Sherman-Morrison
-      for (int m = 0; m < n; ++m) {
-        double s = 0;
-        for (int j = 0; j < n; ++j) s += A[(size_t)m * n + j] * U[j];
-        tmp[m] = s;
-      }
-      tmp[rows[0]] -= 1.0;
-      const double b = Binv[0];
-      for (int m = 0; m < n; ++m)
-        for (int j = 0; j < n; ++j) A[(size_t)m * n + j] -= b * tmp[m] * V[j];
+      sm_update(n, A, U.data(), rows[0], Binv[0], V.data());
     } else {
       // T = A U^T - E ; W = Binv V ; A -= T W
       for (int m = 0; m < n; ++m)
-        for (int d = 0; d < c; ++d) {
-          double s = 0;
-          for (int j = 0; j < n; ++j) s += U[(size_t)d * n + j] * A[(size_t)m * n + j];
-          tmat[(size_t)m * k + d] = s;
-        }
+        for (int d = 0; d < c; ++d) tmat[(size_t)m * k + d] = dot_n(n, &U[(size_t)d * n], A + (size_t)m * n);
       for (int d = 0; d < c; ++d) tmat[(size_t)rows[d] * k + d] -= 1.0;
       std::vector<double> W((size_t)c * n, 0.0);
       for (int e = 0; e < c; ++e)
@@ -534,12 +627,7 @@ struct DelayedUpdate {
           for (int d = 0; d < c; ++d) s += Binv[(size_t)e * k + d] * V[(size_t)d * n + j];
           W[(size_t)e * n + j] = s;
         }
-      for (int m = 0; m < n; ++m)
-        for (int j = 0; j < n; ++j) {
-          double s = 0;
-          for (int e = 0; e < c; ++e) s += tmat[(size_t)m * k + e] * W[(size_t)e * n + j];
-          A[(size_t)m * n + j] -= s;
-        }
+      woodbury_apply(n, k, c, A, tmat.data(), W.data());
     }
     c = 0;
   }
