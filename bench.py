@@ -75,6 +75,9 @@ def main():
     ap.add_argument("--tau", type=float, default=0.3)
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--fp64-steps", type=int, default=4)
+    ap.add_argument("--recompute", type=int, default=1,
+                    help="FP32 form: FP64 LU inverse recompute every this many sweeps (0: never)")
+    ap.add_argument("--err-steps", type=int, default=2, help="sweeps of the FP32-vs-FP64 error leg")
     ap.add_argument("--cpu-walkers", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -101,46 +104,74 @@ def main():
         k, tile, tune = autotune.tune(dev, W, R0, seed=args.seed, tau=args.tau)
         log(f"autotune: delay {k}, tile {tile}: {tune}")
 
-    # ---- error leg: FP32 batched vs FP64 delay-1 from the same start, same random streams
+    # ---- error leg: FP32 batched vs FP64 delay-1 from the same start, same random streams.
+    # A walker's sequences agree until its first accept flip (a move whose u falls between
+    # the two forms' ratio^2); log|Psi| is compared on the walkers without a flip.
     b32 = qmcx.WalkerBatch(dev, W, delay_rank=k, precision=qmcx.FP32, tile=tile)
     b64 = qmcx.WalkerBatch(dev, W, delay_rank=1, precision=qmcx.FP64)
     for b in (b32, b64):
         b.set_positions(R0)
         b.evaluate_log()
         b.set_trace(True)
-    h32 = b32.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
-    h64 = b64.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
-    t32, t64 = b32.trace(), b64.trace()
-    flips = int((t32[:, :, 2] != t64[:, :, 2]).sum())
-    lerr = np.abs(h32[0] - h64[0]) / np.abs(h64[0])
+    flipped = np.zeros(W, dtype=bool)
+    flips = 0
+    ratio_err = []
+    lerr_sweeps = []
+    for _ in range(args.err_steps):
+        h32 = b32.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS, recompute_every=args.recompute)
+        h64 = b64.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
+        t32, t64 = b32.trace(), b64.trace()
+        f = t32[:, :, 2] != t64[:, :, 2]
+        flips += int((f.any(axis=1) & ~flipped).sum())   # first divergence of a walker's chain
+        clean_moves = ~(flipped[:, None] | (np.cumsum(f, axis=1) > 0))
+        rel = np.abs(t32[:, :, 0] - t64[:, :, 0]) / np.maximum(np.abs(t64[:, :, 0]), 1.0)
+        ratio_err.append(rel[clean_moves])
+        flipped |= f.any(axis=1)
+        lerr_sweeps.append(np.abs(h32[0] - h64[0]) / np.abs(h64[0]))
     acc_rate = float(t64[:, :, 2].mean())
+    clean = ~flipped
+    lerr = np.array(lerr_sweeps)[:, clean] if clean.any() else np.full((1, 1), np.nan)
+    ratio_err = np.concatenate(ratio_err)
     for b in (b32, b64):
         b.set_trace(False)
-    log(f"error leg: max rel |dlog| {lerr.max():.2e}, accept flips {flips}, acceptance {acc_rate:.3f}")
+    log(f"error leg ({args.err_steps} sweeps): walkers with identical accept sequences {int(clean.sum())}/{W}, "
+        f"flips {flips}; max rel |dlog| on those {np.nanmax(lerr):.2e}; ratio err max {ratio_err.max():.2e}")
 
-    # ---- headline: batched FP32
-    b32.run(args.warmup, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
+    # ---- headline: batched FP32 (FP64 LU recompute every `recompute` sweeps inside the timed region)
+    b32.run(args.warmup, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS, recompute_every=args.recompute)
     b32.set_kernel_timing(True)
     a0 = b32.accepts().sum()
     t0 = time.perf_counter()
-    b32.run(args.steps, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
+    b32.run(args.steps, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS, recompute_every=args.recompute)
     wall = time.perf_counter() - t0
     a1 = b32.accepts().sum()
-    mv_ms, mv_n, fl_ms, fl_n = b32.kernel_timing()
+    kt = b32.kernel_timing()
     b32.set_kernel_timing(False)
+    mv_ms, mv_n = kt["move"]
     moves = W * N * args.steps
     value = moves / wall
     acc_timed = float(a1 - a0) / moves
-    log(f"FP32 k={k}: {value:.4e} moves/s, {1e3 * wall / args.steps:.2f} ms/sweep; "
-        f"move kernel {mv_ms / max(mv_n, 1):.3f} ms x {mv_n}, flush {fl_ms / max(fl_n, 1):.3f} ms x {fl_n}")
+    stages = {st: {"ms_per_step": ms / args.steps, "launches": n, "ms_per_launch": ms / max(n, 1)}
+              for st, (ms, n) in kt.items()}
+    log(f"FP32 k={k}: {value:.4e} moves/s, {1e3 * wall / args.steps:.2f} ms/sweep; stages " +
+        ", ".join(f"{st} {v['ms_per_step']:.2f} ms/sweep" for st, v in stages.items()))
 
-    # ---- FP64 delay-1 GPU path (same walkers, continuing its own chain)
+    # ---- FP64 delay-1 GPU path: batched (same walkers, continuing its own chain) and one walker alone
     b64.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
     t0 = time.perf_counter()
     b64.run(args.fp64_steps, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
     wall64 = time.perf_counter() - t0
     value64 = W * N * args.fp64_steps / wall64
-    log(f"FP64 k=1: {value64:.4e} moves/s, {1e3 * wall64 / args.fp64_steps:.2f} ms/sweep")
+    b1 = qmcx.WalkerBatch(dev, 1, delay_rank=1, precision=qmcx.FP64)
+    b1.set_positions(R0[:1])
+    b1.evaluate_log()
+    b1.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
+    t0 = time.perf_counter()
+    b1.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
+    value64_1 = N / (time.perf_counter() - t0)
+    b1.close()
+    log(f"FP64 k=1: batched {value64:.4e} moves/s ({1e3 * wall64 / args.fp64_steps:.2f} ms/sweep), "
+        f"one walker {value64_1:.4e} moves/s")
 
     # ---- roofline of the dominant kernel (the fused move kernel)
     bpm, parts = per_move_bytes(sysd, k, acc_timed)
@@ -181,13 +212,20 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp64_delay1": {"value": value64, "unit": "walker-moves/s", "ms_per_step": 1e3 * wall64 / args.fp64_steps,
-                        "steps": args.fp64_steps, "form": "batched, 1 workgroup per walker, in-kernel Sherman-Morrison"},
+                        "steps": args.fp64_steps, "walkers": W,
+                        "form": "batched, 1 workgroup per walker, in-kernel Sherman-Morrison",
+                        "one_walker_value": value64_1},
         "speedup_vs_fp64_delay1": value / value64,
-        "log_psi_err": {"max_rel": float(lerr.max()), "mean_rel": float(lerr.mean()),
-                        "vs": "FP64 delay-1 GPU path (parity-pinned to the FP64 oracle), same start and streams, 1 sweep"},
+        "log_psi_err": {"max_rel": float(np.nanmax(lerr)), "mean_rel": float(np.nanmean(lerr)),
+                        "walkers_compared": int(clean.sum()), "sweeps": args.err_steps,
+                        "vs": "FP64 delay-1 GPU path (parity-pinned to the FP64 oracle), same start and streams; "
+                              "walkers whose accept sequence stayed identical"},
+        "ratio_err_max_rel": float(ratio_err.max()),
         "accept_flips": flips,
+        "walkers_identical_sequence": int(clean.sum()),
         "acceptance": acc_timed,
-        "flush_kernel_ms_per_launch": fl_ms / max(fl_n, 1),
+        "recompute_every_sweeps": args.recompute,
+        "stages": stages,
         "bytes_per_move_parts": parts,
         "autotune": tune,
     }

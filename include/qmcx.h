@@ -144,11 +144,12 @@ int qmcx_get_accepts(qmcx_batch_t b, long long* naccept);
 /* Inverse matrix psiM of one determinant of one walker as FP64, [n][n] (DiracDeterminant::psiM). */
 int qmcx_get_inverse(qmcx_batch_t b, int walker, int det, double* psiM);
 
-/* Per-kernel device time: with timing enabled, HIP events on the batch stream bracket every
- * move-kernel and flush-kernel launch of qmcx_run_sweeps; ms2[0]/launches2[0] accumulate the move
- * kernel, ms2[1]/launches2[1] the delayed-update flush.  Enabling resets the totals. */
+/* Per-stage device time: with timing enabled, HIP events on the batch stream bracket every
+ * launch of qmcx_run_sweeps.  Index 0: move kernel; 1: delayed-update flush; 2: FP64 inverse
+ * recompute (LU); 3: evaluateGL.  ms4[i] accumulates milliseconds and launches4[i] the number of
+ * bracketed launches.  Enabling resets the totals. */
 int qmcx_set_kernel_timing(qmcx_batch_t b, int enable);
-int qmcx_get_kernel_timing(qmcx_batch_t b, double* ms2, long long* launches2);
+int qmcx_get_kernel_timing(qmcx_batch_t b, double* ms4, long long* launches4);
 
 /* ---- delayed-update engine of one determinant (DU_TYPE) --------------------
  * The plug point DiracDeterminant<DU_TYPE> (src/QMCWaveFunctions/DiracDeterminant.h:35,127,

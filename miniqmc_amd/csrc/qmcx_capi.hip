@@ -236,8 +236,8 @@ struct qmcx_batch_s {
   std::vector<hipEvent_t> prof_ev;
   std::vector<int> prof_kind;   // 0 = move kernel, 1 = flush kernel
   size_t prof_used = 0;
-  double prof_ms[2] = {0, 0};
-  long long prof_n[2] = {0, 0};
+  double prof_ms[4] = {0, 0, 0, 0};   // move, flush, FP64 recompute, evaluateGL
+  long long prof_n[4] = {0, 0, 0, 0};
   int prof_record(int kind) {
     if (!prof_on) return 0;
     if (prof_used + 2 > prof_ev.size()) {
@@ -371,9 +371,8 @@ template <typename T>
 int launch_lu_batch(qmcx_batch_t b) {
   const qmcx_system_s* s = b->sys;
   const int n = s->n;
-  const size_t lds = ((n * 4 + 15) & ~15) + 16 * 8 + 16 * 4;
-  hipLaunchKernelGGL((k_lu_invert<T, T>), dim3(b->W * 2), dim3(1024), lds, b->stream, n, (const T*)b->A, (T*)b->Ainv,
-                     b->lu_ws, b->logdet, b->phase, 1);
+  hipLaunchKernelGGL((k_gj_invert<T, T>), dim3(b->W * 2), dim3(kGJThreads), gj_lds(n).total, b->stream, n,
+                     (const T*)b->A, (T*)b->Ainv, b->lu_ws, b->logdet, b->phase, 1);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -649,8 +648,8 @@ int qmcx_invert_transpose(int n, int nmat, const double* a, double* psiM, double
   HIPCHK(hipMalloc(&d_ld, sizeof(double) * nmat));
   HIPCHK(hipMalloc(&d_ph, sizeof(double) * nmat));
   HIPCHK(hipMemcpy(d_a, a, sizeof(double) * nn, hipMemcpyHostToDevice));
-  const size_t lds = ((n * 4 + 15) & ~15) + 16 * 8 + 16 * 4;
-  hipLaunchKernelGGL((k_lu_invert<double, double>), dim3(nmat), dim3(1024), lds, 0, n, d_a, d_p, d_ws, d_ld, d_ph, 1);
+  hipLaunchKernelGGL((k_gj_invert<double, double>), dim3(nmat), dim3(kGJThreads), gj_lds(n).total, 0, n, d_a, d_p,
+                     d_ws, d_ld, d_ph, 1);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(psiM, d_p, sizeof(double) * nn, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(logdet, d_ld, sizeof(double) * nmat, hipMemcpyDeviceToHost));
@@ -847,11 +846,15 @@ int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int 
     int rc = f32 ? sweep_impl<float>(b, seed, tau, accept_mode) : sweep_impl<double>(b, seed, tau, accept_mode);
     if (rc) return rc;
     if (recompute_every > 0 && (sw + 1) % recompute_every == 0) {
+      if (b->prof_record(2)) return QMCX_ERR_HIP;
       rc = f32 ? launch_lu_batch<float>(b) : launch_lu_batch<double>(b);
       if (rc) return rc;
+      if (b->prof_close()) return QMCX_ERR_HIP;
     }
+    if (b->prof_record(3)) return QMCX_ERR_HIP;
     rc = f32 ? launch_gl<float>(b, true) : launch_gl<double>(b, true);
     if (rc) return rc;
+    if (b->prof_close()) return QMCX_ERR_HIP;
     if (b->prof_on && b->prof_used > 4096 && b->prof_collect()) return QMCX_ERR_HIP;
     hipLaunchKernelGGL(k_next_sweep, dim3(1), dim3(64), 0, b->stream, b->sweep);
     HIPCHK(hipGetLastError());
@@ -1002,17 +1005,19 @@ int qmcx_flex_evaluate_gl(qmcx_batch_t b, double* log_out) {
 int qmcx_set_kernel_timing(qmcx_batch_t b, int enable) {
   if (!b) return fail(QMCX_ERR_ARG, "null batch");
   b->prof_on = enable != 0;
-  b->prof_ms[0] = b->prof_ms[1] = 0;
-  b->prof_n[0] = b->prof_n[1] = 0;
+  for (int i = 0; i < 4; ++i) {
+    b->prof_ms[i] = 0;
+    b->prof_n[i] = 0;
+  }
   b->prof_used = 0;
   return 0;
 }
 
-int qmcx_get_kernel_timing(qmcx_batch_t b, double* ms2, long long* launches2) {
-  if (!b || !ms2 || !launches2) return fail(QMCX_ERR_ARG, "null argument");
-  for (int i = 0; i < 2; ++i) {
-    ms2[i] = b->prof_ms[i];
-    launches2[i] = b->prof_n[i];
+int qmcx_get_kernel_timing(qmcx_batch_t b, double* ms4, long long* launches4) {
+  if (!b || !ms4 || !launches4) return fail(QMCX_ERR_ARG, "null argument");
+  for (int i = 0; i < 4; ++i) {
+    ms4[i] = b->prof_ms[i];
+    launches4[i] = b->prof_n[i];
   }
   return 0;
 }
@@ -1153,12 +1158,12 @@ int qmcx_du_invert_transpose(qmcx_du_t h, const double* logdetT, double* Ainv, d
   if (!h || !logdetT) return fail(QMCX_ERR_ARG, "null argument");
   const size_t nn = (size_t)h->n * h->n;
   HIPCHK(hipMemcpyAsync(h->mat, logdetT, sizeof(double) * nn, hipMemcpyHostToDevice, h->stream));
-  const size_t lds = ((h->n * 4 + 15) & ~15) + 16 * 8 + 16 * 4;
+  const size_t lds = gj_lds(h->n).total;
   if (h->prec == QMCX_FP32)
-    hipLaunchKernelGGL((k_lu_invert<double, float>), dim3(1), dim3(1024), lds, h->stream, h->n, h->mat,
+    hipLaunchKernelGGL((k_gj_invert<double, float>), dim3(1), dim3(kGJThreads), lds, h->stream, h->n, h->mat,
                        (float*)h->Ainv, h->lu_ws, h->ldph, h->ldph + 1, 1);
   else
-    hipLaunchKernelGGL((k_lu_invert<double, double>), dim3(1), dim3(1024), lds, h->stream, h->n, h->mat,
+    hipLaunchKernelGGL((k_gj_invert<double, double>), dim3(1), dim3(kGJThreads), lds, h->stream, h->n, h->mat,
                        (double*)h->Ainv, h->lu_ws, h->ldph, h->ldph + 1, 1);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_du_reset, dim3(1), dim3(64), 0, h->stream, h->ndelay);

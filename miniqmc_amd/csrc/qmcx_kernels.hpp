@@ -339,93 +339,236 @@ __global__ __launch_bounds__(256) void k_flush_mfma(SysP<float> S, BatchP<float>
   }
 }
 
-// ------------------------------------------------------------------ FP64 LU + inverse
-// One workgroup per matrix.  Right-looking LU with partial pivoting (largest
-// |a| in the column, first index on ties, as LAPACK getrf), log|det| from the
-// U diagonal, sign from pivots and diagonal signs (DiracMatrix.h:26-38);
-// then column t of a^-1 is solved by lane t, which is row t of psiM = (a^-1)^T.
+// ------------------------------------------------------------------ FP64 inverse (recompute)
+// psiM = (A^T)^-1 = transpose of A^-1 with log|det A| and its phase, the work of
+// DiracMatrix::invert_transpose (DiracMatrix.h:93-121: transpose, getrf, log det
+// from the pivots, getri).  Here: X = A^T in FP64, then an in-place blocked
+// Gauss-Jordan inversion with partial (row) pivoting.  Its pivots are the U
+// diagonal of LU with partial pivoting, so log|det| and the sign come out the
+// same way.  One workgroup per matrix; the panel of b columns (all n rows)
+// lives in LDS; the other columns get the panel's rank-b update
+//   X[i, c] <- (i in pivot rows ? 0 : Xsw[i, c]) + sum_t P[i, t] Xsw[k0 + t, c]
+// in column chunks streamed through HBM (Xsw = rows after the panel's swaps).
+// The column swaps that undo the pivoting are applied on the way out.
+constexpr int kGJThreads = 1024;
+constexpr int kGJChunk = 64;
+
+__host__ __device__ inline int gj_panel(int n) { return n <= 448 ? 32 : (n <= 896 ? 16 : 8); }
+
+struct GJLds {
+  size_t oP, oXR, oXA, oF, oPr, oRed, oRi, oPiv, oSrc, oSlot, oDl, total;
+};
+__host__ __device__ inline GJLds gj_lds(int n) {
+  const int b = gj_panel(n);
+  GJLds L;
+  size_t o = 0;
+  auto take = [&](size_t bytes) { const size_t at = o; o += (bytes + 15) & ~(size_t)15; return at; };
+  L.oP = take(sizeof(double) * (size_t)n * (b + 1));      // panel, row stride b+1
+  L.oXR = take(sizeof(double) * (size_t)b * kGJChunk);   // pivot rows of a column chunk
+  L.oXA = take(sizeof(double) * (size_t)b * kGJChunk);   // rows displaced by the swaps
+  L.oF = take(sizeof(double) * (size_t)n);               // pivot column of a step
+  L.oPr = take(sizeof(double) * (size_t)b);              // scaled pivot row of a step
+  L.oRed = take(sizeof(double) * 16);
+  L.oRi = take(sizeof(int) * 16);
+  L.oPiv = take(sizeof(int) * (size_t)n);
+  L.oSrc = take(sizeof(int) * (size_t)n);                // row source of the panel's swaps / column source at the end
+  L.oSlot = take(sizeof(int) * (size_t)n);
+  L.oDl = take(sizeof(int) * (size_t)b);
+  L.total = o;
+  return L;
+}
+
 template <typename TI, typename TO>
-__global__ __launch_bounds__(1024) void k_lu_invert(int n, const TI* A, TO* psiM, double* ws, double* logdet,
-                                                    double* phase, int ld_stride) {
+__global__ __launch_bounds__(kGJThreads) void k_gj_invert(int n, const TI* A, TO* psiM, double* ws, double* logdet,
+                                                          double* phase, int ld_stride) {
   const int mat = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
+  const int b = gj_panel(n), ps = b + 1;
+  const GJLds L = gj_lds(n);
   extern __shared__ __align__(16) unsigned char smem[];
-  int* piv = (int*)smem;                           // [n]
-  double* rv = (double*)(smem + ((n * 4 + 15) & ~15));   // [16]
-  int* ri = (int*)(rv + 16);                       // [16]
-  double* LU = ws + (size_t)mat * 2 * n * n;
-  double* X = LU + (size_t)n * n;
+  double* P = (double*)(smem + L.oP);
+  double* XR = (double*)(smem + L.oXR);
+  double* XA = (double*)(smem + L.oXA);
+  double* fcol = (double*)(smem + L.oF);
+  double* prow = (double*)(smem + L.oPr);
+  double* rv = (double*)(smem + L.oRed);
+  int* ri = (int*)(smem + L.oRi);
+  int* piv = (int*)(smem + L.oPiv);
+  int* src = (int*)(smem + L.oSrc);
+  int* slot = (int*)(smem + L.oSlot);
+  int* dl = (int*)(smem + L.oDl);
+  double* X = ws + (size_t)mat * 2 * n * n;
   const TI* a = A + (size_t)mat * n * n;
-  for (size_t i = tid; i < (size_t)n * n; i += nt) LU[i] = (double)a[i];
-  __syncthreads();
-  const int lane = tid & 63, wid = tid >> 6, nw = (nt + 63) >> 6;
+
+  // X = A^T through 32 x 32 LDS tiles (the panel buffer is free yet)
+  {
+    double* T = P;
+    const int nb = (n + 31) / 32;
+    for (int tile = 0; tile < nb * nb; ++tile) {
+      const int ti = tile / nb, tj = tile % nb;
+      const int r = tid >> 5, c = tid & 31;
+      if (tid < 1024) {
+        const int gi = ti * 32 + r, gj = tj * 32 + c;
+        if (r < 32 && gi < n && gj < n) T[r * 33 + c] = (double)a[(size_t)gi * n + gj];
+      }
+      __syncthreads();
+      if (tid < 1024) {
+        const int gi = tj * 32 + r, gj = ti * 32 + c;   // X[gi][gj] = A[gj][gi]
+        if (r < 32 && gi < n && gj < n) X[(size_t)gi * n + gj] = T[c * 33 + r];
+      }
+      __syncthreads();
+    }
+  }
+
   double logacc = 0.0;
   int sign = 1;
-  for (int k = 0; k < n; ++k) {
-    double best = -1.0;
-    int bi = n;
-    for (int r = k + tid; r < n; r += nt) {
-      const double v = fabs(LU[(size_t)r * n + k]);
-      if (v > best) { best = v; bi = r; }
+  for (int k0 = 0; k0 < n; k0 += b) {
+    const int bb = min(b, n - k0);
+    // ---- panel: columns [k0, k0 + bb) of every row
+    for (int e = tid; e < n * bb; e += nt) {
+      const int i = e / bb, j = e % bb;
+      P[i * ps + j] = X[(size_t)i * n + k0 + j];
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ov = __shfl_xor(best, off, 64);
-      const int oi = __shfl_xor(bi, off, 64);
-      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-    }
-    if (lane == 0) { rv[wid] = best; ri[wid] = bi; }
     __syncthreads();
-    best = rv[0];
-    bi = ri[0];
-    for (int q = 1; q < nw; ++q)
-      if (rv[q] > best || (rv[q] == best && ri[q] < bi)) { best = rv[q]; bi = ri[q]; }
-    const int p = bi;
-    if (tid == 0) piv[k] = p;
-    if (p != k)
-      for (int col = tid; col < n; col += nt) {
-        const double t = LU[(size_t)k * n + col];
-        LU[(size_t)k * n + col] = LU[(size_t)p * n + col];
-        LU[(size_t)p * n + col] = t;
+    for (int j = 0; j < bb; ++j) {
+      const int k = k0 + j;
+      // pivot: first row of largest |P[i][j]|, i >= k (idamax semantics)
+      double best = -1.0;
+      int bi = k;
+      for (int i = k + tid; i < n; i += nt) {
+        const double v = fabs(P[i * ps + j]);
+        if (v > best) { best = v; bi = i; }
       }
-    __syncthreads();
-    const double d = LU[(size_t)k * n + k];
-    if (p != k) sign = -sign;
-    if (d < 0) sign = -sign;
-    logacc += log(fabs(d));
-    const double dinv = 1.0 / d;
-    for (int r = k + 1 + tid; r < n; r += nt) LU[(size_t)r * n + k] *= dinv;
-    __syncthreads();
-    for (int r = k + 1 + wid; r < n; r += nw) {
-      const double l = LU[(size_t)r * n + k];
-      for (int col = k + 1 + lane; col < n; col += 64) LU[(size_t)r * n + col] -= l * LU[(size_t)k * n + col];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+      }
+      if (lane == 0) { rv[wid] = best; ri[wid] = bi; }
+      __syncthreads();
+      best = rv[0];
+      bi = ri[0];
+      for (int q = 1; q < nw; ++q)
+        if (rv[q] > best || (rv[q] == best && ri[q] < bi)) { best = rv[q]; bi = ri[q]; }
+      const int p = bi;   // in [k, n)
+      if (tid == 0) piv[k] = p;
+      if (p != k && tid < bb) {
+        const double t = P[k * ps + tid];
+        P[k * ps + tid] = P[p * ps + tid];
+        P[p * ps + tid] = t;
+      }
+      __syncthreads();
+      const double d = P[k * ps + j];
+      if (tid == 0) {
+        if (p != k) sign = -sign;
+        if (d < 0) sign = -sign;
+        logacc += log(fabs(d));
+      }
+      const double dinv = 1.0 / d;
+      if (tid < bb) prow[tid] = tid == j ? dinv : P[k * ps + tid] * dinv;
+      for (int i = tid; i < n; i += nt) fcol[i] = P[i * ps + j];
+      __syncthreads();
+      for (int e = tid; e < n * bb; e += nt) {
+        const int i = e / bb, c = e % bb;
+        const double base = (i == k) ? 0.0 : (c == j ? 0.0 : P[i * ps + c]);
+        P[i * ps + c] = (i == k) ? prow[c] : base - fcol[i] * prow[c];
+      }
+      __syncthreads();
+    }
+    // ---- panel back to X; row sources of the panel's swaps; rows the swaps displaced
+    for (int e = tid; e < n * bb; e += nt) {
+      const int i = e / bb, j = e % bb;
+      X[(size_t)i * n + k0 + j] = P[i * ps + j];
+    }
+    for (int i = tid; i < n; i += nt) {
+      src[i] = i;
+      slot[i] = -1;
     }
     __syncthreads();
+    if (tid == 0) {
+      for (int k = k0; k < k0 + bb; ++k) {
+        const int t = src[k];
+        src[k] = src[piv[k]];
+        src[piv[k]] = t;
+      }
+      int nd = 0;
+      for (int k = k0; k < k0 + bb; ++k) {
+        const int p = piv[k];
+        if (p >= k0 + bb && src[p] != p && slot[p] < 0) {
+          slot[p] = nd;
+          dl[nd++] = p;
+        }
+      }
+      rv[0] = (double)nd;
+    }
+    __syncthreads();
+    const int nd = (int)rv[0];
+    // ---- rank-bb update of the other columns, kGJChunk at a time
+    for (int c0 = 0; c0 < n; c0 += kGJChunk) {
+      const int cw = min(kGJChunk, n - c0);
+      if (c0 >= k0 && c0 + cw <= k0 + bb) continue;   // the chunk is the panel
+      for (int e = tid; e < (bb + nd) * cw; e += nt) {
+        const int t = e / cw, c = e % cw;
+        const int col = c0 + c;
+        if (col >= k0 && col < k0 + bb) continue;
+        if (t < bb) XR[t * kGJChunk + c] = X[(size_t)src[k0 + t] * n + col];
+        else XA[(t - bb) * kGJChunk + c] = X[(size_t)src[dl[t - bb]] * n + col];
+      }
+      __syncthreads();
+      // 4 rows x 4 columns per thread
+      const int rg = (n + 3) / 4, cg = (cw + 3) / 4;
+      for (int g = tid; g < rg * cg; g += nt) {
+        const int i0 = (g / cg) * 4, cc0 = (g % cg) * 4;
+        double acc[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int i = i0 + r, cc = cc0 + c, col = c0 + cc;
+            double v = 0.0;
+            if (i < n && cc < cw && !(col >= k0 && col < k0 + bb) && !(i >= k0 && i < k0 + bb))
+              v = slot[i] >= 0 ? XA[slot[i] * kGJChunk + cc] : X[(size_t)i * n + col];
+            acc[r][c] = v;
+          }
+        for (int t = 0; t < bb; ++t) {
+          double pv[4], xv[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) pv[r] = (i0 + r < n) ? P[(i0 + r) * ps + t] : 0.0;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) xv[c] = (cc0 + c < cw) ? XR[t * kGJChunk + cc0 + c] : 0.0;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[r][c] = fma(pv[r], xv[c], acc[r][c]);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int i = i0 + r, cc = cc0 + c, col = c0 + cc;
+            if (i < n && cc < cw && !(col >= k0 && col < k0 + bb)) X[(size_t)i * n + col] = acc[r][c];
+          }
+      }
+      __syncthreads();
+    }
   }
+  // ---- undo the pivoting: columns swapped back in reverse order, then out in TO
   if (tid == 0) {
+    for (int c = 0; c < n; ++c) src[c] = c;
+    for (int k = n - 1; k >= 0; --k) {
+      const int t = src[k];
+      src[k] = src[piv[k]];
+      src[piv[k]] = t;
+    }
     logdet[(size_t)mat * ld_stride] = logacc;
     phase[(size_t)mat * ld_stride] = sign > 0 ? 0.0 : M_PI;
   }
-  for (int t = tid; t < n; t += nt) {
-    double* x = X + (size_t)t * n;
-    for (int i = 0; i < n; ++i) x[i] = (i == t) ? 1.0 : 0.0;
-    for (int k = 0; k < n; ++k) {
-      const int p = piv[k];
-      if (p != k) { const double tmp = x[k]; x[k] = x[p]; x[p] = tmp; }
-    }
-    for (int i = 0; i < n; ++i) {
-      double s = x[i];
-      const double* li = LU + (size_t)i * n;
-      for (int k = 0; k < i; ++k) s -= li[k] * x[k];
-      x[i] = s;
-    }
-    for (int i = n - 1; i >= 0; --i) {
-      double s = x[i];
-      const double* ui = LU + (size_t)i * n;
-      for (int k = i + 1; k < n; ++k) s -= ui[k] * x[k];
-      x[i] = s / ui[i];
-    }
-    TO* out = psiM + (size_t)mat * n * n + (size_t)t * n;
-    for (int i = 0; i < n; ++i) out[i] = (TO)x[i];
+  __syncthreads();
+  TO* out = psiM + (size_t)mat * n * n;
+  for (size_t e = tid; e < (size_t)n * n; e += nt) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    out[e] = (TO)X[(size_t)i * n + src[j]];
   }
 }
 

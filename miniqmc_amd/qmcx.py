@@ -318,10 +318,11 @@ class WalkerBatch:
         _check(load().qmcx_set_kernel_timing(self.h, 1 if on else 0))
 
     def kernel_timing(self):
-        """(move_ms, move_launches, flush_ms, flush_launches) accumulated since timing was enabled."""
-        ms, n = np.zeros(2), np.zeros(2, dtype=np.int64)
+        """{stage: (ms, launches)} accumulated since timing was enabled, for the stages
+        move, flush, recompute (FP64 LU) and gl (evaluateGL)."""
+        ms, n = np.zeros(4), np.zeros(4, dtype=np.int64)
         _check(load().qmcx_get_kernel_timing(self.h, ms, n))
-        return float(ms[0]), int(n[0]), float(ms[1]), int(n[1])
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(("move", "flush", "recompute", "gl"))}
 
     def last_sweep_ms(self):
         ms = C.c_double(0)

@@ -189,9 +189,10 @@ def _flip_margins(tr, rt, seed, sweep, walker0=0):
 
 def test_flush_mfma_vs_oracle(nio32):
     """FP32 Woodbury flush on the matrix cores and on vector FMA against the FP64 oracle
-    (NiO-32, delay 32, Metropolis).  A walker's accept sequence may only part from the
-    oracle's at a near tie (oracle margin |u - ratio^2| < 1e-3 ratio^2, the FP32 ratio
-    error band); up to its first such move, log|Psi| agrees within 1e-6."""
+    (NiO-32, delay 32, Metropolis, the FP32 form's FP64 inverse recompute once per sweep).
+    A walker's accept sequence may only part from the oracle's at a near tie (oracle margin
+    |u - ratio^2| < 1e-3 ratio^2, the FP32 ratio error band); a walker without one ends the
+    sweep with log|Psi| within 1e-6 of the oracle."""
     sysd, dev, orc = nio32
     R0 = systems.initial_positions(sysd, 2, seed=3)
     ref = orc.run_vmc(R0, 1, 5, 0.3, mode=qmcx.ACCEPT_METROPOLIS, delay=32, trace=True)
@@ -202,7 +203,7 @@ def test_flush_mfma_vs_oracle(nio32):
         wb.set_positions(R0)
         wb.evaluate_log()
         wb.set_trace(True)
-        hist = wb.run(1, 5, 0.3, qmcx.ACCEPT_METROPOLIS)
+        hist = wb.run(1, 5, 0.3, qmcx.ACCEPT_METROPOLIS, recompute_every=1)
         tr = wb.trace()
         rt = ref["trace"][:, 0]
         flips = _flip_margins(tr, rt, 5, 0)
@@ -220,3 +221,29 @@ def test_flush_mfma_vs_oracle(nio32):
                 lerr = abs(hist[0][w] - ref["log"][w, 0]) / abs(ref["log"][w, 0])
                 _say(f"  walker {w}: |dlog|/|log| {lerr:.2e}")
                 assert lerr < 1e-6
+
+
+def test_recompute_matches_fresh_evaluate_log(nio32):
+    """The FP64 inverse recompute at the end of a sweep gives the inverse and log|det| that a
+    fresh evaluateLog of the same positions gives (DiracDeterminant.cpp:237-280), in both
+    precisions."""
+    sysd, dev, orc = nio32
+    R0 = systems.initial_positions(sysd, 3, seed=9)
+    for prec, tol in ((qmcx.FP64, 1e-10), (qmcx.FP32, 1e-6)):
+        _say("recompute vs fresh evaluateLog", prec)
+        wb = qmcx.WalkerBatch(dev, 3, delay_rank=16 if prec == qmcx.FP32 else 1, precision=prec)
+        wb.set_positions(R0)
+        wb.evaluate_log()
+        hist = wb.run(2, 7, 0.3, qmcx.ACCEPT_METROPOLIS, recompute_every=1)
+        fresh = qmcx.WalkerBatch(dev, 3, delay_rank=1, precision=qmcx.FP64)
+        fresh.set_positions(wb.positions())
+        log_f = fresh.evaluate_log()
+        lerr = np.abs(hist[1] - log_f) / np.abs(log_f)
+        ierr = 0.0
+        for w in range(3):
+            for det in range(2):
+                ref_inv = fresh.inverse(w, det)
+                ierr = max(ierr, np.abs(wb.inverse(w, det) - ref_inv).max() / np.abs(ref_inv).max())
+        _say(f"  |dlog|/|log| {lerr.max():.2e}, inverse {ierr:.2e}")
+        assert lerr.max() < tol
+        assert ierr < (1e-9 if prec == qmcx.FP64 else 1e-3)
