@@ -52,6 +52,21 @@ def per_move_bytes(sysd, k, acc, s=4):
     return b_spline + b_pos + b_j + b_det, dict(spline=b_spline, pos=b_pos, jastrow=b_j, det=b_det, cbar=cbar)
 
 
+def measured_traffic(sysd, k, moves_per_launch):
+    """HBM bytes per move-kernel launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by a copy of known size), when it was taken on
+    this workload and launch shape; None otherwise."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{sysd.name}_fp32_k{k}.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh)
+    except OSError:
+        return None, None
+    if rec.get("moves_per_launch") != moves_per_launch:
+        return None, None
+    return round(rec["bytes_per_launch"]), os.path.relpath(path, ROOT)
+
+
 def cpu_baseline(sysd, R0, seed, tau, nwalk, nthreads):
     from oracle import pyoracle as po
     orc = po.OracleSystem(sysd, spline_prec=8)
@@ -80,7 +95,20 @@ def main():
     ap.add_argument("--err-steps", type=int, default=2, help="sweeps of the FP32-vs-FP64 error leg")
     ap.add_argument("--cpu-walkers", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device five times (a kernel of known HBM traffic "
+                         "to scale the FETCH_SIZE/WRITE_SIZE counters by) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        nbytes = args.calibrate_copy << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy": {"bytes_read": nbytes, "bytes_written": nbytes, "dispatches": 5}}))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: walkers are independent, so a multi-GPU run is one "
@@ -178,8 +206,9 @@ def main():
     chunk = k if k > 1 else sysd.nup
     launch_ms = mv_ms / max(mv_n, 1)
     achieved = bpm * W * chunk / (launch_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(sysd, k, W * chunk)
     roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src,
                     kernel="k_moves<float>", bytes_per_move=round(bpm, 1),
                     moves_per_launch=W * chunk, avg_launch_ms=round(launch_ms, 4))
 
