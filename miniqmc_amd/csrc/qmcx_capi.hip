@@ -204,6 +204,13 @@ struct qmcx_batch_s {
   int* flex_decide = nullptr;     // [W]
   int flex_iel = -1;
   hipStream_t stream = nullptr;
+  // FP64 recompute stream: a determinant's refresh runs there, on the CUs the move
+  // kernel leaves idle, while the other determinant's electrons move
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_done[2] = {nullptr, nullptr}, ev_fresh[2] = {nullptr, nullptr}, ev_gl = nullptr;
+  bool pending_dn = false;
+  double* ld_fresh = nullptr;   // [W] log|det| and phase of the refreshed second determinant
+  double* ph_fresh = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
   std::vector<void*> allocs;
@@ -238,7 +245,7 @@ struct qmcx_batch_s {
   size_t prof_used = 0;
   double prof_ms[4] = {0, 0, 0, 0};   // move, flush, FP64 recompute, evaluateGL
   long long prof_n[4] = {0, 0, 0, 0};
-  int prof_record(int kind) {
+  int prof_record(int kind, hipStream_t st = nullptr) {
     if (!prof_on) return 0;
     if (prof_used + 2 > prof_ev.size()) {
       for (int i = 0; i < 64; ++i) {
@@ -249,17 +256,18 @@ struct qmcx_batch_s {
       }
     }
     prof_kind[prof_used] = kind;
-    HIPCHK(hipEventRecord(prof_ev[prof_used++], stream));
+    HIPCHK(hipEventRecord(prof_ev[prof_used++], st ? st : stream));
     return 0;
   }
-  int prof_close() {
+  int prof_close(hipStream_t st = nullptr) {
     if (!prof_on) return 0;
-    HIPCHK(hipEventRecord(prof_ev[prof_used++], stream));
+    HIPCHK(hipEventRecord(prof_ev[prof_used++], st ? st : stream));
     return 0;
   }
   int prof_collect() {
     if (!prof_on) return 0;
     HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipStreamSynchronize(aux));
     for (size_t i = 0; i + 1 < prof_used; i += 2) {
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, prof_ev[i], prof_ev[i + 1]));
@@ -371,9 +379,7 @@ template <typename T>
 int launch_lu_batch(qmcx_batch_t b) {
   const qmcx_system_s* s = b->sys;
   const int n = s->n;
-  hipLaunchKernelGGL((k_gj_invert<T, T>), dim3(b->W * 2), dim3(kGJThreads), gj_lds(n).total, b->stream, n,
-                     (const T*)b->A, (T*)b->Ainv, b->lu_ws, b->logdet, b->phase, 1);
-  HIPCHK(hipGetLastError());
+  HIPCHK((launch_gj_invert<T, T>(n, b->W * 2, b->stream, (const T*)b->A, (T*)b->Ainv, b->lu_ws, b->logdet, b->phase, 1)));
   return 0;
 }
 
@@ -410,8 +416,9 @@ bool use_mfma_flush(const qmcx_batch_t b) {
   return sizeof(T) == 4 && b->sys->n % 16 == 0 && b->flush_kind == 0;
 }
 
+// the moves (and delayed-update flushes) of one determinant's electrons in one sweep
 template <typename T>
-int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode) {
+int moves_det(qmcx_batch_t b, int det, uint64_t seed, double tau, int mode) {
   const qmcx_system_s* s = b->sys;
   int nt = 0, slices = 0;
   if (choose_shape<T>(s, b->slices, nt, slices)) return QMCX_ERR_ARG;
@@ -426,29 +433,90 @@ int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode) {
   a.displ = nullptr;
   a.decide = nullptr;
   const size_t flds = sizeof(T) * 2 * kFlushRows * b->K + sizeof(int) * b->K;
-  for (int det = 0; det < 2; ++det) {
-    for (int e = 0; e < s->n; e += chunk) {
-      a.det = det;
-      a.e_begin = det * s->n + e;
-      a.e_end = det * s->n + std::min(s->n, e + chunk);
-      if (b->prof_record(0)) return QMCX_ERR_HIP;
-      if (launch_moves<T>(b, nt, slices, L.total, a)) return QMCX_ERR_HIP;
-      if (b->prof_close()) return QMCX_ERR_HIP;
-      if (b->K > 1) {
-        if (b->prof_record(1)) return QMCX_ERR_HIP;
-        if (use_mfma_flush<T>(b)) {
-          const int cp = (b->K + 15) & ~15;
-          const size_t lds = sizeof(float) * 2 * kFlushBM * cp + sizeof(int) * b->K;
-          hipLaunchKernelGGL(k_flush_mfma, dim3((s->n + kFlushBM - 1) / kFlushBM, b->W), dim3(256), lds, b->stream,
-                             s->params<float>(), b->params<float>(), det);
-        } else {
-          hipLaunchKernelGGL(k_flush_generic<T>, dim3((s->n + kFlushRows - 1) / kFlushRows, b->W), dim3(256), flds,
-                             b->stream, s->params<T>(), b->params<T>(), det);
-        }
-        HIPCHK(hipGetLastError());
-        if (b->prof_close()) return QMCX_ERR_HIP;
+  for (int e = 0; e < s->n; e += chunk) {
+    a.det = det;
+    a.e_begin = det * s->n + e;
+    a.e_end = det * s->n + std::min(s->n, e + chunk);
+    if (b->prof_record(0)) return QMCX_ERR_HIP;
+    if (launch_moves<T>(b, nt, slices, L.total, a)) return QMCX_ERR_HIP;
+    if (b->prof_close()) return QMCX_ERR_HIP;
+    if (b->K > 1) {
+      if (b->prof_record(1)) return QMCX_ERR_HIP;
+      if (use_mfma_flush<T>(b)) {
+        const int cp = (b->K + 15) & ~15;
+        const size_t lds = sizeof(float) * 2 * kFlushBM * cp + sizeof(int) * b->K;
+        hipLaunchKernelGGL(k_flush_mfma, dim3((s->n + kFlushBM - 1) / kFlushBM, b->W), dim3(256), lds, b->stream,
+                           s->params<float>(), b->params<float>(), det);
+      } else {
+        hipLaunchKernelGGL(k_flush_generic<T>, dim3((s->n + kFlushRows - 1) / kFlushRows, b->W), dim3(256), flds,
+                           b->stream, s->params<T>(), b->params<T>(), det);
       }
+      HIPCHK(hipGetLastError());
+      if (b->prof_close()) return QMCX_ERR_HIP;
     }
+  }
+  return 0;
+}
+
+// FP64 refresh of one determinant of every walker on the recompute stream: the inverse in
+// place; log|det| and phase into the batch arrays (out = logdet) or into ld_fresh / ph_fresh
+template <typename T>
+int refresh_det(qmcx_batch_t b, int det, bool into_fresh) {
+  const int n = b->sys->n;
+  const size_t nn = (size_t)n * n;
+  if (b->prof_record(2, b->aux)) return QMCX_ERR_HIP;
+  if (into_fresh)
+    HIPCHK((launch_gj_invert<T, T>(n, b->W, b->aux, (const T*)b->A + det * nn, (T*)b->Ainv + det * nn,
+                                   b->lu_ws + det * 2 * nn, b->ld_fresh, b->ph_fresh, 1, 2)));
+  else
+    HIPCHK((launch_gj_invert<T, T>(n, b->W, b->aux, (const T*)b->A + det * nn, (T*)b->Ainv + det * nn,
+                                   b->lu_ws + det * 2 * nn, b->logdet + det, b->phase + det, 2, 2)));
+  if (b->prof_close(b->aux)) return QMCX_ERR_HIP;
+  return 0;
+}
+
+// One sweep.  With `recompute`, each determinant's inverse and log|det| are refreshed in FP64
+// on the recompute stream as soon as its electrons have moved: the first determinant's refresh
+// runs beside the second determinant's moves and is in place before evaluateGL; the second
+// determinant's runs beside the next sweep's first-determinant moves and, when it lands,
+// corrects this sweep's recorded log|Psi| (k_refresh_det).  The next sweep's second-determinant
+// moves wait for it.
+template <typename T>
+int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompute, int slot) {
+  if (int rc = moves_det<T>(b, 0, seed, tau, mode)) return rc;
+  if (recompute) {
+    HIPCHK(hipEventRecord(b->ev_done[0], b->stream));
+    HIPCHK(hipStreamWaitEvent(b->aux, b->ev_done[0], 0));
+    if (int rc = refresh_det<T>(b, 0, false)) return rc;
+    HIPCHK(hipEventRecord(b->ev_fresh[0], b->aux));
+  }
+  if (b->pending_dn) {
+    HIPCHK(hipStreamWaitEvent(b->stream, b->ev_fresh[1], 0));
+    b->pending_dn = false;
+  }
+  if (int rc = moves_det<T>(b, 1, seed, tau, mode)) return rc;
+  if (recompute) HIPCHK(hipStreamWaitEvent(b->stream, b->ev_fresh[0], 0));
+  if (b->prof_record(3)) return QMCX_ERR_HIP;
+  if (int rc = launch_gl<T>(b, true)) return rc;
+  if (b->prof_close()) return QMCX_ERR_HIP;
+  if (recompute) {
+    HIPCHK(hipEventRecord(b->ev_gl, b->stream));
+    HIPCHK(hipStreamWaitEvent(b->aux, b->ev_gl, 0));
+    if (int rc = refresh_det<T>(b, 1, true)) return rc;
+    hipLaunchKernelGGL(k_refresh_det, dim3((b->W + 63) / 64), dim3(64), 0, b->aux, b->W, 1, b->ld_fresh, b->ph_fresh,
+                       b->logdet, b->phase, b->logv, b->hist, slot);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(b->ev_fresh[1], b->aux));
+    b->pending_dn = true;
+  }
+  return 0;
+}
+
+// wait for a pending refresh of the second determinant (end of a run, before any other use)
+int settle_refresh(qmcx_batch_t b) {
+  if (b->pending_dn) {
+    HIPCHK(hipStreamWaitEvent(b->stream, b->ev_fresh[1], 0));
+    b->pending_dn = false;
   }
   return 0;
 }
@@ -648,9 +716,7 @@ int qmcx_invert_transpose(int n, int nmat, const double* a, double* psiM, double
   HIPCHK(hipMalloc(&d_ld, sizeof(double) * nmat));
   HIPCHK(hipMalloc(&d_ph, sizeof(double) * nmat));
   HIPCHK(hipMemcpy(d_a, a, sizeof(double) * nn, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL((k_gj_invert<double, double>), dim3(nmat), dim3(kGJThreads), gj_lds(n).total, 0, n, d_a, d_p,
-                     d_ws, d_ld, d_ph, 1);
-  HIPCHK(hipGetLastError());
+  HIPCHK((launch_gj_invert<double, double>(n, nmat, 0, d_a, d_p, d_ws, d_ld, d_ph, 1)));
   HIPCHK(hipMemcpy(psiM, d_p, sizeof(double) * nn, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(logdet, d_ld, sizeof(double) * nmat, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(phase, d_ph, sizeof(double) * nmat, hipMemcpyDeviceToHost));
@@ -726,11 +792,19 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
   rc |= b->alloc(b->L, W * N);
   rc |= b->alloc(b->logv, W);
   rc |= b->alloc(b->lu_ws, W * 2 * 2 * n * n);
+  rc |= b->alloc(b->ld_fresh, W);
+  rc |= b->alloc(b->ph_fresh, W);
   if (rc) {
     qmcx_batch_destroy(b);
     return QMCX_ERR_NOMEM;
   }
   HIPCHK(hipStreamCreate(&b->stream));
+  HIPCHK(hipStreamCreate(&b->aux));
+  for (int d = 0; d < 2; ++d) {
+    HIPCHK(hipEventCreateWithFlags(&b->ev_done[d], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&b->ev_fresh[d], hipEventDisableTiming));
+  }
+  HIPCHK(hipEventCreateWithFlags(&b->ev_gl, hipEventDisableTiming));
   HIPCHK(hipEventCreate(&b->ev0));
   HIPCHK(hipEventCreate(&b->ev1));
   *out = b;
@@ -740,12 +814,19 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
 int qmcx_batch_destroy(qmcx_batch_t b) {
   if (!b) return 0;
   if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->aux) hipStreamSynchronize(b->aux);
   for (void* p : b->allocs) hipFree(p);
   if (b->trace) hipFree(b->trace);
   if (b->hist) hipFree(b->hist);
   if (b->flex_displ) hipFree(b->flex_displ);
   if (b->flex_decide) hipFree(b->flex_decide);
   if (b->stream) hipStreamDestroy(b->stream);
+  if (b->aux) hipStreamDestroy(b->aux);
+  for (int d = 0; d < 2; ++d) {
+    if (b->ev_done[d]) hipEventDestroy(b->ev_done[d]);
+    if (b->ev_fresh[d]) hipEventDestroy(b->ev_fresh[d]);
+  }
+  if (b->ev_gl) hipEventDestroy(b->ev_gl);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
   for (hipEvent_t e : b->prof_ev) hipEventDestroy(e);
@@ -843,22 +924,15 @@ int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int 
   const bool f32 = b->prec == QMCX_FP32;
   HIPCHK(hipEventRecord(b->ev0, b->stream));
   for (int sw = 0; sw < nsweeps; ++sw) {
-    int rc = f32 ? sweep_impl<float>(b, seed, tau, accept_mode) : sweep_impl<double>(b, seed, tau, accept_mode);
+    const bool rec = recompute_every > 0 && (sw + 1) % recompute_every == 0;
+    int rc = f32 ? sweep_impl<float>(b, seed, tau, accept_mode, rec, sw)
+                 : sweep_impl<double>(b, seed, tau, accept_mode, rec, sw);
     if (rc) return rc;
-    if (recompute_every > 0 && (sw + 1) % recompute_every == 0) {
-      if (b->prof_record(2)) return QMCX_ERR_HIP;
-      rc = f32 ? launch_lu_batch<float>(b) : launch_lu_batch<double>(b);
-      if (rc) return rc;
-      if (b->prof_close()) return QMCX_ERR_HIP;
-    }
-    if (b->prof_record(3)) return QMCX_ERR_HIP;
-    rc = f32 ? launch_gl<float>(b, true) : launch_gl<double>(b, true);
-    if (rc) return rc;
-    if (b->prof_close()) return QMCX_ERR_HIP;
     if (b->prof_on && b->prof_used > 4096 && b->prof_collect()) return QMCX_ERR_HIP;
     hipLaunchKernelGGL(k_next_sweep, dim3(1), dim3(64), 0, b->stream, b->sweep);
     HIPCHK(hipGetLastError());
   }
+  if (int rc = settle_refresh(b)) return rc;
   HIPCHK(hipEventRecord(b->ev1, b->stream));
   HIPCHK(hipEventSynchronize(b->ev1));
   HIPCHK(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
@@ -1031,14 +1105,15 @@ int qmcx_last_sweep_ms(qmcx_batch_t b, double* ms) {
 }  // extern "C"
 
 #ifdef QMCX_PHASE_TIMING
-// Profiling builds only: per-phase wall_clock64 ticks of the move kernel (walker 0, lane 0).
-extern "C" int qmcx_debug_phase_ticks(double* out8, int reset) {
-  unsigned long long t[8];
+// Profiling builds only: per-phase wall_clock64 ticks (walker / matrix 0, lane 0); slots 0-4
+// the move kernel's phases, 8-11 the FP64 inverse's transpose, panels, updates and output.
+extern "C" int qmcx_debug_phase_ticks(double* out12, int reset) {
+  unsigned long long t[12];
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpyFromSymbol(t, HIP_SYMBOL(qmcx_phase_ticks), sizeof(t)));
-  for (int i = 0; i < 8; ++i) out8[i] = (double)t[i];
+  for (int i = 0; i < 12; ++i) out12[i] = (double)t[i];
   if (reset) {
-    unsigned long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long z[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(qmcx_phase_ticks), z, sizeof(z)));
   }
   return 0;
@@ -1158,14 +1233,12 @@ int qmcx_du_invert_transpose(qmcx_du_t h, const double* logdetT, double* Ainv, d
   if (!h || !logdetT) return fail(QMCX_ERR_ARG, "null argument");
   const size_t nn = (size_t)h->n * h->n;
   HIPCHK(hipMemcpyAsync(h->mat, logdetT, sizeof(double) * nn, hipMemcpyHostToDevice, h->stream));
-  const size_t lds = gj_lds(h->n).total;
   if (h->prec == QMCX_FP32)
-    hipLaunchKernelGGL((k_gj_invert<double, float>), dim3(1), dim3(kGJThreads), lds, h->stream, h->n, h->mat,
-                       (float*)h->Ainv, h->lu_ws, h->ldph, h->ldph + 1, 1);
+    HIPCHK((launch_gj_invert<double, float>(h->n, 1, h->stream, h->mat, (float*)h->Ainv, h->lu_ws, h->ldph,
+                                           h->ldph + 1, 1)));
   else
-    hipLaunchKernelGGL((k_gj_invert<double, double>), dim3(1), dim3(kGJThreads), lds, h->stream, h->n, h->mat,
-                       (double*)h->Ainv, h->lu_ws, h->ldph, h->ldph + 1, 1);
-  HIPCHK(hipGetLastError());
+    HIPCHK((launch_gj_invert<double, double>(h->n, 1, h->stream, h->mat, (double*)h->Ainv, h->lu_ws, h->ldph,
+                                            h->ldph + 1, 1)));
   hipLaunchKernelGGL(k_du_reset, dim3(1), dim3(64), 0, h->stream, h->ndelay);
   HIPCHK(hipGetLastError());
   h->count = 0;

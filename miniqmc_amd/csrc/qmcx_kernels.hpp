@@ -350,7 +350,6 @@ __global__ __launch_bounds__(256) void k_flush_mfma(SysP<float> S, BatchP<float>
 //   X[i, c] <- (i in pivot rows ? 0 : Xsw[i, c]) + sum_t P[i, t] Xsw[k0 + t, c]
 // in column chunks streamed through HBM (Xsw = rows after the panel's swaps).
 // The column swaps that undo the pivoting are applied on the way out.
-constexpr int kGJThreads = 1024;
 constexpr int kGJChunk = 64;
 
 __host__ __device__ inline int gj_panel(int n) { return n <= 448 ? 32 : (n <= 896 ? 16 : 8); }
@@ -378,12 +377,12 @@ __host__ __device__ inline GJLds gj_lds(int n) {
   return L;
 }
 
-template <typename TI, typename TO>
-__global__ __launch_bounds__(kGJThreads) void k_gj_invert(int n, const TI* A, TO* psiM, double* ws, double* logdet,
-                                                          double* phase, int ld_stride) {
+template <typename TI, typename TO, int B, int NT>
+__global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, double* ws, double* logdet,
+                                                          double* phase, int ld_stride, int mat_stride) {
   const int mat = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
-  const int b = gj_panel(n), ps = b + 1;
+  const int b = B, ps = b + 1;
   const GJLds L = gj_lds(n);
   extern __shared__ __align__(16) unsigned char smem[];
   double* P = (double*)(smem + L.oP);
@@ -397,47 +396,72 @@ __global__ __launch_bounds__(kGJThreads) void k_gj_invert(int n, const TI* A, TO
   int* src = (int*)(smem + L.oSrc);
   int* slot = (int*)(smem + L.oSlot);
   int* dl = (int*)(smem + L.oDl);
-  double* X = ws + (size_t)mat * 2 * n * n;
-  const TI* a = A + (size_t)mat * n * n;
+  const size_t mo = (size_t)mat * mat_stride;   // matrix slot (every mat_stride-th matrix)
+  double* X = ws + mo * 2 * n * n;
+  const TI* a = A + mo * n * n;
+#ifdef QMCX_PHASE_TIMING
+  unsigned long long gtick_ = wall_clock64();
+#define GJ_TICK(slot)                                      \
+  if (mat == 0 && tid == 0) {                              \
+    const unsigned long long now_ = wall_clock64();        \
+    qmcx_phase_ticks[slot] += now_ - gtick_;               \
+    gtick_ = now_;                                         \
+  }
+#else
+#define GJ_TICK(slot)
+#endif
 
-  // X = A^T through 32 x 32 LDS tiles (the panel buffer is free yet)
-  {
-    double* T = P;
-    const int nb = (n + 31) / 32;
-    for (int tile = 0; tile < nb * nb; ++tile) {
-      const int ti = tile / nb, tj = tile % nb;
-      const int r = tid >> 5, c = tid & 31;
-      if (tid < 1024) {
-        const int gi = ti * 32 + r, gj = tj * 32 + c;
-        if (r < 32 && gi < n && gj < n) T[r * 33 + c] = (double)a[(size_t)gi * n + gj];
+  // X = A^T, b rows of A at a time through LDS (the panel buffer is free yet;
+  // b (n + 1) <= n (b + 1) doubles)
+  for (int s0 = 0; s0 < n; s0 += b) {
+    const int sb = min(b, n - s0);
+    double* T = P;   // [sb][n + 1]
+    // eight loads in flight per thread before their LDS stores
+    for (int e0 = tid; e0 < sb * n; e0 += nt * 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * nt;
+        v[u] = e < sb * n ? (double)a[(size_t)(s0 + e / n) * n + e % n] : 0.0;
       }
-      __syncthreads();
-      if (tid < 1024) {
-        const int gi = tj * 32 + r, gj = ti * 32 + c;   // X[gi][gj] = A[gj][gi]
-        if (r < 32 && gi < n && gj < n) X[(size_t)gi * n + gj] = T[c * 33 + r];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * nt;
+        if (e < sb * n) T[(e / n) * (n + 1) + e % n] = v[u];
       }
-      __syncthreads();
     }
+    __syncthreads();
+    for (int e = tid; e < sb * n; e += nt) {
+      const int c = e / sb, r = e % sb;   // X[c][s0 + r] = A[s0 + r][c]
+      X[(size_t)c * n + s0 + r] = T[r * (n + 1) + c];
+    }
+    __syncthreads();
   }
 
   double logacc = 0.0;
   int sign = 1;
+  GJ_TICK(8)
   for (int k0 = 0; k0 < n; k0 += b) {
     const int bb = min(b, n - k0);
-    // ---- panel: columns [k0, k0 + bb) of every row
-    for (int e = tid; e < n * bb; e += nt) {
-      const int i = e / bb, j = e % bb;
-      P[i * ps + j] = X[(size_t)i * n + k0 + j];
-    }
-    __syncthreads();
+    // ---- panel: columns [k0, k0 + bb); thread i holds row i in registers
+    const bool own = tid < n;
+    double pr[B];
+#pragma unroll
+    for (int c = 0; c < B; ++c) pr[c] = (own && c < bb) ? X[(size_t)tid * n + k0 + c] : 0.0;
+    double* swp = fcol;   // old pivot-position row, for the thread that takes it
+    // the step loop stays rolled: register entries are picked by compare-and-select on j
+#pragma unroll 1
     for (int j = 0; j < bb; ++j) {
       const int k = k0 + j;
+      double prj = 0.0;
+#pragma unroll
+      for (int c = 0; c < B; ++c) prj = c == j ? pr[c] : prj;
       // pivot: first row of largest |P[i][j]|, i >= k (idamax semantics)
       double best = -1.0;
       int bi = k;
-      for (int i = k + tid; i < n; i += nt) {
-        const double v = fabs(P[i * ps + j]);
-        if (v > best) { best = v; bi = i; }
+      if (own && tid >= k) {
+        best = fabs(prj);
+        bi = tid;
       }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
@@ -452,30 +476,43 @@ __global__ __launch_bounds__(kGJThreads) void k_gj_invert(int n, const TI* A, TO
       for (int q = 1; q < nw; ++q)
         if (rv[q] > best || (rv[q] == best && ri[q] < bi)) { best = rv[q]; bi = ri[q]; }
       const int p = bi;   // in [k, n)
-      if (tid == 0) piv[k] = p;
-      if (p != k && tid < bb) {
-        const double t = P[k * ps + tid];
-        P[k * ps + tid] = P[p * ps + tid];
-        P[p * ps + tid] = t;
-      }
+      if (tid == p)
+#pragma unroll
+        for (int c = 0; c < B; ++c) prow[c] = pr[c];
+      if (tid == k && p != k)
+#pragma unroll
+        for (int c = 0; c < B; ++c) swp[c] = pr[c];
       __syncthreads();
-      const double d = P[k * ps + j];
+      const double d = prow[j];
+      const double dinv = 1.0 / d;
       if (tid == 0) {
+        piv[k] = p;
         if (p != k) sign = -sign;
         if (d < 0) sign = -sign;
         logacc += log(fabs(d));
       }
-      const double dinv = 1.0 / d;
-      if (tid < bb) prow[tid] = tid == j ? dinv : P[k * ps + tid] * dinv;
-      for (int i = tid; i < n; i += nt) fcol[i] = P[i * ps + j];
-      __syncthreads();
-      for (int e = tid; e < n * bb; e += nt) {
-        const int i = e / bb, c = e % bb;
-        const double base = (i == k) ? 0.0 : (c == j ? 0.0 : P[i * ps + c]);
-        P[i * ps + c] = (i == k) ? prow[c] : base - fcol[i] * prow[c];
+      if (own) {
+        if (tid == k) {
+#pragma unroll
+          for (int c = 0; c < B; ++c) pr[c] = c == j ? dinv : prow[c] * dinv;
+        } else {
+          double f = prj;
+          if (tid == p) {
+#pragma unroll
+            for (int c = 0; c < B; ++c) pr[c] = swp[c];
+            f = swp[j];
+          }
+#pragma unroll
+          for (int c = 0; c < B; ++c) pr[c] = c == j ? -f * dinv : pr[c] - f * (prow[c] * dinv);
+        }
       }
-      __syncthreads();
     }
+    if (own)
+#pragma unroll
+      for (int c = 0; c < B; ++c)
+        if (c < bb) P[tid * ps + c] = pr[c];
+    __syncthreads();
+    GJ_TICK(9)
     // ---- panel back to X; row sources of the panel's swaps; rows the swaps displaced
     for (int e = tid; e < n * bb; e += nt) {
       const int i = e / bb, j = e % bb;
@@ -508,21 +545,35 @@ __global__ __launch_bounds__(kGJThreads) void k_gj_invert(int n, const TI* A, TO
     for (int c0 = 0; c0 < n; c0 += kGJChunk) {
       const int cw = min(kGJChunk, n - c0);
       if (c0 >= k0 && c0 + cw <= k0 + bb) continue;   // the chunk is the panel
-      for (int e = tid; e < (bb + nd) * cw; e += nt) {
-        const int t = e / cw, c = e % cw;
-        const int col = c0 + c;
-        if (col >= k0 && col < k0 + bb) continue;
-        if (t < bb) XR[t * kGJChunk + c] = X[(size_t)src[k0 + t] * n + col];
-        else XA[(t - bb) * kGJChunk + c] = X[(size_t)src[dl[t - bb]] * n + col];
+      for (int e0 = tid; e0 < (bb + nd) * cw; e0 += nt * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = e0 + u * nt;
+          const int t = e / cw, col = c0 + e % cw;
+          v[u] = (e < (bb + nd) * cw && !(col >= k0 && col < k0 + bb))
+                     ? X[(size_t)src[t < bb ? k0 + t : dl[t - bb]] * n + col]
+                     : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = e0 + u * nt;
+          const int t = e / cw, c = e % cw;
+          if (e < (bb + nd) * cw) {
+            if (t < bb) XR[t * kGJChunk + c] = v[u];
+            else XA[(t - bb) * kGJChunk + c] = v[u];
+          }
+        }
       }
       __syncthreads();
-      // 4 rows x 4 columns per thread
-      const int rg = (n + 3) / 4, cg = (cw + 3) / 4;
-      for (int g = tid; g < rg * cg; g += nt) {
-        const int i0 = (g / cg) * 4, cc0 = (g % cg) * 4;
-        double acc[4][4];
+      // TR rows x 4 columns per thread, all TR x 4 loads issued before the FMAs
+      constexpr int TR = NT == 512 ? 12 : 4;
+      const int rg = (n + TR - 1) / TR, cg = (cw + 3) / 4, ntile = rg * cg;
+      for (int g = tid; g < ntile; g += nt) {
+        const int i0 = (g / cg) * TR, cc0 = (g % cg) * 4;
+        double acc[TR][4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < TR; ++r)
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const int i = i0 + r, cc = cc0 + c, col = c0 + cc;
@@ -532,18 +583,18 @@ __global__ __launch_bounds__(kGJThreads) void k_gj_invert(int n, const TI* A, TO
             acc[r][c] = v;
           }
         for (int t = 0; t < bb; ++t) {
-          double pv[4], xv[4];
+          double pv[TR], xv[4];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) pv[r] = (i0 + r < n) ? P[(i0 + r) * ps + t] : 0.0;
+          for (int r = 0; r < TR; ++r) pv[r] = (i0 + r < n) ? P[(i0 + r) * ps + t] : 0.0;
 #pragma unroll
           for (int c = 0; c < 4; ++c) xv[c] = (cc0 + c < cw) ? XR[t * kGJChunk + cc0 + c] : 0.0;
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
+          for (int r = 0; r < TR; ++r)
 #pragma unroll
             for (int c = 0; c < 4; ++c) acc[r][c] = fma(pv[r], xv[c], acc[r][c]);
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < TR; ++r)
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const int i = i0 + r, cc = cc0 + c, col = c0 + cc;
@@ -552,6 +603,7 @@ __global__ __launch_bounds__(kGJThreads) void k_gj_invert(int n, const TI* A, TO
       }
       __syncthreads();
     }
+    GJ_TICK(10)
   }
   // ---- undo the pivoting: columns swapped back in reverse order, then out in TO
   if (tid == 0) {
@@ -565,11 +617,54 @@ __global__ __launch_bounds__(kGJThreads) void k_gj_invert(int n, const TI* A, TO
     phase[(size_t)mat * ld_stride] = sign > 0 ? 0.0 : M_PI;
   }
   __syncthreads();
-  TO* out = psiM + (size_t)mat * n * n;
-  for (size_t e = tid; e < (size_t)n * n; e += nt) {
-    const int i = (int)(e / n), j = (int)(e % n);
-    out[e] = (TO)X[(size_t)i * n + src[j]];
+  // psiM[i][j] = X[i][src[j]], b rows at a time through LDS
+  TO* out = psiM + mo * n * n;
+  for (int s0 = 0; s0 < n; s0 += b) {
+    const int sb = min(b, n - s0);
+    double* T = P;   // [sb][n + 1]
+    for (int e0 = tid; e0 < sb * n; e0 += nt * 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * nt;
+        v[u] = e < sb * n ? X[(size_t)(s0 + e / n) * n + e % n] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * nt;
+        if (e < sb * n) T[(e / n) * (n + 1) + e % n] = v[u];
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < sb * n; e += nt) {
+      const int r = e / n, j = e % n;
+      out[(size_t)(s0 + r) * n + j] = (TO)T[r * (n + 1) + src[j]];
+    }
+    __syncthreads();
   }
+  GJ_TICK(11)
+#undef GJ_TICK
+}
+
+// host side: the panel width is a template argument (the panel row lives in registers)
+// nmat matrices at A, A + mat_stride n^2, ...; log|det| and phase of matrix m at [m * ld_stride]
+template <typename TI, typename TO>
+inline hipError_t launch_gj_invert(int n, int nmat, hipStream_t st, const TI* A, TO* psiM, double* ws,
+                                   double* logdet, double* phase, int ld_stride, int mat_stride = 1) {
+  // one thread per row of the panel: 512 threads up to n = 448 (32 registers pairs of panel row),
+  // 1024 threads beyond
+  const size_t lds = gj_lds(n).total;
+  const int b = gj_panel(n);
+  if (b == 32)
+    hipLaunchKernelGGL((k_gj_invert<TI, TO, 32, 512>), dim3(nmat), dim3(512), lds, st, n, A, psiM, ws, logdet, phase,
+                       ld_stride, mat_stride);
+  else if (b == 16)
+    hipLaunchKernelGGL((k_gj_invert<TI, TO, 16, 1024>), dim3(nmat), dim3(1024), lds, st, n, A, psiM, ws, logdet,
+                       phase, ld_stride, mat_stride);
+  else
+    hipLaunchKernelGGL((k_gj_invert<TI, TO, 8, 1024>), dim3(nmat), dim3(1024), lds, st, n, A, psiM, ws, logdet, phase,
+                       ld_stride, mat_stride);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ SPO rows (evaluateLog)
@@ -776,6 +871,20 @@ __global__ __launch_bounds__(256) void k_finish_gl(SysP<T> S, BatchP<T> B, doubl
 
 __global__ void k_next_sweep(int* sweep) {
   if (threadIdx.x == 0 && blockIdx.x == 0) sweep[0] += 1;
+}
+
+// The second determinant's FP64 refresh lands after that sweep's evaluateGL (it runs beside
+// the next sweep's first-determinant moves): swap in the fresh log|det| and phase and move the
+// sweep's recorded log|Psi| by the same difference.
+__global__ void k_refresh_det(int W, int det, const double* ld_fresh, const double* ph_fresh, double* logdet,
+                              double* phase, double* logv, double* hist, int slot) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  const double delta = ld_fresh[w] - logdet[w * 2 + det];
+  logdet[w * 2 + det] = ld_fresh[w];
+  phase[w * 2 + det] = ph_fresh[w];
+  logv[w] += delta;
+  if (hist != nullptr) hist[(size_t)slot * W + w] += delta;
 }
 
 }  // namespace qmcx

@@ -205,7 +205,7 @@ __device__ inline void wave_lds_fence() { asm volatile("s_waitcnt lgkmcnt(0)" ::
 
 #ifdef QMCX_PHASE_TIMING
 // wall_clock64 ticks per phase, accumulated by lane 0 of workgroup 0
-__device__ unsigned long long qmcx_phase_ticks[8];
+__device__ unsigned long long qmcx_phase_ticks[12];
 #define QMCX_TICK(slot)                                                  \
   if (w == 0 && tid == 0) {                                              \
     const unsigned long long now_ = wall_clock64();                      \

@@ -247,3 +247,20 @@ def test_recompute_matches_fresh_evaluate_log(nio32):
         _say(f"  |dlog|/|log| {lerr.max():.2e}, inverse {ierr:.2e}")
         assert lerr.max() < tol
         assert ierr < (1e-9 if prec == qmcx.FP64 else 1e-3)
+
+
+def test_recompute_overlapped_history_vs_oracle(cubic, cubic_dev, cubic_orc):
+    """Several sweeps in one call with the FP64 refresh every sweep: the second determinant's
+    refresh lands during the next sweep and corrects the recorded log|Psi| of its own sweep.
+    Every sweep's recorded value must match the oracle's (FP64, delay 4, Metropolis)."""
+    _say("overlapped recompute history")
+    R0 = systems.initial_positions(cubic, 3, seed=11)
+    ref = cubic_orc.run_vmc(R0, 3, 11, 0.5, mode=qmcx.ACCEPT_METROPOLIS, delay=4, trace=False)
+    wb = qmcx.WalkerBatch(cubic_dev, 3, delay_rank=4, precision=qmcx.FP64)
+    wb.set_positions(R0)
+    wb.evaluate_log()
+    hist = wb.run(3, 11, 0.5, qmcx.ACCEPT_METROPOLIS, recompute_every=1)
+    np.testing.assert_allclose(hist, ref["log"].T, rtol=1e-10)
+    np.testing.assert_allclose(wb.positions(), ref["R"], rtol=0, atol=1e-12)
+    G, L, lv = wb.gl()
+    np.testing.assert_allclose(lv, ref["log"][:, -1], rtol=1e-10)
