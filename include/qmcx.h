@@ -144,12 +144,13 @@ int qmcx_get_accepts(qmcx_batch_t b, long long* naccept);
 /* Inverse matrix psiM of one determinant of one walker as FP64, [n][n] (DiracDeterminant::psiM). */
 int qmcx_get_inverse(qmcx_batch_t b, int walker, int det, double* psiM);
 
-/* Per-stage device time: with timing enabled, HIP events on the batch stream bracket every
+/* Per-stage device time: with timing enabled, HIP events on the batch's streams bracket every
  * launch of qmcx_run_sweeps.  Index 0: move kernel; 1: delayed-update flush; 2: FP64 inverse
- * recompute (LU); 3: evaluateGL.  ms4[i] accumulates milliseconds and launches4[i] the number of
- * bracketed launches.  Enabling resets the totals. */
+ * recompute; 3: evaluateGL; 4: SPO evaluation at the proposals.  ms5[i] accumulates
+ * milliseconds and launches5[i] the number of bracketed launches.  Stages 2 and 4 run on their
+ * own streams, beside the moves.  Enabling resets the totals. */
 int qmcx_set_kernel_timing(qmcx_batch_t b, int enable);
-int qmcx_get_kernel_timing(qmcx_batch_t b, double* ms4, long long* launches4);
+int qmcx_get_kernel_timing(qmcx_batch_t b, double* ms5, long long* launches5);
 
 /* ---- delayed-update engine of one determinant (DU_TYPE) --------------------
  * The plug point DiracDeterminant<DU_TYPE> (src/QMCWaveFunctions/DiracDeterminant.h:35,127,

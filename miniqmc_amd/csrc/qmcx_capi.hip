@@ -208,6 +208,10 @@ struct qmcx_batch_s {
   // kernel leaves idle, while the other determinant's electrons move
   hipStream_t aux = nullptr;
   hipEvent_t ev_done[2] = {nullptr, nullptr}, ev_fresh[2] = {nullptr, nullptr}, ev_gl = nullptr;
+  // proposals stream: the SPO rows of a determinant's next proposals are evaluated there
+  // as soon as its electrons have moved, beside the other determinant's moves
+  hipStream_t pre = nullptr;
+  hipEvent_t ev_pre[2] = {nullptr, nullptr}, ev_start = nullptr;
   bool pending_dn = false;
   double* ld_fresh = nullptr;   // [W] log|det| and phase of the refreshed second determinant
   double* ph_fresh = nullptr;
@@ -223,6 +227,8 @@ struct qmcx_batch_s {
   void* J1 = nullptr;
   void* Vbuf = nullptr;
   void* Binv = nullptr;
+  void* spo = nullptr;   // [W][2][n][5][npad] SPO rows at the sweep's proposals
+  double* j1p = nullptr; // [W][2][n][8] J1 at the sweep's proposals
   int* rows = nullptr;
   int* ndelay = nullptr;
   double* logdet = nullptr;
@@ -243,8 +249,8 @@ struct qmcx_batch_s {
   std::vector<hipEvent_t> prof_ev;
   std::vector<int> prof_kind;   // 0 = move kernel, 1 = flush kernel
   size_t prof_used = 0;
-  double prof_ms[4] = {0, 0, 0, 0};   // move, flush, FP64 recompute, evaluateGL
-  long long prof_n[4] = {0, 0, 0, 0};
+  double prof_ms[5] = {0, 0, 0, 0, 0};   // move, flush, FP64 recompute, evaluateGL, proposals
+  long long prof_n[5] = {0, 0, 0, 0, 0};
   int prof_record(int kind, hipStream_t st = nullptr) {
     if (!prof_on) return 0;
     if (prof_used + 2 > prof_ev.size()) {
@@ -268,6 +274,7 @@ struct qmcx_batch_s {
     if (!prof_on) return 0;
     HIPCHK(hipStreamSynchronize(stream));
     HIPCHK(hipStreamSynchronize(aux));
+    HIPCHK(hipStreamSynchronize(pre));
     for (size_t i = 0; i + 1 < prof_used; i += 2) {
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, prof_ev[i], prof_ev[i + 1]));
@@ -315,41 +322,56 @@ struct qmcx_batch_s {
 
 namespace {
 
-// workgroup size and spline slices for the move kernel: at most kMoveThreads
-// lanes, at most two electrons per lane, 16/slices stencil columns per lane.
-template <typename T>
-int choose_shape(const qmcx_system_s* s, int want_slices, int& nt, int& slices) {
-  const int vw = 16 / (int)sizeof(T);
-  const int nq = s->npad / vw;
-  const int base = std::max(256, ((s->N + 1) / 2 + 63) / 64 * 64);
-  slices = want_slices;
-  if (slices <= 0) {
-    slices = 1;
-    while (slices < 16 && nq * slices * 2 <= kMoveThreads) slices *= 2;
-  }
-  if (16 % slices != 0 || nq * slices > kMoveThreads) return fail(QMCX_ERR_ARG, "spline tile does not fit a workgroup");
-  nt = std::max(base, (nq * slices + 63) / 64 * 64);
-  if (nt > kMoveThreads || s->N > 2 * nt) return fail(QMCX_ERR_ARG, "workgroup too large for this system");
+// workgroup size of the move kernel: a lane per orbital, at most two electrons
+// per lane for the J2 rows, at most kMoveThreads lanes
+int choose_shape(const qmcx_system_s* s, int& nt) {
+  const int need = std::max(s->n, (s->N + 1) / 2);
+  nt = std::max(256, (need + 63) / 64 * 64);
+  if (nt > kMoveThreads) return fail(QMCX_ERR_ARG, "workgroup too large for this system");
   return 0;
 }
 
-template <typename T, int JPT>
-int launch_moves_jpt(qmcx_batch_t b, int nt, int slices, size_t lds, const MoveArgs& a) {
+// spline slices (16/slices stencil columns per lane) of the proposals kernel.  Two by default:
+// small blocks (a few waves, slices x 5 x npad of LDS) that fit beside the move kernel's and
+// the FP64 refresh's workgroups on their CUs, so the proposals use bandwidth those leave idle.
+template <typename T>
+int proposal_slices(const qmcx_system_s* s, int want) {
+  const int nq = s->npad / (16 / (int)sizeof(T));
+  if (want > 0) return want;
+  return nq * 2 <= 1024 ? 2 : 1;
+}
+
+template <typename T, int JPT, bool KD>
+int launch_moves_kd(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
   const qmcx_system_s* s = b->sys;
-  switch (slices) {
-    case 16: hipLaunchKernelGGL((k_moves<T, JPT, 1>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(), a); break;
-    case 8: hipLaunchKernelGGL((k_moves<T, JPT, 2>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(), a); break;
-    case 4: hipLaunchKernelGGL((k_moves<T, JPT, 4>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(), a); break;
-    case 2: hipLaunchKernelGGL((k_moves<T, JPT, 8>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(), a); break;
-    default: hipLaunchKernelGGL((k_moves<T, JPT, 16>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(), a); break;
-  }
+  hipLaunchKernelGGL((k_moves<T, JPT, KD>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(), a);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
+template <typename T, int JPT>
+int launch_moves_jpt(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
+  return b->K > 1 ? launch_moves_kd<T, JPT, true>(b, nt, lds, a) : launch_moves_kd<T, JPT, false>(b, nt, lds, a);
+}
+
 template <typename T>
-int launch_moves(qmcx_batch_t b, int nt, int slices, size_t lds, const MoveArgs& a) {
-  return (b->sys->N <= nt) ? launch_moves_jpt<T, 1>(b, nt, slices, lds, a) : launch_moves_jpt<T, 2>(b, nt, slices, lds, a);
+int launch_moves(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
+  return (b->sys->N <= nt) ? launch_moves_jpt<T, 1>(b, nt, lds, a) : launch_moves_jpt<T, 2>(b, nt, lds, a);
+}
+
+// SPO rows at the proposals of electrons det*n + [e0, e0 + ne) of every walker
+template <typename T>
+int launch_proposals(qmcx_batch_t b, hipStream_t st, int det, int e0, int ne, uint64_t seed, int sweep, double tau,
+                     const double* displ) {
+  const qmcx_system_s* s = b->sys;
+  const int nq = s->npad / (16 / (int)sizeof(T));
+  const int slices = proposal_slices<T>(s, b->slices);
+  const int nt = std::max(64, (nq * slices + 63) / 64 * 64);
+  const size_t lds = std::max(sizeof(T) * (size_t)slices * 5 * s->npad, sizeof(double) * 16 * 5);
+  hipLaunchKernelGGL(k_spo_proposals<T>, dim3(ne, b->W), dim3(nt), lds, st, s->params<T>(), b->params<T>(), det, e0,
+                     seed, sweep, std::sqrt(tau), b->walker0, displ, (T*)b->spo, b->j1p, slices);
+  HIPCHK(hipGetLastError());
+  return 0;
 }
 
 template <typename T>
@@ -420,13 +442,15 @@ bool use_mfma_flush(const qmcx_batch_t b) {
 template <typename T>
 int moves_det(qmcx_batch_t b, int det, uint64_t seed, double tau, int mode) {
   const qmcx_system_s* s = b->sys;
-  int nt = 0, slices = 0;
-  if (choose_shape<T>(s, b->slices, nt, slices)) return QMCX_ERR_ARG;
+  int nt = 0;
+  if (choose_shape(s, nt)) return QMCX_ERR_ARG;
   const int chunk = b->K == 1 ? s->n : b->K;
-  const MoveLds L = move_lds<T>(s->N, s->nions, s->n, b->K, slices, s->npad, chunk);
+  const MoveLds L = move_lds<T>(s->N, s->nions, s->n, b->K, chunk);
   MoveArgs a;
   a.mode = mode;
-  a.slices = slices;
+  a.slices = 0;
+  a.spo = b->spo;
+  a.j1p = b->j1p;
   a.walker0 = b->walker0;
   a.sqrt_tau = std::sqrt(tau);
   a.seed = seed;
@@ -438,13 +462,12 @@ int moves_det(qmcx_batch_t b, int det, uint64_t seed, double tau, int mode) {
     a.e_begin = det * s->n + e;
     a.e_end = det * s->n + std::min(s->n, e + chunk);
     if (b->prof_record(0)) return QMCX_ERR_HIP;
-    if (launch_moves<T>(b, nt, slices, L.total, a)) return QMCX_ERR_HIP;
+    if (launch_moves<T>(b, nt, L.total, a)) return QMCX_ERR_HIP;
     if (b->prof_close()) return QMCX_ERR_HIP;
     if (b->K > 1) {
       if (b->prof_record(1)) return QMCX_ERR_HIP;
       if (use_mfma_flush<T>(b)) {
-        const int cp = (b->K + 15) & ~15;
-        const size_t lds = sizeof(float) * 2 * kFlushBM * cp + sizeof(int) * b->K;
+        const size_t lds = flush_mfma_lds(s->n, b->K);
         hipLaunchKernelGGL(k_flush_mfma, dim3((s->n + kFlushBM - 1) / kFlushBM, b->W), dim3(256), lds, b->stream,
                            s->params<float>(), b->params<float>(), det);
       } else {
@@ -475,17 +498,35 @@ int refresh_det(qmcx_batch_t b, int det, bool into_fresh) {
   return 0;
 }
 
-// One sweep.  With `recompute`, each determinant's inverse and log|det| are refreshed in FP64
-// on the recompute stream as soon as its electrons have moved: the first determinant's refresh
-// runs beside the second determinant's moves and is in place before evaluateGL; the second
-// determinant's runs beside the next sweep's first-determinant moves and, when it lands,
-// corrects this sweep's recorded log|Psi| (k_refresh_det).  The next sweep's second-determinant
-// moves wait for it.
+// SPO rows of determinant det's proposals in sweep `sweep` on the proposals stream, after
+// the event `after` (positions final), announced by ev_pre[det]
 template <typename T>
-int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompute, int slot) {
+int issue_proposals(qmcx_batch_t b, int det, hipEvent_t after, uint64_t seed, int sweep, double tau) {
+  if (after) HIPCHK(hipStreamWaitEvent(b->pre, after, 0));
+  if (b->prof_record(4, b->pre)) return QMCX_ERR_HIP;
+  if (int rc = launch_proposals<T>(b, b->pre, det, 0, b->sys->n, seed, sweep, tau, nullptr)) return rc;
+  if (b->prof_close(b->pre)) return QMCX_ERR_HIP;
+  HIPCHK(hipEventRecord(b->ev_pre[det], b->pre));
+  return 0;
+}
+
+// One sweep (absolute sweep number `sweep`; `slot` in this run).  A determinant's moves wait
+// for its proposals' SPO rows; as soon as they are done, the next sweep's rows for that
+// determinant are evaluated on the proposals stream (unless this is the run's last sweep).
+// With `recompute`, each determinant's inverse and log|det| are refreshed in FP64 on the
+// recompute stream as soon as its electrons have moved: the first determinant's refresh runs
+// beside the second determinant's moves and is in place before evaluateGL; the second
+// determinant's runs beside the next sweep's first-determinant moves and, when it lands,
+// corrects this sweep's recorded log|Psi| (k_refresh_det).  The next sweep's
+// second-determinant moves wait for it.
+template <typename T>
+int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompute, int slot, int sweep, bool last) {
+  HIPCHK(hipStreamWaitEvent(b->stream, b->ev_pre[0], 0));
   if (int rc = moves_det<T>(b, 0, seed, tau, mode)) return rc;
+  HIPCHK(hipEventRecord(b->ev_done[0], b->stream));
+  if (!last)
+    if (int rc = issue_proposals<T>(b, 0, b->ev_done[0], seed, sweep + 1, tau)) return rc;
   if (recompute) {
-    HIPCHK(hipEventRecord(b->ev_done[0], b->stream));
     HIPCHK(hipStreamWaitEvent(b->aux, b->ev_done[0], 0));
     if (int rc = refresh_det<T>(b, 0, false)) return rc;
     HIPCHK(hipEventRecord(b->ev_fresh[0], b->aux));
@@ -494,7 +535,11 @@ int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompu
     HIPCHK(hipStreamWaitEvent(b->stream, b->ev_fresh[1], 0));
     b->pending_dn = false;
   }
+  HIPCHK(hipStreamWaitEvent(b->stream, b->ev_pre[1], 0));
   if (int rc = moves_det<T>(b, 1, seed, tau, mode)) return rc;
+  HIPCHK(hipEventRecord(b->ev_done[1], b->stream));
+  if (!last)
+    if (int rc = issue_proposals<T>(b, 1, b->ev_done[1], seed, sweep + 1, tau)) return rc;
   if (recompute) HIPCHK(hipStreamWaitEvent(b->stream, b->ev_fresh[0], 0));
   if (b->prof_record(3)) return QMCX_ERR_HIP;
   if (int rc = launch_gl<T>(b, true)) return rc;
@@ -746,16 +791,15 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
   const size_t W = nwalkers, n = s->n, N = s->N, K = delay_rank;
   // LDS check for the move kernel
   {
-    int nt, slices;
-    int rc = precision == QMCX_FP32 ? choose_shape<float>(s, 0, nt, slices) : choose_shape<double>(s, 0, nt, slices);
+    int nt;
+    int rc = choose_shape(s, nt);
     if (rc) {
       delete b;
       return rc;
     }
     const int chunk = delay_rank == 1 ? s->n : delay_rank;
-    const size_t lds = precision == QMCX_FP32
-                           ? move_lds<float>(s->N, s->nions, s->n, delay_rank, slices, s->npad, chunk).total
-                           : move_lds<double>(s->N, s->nions, s->n, delay_rank, slices, s->npad, chunk).total;
+    const size_t lds = precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, chunk).total
+                                              : move_lds<double>(s->N, s->nions, s->n, delay_rank, chunk).total;
     if (lds > 160 * 1024) {
       delete b;
       return fail(QMCX_ERR_ARG, "walker state (" + std::to_string(lds / 1024) +
@@ -792,6 +836,9 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
   rc |= b->alloc(b->L, W * N);
   rc |= b->alloc(b->logv, W);
   rc |= b->alloc(b->lu_ws, W * 2 * 2 * n * n);
+  rc |= b->alloc(p8, es * W * 2 * n * 5 * s->npad);
+  b->spo = p8;
+  rc |= b->alloc(b->j1p, W * 2 * n * 8);
   rc |= b->alloc(b->ld_fresh, W);
   rc |= b->alloc(b->ph_fresh, W);
   if (rc) {
@@ -800,6 +847,9 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
   }
   HIPCHK(hipStreamCreate(&b->stream));
   HIPCHK(hipStreamCreate(&b->aux));
+  HIPCHK(hipStreamCreate(&b->pre));
+  for (int d = 0; d < 2; ++d) HIPCHK(hipEventCreateWithFlags(&b->ev_pre[d], hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&b->ev_start, hipEventDisableTiming));
   for (int d = 0; d < 2; ++d) {
     HIPCHK(hipEventCreateWithFlags(&b->ev_done[d], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&b->ev_fresh[d], hipEventDisableTiming));
@@ -815,6 +865,7 @@ int qmcx_batch_destroy(qmcx_batch_t b) {
   if (!b) return 0;
   if (b->stream) hipStreamSynchronize(b->stream);
   if (b->aux) hipStreamSynchronize(b->aux);
+  if (b->pre) hipStreamSynchronize(b->pre);
   for (void* p : b->allocs) hipFree(p);
   if (b->trace) hipFree(b->trace);
   if (b->hist) hipFree(b->hist);
@@ -822,6 +873,10 @@ int qmcx_batch_destroy(qmcx_batch_t b) {
   if (b->flex_decide) hipFree(b->flex_decide);
   if (b->stream) hipStreamDestroy(b->stream);
   if (b->aux) hipStreamDestroy(b->aux);
+  if (b->pre) hipStreamDestroy(b->pre);
+  for (int d = 0; d < 2; ++d)
+    if (b->ev_pre[d]) hipEventDestroy(b->ev_pre[d]);
+  if (b->ev_start) hipEventDestroy(b->ev_start);
   for (int d = 0; d < 2; ++d) {
     if (b->ev_done[d]) hipEventDestroy(b->ev_done[d]);
     if (b->ev_fresh[d]) hipEventDestroy(b->ev_fresh[d]);
@@ -871,10 +926,8 @@ int qmcx_batch_set_flush(qmcx_batch_t b, int kind) {
 
 int qmcx_batch_set_tile(qmcx_batch_t b, int spline_slices) {
   if (!b || spline_slices < 0 || spline_slices > 16 || (spline_slices && 16 % spline_slices)) return fail(QMCX_ERR_ARG, "bad tile");
-  int nt, sl;
-  int rc = b->prec == QMCX_FP32 ? choose_shape<float>(b->sys, spline_slices, nt, sl)
-                                : choose_shape<double>(b->sys, spline_slices, nt, sl);
-  if (rc) return rc;
+  const int nq = b->sys->npad / (b->prec == QMCX_FP32 ? 4 : 2);
+  if (nq * spline_slices > 1024) return fail(QMCX_ERR_ARG, "spline tile does not fit a workgroup");
   b->slices = spline_slices;
   return 0;
 }
@@ -913,20 +966,29 @@ int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int 
     HIPCHK(hipMalloc(&b->hist, sizeof(double) * (size_t)nsweeps * b->W));
     b->hist_cap = nsweeps;
   }
-  // mark the first sweep of this run so evaluateGL writes its history from slot 0
-  {
-    int cur[2];
-    HIPCHK(hipStreamSynchronize(b->stream));
-    HIPCHK(hipMemcpy(cur, b->sweep, sizeof(cur), hipMemcpyDeviceToHost));
-    cur[1] = cur[0];
-    HIPCHK(hipMemcpy(b->sweep, cur, sizeof(cur), hipMemcpyHostToDevice));
-  }
+  // mark the first sweep of this run so evaluateGL writes its history from slot 0; cur[0] is
+  // the absolute number of this run's first sweep
+  int cur[2];
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(cur, b->sweep, sizeof(cur), hipMemcpyDeviceToHost));
+  cur[1] = cur[0];
+  HIPCHK(hipMemcpy(b->sweep, cur, sizeof(cur), hipMemcpyHostToDevice));
   const bool f32 = b->prec == QMCX_FP32;
   HIPCHK(hipEventRecord(b->ev0, b->stream));
+  if (nsweeps > 0) {
+    // the first sweep's proposals, after whatever the batch stream did before this run
+    HIPCHK(hipEventRecord(b->ev_start, b->stream));
+    for (int det = 0; det < 2; ++det) {
+      int rc = f32 ? issue_proposals<float>(b, det, b->ev_start, seed, cur[0], tau)
+                   : issue_proposals<double>(b, det, b->ev_start, seed, cur[0], tau);
+      if (rc) return rc;
+    }
+  }
   for (int sw = 0; sw < nsweeps; ++sw) {
     const bool rec = recompute_every > 0 && (sw + 1) % recompute_every == 0;
-    int rc = f32 ? sweep_impl<float>(b, seed, tau, accept_mode, rec, sw)
-                 : sweep_impl<double>(b, seed, tau, accept_mode, rec, sw);
+    const bool last = sw + 1 == nsweeps;
+    int rc = f32 ? sweep_impl<float>(b, seed, tau, accept_mode, rec, sw, cur[0] + sw, last)
+                 : sweep_impl<double>(b, seed, tau, accept_mode, rec, sw, cur[0] + sw, last);
     if (rc) return rc;
     if (b->prof_on && b->prof_used > 4096 && b->prof_collect()) return QMCX_ERR_HIP;
     hipLaunchKernelGGL(k_next_sweep, dim3(1), dim3(64), 0, b->stream, b->sweep);
@@ -979,35 +1041,40 @@ int qmcx_get_inverse(qmcx_batch_t b, int walker, int det, double* psiM) {
 static int flex_launch(qmcx_batch_t b, int iel, bool apply) {
   const qmcx_system_s* s = b->sys;
   const bool f32 = b->prec == QMCX_FP32;
-  int nt = 0, slices = 0;
-  int rc = f32 ? choose_shape<float>(s, b->slices, nt, slices) : choose_shape<double>(s, b->slices, nt, slices);
+  int nt = 0;
+  int rc = choose_shape(s, nt);
+  if (rc) return rc;
+  const int det = iel < s->n ? 0 : 1;
+  rc = f32 ? launch_proposals<float>(b, b->stream, det, iel - det * s->n, 1, 0, 0, 1.0, b->flex_displ)
+           : launch_proposals<double>(b, b->stream, det, iel - det * s->n, 1, 0, 0, 1.0, b->flex_displ);
   if (rc) return rc;
   MoveArgs a;
   std::memset(&a, 0, sizeof(a));
+  a.spo = b->spo;
+  a.j1p = b->j1p;
   a.det = iel < s->n ? 0 : 1;
   a.e_begin = iel;
   a.e_end = iel + 1;
   a.mode = QMCX_ACCEPT_METROPOLIS;
-  a.slices = slices;
+  a.slices = 0;
   a.walker0 = b->walker0;
   a.displ = b->flex_displ;
   a.decide = b->flex_decide;
   const bool trace_was = b->trace_on;
   b->trace_on = true;
   if (f32) {
-    const MoveLds L = move_lds<float>(s->N, s->nions, s->n, b->K, slices, s->npad, 1);
-    rc = launch_moves<float>(b, nt, slices, L.total, a);
+    const MoveLds L = move_lds<float>(s->N, s->nions, s->n, b->K, 1);
+    rc = launch_moves<float>(b, nt, L.total, a);
   } else {
-    const MoveLds L = move_lds<double>(s->N, s->nions, s->n, b->K, slices, s->npad, 1);
-    rc = launch_moves<double>(b, nt, slices, L.total, a);
+    const MoveLds L = move_lds<double>(s->N, s->nions, s->n, b->K, 1);
+    rc = launch_moves<double>(b, nt, L.total, a);
   }
   b->trace_on = trace_was;
   if (rc) return rc;
   if (apply && b->K > 1) {
     // every flex move is its own chunk: its (at most one) delayed row is applied now
     if (f32 && use_mfma_flush<float>(b)) {
-      const int cp = (b->K + 15) & ~15;
-      const size_t lds = sizeof(float) * 2 * kFlushBM * cp + sizeof(int) * b->K;
+      const size_t lds = flush_mfma_lds(s->n, b->K);
       hipLaunchKernelGGL(k_flush_mfma, dim3((s->n + kFlushBM - 1) / kFlushBM, b->W), dim3(256), lds, b->stream,
                          s->params<float>(), b->params<float>(), a.det);
     } else if (f32) {
@@ -1079,7 +1146,7 @@ int qmcx_flex_evaluate_gl(qmcx_batch_t b, double* log_out) {
 int qmcx_set_kernel_timing(qmcx_batch_t b, int enable) {
   if (!b) return fail(QMCX_ERR_ARG, "null batch");
   b->prof_on = enable != 0;
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < 5; ++i) {
     b->prof_ms[i] = 0;
     b->prof_n[i] = 0;
   }
@@ -1087,11 +1154,11 @@ int qmcx_set_kernel_timing(qmcx_batch_t b, int enable) {
   return 0;
 }
 
-int qmcx_get_kernel_timing(qmcx_batch_t b, double* ms4, long long* launches4) {
-  if (!b || !ms4 || !launches4) return fail(QMCX_ERR_ARG, "null argument");
-  for (int i = 0; i < 4; ++i) {
-    ms4[i] = b->prof_ms[i];
-    launches4[i] = b->prof_n[i];
+int qmcx_get_kernel_timing(qmcx_batch_t b, double* ms5, long long* launches5) {
+  if (!b || !ms5 || !launches5) return fail(QMCX_ERR_ARG, "null argument");
+  for (int i = 0; i < 5; ++i) {
+    ms5[i] = b->prof_ms[i];
+    launches5[i] = b->prof_n[i];
   }
   return 0;
 }
@@ -1165,8 +1232,7 @@ template <typename T>
 int du_flush(qmcx_du_t h) {
   if (h->count == 0) return 0;
   if (sizeof(T) == 4 && h->n % 16 == 0) {
-    const int cp = (h->K + 15) & ~15;
-    const size_t lds = sizeof(float) * 2 * kFlushBM * cp + sizeof(int) * h->K;
+    const size_t lds = flush_mfma_lds(h->n, h->K);
     hipLaunchKernelGGL(k_flush_mfma, dim3((h->n + kFlushBM - 1) / kFlushBM, 1), dim3(256), lds, h->stream,
                        h->sys<float>(), h->params<float>(), 0);
   } else {

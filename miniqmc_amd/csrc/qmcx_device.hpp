@@ -176,18 +176,22 @@ __device__ inline T min_image(const Cell<T>& C, T d0, T d1, T d2, T flip, T& ox,
   const T ex = d0 + s0 * C.rb[0] + s1 * C.rb[3] + s2 * C.rb[6];
   const T ey = d1 + s0 * C.rb[1] + s1 * C.rb[4] + s2 * C.rb[7];
   const T ez = d2 + s0 * C.rb[2] + s1 * C.rb[5] + s2 * C.rb[8];
-  T best = ex * ex + ey * ey + ez * ez;
-  int ic = 0;
+  // corner 0 is the origin; the winner's coordinates are carried by selects (no indexing by
+  // the winning corner, so a cell held in registers stays there)
+  T best = ex * ex + ey * ey + ez * ez, bx = ex + C.corner[0], by = ey + C.corner[1], bz = ez + C.corner[2];
 #pragma unroll
   for (int c = 1; c < 8; ++c) {
     const T x = ex + C.corner[3 * c], y = ey + C.corner[3 * c + 1], z = ez + C.corner[3 * c + 2];
     const T q = x * x + y * y + z * z;
-    ic = q < best ? c : ic;
-    best = q < best ? q : best;
+    const bool lt = q < best;
+    bx = lt ? x : bx;
+    by = lt ? y : by;
+    bz = lt ? z : bz;
+    best = lt ? q : best;
   }
-  ox = flip * (ex + C.corner[3 * ic]);
-  oy = flip * (ey + C.corner[3 * ic + 1]);
-  oz = flip * (ez + C.corner[3 * ic + 2]);
+  ox = flip * bx;
+  oy = flip * by;
+  oz = flip * bz;
   return sqrt(best);
 }
 

@@ -59,6 +59,8 @@ struct MoveArgs {
   uint64_t seed;
   const double* displ;   // [W][3] caller proposals (flex_makeMove) or null: Philox stream
   const int* decide;     // [W] caller accept decisions (flex_acceptrestoreMove) or null
+  const void* spo;       // [W][2][n][5][npad] T: SPO rows at the proposals (k_spo_proposals)
+  const double* j1p;     // [W][2][n][8]: J1 at the proposals: sum u, sum (d2u + 2 du/r), gradient
 };
 
 // ------------------------------------------------------------------ spline gather
@@ -86,8 +88,8 @@ __device__ inline void spline_gather(const SysP<T>& S, const double* spoG, doubl
   using VT = typename V16<T>::type;
   constexpr int VW = V16<T>::n;
   constexpr int SL = 16 / PER;
-  // loads kept in flight per batch: all of them in FP32, two stencil columns in FP64
-  constexpr int G = (sizeof(T) == 8 && PER > 2) ? 2 : PER;
+  // stencil columns with loads in flight per batch: up to four in FP32, two in FP64
+  constexpr int G = sizeof(T) == 8 ? (PER > 2 ? 2 : PER) : (PER > 4 ? 4 : PER);
   const int nq = S.npad / VW;
   const int tid = threadIdx.x;
   const int q = tid % nq, s = tid / nq;
@@ -253,18 +255,31 @@ __global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, i
 // walks columns [g*n/4, (g+1)*n/4) with float4 loads, four MFMAs per load, which
 // sums the same products in another order.
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
-constexpr int kFlushBM = 32;
+constexpr int kFlushBM = 64;        // rows per workgroup: 4 waves x 16 rows
+constexpr int kFlushMaxCT = 8;      // delayed rows up to 128 = 8 tiles of 16
+// LDS of k_flush_mfma: per wave T and T2 (16 x cp floats each), the row list
+__host__ __device__ inline size_t flush_mfma_lds(int n, int K) {
+  const int cp = (K + 15) & ~15;
+  (void)n;
+  return sizeof(float) * 4 * 2 * 16 * cp + sizeof(int) * K;
+}
+// Each wave owns 16 rows of Ainv: T = Ainv_rows U^T - E on the matrix cores with
+// the rows read from HBM once, T2 = -T Binv in FP64, then Ainv_rows += T2 V with
+// the accumulators started from the rows again (now in L2).  U (the new Slater
+// rows) and V (the stale inverse rows) are shared by all waves of the walker.
 __global__ __launch_bounds__(256) void k_flush_mfma(SysP<float> S, BatchP<float> B, int det) {
-  const int w = blockIdx.y, m0 = blockIdx.x * kFlushBM;
+  const int w = blockIdx.y;
   const int c = B.ndelay[w];
   if (c == 0) return;
   const int n = S.n, K = B.K, tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int cp = (c + 15) & ~15;           // delayed rows padded to a 16-wide tile
+  const int nct = cp / 16;
+  const int r0 = blockIdx.x * kFlushBM + wid * 16;   // this wave's first row
   extern __shared__ __align__(16) unsigned char smem[];
-  float* sT = (float*)smem;                // [BM][cp]   T = Ainv U^T - E
-  float* sT2 = sT + kFlushBM * cp;         // [BM][cp]  -T Binv
-  int* sRows = (int*)(sT2 + kFlushBM * cp);
+  float* sT = (float*)smem + wid * 2 * 16 * cp;      // [16][cp]   T = Ainv U^T - E
+  float* sT2 = sT + 16 * cp;                          // [16][cp]  -T Binv
+  int* sRows = (int*)((float*)smem + 4 * 2 * 16 * cp);
   const size_t moff = ((size_t)w * 2 + det) * n * n;
   float* Ainv = B.Ainv + moff;
   const float* Amat = B.A + moff;
@@ -274,39 +289,52 @@ __global__ __launch_bounds__(256) void k_flush_mfma(SysP<float> S, BatchP<float>
   __syncthreads();
   const int li = lane & 15, g = lane >> 4;
   const int q4 = n / 4;                    // columns per lane group (n % 16 == 0)
-  // ---- GEMM 1: T[32][cp] = Ainv[m0:m0+32][:] . U^T
-  const int ntile1 = 2 * (cp / 16);
-  for (int t = wid; t < ntile1; t += 4) {
-    const int rt = t & 1, ct = t >> 1;
-    const int arow = m0 + rt * 16 + li;
-    const bool aok = arow < n;               // the last block may run past row n-1
-    const int d = ct * 16 + li;
-    const bool dok = d < c;
+  // ---- GEMM 1: T[16][cp] = Ainv[r0:r0+16][:] . U^T, every column tile from one read of the rows
+  {
+    const int arow = r0 + li;
+    const bool aok = arow < n;             // the last block may run past row n-1
     const float* ap = Ainv + (size_t)(aok ? arow : 0) * n + g * q4;
-    const float* bp = Amat + (size_t)(dok ? sRows[d] : 0) * n + g * q4;
-    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+    const float* bp[kFlushMaxCT];
+#pragma unroll
+    for (int ct = 0; ct < kFlushMaxCT; ++ct) {
+      const int d = ct * 16 + li;
+      bp[ct] = Amat + (size_t)(d < c ? sRows[d] : 0) * n + g * q4;
+    }
+    f32x4_t acc[kFlushMaxCT];
+#pragma unroll
+    for (int ct = 0; ct < kFlushMaxCT; ++ct) acc[ct] = f32x4_t{0.f, 0.f, 0.f, 0.f};
     for (int k = 0; k < q4; k += 4) {
       f32x4_t av = *reinterpret_cast<const f32x4_t*>(ap + k);
-      f32x4_t bv = *reinterpret_cast<const f32x4_t*>(bp + k);
       if (!aok) av = f32x4_t{0.f, 0.f, 0.f, 0.f};
-      if (!dok) bv = f32x4_t{0.f, 0.f, 0.f, 0.f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[0], bv[0], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[1], bv[1], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[2], bv[2], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[3], bv[3], acc, 0, 0, 0);
+#pragma unroll
+      for (int ct = 0; ct < kFlushMaxCT; ++ct) {
+        if (ct < nct) {
+          f32x4_t bv = *reinterpret_cast<const f32x4_t*>(bp[ct] + k);
+          if (ct * 16 + li >= c) bv = f32x4_t{0.f, 0.f, 0.f, 0.f};
+          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[0], bv[0], acc[ct], 0, 0, 0);
+          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[1], bv[1], acc[ct], 0, 0, 0);
+          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[2], bv[2], acc[ct], 0, 0, 0);
+          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[3], bv[3], acc[ct], 0, 0, 0);
+        }
+      }
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = rt * 16 + 4 * g + r;      // row within the block
-      const int dd = ct * 16 + li;            // delayed-row index
-      float v = acc[r];
-      if (dd < c && sRows[dd] == m0 + i) v -= 1.f;
-      sT[i * cp + dd] = dd < c ? v : 0.f;
+    for (int ct = 0; ct < kFlushMaxCT; ++ct) {
+      if (ct < nct) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 4 * g + r;             // row within the wave's 16
+          const int dd = ct * 16 + li;         // delayed-row index
+          float v = acc[ct][r];
+          if (dd < c && sRows[dd] == r0 + i) v -= 1.f;
+          sT[i * cp + dd] = dd < c ? v : 0.f;
+        }
+      }
     }
   }
   __syncthreads();
   // ---- T2 = -(T Binv), FP64 products of the small capacitance inverse
-  for (int idx = tid; idx < kFlushBM * cp; idx += 256) {
+  for (int idx = lane; idx < 16 * cp; idx += 64) {
     const int i = idx / cp, dd = idx - i * cp;
     double sum = 0;
     if (dd < c)
@@ -314,26 +342,24 @@ __global__ __launch_bounds__(256) void k_flush_mfma(SysP<float> S, BatchP<float>
     sT2[i * cp + dd] = (float)(-sum);
   }
   __syncthreads();
-  // ---- GEMM 2: Ainv[m0:m0+32][:] += T2 . V
-  const int ntile2 = 2 * (n / 16);
-  for (int t = wid; t < ntile2; t += 4) {
-    const int rt = t & 1, ct = t >> 1;
+  // ---- GEMM 2: Ainv[r0:r0+16][:] += T2 . V
+  for (int ct = 0; ct < n / 16; ++ct) {
     const int col = ct * 16 + li;
     f32x4_t acc;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = m0 + rt * 16 + 4 * g + r;
+      const int row = r0 + 4 * g + r;
       acc[r] = row < n ? Ainv[(size_t)row * n + col] : 0.f;
     }
     for (int k0 = 0; k0 < cp; k0 += 4) {
       const int kk = k0 + g;
-      const float av = sT2[(rt * 16 + li) * cp + kk];
+      const float av = sT2[li * cp + kk];
       const float bv = kk < c ? Vb[(size_t)kk * n + col] : 0.f;
       acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = m0 + rt * 16 + 4 * g + r;
+      const int row = r0 + 4 * g + r;
       if (row < n) Ainv[(size_t)row * n + col] = acc[r];
     }
   }
@@ -422,7 +448,8 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = e0 + u * nt;
-        v[u] = e < sb * n ? (double)a[(size_t)(s0 + e / n) * n + e % n] : 0.0;
+        const int ec = e < sb * n ? e : 0;   // clamped: the loads stay unconditional
+        v[u] = (double)a[(size_t)(s0 + ec / n) * n + ec % n];
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -447,7 +474,9 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
     const bool own = tid < n;
     double pr[B];
 #pragma unroll
-    for (int c = 0; c < B; ++c) pr[c] = (own && c < bb) ? X[(size_t)tid * n + k0 + c] : 0.0;
+    for (int c = 0; c < B; ++c) pr[c] = X[(size_t)(own ? tid : 0) * n + k0 + (c < bb ? c : 0)];
+#pragma unroll
+    for (int c = 0; c < B; ++c) pr[c] = (own && c < bb) ? pr[c] : 0.0;
     double* swp = fcol;   // old pivot-position row, for the thread that takes it
     // the step loop stays rolled: register entries are picked by compare-and-select on j
 #pragma unroll 1
@@ -551,9 +580,9 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
         for (int u = 0; u < 8; ++u) {
           const int e = e0 + u * nt;
           const int t = e / cw, col = c0 + e % cw;
-          v[u] = (e < (bb + nd) * cw && !(col >= k0 && col < k0 + bb))
-                     ? X[(size_t)src[t < bb ? k0 + t : dl[t - bb]] * n + col]
-                     : 0.0;
+          const bool ok = e < (bb + nd) * cw;
+          const int srow = ok ? src[t < bb ? k0 + t : dl[t - bb]] : 0;
+          v[u] = X[(size_t)srow * n + (ok ? col : 0)];   // unconditional; panel columns never read back
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -577,10 +606,20 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const int i = i0 + r, cc = cc0 + c, col = c0 + cc;
-            double v = 0.0;
-            if (i < n && cc < cw && !(col >= k0 && col < k0 + bb) && !(i >= k0 && i < k0 + bb))
-              v = slot[i] >= 0 ? XA[slot[i] * kGJChunk + cc] : X[(size_t)i * n + col];
-            acc[r][c] = v;
+            const bool in = i < n && cc < cw;
+            const int ic = in ? i : 0, colc = in ? col : 0;
+            acc[r][c] = X[(size_t)ic * n + colc];   // unconditional load, selected below
+          }
+#pragma unroll
+        for (int r = 0; r < TR; ++r)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int i = i0 + r, cc = cc0 + c, col = c0 + cc;
+            const bool in = i < n && cc < cw;
+            const bool zero = !in || (col >= k0 && col < k0 + bb) || (i >= k0 && i < k0 + bb);
+            const int sl = in ? slot[i] : -1;
+            const double xa = XA[(sl >= 0 ? sl : 0) * kGJChunk + (in ? cc : 0)];
+            acc[r][c] = zero ? 0.0 : (sl >= 0 ? xa : acc[r][c]);
           }
         for (int t = 0; t < bb; ++t) {
           double pv[TR], xv[4];
@@ -627,7 +666,8 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = e0 + u * nt;
-        v[u] = e < sb * n ? X[(size_t)(s0 + e / n) * n + e % n] : 0.0;
+        const int ec = e < sb * n ? e : 0;
+        v[u] = X[(size_t)(s0 + ec / n) * n + ec % n];
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -692,6 +732,76 @@ __global__ __launch_bounds__(1024) void k_spo_rows(SysP<T> S, BatchP<T> B, int s
     dp[n + j] = out[2 * n + j];
     dp[2 * n + j] = out[3 * n + j];
     d2[j] = out[4 * n + j];
+  }
+}
+
+// SPO values (v, d/dx, d/dy, d/dz, d2/dx2; einspline_spo.hpp:204-223) at the proposals of
+// electrons det*n + [e0, e0 + gridDim.x) of every walker, into spo[w][det][e][5][npad].  Within
+// a sweep an electron's position changes only at its own move, so the sweep's proposals
+// r_e + displacement are fixed before the first of them is accepted; evaluating them here takes
+// the spline's HBM traffic out of the move kernel's accept chain.  The displacement is the
+// move kernel's: sqrt(tau) times the walker's Philox normals of (sweep, electron), or the
+// caller's (displ[w][3]).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_spo_proposals(SysP<T> S, BatchP<T> B, int det, int e0, uint64_t seed,
+                                                         int sweep, double sqrt_tau, int walker0, const double* displ,
+                                                         T* spo, double* j1p, int slices) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* part = (T*)smem;
+  const int el = e0 + blockIdx.x, w = blockIdx.y, N = S.N, n = S.n;
+  const int iel = det * n + el;
+  const double* R = B.R + (size_t)w * 3 * N;
+  double d[3];
+  if (displ != nullptr) {
+    d[0] = displ[3 * w + 0];
+    d[1] = displ[3 * w + 1];
+    d[2] = displ[3 * w + 2];
+  } else {
+    double nr[3], u;
+    move_randoms(seed, (uint64_t)(walker0 + w), (uint64_t)sweep, (uint64_t)iel, nr, u);
+    d[0] = sqrt_tau * nr[0];
+    d[1] = sqrt_tau * nr[1];
+    d[2] = sqrt_tau * nr[2];
+  }
+  const double x = R[iel] + d[0], y = R[N + iel] + d[1], z = R[2 * N + iel] + d[2];
+  spline_gather_rt<T, 5>(S, x, y, z, slices, part);
+  __syncthreads();
+  T* out = spo + (((size_t)w * 2 + det) * n + el) * 5 * S.npad;
+  spline_finish<T, 5, T>(S, slices, part, out, S.npad, n);
+  __syncthreads();
+  // J1 at the proposal (OneBodyJastrow::ratioGrad, OneBodyJastrow.h:259-268): it depends on the
+  // proposal and the ions only, so it is evaluated here with the SPO row
+  const Cell<T> C = *S.cell;
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int ci = threadIdx.x; ci < S.M; ci += blockDim.x) {
+    const Functor<T>& f = S.fn[S.species[ci]];
+    if (f.ncoef > 0) {
+      T dx, dy, dz;
+      const T r = min_image(C, (T)(S.ions[ci] - x), (T)(S.ions[S.M + ci] - y), (T)(S.ions[2 * S.M + ci] - z), T(-1),
+                            dx, dy, dz);
+      if (r < f.rcut) {
+        T u, du, d2u;
+        functor_vgl(f, r, u, du, d2u);
+        acc[0] += (double)u;
+        acc[1] += (double)(d2u + T(2) * du);
+        acc[2] += (double)(du * dx);
+        acc[3] += (double)(du * dy);
+        acc[4] += (double)(du * dz);
+      }
+    }
+  }
+  double* red = (double*)smem;   // [waves][5]
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) acc[k] = wave_sum(acc[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) red[wid * 5 + k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    double t = 0;
+    for (int q = 0; q < nw; ++q) t += red[q * 5 + threadIdx.x];
+    j1p[(((size_t)w * 2 + det) * n + el) * 8 + threadIdx.x] = t;
   }
 }
 

@@ -7,14 +7,16 @@
 // inverse rows of the chunk's electrons (Ainv is not modified until the flush).
 // The products U_d . Ainv[row'] of an accepted row with the chunk's later rows
 // are formed at accept time, so getInvRow (DelayedUpdate.h:96-114) reads LDS
-// only.  The spline stencil loads of a move are issued first and consumed after
-// the first barrier; barriers inside a move wait for LDS traffic only
-// (lds_sync), so those loads stay in flight across it.
+// only.  The SPO values at the move's proposal come precomputed from
+// k_spo_proposals (a sweep's proposals do not depend on its accept/reject
+// chain); the move loads its 5 x n row first, without branches, and the
+// barriers inside a move wait for LDS traffic only (lds_sync), so those loads
+// stay in flight across them.
 #pragma once
 
 namespace qmcx {
 
-constexpr int kRed = 17;   // reduction slots: gnow(3) ratio(1) gnew(3) J2(5) J1(5)
+constexpr int kRed = 12;   // reduction slots: gnow(3) ratio(1) gnew(3) J2(5)
 constexpr int kMoveThreads = 512;
 
 struct MoveLds {
@@ -27,7 +29,7 @@ __host__ __device__ inline unsigned lds_take(unsigned& o, size_t bytes) {
 }
 // E = electrons per chunk; the inverse-row cache is used when K > 1
 template <typename T>
-__host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int S, int npad, int E) {
+__host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E) {
   MoveLds L;
   unsigned o = 0;
   const bool cache = K > 1;
@@ -39,8 +41,7 @@ __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int S, i
   L.oPsi = lds_take(o, sizeof(T) * 5 * n);
   L.oBinv = lds_take(o, cache ? sizeof(double) * K * K : 0);
   L.oP = lds_take(o, sizeof(double) * (K + 4));
-  // the spline partials; reused by the Sherman-Morrison pass (n doubles)
-  L.oPart = lds_take(o, sizeof(T) * S * 5 * npad > sizeof(double) * n ? sizeof(T) * S * 5 * npad : sizeof(double) * n);
+  L.oPart = lds_take(o, cache ? 0 : sizeof(double) * n);   // Sherman-Morrison scratch
   L.oRows = lds_take(o, sizeof(int) * K);
   L.oSpec = lds_take(o, sizeof(int) * M);
   L.oFn = lds_take(o, sizeof(Functor<T>) * (kMaxSpecies + 2));
@@ -53,142 +54,20 @@ __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int S, i
   return L;
 }
 
-// ------------------------------------------------------------------ spline, split into issue and compute
-template <typename T, int PER>
-struct StencilLoads {
-  typename V16<T>::type v[PER][4];
-  T tx, ty, tz;
-};
-
-// lane (q, s): q = 16-byte orbital group, s = slice owning PER of the 16 (i,j) stencil columns
-template <typename T, int PER>
-__device__ inline void stencil_issue(const SysP<T>& S, const double* spoG, double x, double y, double z, int q, int s,
-                                     StencilLoads<T, PER>& L) {
-  using VT = typename V16<T>::type;
-  constexpr int VW = V16<T>::n;
-  int ix[3];
-  double tt[3];
-  spline_locate(spoG, S.ng, x, y, z, ix, tt);
-  const T* base0 = S.coefs + (size_t)ix[0] * S.xs + (size_t)ix[1] * S.ys + (size_t)ix[2] * S.zs + (size_t)q * VW;
-  const size_t zs = S.zs / VW;
-#pragma unroll
-  for (int pp = 0; pp < PER; ++pp) {
-    const int p = s * PER + pp;
-    const VT* src = reinterpret_cast<const VT*>(base0 + (size_t)(p >> 2) * S.xs + (size_t)(p & 3) * S.ys);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) L.v[pp][k] = __builtin_nontemporal_load(src + k * zs);
-  }
-  L.tx = (T)tt[0];
-  L.ty = (T)tt[1];
-  L.tz = (T)tt[2];
-}
-
-// FP64 variant: loads and sums in batches of G columns inside one phase
-template <typename T, int PER, int G>
-__device__ inline void stencil_direct(const SysP<T>& S, const double* spoG, double x, double y, double z, int q, int s,
-                                      T* part) {
-  using VT = typename V16<T>::type;
-  constexpr int VW = V16<T>::n;
-  int ix[3];
-  double tt[3];
-  spline_locate(spoG, S.ng, x, y, z, ix, tt);
-  const T* base0 = S.coefs + (size_t)ix[0] * S.xs + (size_t)ix[1] * S.ys + (size_t)ix[2] * S.zs + (size_t)q * VW;
-  const size_t zs = S.zs / VW;
-  T c[4], dc[4], d2c[4];
-  bspline_weights<T>((T)tt[2], c, dc, d2c);
-  const T tx = (T)tt[0], ty = (T)tt[1];
-  T acc[5][VW];
-#pragma unroll
-  for (int o = 0; o < 5; ++o)
-#pragma unroll
-    for (int e = 0; e < VW; ++e) acc[o][e] = 0;
-#pragma unroll
-  for (int g0 = 0; g0 < PER; g0 += G) {
-    VT v[G][4];
-#pragma unroll
-    for (int pp = 0; pp < G; ++pp) {
-      const int p = s * PER + g0 + pp;
-      const VT* src = reinterpret_cast<const VT*>(base0 + (size_t)(p >> 2) * S.xs + (size_t)(p & 3) * S.ys);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) v[pp][k] = __builtin_nontemporal_load(src + k * zs);
-    }
-#pragma unroll
-    for (int pp = 0; pp < G; ++pp) {
-      const int p = s * PER + g0 + pp;
-      T ai, dai, d2ai, bj, dbj, d2bj;
-      knot_weight<T>(p >> 2, tx, ai, dai, d2ai);
-      knot_weight<T>(p & 3, ty, bj, dbj, d2bj);
-      const T w00 = ai * bj, w10 = dai * bj, w01 = ai * dbj, w20 = d2ai * bj;
-#pragma unroll
-      for (int e = 0; e < VW; ++e) {
-        const T q0 = v[pp][0][e], q1 = v[pp][1][e], q2 = v[pp][2][e], q3 = v[pp][3][e];
-        const T s0 = c[0] * q0 + c[1] * q1 + c[2] * q2 + c[3] * q3;
-        const T s1 = dc[0] * q0 + dc[1] * q1 + dc[2] * q2 + dc[3] * q3;
-        acc[0][e] += w00 * s0;
-        acc[1][e] += w10 * s0;
-        acc[2][e] += w01 * s0;
-        acc[3][e] += w00 * s1;
-        acc[4][e] += w20 * s0;
-      }
-    }
-  }
-#pragma unroll
-  for (int o = 0; o < 5; ++o)
-#pragma unroll
-    for (int e = 0; e < VW; ++e) part[((size_t)s * 5 + o) * S.npad + q * VW + e] = acc[o][e];
-}
-
-// v, gx, gy, gz, hxx partial sums of this lane's columns -> part[s][5][npad]
-template <typename T, int PER>
-__device__ inline void stencil_compute(const SysP<T>& S, const StencilLoads<T, PER>& L, int q, int s, T* part) {
-  constexpr int VW = V16<T>::n;
-  T c[4], dc[4], d2c[4];
-  bspline_weights<T>(L.tz, c, dc, d2c);
-  T acc[5][VW];
-#pragma unroll
-  for (int o = 0; o < 5; ++o)
-#pragma unroll
-    for (int e = 0; e < VW; ++e) acc[o][e] = 0;
-#pragma unroll
-  for (int pp = 0; pp < PER; ++pp) {
-    const int p = s * PER + pp;
-    T ai, dai, d2ai, bj, dbj, d2bj;
-    knot_weight<T>(p >> 2, L.tx, ai, dai, d2ai);
-    knot_weight<T>(p & 3, L.ty, bj, dbj, d2bj);
-    const T w00 = ai * bj, w10 = dai * bj, w01 = ai * dbj, w20 = d2ai * bj;
-#pragma unroll
-    for (int e = 0; e < VW; ++e) {
-      const T q0 = L.v[pp][0][e], q1 = L.v[pp][1][e], q2 = L.v[pp][2][e], q3 = L.v[pp][3][e];
-      const T s0 = c[0] * q0 + c[1] * q1 + c[2] * q2 + c[3] * q3;
-      const T s1 = dc[0] * q0 + dc[1] * q1 + dc[2] * q2 + dc[3] * q3;
-      acc[0][e] += w00 * s0;
-      acc[1][e] += w10 * s0;
-      acc[2][e] += w01 * s0;
-      acc[3][e] += w00 * s1;
-      acc[4][e] += w20 * s0;
-    }
-  }
-#pragma unroll
-  for (int o = 0; o < 5; ++o)
-#pragma unroll
-    for (int e = 0; e < VW; ++e) part[((size_t)s * 5 + o) * S.npad + q * VW + e] = acc[o][e];
-}
-
 // Sum the wave partials red[v*16 + w] (w < nw); every lane of every wave gets
 // the same totals, in the same order.
 __device__ inline void combine_partials(const double* red, int nw, double tot[kRed]) {
-  const int lane = threadIdx.x & 63, q = lane & 15;
+  // lane v < kRed sums slot v over the waves in order; the totals are broadcast as uniform values
+  const int lane = threadIdx.x & 63;
+  const int v = lane < kRed ? lane : 0;
+  double x = 0.0;
+  for (int q = 0; q < nw; ++q) x += red[v * 16 + q];
+  const long long bits = __double_as_longlong(x);
+  const int lo = (int)(bits & 0xffffffffLL), hi = (int)(bits >> 32);
 #pragma unroll
-  for (int base = 0; base < kRed; base += 4) {
-    const int v = base + (lane >> 4);
-    double x = (v < kRed && q < nw) ? red[v * 16 + q] : 0.0;
-    x += __shfl_xor(x, 8, 64);
-    x += __shfl_xor(x, 4, 64);
-    x += __shfl_xor(x, 2, 64);
-    x += __shfl_xor(x, 1, 64);
-#pragma unroll
-    for (int g = 0; g < 4; ++g)
-      if (base + g < kRed) tot[base + g] = __shfl(x, g * 16, 64);
+  for (int i = 0; i < kRed; ++i) {
+    const unsigned l = (unsigned)__builtin_amdgcn_readlane(lo, i), h = (unsigned)__builtin_amdgcn_readlane(hi, i);
+    tot[i] = __longlong_as_double((long long)(((unsigned long long)h << 32) | l));
   }
 }
 
@@ -216,20 +95,16 @@ __device__ unsigned long long qmcx_phase_ticks[12];
 #define QMCX_TICK(slot)
 #endif
 
-template <typename T, int JPT, int PER>
+// KD: delayed updates (K > 1: inverse-row cache, Woodbury bordering) or Sherman-Morrison (K = 1)
+template <typename T, int JPT, bool KD>
 __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs a) {
-  constexpr int VW = V16<T>::n;
-  constexpr int SL = 16 / PER;
-  // FP32 keeps the stencil loads in flight across the first barrier; FP64 (twice the
-  // registers per load) loads and sums inside phase 2
-  constexpr bool kEarly = sizeof(T) == 4;
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
-  const int N = S.N, n = S.n, M = S.M, K = B.K;
+  const int N = S.N, n = S.n, M = S.M, K = KD ? B.K : 1;
   const int E = a.e_end - a.e_begin;
   const int r0 = a.e_begin - a.det * n;
-  const MoveLds L = move_lds<T>(N, M, n, K, SL, S.npad, E);
+  const MoveLds L = move_lds<T>(N, M, n, K, E);
   double* sR = (double*)(smem + L.oR);
   double* sIon = (double*)(smem + L.oIon);
   double* sRng = (double*)(smem + L.oRng);
@@ -238,7 +113,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   T* sPsi = (T*)(smem + L.oPsi);
   double* sBinv = (double*)(smem + L.oBinv);
   double* sP = (double*)(smem + L.oP);
-  T* sPart = (T*)(smem + L.oPart);
+  double* sTm = (double*)(smem + L.oPart);
   int* sRows = (int*)(smem + L.oRows);
   int* sSpec = (int*)(smem + L.oSpec);
   Functor<T>* sFn = (Functor<T>*)(smem + L.oFn);
@@ -259,6 +134,8 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   T* J1 = B.J1 + (size_t)w * 5 * N;
   T* Vb = B.Vbuf + (size_t)w * K * n;
   const int sweep = B.sweep[0];
+  // precomputed SPO rows of this chunk's proposals: [e][5][npad]
+  const T* spo = (const T*)a.spo + (((size_t)w * 2 + a.det) * n + r0) * 5 * S.npad;
 
   // ---- chunk set-up: walker state into LDS
   for (int i = tid; i < 3 * N; i += nt) sR[i] = B.R[(size_t)w * 3 * N + i];
@@ -271,68 +148,118 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   }
   if (tid == 0) *sCell = *S.cell;
   if (tid < 9) sSpoG[tid] = S.spoG[tid];
-  if (K > 1)
+  if (KD)
     for (size_t i = tid; i < (size_t)E * n; i += nt) sAc[i] = Ainv[(size_t)r0 * n + i];
   for (int e = tid; e < E; e += nt) {
     double nr[3], u;
     move_randoms(a.seed, (uint64_t)(a.walker0 + w), (uint64_t)sweep, (uint64_t)(a.e_begin + e), nr, u);
-    sRng[e * 4 + 0] = nr[0];
-    sRng[e * 4 + 1] = nr[1];
-    sRng[e * 4 + 2] = nr[2];
-    sRng[e * 4 + 3] = u;
+    // the move's displacement and acceptance uniform; a caller's displacement and accept
+    // decision (the flex stage calls) take their place, the decision as -inf (accept) / +inf
+    const bool given = a.displ != nullptr;
+    sRng[e * 4 + 0] = given ? a.displ[3 * w + 0] : a.sqrt_tau * nr[0];
+    sRng[e * 4 + 1] = given ? a.displ[3 * w + 1] : a.sqrt_tau * nr[1];
+    sRng[e * 4 + 2] = given ? a.displ[3 * w + 2] : a.sqrt_tau * nr[2];
+    sRng[e * 4 + 3] = a.decide == nullptr ? u : (a.decide[w] != 0 ? -INFINITY : INFINITY);
   }
   int c = 0;
   double logd = 0, ph = 0, jl1 = 0, jl2 = 0;
   unsigned long long nacc = 0;
   __syncthreads();
-  const Cell<T>& C = *sCell;
-  const int nq = S.npad / VW;
-  const int sq = tid % nq, ss = tid / nq;
-  const bool spline_lane = ss < SL;
+  const Cell<T> C = *sCell;   // in registers for the minimum-image searches
+  // Rows a move reads from HBM that no earlier move of the chunk changes: the moving electron's
+  // J1 state, its dpsiM row and the SPO row at its proposal.  Move e+1's are loaded at the start
+  // of move e (software pipelining; index clamped, so the loads are unconditional).
+  const int jr = tid < n ? tid : n - 1;
+  T nJ1[4], ndp[3], np5[5];
+  double nj1p[5];
+  const double* j1pw = a.j1p + (((size_t)w * 2 + a.det) * n + r0) * 8;
+  auto prefetch = [&](int ee) {
+    const int ie = a.e_begin + ee;
+    nJ1[0] = J1[ie];
+    nJ1[1] = J1[N + ie];
+    nJ1[2] = J1[2 * N + ie];
+    nJ1[3] = J1[3 * N + ie];
+    const T* dq = dpsiM + (size_t)(r0 + ee) * 3 * n;
+    ndp[0] = dq[jr];
+    ndp[1] = dq[n + jr];
+    ndp[2] = dq[2 * n + jr];
+    const T* pq = spo + (size_t)ee * 5 * S.npad;
+#pragma unroll
+    for (int o = 0; o < 5; ++o) np5[o] = pq[(size_t)o * S.npad + jr];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) nj1p[k] = j1pw[(size_t)ee * 8 + k];
+  };
+  prefetch(0);
 
   for (int e = 0; e < E; ++e) {
     const int iel = a.e_begin + e, row = r0 + e;
     QMCX_TICK(0)
     // ---- phase 1: issue this move's global loads, then getInvRow's small algebra
     const double x0 = sR[iel], y0 = sR[N + iel], z0 = sR[2 * N + iel];
-    const bool given = a.displ != nullptr;
-    const double nx = x0 + (given ? a.displ[3 * w + 0] : a.sqrt_tau * sRng[e * 4 + 0]);
-    const double ny = y0 + (given ? a.displ[3 * w + 1] : a.sqrt_tau * sRng[e * 4 + 1]);
-    const double nz = z0 + (given ? a.displ[3 * w + 2] : a.sqrt_tau * sRng[e * 4 + 2]);
-    StencilLoads<T, kEarly ? PER : 1> st;
-    if constexpr (kEarly)
-      if (spline_lane) stencil_issue<T, PER>(S, sSpoG, nx, ny, nz, sq, ss, st);
-    // stored state of the moving electron (J1 in HBM, J2 in LDS), read before any update
-    const T Vat_old = J1[iel], g1x = J1[N + iel], g1y = J1[2 * N + iel], g1z = J1[3 * N + iel];
+    const double nx = x0 + sRng[e * 4 + 0];
+    const double ny = y0 + sRng[e * 4 + 1];
+    const double nz = z0 + sRng[e * 4 + 2];
+    // stored state of the moving electron (J1 in HBM, J2 in LDS), its dpsiM / inverse rows and
+    // the SPO row at its proposal, read before any update; unconditional loads (clamped
+    // index), so that no branch merge forces an early wait on them
+    const T Vat_old = nJ1[0], g1x = nJ1[1], g1y = nJ1[2], g1z = nJ1[3];
     const T Uat_old = sJ2[iel], g2x = sJ2[N + iel], g2y = sJ2[2 * N + iel], g2z = sJ2[3 * N + iel];
-    T dp0 = 0, dp1 = 0, dp2 = 0, inv_sm = 0;
-    if (tid < n) {
-      const T* dp = dpsiM + (size_t)row * 3 * n;
-      dp0 = dp[tid];
-      dp1 = dp[n + tid];
-      dp2 = dp[2 * n + tid];
-      if (K == 1) inv_sm = Ainv[(size_t)row * n + tid];
-    }
-    if (K > 1 && c > 0) {
+    const T dp0 = ndp[0], dp1 = ndp[1], dp2 = ndp[2];
+    T p5[5];
+#pragma unroll
+    for (int o = 0; o < 5; ++o) p5[o] = np5[o];
+    // J1 at the proposal: sum u, sum (d2u + 2 du/r), gradient (precomputed with the SPO row)
+    double j1v[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) j1v[k] = nj1p[k];
+    prefetch(e + 1 < E ? e + 1 : e);
+    T inv_sm = 0;
+    if constexpr (!KD) inv_sm = Ainv[(size_t)row * n + jr];   // changed by the previous accept
+    const T* prow = spo + (size_t)e * 5 * S.npad;
+    if (KD && c > 0) {
       // y = Binv^T p with p_d = U_d . Ainv[row] formed when row d was accepted; parked in row c of Binv
       for (int r = tid; r < c; r += nt) {
         double s = 0;
-        for (int q = 0; q < c; ++q) s += sBinv[q * K + r] * sQ[q * E + e];
+        int q = 0;
+        for (; q + 4 <= c; q += 4) {
+          const double b0 = sBinv[q * K + r], b1 = sBinv[(q + 1) * K + r], b2 = sBinv[(q + 2) * K + r],
+                       b3 = sBinv[(q + 3) * K + r];
+          const double p0 = sQ[q * E + e], p1 = sQ[(q + 1) * E + e], p2 = sQ[(q + 2) * E + e], p3 = sQ[(q + 3) * E + e];
+          s += b0 * p0;
+          s += b1 * p1;
+          s += b2 * p2;
+          s += b3 * p3;
+        }
+        for (; q < c; ++q) s += sBinv[q * K + r] * sQ[q * E + e];
         sBinv[c * K + r] = s;
       }
     }
     lds_sync();
     QMCX_TICK(1)
 
-    // ---- phase 2: invRow, evalGrad of the determinant, J1/J2 at the proposal, spline sums
-    double gd0 = 0, gd1 = 0, gd2 = 0;
+    // ---- phase 2: invRow, evalGrad of the determinant and ratioGrad_compute against the
+    // proposal's SPO row (einspline_spo.hpp:204-223, DiracDeterminant.cpp:103-124), J1/J2 at
+    // the proposal; all partial sums reduced in one round
+    double gd0 = 0, gd1 = 0, gd2 = 0, d0 = 0, d1 = 0, d2 = 0, d3 = 0;
     T vinv = 0;
     if (tid < n) {
       double v;
-      if (K > 1) {
+      if (KD) {
         v = (double)sAc[(size_t)e * n + tid];
         const double* y = sBinv + c * K;
-        for (int d = 0; d < c; ++d) v -= (double)sAc[(size_t)(sRows[d] - r0) * n + tid] * y[d];
+        // four delayed rows' LDS reads in flight at a time; subtraction order unchanged
+        int d = 0;
+        for (; d + 4 <= c; d += 4) {
+          const int e0 = sRows[d] - r0, e1 = sRows[d + 1] - r0, e2 = sRows[d + 2] - r0, e3 = sRows[d + 3] - r0;
+          const double y0 = y[d], y1 = y[d + 1], y2 = y[d + 2], y3 = y[d + 3];
+          const double a0 = (double)sAc[(size_t)e0 * n + tid], a1 = (double)sAc[(size_t)e1 * n + tid];
+          const double a2 = (double)sAc[(size_t)e2 * n + tid], a3 = (double)sAc[(size_t)e3 * n + tid];
+          v -= a0 * y0;
+          v -= a1 * y1;
+          v -= a2 * y2;
+          v -= a3 * y3;
+        }
+        for (; d < c; ++d) v -= (double)sAc[(size_t)(sRows[d] - r0) * n + tid] * y[d];
       } else {
         v = (double)inv_sm;
       }
@@ -341,20 +268,38 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       gd0 = (double)vinv * (double)dp0;
       gd1 = (double)vinv * (double)dp1;
       gd2 = (double)vinv * (double)dp2;
+#pragma unroll
+      for (int o = 0; o < 5; ++o) sPsi[o * n + tid] = p5[o];
+      d0 = (double)vinv * (double)p5[0];
+      d1 = (double)vinv * (double)p5[1];
+      d2 = (double)vinv * (double)p5[2];
+      d3 = (double)vinv * (double)p5[3];
     }
     for (int j = tid + nt; j < n; j += nt) {
       // only when n exceeds the workgroup (not the case for the shipped systems)
-      double v = K > 1 ? (double)sAc[(size_t)e * n + j] : (double)Ainv[(size_t)row * n + j];
-      if (K > 1)
+      double v = KD ? (double)sAc[(size_t)e * n + j] : (double)Ainv[(size_t)row * n + j];
+      if (KD)
         for (int d = 0; d < c; ++d) v -= (double)sAc[(size_t)(sRows[d] - r0) * n + j] * sBinv[c * K + d];
       sInv[j] = (T)v;
       const T* dp = dpsiM + (size_t)row * 3 * n;
       gd0 += v * (double)dp[j];
       gd1 += v * (double)dp[n + j];
       gd2 += v * (double)dp[2 * n + j];
+      const double vt = (double)(T)v;
+      T o5[5];
+#pragma unroll
+      for (int o = 0; o < 5; ++o) {
+        o5[o] = prow[(size_t)o * S.npad + j];
+        sPsi[o * n + j] = o5[o];
+      }
+      d0 += vt * (double)o5[0];
+      d1 += vt * (double)o5[1];
+      d2 += vt * (double)o5[2];
+      d3 += vt * (double)o5[3];
     }
+    QMCX_TICK(5)
     {
-      T r7 = 0, r8 = 0, r9 = 0, r10 = 0, r11 = 0, r12 = 0, r13 = 0, r14 = 0, r15 = 0, r16 = 0;
+      T r7 = 0, r8 = 0, r9 = 0, r10 = 0, r11 = 0;
       const bool up = iel < n;
 #pragma unroll
       for (int q = 0; q < JPT; ++q) {
@@ -370,87 +315,28 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
           r11 += d2u + T(2) * du;
         }
       }
-      for (int ci = tid; ci < M; ci += nt) {
-        const Functor<T>& f = sFn[sSpec[ci]];
-        if (f.ncoef > 0) {
-          T dx, dy, dz;
-          const T r = min_image(C, (T)(sIon[ci] - nx), (T)(sIon[M + ci] - ny), (T)(sIon[2 * M + ci] - nz), T(-1),
-                                dx, dy, dz);
-          if (r < f.rcut) {
-            T u, du, d2u;
-            functor_vgl(f, r, u, du, d2u);
-            r12 += u;
-            r13 += d2u + T(2) * du;
-            r14 += du * dx;
-            r15 += du * dy;
-            r16 += du * dz;
-          }
-        }
-      }
-      double v[13] = {gd0, gd1, gd2, r7, r8, r9, r10, r11, r12, r13, r14, r15, r16};
+      QMCX_TICK(6)
+      QMCX_TICK(7)
+      // slots: gnow 0-2, det ratio 3, its gradient 4-6, J2 7-11
+      double v[kRed] = {gd0, gd1, gd2, d0, d1, d2, d3, r7, r8, r9, r10, r11};
 #pragma unroll
-      for (int i = 0; i < 13; ++i) v[i] = wave_sum(v[i]);
-      if (lane == 0) {
-        sRed[0 * 16 + wid] = v[0];
-        sRed[1 * 16 + wid] = v[1];
-        sRed[2 * 16 + wid] = v[2];
+      for (int i = 0; i < kRed; ++i) v[i] = wave_sum(v[i]);
+      if (lane == 0)
 #pragma unroll
-        for (int i = 3; i < 13; ++i) sRed[(i + 4) * 16 + wid] = v[i];
-      }
-    }
-    if constexpr (kEarly) {
-      if (spline_lane) stencil_compute<T, PER>(S, st, sq, ss, sPart);
-    } else {
-      if (spline_lane) stencil_direct<T, PER, (PER > 4 ? 4 : PER)>(S, sSpoG, nx, ny, nz, sq, ss, sPart);
+        for (int i = 0; i < kRed; ++i) sRed[i * 16 + wid] = v[i];
     }
     lds_sync();
     QMCX_TICK(2)
 
-    // ---- phase 3: SPO values (einspline_spo.hpp:204-223) and ratioGrad_compute (DiracDeterminant.cpp:103-124)
-    {
-      const T gx = (T)S.ng[0], gy = (T)S.ng[1], gz = (T)S.ng[2];
-      double d0 = 0, d1 = 0, d2 = 0, d3 = 0;
-      for (int j = tid; j < n; j += nt) {
-        T o5[5];
-#pragma unroll
-        for (int o = 0; o < 5; ++o) {
-          T sum = 0;
-          for (int s2 = 0; s2 < SL; ++s2) sum += sPart[((size_t)s2 * 5 + o) * S.npad + j];
-          o5[o] = sum;
-        }
-        o5[1] *= gx;
-        o5[2] *= gy;
-        o5[3] *= gz;
-        o5[4] *= gx * gx;
-#pragma unroll
-        for (int o = 0; o < 5; ++o) sPsi[o * n + j] = o5[o];
-        const double v = (double)(j == tid ? vinv : sInv[j]);
-        d0 += v * (double)o5[0];
-        d1 += v * (double)o5[1];
-        d2 += v * (double)o5[2];
-        d3 += v * (double)o5[3];
-      }
-      d0 = wave_sum(d0);
-      d1 = wave_sum(d1);
-      d2 = wave_sum(d2);
-      d3 = wave_sum(d3);
-      if (lane == 0) {
-        sRed[3 * 16 + wid] = d0;
-        sRed[4 * 16 + wid] = d1;
-        sRed[5 * 16 + wid] = d2;
-        sRed[6 * 16 + wid] = d3;
-      }
-    }
-    lds_sync();
     double red[kRed];
     combine_partials(sRed, nw, red);
     QMCX_TICK(3)
 
     // ---- phase 4: acceptance and updates
     const double rdet = red[3];
-    const double ratio = rdet * exp((double)(Vat_old - (T)red[12])) * exp((double)(Uat_old - (T)red[7]));
+    const double ratio = rdet * exp((double)(Vat_old - (T)j1v[0])) * exp((double)(Uat_old - (T)red[7]));
     const double uacc = sRng[e * 4 + 3];
-    const bool accept = a.decide != nullptr ? (a.decide[w] != 0) : a.mode == 0 ? (uacc < 0.5) : (uacc < ratio * ratio);
+    const bool accept = a.mode == 0 ? (uacc < 0.5) : (uacc < ratio * ratio);
     if (B.trace != nullptr && tid == 0) {
       double* t = B.trace + ((size_t)w * N + iel) * 9;
       t[0] = ratio;
@@ -459,15 +345,15 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       t[3] = red[0] + (double)g1x + (double)g2x;
       t[4] = red[1] + (double)g1y + (double)g2y;
       t[5] = red[2] + (double)g1z + (double)g2z;
-      t[6] = red[4] / rdet + red[14] + red[8];
-      t[7] = red[5] / rdet + red[15] + red[9];
-      t[8] = red[6] / rdet + red[16] + red[10];
+      t[6] = red[4] / rdet + j1v[2] + red[8];
+      t[7] = red[5] / rdet + j1v[3] + red[9];
+      t[8] = red[6] / rdet + j1v[4] + red[10];
     }
     if (accept) {
       if (tid == 0) {
         logd += log(fabs(rdet));
         ph += rdet > 0.0 ? 0.0 : M_PI;
-        jl1 += (double)(Vat_old - (T)red[12]);
+        jl1 += (double)(Vat_old - (T)j1v[0]);
         jl2 += (double)(Uat_old - (T)red[7]);
         ++nacc;
       }
@@ -478,12 +364,12 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
         dpsiM[(size_t)row * 3 * n + n + j] = sPsi[2 * n + j];
         dpsiM[(size_t)row * 3 * n + 2 * n + j] = sPsi[3 * n + j];
         d2psiM[(size_t)row * n + j] = sPsi[4 * n + j];
-        if (K > 1) Vb[(size_t)c * n + j] = sAc[(size_t)e * n + j];
+        if (KD) Vb[(size_t)c * n + j] = sAc[(size_t)e * n + j];
       }
-      if (K == 1) {
+      if (!KD) {
         // Fahy/Sherman-Morrison (DelayedUpdate.h:166-173):
         // Ainv[m] -= ((Ainv[m].psiV - [m == row]) / ratio) * Ainv_old[row]
-        double* tm = (double*)sPart;
+        double* tm = sTm;
         for (int m0 = wid * 4; m0 < n; m0 += nw * 4) {
           double sm[4] = {0, 0, 0, 0};
           const T* am = Ainv + (size_t)m0 * n;
@@ -572,11 +458,11 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
           sJ2[2 * N + iel] = (T)red[9];
           sJ2[3 * N + iel] = (T)red[10];
           sJ2[4 * N + iel] = (T)(-red[11]);
-          J1[iel] = (T)red[12];
-          J1[N + iel] = (T)red[14];
-          J1[2 * N + iel] = (T)red[15];
-          J1[3 * N + iel] = (T)red[16];
-          J1[4 * N + iel] = (T)red[13];
+          J1[iel] = (T)j1v[0];
+          J1[N + iel] = (T)j1v[2];
+          J1[2 * N + iel] = (T)j1v[3];
+          J1[3 * N + iel] = (T)j1v[4];
+          J1[4 * N + iel] = (T)j1v[1];
           sR[iel] = nx;
           sR[N + iel] = ny;
           sR[2 * N + iel] = nz;
@@ -584,7 +470,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       }
     }
     // with K = 1 the next move reads rows of Ainv that other waves just wrote
-    if (K == 1) __syncthreads();
+    if (!KD) __syncthreads();
     else lds_sync();
     QMCX_TICK(4)
   }
@@ -597,9 +483,9 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     B.jlog[w * 2 + 0] += jl1;
     B.jlog[w * 2 + 1] += jl2;
     B.nacc[w] += nacc;
-    B.ndelay[w] = K > 1 ? c : 0;
+    B.ndelay[w] = KD ? c : 0;
   }
-  if (K > 1) {
+  if (KD) {
     for (int i = tid; i < c * K; i += nt) B.Binv[(size_t)w * K * K + i] = sBinv[i];
     for (int i = tid; i < c; i += nt) B.rows[(size_t)w * K + i] = sRows[i];
   }

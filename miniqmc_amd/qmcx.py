@@ -319,10 +319,12 @@ class WalkerBatch:
 
     def kernel_timing(self):
         """{stage: (ms, launches)} accumulated since timing was enabled, for the stages
-        move, flush, recompute (FP64 LU) and gl (evaluateGL)."""
-        ms, n = np.zeros(4), np.zeros(4, dtype=np.int64)
+        move, flush, recompute (FP64 inverse), gl (evaluateGL) and proposals (SPO rows at the
+        sweep's proposals)."""
+        ms, n = np.zeros(5), np.zeros(5, dtype=np.int64)
         _check(load().qmcx_get_kernel_timing(self.h, ms, n))
-        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(("move", "flush", "recompute", "gl"))}
+        return {k: (float(ms[i]), int(n[i]))
+                for i, k in enumerate(("move", "flush", "recompute", "gl", "proposals"))}
 
     def last_sweep_ms(self):
         ms = C.c_double(0)
