@@ -35,13 +35,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def per_move_bytes(sysd, k, acc, s=4):
-    """Algorithmic HBM bytes per walker-move of the move kernel.
+def path_bytes_per_move(sysd, k, acc, s=4):
+    """Algorithmic HBM bytes per walker-move of the whole path (every kernel of a sweep).
 
     Spline stencil 64*Np*s, positions 3(N+M)s, Jastrow state a*2*5*N*s, and the
     determinant row work s*n*(1+2c) + a*2*n*s with c the mean number of delayed
-    rows seen by a move.  Moves inside a chunk of k electrons see on average
-    a*(k-1)/2 delayed rows (the flush's 2*n^2*s/k per move is in the flush kernel).
+    rows seen by a move: moves inside a chunk of k electrons see on average
+    a*(k-1)/2 delayed rows (the flush's 2*n^2*s/k per move is not counted).
     """
     n, N, M, Np = sysd.nup, sysd.nel, sysd.nions, sysd.npad
     cbar = acc * (k - 1) / 2.0 if k > 1 else 0.0
@@ -50,6 +50,26 @@ def per_move_bytes(sysd, k, acc, s=4):
     b_j = acc * 2 * 5 * N * s
     b_det = s * n * (1 + 2 * cbar) + acc * 2 * n * s
     return b_spline + b_pos + b_j + b_det, dict(spline=b_spline, pos=b_pos, jastrow=b_j, det=b_det, cbar=cbar)
+
+
+def move_kernel_bytes_per_move(sysd, k, acc, s=4):
+    """Algorithmic HBM bytes per move of the fused move kernel (the spline stencil is read by
+    the proposals kernel): the proposal's SPO row 5*n*s and J1 terms (5 doubles), the moving
+    electron's dpsiM row 3*n*s and J1 state 4*s; on accept (fraction a) the new A, dpsiM,
+    d2psiM rows and the stale inverse row 6*n*s and the J1 state 5*s; per chunk of k moves the
+    walker's positions (3*N doubles) and J2 state (5*N*s) in and out and the chunk's stale
+    inverse rows k*n*s in."""
+    n, N = sysd.nup, sysd.nel
+    chunk = k if k > 1 else n
+    per = 5 * n * s + 5 * 8 + 3 * n * s + 4 * s + acc * (6 * n * s + 5 * s)
+    per_chunk = 2 * (3 * N * 8 + 5 * N * s) + (chunk * n * s if k > 1 else 0)
+    return per + per_chunk / chunk
+
+
+def proposal_bytes(sysd, s=4):
+    """Algorithmic HBM bytes per proposal of the proposals kernel: the 4x4x4 stencil of the
+    table, 64*Np*s, the SPO row written, 5*n*s, and the J1 terms, 5 doubles."""
+    return 64 * sysd.npad * s + 5 * sysd.nup * s + 5 * 8
 
 
 def measured_traffic(sysd, k, moves_per_launch):
@@ -67,16 +87,20 @@ def measured_traffic(sysd, k, moves_per_launch):
     return round(rec["bytes_per_launch"]), os.path.relpath(path, ROOT)
 
 
-def cpu_baseline(sysd, R0, seed, tau, nwalk, nthreads):
+def cpu_baseline(sysd, R0, seed, tau, nthreads, target_s=12.0):
+    """The all-FP64 CPU oracle on the host's cores over a bounded sample of the same workload:
+    two walkers per thread, one calibration sweep, then as many sweeps (same chains) as fill
+    about target_s seconds."""
     from oracle import pyoracle as po
     orc = po.OracleSystem(sysd, spline_prec=8)
-    t0 = time.time()
-    res = orc.run_vmc(R0[:nwalk], 1, seed, tau, mode=qmcx.ACCEPT_METROPOLIS, delay=1, nthreads=nthreads)
-    wall = time.time() - t0
-    moves = nwalk * sysd.nel
+    nwalk = min(2 * nthreads, len(R0))
+    first = orc.run_vmc(R0[:nwalk], 1, seed, tau, mode=qmcx.ACCEPT_METROPOLIS, delay=1, nthreads=nthreads)
+    nsw = int(max(1, min(200, round(target_s / max(first["seconds"], 1e-3)))))
+    res = orc.run_vmc(first["R"], nsw, seed + 1, tau, mode=qmcx.ACCEPT_METROPOLIS, delay=1, nthreads=nthreads)
+    moves = nwalk * sysd.nel * nsw
     return dict(value=moves / res["seconds"], unit="walker-moves/s", cores=nthreads, kind="port",
-                sample=f"{nwalk} walkers x 1 sweep of {sysd.name} ({moves} moves), all-FP64 oracle, delay 1, "
-                       f"OpenMP over walkers; sweep time {res['seconds']:.2f} s (wall incl. evaluateLog {wall:.2f} s)")
+                sample=f"{nwalk} walkers x {nsw} sweeps of {sysd.name} ({moves} moves, {res['seconds']:.1f} s), "
+                       f"all-FP64 oracle, delay 1, OpenMP over walkers (2 per thread)")
 
 
 def main():
@@ -86,14 +110,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--system", default="nio64")
     ap.add_argument("--walkers", type=int, default=128)
-    ap.add_argument("--delay", type=int, default=0, help="delay rank (0: autotune over {16,32,64})")
+    ap.add_argument("--delay", type=int, default=0, help="delay rank (0: autotune delay rank and spline tile, miniqmc_amd.autotune)")
     ap.add_argument("--tau", type=float, default=0.3)
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--fp64-steps", type=int, default=4)
     ap.add_argument("--recompute", type=int, default=1,
                     help="FP32 form: FP64 LU inverse recompute every this many sweeps (0: never)")
     ap.add_argument("--err-steps", type=int, default=2, help="sweeps of the FP32-vs-FP64 error leg")
-    ap.add_argument("--cpu-walkers", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device five times (a kernel of known HBM traffic "
@@ -201,8 +224,12 @@ def main():
     log(f"FP64 k=1: batched {value64:.4e} moves/s ({1e3 * wall64 / args.fp64_steps:.2f} ms/sweep), "
         f"one walker {value64_1:.4e} moves/s")
 
-    # ---- roofline of the dominant kernel (the fused move kernel)
-    bpm, parts = per_move_bytes(sysd, k, acc_timed)
+    # ---- rooflines.  The dominant kernel (most of the sweep's time) is the fused move kernel:
+    # one workgroup per walker walks its electrons in order, a latency-bound chain that moves
+    # few bytes per move.  The proposals kernel carries the spline's table traffic; the whole
+    # sweep is also put against the path's algorithmic bytes.
+    bpm_path, parts = path_bytes_per_move(sysd, k, acc_timed)
+    bpm = move_kernel_bytes_per_move(sysd, k, acc_timed)
     chunk = k if k > 1 else sysd.nup
     launch_ms = mv_ms / max(mv_n, 1)
     achieved = bpm * W * chunk / (launch_ms * 1e-3) / 1e9
@@ -211,11 +238,25 @@ def main():
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src,
                     kernel="k_moves<float>", bytes_per_move=round(bpm, 1),
                     moves_per_launch=W * chunk, avg_launch_ms=round(launch_ms, 4))
+    pr_ms, pr_n = kt["proposals"]
+    pr_launch = pr_ms / max(pr_n, 1)
+    pr_units = W * sysd.nup
+    pr_ach = proposal_bytes(sysd) * pr_units / (pr_launch * 1e-3) / 1e9 if pr_n else None
+    roofline_proposals = dict(bound="hbm", kernel="k_spo_proposals<float>", unit="GB/s", peak=HBM_PEAK_GBS,
+                              achieved=round(pr_ach, 2) if pr_ach else None,
+                              frac=round(pr_ach / HBM_PEAK_GBS, 4) if pr_ach else None,
+                              bytes_per_proposal=proposal_bytes(sysd), proposals_per_launch=pr_units,
+                              avg_launch_ms=round(pr_launch, 4),
+                              note="runs on its own stream beside the move kernel")
+    sweep_gbs = bpm_path * moves / wall / 1e9
+    roofline_sweep = dict(bound="hbm", unit="GB/s", peak=HBM_PEAK_GBS, achieved=round(sweep_gbs, 2),
+                          frac=round(sweep_gbs / HBM_PEAK_GBS, 4), bytes_per_move=round(bpm_path, 1),
+                          note="whole path's algorithmic bytes per walker-move over the timed sweeps")
 
     cpu = None
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(sysd, R0, args.seed, args.tau, min(args.cpu_walkers, W), nthreads)
+        cpu = cpu_baseline(sysd, R0, args.seed, args.tau, nthreads)
         log(f"CPU oracle: {cpu['value']:.4e} moves/s on {nthreads} threads")
 
     out = {
@@ -239,6 +280,8 @@ def main():
             "spline_tile": tile, "moves_per_step": W * N,
         },
         "roofline": roofline,
+        "roofline_proposals": roofline_proposals,
+        "roofline_sweep": roofline_sweep,
         "cpu_baseline": cpu,
         "fp64_delay1": {"value": value64, "unit": "walker-moves/s", "ms_per_step": 1e3 * wall64 / args.fp64_steps,
                         "steps": args.fp64_steps, "walkers": W,

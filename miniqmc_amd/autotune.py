@@ -2,17 +2,18 @@
 
 For each candidate (delay rank k, spline stencil slices S) a batch is built on
 the real walker configuration, equilibrated for one sweep and timed over a
-few sweeps with HIP events; the fastest pair wins.  The delay rank trades the
-per-move cost of applying pending rows (grows with k) against the per-flush
-cost of rewriting the inverse (shrinks as 1/k); S sets how many workgroup
-lanes share one point's 4x4x4 stencil.
+few sweeps with HIP events (FP64 refresh every sweep, as the FP32 form runs);
+the fastest pair wins.  The delay rank trades the per-move cost of applying
+pending rows (grows with k) against the per-flush cost of rewriting the
+inverse (shrinks as 1/k); S sets how many lanes of the proposals kernel share
+one point's 4x4x4 stencil (0: the library's choice).
 """
 from __future__ import annotations
 
 from . import qmcx
 
-DELAYS = (16, 32, 64, 128)
-TILES = (0, 4, 8, 16)
+DELAYS = (8, 16, 32, 64)
+TILES = (0, 1, 4)
 
 
 def tune(dev: qmcx.System, nwalkers: int, R0, seed: int = 11, tau: float = 0.3, sweeps: int = 2,
@@ -27,8 +28,8 @@ def tune(dev: qmcx.System, nwalkers: int, R0, seed: int = 11, tau: float = 0.3, 
                 continue   # does not fit (LDS) or tile invalid for this system
             b.set_positions(R0)
             b.evaluate_log()
-            b.run(1, seed + 7919, tau, qmcx.ACCEPT_METROPOLIS)
-            b.run(sweeps, seed + 7919, tau, qmcx.ACCEPT_METROPOLIS)
+            b.run(1, seed + 7919, tau, qmcx.ACCEPT_METROPOLIS, recompute_every=1)
+            b.run(sweeps, seed + 7919, tau, qmcx.ACCEPT_METROPOLIS, recompute_every=1)
             ms = b.last_sweep_ms() / sweeps
             b.close()
             results[f"k{k}_s{tile}"] = round(ms, 3)

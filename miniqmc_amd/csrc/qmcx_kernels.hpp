@@ -257,16 +257,61 @@ __global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, i
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 constexpr int kFlushBM = 64;        // rows per workgroup: 4 waves x 16 rows
 constexpr int kFlushMaxCT = 8;      // delayed rows up to 128 = 8 tiles of 16
-// LDS of k_flush_mfma: per wave T and T2 (16 x cp floats each), the row list
+constexpr int kFlushBinvLds = 64;   // Binv staged in LDS up to this many delayed rows
+// LDS of k_flush_mfma: per wave T and T2 (16 x cp floats each), Binv (c x c doubles, when
+// c <= kFlushBinvLds), the row list
 __host__ __device__ inline size_t flush_mfma_lds(int n, int K) {
   const int cp = (K + 15) & ~15;
+  const int kb = K <= kFlushBinvLds ? K : 0;
   (void)n;
-  return sizeof(float) * 4 * 2 * 16 * cp + sizeof(int) * K;
+  return sizeof(float) * 4 * 2 * 16 * cp + sizeof(double) * kb * kb + sizeof(int) * K;
 }
+// GEMM 1 of the flush for NCT 16-column tiles of delayed rows: acc[ct] = Ainv_rows(16) . U^T;
+// the k loop is unrolled four times so that a dozen loads are in flight per wave
+template <int NCT>
+__device__ inline void flush_gemm1(const float* ap, const float* const* bp, bool aok, int li, int c, int q4,
+                                   f32x4_t* acc) {
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) acc[ct] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+  for (int k = 0; k < q4; k += 4) {
+    f32x4_t av = *reinterpret_cast<const f32x4_t*>(ap + k);
+    f32x4_t bv[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) bv[ct] = *reinterpret_cast<const f32x4_t*>(bp[ct] + k);
+    if (!aok) av = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      if (ct * 16 + li >= c) bv[ct] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[0], bv[ct][0], acc[ct], 0, 0, 0);
+      acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[1], bv[ct][1], acc[ct], 0, 0, 0);
+      acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[2], bv[ct][2], acc[ct], 0, 0, 0);
+      acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[3], bv[ct][3], acc[ct], 0, 0, 0);
+    }
+  }
+}
+
+template <int NCT>
+__device__ inline void flush_store_t(const f32x4_t* acc, const int* sRows, int r0, int g, int li, int c, int cp,
+                                     float* sT) {
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 4 * g + r;             // row within the wave's 16
+      const int dd = ct * 16 + li;         // delayed-row index
+      float v = acc[ct][r];
+      if (dd < c && sRows[dd] == r0 + i) v -= 1.f;
+      sT[i * cp + dd] = dd < c ? v : 0.f;
+    }
+}
+
 // Each wave owns 16 rows of Ainv: T = Ainv_rows U^T - E on the matrix cores with
 // the rows read from HBM once, T2 = -T Binv in FP64, then Ainv_rows += T2 V with
 // the accumulators started from the rows again (now in L2).  U (the new Slater
 // rows) and V (the stale inverse rows) are shared by all waves of the walker.
+// Loads are unconditional (clamped indices, zeroed after) and batched, so each
+// wave keeps a dozen or more in flight.
 __global__ __launch_bounds__(256) void k_flush_mfma(SysP<float> S, BatchP<float> B, int det) {
   const int w = blockIdx.y;
   const int c = B.ndelay[w];
@@ -279,13 +324,17 @@ __global__ __launch_bounds__(256) void k_flush_mfma(SysP<float> S, BatchP<float>
   extern __shared__ __align__(16) unsigned char smem[];
   float* sT = (float*)smem + wid * 2 * 16 * cp;      // [16][cp]   T = Ainv U^T - E
   float* sT2 = sT + 16 * cp;                          // [16][cp]  -T Binv
-  int* sRows = (int*)((float*)smem + 4 * 2 * 16 * cp);
+  const bool binv_lds = K <= kFlushBinvLds;   // as sized by flush_mfma_lds (c <= K)
+  double* sB = (double*)((float*)smem + 4 * 2 * 16 * cp);   // [c][c] when binv_lds
+  int* sRows = (int*)(sB + (K <= kFlushBinvLds ? K * K : 0));
   const size_t moff = ((size_t)w * 2 + det) * n * n;
   float* Ainv = B.Ainv + moff;
   const float* Amat = B.A + moff;
   const float* Vb = B.Vbuf + (size_t)w * K * n;
   const double* Bi = B.Binv + (size_t)w * K * K;
   for (int i = tid; i < c; i += 256) sRows[i] = B.rows[(size_t)w * K + i];
+  if (binv_lds)
+    for (int i = tid; i < c * c; i += 256) sB[i] = Bi[(i / c) * K + i % c];
   __syncthreads();
   const int li = lane & 15, g = lane >> 4;
   const int q4 = n / 4;                    // columns per lane group (n % 16 == 0)
@@ -298,38 +347,18 @@ __global__ __launch_bounds__(256) void k_flush_mfma(SysP<float> S, BatchP<float>
 #pragma unroll
     for (int ct = 0; ct < kFlushMaxCT; ++ct) {
       const int d = ct * 16 + li;
-      bp[ct] = Amat + (size_t)(d < c ? sRows[d] : 0) * n + g * q4;
+      bp[ct] = Amat + (size_t)(d < c ? sRows[d] : sRows[0]) * n + g * q4;
     }
     f32x4_t acc[kFlushMaxCT];
-#pragma unroll
-    for (int ct = 0; ct < kFlushMaxCT; ++ct) acc[ct] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-    for (int k = 0; k < q4; k += 4) {
-      f32x4_t av = *reinterpret_cast<const f32x4_t*>(ap + k);
-      if (!aok) av = f32x4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ct = 0; ct < kFlushMaxCT; ++ct) {
-        if (ct < nct) {
-          f32x4_t bv = *reinterpret_cast<const f32x4_t*>(bp[ct] + k);
-          if (ct * 16 + li >= c) bv = f32x4_t{0.f, 0.f, 0.f, 0.f};
-          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[0], bv[0], acc[ct], 0, 0, 0);
-          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[1], bv[1], acc[ct], 0, 0, 0);
-          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[2], bv[2], acc[ct], 0, 0, 0);
-          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[3], bv[3], acc[ct], 0, 0, 0);
-        }
-      }
-    }
-#pragma unroll
-    for (int ct = 0; ct < kFlushMaxCT; ++ct) {
-      if (ct < nct) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = 4 * g + r;             // row within the wave's 16
-          const int dd = ct * 16 + li;         // delayed-row index
-          float v = acc[ct][r];
-          if (dd < c && sRows[dd] == r0 + i) v -= 1.f;
-          sT[i * cp + dd] = dd < c ? v : 0.f;
-        }
-      }
+    switch (nct) {
+      case 1: flush_gemm1<1>(ap, bp, aok, li, c, q4, acc); flush_store_t<1>(acc, sRows, r0, g, li, c, cp, sT); break;
+      case 2: flush_gemm1<2>(ap, bp, aok, li, c, q4, acc); flush_store_t<2>(acc, sRows, r0, g, li, c, cp, sT); break;
+      case 3: flush_gemm1<3>(ap, bp, aok, li, c, q4, acc); flush_store_t<3>(acc, sRows, r0, g, li, c, cp, sT); break;
+      case 4: flush_gemm1<4>(ap, bp, aok, li, c, q4, acc); flush_store_t<4>(acc, sRows, r0, g, li, c, cp, sT); break;
+      case 5: flush_gemm1<5>(ap, bp, aok, li, c, q4, acc); flush_store_t<5>(acc, sRows, r0, g, li, c, cp, sT); break;
+      case 6: flush_gemm1<6>(ap, bp, aok, li, c, q4, acc); flush_store_t<6>(acc, sRows, r0, g, li, c, cp, sT); break;
+      case 7: flush_gemm1<7>(ap, bp, aok, li, c, q4, acc); flush_store_t<7>(acc, sRows, r0, g, li, c, cp, sT); break;
+      default: flush_gemm1<8>(ap, bp, aok, li, c, q4, acc); flush_store_t<8>(acc, sRows, r0, g, li, c, cp, sT); break;
     }
   }
   __syncthreads();
@@ -337,30 +366,53 @@ __global__ __launch_bounds__(256) void k_flush_mfma(SysP<float> S, BatchP<float>
   for (int idx = lane; idx < 16 * cp; idx += 64) {
     const int i = idx / cp, dd = idx - i * cp;
     double sum = 0;
-    if (dd < c)
-      for (int e = 0; e < c; ++e) sum += (double)sT[i * cp + e] * Bi[e * K + dd];
+    if (dd < c) {
+      if (binv_lds) {
+        for (int e = 0; e < c; ++e) sum += (double)sT[i * cp + e] * sB[e * c + dd];
+      } else {
+#pragma unroll 8
+        for (int e = 0; e < c; ++e) sum += (double)sT[i * cp + e] * Bi[e * K + dd];
+      }
+    }
     sT2[i * cp + dd] = (float)(-sum);
   }
   __syncthreads();
-  // ---- GEMM 2: Ainv[r0:r0+16][:] += T2 . V
-  for (int ct = 0; ct < n / 16; ++ct) {
-    const int col = ct * 16 + li;
-    f32x4_t acc;
+  // ---- GEMM 2: Ainv[r0:r0+16][:] += T2 . V, four column tiles at a time
+  const int ntile = n / 16;
+  for (int ct0 = 0; ct0 < ntile; ct0 += 4) {
+    f32x4_t acc[4];
+    int col[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = r0 + 4 * g + r;
-      acc[r] = row < n ? Ainv[(size_t)row * n + col] : 0.f;
+    for (int t = 0; t < 4; ++t) {
+      const int ct = ct0 + t < ntile ? ct0 + t : ntile - 1;
+      col[t] = ct * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = r0 + 4 * g + r;
+        acc[t][r] = Ainv[(size_t)(row < n ? row : n - 1) * n + col[t]];
+      }
     }
+#pragma unroll 2
     for (int k0 = 0; k0 < cp; k0 += 4) {
       const int kk = k0 + g;
       const float av = sT2[li * cp + kk];
-      const float bv = kk < c ? Vb[(size_t)kk * n + col] : 0.f;
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+      float bv[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) bv[t] = Vb[(size_t)(kk < c ? kk : c - 1) * n + col[t]];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (kk >= c) bv[t] = 0.f;
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[t], acc[t], 0, 0, 0);
+      }
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = r0 + 4 * g + r;
-      if (row < n) Ainv[(size_t)row * n + col] = acc[r];
+    for (int t = 0; t < 4; ++t) {
+      if (ct0 + t >= ntile) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = r0 + 4 * g + r;
+        if (row < n) Ainv[(size_t)row * n + col[t]] = acc[t][r];
+      }
     }
   }
 }
