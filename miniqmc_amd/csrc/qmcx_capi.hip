@@ -228,7 +228,7 @@ struct qmcx_batch_s {
   void* Vbuf = nullptr;
   void* Binv = nullptr;
   void* spo = nullptr;   // [W][2][n][5][npad] SPO rows at the sweep's proposals
-  double* j1p = nullptr; // [W][2][n][8] J1 at the sweep's proposals
+  double* prop = nullptr; // [W][2][n][kProp] proposal records of the sweep
   int* rows = nullptr;
   int* ndelay = nullptr;
   double* logdet = nullptr;
@@ -369,7 +369,7 @@ int launch_proposals(qmcx_batch_t b, hipStream_t st, int det, int e0, int ne, ui
   const int nt = std::max(64, (nq * slices + 63) / 64 * 64);
   const size_t lds = std::max(sizeof(T) * (size_t)slices * 5 * s->npad, sizeof(double) * 16 * 5);
   hipLaunchKernelGGL(k_spo_proposals<T>, dim3(ne, b->W), dim3(nt), lds, st, s->params<T>(), b->params<T>(), det, e0,
-                     seed, sweep, std::sqrt(tau), b->walker0, displ, (T*)b->spo, b->j1p, slices);
+                     seed, sweep, std::sqrt(tau), b->walker0, displ, (T*)b->spo, b->prop, slices);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -438,23 +438,23 @@ bool use_mfma_flush(const qmcx_batch_t b) {
   return sizeof(T) == 4 && b->sys->n % 16 == 0 && b->flush_kind == 0;
 }
 
+// Electrons per move-kernel launch: the delay rank (one flush per chunk), or up to 64 with
+// Sherman-Morrison updates (K = 1), which bounds the chunk's LDS records.
+static int move_chunk(int n, int K) { return K == 1 ? std::min(n, 64) : K; }
+
 // the moves (and delayed-update flushes) of one determinant's electrons in one sweep
 template <typename T>
-int moves_det(qmcx_batch_t b, int det, uint64_t seed, double tau, int mode) {
+int moves_det(qmcx_batch_t b, int det, int mode) {
   const qmcx_system_s* s = b->sys;
   int nt = 0;
   if (choose_shape(s, nt)) return QMCX_ERR_ARG;
-  const int chunk = b->K == 1 ? s->n : b->K;
+  const int chunk = move_chunk(s->n, b->K);
   const MoveLds L = move_lds<T>(s->N, s->nions, s->n, b->K, chunk);
   MoveArgs a;
   a.mode = mode;
   a.slices = 0;
   a.spo = b->spo;
-  a.j1p = b->j1p;
-  a.walker0 = b->walker0;
-  a.sqrt_tau = std::sqrt(tau);
-  a.seed = seed;
-  a.displ = nullptr;
+  a.prop = b->prop;
   a.decide = nullptr;
   const size_t flds = sizeof(T) * 2 * kFlushRows * b->K + sizeof(int) * b->K;
   for (int e = 0; e < s->n; e += chunk) {
@@ -522,7 +522,7 @@ int issue_proposals(qmcx_batch_t b, int det, hipEvent_t after, uint64_t seed, in
 template <typename T>
 int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompute, int slot, int sweep, bool last) {
   HIPCHK(hipStreamWaitEvent(b->stream, b->ev_pre[0], 0));
-  if (int rc = moves_det<T>(b, 0, seed, tau, mode)) return rc;
+  if (int rc = moves_det<T>(b, 0, mode)) return rc;
   HIPCHK(hipEventRecord(b->ev_done[0], b->stream));
   if (!last)
     if (int rc = issue_proposals<T>(b, 0, b->ev_done[0], seed, sweep + 1, tau)) return rc;
@@ -536,7 +536,7 @@ int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompu
     b->pending_dn = false;
   }
   HIPCHK(hipStreamWaitEvent(b->stream, b->ev_pre[1], 0));
-  if (int rc = moves_det<T>(b, 1, seed, tau, mode)) return rc;
+  if (int rc = moves_det<T>(b, 1, mode)) return rc;
   HIPCHK(hipEventRecord(b->ev_done[1], b->stream));
   if (!last)
     if (int rc = issue_proposals<T>(b, 1, b->ev_done[1], seed, sweep + 1, tau)) return rc;
@@ -797,7 +797,7 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
       delete b;
       return rc;
     }
-    const int chunk = delay_rank == 1 ? s->n : delay_rank;
+    const int chunk = move_chunk(s->n, delay_rank);
     const size_t lds = precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, chunk).total
                                               : move_lds<double>(s->N, s->nions, s->n, delay_rank, chunk).total;
     if (lds > 160 * 1024) {
@@ -838,7 +838,7 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
   rc |= b->alloc(b->lu_ws, W * 2 * 2 * n * n);
   rc |= b->alloc(p8, es * W * 2 * n * 5 * s->npad);
   b->spo = p8;
-  rc |= b->alloc(b->j1p, W * 2 * n * 8);
+  rc |= b->alloc(b->prop, W * 2 * n * kProp);
   rc |= b->alloc(b->ld_fresh, W);
   rc |= b->alloc(b->ph_fresh, W);
   if (rc) {
@@ -1051,14 +1051,12 @@ static int flex_launch(qmcx_batch_t b, int iel, bool apply) {
   MoveArgs a;
   std::memset(&a, 0, sizeof(a));
   a.spo = b->spo;
-  a.j1p = b->j1p;
+  a.prop = b->prop;
   a.det = iel < s->n ? 0 : 1;
   a.e_begin = iel;
   a.e_end = iel + 1;
   a.mode = QMCX_ACCEPT_METROPOLIS;
   a.slices = 0;
-  a.walker0 = b->walker0;
-  a.displ = b->flex_displ;
   a.decide = b->flex_decide;
   const bool trace_was = b->trace_on;
   b->trace_on = true;
@@ -1172,15 +1170,15 @@ int qmcx_last_sweep_ms(qmcx_batch_t b, double* ms) {
 }  // extern "C"
 
 #ifdef QMCX_PHASE_TIMING
-// Profiling builds only: per-phase wall_clock64 ticks (walker / matrix 0, lane 0); slots 0-4
-// the move kernel's phases, 8-11 the FP64 inverse's transpose, panels, updates and output.
-extern "C" int qmcx_debug_phase_ticks(double* out12, int reset) {
-  unsigned long long t[12];
+// Profiling builds only: per-phase wall_clock64 ticks (walker / matrix 0, lane 0); slots 0-7
+// and 12-15 the move kernel's phases, 8-11 the FP64 inverse's transpose, panels, updates and output.
+extern "C" int qmcx_debug_phase_ticks(double* out16, int reset) {
+  unsigned long long t[qmcx::kPhaseSlots];
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpyFromSymbol(t, HIP_SYMBOL(qmcx_phase_ticks), sizeof(t)));
-  for (int i = 0; i < 12; ++i) out12[i] = (double)t[i];
+  for (int i = 0; i < qmcx::kPhaseSlots; ++i) out16[i] = (double)t[i];
   if (reset) {
-    unsigned long long z[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long z[qmcx::kPhaseSlots] = {};
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(qmcx_phase_ticks), z, sizeof(z)));
   }
   return 0;

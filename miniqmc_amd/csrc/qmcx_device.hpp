@@ -87,6 +87,100 @@ __device__ inline T wave_sum(T v) {
   return lane63(v);
 }
 
+// Exchange of wave halves between two values (v_permlane32_swap: lanes 32-63 of x
+// trade places with lanes 0-31 of y): x <- [x lanes 0-31 | y lanes 0-31],
+// y <- [x lanes 32-63 | y lanes 32-63].
+__device__ inline void swap_halves(double& x, double& y) {
+  const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
+  const auto lo = __builtin_amdgcn_permlane32_swap((int)(bx & 0xffffffff), (int)(by & 0xffffffff), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((int)(bx >> 32), (int)(by >> 32), false, false);
+  x = __longlong_as_double(((long long)hi[0] << 32) | (unsigned int)lo[0]);
+  y = __longlong_as_double(((long long)hi[1] << 32) | (unsigned int)lo[1]);
+}
+// Exchange of 16-lane rows (v_permlane16_swap: odd rows of x trade places with even
+// rows of y): x <- [x0 y0 x2 y2], y <- [x1 y1 x3 y3] by rows.
+__device__ inline void swap_rows(double& x, double& y) {
+  const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
+  const auto lo = __builtin_amdgcn_permlane16_swap((int)(bx & 0xffffffff), (int)(by & 0xffffffff), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((int)(bx >> 32), (int)(by >> 32), false, false);
+  x = __longlong_as_double(((long long)hi[0] << 32) | (unsigned int)lo[0]);
+  y = __longlong_as_double(((long long)hi[1] << 32) | (unsigned int)lo[1]);
+}
+
+// Wave sums of NV values (NV a multiple of 4) with NV/2 + NV/4 lane exchanges and NV/4 row
+// reductions instead of NV full reductions: halves are traded between value pairs (i, i+NV/2),
+// then rows between (i, i+NV/4).  On return v[i] (i < NV/4) holds, in every lane of row r
+// (lanes 16r .. 16r+15), the wave total of value i + r*NV/4.
+template <int NV>
+__device__ inline void wave_sums_rows(double (&v)[NV]) {
+  static_assert(NV % 4 == 0, "four values per row group");
+#pragma unroll
+  for (int i = 0; i < NV / 2; ++i) {
+    swap_halves(v[i], v[i + NV / 2]);
+    v[i] += v[i + NV / 2];
+  }
+#pragma unroll
+  for (int i = 0; i < NV / 4; ++i) {
+    swap_rows(v[i], v[i + NV / 4]);
+    v[i] += v[i + NV / 4];
+  }
+#pragma unroll
+  for (int i = 0; i < NV / 4; ++i) {
+    v[i] = dpp_add<0xB1, 0xf>(v[i]);    // quad_perm [1,0,3,2]
+    v[i] = dpp_add<0x4E, 0xf>(v[i]);    // quad_perm [2,3,0,1]
+    v[i] = dpp_add<0x141, 0xf>(v[i]);   // row_half_mirror
+    v[i] = dpp_add<0x140, 0xf>(v[i]);   // row_mirror
+  }
+}
+
+// Global -> LDS copy of NR regions by the whole workgroup: every thread issues up to eight
+// loads before its first LDS store, so the regions arrive in one or two memory round trips.
+// 16-byte vectors when every address and size allows, 4-byte words otherwise.
+struct LdsCopy {
+  void* dst;
+  const void* src;
+  unsigned bytes;
+};
+template <typename V, int NR>
+__device__ inline void stage_lds_v(const LdsCopy (&c)[NR]) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  unsigned pre[NR + 1];
+  pre[0] = 0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) pre[r + 1] = pre[r] + c[r].bytes / (unsigned)sizeof(V);
+  const unsigned tot = pre[NR];
+  for (unsigned b = 0; b < tot; b += 8u * nt) {
+    V v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const unsigned i = min(b + (unsigned)(k * nt + tid), tot - 1);
+      const V* s = (const V*)c[0].src + i;
+#pragma unroll
+      for (int r = 1; r < NR; ++r)
+        if (i >= pre[r]) s = (const V*)c[r].src + (i - pre[r]);
+      v[k] = *s;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const unsigned i = b + (unsigned)(k * nt + tid);
+      V* d = (V*)c[0].dst + i;
+#pragma unroll
+      for (int r = 1; r < NR; ++r)
+        if (i >= pre[r]) d = (V*)c[r].dst + (i - pre[r]);
+      if (i < tot) *d = v[k];
+    }
+  }
+}
+template <int NR>
+__device__ inline void stage_lds(const LdsCopy (&c)[NR]) {
+  bool vec = true;
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+    vec = vec && (((uintptr_t)c[r].dst | (uintptr_t)c[r].src | (uintptr_t)c[r].bytes) & 15) == 0;
+  if (vec) stage_lds_v<int4, NR>(c);
+  else stage_lds_v<int, NR>(c);
+}
+
 // Barrier for data exchanged through LDS only: waits for this wave's LDS
 // operations, not for its outstanding global loads.
 __device__ inline void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
@@ -159,7 +253,7 @@ __device__ inline T functor_v(const Functor<T>& f, T r) {
 // Periodic general cell: reduced basis rb (rows), its inverse g, eight corner
 // shifts (ParticleBConds.h:99-217).  d is (target - source) before the flip.
 template <typename T>
-struct Cell {
+struct alignas(16) Cell {   // a whole number of 16-byte vectors (copied to LDS as such)
   T g[9];        // g[3*i+j] = g(i,j)
   T rb[9];       // rb[3*i+j] = rb[i][j]
   T corner[24];

@@ -53,14 +53,17 @@ struct BatchP {
   int* sweep;      // [0] global sweep index, [1] index of the first sweep of the current run
 };
 
+// Proposal record of one electron, [W][2][n][kProp] doubles written by k_spo_proposals:
+// J1 at the proposal (sum u, sum (d2u + 2 du/r), gradient), the displacement, the
+// acceptance uniform.
+constexpr int kProp = 10;
+constexpr int kPropJ1 = 0, kPropDispl = 5, kPropU = 8;
+
 struct MoveArgs {
-  int e_begin, e_end, det, mode, slices, walker0;
-  double sqrt_tau;
-  uint64_t seed;
-  const double* displ;   // [W][3] caller proposals (flex_makeMove) or null: Philox stream
+  int e_begin, e_end, det, mode, slices;
   const int* decide;     // [W] caller accept decisions (flex_acceptrestoreMove) or null
   const void* spo;       // [W][2][n][5][npad] T: SPO rows at the proposals (k_spo_proposals)
-  const double* j1p;     // [W][2][n][8]: J1 at the proposals: sum u, sum (d2u + 2 du/r), gradient
+  const double* prop;    // [W][2][n][kProp] proposal records (k_spo_proposals)
 };
 
 // ------------------------------------------------------------------ spline gather
@@ -797,19 +800,19 @@ __global__ __launch_bounds__(1024) void k_spo_rows(SysP<T> S, BatchP<T> B, int s
 template <typename T>
 __global__ __launch_bounds__(1024) void k_spo_proposals(SysP<T> S, BatchP<T> B, int det, int e0, uint64_t seed,
                                                          int sweep, double sqrt_tau, int walker0, const double* displ,
-                                                         T* spo, double* j1p, int slices) {
+                                                         T* spo, double* prop, int slices) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* part = (T*)smem;
   const int el = e0 + blockIdx.x, w = blockIdx.y, N = S.N, n = S.n;
   const int iel = det * n + el;
   const double* R = B.R + (size_t)w * 3 * N;
-  double d[3];
+  double d[3], u = 0.0;   // a caller's displacement comes with its own accept decision
   if (displ != nullptr) {
     d[0] = displ[3 * w + 0];
     d[1] = displ[3 * w + 1];
     d[2] = displ[3 * w + 2];
   } else {
-    double nr[3], u;
+    double nr[3];
     move_randoms(seed, (uint64_t)(walker0 + w), (uint64_t)sweep, (uint64_t)iel, nr, u);
     d[0] = sqrt_tau * nr[0];
     d[1] = sqrt_tau * nr[1];
@@ -853,7 +856,10 @@ __global__ __launch_bounds__(1024) void k_spo_proposals(SysP<T> S, BatchP<T> B, 
   if (threadIdx.x < 5) {
     double t = 0;
     for (int q = 0; q < nw; ++q) t += red[q * 5 + threadIdx.x];
-    j1p[(((size_t)w * 2 + det) * n + el) * 8 + threadIdx.x] = t;
+    prop[(((size_t)w * 2 + det) * n + el) * kProp + kPropJ1 + threadIdx.x] = t;
+  } else if (threadIdx.x < 9) {
+    const int k = threadIdx.x - 5;
+    prop[(((size_t)w * 2 + det) * n + el) * kProp + kPropDispl + k] = k < 3 ? d[k] : u;
   }
 }
 

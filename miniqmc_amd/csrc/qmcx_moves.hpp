@@ -20,33 +20,36 @@ constexpr int kRed = 12;   // reduction slots: gnow(3) ratio(1) gnew(3) J2(5)
 constexpr int kMoveThreads = 512;
 
 struct MoveLds {
-  unsigned oR, oIon, oRng, oRed, oInv, oPsi, oBinv, oP, oPart, oRows, oSpec, oFn, oCell, oSpoG, oJ2, oAc, oQ, total;
+  unsigned oR, oRed, oInv, oPsi, oBinv, oP, oPart, oRows, oAl, oJ1n, oFn, oCell, oProp, oJ1o, oJt, oJ2, oAc, oQ, total;
 };
 __host__ __device__ inline unsigned lds_take(unsigned& o, size_t bytes) {
   unsigned r = o;
   o = (unsigned)((o + bytes + 15) & ~size_t(15));
   return r;
 }
-// E = electrons per chunk; the inverse-row cache is used when K > 1
+// E = electrons per chunk; the inverse-row cache is used when K > 1 (M, the ion count, is
+// not needed: the J1 terms come with the proposals)
 template <typename T>
 __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E) {
+  (void)M;
   MoveLds L;
   unsigned o = 0;
   const bool cache = K > 1;
   L.oR = lds_take(o, sizeof(double) * 3 * N);
-  L.oIon = lds_take(o, sizeof(double) * 3 * M);
-  L.oRng = lds_take(o, sizeof(double) * 4 * E);
   L.oRed = lds_take(o, sizeof(double) * kRed * 16);
   L.oInv = lds_take(o, sizeof(T) * n);
   L.oPsi = lds_take(o, sizeof(T) * 5 * n);
-  L.oBinv = lds_take(o, cache ? sizeof(double) * K * K : 0);
+  L.oBinv = lds_take(o, cache ? sizeof(double) * K * (K + 1) : 0);   // row stride K + 1
   L.oP = lds_take(o, sizeof(double) * (K + 4));
   L.oPart = lds_take(o, cache ? 0 : sizeof(double) * n);   // Sherman-Morrison scratch
   L.oRows = lds_take(o, sizeof(int) * K);
-  L.oSpec = lds_take(o, sizeof(int) * M);
-  L.oFn = lds_take(o, sizeof(Functor<T>) * (kMaxSpecies + 2));
+  L.oAl = lds_take(o, sizeof(int) * E);                    // accepted moves of the chunk
+  L.oJ1n = lds_take(o, sizeof(T) * 5 * E);                 // their new J1 state
+  L.oFn = lds_take(o, sizeof(Functor<T>) * 2);             // J2 functors (same / opposite spin)
   L.oCell = lds_take(o, sizeof(Cell<T>));
-  L.oSpoG = lds_take(o, sizeof(double) * 12);
+  L.oProp = lds_take(o, sizeof(double) * kProp * E);        // the chunk's proposal records
+  L.oJ1o = lds_take(o, sizeof(T) * 4 * E);                  // J1 value and gradient of its electrons
+  L.oJt = lds_take(o, sizeof(T) * 5 * N);                   // J2 pair terms at the current proposal
   L.oJ2 = lds_take(o, sizeof(T) * 5 * N);
   L.oAc = lds_take(o, cache ? sizeof(T) * (size_t)E * n : 0);
   L.oQ = lds_take(o, cache ? sizeof(double) * (size_t)K * E : 0);
@@ -83,17 +86,49 @@ __device__ inline void j2_pair(const Cell<T>& C, const Functor<T>& f, double xj,
 __device__ inline void wave_lds_fence() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
 #ifdef QMCX_PHASE_TIMING
-// wall_clock64 ticks per phase, accumulated by lane 0 of workgroup 0
-__device__ unsigned long long qmcx_phase_ticks[12];
+// wall_clock64 ticks per phase, taken by lane 0 of workgroup 0 into registers and added to
+// this table when the kernel ends (a global read-modify-write per tick would time itself)
+constexpr int kPhaseSlots = 16;
+__device__ unsigned long long qmcx_phase_ticks[kPhaseSlots];
 #define QMCX_TICK(slot)                                                  \
   if (w == 0 && tid == 0) {                                              \
     const unsigned long long now_ = wall_clock64();                      \
-    qmcx_phase_ticks[slot] += now_ - tick_;                              \
+    tacc_[slot] += now_ - tick_;                                         \
     tick_ = now_;                                                        \
   }
+#define QMCX_TICK_FLUSH()                                                \
+  if (w == 0 && tid == 0)                                                \
+    for (int i_ = 0; i_ < kPhaseSlots; ++i_) qmcx_phase_ticks[i_] += tacc_[i_];
 #else
 #define QMCX_TICK(slot)
+#define QMCX_TICK_FLUSH()
 #endif
+
+// J2 pair terms of the items j = tid + q*nt against the electron iel at (x, y, z); items
+// past N and j == iel give zeros.  Identical arithmetic to j2_pair for every valid item; the
+// functor is evaluated without a branch on the cutoff.
+template <typename T, int JPT>
+__device__ inline void j2_items(const Cell<T>& C, const Functor<T>* fn, const double* sR, int N, int n, int iel,
+                                int tid, int nt, double x, double y, double z, T (&u)[JPT], T (&du)[JPT],
+                                T (&d2u)[JPT], T (&dx)[JPT], T (&dy)[JPT], T (&dz)[JPT]) {
+  const bool up = iel < n;
+#pragma unroll
+  for (int q = 0; q < JPT; ++q) {
+    u[q] = du[q] = d2u[q] = dx[q] = dy[q] = dz[q] = T(0);
+    const int j = tid + q * nt;
+    if (j < N) {
+      const Functor<T>& f = fn[((j < n) == up) ? 0 : 1];
+      const T r = min_image(C, (T)(sR[j] - x), (T)(sR[N + j] - y), (T)(sR[2 * N + j] - z), j < iel ? T(1) : T(-1),
+                            dx[q], dy[q], dz[q]);
+      const bool in = j != iel && r < f.rcut;
+      T a, b, c2;
+      functor_vgl(f, in ? r : f.rcut * T(0.5), a, b, c2);   // an interior radius keeps the lookup in range
+      u[q] = in ? a : T(0);
+      du[q] = in ? b : T(0);
+      d2u[q] = in ? c2 : T(0);
+    }
+  }
+}
 
 // KD: delayed updates (K > 1: inverse-row cache, Woodbury bordering) or Sherman-Morrison (K = 1)
 template <typename T, int JPT, bool KD>
@@ -101,13 +136,12 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
-  const int N = S.N, n = S.n, M = S.M, K = KD ? B.K : 1;
+  const int N = S.N, n = S.n, K = KD ? B.K : 1;
+  const int KP = K + 1;   // LDS row stride of Binv: a column walk down its rows is free of bank conflicts
   const int E = a.e_end - a.e_begin;
   const int r0 = a.e_begin - a.det * n;
-  const MoveLds L = move_lds<T>(N, M, n, K, E);
+  const MoveLds L = move_lds<T>(N, S.M, n, K, E);
   double* sR = (double*)(smem + L.oR);
-  double* sIon = (double*)(smem + L.oIon);
-  double* sRng = (double*)(smem + L.oRng);
   double* sRed = (double*)(smem + L.oRed);
   T* sInv = (T*)(smem + L.oInv);
   T* sPsi = (T*)(smem + L.oPsi);
@@ -115,15 +149,19 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   double* sP = (double*)(smem + L.oP);
   double* sTm = (double*)(smem + L.oPart);
   int* sRows = (int*)(smem + L.oRows);
-  int* sSpec = (int*)(smem + L.oSpec);
+  int* sAl = (int*)(smem + L.oAl);
+  T* sJ1n = (T*)(smem + L.oJ1n);
   Functor<T>* sFn = (Functor<T>*)(smem + L.oFn);
   Cell<T>* sCell = (Cell<T>*)(smem + L.oCell);
-  double* sSpoG = (double*)(smem + L.oSpoG);
+  double* sProp = (double*)(smem + L.oProp);
+  T* sJ1o = (T*)(smem + L.oJ1o);
+  T* sJt = (T*)(smem + L.oJt);
   T* sJ2 = (T*)(smem + L.oJ2);
   T* sAc = (T*)(smem + L.oAc);
   double* sQ = (double*)(smem + L.oQ);
 #ifdef QMCX_PHASE_TIMING
   unsigned long long tick_ = wall_clock64();
+  unsigned long long tacc_[kPhaseSlots] = {};
 #endif
   const size_t moff = ((size_t)w * 2 + a.det) * n * n;
   T* Ainv = B.Ainv + moff;
@@ -133,52 +171,39 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   T* J2 = B.J2 + (size_t)w * 5 * N;
   T* J1 = B.J1 + (size_t)w * 5 * N;
   T* Vb = B.Vbuf + (size_t)w * K * n;
-  const int sweep = B.sweep[0];
-  // precomputed SPO rows of this chunk's proposals: [e][5][npad]
+  // this chunk's proposals: SPO rows [e][5][npad] and records [e][kProp]
   const T* spo = (const T*)a.spo + (((size_t)w * 2 + a.det) * n + r0) * 5 * S.npad;
+  const double* prop = a.prop + (((size_t)w * 2 + a.det) * n + r0) * kProp;
 
-  // ---- chunk set-up: walker state into LDS
-  for (int i = tid; i < 3 * N; i += nt) sR[i] = B.R[(size_t)w * 3 * N + i];
-  for (int i = tid; i < 5 * N; i += nt) sJ2[i] = J2[i];
-  for (int i = tid; i < 3 * M; i += nt) sIon[i] = S.ions[i];
-  for (int i = tid; i < M; i += nt) sSpec[i] = S.species[i];
+  // ---- chunk set-up: walker state, the chunk's proposal records and J1 state, J2 functors and
+  // the cell into LDS (one round of loads)
   {
-    const int words = (int)(sizeof(Functor<T>) * (kMaxSpecies + 2) / 4);
-    for (int i = tid; i < words; i += nt) ((int*)sFn)[i] = ((const int*)S.fn)[i];
+    const unsigned je = (unsigned)(sizeof(T) * E);
+    const LdsCopy cp[10] = {{sR, B.R + (size_t)w * 3 * N, (unsigned)(sizeof(double) * 3 * N)},
+                            {sJ2, J2, (unsigned)(sizeof(T) * 5 * N)},
+                            {sAc, Ainv + (size_t)r0 * n, KD ? (unsigned)(sizeof(T) * E * n) : 0u},
+                            {sProp, prop, (unsigned)(sizeof(double) * kProp * E)},
+                            {sJ1o, J1 + a.e_begin, je},
+                            {sJ1o + E, J1 + N + a.e_begin, je},
+                            {sJ1o + 2 * E, J1 + 2 * N + a.e_begin, je},
+                            {sJ1o + 3 * E, J1 + 3 * N + a.e_begin, je},
+                            {sFn, S.fn + kMaxSpecies, (unsigned)(sizeof(Functor<T>) * 2)},
+                            {sCell, S.cell, (unsigned)sizeof(Cell<T>)}};
+    stage_lds(cp);
   }
-  if (tid == 0) *sCell = *S.cell;
-  if (tid < 9) sSpoG[tid] = S.spoG[tid];
-  if (KD)
-    for (size_t i = tid; i < (size_t)E * n; i += nt) sAc[i] = Ainv[(size_t)r0 * n + i];
-  for (int e = tid; e < E; e += nt) {
-    double nr[3], u;
-    move_randoms(a.seed, (uint64_t)(a.walker0 + w), (uint64_t)sweep, (uint64_t)(a.e_begin + e), nr, u);
-    // the move's displacement and acceptance uniform; a caller's displacement and accept
-    // decision (the flex stage calls) take their place, the decision as -inf (accept) / +inf
-    const bool given = a.displ != nullptr;
-    sRng[e * 4 + 0] = given ? a.displ[3 * w + 0] : a.sqrt_tau * nr[0];
-    sRng[e * 4 + 1] = given ? a.displ[3 * w + 1] : a.sqrt_tau * nr[1];
-    sRng[e * 4 + 2] = given ? a.displ[3 * w + 2] : a.sqrt_tau * nr[2];
-    sRng[e * 4 + 3] = a.decide == nullptr ? u : (a.decide[w] != 0 ? -INFINITY : INFINITY);
-  }
-  int c = 0;
-  double logd = 0, ph = 0, jl1 = 0, jl2 = 0;
+  // a caller's accept decision (the flex stage calls) replaces the uniform: -inf accepts, +inf rejects
+  const double flex_u = a.decide == nullptr ? 0.0 : (a.decide[w] != 0 ? -INFINITY : INFINITY);
+  int c = 0, ca = 0;   // delayed rows; accepted moves of the chunk
+  double logd = 0, pdet = 1, ph = 0, jl1 = 0, jl2 = 0;
   unsigned long long nacc = 0;
   __syncthreads();
-  const Cell<T> C = *sCell;   // in registers for the minimum-image searches
+  const Cell<T>& C = *sCell;   // read from LDS where used (a register copy costs spills)
   // Rows a move reads from HBM that no earlier move of the chunk changes: the moving electron's
-  // J1 state, its dpsiM row and the SPO row at its proposal.  Move e+1's are loaded at the start
-  // of move e (software pipelining; index clamped, so the loads are unconditional).
+  // dpsiM row and the SPO row of its proposal.  Move e+1's are loaded at the start of move e
+  // (software pipelining; index clamped, so the loads are unconditional).
   const int jr = tid < n ? tid : n - 1;
-  T nJ1[4], ndp[3], np5[5];
-  double nj1p[5];
-  const double* j1pw = a.j1p + (((size_t)w * 2 + a.det) * n + r0) * 8;
+  T ndp[3], np5[5];
   auto prefetch = [&](int ee) {
-    const int ie = a.e_begin + ee;
-    nJ1[0] = J1[ie];
-    nJ1[1] = J1[N + ie];
-    nJ1[2] = J1[2 * N + ie];
-    nJ1[3] = J1[3 * N + ie];
     const T* dq = dpsiM + (size_t)(r0 + ee) * 3 * n;
     ndp[0] = dq[jr];
     ndp[1] = dq[n + jr];
@@ -186,33 +211,28 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     const T* pq = spo + (size_t)ee * 5 * S.npad;
 #pragma unroll
     for (int o = 0; o < 5; ++o) np5[o] = pq[(size_t)o * S.npad + jr];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) nj1p[k] = j1pw[(size_t)ee * 8 + k];
   };
   prefetch(0);
 
   for (int e = 0; e < E; ++e) {
     const int iel = a.e_begin + e, row = r0 + e;
     QMCX_TICK(0)
-    // ---- phase 1: issue this move's global loads, then getInvRow's small algebra
-    const double x0 = sR[iel], y0 = sR[N + iel], z0 = sR[2 * N + iel];
-    const double nx = x0 + sRng[e * 4 + 0];
-    const double ny = y0 + sRng[e * 4 + 1];
-    const double nz = z0 + sRng[e * 4 + 2];
-    // stored state of the moving electron (J1 in HBM, J2 in LDS), its dpsiM / inverse rows and
-    // the SPO row at its proposal, read before any update; unconditional loads (clamped
-    // index), so that no branch merge forces an early wait on them
-    const T Vat_old = nJ1[0], g1x = nJ1[1], g1y = nJ1[2], g1z = nJ1[3];
+    // ---- phase 1: issue the next move's global loads, then getInvRow's small algebra
+    // stored state of the moving electron (J1, J2), its dpsiM row (loaded one move ahead) and
+    // its proposal record: J1 at the proposal (sum u, sum (d2u + 2 du/r), gradient),
+    // displacement, uniform
+    const T Vat_old = sJ1o[e], g1x = sJ1o[E + e], g1y = sJ1o[2 * E + e], g1z = sJ1o[3 * E + e];
     const T Uat_old = sJ2[iel], g2x = sJ2[N + iel], g2y = sJ2[2 * N + iel], g2z = sJ2[3 * N + iel];
     const T dp0 = ndp[0], dp1 = ndp[1], dp2 = ndp[2];
     T p5[5];
 #pragma unroll
     for (int o = 0; o < 5; ++o) p5[o] = np5[o];
-    // J1 at the proposal: sum u, sum (d2u + 2 du/r), gradient (precomputed with the SPO row)
-    double j1v[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) j1v[k] = nj1p[k];
+    const double* pr = sProp + (size_t)e * kProp;
     prefetch(e + 1 < E ? e + 1 : e);
+    const double x0 = sR[iel], y0 = sR[N + iel], z0 = sR[2 * N + iel];
+    const double nx = x0 + pr[kPropDispl + 0];
+    const double ny = y0 + pr[kPropDispl + 1];
+    const double nz = z0 + pr[kPropDispl + 2];
     T inv_sm = 0;
     if constexpr (!KD) inv_sm = Ainv[(size_t)row * n + jr];   // changed by the previous accept
     const T* prow = spo + (size_t)e * 5 * S.npad;
@@ -222,23 +242,23 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
         double s = 0;
         int q = 0;
         for (; q + 4 <= c; q += 4) {
-          const double b0 = sBinv[q * K + r], b1 = sBinv[(q + 1) * K + r], b2 = sBinv[(q + 2) * K + r],
-                       b3 = sBinv[(q + 3) * K + r];
+          const double b0 = sBinv[q * KP + r], b1 = sBinv[(q + 1) * KP + r], b2 = sBinv[(q + 2) * KP + r],
+                       b3 = sBinv[(q + 3) * KP + r];
           const double p0 = sQ[q * E + e], p1 = sQ[(q + 1) * E + e], p2 = sQ[(q + 2) * E + e], p3 = sQ[(q + 3) * E + e];
           s += b0 * p0;
           s += b1 * p1;
           s += b2 * p2;
           s += b3 * p3;
         }
-        for (; q < c; ++q) s += sBinv[q * K + r] * sQ[q * E + e];
-        sBinv[c * K + r] = s;
+        for (; q < c; ++q) s += sBinv[q * KP + r] * sQ[q * E + e];
+        sBinv[c * KP + r] = s;
       }
     }
     lds_sync();
     QMCX_TICK(1)
 
     // ---- phase 2: invRow, evalGrad of the determinant and ratioGrad_compute against the
-    // proposal's SPO row (einspline_spo.hpp:204-223, DiracDeterminant.cpp:103-124), J1/J2 at
+    // proposal's SPO row (einspline_spo.hpp:204-223, DiracDeterminant.cpp:103-124), J2 at
     // the proposal; all partial sums reduced in one round
     double gd0 = 0, gd1 = 0, gd2 = 0, d0 = 0, d1 = 0, d2 = 0, d3 = 0;
     T vinv = 0;
@@ -246,7 +266,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       double v;
       if (KD) {
         v = (double)sAc[(size_t)e * n + tid];
-        const double* y = sBinv + c * K;
+        const double* y = sBinv + c * KP;
         // four delayed rows' LDS reads in flight at a time; subtraction order unchanged
         int d = 0;
         for (; d + 4 <= c; d += 4) {
@@ -279,7 +299,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       // only when n exceeds the workgroup (not the case for the shipped systems)
       double v = KD ? (double)sAc[(size_t)e * n + j] : (double)Ainv[(size_t)row * n + j];
       if (KD)
-        for (int d = 0; d < c; ++d) v -= (double)sAc[(size_t)(sRows[d] - r0) * n + j] * sBinv[c * K + d];
+        for (int d = 0; d < c; ++d) v -= (double)sAc[(size_t)(sRows[d] - r0) * n + j] * sBinv[c * KP + d];
       sInv[j] = (T)v;
       const T* dp = dpsiM + (size_t)row * 3 * n;
       gd0 += v * (double)dp[j];
@@ -299,31 +319,41 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     }
     QMCX_TICK(5)
     {
+      // pair terms at the proposal: u, du dx, du dy, du dz, d2u + 2 du, parked in LDS for the
+      // J2 update if the move is accepted
+      T cu[JPT], cdu[JPT], cd2u[JPT], cdx[JPT], cdy[JPT], cdz[JPT];
+      j2_items<T, JPT>(C, sFn, sR, N, n, iel, tid, nt, nx, ny, nz, cu, cdu, cd2u, cdx, cdy, cdz);
       T r7 = 0, r8 = 0, r9 = 0, r10 = 0, r11 = 0;
-      const bool up = iel < n;
 #pragma unroll
       for (int q = 0; q < JPT; ++q) {
+        const T t1 = cdu[q] * cdx[q], t2 = cdu[q] * cdy[q], t3 = cdu[q] * cdz[q], t4 = cd2u[q] + T(2) * cdu[q];
+        r7 += cu[q];
+        r8 += t1;
+        r9 += t2;
+        r10 += t3;
+        r11 += t4;
         const int j = tid + q * nt;
-        if (j < N && j != iel) {
-          const Functor<T>& f = sFn[kMaxSpecies + (((j < n) == up) ? 0 : 1)];
-          T u, du, d2u, dx, dy, dz;
-          j2_pair(C, f, sR[j], sR[N + j], sR[2 * N + j], nx, ny, nz, j < iel ? T(1) : T(-1), u, du, d2u, dx, dy, dz);
-          r7 += u;
-          r8 += du * dx;
-          r9 += du * dy;
-          r10 += du * dz;
-          r11 += d2u + T(2) * du;
+        if (j < N) {
+          sJt[j] = cu[q];
+          sJt[N + j] = t1;
+          sJt[2 * N + j] = t2;
+          sJt[3 * N + j] = t3;
+          sJt[4 * N + j] = t4;
         }
       }
       QMCX_TICK(6)
-      QMCX_TICK(7)
-      // slots: gnow 0-2, det ratio 3, its gradient 4-6, J2 7-11
+      // slots: gnow 0-2, det ratio 3, its gradient 4-6, J2 7-11; after the row-wise wave sums
+      // v[i] holds slot i + 3r in the lanes of row r, which lane 16r + i stores
       double v[kRed] = {gd0, gd1, gd2, d0, d1, d2, d3, r7, r8, r9, r10, r11};
+      wave_sums_rows<kRed>(v);
+      const int k = lane & 15;
+      if (k < kRed / 4) {
+        double x = v[0];
 #pragma unroll
-      for (int i = 0; i < kRed; ++i) v[i] = wave_sum(v[i]);
-      if (lane == 0)
-#pragma unroll
-        for (int i = 0; i < kRed; ++i) sRed[i * 16 + wid] = v[i];
+        for (int i = 1; i < kRed / 4; ++i) x = k == i ? v[i] : x;
+        sRed[(k + (lane >> 4) * (kRed / 4)) * 16 + wid] = x;
+      }
+      QMCX_TICK(7)
     }
     lds_sync();
     QMCX_TICK(2)
@@ -334,8 +364,8 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
 
     // ---- phase 4: acceptance and updates
     const double rdet = red[3];
-    const double ratio = rdet * exp((double)(Vat_old - (T)j1v[0])) * exp((double)(Uat_old - (T)red[7]));
-    const double uacc = sRng[e * 4 + 3];
+    const double ratio = rdet * exp((double)(Vat_old - (T)pr[kPropJ1]) + (double)(Uat_old - (T)red[7]));
+    const double uacc = a.decide == nullptr ? pr[kPropU] : flex_u;
     const bool accept = a.mode == 0 ? (uacc < 0.5) : (uacc < ratio * ratio);
     if (B.trace != nullptr && tid == 0) {
       double* t = B.trace + ((size_t)w * N + iel) * 9;
@@ -345,27 +375,33 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       t[3] = red[0] + (double)g1x + (double)g2x;
       t[4] = red[1] + (double)g1y + (double)g2y;
       t[5] = red[2] + (double)g1z + (double)g2z;
-      t[6] = red[4] / rdet + j1v[2] + red[8];
-      t[7] = red[5] / rdet + j1v[3] + red[9];
-      t[8] = red[6] / rdet + j1v[4] + red[10];
+      t[6] = red[4] / rdet + pr[kPropJ1 + 2] + red[8];
+      t[7] = red[5] / rdet + pr[kPropJ1 + 3] + red[9];
+      t[8] = red[6] / rdet + pr[kPropJ1 + 4] + red[10];
     }
     if (accept) {
       if (tid == 0) {
-        logd += log(fabs(rdet));
+        // log|det| of the accepted ratios as a running product, folded into the log before it
+        // can leave the double range
+        pdet *= fabs(rdet);
+        if (!(pdet > 0x1p-500 && pdet < 0x1p500)) {
+          logd += log(pdet);
+          pdet = 1.0;
+        }
         ph += rdet > 0.0 ? 0.0 : M_PI;
-        jl1 += (double)(Vat_old - (T)j1v[0]);
+        jl1 += (double)(Vat_old - (T)pr[kPropJ1]);
         jl2 += (double)(Uat_old - (T)red[7]);
         ++nacc;
+        // the new J1 state of the electron (OneBodyJastrow.h:271-283), written at the chunk's end
+        sAl[ca] = e;
+        sJ1n[ca * 5 + 0] = (T)pr[kPropJ1 + 0];
+        sJ1n[ca * 5 + 1] = (T)pr[kPropJ1 + 2];
+        sJ1n[ca * 5 + 2] = (T)pr[kPropJ1 + 3];
+        sJ1n[ca * 5 + 3] = (T)pr[kPropJ1 + 4];
+        sJ1n[ca * 5 + 4] = (T)pr[kPropJ1 + 1];
       }
-      // new Slater row and its derivatives (DiracDeterminant.cpp:135-141); stale inverse row for the flush
-      for (int j = tid; j < n; j += nt) {
-        Amat[(size_t)row * n + j] = sPsi[j];
-        dpsiM[(size_t)row * 3 * n + j] = sPsi[n + j];
-        dpsiM[(size_t)row * 3 * n + n + j] = sPsi[2 * n + j];
-        dpsiM[(size_t)row * 3 * n + 2 * n + j] = sPsi[3 * n + j];
-        d2psiM[(size_t)row * n + j] = sPsi[4 * n + j];
-        if (KD) Vb[(size_t)c * n + j] = sAc[(size_t)e * n + j];
-      }
+      ++ca;
+      QMCX_TICK(12)
       if (!KD) {
         // Fahy/Sherman-Morrison (DelayedUpdate.h:166-173):
         // Ainv[m] -= ((Ainv[m].psiV - [m == row]) / ratio) * Ainv_old[row]
@@ -380,11 +416,11 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
             for (int r = 0; r < 4; ++r)
               if (r < nr) sm[r] += (double)am[(size_t)r * n + j] * pv;
           }
-#pragma unroll
-          for (int r = 0; r < 4; ++r) sm[r] = wave_sum(sm[r]);
-          if (lane < nr) {
-            const double mine = lane == 0 ? sm[0] : lane == 1 ? sm[1] : lane == 2 ? sm[2] : sm[3];
-            tm[m0 + lane] = (mine - (m0 + lane == row ? 1.0 : 0.0)) / rdet;
+          wave_sums_rows<4>(sm);
+          // row r of the wave holds the sum of row m0 + r
+          if ((lane & 15) == 0 && (lane >> 4) < nr) {
+            const int m = m0 + (lane >> 4);
+            tm[m] = (sm[0] - (m == row ? 1.0 : 0.0)) / rdet;
           }
         }
         lds_sync();
@@ -395,79 +431,94 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
         }
       } else {
         // acceptRow (DelayedUpdate.h:124-153): p_d = -V_d . psiV (d <= c) and, for the
-        // chunk's later rows r', U_c . Ainv[r'] used by their getInvRow
+        // chunk's later rows r', U_c . Ainv[r'] used by their getInvRow; four dot products per
+        // wave share the psiV loads.  Vector t2: delayed row t2 (t2 < c), this row (t2 == c),
+        // the chunk's row e + t2 - c (t2 > c).
         const int nd = c + 1 + (E - 1 - e);
-        for (int t2 = wid; t2 < nd; t2 += nw) {
-          const T* v = t2 <= c ? (t2 == c ? sAc + (size_t)e * n : sAc + (size_t)(sRows[t2] - r0) * n)
-                               : sAc + (size_t)(e + (t2 - c)) * n;
-          double s = 0;
-          for (int j = lane; j < n; j += 64) s += (double)v[j] * (double)sPsi[j];
-          s = wave_sum(s);
-          if (lane == 0) {
-            if (t2 <= c) sP[t2] = -s;
-            else sQ[c * E + e + (t2 - c)] = s;
+        auto dvec = [&](int t2) -> const T* {
+          const int ee = t2 < c ? sRows[t2] - r0 : e + (t2 - c);
+          return sAc + (size_t)ee * n;
+        };
+        for (int t0 = wid * 4; t0 < nd; t0 += nw * 4) {
+          const T* v0 = dvec(t0);
+          const T* v1 = dvec(min(t0 + 1, nd - 1));
+          const T* v2 = dvec(min(t0 + 2, nd - 1));
+          const T* v3 = dvec(min(t0 + 3, nd - 1));
+          double sv[4] = {0, 0, 0, 0};
+          for (int j = lane; j < n; j += 64) {
+            const double pv = (double)sPsi[j];
+            sv[0] += (double)v0[j] * pv;
+            sv[1] += (double)v1[j] * pv;
+            sv[2] += (double)v2[j] * pv;
+            sv[3] += (double)v3[j] * pv;
+          }
+          wave_sums_rows<4>(sv);
+          // row r of the wave holds dot t0 + r
+          const int t2 = t0 + (lane >> 4);
+          if ((lane & 15) == 0 && t2 < nd) {
+            if (t2 <= c) sP[t2] = -sv[0];
+            else sQ[c * E + e + (t2 - c)] = sv[0];
           }
         }
         lds_sync();
-        if (wid == 0) {
-          // the capacitance inverse grows by one row and column, FP64, one wave
-          double y = -sP[c];
-          for (int i = 0; i < c; ++i) y += sBinv[c * K + i] * sP[i];
-          const double yi = 1.0 / y;
-          for (int s2 = lane; s2 < c; s2 += 64) {
-            double t = 0;
-            for (int r = 0; r < c; ++r) t += sBinv[s2 * K + r] * sP[r];
-            sBinv[s2 * K + c] = yi * t;
+        QMCX_TICK(13)
+        if (wid == nw - 1) {
+          // the capacitance inverse grows by one row and column (FP64, the last wave, which
+          // has the least J2 work): z = row c (parked by phase 1), y = z.p - p_c,
+          // Binv[s][c] = (Binv[s].p) / y, Binv[s][r] -= z_r Binv[s][c], row c = -z / y
+          double* z = sBinv + c * KP;
+          double t = 0;
+          for (int i = lane; i < c; i += 64) t += z[i] * sP[i];
+          const double yi = 1.0 / (wave_sum(t) - sP[c]);
+          const int h = lane >> 5;   // half-waves take alternate columns r
+          for (int s0 = 0; s0 < c; s0 += 32) {
+            const int s2 = s0 + (lane & 31);
+            double* bs = sBinv + (s2 < c ? s2 : 0) * KP;
+            double acc = 0;
+            if (s2 < c)
+              for (int r = h; r < c; r += 2) acc += bs[r] * sP[r];
+            acc += __shfl_xor(acc, 32);
+            const double ys = yi * acc;
+            if (s2 < c) {
+              for (int r = h; r < c; r += 2) bs[r] -= z[r] * ys;
+              if (h == 0) bs[c] = ys;
+            }
           }
           wave_lds_fence();
-          for (int s2 = lane; s2 < c; s2 += 64) {
-            const double bsc = sBinv[s2 * K + c];
-            for (int r = 0; r < c; ++r) sBinv[s2 * K + r] -= sBinv[c * K + r] * bsc;
-          }
-          wave_lds_fence();
-          for (int i = lane; i < c; i += 64) sBinv[c * K + i] *= -yi;
+          for (int i = lane; i < c; i += 64) z[i] *= -yi;
           if (lane == 0) {
-            sBinv[c * K + c] = yi;
+            z[c] = yi;
             sRows[c] = row;
           }
         }
         ++c;
+        QMCX_TICK(14)
       }
-      // J1 (OneBodyJastrow.h:271-283) and J2 (TwoBodyJastrow.h:307-353); the pair
-      // terms at the old and the accepted position are recomputed here
-      const bool up = iel < n;
+      // J2 (TwoBodyJastrow.h:307-353): the pair terms at the old position are recomputed here,
+      // those at the accepted one are phase 2's
+      T ou[JPT], odu[JPT], od2u[JPT], odx[JPT], ody[JPT], odz[JPT];
+      j2_items<T, JPT>(C, sFn, sR, N, n, iel, tid, nt, x0, y0, z0, ou, odu, od2u, odx, ody, odz);
 #pragma unroll
       for (int q = 0; q < JPT; ++q) {
         const int j = tid + q * nt;
         if (j < N && j != iel) {
-          const Functor<T>& f = sFn[kMaxSpecies + (((j < n) == up) ? 0 : 1)];
-          const T flip = j < iel ? T(1) : T(-1);
-          const double xj = sR[j], yj = sR[N + j], zj = sR[2 * N + j];
-          T ou, odu, od2u, odx, ody, odz, cu, cdu, cd2u, cdx, cdy, cdz;
-          j2_pair(C, f, xj, yj, zj, x0, y0, z0, flip, ou, odu, od2u, odx, ody, odz);
-          j2_pair(C, f, xj, yj, zj, nx, ny, nz, flip, cu, cdu, cd2u, cdx, cdy, cdz);
-          const T newl = cd2u + T(2) * cdu;
-          sJ2[j] += cu - ou;
-          sJ2[4 * N + j] += od2u + T(2) * odu - newl;
-          sJ2[N + j] -= cdu * cdx - odu * odx;
-          sJ2[2 * N + j] -= cdu * cdy - odu * ody;
-          sJ2[3 * N + j] -= cdu * cdz - odu * odz;
+          sJ2[j] += sJt[j] - ou[q];
+          sJ2[4 * N + j] += od2u[q] + T(2) * odu[q] - sJt[4 * N + j];
+          sJ2[N + j] -= sJt[N + j] - odu[q] * odx[q];
+          sJ2[2 * N + j] -= sJt[2 * N + j] - odu[q] * ody[q];
+          sJ2[3 * N + j] -= sJt[3 * N + j] - odu[q] * odz[q];
         } else if (j == iel) {
           sJ2[iel] = (T)red[7];
           sJ2[N + iel] = (T)red[8];
           sJ2[2 * N + iel] = (T)red[9];
           sJ2[3 * N + iel] = (T)red[10];
           sJ2[4 * N + iel] = (T)(-red[11]);
-          J1[iel] = (T)j1v[0];
-          J1[N + iel] = (T)j1v[2];
-          J1[2 * N + iel] = (T)j1v[3];
-          J1[3 * N + iel] = (T)j1v[4];
-          J1[4 * N + iel] = (T)j1v[1];
           sR[iel] = nx;
           sR[N + iel] = ny;
           sR[2 * N + iel] = nz;
         }
       }
+      QMCX_TICK(15)
     }
     // with K = 1 the next move reads rows of Ainv that other waves just wrote
     if (!KD) __syncthreads();
@@ -475,10 +526,57 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     QMCX_TICK(4)
   }
 
+  // ---- chunk end: walker state back to HBM; the accepted rows' Slater matrix, gradient and
+  // Laplacian rows (DiracDeterminant.cpp:135-141) copied from the proposals' SPO rows, the
+  // stale inverse rows for the flush, the new J1 state
   for (int i = tid; i < 3 * N; i += nt) B.R[(size_t)w * 3 * N + i] = sR[i];
   for (int i = tid; i < 5 * N; i += nt) J2[i] = sJ2[i];
+  for (int d = tid; d < ca; d += nt) {
+    const int ie = a.e_begin + sAl[d];
+    J1[ie] = sJ1n[d * 5 + 0];
+    J1[N + ie] = sJ1n[d * 5 + 1];
+    J1[2 * N + ie] = sJ1n[d * 5 + 2];
+    J1[3 * N + ie] = sJ1n[d * 5 + 3];
+    J1[4 * N + ie] = sJ1n[d * 5 + 4];
+  }
+  if (KD)
+    for (int i = tid; i < ca * n; i += nt) {
+      const int d = i / n, j = i - d * n;
+      Vb[(size_t)d * n + j] = sAc[(size_t)sAl[d] * n + j];
+    }
+  {
+    using VT = typename V16<T>::type;
+    constexpr int VW = V16<T>::n;
+    const bool vec = n % VW == 0 && S.npad % VW == 0;
+    const int nv = vec ? n / VW : n;   // vectors (or scalars) per row
+    const int tot = ca * 5 * nv;
+    for (int b0 = 0; b0 < tot; b0 += 8 * nt) {
+      VT v[8];
+      T sv[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int i = min(b0 + k * nt + tid, tot - 1);
+        const int d = i / (5 * nv), rem = i - d * 5 * nv, o = rem / nv, j = rem - o * nv;
+        const T* src = spo + (size_t)sAl[d] * 5 * S.npad + (size_t)o * S.npad;
+        if (vec) v[k] = ((const VT*)src)[j];
+        else sv[k] = src[j];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int i = b0 + k * nt + tid;
+        if (i < tot) {
+          const int d = i / (5 * nv), rem = i - d * 5 * nv, o = rem / nv, j = rem - o * nv;
+          const int rw = r0 + sAl[d];
+          T* dst = o == 0 ? Amat + (size_t)rw * n
+                          : (o < 4 ? dpsiM + (size_t)rw * 3 * n + (size_t)(o - 1) * n : d2psiM + (size_t)rw * n);
+          if (vec) ((VT*)dst)[j] = v[k];
+          else dst[j] = sv[k];
+        }
+      }
+    }
+  }
   if (tid == 0) {
-    B.logdet[w * 2 + a.det] += logd;
+    B.logdet[w * 2 + a.det] += logd + log(pdet);
     B.phase[w * 2 + a.det] += ph;
     B.jlog[w * 2 + 0] += jl1;
     B.jlog[w * 2 + 1] += jl2;
@@ -486,9 +584,10 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     B.ndelay[w] = KD ? c : 0;
   }
   if (KD) {
-    for (int i = tid; i < c * K; i += nt) B.Binv[(size_t)w * K * K + i] = sBinv[i];
+    for (int i = tid; i < c * K; i += nt) B.Binv[(size_t)w * K * K + i] = sBinv[(i / K) * KP + i % K];
     for (int i = tid; i < c; i += nt) B.rows[(size_t)w * K + i] = sRows[i];
   }
+  QMCX_TICK_FLUSH()
 }
 
 }  // namespace qmcx
