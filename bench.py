@@ -54,14 +54,14 @@ def path_bytes_per_move(sysd, k, acc, s=4):
 
 def move_kernel_bytes_per_move(sysd, k, acc, s=4):
     """Algorithmic HBM bytes per move of the fused move kernel (the spline stencil is read by
-    the proposals kernel): the proposal's SPO row 5*n*s and J1 terms (5 doubles), the moving
-    electron's dpsiM row 3*n*s and J1 state 4*s; on accept (fraction a) the new A, dpsiM,
-    d2psiM rows and the stale inverse row 6*n*s and the J1 state 5*s; per chunk of k moves the
-    walker's positions (3*N doubles) and J2 state (5*N*s) in and out and the chunk's stale
-    inverse rows k*n*s in."""
+    the proposals kernel): the proposal's SPO row 5*n*s and record (J1 terms, displacement,
+    uniform: 9 doubles), the moving electron's dpsiM row 3*n*s and J1 state 4*s; on accept
+    (fraction a) the new A, dpsiM, d2psiM rows and the stale inverse row 6*n*s and the J1 state
+    5*s; per chunk of k moves (64 with k = 1) the walker's positions (3*N doubles) and J2 state
+    (5*N*s) in and out and the chunk's stale inverse rows k*n*s in."""
     n, N = sysd.nup, sysd.nel
-    chunk = k if k > 1 else n
-    per = 5 * n * s + 5 * 8 + 3 * n * s + 4 * s + acc * (6 * n * s + 5 * s)
+    chunk = k if k > 1 else min(n, 64)
+    per = 5 * n * s + 9 * 8 + 3 * n * s + 4 * s + acc * (6 * n * s + 5 * s)
     per_chunk = 2 * (3 * N * 8 + 5 * N * s) + (chunk * n * s if k > 1 else 0)
     return per + per_chunk / chunk
 
@@ -230,7 +230,7 @@ def main():
     # sweep is also put against the path's algorithmic bytes.
     bpm_path, parts = path_bytes_per_move(sysd, k, acc_timed)
     bpm = move_kernel_bytes_per_move(sysd, k, acc_timed)
-    chunk = k if k > 1 else sysd.nup
+    chunk = k if k > 1 else min(sysd.nup, 64)
     launch_ms = mv_ms / max(mv_n, 1)
     achieved = bpm * W * chunk / (launch_ms * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(sysd, k, W * chunk)

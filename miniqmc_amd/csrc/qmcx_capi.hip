@@ -435,12 +435,31 @@ int evaluate_log_impl(qmcx_batch_t b) {
 // clipped); it can be switched off for comparisons
 template <typename T>
 bool use_mfma_flush(const qmcx_batch_t b) {
-  return sizeof(T) == 4 && b->sys->n % 16 == 0 && b->flush_kind == 0;
+  return sizeof(T) == 4 && flush_mfma_fits(b->sys->n) && b->flush_kind == 0;
 }
 
 // Electrons per move-kernel launch: the delay rank (one flush per chunk), or up to 64 with
 // Sherman-Morrison updates (K = 1), which bounds the chunk's LDS records.
 static int move_chunk(int n, int K) { return K == 1 ? std::min(n, 64) : K; }
+
+// updateInvMat of the delayed rows for rows [lo, hi) of determinant det
+template <typename T>
+int launch_flush(qmcx_batch_t b, hipStream_t st, int det, int lo, int hi) {
+  const qmcx_system_s* s = b->sys;
+  const FlushRows FR{lo, hi};
+  if (hi <= lo) return 0;
+  if (use_mfma_flush<T>(b)) {
+    const size_t lds = flush_mfma_lds(s->n, b->K);
+    hipLaunchKernelGGL(k_flush_mfma, dim3((hi - lo + kFlushBM - 1) / kFlushBM, b->W), dim3(64 * kFlushWaves), lds, st,
+                       s->params<float>(), b->params<float>(), det, FR);
+  } else {
+    const size_t lds = sizeof(T) * 2 * kFlushRows * b->K + sizeof(int) * b->K;
+    hipLaunchKernelGGL(k_flush_generic<T>, dim3((hi - lo + kFlushRows - 1) / kFlushRows, b->W), dim3(256), lds, st,
+                       s->params<T>(), b->params<T>(), det, FR);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
 
 // the moves (and delayed-update flushes) of one determinant's electrons in one sweep
 template <typename T>
@@ -448,33 +467,25 @@ int moves_det(qmcx_batch_t b, int det, int mode) {
   const qmcx_system_s* s = b->sys;
   int nt = 0;
   if (choose_shape(s, nt)) return QMCX_ERR_ARG;
-  const int chunk = move_chunk(s->n, b->K);
-  const MoveLds L = move_lds<T>(s->N, s->nions, s->n, b->K, chunk);
+  const int n = s->n;
+  const int chunk = move_chunk(n, b->K);
+  const MoveLds L = move_lds<T>(s->N, s->nions, n, b->K, chunk);
   MoveArgs a;
   a.mode = mode;
   a.slices = 0;
   a.spo = b->spo;
   a.prop = b->prop;
   a.decide = nullptr;
-  const size_t flds = sizeof(T) * 2 * kFlushRows * b->K + sizeof(int) * b->K;
-  for (int e = 0; e < s->n; e += chunk) {
+  for (int e = 0; e < n; e += chunk) {
     a.det = det;
-    a.e_begin = det * s->n + e;
-    a.e_end = det * s->n + std::min(s->n, e + chunk);
+    a.e_begin = det * n + e;
+    a.e_end = det * n + std::min(n, e + chunk);
     if (b->prof_record(0)) return QMCX_ERR_HIP;
     if (launch_moves<T>(b, nt, L.total, a)) return QMCX_ERR_HIP;
     if (b->prof_close()) return QMCX_ERR_HIP;
     if (b->K > 1) {
       if (b->prof_record(1)) return QMCX_ERR_HIP;
-      if (use_mfma_flush<T>(b)) {
-        const size_t lds = flush_mfma_lds(s->n, b->K);
-        hipLaunchKernelGGL(k_flush_mfma, dim3((s->n + kFlushBM - 1) / kFlushBM, b->W), dim3(256), lds, b->stream,
-                           s->params<float>(), b->params<float>(), det);
-      } else {
-        hipLaunchKernelGGL(k_flush_generic<T>, dim3((s->n + kFlushRows - 1) / kFlushRows, b->W), dim3(256), flds,
-                           b->stream, s->params<T>(), b->params<T>(), det);
-      }
-      HIPCHK(hipGetLastError());
+      if (int rc = launch_flush<T>(b, b->stream, det, 0, n)) return rc;
       if (b->prof_close()) return QMCX_ERR_HIP;
     }
   }
@@ -1071,20 +1082,9 @@ static int flex_launch(qmcx_batch_t b, int iel, bool apply) {
   if (rc) return rc;
   if (apply && b->K > 1) {
     // every flex move is its own chunk: its (at most one) delayed row is applied now
-    if (f32 && use_mfma_flush<float>(b)) {
-      const size_t lds = flush_mfma_lds(s->n, b->K);
-      hipLaunchKernelGGL(k_flush_mfma, dim3((s->n + kFlushBM - 1) / kFlushBM, b->W), dim3(256), lds, b->stream,
-                         s->params<float>(), b->params<float>(), a.det);
-    } else if (f32) {
-      const size_t lds = sizeof(float) * 2 * kFlushRows * b->K + sizeof(int) * b->K;
-      hipLaunchKernelGGL(k_flush_generic<float>, dim3((s->n + kFlushRows - 1) / kFlushRows, b->W), dim3(256), lds,
-                         b->stream, s->params<float>(), b->params<float>(), a.det);
-    } else {
-      const size_t lds = sizeof(double) * 2 * kFlushRows * b->K + sizeof(int) * b->K;
-      hipLaunchKernelGGL(k_flush_generic<double>, dim3((s->n + kFlushRows - 1) / kFlushRows, b->W), dim3(256), lds,
-                         b->stream, s->params<double>(), b->params<double>(), a.det);
-    }
-    HIPCHK(hipGetLastError());
+    rc = f32 ? launch_flush<float>(b, b->stream, a.det, 0, s->n)
+             : launch_flush<double>(b, b->stream, a.det, 0, s->n);
+    if (rc) return rc;
   }
   return 0;
 }
@@ -1229,14 +1229,14 @@ namespace {
 template <typename T>
 int du_flush(qmcx_du_t h) {
   if (h->count == 0) return 0;
-  if (sizeof(T) == 4 && h->n % 16 == 0) {
+  if (sizeof(T) == 4 && flush_mfma_fits(h->n)) {
     const size_t lds = flush_mfma_lds(h->n, h->K);
-    hipLaunchKernelGGL(k_flush_mfma, dim3((h->n + kFlushBM - 1) / kFlushBM, 1), dim3(256), lds, h->stream,
-                       h->sys<float>(), h->params<float>(), 0);
+    hipLaunchKernelGGL(k_flush_mfma, dim3((h->n + kFlushBM - 1) / kFlushBM, 1), dim3(64 * kFlushWaves), lds, h->stream,
+                       h->sys<float>(), h->params<float>(), 0, FlushRows{0, h->n});
   } else {
     const size_t lds = sizeof(T) * 2 * kFlushRows * h->K + sizeof(int) * h->K;
     hipLaunchKernelGGL(k_flush_generic<T>, dim3((h->n + kFlushRows - 1) / kFlushRows, 1), dim3(256), lds, h->stream,
-                       h->sys<T>(), h->params<T>(), 0);
+                       h->sys<T>(), h->params<T>(), 0, FlushRows{0, h->n});
   }
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_du_reset, dim3(1), dim3(64), 0, h->stream, h->ndelay);

@@ -198,10 +198,15 @@ namespace qmcx {
 //   T  = Ainv U^T - E          (U rows = new Slater rows, E_{m,d} = [m == rows_d])
 //   T' = T Binv
 //   Ainv -= T' V               (V rows = stale inverse rows)
+// A launch covers rows [lo, hi).
+struct FlushRows {
+  int lo, hi;
+  __device__ bool writes(int m) const { return m < hi; }
+};
 constexpr int kFlushRows = 16;
 template <typename T>
-__global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, int det) {
-  const int w = blockIdx.y, m0 = blockIdx.x * kFlushRows;
+__global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, int det, FlushRows FR) {
+  const int w = blockIdx.y, m0 = FR.lo + blockIdx.x * kFlushRows;
   const int c = B.ndelay[w];
   if (c == 0) return;
   const int n = S.n, K = B.K, tid = threadIdx.x, nt = blockDim.x;
@@ -220,7 +225,7 @@ __global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, i
   for (int idx = wid; idx < kFlushRows * c; idx += nw) {
     const int m = idx / c, d = idx - m * c;
     T s = 0;
-    if (m0 + m < n) {
+    if (m0 + m < FR.hi) {
       const T* ar = Ainv + (size_t)(m0 + m) * n;
       const T* ur = Amat + (size_t)sRows[d] * n;
       for (int j = lane; j < n; j += 64) s += ar[j] * ur[j];
@@ -247,121 +252,116 @@ __global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, i
     }
 #pragma unroll
     for (int m = 0; m < kFlushRows; ++m)
-      if (m0 + m < n) Ainv[(size_t)(m0 + m) * n + j] -= acc[m];
+      if (FR.writes(m0 + m)) Ainv[(size_t)(m0 + m) * n + j] -= acc[m];
   }
 }
 
-// FP32 flush on the matrix cores (v_mfma_f32_16x16x4f32), 32 rows per workgroup,
-// 4 waves.  16x16x4 operand layout: lane l supplies A[l%16][l/16] and
+// FP32 flush on the matrix cores (v_mfma_f32_16x16x4f32), 16 rows per wave, kFlushWaves
+// waves per workgroup (one workgroup per CU covers a walker's rows in one or two launches'
+// worth of workgroups).  16x16x4 operand layout: lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; accumulator register r of lane l is D[4*(l/16)+r][l%16].
-// The K index of GEMM 1 (orbitals) is permuted per lane group g = l/16: group g
-// walks columns [g*n/4, (g+1)*n/4) with float4 loads, four MFMAs per load, which
-// sums the same products in another order.
+//
+// A wave's 16 rows stay in registers between the two products, in the one layout both use:
+// lane (i = l%16, g = l/16) holds row r0+i, columns 16ct + 4g .. 16ct + 4g + 3 of every
+// 16-column tile ct (a float4 per tile; one HBM read of the rows, 64 contiguous bytes per
+// row per load).  Both products are computed transposed:
+//   GEMM 1:  T^T = U Ainv_rows^T     A = U (16 delayed rows x 4 orbitals), B = the rows
+//            (orbital 16ct + 4g + r of MFMA r is the k index g), D[d][i] = T[i][d]
+//   GEMM 2:  Ainv_rows^T += V^T T2^T  A = V^T (16 orbitals x 4 delayed rows), B = T2^T,
+//            accumulators started from the rows themselves
+// V (the stale inverse rows) is staged in LDS per workgroup when the delay rank allows.
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
-constexpr int kFlushBM = 64;        // rows per workgroup: 4 waves x 16 rows
-constexpr int kFlushMaxCT = 8;      // delayed rows up to 128 = 8 tiles of 16
-constexpr int kFlushBinvLds = 64;   // Binv staged in LDS up to this many delayed rows
-// LDS of k_flush_mfma: per wave T and T2 (16 x cp floats each), Binv (c x c doubles, when
-// c <= kFlushBinvLds), the row list
+constexpr int kFlushWaves = 12;     // waves per workgroup, 16 rows each
+constexpr int kFlushBM = 16 * kFlushWaves;   // rows per workgroup
+constexpr int kFlushTiles = 24;     // 16-column tiles held in registers: n <= 384
+constexpr int kFlushBinvLds = 64;   // Binv staged in LDS up to this delay rank
+constexpr int kFlushVLds = 32;      // V staged in LDS up to this delay rank
+__host__ __device__ inline bool flush_mfma_fits(int n) { return n % 16 == 0 && n <= 16 * kFlushTiles; }
+__host__ __device__ inline int flush_v_stride(int n) { return n + 16; }   // LDS row stride of V
+// LDS of k_flush_mfma: per wave T and T2 (16 x cp floats each), Binv (K x K doubles when
+// K <= kFlushBinvLds), V (cp x (n + 16) floats when K <= kFlushVLds), the row list
 __host__ __device__ inline size_t flush_mfma_lds(int n, int K) {
   const int cp = (K + 15) & ~15;
-  const int kb = K <= kFlushBinvLds ? K : 0;
-  (void)n;
-  return sizeof(float) * 4 * 2 * 16 * cp + sizeof(double) * kb * kb + sizeof(int) * K;
-}
-// GEMM 1 of the flush for NCT 16-column tiles of delayed rows: acc[ct] = Ainv_rows(16) . U^T;
-// the k loop is unrolled four times so that a dozen loads are in flight per wave
-template <int NCT>
-__device__ inline void flush_gemm1(const float* ap, const float* const* bp, bool aok, int li, int c, int q4,
-                                   f32x4_t* acc) {
-#pragma unroll
-  for (int ct = 0; ct < NCT; ++ct) acc[ct] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-  for (int k = 0; k < q4; k += 4) {
-    f32x4_t av = *reinterpret_cast<const f32x4_t*>(ap + k);
-    f32x4_t bv[NCT];
-#pragma unroll
-    for (int ct = 0; ct < NCT; ++ct) bv[ct] = *reinterpret_cast<const f32x4_t*>(bp[ct] + k);
-    if (!aok) av = f32x4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ct = 0; ct < NCT; ++ct) {
-      if (ct * 16 + li >= c) bv[ct] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-      acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[0], bv[ct][0], acc[ct], 0, 0, 0);
-      acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[1], bv[ct][1], acc[ct], 0, 0, 0);
-      acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[2], bv[ct][2], acc[ct], 0, 0, 0);
-      acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[3], bv[ct][3], acc[ct], 0, 0, 0);
-    }
-  }
+  const size_t kb = K <= kFlushBinvLds ? (size_t)(K * K + 1) & ~(size_t)1 : 0;   // keeps V 16-byte aligned
+  const size_t vb = K <= kFlushVLds ? (size_t)cp * flush_v_stride(n) : 0;
+  return sizeof(float) * kFlushWaves * 2 * 16 * cp + sizeof(double) * kb + sizeof(float) * vb + sizeof(int) * K;
 }
 
-template <int NCT>
-__device__ inline void flush_store_t(const f32x4_t* acc, const int* sRows, int r0, int g, int li, int c, int cp,
-                                     float* sT) {
-#pragma unroll
-  for (int ct = 0; ct < NCT; ++ct)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = 4 * g + r;             // row within the wave's 16
-      const int dd = ct * 16 + li;         // delayed-row index
-      float v = acc[ct][r];
-      if (dd < c && sRows[dd] == r0 + i) v -= 1.f;
-      sT[i * cp + dd] = dd < c ? v : 0.f;
-    }
-}
-
-// Each wave owns 16 rows of Ainv: T = Ainv_rows U^T - E on the matrix cores with
-// the rows read from HBM once, T2 = -T Binv in FP64, then Ainv_rows += T2 V with
-// the accumulators started from the rows again (now in L2).  U (the new Slater
-// rows) and V (the stale inverse rows) are shared by all waves of the walker.
-// Loads are unconditional (clamped indices, zeroed after) and batched, so each
-// wave keeps a dozen or more in flight.
-__global__ __launch_bounds__(256) void k_flush_mfma(SysP<float> S, BatchP<float> B, int det) {
+__global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, BatchP<float> B, int det,
+                                                                  FlushRows FR) {
   const int w = blockIdx.y;
   const int c = B.ndelay[w];
   if (c == 0) return;
   const int n = S.n, K = B.K, tid = threadIdx.x;
-  const int lane = tid & 63, wid = tid >> 6;
+  const int lane = tid & 63, wid = tid >> 6, li = lane & 15, g = lane >> 4;
   const int cp = (c + 15) & ~15;           // delayed rows padded to a 16-wide tile
-  const int nct = cp / 16;
-  const int r0 = blockIdx.x * kFlushBM + wid * 16;   // this wave's first row
+  const int nct = n / 16;                  // column tiles (n % 16 == 0, nct <= kFlushTiles)
+  const int r0 = FR.lo + blockIdx.x * kFlushBM + wid * 16;   // this wave's first row
+  const int KC = (K + 15) & ~15;           // LDS sizing uses the delay rank, not c
   extern __shared__ __align__(16) unsigned char smem[];
-  float* sT = (float*)smem + wid * 2 * 16 * cp;      // [16][cp]   T = Ainv U^T - E
-  float* sT2 = sT + 16 * cp;                          // [16][cp]  -T Binv
-  const bool binv_lds = K <= kFlushBinvLds;   // as sized by flush_mfma_lds (c <= K)
-  double* sB = (double*)((float*)smem + 4 * 2 * 16 * cp);   // [c][c] when binv_lds
-  int* sRows = (int*)(sB + (K <= kFlushBinvLds ? K * K : 0));
+  float* sT = (float*)smem + wid * 2 * 16 * KC;       // [16][cp]   T = Ainv U^T - E
+  float* sT2 = sT + 16 * KC;                           // [16][cp]  -T Binv
+  const bool binv_lds = K <= kFlushBinvLds;
+  const bool v_lds = K <= kFlushVLds;
+  double* sB = (double*)((float*)smem + kFlushWaves * 2 * 16 * KC);        // [c][c]
+  float* sV = (float*)(sB + (binv_lds ? (K * K + 1) & ~1 : 0));            // [cp][n + 16]
+  int* sRows = (int*)(sV + (v_lds ? (size_t)KC * flush_v_stride(n) : 0));
   const size_t moff = ((size_t)w * 2 + det) * n * n;
   float* Ainv = B.Ainv + moff;
   const float* Amat = B.A + moff;
   const float* Vb = B.Vbuf + (size_t)w * K * n;
   const double* Bi = B.Binv + (size_t)w * K * K;
-  for (int i = tid; i < c; i += 256) sRows[i] = B.rows[(size_t)w * K + i];
+  const int vs = flush_v_stride(n);
+  const int nt = blockDim.x;
+  for (int i = tid; i < c; i += nt) sRows[i] = B.rows[(size_t)w * K + i];
   if (binv_lds)
-    for (int i = tid; i < c * c; i += 256) sB[i] = Bi[(i / c) * K + i % c];
-  __syncthreads();
-  const int li = lane & 15, g = lane >> 4;
-  const int q4 = n / 4;                    // columns per lane group (n % 16 == 0)
-  // ---- GEMM 1: T[16][cp] = Ainv[r0:r0+16][:] . U^T, every column tile from one read of the rows
-  {
-    const int arow = r0 + li;
-    const bool aok = arow < n;             // the last block may run past row n-1
-    const float* ap = Ainv + (size_t)(aok ? arow : 0) * n + g * q4;
-    const float* bp[kFlushMaxCT];
-#pragma unroll
-    for (int ct = 0; ct < kFlushMaxCT; ++ct) {
-      const int d = ct * 16 + li;
-      bp[ct] = Amat + (size_t)(d < c ? sRows[d] : sRows[0]) * n + g * q4;
+    for (int i = tid; i < c * c; i += nt) sB[i] = Bi[(i / c) * K + i % c];
+  if (v_lds) {
+    // V rows, zero-padded to cp rows, float4 at a time
+    const int q = n / 4;
+    for (int i = tid; i < cp * q; i += nt) {
+      const int d = i / q, j = i - d * q;
+      f32x4_t v = *reinterpret_cast<const f32x4_t*>(Vb + (size_t)(d < c ? d : 0) * n + 4 * j);
+      if (d >= c) v = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      *reinterpret_cast<f32x4_t*>(sV + (size_t)d * vs + 4 * j) = v;
     }
-    f32x4_t acc[kFlushMaxCT];
-    switch (nct) {
-      case 1: flush_gemm1<1>(ap, bp, aok, li, c, q4, acc); flush_store_t<1>(acc, sRows, r0, g, li, c, cp, sT); break;
-      case 2: flush_gemm1<2>(ap, bp, aok, li, c, q4, acc); flush_store_t<2>(acc, sRows, r0, g, li, c, cp, sT); break;
-      case 3: flush_gemm1<3>(ap, bp, aok, li, c, q4, acc); flush_store_t<3>(acc, sRows, r0, g, li, c, cp, sT); break;
-      case 4: flush_gemm1<4>(ap, bp, aok, li, c, q4, acc); flush_store_t<4>(acc, sRows, r0, g, li, c, cp, sT); break;
-      case 5: flush_gemm1<5>(ap, bp, aok, li, c, q4, acc); flush_store_t<5>(acc, sRows, r0, g, li, c, cp, sT); break;
-      case 6: flush_gemm1<6>(ap, bp, aok, li, c, q4, acc); flush_store_t<6>(acc, sRows, r0, g, li, c, cp, sT); break;
-      case 7: flush_gemm1<7>(ap, bp, aok, li, c, q4, acc); flush_store_t<7>(acc, sRows, r0, g, li, c, cp, sT); break;
-      default: flush_gemm1<8>(ap, bp, aok, li, c, q4, acc); flush_store_t<8>(acc, sRows, r0, g, li, c, cp, sT); break;
+  }
+  // this wave's rows into registers (row clamped into the range, zeroed after)
+  const int row = r0 + li;
+  const bool rok = row < FR.hi;
+  const float* ar = Ainv + (size_t)(rok ? row : FR.lo) * n + 4 * g;
+  f32x4_t a[kFlushTiles];
+#pragma unroll
+  for (int ct = 0; ct < kFlushTiles; ++ct)
+    if (ct < nct) a[ct] = *reinterpret_cast<const f32x4_t*>(ar + ct * 16);
+#pragma unroll
+  for (int ct = 0; ct < kFlushTiles; ++ct)
+    if (!rok) a[ct] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  __syncthreads();
+  // ---- GEMM 1: T^T tile by tile of 16 delayed rows; two accumulators split the orbitals
+  for (int dt = 0; dt < cp / 16; ++dt) {
+    const int d = dt * 16 + li;
+    const float* ur = Amat + (size_t)(d < c ? sRows[d] : sRows[0]) * n + 4 * g;
+    f32x4_t acc0 = f32x4_t{0.f, 0.f, 0.f, 0.f}, acc1 = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ct = 0; ct < kFlushTiles; ++ct) {
+      if (ct < nct) {
+        f32x4_t u = *reinterpret_cast<const f32x4_t*>(ur + ct * 16);
+        if (d >= c) u = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        f32x4_t& acc = (ct & 1) ? acc1 : acc0;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(u[0], a[ct][0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(u[1], a[ct][1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(u[2], a[ct][2], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(u[3], a[ct][3], acc, 0, 0, 0);
+      }
+    }
+    // lane (i, g) register r: T[i][dt*16 + 4g + r]; minus E (row r0+i is delayed row dd)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int dd = dt * 16 + 4 * g + r;
+      float v = acc0[r] + acc1[r];
+      if (dd < c && sRows[dd] == r0 + li) v -= 1.f;
+      sT[li * KC + dd] = dd < c ? v : 0.f;
     }
   }
   __syncthreads();
@@ -371,52 +371,34 @@ __global__ __launch_bounds__(256) void k_flush_mfma(SysP<float> S, BatchP<float>
     double sum = 0;
     if (dd < c) {
       if (binv_lds) {
-        for (int e = 0; e < c; ++e) sum += (double)sT[i * cp + e] * sB[e * c + dd];
+        for (int e = 0; e < c; ++e) sum += (double)sT[i * KC + e] * sB[e * c + dd];
       } else {
 #pragma unroll 8
-        for (int e = 0; e < c; ++e) sum += (double)sT[i * cp + e] * Bi[e * K + dd];
+        for (int e = 0; e < c; ++e) sum += (double)sT[i * KC + e] * Bi[e * K + dd];
       }
     }
-    sT2[i * cp + dd] = (float)(-sum);
+    sT2[i * KC + dd] = (float)(-sum);
   }
   __syncthreads();
-  // ---- GEMM 2: Ainv[r0:r0+16][:] += T2 . V, four column tiles at a time
-  const int ntile = n / 16;
-  for (int ct0 = 0; ct0 < ntile; ct0 += 4) {
-    f32x4_t acc[4];
-    int col[4];
+  // ---- GEMM 2: rows^T += V^T T2^T over the delayed rows, four at a time
+  for (int d0 = 0; d0 < cp; d0 += 4) {
+    const int dk = d0 + g;
+    const float bv = sT2[li * KC + dk];
+    const float* vr = v_lds ? sV + (size_t)dk * vs + li : Vb + (size_t)(dk < c ? dk : 0) * n + li;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int ct = ct0 + t < ntile ? ct0 + t : ntile - 1;
-      col[t] = ct * 16 + li;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = r0 + 4 * g + r;
-        acc[t][r] = Ainv[(size_t)(row < n ? row : n - 1) * n + col[t]];
+    for (int ct = 0; ct < kFlushTiles; ++ct) {
+      if (ct < nct) {
+        float v = vr[ct * 16];
+        if (!v_lds && dk >= c) v = 0.f;
+        a[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v, bv, a[ct], 0, 0, 0);
       }
     }
-#pragma unroll 2
-    for (int k0 = 0; k0 < cp; k0 += 4) {
-      const int kk = k0 + g;
-      const float av = sT2[li * cp + kk];
-      float bv[4];
+  }
+  if (FR.writes(row)) {
+    float* aw = Ainv + (size_t)row * n + 4 * g;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) bv[t] = Vb[(size_t)(kk < c ? kk : c - 1) * n + col[t]];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        if (kk >= c) bv[t] = 0.f;
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[t], acc[t], 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      if (ct0 + t >= ntile) continue;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = r0 + 4 * g + r;
-        if (row < n) Ainv[(size_t)row * n + col[t]] = acc[t][r];
-      }
-    }
+    for (int ct = 0; ct < kFlushTiles; ++ct)
+      if (ct < nct) *reinterpret_cast<f32x4_t*>(aw + ct * 16) = a[ct];
   }
 }
 

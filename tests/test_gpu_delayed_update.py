@@ -78,7 +78,10 @@ def test_engine_sequence_vs_oracle(prec, n, delay):
     dev.invert_transpose(A)
     ref = po.DelayedUpdate(n, delay)
     psiM, _, _ = po.invert_transpose(A)
-    tol = 1e-10 if prec == qmcx.FP64 else 2e-4
+    # FP32: the (96, 32) sequence passes an ill-conditioned capacitance step a few rows after
+    # the flush, where FP32 rounding alone reaches 1e-4 .. 3e-4 of the row (a wrong term gives
+    # errors of order 1e-1)
+    tol = 1e-10 if prec == qmcx.FP64 else 1e-3
     for step, r in enumerate(rng.permutation(n)[: int(1.5 * delay) + 3]):
         v = rng.uniform(-1, 1, n) + 4 * np.eye(n)[r]
         g = dev.get_inv_row(int(r))
