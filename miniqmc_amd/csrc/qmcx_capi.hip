@@ -199,6 +199,8 @@ struct qmcx_system_s {
 struct qmcx_batch_s {
   qmcx_system_s* sys = nullptr;
   int W = 0, K = 1, prec = QMCX_FP64, walker0 = 0, slices = 0, nt = 0;
+  int nch = 1;   // delayed-update state slots per walker and determinant: one per chunk
+  int flush_group = 1;   // chunks whose delayed updates one flush launch applies to the whole inverse
   int flush_kind = 0;   // 0: matrix cores when possible, 1: vector FMA
   double* flex_displ = nullptr;   // [W][3] proposals of the pending flex move
   int* flex_decide = nullptr;     // [W]
@@ -290,6 +292,7 @@ struct qmcx_batch_s {
     BatchP<T> B;
     B.W = W;
     B.K = K;
+    B.nch = nch;
     B.R = R;
     B.A = (T*)A;
     B.Ainv = (T*)Ainv;
@@ -427,7 +430,7 @@ int evaluate_log_impl(qmcx_batch_t b) {
       sizeof(double) * (3 * (size_t)s->N + 32) + sizeof(Functor<T>) * (kMaxSpecies + 2) + sizeof(Cell<T>);
   hipLaunchKernelGGL(k_jastrow_init<T>, dim3(b->W), dim3(1024), lds, b->stream, s->params<T>(), b->params<T>());
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemsetAsync(b->ndelay, 0, sizeof(int) * b->W, b->stream));
+  HIPCHK(hipMemsetAsync(b->ndelay, 0, sizeof(int) * b->W * 2 * b->nch, b->stream));
   return launch_gl<T>(b, false);
 }
 
@@ -442,51 +445,64 @@ bool use_mfma_flush(const qmcx_batch_t b) {
 // Sherman-Morrison updates (K = 1), which bounds the chunk's LDS records.
 static int move_chunk(int n, int K) { return K == 1 ? std::min(n, 64) : K; }
 
-// updateInvMat of the delayed rows for rows [lo, hi) of determinant det
+// updateInvMat of the delayed rows of state slots [j0, j1) of determinant det, in order
 template <typename T>
-int launch_flush(qmcx_batch_t b, hipStream_t st, int det, int lo, int hi) {
+int launch_flush(qmcx_batch_t b, hipStream_t st, int det, int j0, int j1) {
   const qmcx_system_s* s = b->sys;
-  const FlushRows FR{lo, hi};
-  if (hi <= lo) return 0;
+  const FlushRows FR{0, s->n};
   if (use_mfma_flush<T>(b)) {
     const size_t lds = flush_mfma_lds(s->n, b->K);
-    hipLaunchKernelGGL(k_flush_mfma, dim3((hi - lo + kFlushBM - 1) / kFlushBM, b->W), dim3(64 * kFlushWaves), lds, st,
-                       s->params<float>(), b->params<float>(), det, FR);
+    hipLaunchKernelGGL(k_flush_mfma, dim3((s->n + kFlushBM - 1) / kFlushBM, b->W), dim3(64 * kFlushWaves), lds, st,
+                       s->params<float>(), b->params<float>(), det, j0, j1, FR);
+    HIPCHK(hipGetLastError());
   } else {
     const size_t lds = sizeof(T) * 2 * kFlushRows * b->K + sizeof(int) * b->K;
-    hipLaunchKernelGGL(k_flush_generic<T>, dim3((hi - lo + kFlushRows - 1) / kFlushRows, b->W), dim3(256), lds, st,
-                       s->params<T>(), b->params<T>(), det, FR);
+    for (int j = j0; j < j1; ++j) {
+      hipLaunchKernelGGL(k_flush_generic<T>, dim3((s->n + kFlushRows - 1) / kFlushRows, b->W), dim3(256), lds, st,
+                         s->params<T>(), b->params<T>(), det, j, FR);
+      HIPCHK(hipGetLastError());
+    }
   }
-  HIPCHK(hipGetLastError());
   return 0;
 }
 
-// the moves (and delayed-update flushes) of one determinant's electrons in one sweep
+// The moves of one determinant's electrons in one sweep.  With K > 1 a chunk's delayed updates
+// (recorded in its state slot) reach the whole inverse in groups: one flush launch applies the
+// updates of flush_group chunks, and each chunk's set-up applies the pending updates of the
+// earlier chunks of its group to its own rows.  The last group is flushed when `flush` is set;
+// otherwise the inverse is left for the FP64 refresh to rebuild from the Slater matrix.
 template <typename T>
-int moves_det(qmcx_batch_t b, int det, int mode) {
+int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
   const qmcx_system_s* s = b->sys;
   int nt = 0;
   if (choose_shape(s, nt)) return QMCX_ERR_ARG;
   const int n = s->n;
   const int chunk = move_chunk(n, b->K);
-  const MoveLds L = move_lds<T>(s->N, s->nions, n, b->K, chunk);
+  const MoveLds L = move_lds<T>(s->N, s->nions, n, b->K, chunk, b->flush_group > 1);
   MoveArgs a;
   a.mode = mode;
   a.slices = 0;
   a.spo = b->spo;
   a.prop = b->prop;
   a.decide = nullptr;
-  for (int e = 0; e < n; e += chunk) {
+  const int nchunks = (n + chunk - 1) / chunk;
+  int pending = 0;   // first chunk whose updates are not in the inverse yet
+  for (int ci = 0; ci < nchunks; ++ci) {
+    const int e = ci * chunk;
     a.det = det;
     a.e_begin = det * n + e;
     a.e_end = det * n + std::min(n, e + chunk);
+    a.chunk = ci;
+    a.chunk0 = pending;
     if (b->prof_record(0)) return QMCX_ERR_HIP;
     if (launch_moves<T>(b, nt, L.total, a)) return QMCX_ERR_HIP;
     if (b->prof_close()) return QMCX_ERR_HIP;
-    if (b->K > 1) {
+    const bool last = ci + 1 == nchunks;
+    if (b->K > 1 && (ci + 1 - pending == b->flush_group || (last && flush))) {
       if (b->prof_record(1)) return QMCX_ERR_HIP;
-      if (int rc = launch_flush<T>(b, b->stream, det, 0, n)) return rc;
+      if (int rc = launch_flush<T>(b, b->stream, det, pending, ci + 1)) return rc;
       if (b->prof_close()) return QMCX_ERR_HIP;
+      pending = ci + 1;
     }
   }
   return 0;
@@ -533,7 +549,7 @@ int issue_proposals(qmcx_batch_t b, int det, hipEvent_t after, uint64_t seed, in
 template <typename T>
 int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompute, int slot, int sweep, bool last) {
   HIPCHK(hipStreamWaitEvent(b->stream, b->ev_pre[0], 0));
-  if (int rc = moves_det<T>(b, 0, mode)) return rc;
+  if (int rc = moves_det<T>(b, 0, mode, !recompute)) return rc;
   HIPCHK(hipEventRecord(b->ev_done[0], b->stream));
   if (!last)
     if (int rc = issue_proposals<T>(b, 0, b->ev_done[0], seed, sweep + 1, tau)) return rc;
@@ -547,7 +563,7 @@ int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompu
     b->pending_dn = false;
   }
   HIPCHK(hipStreamWaitEvent(b->stream, b->ev_pre[1], 0));
-  if (int rc = moves_det<T>(b, 1, mode)) return rc;
+  if (int rc = moves_det<T>(b, 1, mode, true)) return rc;
   HIPCHK(hipEventRecord(b->ev_done[1], b->stream));
   if (!last)
     if (int rc = issue_proposals<T>(b, 1, b->ev_done[1], seed, sweep + 1, tau)) return rc;
@@ -797,6 +813,7 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
   b->sys = s;
   b->W = nwalkers;
   b->K = delay_rank;
+  b->nch = delay_rank > 1 ? (s->n + delay_rank - 1) / delay_rank : 1;
   b->prec = precision;
   const size_t es = precision == QMCX_FP32 ? 4 : 8;
   const size_t W = nwalkers, n = s->n, N = s->N, K = delay_rank;
@@ -809,8 +826,8 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
       return rc;
     }
     const int chunk = move_chunk(s->n, delay_rank);
-    const size_t lds = precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, chunk).total
-                                              : move_lds<double>(s->N, s->nions, s->n, delay_rank, chunk).total;
+    const size_t lds = precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, chunk, false).total
+                                              : move_lds<double>(s->N, s->nions, s->n, delay_rank, chunk, false).total;
     if (lds > 160 * 1024) {
       delete b;
       return fail(QMCX_ERR_ARG, "walker state (" + std::to_string(lds / 1024) +
@@ -832,12 +849,13 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
   b->J2 = p8;
   rc |= b->alloc(p8, es * W * 5 * N);
   b->J1 = p8;
-  rc |= b->alloc(p8, es * W * K * n);
+  const size_t S = W * 2 * (size_t)b->nch;   // delayed-update state slots
+  rc |= b->alloc(p8, es * S * K * n);
   b->Vbuf = p8;
-  rc |= b->alloc(p8, sizeof(double) * W * K * K);
+  rc |= b->alloc(p8, sizeof(double) * S * K * K);
   b->Binv = p8;
-  rc |= b->alloc(b->rows, W * K);
-  rc |= b->alloc(b->ndelay, W);
+  rc |= b->alloc(b->rows, S * K);
+  rc |= b->alloc(b->ndelay, S);
   rc |= b->alloc(b->logdet, W * 2);
   rc |= b->alloc(b->phase, W * 2);
   rc |= b->alloc(b->jlog, W * 2);
@@ -940,6 +958,19 @@ int qmcx_batch_set_tile(qmcx_batch_t b, int spline_slices) {
   const int nq = b->sys->npad / (b->prec == QMCX_FP32 ? 4 : 2);
   if (nq * spline_slices > 1024) return fail(QMCX_ERR_ARG, "spline tile does not fit a workgroup");
   b->slices = spline_slices;
+  return 0;
+}
+
+int qmcx_batch_set_flush_group(qmcx_batch_t b, int chunks) {
+  if (!b || chunks < 1 || chunks > b->nch) return fail(QMCX_ERR_ARG, "flush group out of range");
+  if (chunks > 1) {
+    const qmcx_system_s* s = b->sys;
+    const int chunk = move_chunk(s->n, b->K);
+    const size_t lds = b->prec == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, b->K, chunk, true).total
+                                            : move_lds<double>(s->N, s->nions, s->n, b->K, chunk, true).total;
+    if (lds > 160 * 1024) return fail(QMCX_ERR_ARG, "grouped flushes need more LDS than the move kernel has");
+  }
+  b->flush_group = chunks;
   return 0;
 }
 
@@ -1072,18 +1103,17 @@ static int flex_launch(qmcx_batch_t b, int iel, bool apply) {
   const bool trace_was = b->trace_on;
   b->trace_on = true;
   if (f32) {
-    const MoveLds L = move_lds<float>(s->N, s->nions, s->n, b->K, 1);
+    const MoveLds L = move_lds<float>(s->N, s->nions, s->n, b->K, 1, false);
     rc = launch_moves<float>(b, nt, L.total, a);
   } else {
-    const MoveLds L = move_lds<double>(s->N, s->nions, s->n, b->K, 1);
+    const MoveLds L = move_lds<double>(s->N, s->nions, s->n, b->K, 1, false);
     rc = launch_moves<double>(b, nt, L.total, a);
   }
   b->trace_on = trace_was;
   if (rc) return rc;
   if (apply && b->K > 1) {
     // every flex move is its own chunk: its (at most one) delayed row is applied now
-    rc = f32 ? launch_flush<float>(b, b->stream, a.det, 0, s->n)
-             : launch_flush<double>(b, b->stream, a.det, 0, s->n);
+    rc = f32 ? launch_flush<float>(b, b->stream, a.det, 0, 1) : launch_flush<double>(b, b->stream, a.det, 0, 1);
     if (rc) return rc;
   }
   return 0;
@@ -1171,12 +1201,13 @@ int qmcx_last_sweep_ms(qmcx_batch_t b, double* ms) {
 
 #ifdef QMCX_PHASE_TIMING
 // Profiling builds only: per-phase wall_clock64 ticks (walker / matrix 0, lane 0); slots 0-7
-// and 12-15 the move kernel's phases, 8-11 the FP64 inverse's transpose, panels, updates and output.
-extern "C" int qmcx_debug_phase_ticks(double* out16, int reset) {
+// and 12-22 the move kernel's phases, 8-11 the FP64 inverse's transpose, panels, updates and output,
+// 24-29 the flush's row loads, staging, products and stores.
+extern "C" int qmcx_debug_phase_ticks(double* out32, int reset) {
   unsigned long long t[qmcx::kPhaseSlots];
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpyFromSymbol(t, HIP_SYMBOL(qmcx_phase_ticks), sizeof(t)));
-  for (int i = 0; i < qmcx::kPhaseSlots; ++i) out16[i] = (double)t[i];
+  for (int i = 0; i < qmcx::kPhaseSlots; ++i) out32[i] = (double)t[i];
   if (reset) {
     unsigned long long z[qmcx::kPhaseSlots] = {};
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(qmcx_phase_ticks), z, sizeof(z)));
@@ -1207,6 +1238,7 @@ struct qmcx_du_s {
     std::memset(&B, 0, sizeof(B));
     B.W = 1;
     B.K = K;
+    B.nch = 1;
     B.A = (T*)A;
     B.Ainv = (T*)Ainv;
     B.Vbuf = (T*)Vbuf;
@@ -1232,11 +1264,11 @@ int du_flush(qmcx_du_t h) {
   if (sizeof(T) == 4 && flush_mfma_fits(h->n)) {
     const size_t lds = flush_mfma_lds(h->n, h->K);
     hipLaunchKernelGGL(k_flush_mfma, dim3((h->n + kFlushBM - 1) / kFlushBM, 1), dim3(64 * kFlushWaves), lds, h->stream,
-                       h->sys<float>(), h->params<float>(), 0, FlushRows{0, h->n});
+                       h->sys<float>(), h->params<float>(), 0, 0, 1, FlushRows{0, h->n});
   } else {
     const size_t lds = sizeof(T) * 2 * kFlushRows * h->K + sizeof(int) * h->K;
     hipLaunchKernelGGL(k_flush_generic<T>, dim3((h->n + kFlushRows - 1) / kFlushRows, 1), dim3(256), lds, h->stream,
-                       h->sys<T>(), h->params<T>(), 0, FlushRows{0, h->n});
+                       h->sys<T>(), h->params<T>(), 0, 0, FlushRows{0, h->n});
   }
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_du_reset, dim3(1), dim3(64), 0, h->stream, h->ndelay);

@@ -181,6 +181,9 @@ __device__ inline void stage_lds(const LdsCopy (&c)[NR]) {
   else stage_lds_v<int, NR>(c);
 }
 
+// LDS data exchanged between the lanes of one wave: its LDS operations have completed.
+__device__ inline void wave_lds_fence() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
 // Barrier for data exchanged through LDS only: waits for this wave's LDS
 // operations, not for its outstanding global loads.
 __device__ inline void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
@@ -336,6 +339,7 @@ __device__ inline void bspline_weights(T t, T a[4], T da[4], T d2a[4]) {
 
 // 16-byte vector of T (clang ext_vector_type so the load can be non-temporal)
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef f32x4 f32x4_t;
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 template <typename T>
 struct V16;

@@ -34,6 +34,7 @@ struct SysP {
 template <typename T>
 struct BatchP {
   int W, K;
+  int nch;         // delayed-update state slots per walker and determinant (one per chunk)
   double* R;
   T* A;
   T* Ainv;
@@ -41,6 +42,8 @@ struct BatchP {
   T* d2psiM;
   T* J2;
   T* J1;
+  // delayed-update state of chunk j of determinant det of walker w, slot (w*2 + det)*nch + j:
+  // V rows Vbuf[slot][K][n], Binv[slot][K][K], rows[slot][K], ndelay[slot]
   T* Vbuf;
   double* Binv;    // capacitance-matrix inverse kept in FP64 in both precisions
   int* rows;
@@ -59,8 +62,17 @@ struct BatchP {
 constexpr int kProp = 10;
 constexpr int kPropJ1 = 0, kPropDispl = 5, kPropU = 8;
 
+template <typename T>
+__host__ __device__ inline size_t state_slot(const BatchP<T>& B, int w, int det, int j) {
+  return ((size_t)w * 2 + det) * B.nch + j;
+}
+
 struct MoveArgs {
   int e_begin, e_end, det, mode, slices;
+  // index of this chunk in its determinant's sweep; the delayed updates of chunks
+  // chunk0 .. chunk-1 (state slots) are not in Ainv yet and are applied to this chunk's rows
+  // at set-up
+  int chunk, chunk0;
   const int* decide;     // [W] caller accept decisions (flex_acceptrestoreMove) or null
   const void* spo;       // [W][2][n][5][npad] T: SPO rows at the proposals (k_spo_proposals)
   const double* prop;    // [W][2][n][kProp] proposal records (k_spo_proposals)
@@ -204,10 +216,12 @@ struct FlushRows {
   __device__ bool writes(int m) const { return m < hi; }
 };
 constexpr int kFlushRows = 16;
+// one state slot j per launch
 template <typename T>
-__global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, int det, FlushRows FR) {
+__global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, int det, int j, FlushRows FR) {
   const int w = blockIdx.y, m0 = FR.lo + blockIdx.x * kFlushRows;
-  const int c = B.ndelay[w];
+  const size_t sl = state_slot(B, w, det, j);
+  const int c = B.ndelay[sl];
   if (c == 0) return;
   const int n = S.n, K = B.K, tid = threadIdx.x, nt = blockDim.x;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -217,9 +231,9 @@ __global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, i
   const size_t moff = ((size_t)w * 2 + det) * n * n;
   T* Ainv = B.Ainv + moff;
   const T* Amat = B.A + moff;
-  const T* Vb = B.Vbuf + (size_t)w * K * n;
-  const double* Bi = B.Binv + (size_t)w * K * K;
-  for (int i = tid; i < c; i += nt) sRows[i] = B.rows[(size_t)w * K + i];
+  const T* Vb = B.Vbuf + sl * K * n;
+  const double* Bi = B.Binv + sl * K * K;
+  for (int i = tid; i < c; i += nt) sRows[i] = B.rows[sl * K + i];
   __syncthreads();
   const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
   for (int idx = wid; idx < kFlushRows * c; idx += nw) {
@@ -270,7 +284,6 @@ __global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, i
 //   GEMM 2:  Ainv_rows^T += V^T T2^T  A = V^T (16 orbitals x 4 delayed rows), B = T2^T,
 //            accumulators started from the rows themselves
 // V (the stale inverse rows) is staged in LDS per workgroup when the delay rank allows.
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
 constexpr int kFlushWaves = 12;     // waves per workgroup, 16 rows each
 constexpr int kFlushBM = 16 * kFlushWaves;   // rows per workgroup
 constexpr int kFlushTiles = 24;     // 16-column tiles held in registers: n <= 384
@@ -287,14 +300,12 @@ __host__ __device__ inline size_t flush_mfma_lds(int n, int K) {
   return sizeof(float) * kFlushWaves * 2 * 16 * cp + sizeof(double) * kb + sizeof(float) * vb + sizeof(int) * K;
 }
 
-__global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, BatchP<float> B, int det,
-                                                                  FlushRows FR) {
+// State slots [j0, j1) of determinant det are applied in order to the rows held in registers.
+__global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, BatchP<float> B, int det, int j0,
+                                                                  int j1, FlushRows FR) {
   const int w = blockIdx.y;
-  const int c = B.ndelay[w];
-  if (c == 0) return;
-  const int n = S.n, K = B.K, tid = threadIdx.x;
+  const int n = S.n, K = B.K, tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wid = tid >> 6, li = lane & 15, g = lane >> 4;
-  const int cp = (c + 15) & ~15;           // delayed rows padded to a 16-wide tile
   const int nct = n / 16;                  // column tiles (n % 16 == 0, nct <= kFlushTiles)
   const int r0 = FR.lo + blockIdx.x * kFlushBM + wid * 16;   // this wave's first row
   const int KC = (K + 15) & ~15;           // LDS sizing uses the delay rank, not c
@@ -309,23 +320,19 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
   const size_t moff = ((size_t)w * 2 + det) * n * n;
   float* Ainv = B.Ainv + moff;
   const float* Amat = B.A + moff;
-  const float* Vb = B.Vbuf + (size_t)w * K * n;
-  const double* Bi = B.Binv + (size_t)w * K * K;
   const int vs = flush_v_stride(n);
-  const int nt = blockDim.x;
-  for (int i = tid; i < c; i += nt) sRows[i] = B.rows[(size_t)w * K + i];
-  if (binv_lds)
-    for (int i = tid; i < c * c; i += nt) sB[i] = Bi[(i / c) * K + i % c];
-  if (v_lds) {
-    // V rows, zero-padded to cp rows, float4 at a time
-    const int q = n / 4;
-    for (int i = tid; i < cp * q; i += nt) {
-      const int d = i / q, j = i - d * q;
-      f32x4_t v = *reinterpret_cast<const f32x4_t*>(Vb + (size_t)(d < c ? d : 0) * n + 4 * j);
-      if (d >= c) v = f32x4_t{0.f, 0.f, 0.f, 0.f};
-      *reinterpret_cast<f32x4_t*>(sV + (size_t)d * vs + 4 * j) = v;
-    }
+#ifdef QMCX_PHASE_TIMING
+  unsigned long long ftick_ = wall_clock64();
+  unsigned long long facc_[6] = {0, 0, 0, 0, 0, 0};
+#define FL_TICK(slot)                                      \
+  if (w == 0 && blockIdx.x == 0 && tid == 0) {             \
+    const unsigned long long now_ = wall_clock64();        \
+    facc_[(slot) - 24] += now_ - ftick_;                   \
+    ftick_ = now_;                                         \
   }
+#else
+#define FL_TICK(slot)
+#endif
   // this wave's rows into registers (row clamped into the range, zeroed after)
   const int row = r0 + li;
   const bool rok = row < FR.hi;
@@ -337,69 +344,104 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
 #pragma unroll
   for (int ct = 0; ct < kFlushTiles; ++ct)
     if (!rok) a[ct] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-  __syncthreads();
-  // ---- GEMM 1: T^T tile by tile of 16 delayed rows; two accumulators split the orbitals
-  for (int dt = 0; dt < cp / 16; ++dt) {
-    const int d = dt * 16 + li;
-    const float* ur = Amat + (size_t)(d < c ? sRows[d] : sRows[0]) * n + 4 * g;
-    f32x4_t acc0 = f32x4_t{0.f, 0.f, 0.f, 0.f}, acc1 = f32x4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ct = 0; ct < kFlushTiles; ++ct) {
-      if (ct < nct) {
-        f32x4_t u = *reinterpret_cast<const f32x4_t*>(ur + ct * 16);
-        if (d >= c) u = f32x4_t{0.f, 0.f, 0.f, 0.f};
-        f32x4_t& acc = (ct & 1) ? acc1 : acc0;
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(u[0], a[ct][0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(u[1], a[ct][1], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(u[2], a[ct][2], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(u[3], a[ct][3], acc, 0, 0, 0);
+  bool changed = false;
+  FL_TICK(24)
+  for (int j = j0; j < j1; ++j) {
+    const size_t sl = state_slot(B, w, det, j);
+    const int c = B.ndelay[sl];
+    if (c == 0) continue;
+    changed = true;
+    const int cp = (c + 15) & ~15;         // delayed rows padded to a 16-wide tile
+    const float* Vb = B.Vbuf + sl * K * n;
+    const double* Bi = B.Binv + sl * K * K;
+    __syncthreads();                       // the previous slot's LDS reads are done
+    for (int i = tid; i < c; i += nt) sRows[i] = B.rows[sl * K + i];
+    if (binv_lds)
+      for (int i = tid; i < c * c; i += nt) sB[i] = Bi[(i / c) * K + i % c];
+    if (v_lds) {
+      // V rows, zero-padded to cp rows, float4 at a time
+      const int q = n / 4;
+      for (int i = tid; i < cp * q; i += nt) {
+        const int d = i / q, jj = i - d * q;
+        f32x4_t v = *reinterpret_cast<const f32x4_t*>(Vb + (size_t)(d < c ? d : 0) * n + 4 * jj);
+        if (d >= c) v = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        *reinterpret_cast<f32x4_t*>(sV + (size_t)d * vs + 4 * jj) = v;
       }
     }
-    // lane (i, g) register r: T[i][dt*16 + 4g + r]; minus E (row r0+i is delayed row dd)
+    __syncthreads();
+    FL_TICK(25)
+    // ---- GEMM 1: T^T tile by tile of 16 delayed rows; two accumulators split the orbitals
+    for (int dt = 0; dt < cp / 16; ++dt) {
+      const int d = dt * 16 + li;
+      const float* ur = Amat + (size_t)(d < c ? sRows[d] : sRows[0]) * n + 4 * g;
+      f32x4_t acc0 = f32x4_t{0.f, 0.f, 0.f, 0.f}, acc1 = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int dd = dt * 16 + 4 * g + r;
-      float v = acc0[r] + acc1[r];
-      if (dd < c && sRows[dd] == r0 + li) v -= 1.f;
-      sT[li * KC + dd] = dd < c ? v : 0.f;
+      for (int ct = 0; ct < kFlushTiles; ++ct) {
+        if (ct < nct) {
+          f32x4_t u = *reinterpret_cast<const f32x4_t*>(ur + ct * 16);
+          if (d >= c) u = f32x4_t{0.f, 0.f, 0.f, 0.f};
+          f32x4_t& acc = (ct & 1) ? acc1 : acc0;
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(u[0], a[ct][0], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(u[1], a[ct][1], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(u[2], a[ct][2], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(u[3], a[ct][3], acc, 0, 0, 0);
+        }
+      }
+      // lane (i, g) register r: T[i][dt*16 + 4g + r]; minus E (row r0+i is delayed row dd)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int dd = dt * 16 + 4 * g + r;
+        float v = acc0[r] + acc1[r];
+        if (dd < c && sRows[dd] == r0 + li) v -= 1.f;
+        sT[li * KC + dd] = dd < c ? v : 0.f;
+      }
     }
-  }
-  __syncthreads();
-  // ---- T2 = -(T Binv), FP64 products of the small capacitance inverse
-  for (int idx = lane; idx < 16 * cp; idx += 64) {
-    const int i = idx / cp, dd = idx - i * cp;
-    double sum = 0;
-    if (dd < c) {
-      if (binv_lds) {
-        for (int e = 0; e < c; ++e) sum += (double)sT[i * KC + e] * sB[e * c + dd];
-      } else {
+    wave_lds_fence();   // T and T2 are this wave's own
+    FL_TICK(26)
+    // ---- T2 = -(T Binv), FP64 products of the small capacitance inverse
+    for (int idx = lane; idx < 16 * cp; idx += 64) {
+      const int i = idx / cp, dd = idx - i * cp;
+      double sum = 0;
+      if (dd < c) {
+        if (binv_lds) {
+          for (int e = 0; e < c; ++e) sum += (double)sT[i * KC + e] * sB[e * c + dd];
+        } else {
 #pragma unroll 8
-        for (int e = 0; e < c; ++e) sum += (double)sT[i * KC + e] * Bi[e * K + dd];
+          for (int e = 0; e < c; ++e) sum += (double)sT[i * KC + e] * Bi[e * K + dd];
+        }
       }
+      sT2[i * KC + dd] = (float)(-sum);
     }
-    sT2[i * KC + dd] = (float)(-sum);
-  }
-  __syncthreads();
-  // ---- GEMM 2: rows^T += V^T T2^T over the delayed rows, four at a time
-  for (int d0 = 0; d0 < cp; d0 += 4) {
-    const int dk = d0 + g;
-    const float bv = sT2[li * KC + dk];
-    const float* vr = v_lds ? sV + (size_t)dk * vs + li : Vb + (size_t)(dk < c ? dk : 0) * n + li;
+    wave_lds_fence();
+    FL_TICK(27)
+    // ---- GEMM 2: rows^T += V^T T2^T over the delayed rows, four at a time
+    for (int d0 = 0; d0 < cp; d0 += 4) {
+      const int dk = d0 + g;
+      const float bv = sT2[li * KC + dk];
+      const float* vr = v_lds ? sV + (size_t)dk * vs + li : Vb + (size_t)(dk < c ? dk : 0) * n + li;
 #pragma unroll
-    for (int ct = 0; ct < kFlushTiles; ++ct) {
-      if (ct < nct) {
-        float v = vr[ct * 16];
-        if (!v_lds && dk >= c) v = 0.f;
-        a[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v, bv, a[ct], 0, 0, 0);
+      for (int ct = 0; ct < kFlushTiles; ++ct) {
+        if (ct < nct) {
+          float v = vr[ct * 16];
+          if (!v_lds && dk >= c) v = 0.f;
+          a[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v, bv, a[ct], 0, 0, 0);
+        }
       }
     }
+    FL_TICK(28)
   }
-  if (FR.writes(row)) {
+  if (changed && FR.writes(row)) {
     float* aw = Ainv + (size_t)row * n + 4 * g;
 #pragma unroll
     for (int ct = 0; ct < kFlushTiles; ++ct)
       if (ct < nct) *reinterpret_cast<f32x4_t*>(aw + ct * 16) = a[ct];
   }
+  FL_TICK(29)
+#ifdef QMCX_PHASE_TIMING
+  if (w == 0 && blockIdx.x == 0 && tid == 0)
+    for (int i = 0; i < 6; ++i) qmcx_phase_ticks[24 + i] += facc_[i];
+#endif
+#undef FL_TICK
 }
 
 // ------------------------------------------------------------------ FP64 inverse (recompute)

@@ -19,8 +19,21 @@ namespace qmcx {
 constexpr int kRed = 12;   // reduction slots: gnow(3) ratio(1) gnew(3) J2(5)
 constexpr int kMoveThreads = 512;
 
+// LDS of the set-up correction: T and T2 (E x KC each, T) and the MFMA partial tiles
+// (16 x 16 floats per job, at least one job per wave)
+__host__ __device__ inline int cor_jobs(int K, int E) {
+  const int tiles = ((E + 15) / 16) * ((K + 15) / 16);
+  return tiles > kMoveThreads / 64 ? tiles : kMoveThreads / 64;
+}
+template <typename T>
+__host__ __device__ inline size_t cor_lds_bytes(int K, int E) {
+  const int KC = (K + 15) & ~15;
+  return sizeof(T) * 2 * (size_t)E * KC + sizeof(float) * 256 * (size_t)cor_jobs(K, E);
+}
+
 struct MoveLds {
-  unsigned oR, oRed, oInv, oPsi, oBinv, oP, oPart, oRows, oAl, oJ1n, oFn, oCell, oProp, oJ1o, oJt, oJ2, oAc, oQ, total;
+  unsigned oR, oRed, oInv, oPsi, oBinv, oP, oPart, oRows, oAl, oJ1n, oFn, oCell, oProp, oJ1o, oJt, oJ2, oAc, oQ, oCor,
+      oRall, total;
 };
 __host__ __device__ inline unsigned lds_take(unsigned& o, size_t bytes) {
   unsigned r = o;
@@ -28,9 +41,11 @@ __host__ __device__ inline unsigned lds_take(unsigned& o, size_t bytes) {
   return r;
 }
 // E = electrons per chunk; the inverse-row cache is used when K > 1 (M, the ion count, is
-// not needed: the J1 terms come with the proposals)
+// not needed: the J1 terms come with the proposals).  The last two regions, for the set-up
+// correction of grouped flushes, are sized only with `corr` (the kernel touches them only
+// when a chunk has pending updates).
 template <typename T>
-__host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E) {
+__host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E, bool corr = true) {
   (void)M;
   MoveLds L;
   unsigned o = 0;
@@ -53,6 +68,8 @@ __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E) {
   L.oJ2 = lds_take(o, sizeof(T) * 5 * N);
   L.oAc = lds_take(o, cache ? sizeof(T) * (size_t)E * n : 0);
   L.oQ = lds_take(o, cache ? sizeof(double) * (size_t)K * E : 0);
+  L.oCor = lds_take(o, cache && corr ? cor_lds_bytes<T>(K, E) : 0);   // earlier chunks' updates at set-up
+  L.oRall = lds_take(o, cache && corr ? sizeof(int) * (size_t)(K + 1) * ((n + K - 1) / K) : 0);   // row lists, counts
   L.total = o;
   return L;
 }
@@ -83,12 +100,11 @@ __device__ inline void j2_pair(const Cell<T>& C, const Functor<T>& f, double xj,
   if (r < f.rcut) functor_vgl(f, r, u, du, d2u);
 }
 
-__device__ inline void wave_lds_fence() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
 #ifdef QMCX_PHASE_TIMING
 // wall_clock64 ticks per phase, taken by lane 0 of workgroup 0 into registers and added to
 // this table when the kernel ends (a global read-modify-write per tick would time itself)
-constexpr int kPhaseSlots = 16;
+constexpr int kPhaseSlots = 32;
 __device__ unsigned long long qmcx_phase_ticks[kPhaseSlots];
 #define QMCX_TICK(slot)                                                  \
   if (w == 0 && tid == 0) {                                              \
@@ -128,6 +144,211 @@ __device__ inline void j2_items(const Cell<T>& C, const Functor<T>* fn, const do
       d2u[q] = in ? c2 : T(0);
     }
   }
+}
+
+// The chunk's stale inverse rows (sAc, E x n, in LDS) receive the delayed updates that the
+// determinant's earlier chunks j0 .. j0+nj-1 of this sweep left in their state slots, in order:
+// updateInvMat (DelayedUpdate.h:176-193) restricted to these rows,
+//   sAc <- sAc + T2_j V_j,  T2_j = -(sAc U_j^T) B_j
+// (U_j rows: the new Slater rows of chunk j's accepted moves, V_j: their stale inverse rows,
+// B_j: chunk j's capacitance inverse; none of chunk j's rows is among these, so the identity
+// term of the flush is absent).  sRall holds the delayed-row lists of those slots (K each),
+// sNd their counts.  FP32 with n % 16 == 0 runs both products on the matrix cores with their global
+// loads batched; otherwise vector code.  Ends with a barrier.
+template <typename T>
+__device__ inline void correct_rows(const SysP<T>& S, const BatchP<T>& B, int w, int det, int j0, int nj, int E, T* sAc,
+                                    const int* sRall, const int* sNd, double* sBj, T* sT, T* sT2, float* sPart,
+                                    unsigned long long* crt) {
+#ifdef QMCX_PHASE_TIMING
+  unsigned long long ct_ = wall_clock64();
+#define CR_TICK(i)                                      \
+  if (w == 0 && threadIdx.x == 0) {                     \
+    const unsigned long long t_ = wall_clock64();       \
+    crt[i] += t_ - ct_;                                 \
+    ct_ = t_;                                           \
+  }
+#else
+#define CR_TICK(i)
+#endif
+  const int n = S.n, K = B.K, KC = (K + 15) & ~15;
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
+  const int li = lane & 15, g = lane >> 4;
+  const T* Amat = B.A + ((size_t)w * 2 + det) * n * n;
+  const bool mfma = sizeof(T) == 4 && n % 16 == 0;
+  for (int j = 0; j < nj; ++j) {
+    const size_t sl = state_slot(B, w, det, j0 + j);
+    const int cj = sNd[j];
+    if (cj == 0) continue;
+    const int cpj = (cj + 15) & ~15;
+    const T* Vj = B.Vbuf + sl * K * n;
+    const double* Bg = B.Binv + sl * K * K;
+    const int* rj = sRall + j * K;
+    // chunk j's capacitance inverse: loads issued now, stored to LDS after the first product
+    constexpr int kBr = 4;
+    double br[kBr];
+    const int nb = cj * cj;
+#pragma unroll
+    for (int q = 0; q < kBr; ++q) {
+      const int i = min(tid + q * nt, nb - 1);   // unconditional loads
+      br[q] = Bg[(i / cj) * K + i % cj];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    CR_TICK(0)
+    // ---- T = sAc U_j^T (E x cj)
+    if constexpr (sizeof(T) == 4) {
+      if (mfma) {
+        // 16x16 tiles of T, the orbital sum in slices over the waves; lane (i, g) supplies
+        // A[i][g] = sAc[e][k], B[g][i] = U_j[d][k] with k = 16kt + 4g + r for MFMA r; six
+        // orbital tiles' loads in flight at a time
+        const int ET = (E + 15) >> 4, DT = cpj >> 4, tiles = ET * DT;
+        const int ks = tiles >= nw ? 1 : nw / tiles;
+        const int nkt = n >> 4, per = (nkt + ks - 1) / ks;
+        for (int job = wid; job < tiles * ks; job += nw) {
+          const int tile = job % tiles, sc = job / tiles;
+          const int et = tile % ET, dt = tile / ET;
+          const int e = et * 16 + li, d = dt * 16 + li;
+          const float* ar = (const float*)sAc + (size_t)(e < E ? e : 0) * n + 4 * g;
+          const float* ur = (const float*)Amat + (size_t)rj[d < cj ? d : 0] * n + 4 * g;
+          f32x4_t acc = f32x4_t{0.f, 0.f, 0.f, 0.f};
+          const int k1 = min(nkt, (sc + 1) * per);
+          for (int kb = sc * per; kb < k1; kb += 6) {
+            f32x4_t av[6], uv[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+              const int kt = min(kb + q, k1 - 1);
+              uv[q] = *reinterpret_cast<const f32x4_t*>(ur + kt * 16);
+              av[q] = *reinterpret_cast<const f32x4_t*>(ar + kt * 16);
+            }
+            __builtin_amdgcn_sched_barrier(0);   // all six loads issued before the first MFMA
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+              if (e >= E || kb + q >= k1) av[q] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+              if (d >= cj) uv[q] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q][0], uv[q][0], acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q][1], uv[q][1], acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q][2], uv[q][2], acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q][3], uv[q][3], acc, 0, 0, 0);
+            }
+          }
+          // register r of lane (i, g): row e = 16et + 4g + r, delayed row d = 16dt + i
+#pragma unroll
+          for (int r = 0; r < 4; ++r) sPart[job * 256 + (4 * g + r) * 16 + li] = acc[r];
+        }
+#pragma unroll
+        for (int q = 0; q < kBr; ++q)
+          if (tid + q * nt < nb) sBj[tid + q * nt] = br[q];
+        __syncthreads();
+        CR_TICK(1)
+        for (int idx = tid; idx < E * cpj; idx += nt) {
+          const int e = idx / cpj, d = idx - e * cpj;
+          const int tile = (e >> 4) + ET * (d >> 4);
+          float v = 0.f;
+          for (int sc = 0; sc < ks; ++sc) v += sPart[(sc * tiles + tile) * 256 + (e & 15) * 16 + (d & 15)];
+          sT[e * KC + d] = d < cj ? v : 0.f;
+        }
+      }
+    }
+    if (!mfma) {
+      for (int idx = wid; idx < E * cj; idx += nw) {
+        const int e = idx / cj, d = idx - e * cj;
+        const T* ur = Amat + (size_t)rj[d] * n;
+        double v = 0;
+        for (int k = lane; k < n; k += 64) v += (double)sAc[(size_t)e * n + k] * (double)ur[k];
+        v = wave_sum(v);
+        if (lane == 0) sT[e * KC + d] = (T)v;
+      }
+#pragma unroll
+      for (int q = 0; q < kBr; ++q)
+        if (tid + q * nt < nb) sBj[tid + q * nt] = br[q];
+    }
+    for (int i = tid + kBr * nt; i < nb; i += nt) sBj[i] = Bg[(i / cj) * K + i % cj];
+    __syncthreads();
+    CR_TICK(2)
+    // ---- T2 = -(T B_j), FP64 products
+    for (int idx = tid; idx < E * cpj; idx += nt) {
+      const int e = idx / cpj, d = idx - e * cpj;
+      double v = 0;
+      if (d < cj)
+        for (int q = 0; q < cj; ++q) v += (double)sT[e * KC + q] * sBj[q * cj + d];
+      sT2[e * KC + d] = (T)(-v);
+    }
+    __syncthreads();
+    CR_TICK(3)
+    // ---- sAc += T2 V_j
+    if constexpr (sizeof(T) == 4) {
+      if (mfma) {
+        // 16x16 tiles of sAc (16 rows x 16 orbitals); a wave loads the operands of up to six of
+        // its tiles (V from HBM, T2 and the tiles from LDS) for 32 delayed rows at a time before
+        // their MFMAs; lane (i, g) supplies A[i][g] = T2[e][d0+g], B[g][i] = V_j[d0+g][k];
+        // register r of lane (i, g) is row 16et + 4g + r, orbital 16kt + i
+        const int ET = (E + 15) >> 4, nkt = n >> 4, ntile = ET * nkt;
+        constexpr int kTG = 6;
+        for (int t0 = wid; t0 < ntile; t0 += kTG * nw) {
+          f32x4_t acc[kTG];
+          for (int db = 0; db < cpj; db += 32) {
+            const bool full = cpj - db >= 32;   // 8 or 4 steps of 4 delayed rows
+            float bv[kTG][8], av[kTG][8];
+#pragma unroll
+            for (int h = 0; h < kTG; ++h) {
+              const int t = min(t0 + h * nw, ntile - 1);
+              const int et = t % ET, kt = t / ET;
+              const int ea = min(et * 16 + li, E - 1);
+#pragma unroll
+              for (int s4 = 0; s4 < 8; ++s4) {
+                const int dk = db + 4 * s4 + g;
+                bv[h][s4] = (float)Vj[(size_t)min(dk, cj - 1) * n + kt * 16 + li];
+                av[h][s4] = (float)sT2[ea * KC + min(dk, KC - 1)];
+              }
+              if (db == 0)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                  const int e = min(et * 16 + 4 * g + r, E - 1);
+                  acc[h][r] = ((const float*)sAc)[(size_t)e * n + kt * 16 + li];
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);   // every operand requested before the first MFMA
+#pragma unroll
+            for (int h = 0; h < kTG; ++h) {
+              const int t = min(t0 + h * nw, ntile - 1);
+              const bool arow = (t % ET) * 16 + li < E;
+#pragma unroll
+              for (int s4 = 0; s4 < 8; ++s4) {
+                const int dk = db + 4 * s4 + g;
+                const float a2 = arow && dk < cj ? av[h][s4] : 0.f;
+                const float b2 = dk < cj ? bv[h][s4] : 0.f;
+                if (s4 < 4 || full) acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2, acc[h], 0, 0, 0);
+              }
+            }
+            if (db + 32 >= cpj) {
+#pragma unroll
+              for (int h = 0; h < kTG; ++h) {
+                const int t = t0 + h * nw;
+                if (t < ntile) {
+                  const int et = t % ET, kt = t / ET;
+#pragma unroll
+                  for (int r = 0; r < 4; ++r) {
+                    const int e = et * 16 + 4 * g + r;
+                    if (e < E) ((float*)sAc)[(size_t)e * n + kt * 16 + li] = acc[h][r];
+                  }
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+    if (!mfma) {
+      for (int i = tid; i < E * n; i += nt) {
+        const int e = i / n, k = i - e * n;
+        double v = (double)sAc[i];
+        for (int d = 0; d < cj; ++d) v += (double)sT2[e * KC + d] * (double)Vj[(size_t)d * n + k];
+        sAc[i] = (T)v;
+      }
+    }
+    __syncthreads();
+    CR_TICK(4)
+  }
+#undef CR_TICK
 }
 
 // KD: delayed updates (K > 1: inverse-row cache, Woodbury bordering) or Sherman-Morrison (K = 1)
@@ -170,7 +391,9 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   T* d2psiM = B.d2psiM + moff;
   T* J2 = B.J2 + (size_t)w * 5 * N;
   T* J1 = B.J1 + (size_t)w * 5 * N;
-  T* Vb = B.Vbuf + (size_t)w * K * n;
+  // this chunk's delayed-update state slot
+  const size_t slot = state_slot(B, w, a.det, KD ? a.chunk : 0);
+  T* Vb = B.Vbuf + slot * K * n;
   // this chunk's proposals: SPO rows [e][5][npad] and records [e][kProp]
   const T* spo = (const T*)a.spo + (((size_t)w * 2 + a.det) * n + r0) * 5 * S.npad;
   const double* prop = a.prop + (((size_t)w * 2 + a.det) * n + r0) * kProp;
@@ -179,7 +402,8 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   // the cell into LDS (one round of loads)
   {
     const unsigned je = (unsigned)(sizeof(T) * E);
-    const LdsCopy cp[10] = {{sR, B.R + (size_t)w * 3 * N, (unsigned)(sizeof(double) * 3 * N)},
+    const int nprev = KD ? a.chunk - a.chunk0 : 0;
+    const LdsCopy cp[11] = {{sR, B.R + (size_t)w * 3 * N, (unsigned)(sizeof(double) * 3 * N)},
                             {sJ2, J2, (unsigned)(sizeof(T) * 5 * N)},
                             {sAc, Ainv + (size_t)r0 * n, KD ? (unsigned)(sizeof(T) * E * n) : 0u},
                             {sProp, prop, (unsigned)(sizeof(double) * kProp * E)},
@@ -188,9 +412,36 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
                             {sJ1o + 2 * E, J1 + 2 * N + a.e_begin, je},
                             {sJ1o + 3 * E, J1 + 3 * N + a.e_begin, je},
                             {sFn, S.fn + kMaxSpecies, (unsigned)(sizeof(Functor<T>) * 2)},
-                            {sCell, S.cell, (unsigned)sizeof(Cell<T>)}};
+                            {sCell, S.cell, (unsigned)sizeof(Cell<T>)},
+                            {smem + L.oRall, nprev > 0 ? B.rows + state_slot(B, w, a.det, a.chunk0) * K : B.rows,
+                             (unsigned)(sizeof(int) * nprev * K)}};
     stage_lds(cp);
   }
+  QMCX_TICK(16)
+  if (KD && a.chunk > a.chunk0) {
+    const int* sRall = (const int*)(smem + L.oRall);
+    int* sNd = (int*)(smem + L.oRall) + (size_t)K * ((n + K - 1) / K);
+    for (int j = tid; j < a.chunk - a.chunk0; j += nt) sNd[j] = B.ndelay[state_slot(B, w, a.det, a.chunk0 + j)];
+    __syncthreads();
+    const int KC = (K + 15) & ~15;
+    T* sCorT = (T*)(smem + L.oCor);
+    unsigned long long* crt = nullptr;
+#ifdef QMCX_PHASE_TIMING
+    __shared__ unsigned long long crt_s[5];
+    if (tid < 5) crt_s[tid] = 0;
+    __syncthreads();
+    crt = crt_s;
+#endif
+    correct_rows<T>(S, B, w, a.det, a.chunk0, a.chunk - a.chunk0, E, sAc, sRall, sNd, sBinv, sCorT,
+                    sCorT + (size_t)E * KC,
+                    (float*)(sCorT + (size_t)2 * E * KC), crt);
+#ifdef QMCX_PHASE_TIMING
+    if (w == 0 && tid == 0)
+      for (int i = 0; i < 5; ++i) tacc_[18 + i] += crt[i];
+#endif
+    (void)crt;
+  }
+  QMCX_TICK(17)
   // a caller's accept decision (the flex stage calls) replaces the uniform: -inf accepts, +inf rejects
   const double flex_u = a.decide == nullptr ? 0.0 : (a.decide[w] != 0 ? -INFINITY : INFINITY);
   int c = 0, ca = 0;   // delayed rows; accepted moves of the chunk
@@ -581,11 +832,11 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     B.jlog[w * 2 + 0] += jl1;
     B.jlog[w * 2 + 1] += jl2;
     B.nacc[w] += nacc;
-    B.ndelay[w] = KD ? c : 0;
+    B.ndelay[slot] = KD ? c : 0;
   }
   if (KD) {
-    for (int i = tid; i < c * K; i += nt) B.Binv[(size_t)w * K * K + i] = sBinv[(i / K) * KP + i % K];
-    for (int i = tid; i < c; i += nt) B.rows[(size_t)w * K + i] = sRows[i];
+    for (int i = tid; i < c * K; i += nt) B.Binv[slot * K * K + i] = sBinv[(i / K) * KP + i % K];
+    for (int i = tid; i < c; i += nt) B.rows[slot * K + i] = sRows[i];
   }
   QMCX_TICK_FLUSH()
 }
