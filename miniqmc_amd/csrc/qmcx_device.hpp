@@ -87,6 +87,25 @@ __device__ inline T wave_sum(T v) {
   return lane63(v);
 }
 
+// Wave maximum through DPP lane moves (the same pattern as wave_sum; rows outside a move's
+// row mask keep their own value); every lane gets lane 63's result.
+template <int CTRL, int ROWMASK>
+__device__ inline double dpp_max(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp((int)(b & 0xffffffff), (int)(b & 0xffffffff), CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, ROWMASK, 0xf, false);
+  return fmax(x, __longlong_as_double(((long long)hi << 32) | (unsigned int)lo));
+}
+__device__ inline double wave_max(double v) {
+  v = dpp_max<0xB1, 0xf>(v);
+  v = dpp_max<0x4E, 0xf>(v);
+  v = dpp_max<0x141, 0xf>(v);
+  v = dpp_max<0x140, 0xf>(v);
+  v = dpp_max<0x142, 0xa>(v);
+  v = dpp_max<0x143, 0xc>(v);
+  return lane63(v);
+}
+
 // Exchange of wave halves between two values (v_permlane32_swap: lanes 32-63 of x
 // trade places with lanes 0-31 of y): x <- [x lanes 0-31 | y lanes 0-31],
 // y <- [x lanes 32-63 | y lanes 32-63].

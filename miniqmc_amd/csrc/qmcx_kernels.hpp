@@ -544,7 +544,9 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
     __syncthreads();
   }
 
-  double logacc = 0.0;
+  // product of the pivots' magnitudes as mantissa and binary exponent (one log at the end)
+  double pmant = 1.0;
+  int pexp = 0;
   int sign = 1;
   GJ_TICK(8)
   for (int k0 = 0; k0 < n; k0 += b) {
@@ -557,26 +559,19 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
 #pragma unroll
     for (int c = 0; c < B; ++c) pr[c] = (own && c < bb) ? pr[c] : 0.0;
     double* swp = fcol;   // old pivot-position row, for the thread that takes it
-    // the step loop stays rolled: register entries are picked by compare-and-select on j
+    // The step loop stays rolled, and every step works on pr[0]: each thread's panel entries
+    // rotate one place left after a step (all threads alike, the pivot-row copies included),
+    // so no register is picked by a runtime index; B steps rotate them back.
 #pragma unroll 1
     for (int j = 0; j < bb; ++j) {
       const int k = k0 + j;
-      double prj = 0.0;
-#pragma unroll
-      for (int c = 0; c < B; ++c) prj = c == j ? pr[c] : prj;
-      // pivot: first row of largest |P[i][j]|, i >= k (idamax semantics)
-      double best = -1.0;
-      int bi = k;
-      if (own && tid >= k) {
-        best = fabs(prj);
-        bi = tid;
-      }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(best, off, 64);
-        const int oi = __shfl_xor(bi, off, 64);
-        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-      }
+      const double prj = pr[0];
+      // pivot: first row of largest |P[i][j]|, i >= k (idamax semantics): the wave's largest
+      // magnitude, then its first lane (rows ascend with the lanes)
+      const double av = (own && tid >= k) ? fabs(prj) : -1.0;
+      double best = wave_max(av);
+      const unsigned long long hit = __ballot(av == best);
+      int bi = best >= 0.0 ? wid * 64 + (__ffsll((long long)hit) - 1) : k;
       if (lane == 0) { rv[wid] = best; ri[wid] = bi; }
       __syncthreads();
       best = rv[0];
@@ -591,29 +586,44 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
 #pragma unroll
         for (int c = 0; c < B; ++c) swp[c] = pr[c];
       __syncthreads();
-      const double d = prow[j];
+      const double d = prow[0];
       const double dinv = 1.0 / d;
       if (tid == 0) {
         piv[k] = p;
         if (p != k) sign = -sign;
         if (d < 0) sign = -sign;
-        logacc += log(fabs(d));
+        int e;
+        pmant = frexp(pmant * fabs(d), &e);
+        pexp += e;
       }
       if (own) {
         if (tid == k) {
+          pr[0] = dinv;
 #pragma unroll
-          for (int c = 0; c < B; ++c) pr[c] = c == j ? dinv : prow[c] * dinv;
+          for (int c = 1; c < B; ++c) pr[c] = prow[c] * dinv;
         } else {
           double f = prj;
           if (tid == p) {
 #pragma unroll
             for (int c = 0; c < B; ++c) pr[c] = swp[c];
-            f = swp[j];
+            f = swp[0];
           }
+          pr[0] = -f * dinv;
 #pragma unroll
-          for (int c = 0; c < B; ++c) pr[c] = c == j ? -f * dinv : pr[c] - f * (prow[c] * dinv);
+          for (int c = 1; c < B; ++c) pr[c] = pr[c] - f * (prow[c] * dinv);
         }
       }
+      const double r0 = pr[0];
+#pragma unroll
+      for (int c = 0; c + 1 < B; ++c) pr[c] = pr[c + 1];
+      pr[B - 1] = r0;
+    }
+    // a short last panel: rotate the remaining B - bb places
+    for (int j = bb; j < B; ++j) {
+      const double r0 = pr[0];
+#pragma unroll
+      for (int c = 0; c + 1 < B; ++c) pr[c] = pr[c + 1];
+      pr[B - 1] = r0;
     }
     if (own)
 #pragma unroll
@@ -731,7 +741,7 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
       src[k] = src[piv[k]];
       src[piv[k]] = t;
     }
-    logdet[(size_t)mat * ld_stride] = logacc;
+    logdet[(size_t)mat * ld_stride] = log(pmant) + (double)pexp * 0.69314718055994530942;
     phase[(size_t)mat * ld_stride] = sign > 0 ? 0.0 : M_PI;
   }
   __syncthreads();
