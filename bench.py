@@ -52,18 +52,19 @@ def path_bytes_per_move(sysd, k, acc, s=4):
     return b_spline + b_pos + b_j + b_det, dict(spline=b_spline, pos=b_pos, jastrow=b_j, det=b_det, cbar=cbar)
 
 
-def move_kernel_bytes_per_move(sysd, k, acc, s=4):
+def move_kernel_bytes_per_move(sysd, k, acc, moves_per_walker, flushes, s=4):
     """Algorithmic HBM bytes per move of the fused move kernel (the spline stencil is read by
     the proposals kernel): the proposal's SPO row 5*n*s and record (J1 terms, displacement,
-    uniform: 9 doubles), the moving electron's dpsiM row 3*n*s and J1 state 4*s; on accept
-    (fraction a) the new A, dpsiM, d2psiM rows and the stale inverse row 6*n*s and the J1 state
-    5*s; per chunk of k moves (64 with k = 1) the walker's positions (3*N doubles) and J2 state
-    (5*N*s) in and out and the chunk's stale inverse rows k*n*s in."""
+    uniform: 9 doubles), the moving electron's dpsiM row 3*n*s and J1 state 4*s, with k > 1 its
+    stale inverse row n*s (read once per chunk); on accept (fraction a) the new A, dpsiM, d2psiM
+    rows and the stale inverse row 6*n*s (written for a flush launch, or the U row read back by
+    the kernel's own flush) and the J1 state 5*s.  Per launch and walker (moves_per_walker
+    moves): the positions (3*N doubles) and J2 state (5*N*s) in and out, and per delayed-update
+    flush done inside the kernel the inverse read and written, 2*n^2*s."""
     n, N = sysd.nup, sysd.nel
-    chunk = k if k > 1 else min(n, 64)
-    per = 5 * n * s + 9 * 8 + 3 * n * s + 4 * s + acc * (6 * n * s + 5 * s)
-    per_chunk = 2 * (3 * N * 8 + 5 * N * s) + (chunk * n * s if k > 1 else 0)
-    return per + per_chunk / chunk
+    per = 5 * n * s + 9 * 8 + 3 * n * s + 4 * s + (n * s if k > 1 else 0) + acc * (6 * n * s + 5 * s)
+    per_launch = 2 * (3 * N * 8 + 5 * N * s) + flushes * 2 * n * n * s
+    return per + per_launch / moves_per_walker
 
 
 def proposal_bytes(sysd, s=4):
@@ -229,15 +230,23 @@ def main():
     # few bytes per move.  The proposals kernel carries the spline's table traffic; the whole
     # sweep is also put against the path's algorithmic bytes.
     bpm_path, parts = path_bytes_per_move(sysd, k, acc_timed)
-    bpm = move_kernel_bytes_per_move(sysd, k, acc_timed)
-    chunk = k if k > 1 else min(sysd.nup, 64)
+    # moves per walker and launch from the launch count: the whole determinant (n moves) when
+    # the kernel applies each chunk's delayed updates itself (no flush launches) or k = 1, else
+    # one chunk of k; in-kernel flushes per launch: every chunk's but the last one's of the
+    # first determinant when the FP64 refresh follows each sweep
+    per_walker = moves // (W * max(mv_n, 1))
+    nch = -(-sysd.nup // k) if k > 1 else 0
+    inflush = k > 1 and kt["flush"][1] == 0
+    flushes = (nch - (0.5 if args.recompute == 1 else 0.0)) if inflush else 0.0
+    bpm = move_kernel_bytes_per_move(sysd, k, acc_timed, per_walker, flushes)
     launch_ms = mv_ms / max(mv_n, 1)
-    achieved = bpm * W * chunk / (launch_ms * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic(sysd, k, W * chunk)
+    achieved = bpm * W * per_walker / (launch_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(sysd, k, W * per_walker)
     roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src,
                     kernel="k_moves<float>", bytes_per_move=round(bpm, 1),
-                    moves_per_launch=W * chunk, avg_launch_ms=round(launch_ms, 4))
+                    moves_per_launch=W * per_walker, flushes_per_launch=flushes,
+                    avg_launch_ms=round(launch_ms, 4))
     pr_ms, pr_n = kt["proposals"]
     pr_launch = pr_ms / max(pr_n, 1)
     pr_units = W * sysd.nup

@@ -98,12 +98,6 @@ int qmcx_batch_get_positions(qmcx_batch_t b, double* R);
 int qmcx_batch_set_walker_offset(qmcx_batch_t b, int walker0);
 /* spline stencil slices per workgroup (autotuner knob; 0 = default) */
 int qmcx_batch_set_tile(qmcx_batch_t b, int spline_slices);
-/* Delay rank k > 1: chunks of k moves whose delayed updates one flush applies to the whole
- * inverse (autotuner knob, default 1).  With g > 1 a chunk's set-up applies the pending updates
- * of the earlier chunks of its group to its own inverse rows (updateInvMat,
- * src/QMCWaveFunctions/DelayedUpdate.h:176-193, restricted to those rows).  1 <= g <= ceil(n/k). */
-int qmcx_batch_set_flush_group(qmcx_batch_t b, int chunks);
-
 /* Delayed-update flush engine (DelayedUpdate::updateInvMat, src/QMCWaveFunctions/DelayedUpdate.h:176-193):
  * 0 = FP32 matrix cores (v_mfma_f32_16x16x4f32) when the batch is FP32 and n % 16 == 0, 1 = vector FMA. */
 int qmcx_batch_set_flush(qmcx_batch_t b, int kind);

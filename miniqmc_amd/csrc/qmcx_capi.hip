@@ -200,7 +200,6 @@ struct qmcx_batch_s {
   qmcx_system_s* sys = nullptr;
   int W = 0, K = 1, prec = QMCX_FP64, walker0 = 0, slices = 0, nt = 0;
   int nch = 1;   // delayed-update state slots per walker and determinant: one per chunk
-  int flush_group = 1;   // chunks whose delayed updates one flush launch applies to the whole inverse
   int flush_kind = 0;   // 0: matrix cores when possible, 1: vector FMA
   double* flex_displ = nullptr;   // [W][3] proposals of the pending flex move
   int* flex_decide = nullptr;     // [W]
@@ -466,11 +465,11 @@ int launch_flush(qmcx_batch_t b, hipStream_t st, int det, int j0, int j1) {
   return 0;
 }
 
-// The moves of one determinant's electrons in one sweep.  With K > 1 a chunk's delayed updates
-// (recorded in its state slot) reach the whole inverse in groups: one flush launch applies the
-// updates of flush_group chunks, and each chunk's set-up applies the pending updates of the
-// earlier chunks of its group to its own rows.  The last group is flushed when `flush` is set;
-// otherwise the inverse is left for the FP64 refresh to rebuild from the Slater matrix.
+// The moves of one determinant's electrons in one sweep.  With K > 1 each chunk's delayed
+// updates (recorded in its state slot) reach the whole inverse before the next chunk: inside
+// the move kernel when its LDS holds the flush (FP32, n a multiple of 16 up to 384), else by a
+// flush launch after the chunk's move launch.  The last chunk is flushed only when `flush` is
+// set; otherwise the inverse is left for the FP64 refresh to rebuild from the Slater matrix.
 template <typename T>
 int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
   const qmcx_system_s* s = b->sys;
@@ -478,7 +477,7 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
   if (choose_shape(s, nt)) return QMCX_ERR_ARG;
   const int n = s->n;
   const int chunk = move_chunk(n, b->K);
-  const MoveLds L = move_lds<T>(s->N, s->nions, n, b->K, chunk, b->flush_group > 1);
+  const MoveLds L = move_lds<T>(s->N, s->nions, n, b->K, chunk);
   MoveArgs a;
   a.mode = mode;
   a.slices = 0;
@@ -486,23 +485,36 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
   a.prop = b->prop;
   a.decide = nullptr;
   const int nchunks = (n + chunk - 1) / chunk;
-  int pending = 0;   // first chunk whose updates are not in the inverse yet
-  for (int ci = 0; ci < nchunks; ++ci) {
-    const int e = ci * chunk;
-    a.det = det;
-    a.e_begin = det * n + e;
-    a.e_end = det * n + std::min(n, e + chunk);
-    a.chunk = ci;
-    a.chunk0 = pending;
+  a.det = det;
+  a.chunk_len = chunk;
+  a.inflush = 0;
+  // one launch for all the chunks when no flush launch is needed between them: Sherman-Morrison
+  // moves, or each chunk's updates applied by the move kernel itself
+  const bool inflush = b->K > 1 && use_mfma_flush<T>(b) && flush_own_fits<T>(s->N, n, b->K, chunk, nt / 64);
+  if (b->K == 1 || inflush) {
+    a.e_begin = det * n;
+    a.e_end = det * n + n;
+    a.chunk = 0;
+    a.nchunks = nchunks;
+    a.inflush = inflush ? (flush ? 2 : 1) : 0;
     if (b->prof_record(0)) return QMCX_ERR_HIP;
     if (launch_moves<T>(b, nt, L.total, a)) return QMCX_ERR_HIP;
     if (b->prof_close()) return QMCX_ERR_HIP;
-    const bool last = ci + 1 == nchunks;
-    if (b->K > 1 && (ci + 1 - pending == b->flush_group || (last && flush))) {
+    return 0;
+  }
+  a.nchunks = 1;
+  for (int ci = 0; ci < nchunks; ++ci) {
+    const int e = ci * chunk;
+    a.e_begin = det * n + e;
+    a.e_end = det * n + std::min(n, e + chunk);
+    a.chunk = ci;
+    if (b->prof_record(0)) return QMCX_ERR_HIP;
+    if (launch_moves<T>(b, nt, L.total, a)) return QMCX_ERR_HIP;
+    if (b->prof_close()) return QMCX_ERR_HIP;
+    if (b->K > 1 && (ci + 1 < nchunks || flush)) {
       if (b->prof_record(1)) return QMCX_ERR_HIP;
-      if (int rc = launch_flush<T>(b, b->stream, det, pending, ci + 1)) return rc;
+      if (int rc = launch_flush<T>(b, b->stream, det, ci, ci + 1)) return rc;
       if (b->prof_close()) return QMCX_ERR_HIP;
-      pending = ci + 1;
     }
   }
   return 0;
@@ -826,8 +838,8 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
       return rc;
     }
     const int chunk = move_chunk(s->n, delay_rank);
-    const size_t lds = precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, chunk, false).total
-                                              : move_lds<double>(s->N, s->nions, s->n, delay_rank, chunk, false).total;
+    const size_t lds = precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, chunk).total
+                                              : move_lds<double>(s->N, s->nions, s->n, delay_rank, chunk).total;
     if (lds > 160 * 1024) {
       delete b;
       return fail(QMCX_ERR_ARG, "walker state (" + std::to_string(lds / 1024) +
@@ -961,19 +973,6 @@ int qmcx_batch_set_tile(qmcx_batch_t b, int spline_slices) {
   return 0;
 }
 
-int qmcx_batch_set_flush_group(qmcx_batch_t b, int chunks) {
-  if (!b || chunks < 1 || chunks > b->nch) return fail(QMCX_ERR_ARG, "flush group out of range");
-  if (chunks > 1) {
-    const qmcx_system_s* s = b->sys;
-    const int chunk = move_chunk(s->n, b->K);
-    const size_t lds = b->prec == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, b->K, chunk, true).total
-                                            : move_lds<double>(s->N, s->nions, s->n, b->K, chunk, true).total;
-    if (lds > 160 * 1024) return fail(QMCX_ERR_ARG, "grouped flushes need more LDS than the move kernel has");
-  }
-  b->flush_group = chunks;
-  return 0;
-}
-
 int qmcx_evaluate_log(qmcx_batch_t b, double* log_out) {
   if (!b) return fail(QMCX_ERR_ARG, "null batch");
   int rc = b->prec == QMCX_FP32 ? evaluate_log_impl<float>(b) : evaluate_log_impl<double>(b);
@@ -1097,16 +1096,18 @@ static int flex_launch(qmcx_batch_t b, int iel, bool apply) {
   a.det = iel < s->n ? 0 : 1;
   a.e_begin = iel;
   a.e_end = iel + 1;
+  a.chunk_len = 1;
+  a.nchunks = 1;
   a.mode = QMCX_ACCEPT_METROPOLIS;
   a.slices = 0;
   a.decide = b->flex_decide;
   const bool trace_was = b->trace_on;
   b->trace_on = true;
   if (f32) {
-    const MoveLds L = move_lds<float>(s->N, s->nions, s->n, b->K, 1, false);
+    const MoveLds L = move_lds<float>(s->N, s->nions, s->n, b->K, 1);
     rc = launch_moves<float>(b, nt, L.total, a);
   } else {
-    const MoveLds L = move_lds<double>(s->N, s->nions, s->n, b->K, 1, false);
+    const MoveLds L = move_lds<double>(s->N, s->nions, s->n, b->K, 1);
     rc = launch_moves<double>(b, nt, L.total, a);
   }
   b->trace_on = trace_was;
@@ -1203,6 +1204,17 @@ int qmcx_last_sweep_ms(qmcx_batch_t b, double* ms) {
 // Profiling builds only: per-phase wall_clock64 ticks (walker / matrix 0, lane 0); slots 0-7
 // and 12-22 the move kernel's phases, 8-11 the FP64 inverse's transpose, panels, updates and output,
 // 24-29 the flush's row loads, staging, products and stores.
+extern "C" int qmcx_debug_walker_ticks(double* out, int reset) {
+  static unsigned long long t[128 * 256];
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpyFromSymbol(t, HIP_SYMBOL(qmcx_walker_ticks), sizeof(t)));
+  for (int i = 0; i < 128 * 256; ++i) out[i] = (double)t[i];
+  if (reset) {
+    static unsigned long long z[128 * 256];
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(qmcx_walker_ticks), z, sizeof(z)));
+  }
+  return 0;
+}
 extern "C" int qmcx_debug_phase_ticks(double* out32, int reset) {
   unsigned long long t[qmcx::kPhaseSlots];
   HIPCHK(hipDeviceSynchronize());

@@ -69,10 +69,11 @@ __host__ __device__ inline size_t state_slot(const BatchP<T>& B, int w, int det,
 
 struct MoveArgs {
   int e_begin, e_end, det, mode, slices;
-  // index of this chunk in its determinant's sweep; the delayed updates of chunks
-  // chunk0 .. chunk-1 (state slots) are not in Ainv yet and are applied to this chunk's rows
-  // at set-up
-  int chunk, chunk0;
+  // The launch moves electrons [e_begin, e_end) in nchunks chunks of chunk_len; chunk is the
+  // index of its first chunk in the determinant's sweep.  inflush: the kernel applies each
+  // chunk's delayed updates to the walker's whole inverse before the next chunk (1), and after
+  // the last chunk too (2); 0: a flush launch follows each single-chunk launch.
+  int chunk, chunk_len, nchunks, inflush;
   const int* decide;     // [W] caller accept decisions (flex_acceptrestoreMove) or null
   const void* spo;       // [W][2][n][5][npad] T: SPO rows at the proposals (k_spo_proposals)
   const double* prop;    // [W][2][n][kProp] proposal records (k_spo_proposals)
@@ -283,20 +284,21 @@ __global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, i
 //            (orbital 16ct + 4g + r of MFMA r is the k index g), D[d][i] = T[i][d]
 //   GEMM 2:  Ainv_rows^T += V^T T2^T  A = V^T (16 orbitals x 4 delayed rows), B = T2^T,
 //            accumulators started from the rows themselves
-// V (the stale inverse rows) is staged in LDS per workgroup when the delay rank allows.
+// V (the stale inverse rows) and U (the new Slater rows) are staged in LDS per workgroup when
+// the delay rank allows.
 constexpr int kFlushWaves = 12;     // waves per workgroup, 16 rows each
 constexpr int kFlushBM = 16 * kFlushWaves;   // rows per workgroup
 constexpr int kFlushTiles = 24;     // 16-column tiles held in registers: n <= 384
 constexpr int kFlushBinvLds = 64;   // Binv staged in LDS up to this delay rank
-constexpr int kFlushVLds = 32;      // V staged in LDS up to this delay rank
+constexpr int kFlushVLds = 32;      // V and U staged in LDS up to this delay rank
 __host__ __device__ inline bool flush_mfma_fits(int n) { return n % 16 == 0 && n <= 16 * kFlushTiles; }
 __host__ __device__ inline int flush_v_stride(int n) { return n + 16; }   // LDS row stride of V
 // LDS of k_flush_mfma: per wave T and T2 (16 x cp floats each), Binv (K x K doubles when
-// K <= kFlushBinvLds), V (cp x (n + 16) floats when K <= kFlushVLds), the row list
+// K <= kFlushBinvLds), V and U (cp x (n + 16) floats each when K <= kFlushVLds), the row list
 __host__ __device__ inline size_t flush_mfma_lds(int n, int K) {
   const int cp = (K + 15) & ~15;
   const size_t kb = K <= kFlushBinvLds ? (size_t)(K * K + 1) & ~(size_t)1 : 0;   // keeps V 16-byte aligned
-  const size_t vb = K <= kFlushVLds ? (size_t)cp * flush_v_stride(n) : 0;
+  const size_t vb = K <= kFlushVLds ? (size_t)2 * cp * flush_v_stride(n) : 0;
   return sizeof(float) * kFlushWaves * 2 * 16 * cp + sizeof(double) * kb + sizeof(float) * vb + sizeof(int) * K;
 }
 
@@ -316,7 +318,8 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
   const bool v_lds = K <= kFlushVLds;
   double* sB = (double*)((float*)smem + kFlushWaves * 2 * 16 * KC);        // [c][c]
   float* sV = (float*)(sB + (binv_lds ? (K * K + 1) & ~1 : 0));            // [cp][n + 16]
-  int* sRows = (int*)(sV + (v_lds ? (size_t)KC * flush_v_stride(n) : 0));
+  float* sU = sV + (v_lds ? (size_t)KC * flush_v_stride(n) : 0);           // [cp][n + 16]
+  int* sRows = (int*)(sU + (v_lds ? (size_t)KC * flush_v_stride(n) : 0));
   const size_t moff = ((size_t)w * 2 + det) * n * n;
   float* Ainv = B.Ainv + moff;
   const float* Amat = B.A + moff;
@@ -359,21 +362,27 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
     if (binv_lds)
       for (int i = tid; i < c * c; i += nt) sB[i] = Bi[(i / c) * K + i % c];
     if (v_lds) {
-      // V rows, zero-padded to cp rows, float4 at a time
+      // V rows and the U rows they pair with, zero-padded to cp rows, float4 at a time; the
+      // row list comes from HBM here too (it is being staged beside)
       const int q = n / 4;
       for (int i = tid; i < cp * q; i += nt) {
         const int d = i / q, jj = i - d * q;
-        f32x4_t v = *reinterpret_cast<const f32x4_t*>(Vb + (size_t)(d < c ? d : 0) * n + 4 * jj);
-        if (d >= c) v = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        const int dc = d < c ? d : 0;
+        f32x4_t v = *reinterpret_cast<const f32x4_t*>(Vb + (size_t)dc * n + 4 * jj);
+        f32x4_t u = *reinterpret_cast<const f32x4_t*>(Amat + (size_t)B.rows[sl * K + dc] * n + 4 * jj);
+        if (d >= c) {
+          v = f32x4_t{0.f, 0.f, 0.f, 0.f};
+          u = v;
+        }
         *reinterpret_cast<f32x4_t*>(sV + (size_t)d * vs + 4 * jj) = v;
+        *reinterpret_cast<f32x4_t*>(sU + (size_t)d * vs + 4 * jj) = u;
       }
     }
     __syncthreads();
     FL_TICK(25)
-    // ---- GEMM 1: T^T tile by tile of 16 delayed rows; two accumulators split the orbitals
-    for (int dt = 0; dt < cp / 16; ++dt) {
-      const int d = dt * 16 + li;
-      const float* ur = Amat + (size_t)(d < c ? sRows[d] : sRows[0]) * n + 4 * g;
+    // ---- GEMM 1: T^T tile by tile of 16 delayed rows; two accumulators split the orbitals.
+    // U rows from LDS (staged) or HBM, in separate loops so that each is its own kind of load.
+    auto gemm1 = [&](const float* ur, int dt, int d) {
       f32x4_t acc0 = f32x4_t{0.f, 0.f, 0.f, 0.f}, acc1 = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int ct = 0; ct < kFlushTiles; ++ct) {
@@ -395,6 +404,11 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
         if (dd < c && sRows[dd] == r0 + li) v -= 1.f;
         sT[li * KC + dd] = dd < c ? v : 0.f;
       }
+    };
+    for (int dt = 0; dt < cp / 16; ++dt) {
+      const int d = dt * 16 + li;
+      if (v_lds) gemm1(sU + (size_t)d * vs + 4 * g, dt, d);
+      else gemm1(Amat + (size_t)(d < c ? sRows[d] : sRows[0]) * n + 4 * g, dt, d);
     }
     wave_lds_fence();   // T and T2 are this wave's own
     FL_TICK(26)

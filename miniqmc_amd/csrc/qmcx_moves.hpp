@@ -19,21 +19,8 @@ namespace qmcx {
 constexpr int kRed = 12;   // reduction slots: gnow(3) ratio(1) gnew(3) J2(5)
 constexpr int kMoveThreads = 512;
 
-// LDS of the set-up correction: T and T2 (E x KC each, T) and the MFMA partial tiles
-// (16 x 16 floats per job, at least one job per wave)
-__host__ __device__ inline int cor_jobs(int K, int E) {
-  const int tiles = ((E + 15) / 16) * ((K + 15) / 16);
-  return tiles > kMoveThreads / 64 ? tiles : kMoveThreads / 64;
-}
-template <typename T>
-__host__ __device__ inline size_t cor_lds_bytes(int K, int E) {
-  const int KC = (K + 15) & ~15;
-  return sizeof(T) * 2 * (size_t)E * KC + sizeof(float) * 256 * (size_t)cor_jobs(K, E);
-}
-
 struct MoveLds {
-  unsigned oR, oRed, oInv, oPsi, oBinv, oP, oPart, oRows, oAl, oJ1n, oFn, oCell, oProp, oJ1o, oJt, oJ2, oAc, oQ, oCor,
-      oRall, total;
+  unsigned oR, oRed, oInv, oPsi, oBinv, oP, oPart, oRows, oAl, oJ1n, oFn, oCell, oProp, oJ1o, oJt, oJ2, oAc, oQ, oAcc, total;
 };
 __host__ __device__ inline unsigned lds_take(unsigned& o, size_t bytes) {
   unsigned r = o;
@@ -41,11 +28,9 @@ __host__ __device__ inline unsigned lds_take(unsigned& o, size_t bytes) {
   return r;
 }
 // E = electrons per chunk; the inverse-row cache is used when K > 1 (M, the ion count, is
-// not needed: the J1 terms come with the proposals).  The last two regions, for the set-up
-// correction of grouped flushes, are sized only with `corr` (the kernel touches them only
-// when a chunk has pending updates).
+// not needed: the J1 terms come with the proposals)
 template <typename T>
-__host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E, bool corr = true) {
+__host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E) {
   (void)M;
   MoveLds L;
   unsigned o = 0;
@@ -62,14 +47,15 @@ __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E, b
   L.oJ1n = lds_take(o, sizeof(T) * 5 * E);                 // their new J1 state
   L.oFn = lds_take(o, sizeof(Functor<T>) * 2);             // J2 functors (same / opposite spin)
   L.oCell = lds_take(o, sizeof(Cell<T>));
+  L.oAcc = lds_take(o, sizeof(double) * 8);                 // the walker's sums over the launch
   L.oProp = lds_take(o, sizeof(double) * kProp * E);        // the chunk's proposal records
   L.oJ1o = lds_take(o, sizeof(T) * 4 * E);                  // J1 value and gradient of its electrons
-  L.oJt = lds_take(o, sizeof(T) * 5 * N);                   // J2 pair terms at the current proposal
   L.oJ2 = lds_take(o, sizeof(T) * 5 * N);
   L.oAc = lds_take(o, cache ? sizeof(T) * (size_t)E * n : 0);
   L.oQ = lds_take(o, cache ? sizeof(double) * (size_t)K * E : 0);
-  L.oCor = lds_take(o, cache && corr ? cor_lds_bytes<T>(K, E) : 0);   // earlier chunks' updates at set-up
-  L.oRall = lds_take(o, cache && corr ? sizeof(int) * (size_t)(K + 1) * ((n + K - 1) / K) : 0);   // row lists, counts
+  // J2 pair terms at the current proposal; with the region before it, the scratch of the
+  // in-kernel flush between chunks
+  L.oJt = lds_take(o, sizeof(T) * 5 * N);
   L.total = o;
   return L;
 }
@@ -106,6 +92,8 @@ __device__ inline void j2_pair(const Cell<T>& C, const Functor<T>& f, double xj,
 // this table when the kernel ends (a global read-modify-write per tick would time itself)
 constexpr int kPhaseSlots = 32;
 __device__ unsigned long long qmcx_phase_ticks[kPhaseSlots];
+// per-walker duration of each chunk launch: [det * 64 + chunk][walker < 256]
+__device__ unsigned long long qmcx_walker_ticks[128 * 256];
 #define QMCX_TICK(slot)                                                  \
   if (w == 0 && tid == 0) {                                              \
     const unsigned long long now_ = wall_clock64();                      \
@@ -114,7 +102,9 @@ __device__ unsigned long long qmcx_phase_ticks[kPhaseSlots];
   }
 #define QMCX_TICK_FLUSH()                                                \
   if (w == 0 && tid == 0)                                                \
-    for (int i_ = 0; i_ < kPhaseSlots; ++i_) qmcx_phase_ticks[i_] += tacc_[i_];
+    for (int i_ = 0; i_ < kPhaseSlots; ++i_) qmcx_phase_ticks[i_] += tacc_[i_]; \
+  if (tid == 0 && w < 256 && a.chunk < 64)                               \
+    qmcx_walker_ticks[(a.det * 64 + a.chunk) * 256 + w] += wall_clock64() - wstart_;
 #else
 #define QMCX_TICK(slot)
 #define QMCX_TICK_FLUSH()
@@ -146,209 +136,122 @@ __device__ inline void j2_items(const Cell<T>& C, const Functor<T>* fn, const do
   }
 }
 
-// The chunk's stale inverse rows (sAc, E x n, in LDS) receive the delayed updates that the
-// determinant's earlier chunks j0 .. j0+nj-1 of this sweep left in their state slots, in order:
-// updateInvMat (DelayedUpdate.h:176-193) restricted to these rows,
-//   sAc <- sAc + T2_j V_j,  T2_j = -(sAc U_j^T) B_j
-// (U_j rows: the new Slater rows of chunk j's accepted moves, V_j: their stale inverse rows,
-// B_j: chunk j's capacitance inverse; none of chunk j's rows is among these, so the identity
-// term of the flush is absent).  sRall holds the delayed-row lists of those slots (K each),
-// sNd their counts.  FP32 with n % 16 == 0 runs both products on the matrix cores with their global
-// loads batched; otherwise vector code.  Ends with a barrier.
+// Scratch of flush_own: T (in place T2) per wave, 16 x KC floats
+__host__ __device__ inline size_t flush_own_scratch(int K, int nwaves) {
+  return sizeof(float) * (size_t)nwaves * 16 * ((K + 15) & ~15);
+}
+// Whether the move kernel can apply each chunk's delayed updates to its walker's whole
+// inverse itself (FP32, n a multiple of 16 up to 16 kFlushOwnTiles, K <= 32, the scratch
+// within the LDS the chunk regions leave free).
+constexpr int kFlushOwnTiles = 24;
 template <typename T>
-__device__ inline void correct_rows(const SysP<T>& S, const BatchP<T>& B, int w, int det, int j0, int nj, int E, T* sAc,
-                                    const int* sRall, const int* sNd, double* sBj, T* sT, T* sT2, float* sPart,
-                                    unsigned long long* crt) {
-#ifdef QMCX_PHASE_TIMING
-  unsigned long long ct_ = wall_clock64();
-#define CR_TICK(i)                                      \
-  if (w == 0 && threadIdx.x == 0) {                     \
-    const unsigned long long t_ = wall_clock64();       \
-    crt[i] += t_ - ct_;                                 \
-    ct_ = t_;                                           \
-  }
-#else
-#define CR_TICK(i)
-#endif
-  const int n = S.n, K = B.K, KC = (K + 15) & ~15;
-  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
+__host__ __device__ inline bool flush_own_fits(int N, int n, int K, int E, int nwaves) {
+  return sizeof(T) == 4 && K > 1 && K <= 32 && n % 16 == 0 && n <= 16 * kFlushOwnTiles &&
+         sizeof(double) * (size_t)K * E + sizeof(T) * 5 * (size_t)N >= flush_own_scratch(K, nwaves);
+}
+
+// updateInvMat (DelayedUpdate.h:176-193) of this walker's whole inverse by its own workgroup,
+// between chunks: Ainv += T2 V with T = Ainv U^T - E, T2 = -T Binv, in 16-row tiles over the
+// waves.  A tile is held in registers half its orbitals at a time, in the layout of
+// k_flush_mfma (lane (i, g): row i, orbitals 16ct + 4g .. +3): the first product sums both
+// halves, the second updates each half and stores it.  V rows are the chunk's stale inverse
+// rows (in LDS: sAc rows sAl[d]), U rows the new Slater rows (HBM, written at the chunk's end,
+// six orbital tiles' loads in flight), Binv in LDS (row stride KP).
+constexpr int kFlushOwnHalf = kFlushOwnTiles / 2;
+__device__ inline void flush_own(int n, int K, int KP, int c, const int* sRows, const int* sAl, const double* sBinv,
+                                 const float* sAc, const float* Amat, float* Ainv, float* scratch) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
   const int li = lane & 15, g = lane >> 4;
-  const T* Amat = B.A + ((size_t)w * 2 + det) * n * n;
-  const bool mfma = sizeof(T) == 4 && n % 16 == 0;
-  for (int j = 0; j < nj; ++j) {
-    const size_t sl = state_slot(B, w, det, j0 + j);
-    const int cj = sNd[j];
-    if (cj == 0) continue;
-    const int cpj = (cj + 15) & ~15;
-    const T* Vj = B.Vbuf + sl * K * n;
-    const double* Bg = B.Binv + sl * K * K;
-    const int* rj = sRall + j * K;
-    // chunk j's capacitance inverse: loads issued now, stored to LDS after the first product
-    constexpr int kBr = 4;
-    double br[kBr];
-    const int nb = cj * cj;
+  const int KC = (K + 15) & ~15, cp = (c + 15) & ~15, nct = n >> 4;
+  float* sT = scratch + (size_t)wid * 16 * KC;
+  for (int tile = wid; tile < nct; tile += nw) {
+    const int r0 = tile * 16;
+    const float* ar = Ainv + (size_t)(r0 + li) * n + 4 * g;
+    // ---- GEMM 1: T^T = U rows^T per 16 delayed rows, the orbitals in two halves
+    for (int dt = 0; dt < cp / 16; ++dt) {
+      const int d = dt * 16 + li;
+      const float* ur = Amat + (size_t)sRows[d < c ? d : 0] * n + 4 * g;
+      f32x4_t acc0 = f32x4_t{0.f, 0.f, 0.f, 0.f}, acc1 = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      for (int h0 = 0; h0 < nct; h0 += kFlushOwnHalf) {
+        f32x4_t a[kFlushOwnHalf];
 #pragma unroll
-    for (int q = 0; q < kBr; ++q) {
-      const int i = min(tid + q * nt, nb - 1);   // unconditional loads
-      br[q] = Bg[(i / cj) * K + i % cj];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    CR_TICK(0)
-    // ---- T = sAc U_j^T (E x cj)
-    if constexpr (sizeof(T) == 4) {
-      if (mfma) {
-        // 16x16 tiles of T, the orbital sum in slices over the waves; lane (i, g) supplies
-        // A[i][g] = sAc[e][k], B[g][i] = U_j[d][k] with k = 16kt + 4g + r for MFMA r; six
-        // orbital tiles' loads in flight at a time
-        const int ET = (E + 15) >> 4, DT = cpj >> 4, tiles = ET * DT;
-        const int ks = tiles >= nw ? 1 : nw / tiles;
-        const int nkt = n >> 4, per = (nkt + ks - 1) / ks;
-        for (int job = wid; job < tiles * ks; job += nw) {
-          const int tile = job % tiles, sc = job / tiles;
-          const int et = tile % ET, dt = tile / ET;
-          const int e = et * 16 + li, d = dt * 16 + li;
-          const float* ar = (const float*)sAc + (size_t)(e < E ? e : 0) * n + 4 * g;
-          const float* ur = (const float*)Amat + (size_t)rj[d < cj ? d : 0] * n + 4 * g;
-          f32x4_t acc = f32x4_t{0.f, 0.f, 0.f, 0.f};
-          const int k1 = min(nkt, (sc + 1) * per);
-          for (int kb = sc * per; kb < k1; kb += 6) {
-            f32x4_t av[6], uv[6];
+        for (int q = 0; q < kFlushOwnHalf; ++q)
+          a[q] = *reinterpret_cast<const f32x4_t*>(ar + min(h0 + q, nct - 1) * 16);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-              const int kt = min(kb + q, k1 - 1);
-              uv[q] = *reinterpret_cast<const f32x4_t*>(ur + kt * 16);
-              av[q] = *reinterpret_cast<const f32x4_t*>(ar + kt * 16);
-            }
-            __builtin_amdgcn_sched_barrier(0);   // all six loads issued before the first MFMA
+        for (int kb = 0; kb < kFlushOwnHalf; kb += 6) {
+          f32x4_t u[6];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-              if (e >= E || kb + q >= k1) av[q] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-              if (d >= cj) uv[q] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q][0], uv[q][0], acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q][1], uv[q][1], acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q][2], uv[q][2], acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q][3], uv[q][3], acc, 0, 0, 0);
-            }
-          }
-          // register r of lane (i, g): row e = 16et + 4g + r, delayed row d = 16dt + i
+          for (int q = 0; q < 6; ++q)
+            u[q] = *reinterpret_cast<const f32x4_t*>(ur + min(h0 + kb + q, nct - 1) * 16);
+          __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-          for (int r = 0; r < 4; ++r) sPart[job * 256 + (4 * g + r) * 16 + li] = acc[r];
-        }
-#pragma unroll
-        for (int q = 0; q < kBr; ++q)
-          if (tid + q * nt < nb) sBj[tid + q * nt] = br[q];
-        __syncthreads();
-        CR_TICK(1)
-        for (int idx = tid; idx < E * cpj; idx += nt) {
-          const int e = idx / cpj, d = idx - e * cpj;
-          const int tile = (e >> 4) + ET * (d >> 4);
-          float v = 0.f;
-          for (int sc = 0; sc < ks; ++sc) v += sPart[(sc * tiles + tile) * 256 + (e & 15) * 16 + (d & 15)];
-          sT[e * KC + d] = d < cj ? v : 0.f;
-        }
-      }
-    }
-    if (!mfma) {
-      for (int idx = wid; idx < E * cj; idx += nw) {
-        const int e = idx / cj, d = idx - e * cj;
-        const T* ur = Amat + (size_t)rj[d] * n;
-        double v = 0;
-        for (int k = lane; k < n; k += 64) v += (double)sAc[(size_t)e * n + k] * (double)ur[k];
-        v = wave_sum(v);
-        if (lane == 0) sT[e * KC + d] = (T)v;
-      }
-#pragma unroll
-      for (int q = 0; q < kBr; ++q)
-        if (tid + q * nt < nb) sBj[tid + q * nt] = br[q];
-    }
-    for (int i = tid + kBr * nt; i < nb; i += nt) sBj[i] = Bg[(i / cj) * K + i % cj];
-    __syncthreads();
-    CR_TICK(2)
-    // ---- T2 = -(T B_j), FP64 products
-    for (int idx = tid; idx < E * cpj; idx += nt) {
-      const int e = idx / cpj, d = idx - e * cpj;
-      double v = 0;
-      if (d < cj)
-        for (int q = 0; q < cj; ++q) v += (double)sT[e * KC + q] * sBj[q * cj + d];
-      sT2[e * KC + d] = (T)(-v);
-    }
-    __syncthreads();
-    CR_TICK(3)
-    // ---- sAc += T2 V_j
-    if constexpr (sizeof(T) == 4) {
-      if (mfma) {
-        // 16x16 tiles of sAc (16 rows x 16 orbitals); a wave loads the operands of up to six of
-        // its tiles (V from HBM, T2 and the tiles from LDS) for 32 delayed rows at a time before
-        // their MFMAs; lane (i, g) supplies A[i][g] = T2[e][d0+g], B[g][i] = V_j[d0+g][k];
-        // register r of lane (i, g) is row 16et + 4g + r, orbital 16kt + i
-        const int ET = (E + 15) >> 4, nkt = n >> 4, ntile = ET * nkt;
-        constexpr int kTG = 6;
-        for (int t0 = wid; t0 < ntile; t0 += kTG * nw) {
-          f32x4_t acc[kTG];
-          for (int db = 0; db < cpj; db += 32) {
-            const bool full = cpj - db >= 32;   // 8 or 4 steps of 4 delayed rows
-            float bv[kTG][8], av[kTG][8];
-#pragma unroll
-            for (int h = 0; h < kTG; ++h) {
-              const int t = min(t0 + h * nw, ntile - 1);
-              const int et = t % ET, kt = t / ET;
-              const int ea = min(et * 16 + li, E - 1);
-#pragma unroll
-              for (int s4 = 0; s4 < 8; ++s4) {
-                const int dk = db + 4 * s4 + g;
-                bv[h][s4] = (float)Vj[(size_t)min(dk, cj - 1) * n + kt * 16 + li];
-                av[h][s4] = (float)sT2[ea * KC + min(dk, KC - 1)];
-              }
-              if (db == 0)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                  const int e = min(et * 16 + 4 * g + r, E - 1);
-                  acc[h][r] = ((const float*)sAc)[(size_t)e * n + kt * 16 + li];
-                }
-            }
-            __builtin_amdgcn_sched_barrier(0);   // every operand requested before the first MFMA
-#pragma unroll
-            for (int h = 0; h < kTG; ++h) {
-              const int t = min(t0 + h * nw, ntile - 1);
-              const bool arow = (t % ET) * 16 + li < E;
-#pragma unroll
-              for (int s4 = 0; s4 < 8; ++s4) {
-                const int dk = db + 4 * s4 + g;
-                const float a2 = arow && dk < cj ? av[h][s4] : 0.f;
-                const float b2 = dk < cj ? bv[h][s4] : 0.f;
-                if (s4 < 4 || full) acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2, acc[h], 0, 0, 0);
-              }
-            }
-            if (db + 32 >= cpj) {
-#pragma unroll
-              for (int h = 0; h < kTG; ++h) {
-                const int t = t0 + h * nw;
-                if (t < ntile) {
-                  const int et = t % ET, kt = t / ET;
-#pragma unroll
-                  for (int r = 0; r < 4; ++r) {
-                    const int e = et * 16 + 4 * g + r;
-                    if (e < E) ((float*)sAc)[(size_t)e * n + kt * 16 + li] = acc[h][r];
-                  }
-                }
-              }
-            }
+          for (int q = 0; q < 6; ++q) {
+            const bool in = d < c && h0 + kb + q < nct;
+            const f32x4_t uv = in ? u[q] : f32x4_t{0.f, 0.f, 0.f, 0.f};
+            f32x4_t& acc = (q & 1) ? acc1 : acc0;
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[0], a[kb + q][0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[1], a[kb + q][1], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[2], a[kb + q][2], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[3], a[kb + q][3], acc, 0, 0, 0);
           }
         }
       }
-    }
-    if (!mfma) {
-      for (int i = tid; i < E * n; i += nt) {
-        const int e = i / n, k = i - e * n;
-        double v = (double)sAc[i];
-        for (int d = 0; d < cj; ++d) v += (double)sT2[e * KC + d] * (double)Vj[(size_t)d * n + k];
-        sAc[i] = (T)v;
+      // lane (i, g) register r: T[i][16dt + 4g + r], minus E (row r0+i is delayed row dd)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int dd = dt * 16 + 4 * g + r;
+        float v = acc0[r] + acc1[r];
+        if (dd < c && sRows[dd] == r0 + li) v -= 1.f;
+        sT[li * KC + dd] = dd < c ? v : 0.f;
       }
     }
-    __syncthreads();
-    CR_TICK(4)
+    wave_lds_fence();
+    // ---- T2 = -(T Binv) in FP64, written over T once every lane has read its T row
+    float t2v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int idx = lane + 64 * q;
+      double sum = 0;
+      if (idx < 16 * cp) {
+        const int i = idx / cp, dd = idx - i * cp;
+        if (dd < c)
+          for (int e = 0; e < c; ++e) sum += (double)sT[i * KC + e] * sBinv[e * KP + dd];
+      }
+      t2v[q] = (float)(-sum);
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int idx = lane + 64 * q;
+      if (idx < 16 * cp) {
+        const int i = idx / cp, dd = idx - i * cp;
+        sT[i * KC + dd] = t2v[q];
+      }
+    }
+    wave_lds_fence();
+    // ---- GEMM 2: rows^T += V^T T2^T per half of the orbitals; V rows from LDS
+    for (int h0 = 0; h0 < nct; h0 += kFlushOwnHalf) {
+      f32x4_t a[kFlushOwnHalf];
+#pragma unroll
+      for (int q = 0; q < kFlushOwnHalf; ++q)
+        a[q] = *reinterpret_cast<const f32x4_t*>(ar + min(h0 + q, nct - 1) * 16);
+      for (int d0 = 0; d0 < cp; d0 += 4) {
+        const int dk = d0 + g;
+        const float bv = sT[li * KC + dk];
+        const float* vr = sAc + (size_t)sAl[dk < c ? dk : 0] * n + li;
+#pragma unroll
+        for (int q = 0; q < kFlushOwnHalf; ++q) {
+          const float v = dk < c ? vr[min(h0 + q, nct - 1) * 16] : 0.f;
+          a[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(v, bv, a[q], 0, 0, 0);
+        }
+      }
+      float* aw = Ainv + (size_t)(r0 + li) * n + 4 * g;
+#pragma unroll
+      for (int q = 0; q < kFlushOwnHalf; ++q)
+        if (h0 + q < nct) *reinterpret_cast<f32x4_t*>(aw + (h0 + q) * 16) = a[q];
+    }
   }
-#undef CR_TICK
+  __syncthreads();   // the next chunk reads these rows
 }
 
 // KD: delayed updates (K > 1: inverse-row cache, Woodbury bordering) or Sherman-Morrison (K = 1)
@@ -359,9 +262,8 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
   const int N = S.N, n = S.n, K = KD ? B.K : 1;
   const int KP = K + 1;   // LDS row stride of Binv: a column walk down its rows is free of bank conflicts
-  const int E = a.e_end - a.e_begin;
-  const int r0 = a.e_begin - a.det * n;
-  const MoveLds L = move_lds<T>(N, S.M, n, K, E);
+  const int Emax = a.chunk_len;   // electrons per chunk (the LDS layout is sized for it)
+  const MoveLds L = move_lds<T>(N, S.M, n, K, Emax);
   double* sR = (double*)(smem + L.oR);
   double* sRed = (double*)(smem + L.oRed);
   T* sInv = (T*)(smem + L.oInv);
@@ -382,6 +284,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   double* sQ = (double*)(smem + L.oQ);
 #ifdef QMCX_PHASE_TIMING
   unsigned long long tick_ = wall_clock64();
+  const unsigned long long wstart_ = tick_;
   unsigned long long tacc_[kPhaseSlots] = {};
 #endif
   const size_t moff = ((size_t)w * 2 + a.det) * n * n;
@@ -391,68 +294,65 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   T* d2psiM = B.d2psiM + moff;
   T* J2 = B.J2 + (size_t)w * 5 * N;
   T* J1 = B.J1 + (size_t)w * 5 * N;
+
+  // ---- walker state that stays in LDS across the launch's chunks: positions, J2 sums, J2
+  // functors, the cell
+  {
+    const LdsCopy cp[4] = {{sR, B.R + (size_t)w * 3 * N, (unsigned)(sizeof(double) * 3 * N)},
+                           {sJ2, J2, (unsigned)(sizeof(T) * 5 * N)},
+                           {sFn, S.fn + kMaxSpecies, (unsigned)(sizeof(Functor<T>) * 2)},
+                           {sCell, S.cell, (unsigned)sizeof(Cell<T>)}};
+    stage_lds(cp);
+  }
+  // a caller's accept decision (the flex stage calls) replaces the uniform: -inf accepts, +inf rejects
+  const double flex_u = a.decide == nullptr ? 0.0 : (a.decide[w] != 0 ? -INFINITY : INFINITY);
+  // sums over the launch kept by thread 0 in LDS: log|det| (logd, and the running product pdet
+  // folded into it before it leaves the double range), phase, J1 and J2 log changes, accepts
+  double* sAcc = (double*)(smem + L.oAcc);
+  enum { aLogd, aPdet, aPh, aJl1, aJl2, aNacc };
+  if (tid == 0) {
+    sAcc[aLogd] = 0;
+    sAcc[aPdet] = 1;
+    sAcc[aPh] = 0;
+    sAcc[aJl1] = 0;
+    sAcc[aJl2] = 0;
+    sAcc[aNacc] = 0;
+  }
+  const Cell<T>& C = *sCell;   // read from LDS where used (a register copy costs spills)
+  const int jr = tid < n ? tid : n - 1;
+
+  for (int ci = 0; ci < a.nchunks; ++ci) {
+  const int cb = a.e_begin + ci * Emax;     // the chunk's first electron
+  const int E = min(Emax, a.e_end - cb);
+  const int r0 = cb - a.det * n;
+  const int chunk = a.chunk + ci;           // index of the chunk in its determinant's sweep
   // this chunk's delayed-update state slot
-  const size_t slot = state_slot(B, w, a.det, KD ? a.chunk : 0);
+  const size_t slot = state_slot(B, w, a.det, KD ? chunk : 0);
   T* Vb = B.Vbuf + slot * K * n;
   // this chunk's proposals: SPO rows [e][5][npad] and records [e][kProp]
   const T* spo = (const T*)a.spo + (((size_t)w * 2 + a.det) * n + r0) * 5 * S.npad;
   const double* prop = a.prop + (((size_t)w * 2 + a.det) * n + r0) * kProp;
+  __syncthreads();   // the previous chunk's reads of the chunk regions are done
 
-  // ---- chunk set-up: walker state, the chunk's proposal records and J1 state, J2 functors and
-  // the cell into LDS (one round of loads)
+  // ---- chunk set-up: its stale inverse rows, proposal records and J1 state into LDS (one
+  // round of loads)
   {
     const unsigned je = (unsigned)(sizeof(T) * E);
-    const int nprev = KD ? a.chunk - a.chunk0 : 0;
-    const LdsCopy cp[11] = {{sR, B.R + (size_t)w * 3 * N, (unsigned)(sizeof(double) * 3 * N)},
-                            {sJ2, J2, (unsigned)(sizeof(T) * 5 * N)},
-                            {sAc, Ainv + (size_t)r0 * n, KD ? (unsigned)(sizeof(T) * E * n) : 0u},
-                            {sProp, prop, (unsigned)(sizeof(double) * kProp * E)},
-                            {sJ1o, J1 + a.e_begin, je},
-                            {sJ1o + E, J1 + N + a.e_begin, je},
-                            {sJ1o + 2 * E, J1 + 2 * N + a.e_begin, je},
-                            {sJ1o + 3 * E, J1 + 3 * N + a.e_begin, je},
-                            {sFn, S.fn + kMaxSpecies, (unsigned)(sizeof(Functor<T>) * 2)},
-                            {sCell, S.cell, (unsigned)sizeof(Cell<T>)},
-                            {smem + L.oRall, nprev > 0 ? B.rows + state_slot(B, w, a.det, a.chunk0) * K : B.rows,
-                             (unsigned)(sizeof(int) * nprev * K)}};
+    const LdsCopy cp[6] = {{sAc, Ainv + (size_t)r0 * n, KD ? (unsigned)(sizeof(T) * E * n) : 0u},
+                           {sProp, prop, (unsigned)(sizeof(double) * kProp * E)},
+                           {sJ1o, J1 + cb, je},
+                           {sJ1o + E, J1 + N + cb, je},
+                           {sJ1o + 2 * E, J1 + 2 * N + cb, je},
+                           {sJ1o + 3 * E, J1 + 3 * N + cb, je}};
     stage_lds(cp);
   }
   QMCX_TICK(16)
-  if (KD && a.chunk > a.chunk0) {
-    const int* sRall = (const int*)(smem + L.oRall);
-    int* sNd = (int*)(smem + L.oRall) + (size_t)K * ((n + K - 1) / K);
-    for (int j = tid; j < a.chunk - a.chunk0; j += nt) sNd[j] = B.ndelay[state_slot(B, w, a.det, a.chunk0 + j)];
-    __syncthreads();
-    const int KC = (K + 15) & ~15;
-    T* sCorT = (T*)(smem + L.oCor);
-    unsigned long long* crt = nullptr;
-#ifdef QMCX_PHASE_TIMING
-    __shared__ unsigned long long crt_s[5];
-    if (tid < 5) crt_s[tid] = 0;
-    __syncthreads();
-    crt = crt_s;
-#endif
-    correct_rows<T>(S, B, w, a.det, a.chunk0, a.chunk - a.chunk0, E, sAc, sRall, sNd, sBinv, sCorT,
-                    sCorT + (size_t)E * KC,
-                    (float*)(sCorT + (size_t)2 * E * KC), crt);
-#ifdef QMCX_PHASE_TIMING
-    if (w == 0 && tid == 0)
-      for (int i = 0; i < 5; ++i) tacc_[18 + i] += crt[i];
-#endif
-    (void)crt;
-  }
   QMCX_TICK(17)
-  // a caller's accept decision (the flex stage calls) replaces the uniform: -inf accepts, +inf rejects
-  const double flex_u = a.decide == nullptr ? 0.0 : (a.decide[w] != 0 ? -INFINITY : INFINITY);
   int c = 0, ca = 0;   // delayed rows; accepted moves of the chunk
-  double logd = 0, pdet = 1, ph = 0, jl1 = 0, jl2 = 0;
-  unsigned long long nacc = 0;
   __syncthreads();
-  const Cell<T>& C = *sCell;   // read from LDS where used (a register copy costs spills)
   // Rows a move reads from HBM that no earlier move of the chunk changes: the moving electron's
   // dpsiM row and the SPO row of its proposal.  Move e+1's are loaded at the start of move e
   // (software pipelining; index clamped, so the loads are unconditional).
-  const int jr = tid < n ? tid : n - 1;
   T ndp[3], np5[5];
   auto prefetch = [&](int ee) {
     const T* dq = dpsiM + (size_t)(r0 + ee) * 3 * n;
@@ -466,7 +366,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   prefetch(0);
 
   for (int e = 0; e < E; ++e) {
-    const int iel = a.e_begin + e, row = r0 + e;
+    const int iel = cb + e, row = r0 + e;
     QMCX_TICK(0)
     // ---- phase 1: issue the next move's global loads, then getInvRow's small algebra
     // stored state of the moving electron (J1, J2), its dpsiM row (loaded one move ahead) and
@@ -634,15 +534,16 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       if (tid == 0) {
         // log|det| of the accepted ratios as a running product, folded into the log before it
         // can leave the double range
-        pdet *= fabs(rdet);
+        double pdet = sAcc[aPdet] * fabs(rdet);
         if (!(pdet > 0x1p-500 && pdet < 0x1p500)) {
-          logd += log(pdet);
+          sAcc[aLogd] += log(pdet);
           pdet = 1.0;
         }
-        ph += rdet > 0.0 ? 0.0 : M_PI;
-        jl1 += (double)(Vat_old - (T)pr[kPropJ1]);
-        jl2 += (double)(Uat_old - (T)red[7]);
-        ++nacc;
+        sAcc[aPdet] = pdet;
+        sAcc[aPh] += rdet > 0.0 ? 0.0 : M_PI;
+        sAcc[aJl1] += (double)(Vat_old - (T)pr[kPropJ1]);
+        sAcc[aJl2] += (double)(Uat_old - (T)red[7]);
+        sAcc[aNacc] += 1.0;
         // the new J1 state of the electron (OneBodyJastrow.h:271-283), written at the chunk's end
         sAl[ca] = e;
         sJ1n[ca * 5 + 0] = (T)pr[kPropJ1 + 0];
@@ -777,20 +678,18 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     QMCX_TICK(4)
   }
 
-  // ---- chunk end: walker state back to HBM; the accepted rows' Slater matrix, gradient and
-  // Laplacian rows (DiracDeterminant.cpp:135-141) copied from the proposals' SPO rows, the
-  // stale inverse rows for the flush, the new J1 state
-  for (int i = tid; i < 3 * N; i += nt) B.R[(size_t)w * 3 * N + i] = sR[i];
-  for (int i = tid; i < 5 * N; i += nt) J2[i] = sJ2[i];
+  // ---- chunk end: the accepted rows' Slater matrix, gradient and Laplacian rows
+  // (DiracDeterminant.cpp:135-141) copied from the proposals' SPO rows, the stale inverse rows
+  // for the flush, the new J1 state
   for (int d = tid; d < ca; d += nt) {
-    const int ie = a.e_begin + sAl[d];
+    const int ie = cb + sAl[d];
     J1[ie] = sJ1n[d * 5 + 0];
     J1[N + ie] = sJ1n[d * 5 + 1];
     J1[2 * N + ie] = sJ1n[d * 5 + 2];
     J1[3 * N + ie] = sJ1n[d * 5 + 3];
     J1[4 * N + ie] = sJ1n[d * 5 + 4];
   }
-  if (KD)
+  if (KD && !a.inflush)
     for (int i = tid; i < ca * n; i += nt) {
       const int d = i / n, j = i - d * n;
       Vb[(size_t)d * n + j] = sAc[(size_t)sAl[d] * n + j];
@@ -826,17 +725,32 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       }
     }
   }
-  if (tid == 0) {
-    B.logdet[w * 2 + a.det] += logd + log(pdet);
-    B.phase[w * 2 + a.det] += ph;
-    B.jlog[w * 2 + 0] += jl1;
-    B.jlog[w * 2 + 1] += jl2;
-    B.nacc[w] += nacc;
-    B.ndelay[slot] = KD ? c : 0;
-  }
-  if (KD) {
+  // the chunk's delayed-update state for a flush launch (none when the kernel applies it)
+  if (tid == 0) B.ndelay[slot] = KD && !a.inflush ? c : 0;
+  if (KD && !a.inflush) {
     for (int i = tid; i < c * K; i += nt) B.Binv[slot * K * K + i] = sBinv[(i / K) * KP + i % K];
     for (int i = tid; i < c; i += nt) B.rows[slot * K + i] = sRows[i];
+  }
+  if constexpr (KD && sizeof(T) == 4) {
+    if (a.inflush && c > 0 && (ci + 1 < a.nchunks || a.inflush == 2)) {
+      // the chunk's delayed updates reach this walker's whole inverse before the next chunk
+      __syncthreads();
+      flush_own(n, K, KP, c, sRows, sAl, sBinv, (const float*)sAc, (const float*)Amat, (float*)Ainv,
+                (float*)(smem + L.oQ));
+    }
+  }
+  QMCX_TICK(23)
+  }   // chunks
+
+  // ---- walker state back to HBM
+  for (int i = tid; i < 3 * N; i += nt) B.R[(size_t)w * 3 * N + i] = sR[i];
+  for (int i = tid; i < 5 * N; i += nt) J2[i] = sJ2[i];
+  if (tid == 0) {
+    B.logdet[w * 2 + a.det] += sAcc[aLogd] + log(sAcc[aPdet]);
+    B.phase[w * 2 + a.det] += sAcc[aPh];
+    B.jlog[w * 2 + 0] += sAcc[aJl1];
+    B.jlog[w * 2 + 1] += sAcc[aJl2];
+    B.nacc[w] += (unsigned long long)sAcc[aNacc];
   }
   QMCX_TICK_FLUSH()
 }

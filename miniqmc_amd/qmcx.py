@@ -49,7 +49,6 @@ _SIGS = {
     "qmcx_batch_get_positions": [_vp, _dp],
     "qmcx_batch_set_walker_offset": [_vp, _i],
     "qmcx_batch_set_tile": [_vp, _i],
-    "qmcx_batch_set_flush_group": [_vp, _i],
     "qmcx_batch_set_flush": [_vp, _i],
     "qmcx_evaluate_log": [_vp, _vp],
     "qmcx_run_sweeps": [_vp, _i, C.c_uint64, C.c_double, _i, _i, _vp],
@@ -228,7 +227,7 @@ class WalkerBatch:
     """Walker state on the device and the batched VMC sweep."""
 
     def __init__(self, system: System, nwalkers: int, delay_rank: int = 1, precision: int = FP64,
-                 walker0: int = 0, tile: int = 0, flush_group: int = 0):
+                 walker0: int = 0, tile: int = 0):
         L = load()
         self.system = system
         self.nw = nwalkers
@@ -241,8 +240,6 @@ class WalkerBatch:
             _check(L.qmcx_batch_set_walker_offset(h, walker0))
         if tile:
             _check(L.qmcx_batch_set_tile(h, tile))
-        if flush_group:
-            _check(L.qmcx_batch_set_flush_group(h, flush_group))
 
     def close(self):
         if getattr(self, "h", None):

@@ -90,10 +90,11 @@ def test_invert_transpose_batched_vs_oracle():
         assert np.abs(psiM[m] - ref).max() < 1e-9 * np.abs(ref).max()
 
 
-def _compare_run(sysd, dev, orc, nw, nsweeps, delay, prec, mode, tau, tol_ratio, tol_log, seed=11, flush_group=0):
+def _compare_run(sysd, dev, orc, nw, nsweeps, delay, prec, mode, tau, tol_ratio, tol_log, seed=11, flush_kind=0):
     R0 = systems.initial_positions(sysd, nw, seed=seed)
     ref = orc.run_vmc(R0, nsweeps, seed, tau, mode=mode, delay=delay, trace=True)
-    wb = qmcx.WalkerBatch(dev, nw, delay_rank=delay, precision=prec, flush_group=flush_group)
+    wb = qmcx.WalkerBatch(dev, nw, delay_rank=delay, precision=prec)
+    wb.set_flush(flush_kind)
     wb.set_positions(R0)
     log0 = wb.evaluate_log()
     np.testing.assert_allclose(log0, ref["log_init"], rtol=tol_log)
@@ -122,13 +123,12 @@ def _compare_run(sysd, dev, orc, nw, nsweeps, delay, prec, mode, tau, tol_ratio,
     return dict(flips=flips, ref=ref, G=G, L=L, log=lv, acc=wb.accepts(), R=wb.positions())
 
 
-@pytest.mark.parametrize("delay,group", [(1, 0), (4, 0), (16, 0), (4, 5)])
-def test_sweep_fp64_vs_oracle(cubic, cubic_dev, cubic_orc, delay, group):
-    """group > 0: chunks of delayed updates flushed together, the pending ones applied to each
-    chunk's own rows at its set-up (vector code path)"""
-    _say("sweep FP64 delay", delay, "flush group", group)
+@pytest.mark.parametrize("delay", [1, 4, 16])
+def test_sweep_fp64_vs_oracle(cubic, cubic_dev, cubic_orc, delay):
+    """FP64: a move launch per chunk of k electrons, each followed by a flush launch"""
+    _say("sweep FP64 delay", delay)
     out = _compare_run(cubic, cubic_dev, cubic_orc, nw=3, nsweeps=2, delay=delay, prec=qmcx.FP64,
-                       mode=qmcx.ACCEPT_METROPOLIS, tau=0.5, tol_ratio=1e-9, tol_log=1e-11, flush_group=group)
+                       mode=qmcx.ACCEPT_METROPOLIS, tau=0.5, tol_ratio=1e-9, tol_log=1e-11)
     assert out["flips"] == 0
     ref = out["ref"]
     assert np.array_equal(out["acc"], ref["naccept"])
@@ -137,14 +137,15 @@ def test_sweep_fp64_vs_oracle(cubic, cubic_dev, cubic_orc, delay, group):
     np.testing.assert_allclose(out["L"], ref["L"], rtol=1e-8, atol=1e-7)
 
 
-@pytest.mark.parametrize("delay,group", [(1, 0), (8, 0), (32, 0), (8, 3), (16, 3)])
-def test_sweep_fp32_vs_fp64_oracle(cubic, cubic_dev, cubic_orc, delay, group):
-    """group > 0: grouped flushes with the set-up correction on the matrix cores"""
-    _say("sweep FP32 delay", delay, "flush group", group)
+@pytest.mark.parametrize("delay,kind", [(1, 0), (8, 0), (32, 0), (8, 1), (16, 1)])
+def test_sweep_fp32_vs_fp64_oracle(cubic, cubic_dev, cubic_orc, delay, kind):
+    """kind 0: one move launch per determinant, each chunk's updates applied by the move kernel
+    on the matrix cores; kind 1: a move launch per chunk and a vector-FMA flush launch"""
+    _say("sweep FP32 delay", delay, "flush kind", kind)
     # Metropolis acceptance: the FP32 path must reproduce the FP64 accept/reject sequence;
     # log|Psi| within 1e-6 relative of the FP64 oracle
     out = _compare_run(cubic, cubic_dev, cubic_orc, nw=3, nsweeps=2, delay=delay, prec=qmcx.FP32,
-                       mode=qmcx.ACCEPT_METROPOLIS, tau=0.5, tol_ratio=2e-3, tol_log=1e-6, flush_group=group)
+                       mode=qmcx.ACCEPT_METROPOLIS, tau=0.5, tol_ratio=2e-3, tol_log=1e-6, flush_kind=kind)
     assert out["flips"] == 0
 
 
