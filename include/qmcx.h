@@ -99,7 +99,8 @@ int qmcx_batch_set_walker_offset(qmcx_batch_t b, int walker0);
 /* spline stencil slices per workgroup (autotuner knob; 0 = default) */
 int qmcx_batch_set_tile(qmcx_batch_t b, int spline_slices);
 /* Delayed-update flush engine (DelayedUpdate::updateInvMat, src/QMCWaveFunctions/DelayedUpdate.h:176-193):
- * 0 = FP32 matrix cores (v_mfma_f32_16x16x4f32) when the batch is FP32 and n % 16 == 0, 1 = vector FMA. */
+ * 0 = FP32 matrix cores (v_mfma_f32_16x16x4f32) when the batch is FP32 and n % 16 == 0, 1 = vector FMA,
+ * 2 = matrix cores with the new Slater rows read from HBM rather than staged in LDS. */
 int qmcx_batch_set_flush(qmcx_batch_t b, int kind);
 
 /* WaveFunction::evaluateLog (src/QMCWaveFunctions/WaveFunction.cpp:178-195): SPO for all

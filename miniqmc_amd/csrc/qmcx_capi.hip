@@ -200,7 +200,7 @@ struct qmcx_batch_s {
   qmcx_system_s* sys = nullptr;
   int W = 0, K = 1, prec = QMCX_FP64, walker0 = 0, slices = 0, nt = 0;
   int nch = 1;   // delayed-update state slots per walker and determinant: one per chunk
-  int flush_kind = 0;   // 0: matrix cores when possible, 1: vector FMA
+  int flush_kind = 0;   // 0: matrix cores when possible, 1: vector FMA, 2: matrix cores, U rows from HBM
   double* flex_displ = nullptr;   // [W][3] proposals of the pending flex move
   int* flex_decide = nullptr;     // [W]
   int flex_iel = -1;
@@ -324,11 +324,12 @@ struct qmcx_batch_s {
 
 namespace {
 
-// workgroup size of the move kernel: a lane per orbital, at most two electrons
-// per lane for the J2 rows, at most kMoveThreads lanes
+// workgroup size of the move kernel: a lane per orbital, at most two electrons per lane for
+// the J2 rows, at most kMoveThreads lanes; a whole number of waves per SIMD (multiples of 256
+// lanes), so the four SIMDs of the CU share the J2 rows and the in-kernel flush evenly
 int choose_shape(const qmcx_system_s* s, int& nt) {
   const int need = std::max(s->n, (s->N + 1) / 2);
-  nt = std::max(256, (need + 63) / 64 * 64);
+  nt = std::max(256, (need + 255) / 256 * 256);
   if (nt > kMoveThreads) return fail(QMCX_ERR_ARG, "workgroup too large for this system");
   return 0;
 }
@@ -437,7 +438,7 @@ int evaluate_log_impl(qmcx_batch_t b) {
 // clipped); it can be switched off for comparisons
 template <typename T>
 bool use_mfma_flush(const qmcx_batch_t b) {
-  return sizeof(T) == 4 && flush_mfma_fits(b->sys->n) && b->flush_kind == 0;
+  return sizeof(T) == 4 && flush_mfma_fits(b->sys->n) && b->flush_kind != 1;
 }
 
 // Electrons per move-kernel launch: the delay rank (one flush per chunk), or up to 64 with
@@ -488,15 +489,17 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
   a.det = det;
   a.chunk_len = chunk;
   a.inflush = 0;
+  a.flush_u = 0;
   // one launch for all the chunks when no flush launch is needed between them: Sherman-Morrison
   // moves, or each chunk's updates applied by the move kernel itself
-  const bool inflush = b->K > 1 && use_mfma_flush<T>(b) && flush_own_fits<T>(s->N, n, b->K, chunk, nt / 64);
+  const bool inflush = b->K > 1 && use_mfma_flush<T>(b) && flush_own_fits<T>(s->N, n, b->K, chunk);
   if (b->K == 1 || inflush) {
     a.e_begin = det * n;
     a.e_end = det * n + n;
     a.chunk = 0;
     a.nchunks = nchunks;
     a.inflush = inflush ? (flush ? 2 : 1) : 0;
+    a.flush_u = b->flush_kind == 2 ? 0 : kFlushOwnU;
     if (b->prof_record(0)) return QMCX_ERR_HIP;
     if (launch_moves<T>(b, nt, L.total, a)) return QMCX_ERR_HIP;
     if (b->prof_close()) return QMCX_ERR_HIP;
@@ -840,7 +843,7 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
     const int chunk = move_chunk(s->n, delay_rank);
     const size_t lds = precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, chunk).total
                                               : move_lds<double>(s->N, s->nions, s->n, delay_rank, chunk).total;
-    if (lds > 160 * 1024) {
+    if (lds > kMoveLdsMax) {
       delete b;
       return fail(QMCX_ERR_ARG, "walker state (" + std::to_string(lds / 1024) +
                                     " KB) does not fit in the 160 KB of LDS; use a smaller delay rank");
@@ -960,7 +963,8 @@ int qmcx_batch_set_walker_offset(qmcx_batch_t b, int walker0) {
 }
 
 int qmcx_batch_set_flush(qmcx_batch_t b, int kind) {
-  if (!b || kind < 0 || kind > 1) return fail(QMCX_ERR_ARG, "flush kind must be 0 (matrix cores) or 1 (vector)");
+  if (!b || kind < 0 || kind > 2)
+    return fail(QMCX_ERR_ARG, "flush kind must be 0 (matrix cores), 1 (vector) or 2 (matrix cores, U rows from HBM)");
   b->flush_kind = kind;
   return 0;
 }

@@ -72,8 +72,10 @@ struct MoveArgs {
   // The launch moves electrons [e_begin, e_end) in nchunks chunks of chunk_len; chunk is the
   // index of its first chunk in the determinant's sweep.  inflush: the kernel applies each
   // chunk's delayed updates to the walker's whole inverse before the next chunk (1), and after
-  // the last chunk too (2); 0: a flush launch follows each single-chunk launch.
-  int chunk, chunk_len, nchunks, inflush;
+  // the last chunk too (2); 0: a flush launch follows each single-chunk launch.  The kernel's
+  // flush takes the new Slater rows from LDS when at most flush_u moves of the chunk were
+  // accepted, else from HBM.
+  int chunk, chunk_len, nchunks, inflush, flush_u;
   const int* decide;     // [W] caller accept decisions (flex_acceptrestoreMove) or null
   const void* spo;       // [W][2][n][5][npad] T: SPO rows at the proposals (k_spo_proposals)
   const double* prop;    // [W][2][n][kProp] proposal records (k_spo_proposals)

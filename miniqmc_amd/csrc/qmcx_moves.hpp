@@ -18,6 +18,14 @@ namespace qmcx {
 
 constexpr int kRed = 12;   // reduction slots: gnow(3) ratio(1) gnew(3) J2(5)
 constexpr int kMoveThreads = 512;
+constexpr size_t kMoveLdsMax = 160 * 1024;   // LDS of a workgroup
+
+// LDS of the in-kernel flush between chunks (FP32): the new Slater rows U of up to kFlushOwnU
+// accepted moves (row stride n + 4) and per wave a 16 x KC tile of T (in place T2)
+constexpr int kFlushOwnU = 16;
+__host__ __device__ inline size_t flush_own_lds(int n, int K) {
+  return sizeof(float) * ((size_t)kFlushOwnU * (n + 4) + (size_t)(kMoveThreads / 64) * 16 * ((K + 15) & ~15));
+}
 
 struct MoveLds {
   unsigned oR, oRed, oInv, oPsi, oBinv, oP, oPart, oRows, oAl, oJ1n, oFn, oCell, oProp, oJ1o, oJt, oJ2, oAc, oQ, oAcc, total;
@@ -56,6 +64,12 @@ __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E) {
   // J2 pair terms at the current proposal; with the region before it, the scratch of the
   // in-kernel flush between chunks
   L.oJt = lds_take(o, sizeof(T) * 5 * N);
+  // the in-kernel flush (FP32) reuses the regions from oQ on, grown to its size when the
+  // workgroup's LDS allows
+  if (cache && sizeof(T) == 4) {
+    const size_t need = L.oQ + flush_own_lds(n, K);
+    if (need > o && need <= kMoveLdsMax) o = (unsigned)need;
+  }
   L.total = o;
   return L;
 }
@@ -136,29 +150,141 @@ __device__ inline void j2_items(const Cell<T>& C, const Functor<T>* fn, const do
   }
 }
 
-// Scratch of flush_own: T (in place T2) per wave, 16 x KC floats
-__host__ __device__ inline size_t flush_own_scratch(int K, int nwaves) {
-  return sizeof(float) * (size_t)nwaves * 16 * ((K + 15) & ~15);
-}
 // Whether the move kernel can apply each chunk's delayed updates to its walker's whole
-// inverse itself (FP32, n a multiple of 16 up to 16 kFlushOwnTiles, K <= 32, the scratch
-// within the LDS the chunk regions leave free).
+// inverse itself (FP32, n a multiple of 16 up to 16 kFlushOwnTiles, K <= 32, the flush's LDS
+// within the workgroup's).
 constexpr int kFlushOwnTiles = 24;
 template <typename T>
-__host__ __device__ inline bool flush_own_fits(int N, int n, int K, int E, int nwaves) {
-  return sizeof(T) == 4 && K > 1 && K <= 32 && n % 16 == 0 && n <= 16 * kFlushOwnTiles &&
-         sizeof(double) * (size_t)K * E + sizeof(T) * 5 * (size_t)N >= flush_own_scratch(K, nwaves);
+__host__ __device__ inline bool flush_own_fits(int N, int n, int K, int E) {
+  if (!(sizeof(T) == 4 && K > 1 && K <= 32 && n % 16 == 0 && n <= 16 * kFlushOwnTiles)) return false;
+  const MoveLds L = move_lds<T>(N, 0, n, K, E);
+  return L.total - L.oQ >= flush_own_lds(n, K);
+}
+
+// T2 = -(T Binv) in FP64 for one wave's 16-row tile: T (16 x cp, row stride KC) read from sT,
+// T2 written over it once every lane has read its T row
+__device__ inline void flush_own_t2(int c, int cp, int KC, int KP, const double* sBinv, float* sT) {
+  const int lane = threadIdx.x & 63;
+  float t2v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int idx = lane + 64 * q;
+    double sum = 0;
+    if (idx < 16 * cp) {
+      const int i = idx / cp, dd = idx - i * cp;
+      if (dd < c)
+        for (int e = 0; e < c; ++e) sum += (double)sT[i * KC + e] * sBinv[e * KP + dd];
+    }
+    t2v[q] = (float)(-sum);
+  }
+  wave_lds_fence();
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int idx = lane + 64 * q;
+    if (idx < 16 * cp) {
+      const int i = idx / cp, dd = idx - i * cp;
+      sT[i * KC + dd] = t2v[q];
+    }
+  }
+  wave_lds_fence();
 }
 
 // updateInvMat (DelayedUpdate.h:176-193) of this walker's whole inverse by its own workgroup,
-// between chunks: Ainv += T2 V with T = Ainv U^T - E, T2 = -T Binv, in 16-row tiles over the
-// waves.  A tile is held in registers half its orbitals at a time, in the layout of
-// k_flush_mfma (lane (i, g): row i, orbitals 16ct + 4g .. +3): the first product sums both
-// halves, the second updates each half and stores it.  V rows are the chunk's stale inverse
-// rows (in LDS: sAc rows sAl[d]), U rows the new Slater rows (HBM, written at the chunk's end,
-// six orbital tiles' loads in flight), Binv in LDS (row stride KP).
+// between chunks, when at most kFlushOwnU moves of the chunk were accepted: Ainv += T2 V with
+// T = Ainv U^T - E, T2 = -T Binv, in 16-row tiles over the waves.  A wave loads its tile
+// (all orbitals, lane (i, g): row i, orbitals 16q + 4g .. +3, the layout of k_flush_mfma) into
+// registers once: it is the B operand of the first product and the accumulator of the
+// second, then stored.  U rows come from LDS (sU, row stride n + 4, staged at the chunk's
+// end), V rows are the chunk's stale inverse rows (sAc rows sAl[d]), Binv in LDS (row stride
+// KP).
+__device__ __attribute__((noinline)) void flush_own_tile(int n, int K, int KP, int c, const int* sRows, const int* sAl,
+                                      const double* sBinv, const float* sAc, const float* sU, float* Ainv,
+                                      float* scratch) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+  const int li = lane & 15, g = lane >> 4;
+  const int KC = (K + 15) & ~15, c4 = (c + 3) & ~3, nct = n >> 4, us = n + 4;
+  float* sT = scratch + (size_t)wid * 16 * KC;
+  const bool din = li < c;   // U row li of the first product's A operand is a delayed row
+#ifdef QMCX_PHASE_TIMING
+  unsigned long long ft_ = wall_clock64(), fa_[6] = {};
+#define FT_TICK(i)                                        \
+  if (blockIdx.x == 0 && tid == 0) {                      \
+    const unsigned long long now_ = wall_clock64();       \
+    fa_[i] += now_ - ft_;                                 \
+    ft_ = now_;                                           \
+  }
+#else
+#define FT_TICK(i)
+#endif
+  const float* ul = sU + (size_t)(din ? li : 0) * us + 4 * g;
+  for (int tile = wid; tile < nct; tile += nw) {
+    const int r0 = tile * 16;
+    float* ar = Ainv + (size_t)(r0 + li) * n + 4 * g;
+    f32x4_t a[kFlushOwnTiles];
+#pragma unroll
+    for (int q = 0; q < kFlushOwnTiles; ++q)
+      a[q] = *reinterpret_cast<const f32x4_t*>(ar + min(q, nct - 1) * 16);
+    FT_TICK(0)
+    // ---- GEMM 1: T^T = U rows^T (16 delayed rows) over all orbitals, two accumulators
+    f32x4_t acc0 = f32x4_t{0.f, 0.f, 0.f, 0.f}, acc1 = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < kFlushOwnTiles; ++q) {
+      if (q < nct) {
+        f32x4_t uv = *reinterpret_cast<const f32x4_t*>(ul + q * 16);
+        uv = din ? uv : f32x4_t{0.f, 0.f, 0.f, 0.f};
+        f32x4_t& acc = (q & 1) ? acc1 : acc0;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[0], a[q][0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[1], a[q][1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[2], a[q][2], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[3], a[q][3], acc, 0, 0, 0);
+      }
+    }
+    // lane (i, g) register r: T[i][4g + r], minus E (row r0+i is delayed row dd)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int dd = 4 * g + r;
+      float v = acc0[r] + acc1[r];
+      if (dd < c && sRows[dd] == r0 + li) v -= 1.f;
+      sT[li * KC + dd] = dd < c ? v : 0.f;
+    }
+    wave_lds_fence();
+    FT_TICK(2)
+    flush_own_t2(c, 16, KC, KP, sBinv, sT);
+    FT_TICK(3)
+    // ---- GEMM 2: rows^T += V^T T2^T, four delayed rows per step; V rows from LDS
+    for (int d0 = 0; d0 < c4; d0 += 4) {
+      const int dk = d0 + g;
+      const float bv = sT[li * KC + dk];
+      const float* vr = sAc + (size_t)sAl[dk < c ? dk : 0] * n + li;
+#pragma unroll
+      for (int q = 0; q < kFlushOwnTiles; ++q)
+        if (q < nct) {
+          const float v = dk < c ? vr[q * 16] : 0.f;
+          a[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(v, bv, a[q], 0, 0, 0);
+        }
+    }
+    FT_TICK(4)
+#pragma unroll
+    for (int q = 0; q < kFlushOwnTiles; ++q)
+      if (q < nct) *reinterpret_cast<f32x4_t*>(ar + q * 16) = a[q];
+    wave_lds_fence();   // the next tile's T overwrites this one's T2
+    FT_TICK(5)
+  }
+  __syncthreads();   // the next chunk reads these rows
+#ifdef QMCX_PHASE_TIMING
+  FT_TICK(1)
+  if (blockIdx.x == 0 && tid == 0)
+    for (int i = 0; i < 6; ++i) qmcx_phase_ticks[24 + i] += fa_[i];
+#endif
+#undef FT_TICK
+}
+
+// The same update when more than kFlushOwnU moves of the chunk were accepted: a tile is held
+// in registers half its orbitals at a time: the first product sums both halves, the second
+// updates each half and stores it.  U rows are the new Slater rows in HBM (written at the
+// chunk's end, six orbital tiles' loads in flight).
 constexpr int kFlushOwnHalf = kFlushOwnTiles / 2;
-__device__ inline void flush_own(int n, int K, int KP, int c, const int* sRows, const int* sAl, const double* sBinv,
+__device__ __attribute__((noinline)) void flush_own(int n, int K, int KP, int c, const int* sRows, const int* sAl, const double* sBinv,
                                  const float* sAc, const float* Amat, float* Ainv, float* scratch) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
   const int li = lane & 15, g = lane >> 4;
@@ -206,29 +332,7 @@ __device__ inline void flush_own(int n, int K, int KP, int c, const int* sRows, 
       }
     }
     wave_lds_fence();
-    // ---- T2 = -(T Binv) in FP64, written over T once every lane has read its T row
-    float t2v[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int idx = lane + 64 * q;
-      double sum = 0;
-      if (idx < 16 * cp) {
-        const int i = idx / cp, dd = idx - i * cp;
-        if (dd < c)
-          for (int e = 0; e < c; ++e) sum += (double)sT[i * KC + e] * sBinv[e * KP + dd];
-      }
-      t2v[q] = (float)(-sum);
-    }
-    wave_lds_fence();
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int idx = lane + 64 * q;
-      if (idx < 16 * cp) {
-        const int i = idx / cp, dd = idx - i * cp;
-        sT[i * KC + dd] = t2v[q];
-      }
-    }
-    wave_lds_fence();
+    flush_own_t2(c, cp, KC, KP, sBinv, sT);
     // ---- GEMM 2: rows^T += V^T T2^T per half of the orbitals; V rows from LDS
     for (int h0 = 0; h0 < nct; h0 += kFlushOwnHalf) {
       f32x4_t a[kFlushOwnHalf];
@@ -700,6 +804,9 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     const bool vec = n % VW == 0 && S.npad % VW == 0;
     const int nv = vec ? n / VW : n;   // vectors (or scalars) per row
     const int tot = ca * 5 * nv;
+    // the new Slater rows also go to LDS for the kernel's own flush (FP32, up to kFlushOwnU rows)
+    float* sU = (float*)(smem + L.oQ);
+    const bool stage_u = sizeof(T) == 4 && KD && a.inflush && ca <= a.flush_u;
     for (int b0 = 0; b0 < tot; b0 += 8 * nt) {
       VT v[8];
       T sv[8];
@@ -721,6 +828,12 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
                           : (o < 4 ? dpsiM + (size_t)rw * 3 * n + (size_t)(o - 1) * n : d2psiM + (size_t)rw * n);
           if (vec) ((VT*)dst)[j] = v[k];
           else dst[j] = sv[k];
+          if constexpr (sizeof(T) == 4) {
+            if (stage_u && o == 0) {
+              if (vec) *reinterpret_cast<VT*>(sU + (size_t)d * (n + 4) + j * VW) = v[k];
+              else sU[(size_t)d * (n + 4) + j] = sv[k];
+            }
+          }
         }
       }
     }
@@ -731,12 +844,17 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     for (int i = tid; i < c * K; i += nt) B.Binv[slot * K * K + i] = sBinv[(i / K) * KP + i % K];
     for (int i = tid; i < c; i += nt) B.rows[slot * K + i] = sRows[i];
   }
+  QMCX_TICK(22)
   if constexpr (KD && sizeof(T) == 4) {
     if (a.inflush && c > 0 && (ci + 1 < a.nchunks || a.inflush == 2)) {
       // the chunk's delayed updates reach this walker's whole inverse before the next chunk
       __syncthreads();
-      flush_own(n, K, KP, c, sRows, sAl, sBinv, (const float*)sAc, (const float*)Amat, (float*)Ainv,
-                (float*)(smem + L.oQ));
+      float* sU = (float*)(smem + L.oQ);
+      float* scratch = sU + (size_t)kFlushOwnU * (n + 4);
+      if (c <= a.flush_u)
+        flush_own_tile(n, K, KP, c, sRows, sAl, sBinv, (const float*)sAc, sU, (float*)Ainv, scratch);
+      else
+        flush_own(n, K, KP, c, sRows, sAl, sBinv, (const float*)sAc, (const float*)Amat, (float*)Ainv, scratch);
     }
   }
   QMCX_TICK(23)

@@ -137,10 +137,12 @@ def test_sweep_fp64_vs_oracle(cubic, cubic_dev, cubic_orc, delay):
     np.testing.assert_allclose(out["L"], ref["L"], rtol=1e-8, atol=1e-7)
 
 
-@pytest.mark.parametrize("delay,kind", [(1, 0), (8, 0), (32, 0), (8, 1), (16, 1)])
+@pytest.mark.parametrize("delay,kind", [(1, 0), (8, 0), (32, 0), (32, 2), (8, 1), (16, 1)])
 def test_sweep_fp32_vs_fp64_oracle(cubic, cubic_dev, cubic_orc, delay, kind):
     """kind 0: one move launch per determinant, each chunk's updates applied by the move kernel
-    on the matrix cores; kind 1: a move launch per chunk and a vector-FMA flush launch"""
+    on the matrix cores (new Slater rows staged in LDS when at most 16 moves were accepted);
+    kind 2: the same with those rows always read from HBM; kind 1: a move launch per chunk and
+    a vector-FMA flush launch"""
     _say("sweep FP32 delay", delay, "flush kind", kind)
     # Metropolis acceptance: the FP32 path must reproduce the FP64 accept/reject sequence;
     # log|Psi| within 1e-6 relative of the FP64 oracle
@@ -192,15 +194,15 @@ def _flip_margins(tr, rt, seed, sweep, walker0=0):
 
 
 def test_flush_mfma_vs_oracle(nio32):
-    """FP32 Woodbury flush on the matrix cores and on vector FMA against the FP64 oracle
-    (NiO-32, delay 32, Metropolis, the FP32 form's FP64 inverse recompute once per sweep).
-    A walker's accept sequence may only part from the oracle's at a near tie (oracle margin
-    |u - ratio^2| < 1e-3 ratio^2, the FP32 ratio error band); a walker without one ends the
-    sweep with log|Psi| within 1e-6 of the oracle."""
+    """FP32 Woodbury flush on the matrix cores (U rows from LDS or HBM) and on vector FMA against
+    the FP64 oracle (NiO-32, delay 32, Metropolis, the FP32 form's FP64 inverse recompute once
+    per sweep).  A walker's accept sequence may only part from the oracle's at a near tie
+    (oracle margin |u - ratio^2| < 1e-3 ratio^2, the FP32 ratio error band); a walker without
+    one ends the sweep with log|Psi| within 1e-6 of the oracle."""
     sysd, dev, orc = nio32
     R0 = systems.initial_positions(sysd, 2, seed=3)
     ref = orc.run_vmc(R0, 1, 5, 0.3, mode=qmcx.ACCEPT_METROPOLIS, delay=32, trace=True)
-    for kind in (0, 1):
+    for kind in (0, 1, 2):
         _say("flush kind", kind)
         wb = qmcx.WalkerBatch(dev, 2, delay_rank=32, precision=qmcx.FP32)
         wb.set_flush(kind)
