@@ -360,6 +360,7 @@ __device__ inline void bspline_weights(T t, T a[4], T da[4], T d2a[4]) {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef f32x4 f32x4_t;
 typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));   // FP64 MFMA accumulator (16x16x4)
 template <typename T>
 struct V16;
 template <>

@@ -472,14 +472,15 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
 // in column chunks streamed through HBM (Xsw = rows after the panel's swaps).
 // The column swaps that undo the pivoting are applied on the way out.
 constexpr int kGJChunk = 64;
-
-__host__ __device__ inline int gj_panel(int n) { return n <= 448 ? 32 : (n <= 896 ? 16 : 8); }
+// 16-row tiles per wave in a column chunk's update: three with 512 threads (a 384-row matrix in
+// one round), one with 1024 (the register budget of four waves per SIMD)
+template <int NT>
+constexpr int gj_row_tiles() { return NT == 512 ? 3 : 1; }
 
 struct GJLds {
   size_t oP, oXR, oXA, oF, oPr, oRed, oRi, oPiv, oSrc, oSlot, oDl, total;
 };
-__host__ __device__ inline GJLds gj_lds(int n) {
-  const int b = gj_panel(n);
+__host__ __device__ inline GJLds gj_lds_b(int n, int b) {
   GJLds L;
   size_t o = 0;
   auto take = [&](size_t bytes) { const size_t at = o; o += (bytes + 15) & ~(size_t)15; return at; };
@@ -497,6 +498,13 @@ __host__ __device__ inline GJLds gj_lds(int n) {
   L.total = o;
   return L;
 }
+// panel width: a thread per row (512 threads for b = 32, 1024 below), the panel within the LDS
+__host__ __device__ inline int gj_panel(int n) {
+  if (n <= 512 && gj_lds_b(n, 32).total <= 160 * 1024) return 32;
+  if (n <= 1024 && gj_lds_b(n, 16).total <= 160 * 1024) return 16;
+  return 8;
+}
+__host__ __device__ inline GJLds gj_lds(int n) { return gj_lds_b(n, gj_panel(n)); }
 
 template <typename TI, typename TO, int B, int NT>
 __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, double* ws, double* logdet,
@@ -585,16 +593,18 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
       // pivot: first row of largest |P[i][j]|, i >= k (idamax semantics): the wave's largest
       // magnitude, then its first lane (rows ascend with the lanes)
       const double av = (own && tid >= k) ? fabs(prj) : -1.0;
-      double best = wave_max(av);
+      const double best = wave_max(av);
       const unsigned long long hit = __ballot(av == best);
-      int bi = best >= 0.0 ? wid * 64 + (__ffsll((long long)hit) - 1) : k;
+      const int bi = best >= 0.0 ? wid * 64 + (__ffsll((long long)hit) - 1) : k;
       if (lane == 0) { rv[wid] = best; ri[wid] = bi; }
       __syncthreads();
-      best = rv[0];
-      bi = ri[0];
-      for (int q = 1; q < nw; ++q)
-        if (rv[q] > best || (rv[q] == best && ri[q] < bi)) { best = rv[q]; bi = ri[q]; }
-      const int p = bi;   // in [k, n)
+      // the waves' candidates side by side in the lanes of every wave: the largest magnitude,
+      // ties to the first wave (its rows are the lowest)
+      const double cv = lane < nw ? rv[lane] : -2.0;
+      const int cr = lane < nw ? ri[lane] : 0;
+      const double cbest = wave_max(cv);
+      const int qb = max(__ffsll((long long)__ballot(cv == cbest)) - 1, 0);   // 0 if a NaN hid the maximum
+      const int p = __builtin_amdgcn_readlane(cr, qb);   // in [k, n)
       if (tid == p)
 #pragma unroll
         for (int c = 0; c < B; ++c) prow[c] = pr[c];
@@ -675,10 +685,34 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
     }
     __syncthreads();
     const int nd = (int)rv[0];
-    // ---- rank-bb update of the other columns, kGJChunk at a time
+    // ---- rank-bb update of the other columns, kGJChunk at a time, on the FP64 matrix cores
+    // (v_mfma_f64_16x16x4f64).  A wave takes kGJRowTiles 16-row tiles (rows 16 rt .., rt =
+    // base + wid + q nw) across the chunk's four 16-column tiles.  Operand layouts, lane
+    // (j, g) = lane j + 16 g: A (16 x 4) = P[16 rt + j][t0 + g], B (4 x 16) = XR[t0 + g][16 ct + j],
+    // C/D register r = row 16 rt + g + 4 r, column 16 ct + j (rows interleave over the lane
+    // groups, unlike the FP32 16x16x4 layout).  The wave's X values of a chunk
+    // are loaded before the chunk's pivot and displaced rows are staged, so both rounds of loads
+    // are in flight together.
+    constexpr int kGJRowTiles = gj_row_tiles<NT>();
+    const int nrt = (n + 15) >> 4;
+    const int lj = lane & 15, lg = lane >> 4;
     for (int c0 = 0; c0 < n; c0 += kGJChunk) {
       const int cw = min(kGJChunk, n - c0);
       if (c0 >= k0 && c0 + cw <= k0 + bb) continue;   // the chunk is the panel
+      f64x4 acc[kGJRowTiles][4];
+      auto load_group = [&](int base) {
+#pragma unroll
+        for (int q = 0; q < kGJRowTiles; ++q)
+#pragma unroll
+          for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int i = 16 * (base + wid + q * nw) + lg + 4 * r, cc = 16 * ct + lj;
+              const bool in = i < n && cc < cw;
+              acc[q][ct][r] = X[in ? (size_t)i * n + c0 + cc : 0];   // unconditional load, selected below
+            }
+      };
+      load_group(0);
       for (int e0 = tid; e0 < (bb + nd) * cw; e0 += nt * 8) {
         double v[8];
 #pragma unroll
@@ -700,50 +734,53 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
         }
       }
       __syncthreads();
-      // TR rows x 4 columns per thread, all TR x 4 loads issued before the FMAs
-      constexpr int TR = NT == 512 ? 12 : 4;
-      const int rg = (n + TR - 1) / TR, cg = (cw + 3) / 4, ntile = rg * cg;
-      for (int g = tid; g < ntile; g += nt) {
-        const int i0 = (g / cg) * TR, cc0 = (g % cg) * 4;
-        double acc[TR][4];
+      for (int base = 0; base < nrt; base += kGJRowTiles * nw) {
+        if (base > 0) load_group(base);
+        // the base of each row: 0 for the pivot rows, the displaced row's old entries for rows a
+        // swap moved, else the row itself
 #pragma unroll
-        for (int r = 0; r < TR; ++r)
+        for (int q = 0; q < kGJRowTiles; ++q)
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            const int i = i0 + r, cc = cc0 + c, col = c0 + cc;
-            const bool in = i < n && cc < cw;
-            const int ic = in ? i : 0, colc = in ? col : 0;
-            acc[r][c] = X[(size_t)ic * n + colc];   // unconditional load, selected below
+          for (int r = 0; r < 4; ++r) {
+            const int i = 16 * (base + wid + q * nw) + lg + 4 * r;
+            const int sl = i < n ? slot[i] : -1;
+            const bool prow_i = i >= k0 && i < k0 + bb;
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+              const int cc = 16 * ct + lj;
+              double x = acc[q][ct][r];
+              if (sl >= 0) x = XA[sl * kGJChunk + (cc < cw ? cc : 0)];
+              acc[q][ct][r] = prow_i ? 0.0 : x;
+            }
+          }
+        for (int t0 = 0; t0 < bb; t0 += 4) {
+          const int t = t0 + lg;
+          double av[kGJRowTiles], bv[4];
+#pragma unroll
+          for (int q = 0; q < kGJRowTiles; ++q) {
+            const int i = 16 * (base + wid + q * nw) + lj;
+            av[q] = (i < n && t < bb) ? P[i * ps + t] : 0.0;
           }
 #pragma unroll
-        for (int r = 0; r < TR; ++r)
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            const int i = i0 + r, cc = cc0 + c, col = c0 + cc;
-            const bool in = i < n && cc < cw;
-            const bool zero = !in || (col >= k0 && col < k0 + bb) || (i >= k0 && i < k0 + bb);
-            const int sl = in ? slot[i] : -1;
-            const double xa = XA[(sl >= 0 ? sl : 0) * kGJChunk + (in ? cc : 0)];
-            acc[r][c] = zero ? 0.0 : (sl >= 0 ? xa : acc[r][c]);
+          for (int ct = 0; ct < 4; ++ct) {
+            const int cc = 16 * ct + lj;
+            bv[ct] = (t < bb && cc < cw) ? XR[t * kGJChunk + cc] : 0.0;
           }
-        for (int t = 0; t < bb; ++t) {
-          double pv[TR], xv[4];
 #pragma unroll
-          for (int r = 0; r < TR; ++r) pv[r] = (i0 + r < n) ? P[(i0 + r) * ps + t] : 0.0;
+          for (int q = 0; q < kGJRowTiles; ++q)
 #pragma unroll
-          for (int c = 0; c < 4; ++c) xv[c] = (cc0 + c < cw) ? XR[t * kGJChunk + cc0 + c] : 0.0;
-#pragma unroll
-          for (int r = 0; r < TR; ++r)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) acc[r][c] = fma(pv[r], xv[c], acc[r][c]);
+            for (int ct = 0; ct < 4; ++ct)
+              acc[q][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[ct], acc[q][ct], 0, 0, 0);
         }
 #pragma unroll
-        for (int r = 0; r < TR; ++r)
+        for (int q = 0; q < kGJRowTiles; ++q)
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            const int i = i0 + r, cc = cc0 + c, col = c0 + cc;
-            if (i < n && cc < cw && !(col >= k0 && col < k0 + bb)) X[(size_t)i * n + col] = acc[r][c];
-          }
+          for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int i = 16 * (base + wid + q * nw) + lg + 4 * r, cc = 16 * ct + lj, col = c0 + cc;
+              if (i < n && cc < cw && !(col >= k0 && col < k0 + bb)) X[(size_t)i * n + col] = acc[q][ct][r];
+            }
       }
       __syncthreads();
     }
