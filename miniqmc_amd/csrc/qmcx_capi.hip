@@ -115,6 +115,12 @@ Cell<T> make_cell(const double lat[9]) {
     C.corner[18 + d] = (T)(-(b1[d] + b2[d]));
     C.corner[21 + d] = (T)(-(b0[d] + b1[d] + b2[d]));
   }
+  // orthogonal reduced basis: every pair of vectors at right angles to 1e-12 relative
+  const double d01 = dot3(b0, b1), d02 = dot3(b0, b2), d12 = dot3(b1, b2);
+  const double n0 = std::sqrt(dot3(b0, b0)), n1 = std::sqrt(dot3(b1, b1)), n2 = std::sqrt(dot3(b2, b2));
+  const bool ortho = std::fabs(d01) <= 1e-12 * n0 * n1 && std::fabs(d02) <= 1e-12 * n0 * n2 &&
+                     std::fabs(d12) <= 1e-12 * n1 * n2;
+  C.ortho = ortho ? T(1) : T(0);
   return C;
 }
 

@@ -279,6 +279,7 @@ struct alignas(16) Cell {   // a whole number of 16-byte vectors (copied to LDS 
   T g[9];        // g[3*i+j] = g(i,j)
   T rb[9];       // rb[3*i+j] = rb[i][j]
   T corner[24];
+  T ortho;       // 1 when the reduced basis vectors are orthogonal (min_image_ortho applies)
 };
 
 template <typename T>
@@ -309,6 +310,33 @@ __device__ inline T min_image(const Cell<T>& C, T d0, T d1, T d2, T flip, T& ox,
   oy = flip * by;
   oz = flip * bz;
   return sqrt(best);
+}
+
+// min_image for an orthogonal reduced basis: over the eight corners the squared distance splits
+// into one term per axis, so the winner subtracts axis a exactly when the fractional coordinate
+// after the floor exceeds one half; its corner shift is the same table entry min_image adds.
+// The image is min_image's wherever the nearest image is unique beyond rounding, in particular
+// for every distance under half the shortest axis (the radial functors' range).
+template <typename T>
+__device__ inline T min_image_ortho(const Cell<T>& C, T d0, T d1, T d2, T flip, T& ox, T& oy, T& oz) {
+  d0 *= flip;
+  d1 *= flip;
+  d2 *= flip;
+  const T f0 = d0 * C.g[0] + d1 * C.g[3] + d2 * C.g[6];
+  const T f1 = d0 * C.g[1] + d1 * C.g[4] + d2 * C.g[7];
+  const T f2 = d0 * C.g[2] + d1 * C.g[5] + d2 * C.g[8];
+  const T s0 = -floor(f0), s1 = -floor(f1), s2 = -floor(f2);
+  const T ex = d0 + s0 * C.rb[0] + s1 * C.rb[3] + s2 * C.rb[6];
+  const T ey = d1 + s0 * C.rb[1] + s1 * C.rb[4] + s2 * C.rb[7];
+  const T ez = d2 + s0 * C.rb[2] + s1 * C.rb[5] + s2 * C.rb[8];
+  // corner order: 0, -b0, -b1, -b2, -(b0+b1), -(b0+b2), -(b1+b2), -(b0+b1+b2)
+  const int bits = (f0 + s0 > T(0.5) ? 1 : 0) | (f1 + s1 > T(0.5) ? 2 : 0) | (f2 + s2 > T(0.5) ? 4 : 0);
+  const int c = bits == 3 ? 4 : (bits == 4 ? 3 : bits);
+  const T x = ex + C.corner[3 * c], y = ey + C.corner[3 * c + 1], z = ez + C.corner[3 * c + 2];
+  ox = flip * x;
+  oy = flip * y;
+  oz = flip * z;
+  return sqrt(x * x + y * y + z * z);
 }
 
 // ------------------------------------------------------------------ spline location

@@ -487,7 +487,7 @@ __host__ __device__ inline GJLds gj_lds_b(int n, int b) {
   L.oP = take(sizeof(double) * (size_t)n * (b + 1));      // panel, row stride b+1
   L.oXR = take(sizeof(double) * (size_t)b * kGJChunk);   // pivot rows of a column chunk
   L.oXA = take(sizeof(double) * (size_t)b * kGJChunk);   // rows displaced by the swaps
-  L.oF = take(sizeof(double) * (size_t)n);               // pivot column of a step
+  L.oF = take(sizeof(double) * (size_t)(n > b ? n : b)); // the pivot position's old row (b entries)
   L.oPr = take(sizeof(double) * (size_t)b);              // scaled pivot row of a step
   L.oRed = take(sizeof(double) * 16);
   L.oRi = take(sizeof(int) * 16);

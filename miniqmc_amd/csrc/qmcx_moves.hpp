@@ -138,8 +138,10 @@ __device__ inline void j2_items(const Cell<T>& C, const Functor<T>* fn, const do
     const int j = tid + q * nt;
     if (j < N) {
       const Functor<T>& f = fn[((j < n) == up) ? 0 : 1];
-      const T r = min_image(C, (T)(sR[j] - x), (T)(sR[N + j] - y), (T)(sR[2 * N + j] - z), j < iel ? T(1) : T(-1),
-                            dx[q], dy[q], dz[q]);
+      const T ddx = (T)(sR[j] - x), ddy = (T)(sR[N + j] - y), ddz = (T)(sR[2 * N + j] - z);
+      const T flip = j < iel ? T(1) : T(-1);
+      const T r = C.ortho != T(0) ? min_image_ortho(C, ddx, ddy, ddz, flip, dx[q], dy[q], dz[q])
+                                  : min_image(C, ddx, ddy, ddz, flip, dx[q], dy[q], dz[q]);
       const bool in = j != iel && r < f.rcut;
       T a, b, c2;
       functor_vgl(f, in ? r : f.rcut * T(0.5), a, b, c2);   // an interior radius keeps the lookup in range

@@ -78,10 +78,12 @@ def test_invert_transpose_kat():
     assert np.abs(psiM[0] - b).max() < 1e-9
 
 
-def test_invert_transpose_batched_vs_oracle():
-    _say("LU batched n=96")
+@pytest.mark.parametrize("n", [4, 8, 24, 96])
+def test_invert_transpose_batched_vs_oracle(n):
+    """Regression (n < 32): the LDS row that a pivot swap parks must hold a whole panel row."""
+    _say("LU batched n =", n)
     rng = np.random.default_rng(7)
-    a = rng.uniform(-1, 1, size=(5, 96, 96))
+    a = rng.uniform(-1, 1, size=(5, n, n))
     psiM, ld, ph = qmcx.invert_transpose(a)
     for m in range(5):
         ref, rld, rph = po.invert_transpose(a[m])
@@ -148,6 +150,23 @@ def test_sweep_fp32_vs_fp64_oracle(cubic, cubic_dev, cubic_orc, delay, kind):
     # log|Psi| within 1e-6 relative of the FP64 oracle
     out = _compare_run(cubic, cubic_dev, cubic_orc, nw=3, nsweeps=2, delay=delay, prec=qmcx.FP32,
                        mode=qmcx.ACCEPT_METROPOLIS, tau=0.5, tol_ratio=2e-3, tol_log=1e-6, flush_kind=kind)
+    assert out["flips"] == 0
+
+
+@pytest.mark.parametrize("prec,delay", [(qmcx.FP64, 1), (qmcx.FP32, 4)])
+def test_sweep_skewed_cell_vs_oracle(prec, delay):
+    """A non-orthogonal cell (the 'tiny' system): the minimum image takes the general
+    eight-corner search in the J2 rows rather than the orthogonal-cell shortcut."""
+    _say("sweep skewed cell", prec, "delay", delay)
+    sysd = systems.make_system("tiny")
+    dev, orc = qmcx.System(sysd), po.OracleSystem(sysd)
+    if prec == qmcx.FP64:
+        out = _compare_run(sysd, dev, orc, nw=2, nsweeps=2, delay=delay, prec=prec, mode=qmcx.ACCEPT_METROPOLIS,
+                           tau=0.5, tol_ratio=1e-9, tol_log=1e-11)
+        np.testing.assert_allclose(out["G"], out["ref"]["G"], rtol=1e-8, atol=1e-8)
+    else:
+        out = _compare_run(sysd, dev, orc, nw=2, nsweeps=2, delay=delay, prec=prec, mode=qmcx.ACCEPT_METROPOLIS,
+                           tau=0.5, tol_ratio=2e-3, tol_log=1e-6)
     assert out["flips"] == 0
 
 
