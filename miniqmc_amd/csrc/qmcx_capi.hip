@@ -369,6 +369,8 @@ int launch_moves(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
 }
 
 // SPO rows at the proposals of electrons det*n + [e0, e0 + ne) of every walker
+constexpr int kPropWorkgroups = 1024;
+
 template <typename T>
 int launch_proposals(qmcx_batch_t b, hipStream_t st, int det, int e0, int ne, uint64_t seed, int sweep, double tau,
                      const double* displ) {
@@ -377,8 +379,11 @@ int launch_proposals(qmcx_batch_t b, hipStream_t st, int det, int e0, int ne, ui
   const int slices = proposal_slices<T>(s, b->slices);
   const int nt = std::max(64, (nq * slices + 63) / 64 * 64);
   const size_t lds = std::max(sizeof(T) * (size_t)slices * 5 * s->npad, sizeof(double) * 16 * 5);
-  hipLaunchKernelGGL(k_spo_proposals<T>, dim3(ne, b->W), dim3(nt), lds, st, s->params<T>(), b->params<T>(), det, e0,
-                     seed, sweep, std::sqrt(tau), b->walker0, displ, (T*)b->spo, b->prop, slices);
+  // about 1024 workgroups (four per CU), each looping over its electrons: the gather's burst of
+  // HBM traffic stays below what slows the latency-bound move kernel beside it
+  const int gx = std::max(1, std::min(ne, kPropWorkgroups / b->W));
+  hipLaunchKernelGGL(k_spo_proposals<T>, dim3(gx, b->W), dim3(nt), lds, st, s->params<T>(), b->params<T>(), det, e0,
+                     ne, seed, sweep, std::sqrt(tau), b->walker0, displ, (T*)b->spo, b->prop, slices);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -885,12 +885,15 @@ __global__ __launch_bounds__(1024) void k_spo_rows(SysP<T> S, BatchP<T> B, int s
 // move kernel's: sqrt(tau) times the walker's Philox normals of (sweep, electron), or the
 // caller's (displ[w][3]).
 template <typename T>
-__global__ __launch_bounds__(1024) void k_spo_proposals(SysP<T> S, BatchP<T> B, int det, int e0, uint64_t seed,
+__global__ __launch_bounds__(1024) void k_spo_proposals(SysP<T> S, BatchP<T> B, int det, int e0, int ne, uint64_t seed,
                                                          int sweep, double sqrt_tau, int walker0, const double* displ,
                                                          T* spo, double* prop, int slices) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* part = (T*)smem;
-  const int el = e0 + blockIdx.x, w = blockIdx.y, N = S.N, n = S.n;
+  const int w = blockIdx.y, N = S.N, n = S.n;
+  // a workgroup takes electrons e0 + blockIdx.x, + gridDim.x, ... (fewer workgroups than
+  // electrons bound the gather's share of HBM beside the move kernel)
+  for (int el = e0 + blockIdx.x; el < e0 + ne; el += gridDim.x) {
   const int iel = det * n + el;
   const double* R = B.R + (size_t)w * 3 * N;
   double d[3], u = 0.0;   // a caller's displacement comes with its own accept decision
@@ -947,6 +950,8 @@ __global__ __launch_bounds__(1024) void k_spo_proposals(SysP<T> S, BatchP<T> B, 
   } else if (threadIdx.x < 9) {
     const int k = threadIdx.x - 5;
     prop[(((size_t)w * 2 + det) * n + el) * kProp + kPropDispl + k] = k < 3 ? d[k] : u;
+  }
+  __syncthreads();   // the next electron's gather reuses the LDS
   }
 }
 
