@@ -472,10 +472,13 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
 // in column chunks streamed through HBM (Xsw = rows after the panel's swaps).
 // The column swaps that undo the pivoting are applied on the way out.
 constexpr int kGJChunk = 64;
-// 16-row tiles per wave in a column chunk's update: three with 512 threads (a 384-row matrix in
-// one round), one with 1024 (the register budget of four waves per SIMD)
+// Tiles per wave in a column chunk's update: three 16-row tiles by four 16-column tiles with 512
+// threads (a 384-row matrix in one round), three by one with 1024 (the register budget of four
+// waves per SIMD; the waves split into four column groups)
 template <int NT>
-constexpr int gj_row_tiles() { return NT == 512 ? 3 : 1; }
+constexpr int gj_row_tiles() { return 3; }
+template <int NT>
+constexpr int gj_col_tiles() { return NT == 512 ? 4 : 1; }
 
 struct GJLds {
   size_t oP, oXR, oXA, oF, oPr, oRed, oRi, oPiv, oSrc, oSlot, oDl, total;
@@ -583,73 +586,84 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
 #pragma unroll
     for (int c = 0; c < B; ++c) pr[c] = (own && c < bb) ? pr[c] : 0.0;
     double* swp = fcol;   // old pivot-position row, for the thread that takes it
-    // The step loop stays rolled, and every step works on pr[0]: each thread's panel entries
-    // rotate one place left after a step (all threads alike, the pivot-row copies included),
-    // so no register is picked by a runtime index; B steps rotate them back.
+    // Steps go in groups of four: within a group step u works on pr[u] (a register fixed at
+    // compile time), and after the group each thread's panel entries rotate four places left (all
+    // threads alike, the pivot-row copies included), so no register is picked by a runtime index
+    // and B places of rotation bring them back.
 #pragma unroll 1
-    for (int j = 0; j < bb; ++j) {
-      const int k = k0 + j;
-      const double prj = pr[0];
-      // pivot: first row of largest |P[i][j]|, i >= k (idamax semantics): the wave's largest
-      // magnitude, then its first lane (rows ascend with the lanes)
-      const double av = (own && tid >= k) ? fabs(prj) : -1.0;
-      const double best = wave_max(av);
-      const unsigned long long hit = __ballot(av == best);
-      const int bi = best >= 0.0 ? wid * 64 + (__ffsll((long long)hit) - 1) : k;
-      if (lane == 0) { rv[wid] = best; ri[wid] = bi; }
-      __syncthreads();
-      // the waves' candidates side by side in the lanes of every wave: the largest magnitude,
-      // ties to the first wave (its rows are the lowest)
-      const double cv = lane < nw ? rv[lane] : -2.0;
-      const int cr = lane < nw ? ri[lane] : 0;
-      const double cbest = wave_max(cv);
-      const int qb = max(__ffsll((long long)__ballot(cv == cbest)) - 1, 0);   // 0 if a NaN hid the maximum
-      const int p = __builtin_amdgcn_readlane(cr, qb);   // in [k, n)
-      if (tid == p)
+    for (int j0 = 0; j0 < bb; j0 += 4) {
 #pragma unroll
-        for (int c = 0; c < B; ++c) prow[c] = pr[c];
-      if (tid == k && p != k)
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 + u;
+        if (j < bb) {
+          const int k = k0 + j;
+          const double prj = pr[u];
+          // pivot: first row of largest |P[i][j]|, i >= k (idamax semantics): the wave's largest
+          // magnitude, then its first lane (rows ascend with the lanes)
+          const double av = (own && tid >= k) ? fabs(prj) : -1.0;
+          const double best = wave_max(av);
+          const unsigned long long hit = __ballot(av == best);
+          const int bi = best >= 0.0 ? wid * 64 + (__ffsll((long long)hit) - 1) : k;
+          if (lane == 0) { rv[wid] = best; ri[wid] = bi; }
+          __syncthreads();
+          // the waves' candidates side by side in the lanes of every wave: the largest
+          // magnitude, ties to the first wave (its rows are the lowest)
+          const double cv = lane < nw ? rv[lane] : -2.0;
+          const int cr = lane < nw ? ri[lane] : 0;
+          const double cbest = wave_max(cv);
+          const int qb = max(__ffsll((long long)__ballot(cv == cbest)) - 1, 0);   // 0 if a NaN hid the maximum
+          const int p = __builtin_amdgcn_readlane(cr, qb);   // in [k, n)
+          if (tid == p)
 #pragma unroll
-        for (int c = 0; c < B; ++c) swp[c] = pr[c];
-      __syncthreads();
-      const double d = prow[0];
-      const double dinv = 1.0 / d;
-      if (tid == 0) {
-        piv[k] = p;
-        if (p != k) sign = -sign;
-        if (d < 0) sign = -sign;
-        int e;
-        pmant = frexp(pmant * fabs(d), &e);
-        pexp += e;
-      }
-      if (own) {
-        if (tid == k) {
-          pr[0] = dinv;
+            for (int c = 0; c < B; ++c) prow[c] = pr[c];
+          if (tid == k && p != k)
 #pragma unroll
-          for (int c = 1; c < B; ++c) pr[c] = prow[c] * dinv;
-        } else {
-          double f = prj;
-          if (tid == p) {
-#pragma unroll
-            for (int c = 0; c < B; ++c) pr[c] = swp[c];
-            f = swp[0];
+            for (int c = 0; c < B; ++c) swp[c] = pr[c];
+          __syncthreads();
+          const double d = prow[u];
+          const double dinv = 1.0 / d;
+          if (tid == 0) {
+            piv[k] = p;
+            if (p != k) sign = -sign;
+            if (d < 0) sign = -sign;
+            int e;
+            pmant = frexp(pmant * fabs(d), &e);
+            pexp += e;
           }
-          pr[0] = -f * dinv;
+          if (own) {
+            if (tid == k) {
 #pragma unroll
-          for (int c = 1; c < B; ++c) pr[c] = pr[c] - f * (prow[c] * dinv);
+              for (int c = 0; c < B; ++c) pr[c] = c == u ? dinv : prow[c] * dinv;
+            } else {
+              double f = prj;
+              if (tid == p) {
+#pragma unroll
+                for (int c = 0; c < B; ++c) pr[c] = swp[c];
+                f = swp[u];
+              }
+#pragma unroll
+              for (int c = 0; c < B; ++c) pr[c] = c == u ? -f * dinv : pr[c] - f * (prow[c] * dinv);
+            }
+          }
         }
       }
-      const double r0 = pr[0];
+      double r4[4];
 #pragma unroll
-      for (int c = 0; c + 1 < B; ++c) pr[c] = pr[c + 1];
-      pr[B - 1] = r0;
+      for (int c = 0; c < 4; ++c) r4[c] = pr[c];
+#pragma unroll
+      for (int c = 0; c + 4 < B; ++c) pr[c] = pr[c + 4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) pr[B - 4 + c] = r4[c];
     }
-    // a short last panel: rotate the remaining B - bb places
-    for (int j = bb; j < B; ++j) {
-      const double r0 = pr[0];
+    // a short last panel: rotate the remaining B - 4 ceil(bb / 4) places
+    for (int j = (bb + 3) / 4 * 4; j < B; j += 4) {
+      double r4[4];
 #pragma unroll
-      for (int c = 0; c + 1 < B; ++c) pr[c] = pr[c + 1];
-      pr[B - 1] = r0;
+      for (int c = 0; c < 4; ++c) r4[c] = pr[c];
+#pragma unroll
+      for (int c = 0; c + 4 < B; ++c) pr[c] = pr[c + 4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) pr[B - 4 + c] = r4[c];
     }
     if (own)
 #pragma unroll
@@ -687,27 +701,29 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
     const int nd = (int)rv[0];
     // ---- rank-bb update of the other columns, kGJChunk at a time, on the FP64 matrix cores
     // (v_mfma_f64_16x16x4f64).  A wave takes kGJRowTiles 16-row tiles (rows 16 rt .., rt =
-    // base + wid + q nw) across the chunk's four 16-column tiles.  Operand layouts, lane
+    // base + wr + q nrg) across kGJColTiles of the chunk's four 16-column tiles.  Operand layouts, lane
     // (j, g) = lane j + 16 g: A (16 x 4) = P[16 rt + j][t0 + g], B (4 x 16) = XR[t0 + g][16 ct + j],
     // C/D register r = row 16 rt + g + 4 r, column 16 ct + j (rows interleave over the lane
     // groups, unlike the FP32 16x16x4 layout).  The wave's X values of a chunk
     // are loaded before the chunk's pivot and displaced rows are staged, so both rounds of loads
     // are in flight together.
-    constexpr int kGJRowTiles = gj_row_tiles<NT>();
+    constexpr int kGJRowTiles = gj_row_tiles<NT>(), kGJColTiles = gj_col_tiles<NT>();
     const int nrt = (n + 15) >> 4;
     const int lj = lane & 15, lg = lane >> 4;
+    constexpr int nrg = NT / 64 / (4 / kGJColTiles);          // waves per column group
+    const int wr = wid % nrg, ct0 = (wid / nrg) * kGJColTiles;   // row slot, first column tile
     for (int c0 = 0; c0 < n; c0 += kGJChunk) {
       const int cw = min(kGJChunk, n - c0);
       if (c0 >= k0 && c0 + cw <= k0 + bb) continue;   // the chunk is the panel
-      f64x4 acc[kGJRowTiles][4];
+      f64x4 acc[kGJRowTiles][kGJColTiles];
       auto load_group = [&](int base) {
 #pragma unroll
         for (int q = 0; q < kGJRowTiles; ++q)
 #pragma unroll
-          for (int ct = 0; ct < 4; ++ct)
+          for (int ct = 0; ct < kGJColTiles; ++ct)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              const int i = 16 * (base + wid + q * nw) + lg + 4 * r, cc = 16 * ct + lj;
+              const int i = 16 * (base + wr + q * nrg) + lg + 4 * r, cc = 16 * (ct0 + ct) + lj;
               const bool in = i < n && cc < cw;
               acc[q][ct][r] = X[in ? (size_t)i * n + c0 + cc : 0];   // unconditional load, selected below
             }
@@ -734,7 +750,7 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
         }
       }
       __syncthreads();
-      for (int base = 0; base < nrt; base += kGJRowTiles * nw) {
+      for (int base = 0; base < nrt; base += kGJRowTiles * nrg) {
         if (base > 0) load_group(base);
         // the base of each row: 0 for the pivot rows, the displaced row's old entries for rows a
         // swap moved, else the row itself
@@ -742,12 +758,12 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
         for (int q = 0; q < kGJRowTiles; ++q)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = 16 * (base + wid + q * nw) + lg + 4 * r;
+            const int i = 16 * (base + wr + q * nrg) + lg + 4 * r;
             const int sl = i < n ? slot[i] : -1;
             const bool prow_i = i >= k0 && i < k0 + bb;
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-              const int cc = 16 * ct + lj;
+            for (int ct = 0; ct < kGJColTiles; ++ct) {
+              const int cc = 16 * (ct0 + ct) + lj;
               double x = acc[q][ct][r];
               if (sl >= 0) x = XA[sl * kGJChunk + (cc < cw ? cc : 0)];
               acc[q][ct][r] = prow_i ? 0.0 : x;
@@ -755,30 +771,30 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
           }
         for (int t0 = 0; t0 < bb; t0 += 4) {
           const int t = t0 + lg;
-          double av[kGJRowTiles], bv[4];
+          double av[kGJRowTiles], bv[kGJColTiles];
 #pragma unroll
           for (int q = 0; q < kGJRowTiles; ++q) {
-            const int i = 16 * (base + wid + q * nw) + lj;
+            const int i = 16 * (base + wr + q * nrg) + lj;
             av[q] = (i < n && t < bb) ? P[i * ps + t] : 0.0;
           }
 #pragma unroll
-          for (int ct = 0; ct < 4; ++ct) {
-            const int cc = 16 * ct + lj;
+          for (int ct = 0; ct < kGJColTiles; ++ct) {
+            const int cc = 16 * (ct0 + ct) + lj;
             bv[ct] = (t < bb && cc < cw) ? XR[t * kGJChunk + cc] : 0.0;
           }
 #pragma unroll
           for (int q = 0; q < kGJRowTiles; ++q)
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct)
+            for (int ct = 0; ct < kGJColTiles; ++ct)
               acc[q][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[ct], acc[q][ct], 0, 0, 0);
         }
 #pragma unroll
         for (int q = 0; q < kGJRowTiles; ++q)
 #pragma unroll
-          for (int ct = 0; ct < 4; ++ct)
+          for (int ct = 0; ct < kGJColTiles; ++ct)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              const int i = 16 * (base + wid + q * nw) + lg + 4 * r, cc = 16 * ct + lj, col = c0 + cc;
+              const int i = 16 * (base + wr + q * nrg) + lg + 4 * r, cc = 16 * (ct0 + ct) + lj, col = c0 + cc;
               if (i < n && cc < cw && !(col >= k0 && col < k0 + bb)) X[(size_t)i * n + col] = acc[q][ct][r];
             }
       }
