@@ -88,16 +88,17 @@ def measured_traffic(sysd, k, moves_per_launch):
     return round(rec["bytes_per_launch"]), os.path.relpath(path, ROOT)
 
 
-def cpu_baseline(sysd, R0, seed, tau, nthreads, target_s=12.0):
+def cpu_baseline(sysd, R0, seed, tau, nthreads, target_s=15.0):
     """The all-FP64 CPU oracle on the host's cores over a bounded sample of the same workload:
-    two walkers per thread, one calibration sweep, then as many sweeps (same chains) as fill
-    about target_s seconds."""
+    two walkers per thread, a warm-up sweep and two timed calibration sweeps, then as many sweeps
+    (same chains) as fill about target_s seconds."""
     from oracle import pyoracle as po
     orc = po.OracleSystem(sysd, spline_prec=8)
     nwalk = min(2 * nthreads, len(R0))
     first = orc.run_vmc(R0[:nwalk], 1, seed, tau, mode=qmcx.ACCEPT_METROPOLIS, delay=1, nthreads=nthreads)
-    nsw = int(max(1, min(200, round(target_s / max(first["seconds"], 1e-3)))))
-    res = orc.run_vmc(first["R"], nsw, seed + 1, tau, mode=qmcx.ACCEPT_METROPOLIS, delay=1, nthreads=nthreads)
+    cal = orc.run_vmc(first["R"], 2, seed + 1, tau, mode=qmcx.ACCEPT_METROPOLIS, delay=1, nthreads=nthreads)
+    nsw = int(max(1, min(2000, round(target_s / max(cal["seconds"] / 2, 1e-3)))))
+    res = orc.run_vmc(cal["R"], nsw, seed + 2, tau, mode=qmcx.ACCEPT_METROPOLIS, delay=1, nthreads=nthreads)
     moves = nwalk * sysd.nel * nsw
     return dict(value=moves / res["seconds"], unit="walker-moves/s", cores=nthreads, kind="port",
                 sample=f"{nwalk} walkers x {nsw} sweeps of {sysd.name} ({moves} moves, {res['seconds']:.1f} s), "
