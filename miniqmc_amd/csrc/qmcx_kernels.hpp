@@ -472,13 +472,13 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
 // in column chunks streamed through HBM (Xsw = rows after the panel's swaps).
 // The column swaps that undo the pivoting are applied on the way out.
 constexpr int kGJChunk = 64;
-// Tiles per wave in a column chunk's update: three 16-row tiles by four 16-column tiles with 512
-// threads (a 384-row matrix in one round), three by one with 1024 (the register budget of four
-// waves per SIMD; the waves split into four column groups)
+// Tiles per wave in a column chunk's update: three 16-row tiles by one 16-column tile (the waves
+// split into four column groups).  With 512 threads the kernel then needs about 150 registers, so
+// its CU keeps room for the proposals kernel's waves beside it.
 template <int NT>
 constexpr int gj_row_tiles() { return 3; }
 template <int NT>
-constexpr int gj_col_tiles() { return NT == 512 ? 4 : 1; }
+constexpr int gj_col_tiles() { return 1; }
 
 struct GJLds {
   size_t oP, oXR, oXA, oF, oPr, oRed, oRi, oPiv, oSrc, oSlot, oDl, total;
