@@ -191,6 +191,26 @@ __device__ inline void flush_own_t2(int c, int cp, int KC, int KP, const double*
   wave_lds_fence();
 }
 
+// The same for at most 16 delayed rows on the FP64 matrix cores: T2 (16 x 16) = -T Binv in four
+// v_mfma_f64_16x16x4f64 steps.  Lane (j, g) = lane j + 16 g supplies A = T[j][4s + g] and
+// B = Binv[4s + g][j]; its result register r is T2[g + 4r][j] (the FP64 16x16x4 layout).  Entries
+// past c are zero (T's are stored as zeros, Binv's are masked here).
+__device__ inline void flush_own_t2_mfma(int c, int KC, int KP, const double* sBinv, float* sT) {
+  const int lane = threadIdx.x & 63, lj = lane & 15, lg = lane >> 4;
+  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s4 = 0; s4 < 16; s4 += 4) {
+    const int e = s4 + lg;
+    const double a = (double)sT[lj * KC + e];
+    const double bv = (e < c && lj < c) ? sBinv[e * KP + lj] : 0.0;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+  }
+  wave_lds_fence();   // every lane has read its T entries
+#pragma unroll
+  for (int r = 0; r < 4; ++r) sT[(lg + 4 * r) * KC + lj] = (float)(-acc[r]);
+  wave_lds_fence();
+}
+
 // updateInvMat (DelayedUpdate.h:176-193) of this walker's whole inverse by its own workgroup,
 // between chunks, when at most kFlushOwnU moves of the chunk were accepted: Ainv += T2 V with
 // T = Ainv U^T - E, T2 = -T Binv, in 16-row tiles over the waves.  A wave loads its tile
@@ -251,7 +271,7 @@ __device__ __attribute__((noinline)) void flush_own_tile(int n, int K, int KP, i
     }
     wave_lds_fence();
     FT_TICK(2)
-    flush_own_t2(c, 16, KC, KP, sBinv, sT);
+    flush_own_t2_mfma(c, KC, KP, sBinv, sT);
     FT_TICK(3)
     // ---- GEMM 2: rows^T += V^T T2^T, four delayed rows per step; V rows from LDS
     for (int d0 = 0; d0 < c4; d0 += 4) {
