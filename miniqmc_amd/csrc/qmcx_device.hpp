@@ -155,10 +155,13 @@ __device__ inline void wave_sums_rows(double (&v)[NV]) {
 // Global -> LDS copy of NR regions by the whole workgroup: every thread issues up to eight
 // loads before its first LDS store, so the regions arrive in one or two memory round trips.
 // 16-byte vectors when every address and size allows, 4-byte words otherwise.
+// A copy of `bytes` contiguous bytes into LDS; with row > 0 the source is rows of `row` bytes,
+// each landing `pitch` bytes after the previous one in LDS.
 struct LdsCopy {
   void* dst;
   const void* src;
   unsigned bytes;
+  unsigned row = 0, pitch = 0;
 };
 template <typename V, int NR>
 __device__ inline void stage_lds_v(const LdsCopy (&c)[NR]) {
@@ -182,10 +185,15 @@ __device__ inline void stage_lds_v(const LdsCopy (&c)[NR]) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const unsigned i = b + (unsigned)(k * nt + tid);
-      V* d = (V*)c[0].dst + i;
+      auto at = [&](const LdsCopy& cr, unsigned l) -> V* {
+        if (cr.row == 0) return (V*)cr.dst + l;
+        const unsigned rv = cr.row / (unsigned)sizeof(V);
+        return (V*)cr.dst + (l / rv) * (cr.pitch / (unsigned)sizeof(V)) + l % rv;
+      };
+      V* d = at(c[0], i);
 #pragma unroll
       for (int r = 1; r < NR; ++r)
-        if (i >= pre[r]) d = (V*)c[r].dst + (i - pre[r]);
+        if (i >= pre[r]) d = at(c[r], i - pre[r]);
       if (i < tot) *d = v[k];
     }
   }
@@ -195,7 +203,7 @@ __device__ inline void stage_lds(const LdsCopy (&c)[NR]) {
   bool vec = true;
 #pragma unroll
   for (int r = 0; r < NR; ++r)
-    vec = vec && (((uintptr_t)c[r].dst | (uintptr_t)c[r].src | (uintptr_t)c[r].bytes) & 15) == 0;
+    vec = vec && (((uintptr_t)c[r].dst | (uintptr_t)c[r].src | (uintptr_t)c[r].bytes | c[r].row | c[r].pitch) & 15) == 0;
   if (vec) stage_lds_v<int4, NR>(c);
   else stage_lds_v<int, NR>(c);
 }

@@ -19,6 +19,10 @@ namespace qmcx {
 constexpr int kRed = 12;   // reduction slots: gnow(3) ratio(1) gnew(3) J2(5)
 constexpr int kMoveThreads = 512;
 constexpr size_t kMoveLdsMax = 160 * 1024;   // LDS of a workgroup
+// LDS row stride (elements) of the chunk's stale inverse rows: 64 bytes of padding, so four rows
+// read at one column offset by the four 16-lane groups of a wave fall in different banks
+template <typename T>
+__host__ __device__ inline int ac_stride(int n) { return n + 64 / (int)sizeof(T); }
 
 // LDS of the in-kernel flush between chunks (FP32): the new Slater rows U of up to kFlushOwnU
 // accepted moves (row stride n + 4) and per wave a 16 x KC tile of T (in place T2)
@@ -59,7 +63,7 @@ __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E) {
   L.oProp = lds_take(o, sizeof(double) * kProp * E);        // the chunk's proposal records
   L.oJ1o = lds_take(o, sizeof(T) * 4 * E);                  // J1 value and gradient of its electrons
   L.oJ2 = lds_take(o, sizeof(T) * 5 * N);
-  L.oAc = lds_take(o, cache ? sizeof(T) * (size_t)E * n : 0);
+  L.oAc = lds_take(o, cache ? sizeof(T) * (size_t)E * ac_stride<T>(n) : 0);
   L.oQ = lds_take(o, cache ? sizeof(double) * (size_t)K * E : 0);
   // J2 pair terms at the current proposal; with the region before it, the scratch of the
   // in-kernel flush between chunks
@@ -220,7 +224,7 @@ __device__ inline void flush_own_t2_mfma(int c, int KC, int KP, const double* sB
 // end), V rows are the chunk's stale inverse rows (sAc rows sAl[d]), Binv in LDS (row stride
 // KP).
 __device__ __attribute__((noinline)) void flush_own_tile(int n, int K, int KP, int c, const int* sRows, const int* sAl,
-                                      const double* sBinv, const float* sAc, const float* sU, float* Ainv,
+                                      const double* sBinv, const float* sAc, int sa, const float* sU, float* Ainv,
                                       float* scratch) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
   const int li = lane & 15, g = lane >> 4;
@@ -277,7 +281,7 @@ __device__ __attribute__((noinline)) void flush_own_tile(int n, int K, int KP, i
     for (int d0 = 0; d0 < c4; d0 += 4) {
       const int dk = d0 + g;
       const float bv = sT[li * KC + dk];
-      const float* vr = sAc + (size_t)sAl[dk < c ? dk : 0] * n + li;
+      const float* vr = sAc + (size_t)sAl[dk < c ? dk : 0] * sa + li;
 #pragma unroll
       for (int q = 0; q < kFlushOwnTiles; ++q)
         if (q < nct) {
@@ -307,7 +311,7 @@ __device__ __attribute__((noinline)) void flush_own_tile(int n, int K, int KP, i
 // chunk's end, six orbital tiles' loads in flight).
 constexpr int kFlushOwnHalf = kFlushOwnTiles / 2;
 __device__ __attribute__((noinline)) void flush_own(int n, int K, int KP, int c, const int* sRows, const int* sAl, const double* sBinv,
-                                 const float* sAc, const float* Amat, float* Ainv, float* scratch) {
+                                 const float* sAc, int sa, const float* Amat, float* Ainv, float* scratch) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
   const int li = lane & 15, g = lane >> 4;
   const int KC = (K + 15) & ~15, cp = (c + 15) & ~15, nct = n >> 4;
@@ -364,7 +368,7 @@ __device__ __attribute__((noinline)) void flush_own(int n, int K, int KP, int c,
       for (int d0 = 0; d0 < cp; d0 += 4) {
         const int dk = d0 + g;
         const float bv = sT[li * KC + dk];
-        const float* vr = sAc + (size_t)sAl[dk < c ? dk : 0] * n + li;
+        const float* vr = sAc + (size_t)sAl[dk < c ? dk : 0] * sa + li;
 #pragma unroll
         for (int q = 0; q < kFlushOwnHalf; ++q) {
           const float v = dk < c ? vr[min(h0 + q, nct - 1) * 16] : 0.f;
@@ -407,6 +411,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   T* sJt = (T*)(smem + L.oJt);
   T* sJ2 = (T*)(smem + L.oJ2);
   T* sAc = (T*)(smem + L.oAc);
+  const int sa = ac_stride<T>(n);   // its row stride
   double* sQ = (double*)(smem + L.oQ);
 #ifdef QMCX_PHASE_TIMING
   unsigned long long tick_ = wall_clock64();
@@ -464,7 +469,8 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   // round of loads)
   {
     const unsigned je = (unsigned)(sizeof(T) * E);
-    const LdsCopy cp[6] = {{sAc, Ainv + (size_t)r0 * n, KD ? (unsigned)(sizeof(T) * E * n) : 0u},
+    const LdsCopy cp[6] = {{sAc, Ainv + (size_t)r0 * n, KD ? (unsigned)(sizeof(T) * E * n) : 0u,
+                            (unsigned)(sizeof(T) * n), (unsigned)(sizeof(T) * sa)},
                            {sProp, prop, (unsigned)(sizeof(double) * kProp * E)},
                            {sJ1o, J1 + cb, je},
                            {sJ1o + E, J1 + N + cb, je},
@@ -542,21 +548,21 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     if (tid < n) {
       double v;
       if (KD) {
-        v = (double)sAc[(size_t)e * n + tid];
+        v = (double)sAc[(size_t)e * sa + tid];
         const double* y = sBinv + c * KP;
         // four delayed rows' LDS reads in flight at a time; subtraction order unchanged
         int d = 0;
         for (; d + 4 <= c; d += 4) {
           const int e0 = sRows[d] - r0, e1 = sRows[d + 1] - r0, e2 = sRows[d + 2] - r0, e3 = sRows[d + 3] - r0;
           const double y0 = y[d], y1 = y[d + 1], y2 = y[d + 2], y3 = y[d + 3];
-          const double a0 = (double)sAc[(size_t)e0 * n + tid], a1 = (double)sAc[(size_t)e1 * n + tid];
-          const double a2 = (double)sAc[(size_t)e2 * n + tid], a3 = (double)sAc[(size_t)e3 * n + tid];
+          const double a0 = (double)sAc[(size_t)e0 * sa + tid], a1 = (double)sAc[(size_t)e1 * sa + tid];
+          const double a2 = (double)sAc[(size_t)e2 * sa + tid], a3 = (double)sAc[(size_t)e3 * sa + tid];
           v -= a0 * y0;
           v -= a1 * y1;
           v -= a2 * y2;
           v -= a3 * y3;
         }
-        for (; d < c; ++d) v -= (double)sAc[(size_t)(sRows[d] - r0) * n + tid] * y[d];
+        for (; d < c; ++d) v -= (double)sAc[(size_t)(sRows[d] - r0) * sa + tid] * y[d];
       } else {
         v = (double)inv_sm;
       }
@@ -574,9 +580,9 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     }
     for (int j = tid + nt; j < n; j += nt) {
       // only when n exceeds the workgroup (not the case for the shipped systems)
-      double v = KD ? (double)sAc[(size_t)e * n + j] : (double)Ainv[(size_t)row * n + j];
+      double v = KD ? (double)sAc[(size_t)e * sa + j] : (double)Ainv[(size_t)row * n + j];
       if (KD)
-        for (int d = 0; d < c; ++d) v -= (double)sAc[(size_t)(sRows[d] - r0) * n + j] * sBinv[c * KP + d];
+        for (int d = 0; d < c; ++d) v -= (double)sAc[(size_t)(sRows[d] - r0) * sa + j] * sBinv[c * KP + d];
       sInv[j] = (T)v;
       const T* dp = dpsiM + (size_t)row * 3 * n;
       gd0 += v * (double)dp[j];
@@ -715,7 +721,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
         const int nd = c + 1 + (E - 1 - e);
         auto dvec = [&](int t2) -> const T* {
           const int ee = t2 < c ? sRows[t2] - r0 : e + (t2 - c);
-          return sAc + (size_t)ee * n;
+          return sAc + (size_t)ee * sa;
         };
         for (int t0 = wid * 4; t0 < nd; t0 += nw * 4) {
           const T* v0 = dvec(t0);
@@ -818,7 +824,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   if (KD && !a.inflush)
     for (int i = tid; i < ca * n; i += nt) {
       const int d = i / n, j = i - d * n;
-      Vb[(size_t)d * n + j] = sAc[(size_t)sAl[d] * n + j];
+      Vb[(size_t)d * n + j] = sAc[(size_t)sAl[d] * sa + j];
     }
   {
     using VT = typename V16<T>::type;
@@ -874,9 +880,9 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       float* sU = (float*)(smem + L.oQ);
       float* scratch = sU + (size_t)kFlushOwnU * (n + 4);
       if (c <= a.flush_u)
-        flush_own_tile(n, K, KP, c, sRows, sAl, sBinv, (const float*)sAc, sU, (float*)Ainv, scratch);
+        flush_own_tile(n, K, KP, c, sRows, sAl, sBinv, (const float*)sAc, sa, sU, (float*)Ainv, scratch);
       else
-        flush_own(n, K, KP, c, sRows, sAl, sBinv, (const float*)sAc, (const float*)Amat, (float*)Ainv, scratch);
+        flush_own(n, K, KP, c, sRows, sAl, sBinv, (const float*)sAc, sa, (const float*)Amat, (float*)Ainv, scratch);
     }
   }
   QMCX_TICK(23)
