@@ -88,21 +88,102 @@ def measured_traffic(sysd, k, moves_per_launch):
     return round(rec["bytes_per_launch"]), os.path.relpath(path, ROOT)
 
 
-def cpu_baseline(sysd, R0, seed, tau, nthreads, target_s=15.0):
+def cpu_baseline(sysd, R0, seed, tau, nthreads, mode, target_s=15.0):
     """The all-FP64 CPU oracle on the host's cores over a bounded sample of the same workload:
     two walkers per thread, a warm-up sweep and two timed calibration sweeps, then as many sweeps
     (same chains) as fill about target_s seconds."""
     from oracle import pyoracle as po
     orc = po.OracleSystem(sysd, spline_prec=8)
     nwalk = min(2 * nthreads, len(R0))
-    first = orc.run_vmc(R0[:nwalk], 1, seed, tau, mode=qmcx.ACCEPT_METROPOLIS, delay=1, nthreads=nthreads)
-    cal = orc.run_vmc(first["R"], 2, seed + 1, tau, mode=qmcx.ACCEPT_METROPOLIS, delay=1, nthreads=nthreads)
+    first = orc.run_vmc(R0[:nwalk], 1, seed, tau, mode=mode, delay=1, nthreads=nthreads)
+    cal = orc.run_vmc(first["R"], 2, seed + 1, tau, mode=mode, delay=1, nthreads=nthreads)
     nsw = int(max(1, min(2000, round(target_s / max(cal["seconds"] / 2, 1e-3)))))
-    res = orc.run_vmc(cal["R"], nsw, seed + 2, tau, mode=qmcx.ACCEPT_METROPOLIS, delay=1, nthreads=nthreads)
+    res = orc.run_vmc(cal["R"], nsw, seed + 2, tau, mode=mode, delay=1, nthreads=nthreads)
     moves = nwalk * sysd.nel * nsw
     return dict(value=moves / res["seconds"], unit="walker-moves/s", cores=nthreads, kind="port",
                 sample=f"{nwalk} walkers x {nsw} sweeps of {sysd.name} ({moves} moves, {res['seconds']:.1f} s), "
-                       f"all-FP64 oracle, delay 1, OpenMP over walkers (2 per thread)")
+                       f"all-FP64 oracle, delay 1, OpenMP over walkers (2 per thread), "
+                       f"{'u < 0.5' if mode == qmcx.ACCEPT_REFERENCE else 'Metropolis'} acceptance")
+
+
+def measure_leg(dev, sysd, R0, k, tile, mode, steps, warmup, fp64_steps, err_steps, recompute, seed, tau, one_walker):
+    """One acceptance rule: the FP32-vs-FP64 error leg, the timed FP32 sweeps, the FP64 delay-1 sweeps."""
+    W, N = len(R0), sysd.nel
+    # ---- error leg: FP32 batched vs FP64 delay-1 from the same start, same random streams.
+    # A walker's sequences agree until its first accept flip (a move whose u falls between
+    # the two forms' ratio^2); log|Psi| is compared on the walkers without a flip.
+    b32 = qmcx.WalkerBatch(dev, W, delay_rank=k, precision=qmcx.FP32, tile=tile)
+    b64 = qmcx.WalkerBatch(dev, W, delay_rank=1, precision=qmcx.FP64)
+    for b in (b32, b64):
+        b.set_positions(R0)
+        b.evaluate_log()
+        b.set_trace(True)
+    flipped = np.zeros(W, dtype=bool)
+    flips = 0
+    ratio_err = []
+    lerr_sweeps = []
+    for _ in range(err_steps):
+        h32 = b32.run(1, seed, tau, mode, recompute_every=recompute)
+        h64 = b64.run(1, seed, tau, mode)
+        t32, t64 = b32.trace(), b64.trace()
+        f = t32[:, :, 2] != t64[:, :, 2]
+        flips += int((f.any(axis=1) & ~flipped).sum())   # first divergence of a walker's chain
+        clean_moves = ~(flipped[:, None] | (np.cumsum(f, axis=1) > 0))
+        rel = np.abs(t32[:, :, 0] - t64[:, :, 0]) / np.maximum(np.abs(t64[:, :, 0]), 1.0)
+        ratio_err.append(rel[clean_moves])
+        flipped |= f.any(axis=1)
+        lerr_sweeps.append(np.abs(h32[0] - h64[0]) / np.abs(h64[0]))
+    acc_rate = float(t64[:, :, 2].mean())
+    clean = ~flipped
+    lerr = np.array(lerr_sweeps)[:, clean] if clean.any() else np.full((1, 1), np.nan)
+    ratio_err = np.concatenate(ratio_err)
+    for b in (b32, b64):
+        b.set_trace(False)
+    log(f"error leg ({err_steps} sweeps): walkers with identical accept sequences {int(clean.sum())}/{W}, "
+        f"flips {flips}; max rel |dlog| on those {np.nanmax(lerr):.2e}; ratio err max {ratio_err.max():.2e}")
+
+    # ---- headline: batched FP32 (FP64 LU recompute every `recompute` sweeps inside the timed region)
+    b32.run(warmup, seed, tau, mode, recompute_every=recompute)
+    b32.set_kernel_timing(True)
+    a0 = b32.accepts().sum()
+    t0 = time.perf_counter()
+    b32.run(steps, seed, tau, mode, recompute_every=recompute)
+    wall = time.perf_counter() - t0
+    a1 = b32.accepts().sum()
+    kt = b32.kernel_timing()
+    b32.set_kernel_timing(False)
+    mv_ms, mv_n = kt["move"]
+    moves = W * N * steps
+    value = moves / wall
+    acc_timed = float(a1 - a0) / moves
+    stages = {st: {"ms_per_step": ms / steps, "launches": n, "ms_per_launch": ms / max(n, 1)}
+              for st, (ms, n) in kt.items()}
+    log(f"FP32 k={k} ({'u < 0.5' if mode == qmcx.ACCEPT_REFERENCE else 'Metropolis'}): {value:.4e} moves/s, {1e3 * wall / steps:.2f} ms/sweep; stages " +
+        ", ".join(f"{st} {v['ms_per_step']:.2f} ms/sweep" for st, v in stages.items()))
+
+    # ---- FP64 delay-1 GPU path: batched (same walkers, continuing its own chain) and one walker alone
+    b64.run(1, seed, tau, mode)
+    t0 = time.perf_counter()
+    b64.run(fp64_steps, seed, tau, mode)
+    wall64 = time.perf_counter() - t0
+    value64 = W * N * fp64_steps / wall64
+    value64_1 = None
+    if one_walker:
+        b1 = qmcx.WalkerBatch(dev, 1, delay_rank=1, precision=qmcx.FP64)
+        b1.set_positions(R0[:1])
+        b1.evaluate_log()
+        b1.run(1, seed, tau, mode)
+        t0 = time.perf_counter()
+        b1.run(1, seed, tau, mode)
+        value64_1 = N / (time.perf_counter() - t0)
+        b1.close()
+    log(f"FP64 k=1: batched {value64:.4e} moves/s ({1e3 * wall64 / fp64_steps:.2f} ms/sweep)" +
+        (f", one walker {value64_1:.4e} moves/s" if one_walker else ""))
+    b32.close()
+    b64.close()
+    return dict(flips=flips, clean=clean, lerr=lerr, ratio_err=ratio_err, wall=wall, kt=kt, moves=moves, value=value,
+                acc_timed=acc_timed, stages=stages, wall64=wall64, value64=value64, value64_1=value64_1)
+
 
 
 def main():
@@ -114,6 +195,11 @@ def main():
     ap.add_argument("--walkers", type=int, default=128)
     ap.add_argument("--delay", type=int, default=0, help="delay rank (0: autotune delay rank and spline tile, miniqmc_amd.autotune)")
     ap.add_argument("--tau", type=float, default=0.3)
+    ap.add_argument("--accept", choices=("reference", "metropolis"), default="reference",
+                    help="acceptance rule: the reference driver's u < r with its default r = 0.5 "
+                         "(miniqmc.cpp:202,445), or Metropolis u < |ratio|^2")
+    ap.add_argument("--other-steps", type=int, default=20,
+                    help="timed sweeps of the other acceptance rule, reported beside the headline (0: skip)")
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--fp64-steps", type=int, default=4)
     ap.add_argument("--recompute", type=int, default=1,
@@ -124,6 +210,7 @@ def main():
                     help="only copy MIB MiB device-to-device five times (a kernel of known HBM traffic "
                          "to scale the FETCH_SIZE/WRITE_SIZE counters by) and exit")
     args = ap.parse_args()
+    mode = qmcx.ACCEPT_REFERENCE if args.accept == "reference" else qmcx.ACCEPT_METROPOLIS
     if args.calibrate_copy:
         import torch
         nbytes = args.calibrate_copy << 20
@@ -154,77 +241,27 @@ def main():
         k, tile = args.delay, 0
         tune = None
     else:
-        k, tile, tune = autotune.tune(dev, W, R0, seed=args.seed, tau=args.tau)
+        k, tile, tune = autotune.tune(dev, W, R0, seed=args.seed, tau=args.tau, mode=mode)
         log(f"autotune: delay {k}, tile {tile}: {tune}")
 
-    # ---- error leg: FP32 batched vs FP64 delay-1 from the same start, same random streams.
-    # A walker's sequences agree until its first accept flip (a move whose u falls between
-    # the two forms' ratio^2); log|Psi| is compared on the walkers without a flip.
-    b32 = qmcx.WalkerBatch(dev, W, delay_rank=k, precision=qmcx.FP32, tile=tile)
-    b64 = qmcx.WalkerBatch(dev, W, delay_rank=1, precision=qmcx.FP64)
-    for b in (b32, b64):
-        b.set_positions(R0)
-        b.evaluate_log()
-        b.set_trace(True)
-    flipped = np.zeros(W, dtype=bool)
-    flips = 0
-    ratio_err = []
-    lerr_sweeps = []
-    for _ in range(args.err_steps):
-        h32 = b32.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS, recompute_every=args.recompute)
-        h64 = b64.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
-        t32, t64 = b32.trace(), b64.trace()
-        f = t32[:, :, 2] != t64[:, :, 2]
-        flips += int((f.any(axis=1) & ~flipped).sum())   # first divergence of a walker's chain
-        clean_moves = ~(flipped[:, None] | (np.cumsum(f, axis=1) > 0))
-        rel = np.abs(t32[:, :, 0] - t64[:, :, 0]) / np.maximum(np.abs(t64[:, :, 0]), 1.0)
-        ratio_err.append(rel[clean_moves])
-        flipped |= f.any(axis=1)
-        lerr_sweeps.append(np.abs(h32[0] - h64[0]) / np.abs(h64[0]))
-    acc_rate = float(t64[:, :, 2].mean())
-    clean = ~flipped
-    lerr = np.array(lerr_sweeps)[:, clean] if clean.any() else np.full((1, 1), np.nan)
-    ratio_err = np.concatenate(ratio_err)
-    for b in (b32, b64):
-        b.set_trace(False)
-    log(f"error leg ({args.err_steps} sweeps): walkers with identical accept sequences {int(clean.sum())}/{W}, "
-        f"flips {flips}; max rel |dlog| on those {np.nanmax(lerr):.2e}; ratio err max {ratio_err.max():.2e}")
-
-    # ---- headline: batched FP32 (FP64 LU recompute every `recompute` sweeps inside the timed region)
-    b32.run(args.warmup, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS, recompute_every=args.recompute)
-    b32.set_kernel_timing(True)
-    a0 = b32.accepts().sum()
-    t0 = time.perf_counter()
-    b32.run(args.steps, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS, recompute_every=args.recompute)
-    wall = time.perf_counter() - t0
-    a1 = b32.accepts().sum()
-    kt = b32.kernel_timing()
-    b32.set_kernel_timing(False)
+    L = measure_leg(dev, sysd, R0, k, tile, mode, args.steps, args.warmup, args.fp64_steps, args.err_steps,
+                    args.recompute, args.seed, args.tau, one_walker=True)
+    flips, clean, lerr, ratio_err = L["flips"], L["clean"], L["lerr"], L["ratio_err"]
+    wall, kt, moves, value, acc_timed = L["wall"], L["kt"], L["moves"], L["value"], L["acc_timed"]
+    stages, wall64, value64, value64_1 = L["stages"], L["wall64"], L["value64"], L["value64_1"]
     mv_ms, mv_n = kt["move"]
-    moves = W * N * args.steps
-    value = moves / wall
-    acc_timed = float(a1 - a0) / moves
-    stages = {st: {"ms_per_step": ms / args.steps, "launches": n, "ms_per_launch": ms / max(n, 1)}
-              for st, (ms, n) in kt.items()}
-    log(f"FP32 k={k}: {value:.4e} moves/s, {1e3 * wall / args.steps:.2f} ms/sweep; stages " +
-        ", ".join(f"{st} {v['ms_per_step']:.2f} ms/sweep" for st, v in stages.items()))
-
-    # ---- FP64 delay-1 GPU path: batched (same walkers, continuing its own chain) and one walker alone
-    b64.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
-    t0 = time.perf_counter()
-    b64.run(args.fp64_steps, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
-    wall64 = time.perf_counter() - t0
-    value64 = W * N * args.fp64_steps / wall64
-    b1 = qmcx.WalkerBatch(dev, 1, delay_rank=1, precision=qmcx.FP64)
-    b1.set_positions(R0[:1])
-    b1.evaluate_log()
-    b1.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
-    t0 = time.perf_counter()
-    b1.run(1, args.seed, args.tau, qmcx.ACCEPT_METROPOLIS)
-    value64_1 = N / (time.perf_counter() - t0)
-    b1.close()
-    log(f"FP64 k=1: batched {value64:.4e} moves/s ({1e3 * wall64 / args.fp64_steps:.2f} ms/sweep), "
-        f"one walker {value64_1:.4e} moves/s")
+    # the other acceptance rule, shorter, reported beside the headline
+    other_rule = "metropolis" if args.accept == "reference" else "reference"
+    other = None
+    if args.other_steps > 0:
+        O = measure_leg(dev, sysd, R0, k, tile,
+                        qmcx.ACCEPT_METROPOLIS if other_rule == "metropolis" else qmcx.ACCEPT_REFERENCE,
+                        args.other_steps, 1, 2, args.err_steps, args.recompute, args.seed, args.tau, one_walker=False)
+        other = {"accept_rule": other_rule, "value": O["value"], "ms_per_step": 1e3 * O["wall"] / args.other_steps,
+                 "steps": args.other_steps, "acceptance": O["acc_timed"],
+                 "fp64_delay1_value": O["value64"], "speedup_vs_fp64_delay1": O["value"] / O["value64"],
+                 "accept_flips": O["flips"], "walkers_identical_sequence": int(O["clean"].sum()),
+                 "log_psi_err_max_rel": float(np.nanmax(O["lerr"]))}
 
     # ---- rooflines.  The dominant kernel (most of the sweep's time) is the fused move kernel:
     # one workgroup per walker walks its electrons in order, a latency-bound chain that moves
@@ -266,7 +303,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(sysd, R0, args.seed, args.tau, nthreads)
+        cpu = cpu_baseline(sysd, R0, args.seed, args.tau, nthreads, mode)
         log(f"CPU oracle: {cpu['value']:.4e} moves/s on {nthreads} threads")
 
     out = {
@@ -284,9 +321,11 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{sysd.name} VMC sweep: {N} e-, {sysd.norb} orbitals, {sysd.ng[0]}^3 FP32 spline "
-                        f"({sysd.table_bytes() / 1e6:.0f} MB), {W} walkers, tau {args.tau}, Metropolis; "
+                        f"({sysd.table_bytes() / 1e6:.0f} MB), {W} walkers, tau {args.tau}, "
+                        f"{'u < 0.5 acceptance (the reference driver)' if args.accept == 'reference' else 'Metropolis'}; "
                         f"batched FP32, delay rank {k}",
             "walkers": W, "electrons": N, "orbitals": sysd.norb, "grid": list(sysd.ng), "delay_rank": k,
+            "accept_rule": args.accept,
             "spline_tile": tile, "moves_per_step": W * N,
         },
         "roofline": roofline,
@@ -298,6 +337,7 @@ def main():
                         "form": "batched, 1 workgroup per walker, in-kernel Sherman-Morrison",
                         "one_walker_value": value64_1},
         "speedup_vs_fp64_delay1": value / value64,
+        "other_accept_rule": other,
         "log_psi_err": {"max_rel": float(np.nanmax(lerr)), "mean_rel": float(np.nanmean(lerr)),
                         "walkers_compared": int(clean.sum()), "sweeps": args.err_steps,
                         "vs": "FP64 delay-1 GPU path (parity-pinned to the FP64 oracle), same start and streams; "

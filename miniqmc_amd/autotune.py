@@ -17,7 +17,7 @@ TILES = (0, 1, 4)
 
 
 def tune(dev: qmcx.System, nwalkers: int, R0, seed: int = 11, tau: float = 0.3, sweeps: int = 2,
-         delays=DELAYS, tiles=TILES):
+         delays=DELAYS, tiles=TILES, mode: int = qmcx.ACCEPT_REFERENCE):
     results = {}
     best = None
     for k in delays:
@@ -28,8 +28,8 @@ def tune(dev: qmcx.System, nwalkers: int, R0, seed: int = 11, tau: float = 0.3, 
                 continue   # does not fit (LDS) or tile invalid for this system
             b.set_positions(R0)
             b.evaluate_log()
-            b.run(1, seed + 7919, tau, qmcx.ACCEPT_METROPOLIS, recompute_every=1)
-            b.run(sweeps, seed + 7919, tau, qmcx.ACCEPT_METROPOLIS, recompute_every=1)
+            b.run(1, seed + 7919, tau, mode, recompute_every=1)
+            b.run(sweeps, seed + 7919, tau, mode, recompute_every=1)
             ms = b.last_sweep_ms() / sweeps
             b.close()
             results[f"k{k}_s{tile}"] = round(ms, 3)
