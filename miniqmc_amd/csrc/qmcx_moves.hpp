@@ -195,42 +195,50 @@ __device__ inline void flush_own_t2(int c, int cp, int KC, int KP, const double*
   wave_lds_fence();
 }
 
-// The same for at most 16 delayed rows on the FP64 matrix cores: T2 (16 x 16) = -T Binv in four
-// v_mfma_f64_16x16x4f64 steps.  Lane (j, g) = lane j + 16 g supplies A = T[j][4s + g] and
-// B = Binv[4s + g][j]; its result register r is T2[g + 4r][j] (the FP64 16x16x4 layout).  Entries
-// past c are zero (T's are stored as zeros, Binv's are masked here).
-__device__ inline void flush_own_t2_mfma(int c, int KC, int KP, const double* sBinv, float* sT) {
+// The same for at most 32 delayed rows on the FP64 matrix cores: T2 (16 x cp) = -T Binv in
+// v_mfma_f64_16x16x4f64 steps, one 16-column block per 16 delayed rows.  Lane (j, g) = lane
+// j + 16 g supplies A = T[j][4s + g] and B = Binv[4s + g][16b + j]; its result register r is
+// T2[g + 4r][16b + j] (the FP64 16x16x4 layout).  Entries past c are zero (T's are stored as
+// zeros, Binv's are masked here).
+__device__ inline void flush_own_t2_mfma(int c, int cp, int KC, int KP, const double* sBinv, float* sT) {
   const int lane = threadIdx.x & 63, lj = lane & 15, lg = lane >> 4;
-  f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int s4 = 0; s4 < 16; s4 += 4) {
+  f64x4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+  for (int s4 = 0; s4 < cp; s4 += 4) {
     const int e = s4 + lg;
     const double a = (double)sT[lj * KC + e];
-    const double bv = (e < c && lj < c) ? sBinv[e * KP + lj] : 0.0;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+      if (16 * b < cp) {
+        const int dd = 16 * b + lj;
+        const double bv = (e < c && dd < c) ? sBinv[e * KP + dd] : 0.0;
+        acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[b], 0, 0, 0);
+      }
   }
   wave_lds_fence();   // every lane has read its T entries
 #pragma unroll
-  for (int r = 0; r < 4; ++r) sT[(lg + 4 * r) * KC + lj] = (float)(-acc[r]);
+  for (int b = 0; b < 2; ++b)
+    if (16 * b < cp)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sT[(lg + 4 * r) * KC + 16 * b + lj] = (float)(-acc[b][r]);
   wave_lds_fence();
 }
 
 // updateInvMat (DelayedUpdate.h:176-193) of this walker's whole inverse by its own workgroup,
-// between chunks, when at most kFlushOwnU moves of the chunk were accepted: Ainv += T2 V with
-// T = Ainv U^T - E, T2 = -T Binv, in 16-row tiles over the waves.  A wave loads its tile
-// (all orbitals, lane (i, g): row i, orbitals 16q + 4g .. +3, the layout of k_flush_mfma) into
-// registers once: it is the B operand of the first product and the accumulator of the
-// second, then stored.  U rows come from LDS (sU, row stride n + 4, staged at the chunk's
-// end), V rows are the chunk's stale inverse rows (sAc rows sAl[d]), Binv in LDS (row stride
-// KP).
+// between chunks, when the new Slater rows U of the chunk's accepted moves are all in LDS:
+// Ainv += T2 V with T = Ainv U^T - E, T2 = -T Binv, in 16-row tiles over the waves.  A wave loads
+// its tile (all orbitals, lane (i, g): row i, orbitals 16q + 4g .. +3, the layout of
+// k_flush_mfma) into registers once: it is the B operand of the first product (one pass per 16
+// delayed rows) and the accumulator of the second, then stored.  U rows 0-15 come from sU (row
+// stride n + 4), rows 16 .. c-1 from the slots of the chunk's rejected moves in sAc (sRej[d -
+// 16], row stride sa), both staged at the chunk's end; V rows are the chunk's stale inverse rows
+// (sAc rows sAl[d]), Binv in LDS (row stride KP).  c <= 16, or c <= 32 with K >= 32.
 __device__ __attribute__((noinline)) void flush_own_tile(int n, int K, int KP, int c, const int* sRows, const int* sAl,
-                                      const double* sBinv, const float* sAc, int sa, const float* sU, float* Ainv,
-                                      float* scratch) {
+                                      const double* sBinv, const float* sAc, int sa, const float* sU,
+                                      const int* sRej, float* Ainv, float* scratch) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
   const int li = lane & 15, g = lane >> 4;
-  const int KC = (K + 15) & ~15, c4 = (c + 3) & ~3, nct = n >> 4, us = n + 4;
+  const int KC = (K + 15) & ~15, c4 = (c + 3) & ~3, cp = (c + 15) & ~15, nct = n >> 4, us = n + 4;
   float* sT = scratch + (size_t)wid * 16 * KC;
-  const bool din = li < c;   // U row li of the first product's A operand is a delayed row
 #ifdef QMCX_PHASE_TIMING
   unsigned long long ft_ = wall_clock64(), fa_[6] = {};
 #define FT_TICK(i)                                        \
@@ -242,7 +250,6 @@ __device__ __attribute__((noinline)) void flush_own_tile(int n, int K, int KP, i
 #else
 #define FT_TICK(i)
 #endif
-  const float* ul = sU + (size_t)(din ? li : 0) * us + 4 * g;
   for (int tile = wid; tile < nct; tile += nw) {
     const int r0 = tile * 16;
     float* ar = Ainv + (size_t)(r0 + li) * n + 4 * g;
@@ -251,31 +258,36 @@ __device__ __attribute__((noinline)) void flush_own_tile(int n, int K, int KP, i
     for (int q = 0; q < kFlushOwnTiles; ++q)
       a[q] = *reinterpret_cast<const f32x4_t*>(ar + min(q, nct - 1) * 16);
     FT_TICK(0)
-    // ---- GEMM 1: T^T = U rows^T (16 delayed rows) over all orbitals, two accumulators
-    f32x4_t acc0 = f32x4_t{0.f, 0.f, 0.f, 0.f}, acc1 = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    // ---- GEMM 1: T^T = U rows^T over all orbitals, 16 delayed rows per pass, two accumulators
+    for (int dt = 0; dt < cp; dt += 16) {
+      const int d = dt + li;
+      const bool din = d < c;   // U row d of the A operand is a delayed row
+      const float* ul = (dt == 0 ? sU + (size_t)(din ? li : 0) * us : sAc + (size_t)(din ? sRej[li] : 0) * sa) + 4 * g;
+      f32x4_t acc0 = f32x4_t{0.f, 0.f, 0.f, 0.f}, acc1 = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int q = 0; q < kFlushOwnTiles; ++q) {
-      if (q < nct) {
-        f32x4_t uv = *reinterpret_cast<const f32x4_t*>(ul + q * 16);
-        uv = din ? uv : f32x4_t{0.f, 0.f, 0.f, 0.f};
-        f32x4_t& acc = (q & 1) ? acc1 : acc0;
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[0], a[q][0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[1], a[q][1], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[2], a[q][2], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[3], a[q][3], acc, 0, 0, 0);
+      for (int q = 0; q < kFlushOwnTiles; ++q) {
+        if (q < nct) {
+          f32x4_t uv = *reinterpret_cast<const f32x4_t*>(ul + q * 16);
+          uv = din ? uv : f32x4_t{0.f, 0.f, 0.f, 0.f};
+          f32x4_t& acc = (q & 1) ? acc1 : acc0;
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[0], a[q][0], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[1], a[q][1], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[2], a[q][2], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(uv[3], a[q][3], acc, 0, 0, 0);
+        }
       }
-    }
-    // lane (i, g) register r: T[i][4g + r], minus E (row r0+i is delayed row dd)
+      // lane (i, g) register r: T[i][dt + 4g + r], minus E (row r0+i is delayed row dd)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int dd = 4 * g + r;
-      float v = acc0[r] + acc1[r];
-      if (dd < c && sRows[dd] == r0 + li) v -= 1.f;
-      sT[li * KC + dd] = dd < c ? v : 0.f;
+      for (int r = 0; r < 4; ++r) {
+        const int dd = dt + 4 * g + r;
+        float v = acc0[r] + acc1[r];
+        if (dd < c && sRows[dd] == r0 + li) v -= 1.f;
+        sT[li * KC + dd] = dd < c ? v : 0.f;
+      }
     }
     wave_lds_fence();
     FT_TICK(2)
-    flush_own_t2_mfma(c, KC, KP, sBinv, sT);
+    flush_own_t2_mfma(c, cp, KC, KP, sBinv, sT);
     FT_TICK(3)
     // ---- GEMM 2: rows^T += V^T T2^T, four delayed rows per step; V rows from LDS
     for (int d0 = 0; d0 < c4; d0 += 4) {
@@ -832,9 +844,24 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     const bool vec = n % VW == 0 && S.npad % VW == 0;
     const int nv = vec ? n / VW : n;   // vectors (or scalars) per row
     const int tot = ca * 5 * nv;
-    // the new Slater rows also go to LDS for the kernel's own flush (FP32, up to kFlushOwnU rows)
+    // The new Slater rows also go to LDS for the kernel's own flush (FP32): rows 0-15 to sU,
+    // rows 16 on (k >= 32, while they fit) to the chunk's rejected moves' slots of sAc, whose
+    // stale rows the flush does not read; their list goes to sRej (sInv's LDS, free here).
     float* sU = (float*)(smem + L.oQ);
-    const bool stage_u = sizeof(T) == 4 && KD && a.inflush && ca <= a.flush_u;
+    int* sRej = (int*)(smem + L.oInv);
+    const bool stage_u = sizeof(T) == 4 && KD && a.inflush && a.flush_u > 0 &&
+                         (ca <= kFlushOwnU || (K >= 32 && ca <= 32 && 2 * ca <= kFlushOwnU + E));
+    if (stage_u && ca > kFlushOwnU) {
+      if (wid == 0) {
+        unsigned long long m = 0;
+        for (int d = 0; d < ca; ++d) m |= 1ull << sAl[d];
+        const bool rej = lane < E && !((m >> lane) & 1ull);
+        const unsigned long long rb = __ballot(rej);
+        const int rank = __popcll(rb & ((1ull << lane) - 1ull));
+        if (rej) sRej[rank] = lane;
+      }
+      __syncthreads();
+    }
     for (int b0 = 0; b0 < tot; b0 += 8 * nt) {
       VT v[8];
       T sv[8];
@@ -858,8 +885,9 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
           else dst[j] = sv[k];
           if constexpr (sizeof(T) == 4) {
             if (stage_u && o == 0) {
-              if (vec) *reinterpret_cast<VT*>(sU + (size_t)d * (n + 4) + j * VW) = v[k];
-              else sU[(size_t)d * (n + 4) + j] = sv[k];
+              float* ur = d < kFlushOwnU ? sU + (size_t)d * (n + 4) : (float*)sAc + (size_t)sRej[d - kFlushOwnU] * sa;
+              if (vec) *reinterpret_cast<VT*>(ur + j * VW) = v[k];
+              else ur[j] = sv[k];
             }
           }
         }
@@ -878,9 +906,11 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       // the chunk's delayed updates reach this walker's whole inverse before the next chunk
       __syncthreads();
       float* sU = (float*)(smem + L.oQ);
+      const int* sRej = (const int*)(smem + L.oInv);
       float* scratch = sU + (size_t)kFlushOwnU * (n + 4);
-      if (c <= a.flush_u)
-        flush_own_tile(n, K, KP, c, sRows, sAl, sBinv, (const float*)sAc, sa, sU, (float*)Ainv, scratch);
+      // the same test as the staging of the new Slater rows at the chunk's end
+      if (a.flush_u > 0 && (c <= kFlushOwnU || (K >= 32 && c <= 32 && 2 * c <= kFlushOwnU + E)))
+        flush_own_tile(n, K, KP, c, sRows, sAl, sBinv, (const float*)sAc, sa, sU, sRej, (float*)Ainv, scratch);
       else
         flush_own(n, K, KP, c, sRows, sAl, sBinv, (const float*)sAc, sa, (const float*)Amat, (float*)Ainv, scratch);
     }

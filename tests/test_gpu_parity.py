@@ -153,6 +153,31 @@ def test_sweep_fp32_vs_fp64_oracle(cubic, cubic_dev, cubic_orc, delay, kind):
     assert out["flips"] == 0
 
 
+@pytest.mark.parametrize("delay,kind", [(32, 0), (32, 2), (16, 0)])
+def test_sweep_fp32_reference_rule_vs_oracle(nio32, delay, kind):
+    """The reference driver's acceptance (u < 0.5, about half the moves accepted): chunks of 32
+    with more than 16 accepts stage their new Slater rows beyond the 16th in the rejected moves'
+    LDS slots (kind 0) or read them from HBM (kind 2).  Accept sequences match by construction;
+    log|Psi| within 1e-6 of the FP64 oracle after every sweep (NiO-32, 2 walkers, 2 sweeps)."""
+    sysd, dev, orc = nio32
+    _say("sweep FP32 reference rule delay", delay, "flush kind", kind)
+    R0 = systems.initial_positions(sysd, 2, seed=5)
+    ref = orc.run_vmc(R0, 2, 5, 0.3, mode=qmcx.ACCEPT_REFERENCE, delay=delay, trace=True)
+    wb = qmcx.WalkerBatch(dev, 2, delay_rank=delay, precision=qmcx.FP32)
+    wb.set_flush(kind)
+    wb.set_positions(R0)
+    wb.evaluate_log()
+    wb.set_trace(True)
+    for s in range(2):
+        hist = wb.run(1, 5, 0.3, qmcx.ACCEPT_REFERENCE, recompute_every=1)
+        tr = wb.trace()
+        assert np.array_equal(tr[:, :, 2], ref["trace"][:, s][:, :, 2])
+        lerr = np.abs(hist[0] - ref["log"][:, s]) / np.abs(ref["log"][:, s])
+        _say(f"  sweep {s}: max |dlog|/|log| {lerr.max():.3e}")
+        assert lerr.max() < 1e-6
+    assert np.array_equal(wb.accepts(), ref["naccept"])
+
+
 @pytest.mark.parametrize("prec,delay", [(qmcx.FP64, 1), (qmcx.FP32, 4)])
 def test_sweep_skewed_cell_vs_oracle(prec, delay):
     """A non-orthogonal cell (the 'tiny' system): the minimum image takes the general
