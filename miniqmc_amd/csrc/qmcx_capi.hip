@@ -551,11 +551,33 @@ int refresh_det(qmcx_batch_t b, int det, bool into_fresh) {
   return 0;
 }
 
+// Side-stream holds (microseconds) after the event that releases the side work together with a
+// move launch: the FP64 refresh, and the proposals of each determinant (the second's are released
+// before evaluateGL, so they wait longer).
+#ifndef QMCX_HOLD_REFRESH_US
+#define QMCX_HOLD_REFRESH_US 100
+#endif
+#ifndef QMCX_HOLD_PROP0_US
+#define QMCX_HOLD_PROP0_US 100
+#endif
+#ifndef QMCX_HOLD_PROP1_US
+#define QMCX_HOLD_PROP1_US 500
+#endif
+int hold_stream(hipStream_t st, int us) {
+  if (us <= 0) return 0;
+  hipLaunchKernelGGL(k_hold, dim3(1), dim3(64), 0, st, (unsigned long long)us * 100ull);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 // SPO rows of determinant det's proposals in sweep `sweep` on the proposals stream, after
-// the event `after` (positions final), announced by ev_pre[det]
+// the event `after` (positions final), announced by ev_pre[det]; `hold` when a move launch is
+// released by the same event
 template <typename T>
-int issue_proposals(qmcx_batch_t b, int det, hipEvent_t after, uint64_t seed, int sweep, double tau) {
+int issue_proposals(qmcx_batch_t b, int det, hipEvent_t after, uint64_t seed, int sweep, double tau,
+                    bool hold = false) {
   if (after) HIPCHK(hipStreamWaitEvent(b->pre, after, 0));
+  if (hold && hold_stream(b->pre, det == 0 ? QMCX_HOLD_PROP0_US : QMCX_HOLD_PROP1_US)) return QMCX_ERR_HIP;
   if (b->prof_record(4, b->pre)) return QMCX_ERR_HIP;
   if (int rc = launch_proposals<T>(b, b->pre, det, 0, b->sys->n, seed, sweep, tau, nullptr)) return rc;
   if (b->prof_close(b->pre)) return QMCX_ERR_HIP;
@@ -578,9 +600,10 @@ int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompu
   if (int rc = moves_det<T>(b, 0, mode, !recompute)) return rc;
   HIPCHK(hipEventRecord(b->ev_done[0], b->stream));
   if (!last)
-    if (int rc = issue_proposals<T>(b, 0, b->ev_done[0], seed, sweep + 1, tau)) return rc;
+    if (int rc = issue_proposals<T>(b, 0, b->ev_done[0], seed, sweep + 1, tau, true)) return rc;
   if (recompute) {
     HIPCHK(hipStreamWaitEvent(b->aux, b->ev_done[0], 0));
+    if (hold_stream(b->aux, QMCX_HOLD_REFRESH_US)) return QMCX_ERR_HIP;
     if (int rc = refresh_det<T>(b, 0, false)) return rc;
     HIPCHK(hipEventRecord(b->ev_fresh[0], b->aux));
   }
@@ -592,7 +615,7 @@ int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompu
   if (int rc = moves_det<T>(b, 1, mode, true)) return rc;
   HIPCHK(hipEventRecord(b->ev_done[1], b->stream));
   if (!last)
-    if (int rc = issue_proposals<T>(b, 1, b->ev_done[1], seed, sweep + 1, tau)) return rc;
+    if (int rc = issue_proposals<T>(b, 1, b->ev_done[1], seed, sweep + 1, tau, true)) return rc;
   if (recompute) HIPCHK(hipStreamWaitEvent(b->stream, b->ev_fresh[0], 0));
   if (b->prof_record(3)) return QMCX_ERR_HIP;
   if (int rc = launch_gl<T>(b, true)) return rc;
@@ -600,6 +623,7 @@ int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompu
   if (recompute) {
     HIPCHK(hipEventRecord(b->ev_gl, b->stream));
     HIPCHK(hipStreamWaitEvent(b->aux, b->ev_gl, 0));
+    if (hold_stream(b->aux, QMCX_HOLD_REFRESH_US)) return QMCX_ERR_HIP;
     if (int rc = refresh_det<T>(b, 1, true)) return rc;
     hipLaunchKernelGGL(k_refresh_det, dim3((b->W + 63) / 64), dim3(64), 0, b->aux, b->W, 1, b->ld_fresh, b->ph_fresh,
                        b->logdet, b->phase, b->logv, b->hist, slot);

@@ -1145,6 +1145,17 @@ __global__ __launch_bounds__(256) void k_finish_gl(SysP<T> S, BatchP<T> B, doubl
   }
 }
 
+// Holds its stream for `ticks` of the constant 100 MHz wall clock (one lane, a bounded wait that
+// always ends).  Side-stream work released by the same event as a move launch waits behind it, so
+// the move kernel's workgroups, which each need a whole CU, are placed before the side kernels'
+// workgroups take a share of every CU.
+__global__ void k_hold(unsigned long long ticks) {
+  if (threadIdx.x == 0) {
+    const unsigned long long t0 = wall_clock64();
+    while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(8);
+  }
+}
+
 __global__ void k_next_sweep(int* sweep) {
   if (threadIdx.x == 0 && blockIdx.x == 0) sweep[0] += 1;
 }
