@@ -263,7 +263,10 @@ __device__ inline void functor_vgl(const Functor<T>& f, T r, T& u, T& du, T& d2u
   const T d = c0 * (T(-0.5) * t2 + t - T(0.5)) + c1 * (T(1.5) * t2 - T(2) * t) +
               c2 * (T(-1.5) * t2 + t + T(0.5)) + c3 * (T(0.5) * t2);
   const T dd = c0 * (T(1) - t) + c1 * (T(3) * t - T(2)) + c2 * (T(1) - T(3) * t) + c3 * t;
-  du = f.drinv * d / r;
+  if constexpr (sizeof(T) == 4)
+    du = f.drinv * d * __builtin_amdgcn_rcpf(r);   // v_rcp_f32 (1 ulp) in the FP32 form
+  else
+    du = f.drinv * d / r;
   d2u = f.drinv * f.drinv * dd;
 }
 
@@ -344,7 +347,10 @@ __device__ inline T min_image_ortho(const Cell<T>& C, T d0, T d1, T d2, T flip, 
   ox = flip * x;
   oy = flip * y;
   oz = flip * z;
-  return sqrt(x * x + y * y + z * z);
+  if constexpr (sizeof(T) == 4)
+    return __builtin_amdgcn_sqrtf(x * x + y * y + z * z);   // v_sqrt_f32 (1 ulp) in the FP32 form
+  else
+    return sqrt(x * x + y * y + z * z);
 }
 
 // ------------------------------------------------------------------ spline location
