@@ -555,7 +555,7 @@ int refresh_det(qmcx_batch_t b, int det, bool into_fresh) {
 // move launch: the FP64 refresh, and the proposals of each determinant (the second's are released
 // before evaluateGL, so they wait longer).
 #ifndef QMCX_HOLD_REFRESH_US
-#define QMCX_HOLD_REFRESH_US 100
+#define QMCX_HOLD_REFRESH_US 30
 #endif
 #ifndef QMCX_HOLD_PROP0_US
 #define QMCX_HOLD_PROP0_US 100
