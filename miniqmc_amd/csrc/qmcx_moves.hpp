@@ -741,12 +741,31 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
           const T* v2 = dvec(min(t0 + 2, nd - 1));
           const T* v3 = dvec(min(t0 + 3, nd - 1));
           double sv[4] = {0, 0, 0, 0};
-          for (int j = lane; j < n; j += 64) {
-            const double pv = (double)sPsi[j];
-            sv[0] += (double)v0[j] * pv;
-            sv[1] += (double)v1[j] * pv;
-            sv[2] += (double)v2[j] * pv;
-            sv[3] += (double)v3[j] * pv;
+          using VT = typename V16<T>::type;
+          constexpr int VW = V16<T>::n;
+          if (n % VW == 0) {
+            // 16-byte LDS reads: a lane takes VW consecutive orbitals per step
+            for (int jv = lane; jv < n / VW; jv += 64) {
+              const VT p4 = reinterpret_cast<const VT*>(sPsi)[jv];
+              const VT a0 = reinterpret_cast<const VT*>(v0)[jv], a1 = reinterpret_cast<const VT*>(v1)[jv];
+              const VT a2 = reinterpret_cast<const VT*>(v2)[jv], a3 = reinterpret_cast<const VT*>(v3)[jv];
+#pragma unroll
+              for (int q = 0; q < VW; ++q) {
+                const double pv = (double)p4[q];
+                sv[0] += (double)a0[q] * pv;
+                sv[1] += (double)a1[q] * pv;
+                sv[2] += (double)a2[q] * pv;
+                sv[3] += (double)a3[q] * pv;
+              }
+            }
+          } else {
+            for (int j = lane; j < n; j += 64) {
+              const double pv = (double)sPsi[j];
+              sv[0] += (double)v0[j] * pv;
+              sv[1] += (double)v1[j] * pv;
+              sv[2] += (double)v2[j] * pv;
+              sv[3] += (double)v3[j] * pv;
+            }
           }
           wave_sums_rows<4>(sv);
           // row r of the wave holds dot t0 + r
