@@ -131,6 +131,13 @@ int qmcx_flex_ratio_grad(qmcx_batch_t b, int iel, const double* displ, double* r
 int qmcx_flex_accept_restore(qmcx_batch_t b, int iel, const int* accepted);
 int qmcx_flex_complete_updates(qmcx_batch_t b);
 int qmcx_flex_evaluate_gl(qmcx_batch_t b, double* log_out);
+/* WaveFunction::evaluateRatios (src/QMCWaveFunctions/WaveFunction.cpp:281-297) batched over walkers: the
+ * virtual-move ratios Psi(r_iel -> vpos[w][k]) / Psi for electron iel of every walker, as the non-local
+ * pseudopotential's quadrature asks for them (src/Drivers/NonLocalPP.hpp:115-140).  Value-only SPO row dotted
+ * with the inverse row (DiracDeterminant.cpp:216-224, SPOSet.h:72-83) times the J1 and J2 ratios
+ * (OneBodyJastrow.h:136-140, TwoBodyJastrow.h:103-107).  vpos[nw][nvp][3] -> ratios[nw][nvp]; no state
+ * changes; 1 <= nvp <= 65536. */
+int qmcx_flex_evaluate_ratios(qmcx_batch_t b, int iel, int nvp, const double* vpos, double* ratios);
 
 /* Per-move record of the most recent sweep, [nw][N][9]:
  * ratio, det ratio, accepted, grad_now[3], grad_new[3].  enable must be set before the sweep. */

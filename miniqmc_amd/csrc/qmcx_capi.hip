@@ -210,6 +210,9 @@ struct qmcx_batch_s {
   double* flex_displ = nullptr;   // [W][3] proposals of the pending flex move
   int* flex_decide = nullptr;     // [W]
   int flex_iel = -1;
+  double* vr_pos = nullptr;       // [W][vr_cap][3] virtual positions of qmcx_flex_evaluate_ratios
+  double* vr_out = nullptr;       // [W][vr_cap] their ratios
+  int vr_cap = 0;
   hipStream_t stream = nullptr;
   // FP64 recompute stream: a determinant's refresh runs there, on the CUs the move
   // kernel leaves idle, while the other determinant's electrons move
@@ -370,6 +373,8 @@ int launch_moves(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
 
 // SPO rows at the proposals of electrons det*n + [e0, e0 + ne) of every walker
 constexpr int kPropWorkgroups = 1024;
+// virtual positions per walker in one qmcx_flex_evaluate_ratios call (grid x dimension)
+constexpr int kMaxVirtualPoints = 1 << 16;
 
 template <typename T>
 int launch_proposals(qmcx_batch_t b, hipStream_t st, int det, int e0, int ne, uint64_t seed, int sweep, double tau,
@@ -646,6 +651,20 @@ int settle_refresh(qmcx_batch_t b) {
 }  // namespace
 
 // ====================================================================== C ABI
+template <typename T>
+static int launch_virtual_ratios(qmcx_batch_t b, int iel, int nvp) {
+  const qmcx_system_s* s = b->sys;
+  const int nq = s->npad / (16 / (int)sizeof(T));
+  const int slices = proposal_slices<T>(s, b->slices);
+  const int nt = std::max(64, (nq * slices + 63) / 64 * 64);
+  if (nt > 1024) return fail(QMCX_ERR_ARG, "spline row does not fit a workgroup");
+  const size_t lds = sizeof(T) * (size_t)slices * s->npad + sizeof(double) * 16 * 3;
+  hipLaunchKernelGGL(k_virtual_ratios<T>, dim3(nvp, b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(),
+                     iel, nvp, b->vr_pos, b->vr_out, slices);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 extern "C" {
 
 const char* qmcx_last_error(void) { return g_err.c_str(); }
@@ -950,6 +969,8 @@ int qmcx_batch_destroy(qmcx_batch_t b) {
   if (b->hist) hipFree(b->hist);
   if (b->flex_displ) hipFree(b->flex_displ);
   if (b->flex_decide) hipFree(b->flex_decide);
+  if (b->vr_pos) hipFree(b->vr_pos);
+  if (b->vr_out) hipFree(b->vr_out);
   if (b->stream) hipStreamDestroy(b->stream);
   if (b->aux) hipStreamDestroy(b->aux);
   if (b->pre) hipStreamDestroy(b->pre);
@@ -1208,6 +1229,29 @@ int qmcx_flex_evaluate_gl(qmcx_batch_t b, double* log_out) {
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(b->stream));
   if (log_out) HIPCHK(hipMemcpy(log_out, b->logv, sizeof(double) * b->W, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int qmcx_flex_evaluate_ratios(qmcx_batch_t b, int iel, int nvp, const double* vpos, double* ratios) {
+  if (!b || !vpos || !ratios || iel < 0 || iel >= b->sys->N || nvp < 1 || nvp > kMaxVirtualPoints)
+    return fail(QMCX_ERR_ARG, "bad argument");
+  if (nvp > b->vr_cap) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (b->vr_pos) hipFree(b->vr_pos);
+    if (b->vr_out) hipFree(b->vr_out);
+    b->vr_pos = b->vr_out = nullptr;
+    b->vr_cap = 0;
+    HIPCHK(hipMalloc(&b->vr_pos, sizeof(double) * 3 * (size_t)nvp * b->W));
+    HIPCHK(hipMalloc(&b->vr_out, sizeof(double) * (size_t)nvp * b->W));
+    b->vr_cap = nvp;
+  }
+  const size_t np = (size_t)nvp * b->W;
+  HIPCHK(hipMemcpyAsync(b->vr_pos, vpos, sizeof(double) * 3 * np, hipMemcpyHostToDevice, b->stream));
+  const int rc = b->prec == QMCX_FP32 ? launch_virtual_ratios<float>(b, iel, nvp)
+                                      : launch_virtual_ratios<double>(b, iel, nvp);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(ratios, b->vr_out, sizeof(double) * np, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
 }
 

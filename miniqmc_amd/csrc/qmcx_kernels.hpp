@@ -82,7 +82,7 @@ struct MoveArgs {
 };
 
 // ------------------------------------------------------------------ spline gather
-// One workgroup evaluates NO outputs (5: v, gx, gy, gz, hxx; 10: full VGH) of
+// One workgroup evaluates NO outputs (1: v; 5: v, gx, gy, gz, hxx; 10: full VGH) of
 // every orbital at one point.  Lane (q, s): q = 16-byte orbital group, s =
 // stencil slice owning 16/S of the (i,j) columns of the 4x4x4 stencil; each
 // lane issues 4*16/S coalesced 16-byte loads along the orbital axis.  Slice
@@ -147,19 +147,21 @@ __device__ inline void spline_gather(const SysP<T>& S, const double* spoG, doubl
     for (int e = 0; e < VW; ++e) {
       const T q0 = v4[pp][0][e], q1 = v4[pp][1][e], q2 = v4[pp][2][e], q3 = v4[pp][3][e];
       const T s0 = c[0] * q0 + c[1] * q1 + c[2] * q2 + c[3] * q3;
-      const T s1 = dc[0] * q0 + dc[1] * q1 + dc[2] * q2 + dc[3] * q3;
       acc[0][e] += w00 * s0;
-      acc[1][e] += w10 * s0;
-      acc[2][e] += w01 * s0;
-      acc[3][e] += w00 * s1;
-      acc[4][e] += w20 * s0;
-      if constexpr (NO == 10) {
-        const T s2 = d2c[0] * q0 + d2c[1] * q1 + d2c[2] * q2 + d2c[3] * q3;
-        acc[5][e] += w11 * s0;
-        acc[6][e] += w10 * s1;
-        acc[7][e] += w02 * s0;
-        acc[8][e] += w01 * s1;
-        acc[9][e] += w00 * s2;
+      if constexpr (NO >= 5) {
+        const T s1 = dc[0] * q0 + dc[1] * q1 + dc[2] * q2 + dc[3] * q3;
+        acc[1][e] += w10 * s0;
+        acc[2][e] += w01 * s0;
+        acc[3][e] += w00 * s1;
+        acc[4][e] += w20 * s0;
+        if constexpr (NO == 10) {
+          const T s2 = d2c[0] * q0 + d2c[1] * q1 + d2c[2] * q2 + d2c[3] * q3;
+          acc[5][e] += w11 * s0;
+          acc[6][e] += w10 * s1;
+          acc[7][e] += w02 * s0;
+          acc[8][e] += w01 * s1;
+          acc[9][e] += w00 * s2;
+        }
       }
     }
   }
@@ -980,6 +982,65 @@ __global__ __launch_bounds__(1024) void k_spline_vgh(SysP<T> S, const double* r,
   spline_gather_rt<T, 10>(S, r[3 * p], r[3 * p + 1], r[3 * p + 2], slices, part);
   __syncthreads();
   spline_finish<T, 10, double>(S, slices, part, out + (size_t)p * 10 * norb, norb, norb);
+}
+
+// ------------------------------------------------------------------ virtual-move ratios
+// WaveFunction::evaluateRatios (WaveFunction.cpp:281-297) for electron iel of every walker at
+// nvp virtual positions vpos[w][k][3], ratios[w][k]:
+//   determinant: value-only SPO row at the point dotted with the electron's inverse row
+//     (DiracDeterminant::evaluateRatios, DiracDeterminant.cpp:216-224; SPOSet::evaluateDetRatios,
+//     SPOSet.h:72-83; MultiBspline::evaluate_v, MultiBspline.hpp:34-63);
+//   J1: exp(Vat[iel] - sum over ions of u(|ion - point|))   (OneBodyJastrow.h:136-140);
+//   J2: exp(Uat[iel] - sum over j != iel of u(|r_j - point|)) (TwoBodyJastrow.h:103-107).
+// One workgroup per (point, walker).  The inverse must carry no pending delayed rows, which
+// holds between the batch's calls.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_virtual_ratios(SysP<T> S, BatchP<T> B, int iel, int nvp, const double* vpos,
+                                                         double* ratios, int slices) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* part = (T*)smem;                                  // [slices][npad]
+  double* red = (double*)(part + (size_t)slices * S.npad);
+  const int k = blockIdx.x, w = blockIdx.y, N = S.N, n = S.n, M = S.M;
+  const double* p = vpos + ((size_t)w * nvp + k) * 3;
+  const double x = p[0], y = p[1], z = p[2];
+  spline_gather_rt<T, 1>(S, x, y, z, slices, part);
+  __syncthreads();
+  const int det = iel < n ? 0 : 1, row = iel - det * n;
+  const T* arow = B.Ainv + ((size_t)w * 2 + det) * n * n + (size_t)row * n;
+  double v[3] = {0.0, 0.0, 0.0};   // det ratio, J1 sum, J2 sum
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    T s = 0;
+    for (int sl = 0; sl < slices; ++sl) s += part[(size_t)sl * S.npad + j];
+    v[0] += (double)s * (double)arow[j];
+  }
+  const Cell<T> C = *S.cell;
+  T dx, dy, dz, u, du, d2u;
+  for (int ci = threadIdx.x; ci < M; ci += blockDim.x) {
+    const Functor<T>& f = S.fn[S.species[ci]];
+    if (f.ncoef == 0) continue;
+    const T r = min_image(C, (T)(S.ions[ci] - x), (T)(S.ions[M + ci] - y), (T)(S.ions[2 * M + ci] - z), T(-1), dx,
+                          dy, dz);
+    if (r < f.rcut) {
+      functor_vgl(f, r, u, du, d2u);
+      v[1] += (double)u;
+    }
+  }
+  const double* R = B.R + (size_t)w * 3 * N;
+  const bool up = iel < n;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    if (j == iel) continue;
+    const Functor<T>& f = S.fn[kMaxSpecies + (((j < n) == up) ? 0 : 1)];
+    const T r = min_image(C, (T)(R[j] - x), (T)(R[N + j] - y), (T)(R[2 * N + j] - z), T(-1), dx, dy, dz);
+    if (r < f.rcut) {
+      functor_vgl(f, r, u, du, d2u);
+      v[2] += (double)u;
+    }
+  }
+  block_sum<double, 3>(v, red);
+  if (threadIdx.x == 0) {
+    const size_t o = (size_t)w * 5 * N + iel;
+    ratios[(size_t)w * nvp + k] = v[0] * exp(((double)B.J1[o] - v[1]) + ((double)B.J2[o] - v[2]));
+  }
 }
 
 // stand-alone distance rows: lane per (source, target)

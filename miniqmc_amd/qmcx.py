@@ -63,6 +63,7 @@ _SIGS = {
     "qmcx_flex_accept_restore": [_vp, _i, _ip],
     "qmcx_flex_complete_updates": [_vp],
     "qmcx_flex_evaluate_gl": [_vp, _vp],
+    "qmcx_flex_evaluate_ratios": [_vp, _i, _i, _dp, _vp],
     "qmcx_du_create": [C.POINTER(_vp), _i, _i, _i],
     "qmcx_du_destroy": [_vp],
     "qmcx_du_invert_transpose": [_vp, _dp, _vp, _vp, _vp],
@@ -312,6 +313,14 @@ class WalkerBatch:
     def flex_evaluate_gl(self):
         out = np.zeros(self.nw)
         _check(load().qmcx_flex_evaluate_gl(self.h, out.ctypes.data))
+        return out
+
+    def flex_evaluate_ratios(self, iel, vpos):
+        """WaveFunction::evaluateRatios for electron iel of every walker at the virtual positions
+        vpos[nw][nvp][3]; returns ratios[nw][nvp]."""
+        vpos = _c(vpos).reshape(self.nw, -1, 3)
+        out = np.zeros((self.nw, vpos.shape[1]))
+        _check(load().qmcx_flex_evaluate_ratios(self.h, iel, vpos.shape[1], vpos.reshape(-1), out.ctypes.data))
         return out
 
     def set_kernel_timing(self, on=True):

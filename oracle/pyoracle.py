@@ -64,6 +64,7 @@ def lib():
         L.orc_run_vmc.restype = C.c_double
         L.orc_evaluate_log.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p]
         L.orc_evaluate_log.restype = C.c_double
+        L.orc_evaluate_ratios.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -193,6 +194,13 @@ class OracleSystem:
         G, L = np.zeros((N, 3)), np.zeros(N)
         v = lib().orc_evaluate_log(self.h, _c(R).reshape(-1), G.ctypes.data, L.ctypes.data)
         return v, G, L
+
+    def evaluate_ratios(self, R, iel, vpos):
+        """WaveFunction::evaluateRatios of electron iel at vpos[nvp][3] for configuration R[N][3]."""
+        vpos = _c(vpos).reshape(-1, 3)
+        out = np.zeros(len(vpos))
+        lib().orc_evaluate_ratios(self.h, _c(R).reshape(-1), iel, len(vpos), vpos.reshape(-1), out)
+        return out
 
     def run_vmc(self, R, nsweeps, seed, tau, mode=1, delay=1, walker0=0, trace=False, nthreads=0):
         R = _c(R).copy()

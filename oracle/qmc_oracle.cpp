@@ -1277,6 +1277,41 @@ double orc_run_vmc(void* h, int nw, int walker0, double* R, int nsweeps, uint64_
 }
 
 // evaluateLog only (fresh), with G and L: for checking the incremental state
+// WaveFunction::evaluateRatios (WaveFunction.cpp:281-297) for electron iel at nvp virtual
+// positions vpos[nvp][3], after evaluateLog of R:
+//   determinant  psi(vpos) . invRow(iel)  (DiracDeterminant.cpp:216-224, SPOSet.h:72-83)
+//   J1           exp(Vat[iel] - sum_I u(|r_I - vpos|))      (OneBodyJastrow.h:136-140)
+//   J2           exp(Uat[iel] - sum_{j != iel} u(|r_j - vpos|)) (TwoBodyJastrow.h:103-107)
+// The virtual particle set's tables are BA rows from the point (VirtualParticleSet.cpp:35-56).
+void orc_evaluate_ratios(void* h, const double* R, int iel, int nvp, const double* vpos, double* ratios) {
+  const System& s = ((OrcSystem*)h)->sys;
+  Walker w;
+  walker_init(w, s, R, 1);
+  evaluate_log(w);
+  Determinant& D = det_of(w, iel);
+  const int n = D.n, row = iel - D.first, N = w.N;
+  std::vector<double> psi(n), dpsi(3 * (size_t)n), d2psi(n), scratch;
+  std::vector<double> r(std::max(N, s.nions)), dx(r.size()), dy(r.size()), dz(r.size());
+  for (int k = 0; k < nvp; ++k) {
+    const double* p = vpos + 3 * k;
+    spo_eval(s, p, psi.data(), dpsi.data(), d2psi.data(), scratch);
+    const double det_ratio = dot_n(n, psi.data(), &D.psiM[(size_t)row * n]);
+    ei_row(w, p, r.data(), dx.data(), dy.data(), dz.data());
+    double v1 = 0.0;
+    for (int c = 0; c < s.nions; ++c) {
+      const Functor& f = s.j1[s.ion_species[c]];
+      if (f.present && r[c] < f.rcut) v1 += functor_v(f, r[c]);
+    }
+    min_image_row(s.lat, p, w.Rx.data(), w.Ry.data(), w.Rz.data(), N, 0, r.data(), dx.data(), dy.data(), dz.data());
+    double v2 = 0.0;
+    for (int j = 0; j < N; ++j) {
+      const Functor& f = j2f(s, iel, j);
+      if (j != iel && r[j] < f.rcut) v2 += functor_v(f, r[j]);
+    }
+    ratios[k] = det_ratio * std::exp((w.Vat[iel] - v1) + (w.Uat[iel] - v2));
+  }
+}
+
 double orc_evaluate_log(void* h, const double* R, double* G, double* L) {
   const System& s = ((OrcSystem*)h)->sys;
   Walker w;

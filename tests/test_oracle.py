@@ -20,7 +20,7 @@ import os
 import numpy as np
 import pytest
 
-from miniqmc_amd import systems
+from miniqmc_amd import nlpp, systems
 from oracle import pyoracle as po
 
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
@@ -307,3 +307,79 @@ def test_prime_seed_kat():
     primes[i].  The build's walker streams use Philox keys instead; the KAT pins the reference layout."""
     primes = [p for p in range(3, 30) if all(p % d for d in range(2, p))]
     assert primes[:8] == GOLD["prime_seeds"]["first"]
+
+
+# ------------------------------------------------------------------ virtual-move ratios
+def test_quadrature_knots_kat():
+    q = nlpp.quadrature_knots()
+    assert np.allclose(np.linalg.norm(q, axis=1), 1.0)
+    assert np.allclose(q.sum(axis=0), 0.0)                     # antipodal pairs
+    # every vertex has five nearest neighbours at the icosahedron's edge angle
+    c = q @ q.T
+    assert all(np.sum(np.isclose(row, 1 / np.sqrt(5))) == 5 for row in c)
+    assert np.allclose(q[3], [-0.4472135955, 0.7236067977, 0.5257311121], atol=1e-10)
+    # a rule of degree 5: the second moments are isotropic
+    assert np.allclose(q.T @ q / len(q), np.eye(3) / 3)
+
+
+def test_random_rotation_is_proper():
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        m = nlpp.random_rotation(rng.random(3))
+        assert np.allclose(m @ m.T, np.eye(3), atol=1e-12)
+        assert abs(np.linalg.det(m) - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["nio8-cubic", "nio32"])
+def test_electron_ion_displacements_vs_oracle(name):
+    sysd = systems.make_system(name)
+    R = systems.initial_positions(sysd, 1, seed=2)[0]
+    for i in range(0, sysd.nel, max(1, sysd.nel // 7)):
+        disp, dist = nlpp.electron_ion_displacements(sysd.lattice, sysd.ion_pos, R[i])
+        r, dr = po.distance_row(sysd.lattice, R[i], sysd.ion_pos)
+        np.testing.assert_allclose(dist, r, atol=1e-10)
+        np.testing.assert_allclose(disp, dr, atol=1e-10)
+
+
+def _vpos_near_ion(sysd, R, iel, rng, npts=12):
+    """Points on a sphere around the ion nearest electron iel, radius its distance (NonLocalPP::evaluate)."""
+    d = sysd.ion_pos - R[iel]
+    frac = d @ np.linalg.inv(sysd.lattice)
+    d = (frac - np.round(frac)) @ sysd.lattice
+    c = int(np.argmin(np.linalg.norm(d, axis=1)))
+    rot = np.linalg.qr(rng.standard_normal((3, 3)))[0]
+    return R[iel] + d[c] + np.linalg.norm(d[c]) * (nlpp.quadrature_knots() @ rot.T)[:npts]
+
+
+def test_evaluate_ratios_equal_log_differences():
+    """|Psi(r_iel -> v) / Psi| from the inverse row and the stored Jastrow sums equals
+    exp(log|Psi(R')| - log|Psi(R)|) from two full evaluateLog calls (LU and Jastrow from scratch)."""
+    sysd = systems.make_system("nio8-cubic")
+    orc = po.OracleSystem(sysd)
+    R = systems.initial_positions(sysd, 1, seed=5)[0]
+    rng = np.random.default_rng(3)
+    log0 = orc.evaluate_log(R)[0]
+    for iel in (0, 3, sysd.nup, sysd.nel - 1):
+        v = _vpos_near_ion(sysd, R, iel, rng)
+        rat = orc.evaluate_ratios(R, iel, v)
+        for k in range(len(v)):
+            R2 = R.copy()
+            R2[iel] = v[k]
+            ref = np.exp(orc.evaluate_log(R2)[0] - log0)
+            assert abs(abs(rat[k]) - ref) <= 1e-9 * ref, (iel, k, rat[k], ref)
+        # the electron's own position is a ratio of exactly one
+        assert abs(orc.evaluate_ratios(R, iel, R[iel])[0] - 1.0) < 1e-11
+
+
+def test_evaluate_ratios_equal_first_move_ratio():
+    """The signed ratio at the first proposal of a sweep equals ratioGrad's ratio in the sweep's trace."""
+    sysd = systems.make_system("nio8-cubic")
+    orc = po.OracleSystem(sysd)
+    R0 = systems.initial_positions(sysd, 2)
+    seed, tau = 11, 0.5
+    res = orc.run_vmc(R0, 1, seed, tau, mode=1, delay=1, trace=True)
+    for w in range(2):
+        g, _ = po.move_randoms(seed, w, 0, 0)
+        v = R0[w, 0] + np.sqrt(tau) * np.asarray(g)
+        rat = orc.evaluate_ratios(R0[w], 0, v)[0]
+        assert abs(rat - res["trace"][w, 0, 0, 0]) < 1e-10 * max(1.0, abs(rat))
