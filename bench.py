@@ -24,7 +24,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from miniqmc_amd import qmcx, systems  # noqa: E402
+from miniqmc_amd import nlpp, qmcx, systems  # noqa: E402
 from miniqmc_amd import autotune  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -186,6 +186,37 @@ def measure_leg(dev, sysd, R0, k, tile, mode, steps, warmup, fp64_steps, err_ste
 
 
 
+def measure_nlpp(dev, sysd, R0, prec, steps, seed, tau, rmax=1.7):
+    """NonLocalPP::evaluate over the batch (qmcx_nlpp_evaluate) after evaluateLog and one sweep:
+    virtual-move ratios per second, outputs left in HBM (only the pair counts come back)."""
+    W = R0.shape[0]
+    wb = qmcx.WalkerBatch(dev, W, delay_rank=1, precision=prec)
+    wb.set_positions(R0)
+    wb.evaluate_log()
+    wb.run(1, seed, tau, qmcx.ACCEPT_REFERENCE)
+    pp = nlpp.NonLocalPP(sysd, rmax=rmax, seed=seed)
+    sphere = pp.randomize()
+    _, cnt = wb.nlpp_evaluate(rmax, sphere, max_pairs=8)
+    ratios = int(cnt.sum()) * nlpp.NKNOTS
+    wb.nlpp_evaluate(rmax, sphere, max_pairs=8, copy_out=False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        wb.nlpp_evaluate(rmax, sphere, max_pairs=8, copy_out=False)
+    dt = (time.perf_counter() - t0) / steps
+    s = 4 if prec == qmcx.FP32 else 8
+    # per ratio: the value-only 4x4x4 stencil of every orbital, the inverse row, the other
+    # electrons' positions and the J1/J2 state read once
+    bpr = 64 * sysd.npad * s + sysd.nup * s + 3 * 8 * sysd.nel
+    gbs = bpr * ratios / dt / 1e9
+    return {"value": ratios / dt, "unit": "virtual-move ratios/s", "ms_per_evaluate": 1e3 * dt, "steps": steps,
+            "dtype": "f32" if prec == qmcx.FP32 else "f64", "rmax": rmax, "electron_ion_pairs": int(cnt.sum()),
+            "ratios_per_evaluate": ratios,
+            "roofline": {"bound": "hbm", "kernel": "k_nlpp_ratios", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                         "achieved": round(gbs, 2), "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_ratio": bpr,
+                         "note": "algorithmic bytes; the twelve knots of a pair share most spline stencils in L2, "
+                                 "so this can exceed the HBM peak"}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +236,7 @@ def main():
     ap.add_argument("--recompute", type=int, default=1,
                     help="FP32 form: FP64 LU inverse recompute every this many sweeps (0: never)")
     ap.add_argument("--err-steps", type=int, default=2, help="sweeps of the FP32-vs-FP64 error leg")
+    ap.add_argument("--nlpp-steps", type=int, default=5, help="timed NonLocalPP evaluations per precision (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device five times (a kernel of known HBM traffic "
@@ -300,6 +332,14 @@ def main():
                           frac=round(sweep_gbs / HBM_PEAK_GBS, 4), bytes_per_move=round(bpm_path, 1),
                           note="whole path's algorithmic bytes per walker-move over the timed sweeps")
 
+    nl = {}
+    if args.nlpp_steps > 0:
+        for prec in (qmcx.FP32, qmcx.FP64):
+            r = measure_nlpp(dev, sysd, R0, prec, args.nlpp_steps, args.seed, args.tau)
+            nl[r["dtype"]] = r
+            log(f"NonLocalPP evaluate ({r['dtype']}): {r['value']:.4e} ratios/s, {r['ms_per_evaluate']:.2f} ms for "
+                f"{r['ratios_per_evaluate']} ratios ({r['electron_ion_pairs']} electron-ion pairs)")
+
     cpu = None
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
@@ -350,6 +390,7 @@ def main():
         "stages": stages,
         "bytes_per_move_parts": parts,
         "autotune": tune,
+        "nlpp_virtual_ratios": nl or None,
     }
     print(json.dumps(out))
 

@@ -138,6 +138,13 @@ int qmcx_flex_evaluate_gl(qmcx_batch_t b, double* log_out);
  * (OneBodyJastrow.h:136-140, TwoBodyJastrow.h:103-107).  vpos[nw][nvp][3] -> ratios[nw][nvp]; no state
  * changes; 1 <= nvp <= 65536. */
 int qmcx_flex_evaluate_ratios(qmcx_batch_t b, int iel, int nvp, const double* vpos, double* ratios);
+/* NonLocalPP::evaluate (src/Drivers/NonLocalPP.hpp:115-140; driver src/Drivers/miniqmc.cpp:470-495) for every
+ * electron of every walker: for each ion within rmax of the electron (minimum image, ions in index order) the twelve
+ * ratios at dist * sphere[k] + displ + r_e, with sphere[12][3] the rotated quadrature knots of this evaluation.
+ * ratios[nw][N][max_pairs][12] and npairs[nw][N] (either may be NULL; unused slots are left as they were);
+ * 1 <= max_pairs <= 64; QMCX_ERR_ARG when an electron has more than max_pairs ions inside rmax. */
+int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* sphere, int max_pairs, double* ratios,
+                       int* npairs);
 
 /* Per-move record of the most recent sweep, [nw][N][9]:
  * ratio, det ratio, accepted, grad_now[3], grad_new[3].  enable must be set before the sweep. */

@@ -213,6 +213,9 @@ struct qmcx_batch_s {
   double* vr_pos = nullptr;       // [W][vr_cap][3] virtual positions of qmcx_flex_evaluate_ratios
   double* vr_out = nullptr;       // [W][vr_cap] their ratios
   int vr_cap = 0;
+  double* nl_out = nullptr;       // [W][N][nl_cap][12] NonLocalPP ratios
+  int* nl_cnt = nullptr;          // [W][N] electron-ion pairs inside rmax
+  int nl_cap = 0;
   hipStream_t stream = nullptr;
   // FP64 recompute stream: a determinant's refresh runs there, on the CUs the move
   // kernel leaves idle, while the other determinant's electrons move
@@ -375,6 +378,8 @@ int launch_moves(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
 constexpr int kPropWorkgroups = 1024;
 // virtual positions per walker in one qmcx_flex_evaluate_ratios call (grid x dimension)
 constexpr int kMaxVirtualPoints = 1 << 16;
+// electron-ion pairs per electron held by qmcx_nlpp_evaluate
+constexpr int kMaxNlppPairs = 64;
 
 template <typename T>
 int launch_proposals(qmcx_batch_t b, hipStream_t st, int det, int e0, int ne, uint64_t seed, int sweep, double tau,
@@ -661,6 +666,20 @@ static int launch_virtual_ratios(qmcx_batch_t b, int iel, int nvp) {
   const size_t lds = sizeof(T) * (size_t)slices * s->npad + sizeof(double) * 16 * 3;
   hipLaunchKernelGGL(k_virtual_ratios<T>, dim3(nvp, b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(),
                      iel, nvp, b->vr_pos, b->vr_out, slices);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+static int launch_nlpp(qmcx_batch_t b, double rmax, const NlppKnots& K, int maxp) {
+  const qmcx_system_s* s = b->sys;
+  const int nq = s->npad / (16 / (int)sizeof(T));
+  const int slices = proposal_slices<T>(s, b->slices);
+  const int nt = std::max(64, (nq * slices + 63) / 64 * 64);
+  if (nt > 1024) return fail(QMCX_ERR_ARG, "spline row does not fit a workgroup");
+  const size_t lds = sizeof(T) * (size_t)slices * s->npad + sizeof(double) * (48 + 4 * (size_t)maxp) + 16;
+  hipLaunchKernelGGL(k_nlpp_ratios<T>, dim3(s->N, b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(),
+                     rmax, K, maxp, b->nl_out, b->nl_cnt, slices);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -971,6 +990,8 @@ int qmcx_batch_destroy(qmcx_batch_t b) {
   if (b->flex_decide) hipFree(b->flex_decide);
   if (b->vr_pos) hipFree(b->vr_pos);
   if (b->vr_out) hipFree(b->vr_out);
+  if (b->nl_out) hipFree(b->nl_out);
+  if (b->nl_cnt) hipFree(b->nl_cnt);
   if (b->stream) hipStreamDestroy(b->stream);
   if (b->aux) hipStreamDestroy(b->aux);
   if (b->pre) hipStreamDestroy(b->pre);
@@ -1252,6 +1273,38 @@ int qmcx_flex_evaluate_ratios(qmcx_batch_t b, int iel, int nvp, const double* vp
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(ratios, b->vr_out, sizeof(double) * np, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* sphere, int max_pairs, double* ratios,
+                       int* npairs) {
+  if (!b || !sphere || !(rmax > 0) || max_pairs < 1 || max_pairs > kMaxNlppPairs) return fail(QMCX_ERR_ARG, "bad argument");
+  const size_t WN = (size_t)b->W * b->sys->N;
+  if (max_pairs > b->nl_cap) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (b->nl_out) hipFree(b->nl_out);
+    if (b->nl_cnt) hipFree(b->nl_cnt);
+    b->nl_out = nullptr;
+    b->nl_cnt = nullptr;
+    b->nl_cap = 0;
+    HIPCHK(hipMalloc(&b->nl_out, sizeof(double) * WN * max_pairs * kNlppKnots));
+    HIPCHK(hipMalloc(&b->nl_cnt, sizeof(int) * WN));
+    b->nl_cap = max_pairs;
+  }
+  NlppKnots K;
+  for (int k = 0; k < kNlppKnots; ++k)
+    for (int d = 0; d < 3; ++d) K.p[k][d] = sphere[3 * k + d];
+  const int rc = b->prec == QMCX_FP32 ? launch_nlpp<float>(b, rmax, K, max_pairs) : launch_nlpp<double>(b, rmax, K, max_pairs);
+  if (rc) return rc;
+  std::vector<int> cnt(WN);
+  HIPCHK(hipMemcpyAsync(cnt.data(), b->nl_cnt, sizeof(int) * WN, hipMemcpyDeviceToHost, b->stream));
+  if (ratios)
+    HIPCHK(hipMemcpyAsync(ratios, b->nl_out, sizeof(double) * WN * max_pairs * kNlppKnots, hipMemcpyDeviceToHost,
+                          b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (npairs) std::copy(cnt.begin(), cnt.end(), npairs);
+  for (size_t i = 0; i < WN; ++i)
+    if (cnt[i] > max_pairs) return fail(QMCX_ERR_ARG, "more ions within rmax of an electron than max_pairs");
   return 0;
 }
 

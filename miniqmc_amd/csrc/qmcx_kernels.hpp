@@ -994,15 +994,12 @@ __global__ __launch_bounds__(1024) void k_spline_vgh(SysP<T> S, const double* r,
 //   J2: exp(Uat[iel] - sum over j != iel of u(|r_j - point|)) (TwoBodyJastrow.h:103-107).
 // One workgroup per (point, walker).  The inverse must carry no pending delayed rows, which
 // holds between the batch's calls.
+// Ratio at one point for walker w, electron iel; the workgroup's lanes share the work and
+// every lane returns the ratio.  part: LDS [slices][npad] T; red: LDS 48 doubles.
 template <typename T>
-__global__ __launch_bounds__(1024) void k_virtual_ratios(SysP<T> S, BatchP<T> B, int iel, int nvp, const double* vpos,
-                                                         double* ratios, int slices) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  T* part = (T*)smem;                                  // [slices][npad]
-  double* red = (double*)(part + (size_t)slices * S.npad);
-  const int k = blockIdx.x, w = blockIdx.y, N = S.N, n = S.n, M = S.M;
-  const double* p = vpos + ((size_t)w * nvp + k) * 3;
-  const double x = p[0], y = p[1], z = p[2];
+__device__ inline double virtual_ratio(const SysP<T>& S, const BatchP<T>& B, const Cell<T>& C, int w, int iel,
+                                       double x, double y, double z, int slices, T* part, double* red) {
+  const int N = S.N, n = S.n, M = S.M;
   spline_gather_rt<T, 1>(S, x, y, z, slices, part);
   __syncthreads();
   const int det = iel < n ? 0 : 1, row = iel - det * n;
@@ -1013,7 +1010,6 @@ __global__ __launch_bounds__(1024) void k_virtual_ratios(SysP<T> S, BatchP<T> B,
     for (int sl = 0; sl < slices; ++sl) s += part[(size_t)sl * S.npad + j];
     v[0] += (double)s * (double)arow[j];
   }
-  const Cell<T> C = *S.cell;
   T dx, dy, dz, u, du, d2u;
   for (int ci = threadIdx.x; ci < M; ci += blockDim.x) {
     const Functor<T>& f = S.fn[S.species[ci]];
@@ -1036,10 +1032,88 @@ __global__ __launch_bounds__(1024) void k_virtual_ratios(SysP<T> S, BatchP<T> B,
       v[2] += (double)u;
     }
   }
-  block_sum<double, 3>(v, red);
-  if (threadIdx.x == 0) {
-    const size_t o = (size_t)w * 5 * N + iel;
-    ratios[(size_t)w * nvp + k] = v[0] * exp(((double)B.J1[o] - v[1]) + ((double)B.J2[o] - v[2]));
+  block_sum<double, 3>(v, red);   // ends with a barrier: part and red are free again
+  const size_t o = (size_t)w * 5 * N + iel;
+  return v[0] * exp(((double)B.J1[o] - v[1]) + ((double)B.J2[o] - v[2]));
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_virtual_ratios(SysP<T> S, BatchP<T> B, int iel, int nvp, const double* vpos,
+                                                         double* ratios, int slices) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* part = (T*)smem;                                  // [slices][npad]
+  double* red = (double*)(part + (size_t)slices * S.npad);
+  const int k = blockIdx.x, w = blockIdx.y;
+  const double* p = vpos + ((size_t)w * nvp + k) * 3;
+  const Cell<T> C = *S.cell;
+  const double r = virtual_ratio<T>(S, B, C, w, iel, p[0], p[1], p[2], slices, part, red);
+  if (threadIdx.x == 0) ratios[(size_t)w * nvp + k] = r;
+}
+
+// NonLocalPP::evaluate (NonLocalPP.hpp:115-140) for every electron of every walker: the ions
+// within rmax of the electron (minimum image, ion - electron as the BA table's displacement)
+// are listed in ion order, and for each the twelve knots are placed at
+// dist * sphere[k] + displ + r_e (NonLocalPP.hpp:127-128).  One workgroup per (electron,
+// walker); ratios[w][e][maxp][kNlppKnots], npairs[w][e] (the full count, also when it exceeds
+// maxp: then only the first maxp pairs are evaluated and the caller reports the overflow).
+constexpr int kNlppKnots = 12;
+struct NlppKnots {
+  double p[kNlppKnots][3];
+};
+template <typename T>
+__global__ __launch_bounds__(1024) void k_nlpp_ratios(SysP<T> S, BatchP<T> B, double rmax, NlppKnots K, int maxp,
+                                                      double* ratios, int* npairs, int slices) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* part = (T*)smem;
+  double* red = (double*)(part + (size_t)slices * S.npad);
+  double* near = red + 48;                              // [maxp][4]: dist, displacement
+  int* cnt = (int*)(near + 4 * (size_t)maxp);
+  const int e = blockIdx.x, w = blockIdx.y, N = S.N, M = S.M;
+  const Cell<T> C = *S.cell;
+  const double* R = B.R + (size_t)w * 3 * N;
+  const double x = R[e], y = R[N + e], z = R[2 * N + e];
+  if (threadIdx.x == 0) *cnt = 0;
+  __syncthreads();
+  // ion order is kept: a wave's lanes take consecutive ions and the waves run in turn
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const int nwv = (blockDim.x + kWave - 1) / kWave;
+  for (int c0 = 0; c0 < M; c0 += kWave * nwv) {
+    for (int q = 0; q < nwv; ++q) {
+      const int ci = c0 + q * kWave + lane;
+      if (wid == q) {
+        T dx = 0, dy = 0, dz = 0, r = T(0);
+        bool in = false;
+        if (ci < M) {
+          r = min_image(C, (T)(S.ions[ci] - x), (T)(S.ions[M + ci] - y), (T)(S.ions[2 * M + ci] - z), T(-1), dx, dy,
+                        dz);
+          in = (double)r < rmax;
+        }
+        const unsigned long long mask = __ballot(in);
+        const int base = *cnt;
+        const int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+        if (in && slot < maxp) {
+          near[4 * slot] = (double)r;
+          near[4 * slot + 1] = (double)dx;
+          near[4 * slot + 2] = (double)dy;
+          near[4 * slot + 3] = (double)dz;
+        }
+        if (lane == 0) *cnt = base + __popcll(mask);
+      }
+      __syncthreads();
+    }
+  }
+  const int np = *cnt;
+  if (threadIdx.x == 0) npairs[(size_t)w * N + e] = np;
+  const int use = np < maxp ? np : maxp;
+  double* out = ratios + ((size_t)w * N + e) * maxp * kNlppKnots;
+  for (int pi = 0; pi < use; ++pi) {
+    const double d = near[4 * pi];
+    const double ox = x + near[4 * pi + 1], oy = y + near[4 * pi + 2], oz = z + near[4 * pi + 3];
+    for (int k = 0; k < kNlppKnots; ++k) {
+      const double r = virtual_ratio<T>(S, B, C, w, e, d * K.p[k][0] + ox, d * K.p[k][1] + oy, d * K.p[k][2] + oz,
+                                        slices, part, red);
+      if (threadIdx.x == 0) out[pi * kNlppKnots + k] = r;
+    }
   }
 }
 

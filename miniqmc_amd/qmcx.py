@@ -64,6 +64,7 @@ _SIGS = {
     "qmcx_flex_complete_updates": [_vp],
     "qmcx_flex_evaluate_gl": [_vp, _vp],
     "qmcx_flex_evaluate_ratios": [_vp, _i, _i, _dp, _vp],
+    "qmcx_nlpp_evaluate": [_vp, C.c_double, _dp, _i, _vp, _vp],
     "qmcx_du_create": [C.POINTER(_vp), _i, _i, _i],
     "qmcx_du_destroy": [_vp],
     "qmcx_du_invert_transpose": [_vp, _dp, _vp, _vp, _vp],
@@ -322,6 +323,19 @@ class WalkerBatch:
         out = np.zeros((self.nw, vpos.shape[1]))
         _check(load().qmcx_flex_evaluate_ratios(self.h, iel, vpos.shape[1], vpos.reshape(-1), out.ctypes.data))
         return out
+
+    def nlpp_evaluate(self, rmax, sphere, max_pairs=4, copy_out=True):
+        """NonLocalPP::evaluate for every electron of every walker on the GPU: ratios
+        [nw][N][max_pairs][12] and npairs [nw][N] (None, None without copy_out)."""
+        sphere = _c(sphere).reshape(12, 3)
+        if not copy_out:
+            _check(load().qmcx_nlpp_evaluate(self.h, float(rmax), sphere.reshape(-1), max_pairs, None, None))
+            return None, None
+        out = np.zeros((self.nw, self.N, max_pairs, 12))
+        cnt = np.zeros((self.nw, self.N), dtype=np.int32)
+        _check(load().qmcx_nlpp_evaluate(self.h, float(rmax), sphere.reshape(-1), max_pairs, out.ctypes.data,
+                                         cnt.ctypes.data))
+        return out, cnt
 
     def set_kernel_timing(self, on=True):
         _check(load().qmcx_set_kernel_timing(self.h, 1 if on else 0))

@@ -112,3 +112,60 @@ def test_virtual_ratios_argument_errors():
         wb.flex_evaluate_ratios(sysd.nel, np.zeros((2, 12, 3)))
     with pytest.raises(qmcx.QmcxError):
         wb.flex_evaluate_ratios(-1, np.zeros((2, 12, 3)))
+
+
+def _expected_pairs(sysd, R, jel, rmax, sphere):
+    disp, dist = nlpp.electron_ion_displacements(sysd.lattice, sysd.ion_pos, R[jel])
+    near = np.nonzero(dist < rmax)[0]
+    border = np.any(np.abs(dist - rmax) < 1e-5)
+    return [dist[c] * sphere + disp[c] + R[jel] for c in near], border
+
+
+@pytest.mark.parametrize("name,prec,delay,tol,stride", [
+    ("nio8-cubic", qmcx.FP64, 4, 1e-10, 1),
+    ("nio8-cubic", qmcx.FP32, 4, 2e-3, 1),
+    ("nio32", qmcx.FP32, 32, 2e-3, 23),
+])
+def test_nlpp_device_evaluate_vs_oracle(name, prec, delay, tol, stride):
+    """qmcx_nlpp_evaluate: every electron's ion list inside rmax (ion order) and its 12 ratios per pair."""
+    sysd = systems.make_system(name)
+    _say("nlpp device evaluate", name, prec, delay)
+    nw, rmax = 3, 1.7
+    dev = qmcx.System(sysd)
+    wb = qmcx.WalkerBatch(dev, nw, delay_rank=delay, precision=prec)
+    wb.set_positions(systems.initial_positions(sysd, nw, seed=13))
+    wb.evaluate_log()
+    wb.run(1, 9, 0.3, qmcx.ACCEPT_METROPOLIS, recompute_every=1)
+    pp = nlpp.NonLocalPP(sysd, rmax=rmax, seed=8)
+    sphere = pp.randomize()
+    rat, cnt = wb.nlpp_evaluate(rmax, sphere, max_pairs=4)
+    R = wb.positions()
+    orc = po.OracleSystem(sysd)
+    checked = 0
+    for w in range(nw):
+        for jel in range(sysd.nel):
+            exp_v, border = _expected_pairs(sysd, R[w], jel, rmax, sphere)
+            if border:
+                continue
+            assert cnt[w, jel] == len(exp_v), (w, jel)
+            if jel % stride or not exp_v:
+                continue
+            for p, v in enumerate(exp_v):
+                ref = orc.evaluate_ratios(R[w], jel, v)
+                err = np.abs(rat[w, jel, p] - ref) / np.maximum(np.abs(ref), 1.0)
+                assert err.max() < tol, (w, jel, p, err.max())
+                checked += 1
+    _say(f"  pairs {int(cnt.sum())}, checked {checked}")
+    assert checked > 0
+
+
+def test_nlpp_device_evaluate_overflow_is_an_error():
+    sysd = systems.make_system("nio8-cubic")
+    dev = qmcx.System(sysd)
+    wb = qmcx.WalkerBatch(dev, 2, delay_rank=1, precision=qmcx.FP64)
+    wb.set_positions(systems.initial_positions(sysd, 2))
+    wb.evaluate_log()
+    with pytest.raises(qmcx.QmcxError):
+        wb.nlpp_evaluate(5.0, nlpp.quadrature_knots(), max_pairs=1)
+    with pytest.raises(qmcx.QmcxError):
+        wb.nlpp_evaluate(1.7, nlpp.quadrature_knots(), max_pairs=0)
