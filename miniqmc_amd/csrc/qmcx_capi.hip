@@ -215,6 +215,8 @@ struct qmcx_batch_s {
   int vr_cap = 0;
   double* nl_out = nullptr;       // [W][N][nl_cap][12] NonLocalPP ratios
   int* nl_cnt = nullptr;          // [W][N] electron-ion pairs inside rmax
+  double* nl_near = nullptr;      // [W][N][nl_cap][4] listed pairs: distance, displacement
+  int* nl_items = nullptr;        // [W*N*nl_cap] work list of listed pairs, then its length
   int nl_cap = 0;
   hipStream_t stream = nullptr;
   // FP64 recompute stream: a determinant's refresh runs there, on the CUs the move
@@ -380,6 +382,8 @@ constexpr int kPropWorkgroups = 1024;
 constexpr int kMaxVirtualPoints = 1 << 16;
 // electron-ion pairs per electron held by qmcx_nlpp_evaluate
 constexpr int kMaxNlppPairs = 64;
+// workgroups of the NonLocalPP ratio launch (they walk the pair list)
+constexpr int kNlppWorkgroups = 8192;
 
 template <typename T>
 int launch_proposals(qmcx_batch_t b, hipStream_t st, int det, int e0, int ne, uint64_t seed, int sweep, double tau,
@@ -670,18 +674,39 @@ static int launch_virtual_ratios(qmcx_batch_t b, int iel, int nvp) {
   return 0;
 }
 
-template <typename T>
-static int launch_nlpp(qmcx_batch_t b, double rmax, const NlppKnots& K, int maxp) {
+template <typename T, int P, int PER>
+static int launch_nlpp_p(qmcx_batch_t b, double rmax, const NlppKnots& K, int maxp) {
   const qmcx_system_s* s = b->sys;
+  const int WN = b->W * s->N;
+  int* count = b->nl_items + (size_t)WN * maxp;
+  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, b->stream, count);
+  hipLaunchKernelGGL(k_nlpp_pairs<T>, dim3((WN + 255) / 256), dim3(256), 0, b->stream, s->params<T>(), b->params<T>(),
+                     rmax, maxp, b->nl_near, b->nl_cnt, b->nl_items, count);
+  constexpr int SL = 16 / PER;
   const int nq = s->npad / (16 / (int)sizeof(T));
-  const int slices = proposal_slices<T>(s, b->slices);
-  const int nt = std::max(64, (nq * slices + 63) / 64 * 64);
-  if (nt > 1024) return fail(QMCX_ERR_ARG, "spline row does not fit a workgroup");
-  const size_t lds = sizeof(T) * (size_t)slices * s->npad + sizeof(double) * (48 + 4 * (size_t)maxp) + 16;
-  hipLaunchKernelGGL(k_nlpp_ratios<T>, dim3(s->N, b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(),
-                     rmax, K, maxp, b->nl_out, b->nl_cnt, slices);
+  const int nt = std::max(64, (P * SL * nq + 63) / 64 * 64);
+  const size_t lds = sizeof(T) * (size_t)P * SL * s->npad + sizeof(double) * 3 * P * 16;
+  // workgroups walk the work list (P knots of a pair per unit); its length is only known on the device
+  const int grid = std::max(1, std::min(WN * maxp * (kNlppKnots / P), kNlppWorkgroups));
+  hipLaunchKernelGGL((k_nlpp_ratios<T, P, PER>), dim3(grid), dim3(nt), lds, b->stream, s->params<T>(),
+                     b->params<T>(), K, maxp, b->nl_near, b->nl_items, count, b->nl_out);
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// Work-unit shape.  At NiO-64 (384 orbitals, 128 walkers, 33k electron-ion pairs) FP32 runs
+// fastest with one knot per unit and two stencil slices per lane group (7.1 ms per evaluation
+// against 8.9 ms for six knots per unit, 9.7 ms for two knots with two slices); FP64 with the
+// most knots whose value rows fit (four: 13.4 ms against 16.3 ms for one knot, two slices).
+template <typename T>
+static int launch_nlpp(qmcx_batch_t b, double rmax, const NlppKnots& K, int maxp) {
+  const int nq = b->sys->npad / (16 / (int)sizeof(T));
+  if (sizeof(T) == 4 && 2 * nq <= kNlppThreads) return launch_nlpp_p<T, 1, 8>(b, rmax, K, maxp);
+  if (6 * nq <= kNlppThreads) return launch_nlpp_p<T, 6, 16>(b, rmax, K, maxp);
+  if (4 * nq <= kNlppThreads) return launch_nlpp_p<T, 4, 16>(b, rmax, K, maxp);
+  if (2 * nq <= kNlppThreads) return launch_nlpp_p<T, 2, 16>(b, rmax, K, maxp);
+  if (nq <= kNlppThreads) return launch_nlpp_p<T, 1, 16>(b, rmax, K, maxp);
+  return fail(QMCX_ERR_ARG, "spline row does not fit a workgroup");
 }
 
 extern "C" {
@@ -992,6 +1017,8 @@ int qmcx_batch_destroy(qmcx_batch_t b) {
   if (b->vr_out) hipFree(b->vr_out);
   if (b->nl_out) hipFree(b->nl_out);
   if (b->nl_cnt) hipFree(b->nl_cnt);
+  if (b->nl_near) hipFree(b->nl_near);
+  if (b->nl_items) hipFree(b->nl_items);
   if (b->stream) hipStreamDestroy(b->stream);
   if (b->aux) hipStreamDestroy(b->aux);
   if (b->pre) hipStreamDestroy(b->pre);
@@ -1287,8 +1314,14 @@ int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* sphere, int ma
     b->nl_out = nullptr;
     b->nl_cnt = nullptr;
     b->nl_cap = 0;
+    if (b->nl_near) hipFree(b->nl_near);
+    if (b->nl_items) hipFree(b->nl_items);
+    b->nl_near = nullptr;
+    b->nl_items = nullptr;
     HIPCHK(hipMalloc(&b->nl_out, sizeof(double) * WN * max_pairs * kNlppKnots));
     HIPCHK(hipMalloc(&b->nl_cnt, sizeof(int) * WN));
+    HIPCHK(hipMalloc(&b->nl_near, sizeof(double) * 4 * WN * max_pairs));
+    HIPCHK(hipMalloc(&b->nl_items, sizeof(int) * (WN * max_pairs + 1)));
     b->nl_cap = max_pairs;
   }
   NlppKnots K;

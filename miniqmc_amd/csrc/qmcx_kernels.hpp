@@ -102,14 +102,14 @@ __device__ inline void knot_weight(int i, T t, T& w, T& dw, T& d2w) {
 }
 
 template <typename T, int NO, int PER>
-__device__ inline void spline_gather(const SysP<T>& S, const double* spoG, double x, double y, double z, T* part) {
+__device__ inline void spline_gather(const SysP<T>& S, const double* spoG, double x, double y, double z, T* part,
+                                     int tid) {
   using VT = typename V16<T>::type;
   constexpr int VW = V16<T>::n;
   constexpr int SL = 16 / PER;
   // stencil columns with loads in flight per batch: up to four in FP32, two in FP64
   constexpr int G = sizeof(T) == 8 ? (PER > 2 ? 2 : PER) : (PER > 4 ? 4 : PER);
   const int nq = S.npad / VW;
-  const int tid = threadIdx.x;
   const int q = tid % nq, s = tid / nq;
   if (s >= SL) return;
   int ix[3];
@@ -175,11 +175,11 @@ __device__ inline void spline_gather(const SysP<T>& S, const double* spoG, doubl
 template <typename T, int NO>
 __device__ inline void spline_gather_rt(const SysP<T>& S, double x, double y, double z, int slices, T* part) {
   switch (slices) {
-    case 16: spline_gather<T, NO, 1>(S, S.spoG, x, y, z, part); break;
-    case 8: spline_gather<T, NO, 2>(S, S.spoG, x, y, z, part); break;
-    case 4: spline_gather<T, NO, 4>(S, S.spoG, x, y, z, part); break;
-    case 2: spline_gather<T, NO, 8>(S, S.spoG, x, y, z, part); break;
-    default: spline_gather<T, NO, 16>(S, S.spoG, x, y, z, part); break;
+    case 16: spline_gather<T, NO, 1>(S, S.spoG, x, y, z, part, threadIdx.x); break;
+    case 8: spline_gather<T, NO, 2>(S, S.spoG, x, y, z, part, threadIdx.x); break;
+    case 4: spline_gather<T, NO, 4>(S, S.spoG, x, y, z, part, threadIdx.x); break;
+    case 2: spline_gather<T, NO, 8>(S, S.spoG, x, y, z, part, threadIdx.x); break;
+    default: spline_gather<T, NO, 16>(S, S.spoG, x, y, z, part, threadIdx.x); break;
   }
 }
 
@@ -1050,69 +1050,141 @@ __global__ __launch_bounds__(1024) void k_virtual_ratios(SysP<T> S, BatchP<T> B,
   if (threadIdx.x == 0) ratios[(size_t)w * nvp + k] = r;
 }
 
-// NonLocalPP::evaluate (NonLocalPP.hpp:115-140) for every electron of every walker: the ions
-// within rmax of the electron (minimum image, ion - electron as the BA table's displacement)
-// are listed in ion order, and for each the twelve knots are placed at
-// dist * sphere[k] + displ + r_e (NonLocalPP.hpp:127-128).  One workgroup per (electron,
-// walker); ratios[w][e][maxp][kNlppKnots], npairs[w][e] (the full count, also when it exceeds
-// maxp: then only the first maxp pairs are evaluated and the caller reports the overflow).
+// NonLocalPP::evaluate (NonLocalPP.hpp:115-140) for every electron of every walker, in two
+// launches.  k_nlpp_pairs: one lane per (walker, electron) lists the ions within rmax in ion
+// order (minimum image, ion - electron as the BA table's displacement) into
+// near[(w*N + e)*maxp + p][4] = dist, displacement, counts them in npairs[w][e] (the full count,
+// also beyond maxp) and appends each listed pair to the work list items[] (count[0] entries, in
+// no particular order).  k_nlpp_ratios: workgroups take work-list entries in turn and place the
+// twelve knots at dist * sphere[k] + displ + r_e (NonLocalPP.hpp:133);
+// ratios[w][e][maxp][kNlppKnots].
 constexpr int kNlppKnots = 12;
 struct NlppKnots {
   double p[kNlppKnots][3];
 };
+
+__global__ void k_zero_count(int* count) {
+  if (threadIdx.x == 0) count[0] = 0;
+}
+
 template <typename T>
-__global__ __launch_bounds__(1024) void k_nlpp_ratios(SysP<T> S, BatchP<T> B, double rmax, NlppKnots K, int maxp,
-                                                      double* ratios, int* npairs, int slices) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  T* part = (T*)smem;
-  double* red = (double*)(part + (size_t)slices * S.npad);
-  double* near = red + 48;                              // [maxp][4]: dist, displacement
-  int* cnt = (int*)(near + 4 * (size_t)maxp);
-  const int e = blockIdx.x, w = blockIdx.y, N = S.N, M = S.M;
+__global__ __launch_bounds__(256) void k_nlpp_pairs(SysP<T> S, BatchP<T> B, double rmax, int maxp, double* near,
+                                                    int* npairs, int* items, int* count) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x, N = S.N, M = S.M;
+  if (idx >= B.W * N) return;
+  const int w = idx / N, e = idx - w * N;
   const Cell<T> C = *S.cell;
   const double* R = B.R + (size_t)w * 3 * N;
   const double x = R[e], y = R[N + e], z = R[2 * N + e];
-  if (threadIdx.x == 0) *cnt = 0;
-  __syncthreads();
-  // ion order is kept: a wave's lanes take consecutive ions and the waves run in turn
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  const int nwv = (blockDim.x + kWave - 1) / kWave;
-  for (int c0 = 0; c0 < M; c0 += kWave * nwv) {
-    for (int q = 0; q < nwv; ++q) {
-      const int ci = c0 + q * kWave + lane;
-      if (wid == q) {
-        T dx = 0, dy = 0, dz = 0, r = T(0);
-        bool in = false;
-        if (ci < M) {
-          r = min_image(C, (T)(S.ions[ci] - x), (T)(S.ions[M + ci] - y), (T)(S.ions[2 * M + ci] - z), T(-1), dx, dy,
-                        dz);
-          in = (double)r < rmax;
-        }
-        const unsigned long long mask = __ballot(in);
-        const int base = *cnt;
-        const int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
-        if (in && slot < maxp) {
-          near[4 * slot] = (double)r;
-          near[4 * slot + 1] = (double)dx;
-          near[4 * slot + 2] = (double)dy;
-          near[4 * slot + 3] = (double)dz;
-        }
-        if (lane == 0) *cnt = base + __popcll(mask);
+  int c = 0;
+  for (int ci = 0; ci < M; ++ci) {
+    T dx, dy, dz;
+    const T r = min_image(C, (T)(S.ions[ci] - x), (T)(S.ions[M + ci] - y), (T)(S.ions[2 * M + ci] - z), T(-1), dx, dy,
+                          dz);
+    if ((double)r < rmax) {
+      if (c < maxp) {
+        double* q = near + ((size_t)idx * maxp + c) * 4;
+        q[0] = (double)r;
+        q[1] = (double)dx;
+        q[2] = (double)dy;
+        q[3] = (double)dz;
+        items[atomicAdd(count, 1)] = idx * maxp + c;
       }
-      __syncthreads();
+      ++c;
     }
   }
-  const int np = *cnt;
-  if (threadIdx.x == 0) npairs[(size_t)w * N + e] = np;
-  const int use = np < maxp ? np : maxp;
-  double* out = ratios + ((size_t)w * N + e) * maxp * kNlppKnots;
-  for (int pi = 0; pi < use; ++pi) {
-    const double d = near[4 * pi];
-    const double ox = x + near[4 * pi + 1], oy = y + near[4 * pi + 2], oz = z + near[4 * pi + 3];
-    for (int k = 0; k < kNlppKnots; ++k) {
-      const double r = virtual_ratio<T>(S, B, C, w, e, d * K.p[k][0] + ox, d * K.p[k][1] + oy, d * K.p[k][2] + oz,
-                                        slices, part, red);
-      if (threadIdx.x == 0) out[pi * kNlppKnots + k] = r;
+  npairs[idx] = c;
+}
+
+// A work unit is P of a pair's twelve knots.  Lanes [g*L, (g+1)*L), L = (16/PER)*nq, gather
+// knot g's value row (a lane: one 16-byte orbital group, PER of the 16 stencil columns), then
+// every lane takes a share of the P dot products and of the P Jastrow sums, reading each
+// electron's position once for all P knots.
+constexpr int kNlppThreads = 768;
+template <typename T, int P, int PER>
+__global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<T> B, NlppKnots K, int maxp,
+                                                              const double* near, const int* items, const int* count,
+                                                              double* ratios) {
+  constexpr int SL = 16 / PER, KG = kNlppKnots / P;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* part = (T*)smem;                                        // [P][SL][npad]
+  double* red = (double*)(part + (size_t)P * SL * S.npad);   // [3P][16]
+  const int N = S.N, n = S.n, M = S.M;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const Cell<T> C = *S.cell;
+  const int L = SL * (S.npad / V16<T>::n);
+  const int g = tid / L;
+  const int nunits = count[0] * KG;
+  for (int un = blockIdx.x; un < nunits; un += gridDim.x) {
+    const int item = items[un / KG];
+    const int k0 = (un % KG) * P;
+    const int idx = item / maxp;
+    const int w = idx / N, e = idx - w * N;
+    const double* R = B.R + (size_t)w * 3 * N;
+    const double* q = near + (size_t)item * 4;
+    const double d = q[0];
+    const double ox = R[e] + q[1], oy = R[N + e] + q[2], oz = R[2 * N + e] + q[3];
+    const int det = e < n ? 0 : 1, row = e - det * n;
+    const T* arow = B.Ainv + ((size_t)w * 2 + det) * n * n + (size_t)row * n;
+    const size_t o = (size_t)w * 5 * N + e;
+    const double j1o = (double)B.J1[o], j2o = (double)B.J2[o];
+    const bool up = e < n;
+    double* out = ratios + (size_t)item * kNlppKnots;
+    {
+      if (g < P) {
+        const int k = k0 + g;
+        spline_gather<T, 1, PER>(S, S.spoG, d * K.p[k][0] + ox, d * K.p[k][1] + oy, d * K.p[k][2] + oz,
+                                 part + (size_t)g * SL * S.npad, tid - g * L);
+      }
+      double v[3 * P];
+#pragma unroll
+      for (int i = 0; i < 3 * P; ++i) v[i] = 0.0;
+      T dx, dy, dz, u, du, d2u;
+      for (int ci = tid; ci < M; ci += nt) {
+        const Functor<T>& f = S.fn[S.species[ci]];
+        if (f.ncoef == 0) continue;
+        const double ix = S.ions[ci], iy = S.ions[M + ci], iz = S.ions[2 * M + ci];
+#pragma unroll
+        for (int gg = 0; gg < P; ++gg) {
+          const double px = d * K.p[k0 + gg][0] + ox, py = d * K.p[k0 + gg][1] + oy, pz = d * K.p[k0 + gg][2] + oz;
+          const T r = min_image(C, (T)(ix - px), (T)(iy - py), (T)(iz - pz), T(-1), dx, dy, dz);
+          if (r < f.rcut) {
+            functor_vgl(f, r, u, du, d2u);
+            v[P + gg] += (double)u;
+          }
+        }
+      }
+      for (int j = tid; j < N; j += nt) {
+        if (j == e) continue;
+        const Functor<T>& f = S.fn[kMaxSpecies + (((j < n) == up) ? 0 : 1)];
+        const double rx = R[j], ry = R[N + j], rz = R[2 * N + j];
+#pragma unroll
+        for (int gg = 0; gg < P; ++gg) {
+          const double px = d * K.p[k0 + gg][0] + ox, py = d * K.p[k0 + gg][1] + oy, pz = d * K.p[k0 + gg][2] + oz;
+          const T r = min_image(C, (T)(rx - px), (T)(ry - py), (T)(rz - pz), T(-1), dx, dy, dz);
+          if (r < f.rcut) {
+            functor_vgl(f, r, u, du, d2u);
+            v[2 * P + gg] += (double)u;
+          }
+        }
+      }
+      __syncthreads();   // the P value rows are in LDS
+      for (int ii = tid; ii < P * n; ii += nt) {
+        const int gi = ii / n, j = ii - gi * n;
+        T sv = 0;
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) sv += part[((size_t)gi * SL + sl) * S.npad + j];
+        const double t = (double)sv * (double)arow[j];
+#pragma unroll
+        for (int gg = 0; gg < P; ++gg)
+          if (gg == gi) v[gg] += t;
+      }
+      block_sum<double, 3 * P>(v, red);   // ends with a barrier: part and red are free again
+      if (tid < P) {
+#pragma unroll
+        for (int gg = 0; gg < P; ++gg)
+          if (gg == tid) out[k0 + gg] = v[gg] * exp((j1o - v[P + gg]) + (j2o - v[2 * P + gg]));
+      }
     }
   }
 }

@@ -6,7 +6,8 @@ asks the wave function for the twelve virtual-move ratios (NonLocalPP::evaluate,
 NonLocalPP.hpp:115-140; driver miniqmc.cpp:470-495).  Here one call per electron covers every
 walker: the ratios run on the GPU through ``qmcx_flex_evaluate_ratios``, and walkers with fewer
 near ions than the batch's largest count are padded with the electron's own position, whose
-ratio is exactly one and is dropped.
+ratio is one up to rounding and is dropped.  ``NonLocalPP.evaluate`` is this host loop; the
+batch's ``nlpp_evaluate`` does the same work in two device launches.
 """
 from __future__ import annotations
 
@@ -18,7 +19,7 @@ NKNOTS = 12
 def quadrature_knots() -> np.ndarray:
     """The twelve vertices of the icosahedron: (+-1, 0, 0) and ten points at x = +-1/sqrt(5) on the
     circle of radius 2/sqrt(5), 36 degrees apart with alternating sign of x (the knot set and
-    order of NonLocalPP.hpp:56-80).  Equal weights 1/12."""
+    order of NonLocalPP.hpp:70-81).  Equal weights 1/12."""
     pts = [(1.0, 0.0, 0.0), (-1.0, 0.0, 0.0)]
     for k in range(10):
         a = k * np.pi / 5
@@ -29,7 +30,7 @@ def quadrature_knots() -> np.ndarray:
 
 def random_rotation(u: np.ndarray) -> np.ndarray:
     """Uniform random rotation from three uniforms in [0, 1) (Shoemake's unit-quaternion method,
-    Graphics Gems III, 1992).  Plays the part of NonLocalPP::randomize (NonLocalPP.hpp:93-105),
+    Graphics Gems III, 1992).  Plays the part of NonLocalPP::randomize (NonLocalPP.hpp:100-113),
     whose Euler-angle draw from the reference's own generator is not reproduced."""
     u1, u2, u3 = (float(x) for x in u)
     a, b = np.sqrt(1 - u1), np.sqrt(u1)
@@ -78,7 +79,7 @@ class NonLocalPP:
         for w in range(nw):
             disp, dist = electron_ion_displacements(self.sys.lattice, self.sys.ion_pos, R[w, jel])
             near = np.nonzero(dist < self.rmax)[0]
-            # virtualPos[k] = dist * rOnSphere[k] + displ + R[jel]  (NonLocalPP.hpp:127-128)
+            # virtualPos[k] = dist * rOnSphere[k] + displ + R[jel]  (NonLocalPP.hpp:133)
             rows.append([dist[c] * sphere + disp[c] + R[w, jel] for c in near])
         npair = np.array([len(r) for r in rows])
         width = max(1, int(npair.max())) * NKNOTS
