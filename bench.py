@@ -217,6 +217,21 @@ def measure_nlpp(dev, sysd, R0, prec, steps, seed, tau, rmax=1.7):
                                  "so this can exceed the HBM peak"}}
 
 
+def measure_evaluate_log(dev, sysd, R0, prec, steps):
+    """WaveFunction::evaluateLog from scratch for the whole batch (SPO rows at every electron, FP64 LU,
+    J1/J2 over all pairs, evaluateGL): walkers per second."""
+    W = R0.shape[0]
+    wb = qmcx.WalkerBatch(dev, W, delay_rank=1, precision=prec)
+    wb.set_positions(R0)
+    wb.evaluate_log()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        wb.evaluate_log()
+    dt = (time.perf_counter() - t0) / steps
+    return {"value": W / dt, "unit": "walkers/s", "ms_per_batch": 1e3 * dt, "steps": steps, "walkers": W,
+            "dtype": "f32" if prec == qmcx.FP32 else "f64"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -340,6 +355,13 @@ def main():
             log(f"NonLocalPP evaluate ({r['dtype']}): {r['value']:.4e} ratios/s, {r['ms_per_evaluate']:.2f} ms for "
                 f"{r['ratios_per_evaluate']} ratios ({r['electron_ion_pairs']} electron-ion pairs)")
 
+    elog = {}
+    if args.nlpp_steps > 0:
+        for prec in (qmcx.FP32, qmcx.FP64):
+            r = measure_evaluate_log(dev, sysd, R0, prec, args.nlpp_steps)
+            elog[r["dtype"]] = r
+            log(f"evaluateLog ({r['dtype']}): {r['ms_per_batch']:.2f} ms for {W} walkers")
+
     cpu = None
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
@@ -391,6 +413,7 @@ def main():
         "bytes_per_move_parts": parts,
         "autotune": tune,
         "nlpp_virtual_ratios": nl or None,
+        "evaluate_log": elog or None,
     }
     print(json.dumps(out))
 
