@@ -678,16 +678,23 @@ template <typename T, int P, int PER>
 static int launch_nlpp_p(qmcx_batch_t b, double rmax, const NlppKnots& K, int maxp) {
   const qmcx_system_s* s = b->sys;
   const int WN = b->W * s->N;
+  // nl_items: work list [WN*maxp], its length, per-ion counts [M] and cursors [M], pair ions [WN*maxp]
   int* count = b->nl_items + (size_t)WN * maxp;
-  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, b->stream, count);
+  int* icnt = count + 1;
+  int* icur = icnt + s->nions;
+  int* pion = icur + s->nions;
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(256), 0, b->stream, icnt, s->nions);
   hipLaunchKernelGGL(k_nlpp_pairs<T>, dim3((WN + 255) / 256), dim3(256), 0, b->stream, s->params<T>(), b->params<T>(),
-                     rmax, maxp, b->nl_near, b->nl_cnt, b->nl_items, count);
+                     rmax, maxp, b->nl_near, pion, b->nl_cnt, icnt);
+  hipLaunchKernelGGL(k_nlpp_offsets, dim3(1), dim3(64), 0, b->stream, icnt, icur, s->nions, count);
+  hipLaunchKernelGGL(k_nlpp_scatter, dim3((WN + 255) / 256), dim3(256), 0, b->stream, WN, maxp, b->nl_cnt, pion, icur,
+                     b->nl_items);
   constexpr int SL = 16 / PER;
   const int nq = s->npad / (16 / (int)sizeof(T));
   const int nt = std::max(64, (P * SL * nq + 63) / 64 * 64);
   const size_t lds = sizeof(T) * (size_t)P * SL * s->npad + sizeof(double) * 3 * P * 16;
   // workgroups walk the work list (P knots of a pair per unit); its length is only known on the device
-  const int grid = std::max(1, std::min(WN * maxp * (kNlppKnots / P), kNlppWorkgroups));
+  const int grid = (std::max(1, std::min(WN * maxp * (kNlppKnots / P), kNlppWorkgroups)) + 7) / 8 * 8;
   hipLaunchKernelGGL((k_nlpp_ratios<T, P, PER>), dim3(grid), dim3(nt), lds, b->stream, s->params<T>(),
                      b->params<T>(), K, maxp, b->nl_near, b->nl_items, count, b->nl_out);
   HIPCHK(hipGetLastError());
@@ -1321,7 +1328,7 @@ int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* sphere, int ma
     HIPCHK(hipMalloc(&b->nl_out, sizeof(double) * WN * max_pairs * kNlppKnots));
     HIPCHK(hipMalloc(&b->nl_cnt, sizeof(int) * WN));
     HIPCHK(hipMalloc(&b->nl_near, sizeof(double) * 4 * WN * max_pairs));
-    HIPCHK(hipMalloc(&b->nl_items, sizeof(int) * (WN * max_pairs + 1)));
+    HIPCHK(hipMalloc(&b->nl_items, sizeof(int) * (2 * WN * max_pairs + 1 + 2 * (size_t)b->sys->nions)));
     b->nl_cap = max_pairs;
   }
   NlppKnots K;

@@ -1050,26 +1050,27 @@ __global__ __launch_bounds__(1024) void k_virtual_ratios(SysP<T> S, BatchP<T> B,
   if (threadIdx.x == 0) ratios[(size_t)w * nvp + k] = r;
 }
 
-// NonLocalPP::evaluate (NonLocalPP.hpp:115-140) for every electron of every walker, in two
-// launches.  k_nlpp_pairs: one lane per (walker, electron) lists the ions within rmax in ion
-// order (minimum image, ion - electron as the BA table's displacement) into
-// near[(w*N + e)*maxp + p][4] = dist, displacement, counts them in npairs[w][e] (the full count,
-// also beyond maxp) and appends each listed pair to the work list items[] (count[0] entries, in
-// no particular order).  k_nlpp_ratios: workgroups take work-list entries in turn and place the
-// twelve knots at dist * sphere[k] + displ + r_e (NonLocalPP.hpp:133);
-// ratios[w][e][maxp][kNlppKnots].
+// NonLocalPP::evaluate (NonLocalPP.hpp:115-140) for every electron of every walker.
+// k_nlpp_pairs: one lane per (walker, electron) lists the ions within rmax in ion order
+// (minimum image, ion - electron as the BA table's displacement) into
+// near[(w*N + e)*maxp + p][4] = dist, displacement and pion[...] = ion, counts them in
+// npairs[w][e] (the full count, also beyond maxp) and per ion in icnt[ion].  k_nlpp_offsets and
+// k_nlpp_scatter order the listed pairs by ion into the work list items[] (count[0] entries):
+// every walker's knots around one ion then run together and share that ion's stencil rows in
+// L2.  k_nlpp_ratios places the twelve knots at dist * sphere[k] + displ + r_e
+// (NonLocalPP.hpp:133); ratios[w][e][maxp][kNlppKnots].
 constexpr int kNlppKnots = 12;
 struct NlppKnots {
   double p[kNlppKnots][3];
 };
 
-__global__ void k_zero_count(int* count) {
-  if (threadIdx.x == 0) count[0] = 0;
+__global__ void k_zero_ints(int* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_nlpp_pairs(SysP<T> S, BatchP<T> B, double rmax, int maxp, double* near,
-                                                    int* npairs, int* items, int* count) {
+                                                    int* pion, int* npairs, int* icnt) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x, N = S.N, M = S.M;
   if (idx >= B.W * N) return;
   const int w = idx / N, e = idx - w * N;
@@ -1088,12 +1089,32 @@ __global__ __launch_bounds__(256) void k_nlpp_pairs(SysP<T> S, BatchP<T> B, doub
         q[1] = (double)dx;
         q[2] = (double)dy;
         q[3] = (double)dz;
-        items[atomicAdd(count, 1)] = idx * maxp + c;
+        pion[(size_t)idx * maxp + c] = ci;
+        atomicAdd(icnt + ci, 1);
       }
       ++c;
     }
   }
   npairs[idx] = c;
+}
+
+// exclusive prefix of the per-ion counts into the per-ion cursors; count[0] = listed pairs
+__global__ void k_nlpp_offsets(const int* icnt, int* icur, int M, int* count) {
+  if (threadIdx.x != 0) return;
+  int s = 0;
+  for (int c = 0; c < M; ++c) {
+    icur[c] = s;
+    s += icnt[c];
+  }
+  count[0] = s;
+}
+
+__global__ __launch_bounds__(256) void k_nlpp_scatter(int WN, int maxp, const int* npairs, const int* pion, int* icur,
+                                                      int* items) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= WN) return;
+  const int c = npairs[idx] < maxp ? npairs[idx] : maxp;
+  for (int p = 0; p < c; ++p) items[atomicAdd(icur + pion[(size_t)idx * maxp + p], 1)] = idx * maxp + p;
 }
 
 // A work unit is P of a pair's twelve knots.  Lanes [g*L, (g+1)*L), L = (16/PER)*nq, gather
@@ -1114,8 +1135,12 @@ __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<
   const Cell<T> C = *S.cell;
   const int L = SL * (S.npad / V16<T>::n);
   const int g = tid / L;
+  // XCD-aware walk: workgroup b runs on XCD b % 8, and XCD x takes the x-th eighth of the
+  // ion-ordered units, so each XCD's L2 sees a contiguous run of ions (gridDim.x % 8 == 0)
   const int nunits = count[0] * KG;
-  for (int un = blockIdx.x; un < nunits; un += gridDim.x) {
+  const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
+  const int lo = (int)((long long)nunits * xcd / 8), hi = (int)((long long)nunits * (xcd + 1) / 8);
+  for (int un = lo + (int)(blockIdx.x >> 3); un < hi; un += per) {
     const int item = items[un / KG];
     const int k0 = (un % KG) * P;
     const int idx = item / maxp;
