@@ -678,16 +678,18 @@ template <typename T, int P, int PER>
 static int launch_nlpp_p(qmcx_batch_t b, double rmax, const NlppKnots& K, int maxp) {
   const qmcx_system_s* s = b->sys;
   const int WN = b->W * s->N;
-  // nl_items: work list [WN*maxp], its length, per-ion counts [M] and cursors [M], pair ions [WN*maxp]
+  // nl_items: work list [WN*maxp], its length, per-key counts and cursors [M*kNlppRadialBins each],
+  // pair keys [WN*maxp]
+  const int nkeys = s->nions * kNlppRadialBins;
   int* count = b->nl_items + (size_t)WN * maxp;
   int* icnt = count + 1;
-  int* icur = icnt + s->nions;
-  int* pion = icur + s->nions;
-  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(256), 0, b->stream, icnt, s->nions);
+  int* icur = icnt + nkeys;
+  int* pkey = icur + nkeys;
+  hipLaunchKernelGGL(k_zero_ints, dim3(4), dim3(256), 0, b->stream, icnt, nkeys);
   hipLaunchKernelGGL(k_nlpp_pairs<T>, dim3((WN + 255) / 256), dim3(256), 0, b->stream, s->params<T>(), b->params<T>(),
-                     rmax, maxp, b->nl_near, pion, b->nl_cnt, icnt);
-  hipLaunchKernelGGL(k_nlpp_offsets, dim3(1), dim3(64), 0, b->stream, icnt, icur, s->nions, count);
-  hipLaunchKernelGGL(k_nlpp_scatter, dim3((WN + 255) / 256), dim3(256), 0, b->stream, WN, maxp, b->nl_cnt, pion, icur,
+                     rmax, maxp, b->nl_near, pkey, b->nl_cnt, icnt);
+  hipLaunchKernelGGL(k_nlpp_offsets, dim3(1), dim3(64), 0, b->stream, icnt, icur, nkeys, count);
+  hipLaunchKernelGGL(k_nlpp_scatter, dim3((WN + 255) / 256), dim3(256), 0, b->stream, WN, maxp, b->nl_cnt, pkey, icur,
                      b->nl_items);
   constexpr int SL = 16 / PER;
   const int nq = s->npad / (16 / (int)sizeof(T));
@@ -1328,7 +1330,8 @@ int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* sphere, int ma
     HIPCHK(hipMalloc(&b->nl_out, sizeof(double) * WN * max_pairs * kNlppKnots));
     HIPCHK(hipMalloc(&b->nl_cnt, sizeof(int) * WN));
     HIPCHK(hipMalloc(&b->nl_near, sizeof(double) * 4 * WN * max_pairs));
-    HIPCHK(hipMalloc(&b->nl_items, sizeof(int) * (2 * WN * max_pairs + 1 + 2 * (size_t)b->sys->nions)));
+    HIPCHK(hipMalloc(&b->nl_items,
+                     sizeof(int) * (2 * WN * max_pairs + 1 + 2 * (size_t)b->sys->nions * kNlppRadialBins)));
     b->nl_cap = max_pairs;
   }
   NlppKnots K;

@@ -1053,13 +1053,16 @@ __global__ __launch_bounds__(1024) void k_virtual_ratios(SysP<T> S, BatchP<T> B,
 // NonLocalPP::evaluate (NonLocalPP.hpp:115-140) for every electron of every walker.
 // k_nlpp_pairs: one lane per (walker, electron) lists the ions within rmax in ion order
 // (minimum image, ion - electron as the BA table's displacement) into
-// near[(w*N + e)*maxp + p][4] = dist, displacement and pion[...] = ion, counts them in
-// npairs[w][e] (the full count, also beyond maxp) and per ion in icnt[ion].  k_nlpp_offsets and
-// k_nlpp_scatter order the listed pairs by ion into the work list items[] (count[0] entries):
-// every walker's knots around one ion then run together and share that ion's stencil rows in
-// L2.  k_nlpp_ratios places the twelve knots at dist * sphere[k] + displ + r_e
-// (NonLocalPP.hpp:133); ratios[w][e][maxp][kNlppKnots].
+// near[(w*N + e)*maxp + p][4] = dist, displacement and pkey[...] = ion * kNlppRadialBins + the
+// distance's bin in [0, rmax), counts them in npairs[w][e] (the full count, also beyond maxp) and
+// per key in icnt[key].  k_nlpp_offsets and k_nlpp_scatter order the listed pairs by key into the
+// work list items[] (count[0] entries).  k_nlpp_ratios walks knot-major over that list: one
+// sphere serves the whole call, so knot k of every pair around one ion at a similar distance sits
+// at nearly the same point, and consecutive work units reuse its stencil rows in L2.  The knots
+// are placed at dist * sphere[k] + displ + r_e (NonLocalPP.hpp:133);
+// ratios[w][e][maxp][kNlppKnots].
 constexpr int kNlppKnots = 12;
+constexpr int kNlppRadialBins = 16;
 struct NlppKnots {
   double p[kNlppKnots][3];
 };
@@ -1070,7 +1073,7 @@ __global__ void k_zero_ints(int* p, int n) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_nlpp_pairs(SysP<T> S, BatchP<T> B, double rmax, int maxp, double* near,
-                                                    int* pion, int* npairs, int* icnt) {
+                                                    int* pkey, int* npairs, int* icnt) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x, N = S.N, M = S.M;
   if (idx >= B.W * N) return;
   const int w = idx / N, e = idx - w * N;
@@ -1089,8 +1092,10 @@ __global__ __launch_bounds__(256) void k_nlpp_pairs(SysP<T> S, BatchP<T> B, doub
         q[1] = (double)dx;
         q[2] = (double)dy;
         q[3] = (double)dz;
-        pion[(size_t)idx * maxp + c] = ci;
-        atomicAdd(icnt + ci, 1);
+        const int bin = (int)((double)r / rmax * kNlppRadialBins);
+        const int key = ci * kNlppRadialBins + (bin < kNlppRadialBins - 1 ? bin : kNlppRadialBins - 1);
+        pkey[(size_t)idx * maxp + c] = key;
+        atomicAdd(icnt + key, 1);
       }
       ++c;
     }
@@ -1098,23 +1103,23 @@ __global__ __launch_bounds__(256) void k_nlpp_pairs(SysP<T> S, BatchP<T> B, doub
   npairs[idx] = c;
 }
 
-// exclusive prefix of the per-ion counts into the per-ion cursors; count[0] = listed pairs
-__global__ void k_nlpp_offsets(const int* icnt, int* icur, int M, int* count) {
+// exclusive prefix of the per-key counts into the per-key cursors; count[0] = listed pairs
+__global__ void k_nlpp_offsets(const int* icnt, int* icur, int nkeys, int* count) {
   if (threadIdx.x != 0) return;
   int s = 0;
-  for (int c = 0; c < M; ++c) {
+  for (int c = 0; c < nkeys; ++c) {
     icur[c] = s;
     s += icnt[c];
   }
   count[0] = s;
 }
 
-__global__ __launch_bounds__(256) void k_nlpp_scatter(int WN, int maxp, const int* npairs, const int* pion, int* icur,
+__global__ __launch_bounds__(256) void k_nlpp_scatter(int WN, int maxp, const int* npairs, const int* pkey, int* icur,
                                                       int* items) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= WN) return;
   const int c = npairs[idx] < maxp ? npairs[idx] : maxp;
-  for (int p = 0; p < c; ++p) items[atomicAdd(icur + pion[(size_t)idx * maxp + p], 1)] = idx * maxp + p;
+  for (int p = 0; p < c; ++p) items[atomicAdd(icur + pkey[(size_t)idx * maxp + p], 1)] = idx * maxp + p;
 }
 
 // A work unit is P of a pair's twelve knots.  Lanes [g*L, (g+1)*L), L = (16/PER)*nq, gather
@@ -1136,13 +1141,15 @@ __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<
   const int L = SL * (S.npad / V16<T>::n);
   const int g = tid / L;
   // XCD-aware walk: workgroup b runs on XCD b % 8, and XCD x takes the x-th eighth of the
-  // ion-ordered units, so each XCD's L2 sees a contiguous run of ions (gridDim.x % 8 == 0)
-  const int nunits = count[0] * KG;
+  // knot-major units, so each XCD's L2 sees a contiguous run of keys (gridDim.x % 8 == 0)
+  const int nitems = count[0];
+  const int nunits = nitems * KG;
   const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
   const int lo = (int)((long long)nunits * xcd / 8), hi = (int)((long long)nunits * (xcd + 1) / 8);
   for (int un = lo + (int)(blockIdx.x >> 3); un < hi; un += per) {
-    const int item = items[un / KG];
-    const int k0 = (un % KG) * P;
+    const int kg = un / nitems;
+    const int item = items[un - kg * nitems];
+    const int k0 = kg * P;
     const int idx = item / maxp;
     const int w = idx / N, e = idx - w * N;
     const double* R = B.R + (size_t)w * 3 * N;
