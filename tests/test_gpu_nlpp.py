@@ -169,3 +169,40 @@ def test_nlpp_device_evaluate_overflow_is_an_error():
         wb.nlpp_evaluate(5.0, nlpp.quadrature_knots(), max_pairs=1)
     with pytest.raises(qmcx.QmcxError):
         wb.nlpp_evaluate(1.7, nlpp.quadrature_knots(), max_pairs=0)
+
+
+@pytest.mark.parametrize("prec,tol", [(qmcx.FP64, 1e-10), (qmcx.FP32, 2e-3)])
+def test_nlpp_skewed_cell_vs_oracle(prec, tol):
+    """A non-orthogonal cell (the 'tiny' system): distances take the general eight-corner search."""
+    sysd = systems.make_system("tiny")
+    L = sysd.lattice
+    vol = abs(np.linalg.det(L))
+    width = min(vol / np.linalg.norm(np.cross(L[(i + 1) % 3], L[(i + 2) % 3])) for i in range(3))
+    rmax = 0.4 * width
+    _say("nlpp skewed", prec, f"rmax {rmax:.3f}")
+    nw = 2
+    dev = qmcx.System(sysd)
+    wb = qmcx.WalkerBatch(dev, nw, delay_rank=1, precision=prec)
+    wb.set_positions(systems.initial_positions(sysd, nw, seed=4))
+    wb.evaluate_log()
+    wb.run(1, 3, 0.4, qmcx.ACCEPT_METROPOLIS)
+    orc = po.OracleSystem(sysd)
+    pp = nlpp.NonLocalPP(sysd, rmax=rmax, seed=6)
+    worst, count = _check_all(sysd, orc, wb, wb.positions(), pp, tol, range(sysd.nel))
+    assert count > 0
+    sphere = pp.randomize()
+    rat, cnt = wb.nlpp_evaluate(rmax, sphere, max_pairs=8)
+    R = wb.positions()
+    checked = 0
+    for w in range(nw):
+        for jel in range(sysd.nel):
+            exp_v, border = _expected_pairs(sysd, R[w], jel, rmax, sphere)
+            if border:
+                continue
+            assert cnt[w, jel] == len(exp_v)
+            for p, v in enumerate(exp_v):
+                ref = orc.evaluate_ratios(R[w], jel, v)
+                assert (np.abs(rat[w, jel, p] - ref) / np.maximum(np.abs(ref), 1.0)).max() < tol
+                checked += 1
+    _say(f"  {count} flex ratios (worst {worst:.2e}), {checked} pairs")
+    assert checked > 0
