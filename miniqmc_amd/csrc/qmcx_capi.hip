@@ -688,7 +688,7 @@ static int launch_nlpp_p(qmcx_batch_t b, double rmax, const NlppKnots& K, int ma
   hipLaunchKernelGGL(k_zero_ints, dim3(4), dim3(256), 0, b->stream, icnt, nkeys);
   hipLaunchKernelGGL(k_nlpp_pairs<T>, dim3((WN + 255) / 256), dim3(256), 0, b->stream, s->params<T>(), b->params<T>(),
                      rmax, maxp, b->nl_near, pkey, b->nl_cnt, icnt);
-  hipLaunchKernelGGL(k_nlpp_offsets, dim3(1), dim3(64), 0, b->stream, icnt, icur, nkeys, count);
+  hipLaunchKernelGGL(k_nlpp_offsets, dim3(1), dim3(256), 0, b->stream, icnt, icur, nkeys, count);
   hipLaunchKernelGGL(k_nlpp_scatter, dim3((WN + 255) / 256), dim3(256), 0, b->stream, WN, maxp, b->nl_cnt, pkey, icur,
                      b->nl_items);
   constexpr int SL = 16 / PER;

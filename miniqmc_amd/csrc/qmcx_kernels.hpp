@@ -1103,15 +1103,32 @@ __global__ __launch_bounds__(256) void k_nlpp_pairs(SysP<T> S, BatchP<T> B, doub
   npairs[idx] = c;
 }
 
-// exclusive prefix of the per-key counts into the per-key cursors; count[0] = listed pairs
-__global__ void k_nlpp_offsets(const int* icnt, int* icur, int nkeys, int* count) {
-  if (threadIdx.x != 0) return;
+// exclusive prefix of the per-key counts into the per-key cursors; count[0] = listed pairs.
+// One workgroup of 256 lanes: each lane sums a contiguous run of keys, lane 0 scans the 256 run
+// sums in LDS, and each lane writes its run's cursors.
+__global__ __launch_bounds__(256) void k_nlpp_offsets(const int* icnt, int* icur, int nkeys, int* count) {
+  __shared__ int run[256];
+  const int t = threadIdx.x, per = (nkeys + 255) / 256, a = t * per;
+  const int b = a + per < nkeys ? a + per : nkeys;
   int s = 0;
-  for (int c = 0; c < nkeys; ++c) {
-    icur[c] = s;
-    s += icnt[c];
+  for (int c = a; c < b; ++c) s += icnt[c];
+  run[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int i = 0; i < 256; ++i) {
+      const int v = run[i];
+      run[i] = acc;
+      acc += v;
+    }
+    count[0] = acc;
   }
-  count[0] = s;
+  __syncthreads();
+  int o = run[t];
+  for (int c = a; c < b; ++c) {
+    icur[c] = o;
+    o += icnt[c];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_nlpp_scatter(int WN, int maxp, const int* npairs, const int* pkey, int* icur,
