@@ -208,13 +208,19 @@ def measure_nlpp(dev, sysd, R0, prec, steps, seed, tau, rmax=1.7):
     # electrons' positions and the J1/J2 state read once
     bpr = 64 * sysd.npad * s + sysd.nup * s + 3 * 8 * sysd.nel
     gbs = bpr * ratios / dt / 1e9
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "hbm_traffic_k_nlpp_ratios_nio64.json")
+    if sysd.name == "nio64" and R0.shape[0] == 128 and os.path.exists(tp):
+        traffic = json.load(open(tp))["f32" if prec == qmcx.FP32 else "f64"]["bytes_per_launch"]
     return {"value": ratios / dt, "unit": "virtual-move ratios/s", "ms_per_evaluate": 1e3 * dt, "steps": steps,
             "dtype": "f32" if prec == qmcx.FP32 else "f64", "rmax": rmax, "electron_ion_pairs": int(cnt.sum()),
             "ratios_per_evaluate": ratios,
             "roofline": {"bound": "hbm", "kernel": "k_nlpp_ratios", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                          "achieved": round(gbs, 2), "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_ratio": bpr,
-                         "note": "algorithmic bytes; the twelve knots of a pair share most spline stencils in L2, "
-                                 "so this can exceed the HBM peak"}}
+                         "traffic": traffic, "traffic_source": "profiles/hbm_traffic_k_nlpp_ratios_nio64.json",
+                         "note": "algorithmic bytes over the wall clock of an evaluation; knot k of the pairs around "
+                                 "one ion at similar distances shares stencil rows in L2, so HBM traffic is below "
+                                 "the algorithmic bytes"}}
 
 
 def measure_evaluate_log(dev, sysd, R0, prec, steps):
