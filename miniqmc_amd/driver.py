@@ -1,0 +1,164 @@
+"""miniqmc-style driver over one walker batch (src/Drivers/miniqmc.cpp).
+
+    python -m miniqmc_amd.driver -g "1 1 2" -w 128 -n 5 -k 32 [-x 1.7] [-s 11] [--fp64]
+
+Per step, as the reference's main loop (miniqmc.cpp:406-495): one VMC sweep of every electron of
+every walker ("Diffusion": evaluateLog's refresh, the moves, completeUpdates, evaluateGL), then
+the non-local pseudopotential quadrature over every electron-ion pair inside Rmax ("Pseudopotential").
+At the end it prints a flat timer table and the reference's three throughput lines
+(miniqmc.cpp:509-523), and writes info_<na>_<nb>_<nc>.xml with the same elements
+(resources/hardware, timing, particles, run/driver; miniqmc.cpp:525-551).
+
+Differences from the reference, by design:
+  * every walker runs in one batch on the GPU, so there is one timer per stage instead of one per
+    walker thread, and the device stages come from HIP events (move, flush, recompute, evaluateGL,
+    proposals);
+  * acceptance: the reference's rule u < r with r = 0.5 (``-r`` only accepts 0.5), or Metropolis;
+  * one random sphere per pseudopotential evaluation for the whole batch;
+  * systems: the tilings with a prepared workload, -g "1 1 1" (NiO-32) and -g "1 1 2" (NiO-64).
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+import xml.etree.ElementTree as ET
+
+from miniqmc_amd import nlpp, qmcx, systems
+
+TILINGS = {(1, 1, 1): "nio32", (1, 1, 2): "nio64"}
+
+
+def parse_args(argv=None):
+    ap = argparse.ArgumentParser(prog="miniqmc_amd.driver", description=__doc__.splitlines()[0])
+    ap.add_argument("-g", default="1 1 1", help='3D tiling, "na nb nc"')
+    ap.add_argument("-w", type=int, default=16, help="number of walkers")
+    ap.add_argument("-n", type=int, default=5, help="number of MC steps")
+    ap.add_argument("-k", type=int, default=32, help="delayed update rank")
+    ap.add_argument("-x", type=float, default=1.7, help="Rmax of the pseudopotential")
+    ap.add_argument("-r", type=float, default=0.5, help="acceptance threshold of the reference rule (0.5 only)")
+    ap.add_argument("-s", type=int, default=11, help="random seed")
+    ap.add_argument("--tau", type=float, default=0.3)
+    ap.add_argument("--metropolis", action="store_true", help="Metropolis acceptance instead of u < r")
+    ap.add_argument("--fp64", action="store_true", help="FP64 batch (default FP32 with the FP64 refresh)")
+    ap.add_argument("--system", default=None, help="a named workload instead of -g (e.g. nio8-cubic)")
+    ap.add_argument("--info-dir", default=".", help="where info_<na>_<nb>_<nc>.xml is written")
+    a = ap.parse_args(argv)
+    try:
+        a.tiling = tuple(int(t) for t in a.g.split())
+    except ValueError:
+        ap.error(f"-g takes three integers, got {a.g!r}")
+    if len(a.tiling) != 3:
+        ap.error(f"-g takes three integers, got {a.g!r}")
+    if a.system is None:
+        if a.tiling not in TILINGS:
+            ap.error(f"tiling {a.tiling} has no prepared workload; available: {sorted(TILINGS)}")
+        a.system = TILINGS[a.tiling]
+    if a.r != 0.5:
+        ap.error("the reference acceptance rule runs with r = 0.5 only")
+    if a.w < 1 or a.n < 1 or a.k < 1:
+        ap.error("-w, -n and -k must be positive")
+    return a
+
+
+def timer_table(timers: dict) -> str:
+    """Flat timer table: name, total seconds, calls, seconds per call (TimerManager::print_flat,
+    Utilities/NewTimer.cpp)."""
+    lines = [f"{'Timer':<40s}  {'Inclusive_time':>14s}  {'Calls':>8s}  {'Time_per_call':>14s}"]
+    for name in sorted(timers):
+        t, n = timers[name]
+        lines.append(f"{name:<40s}  {t:14.4f}  {n:8d}  {t / max(n, 1):14.9f}")
+    return "\n".join(lines)
+
+
+def throughput(nw: int, nel: int, timers: dict) -> dict:
+    """The reference's three throughput figures (miniqmc.cpp:514-522)."""
+    out = {}
+    for key, name, power in (("total", "Total", 3), ("diffusion", "Diffusion", 3), ("pseudopotential", "Pseudopotential", 2)):
+        t = timers.get(name, (0.0, 0))[0]
+        out[key] = nw * float(nel) ** power / t if t > 0 else float("nan")
+    return out
+
+
+def info_xml(path: str, nel: int, nsteps: int, timers: dict) -> None:
+    """info_<na>_<nb>_<nc>.xml with the reference's element layout (miniqmc.cpp:525-551)."""
+    res = ET.Element("resources")
+    ET.SubElement(res, "hardware")
+    timing = ET.SubElement(res, "timing")
+    for name in sorted(timers):
+        t, n = timers[name]
+        tm = ET.SubElement(timing, "timer")
+        ET.SubElement(tm, "name").text = name
+        ET.SubElement(tm, "time_incl").text = f"{t:.6f}"
+        ET.SubElement(tm, "calls").text = str(n)
+    parts = ET.SubElement(res, "particles")
+    p = ET.SubElement(parts, "particle")
+    ET.SubElement(p, "name").text = "e"
+    ET.SubElement(p, "size").text = str(nel)
+    run = ET.SubElement(res, "run")
+    drv = ET.SubElement(run, "driver")
+    ET.SubElement(drv, "name").text = "miniqmc"
+    ET.SubElement(drv, "steps").text = str(nsteps)
+    ET.SubElement(drv, "substeps").text = "1"
+    ET.ElementTree(res).write(path, xml_declaration=True, encoding="utf-8")
+
+
+def main(argv=None) -> int:
+    a = parse_args(argv)
+    sysd = systems.make_system(a.system, seed=a.s)
+    prec = qmcx.FP64 if a.fp64 else qmcx.FP32
+    dev = qmcx.System(sysd, precisions=(prec,))
+    wb = qmcx.WalkerBatch(dev, a.w, delay_rank=a.k, precision=prec)
+    wb.set_positions(systems.initial_positions(sysd, a.w, seed=a.s))
+    mode = qmcx.ACCEPT_METROPOLIS if a.metropolis else qmcx.ACCEPT_REFERENCE
+    pp = nlpp.NonLocalPP(sysd, rmax=a.x, seed=a.s)
+    print(f"miniqmc_amd: {sysd.name}, {sysd.nel} electrons, {sysd.norb} orbitals per determinant, {a.w} walkers, "
+          f"{a.n} steps, delay rank {a.k}, {'FP64' if a.fp64 else 'FP32'}, Rmax {a.x}", flush=True)
+
+    timers = {}
+
+    def add(name, t, n=1):
+        tt, nn = timers.get(name, (0.0, 0))
+        timers[name] = (tt + t, nn + n)
+
+    t0 = time.perf_counter()
+    t = time.perf_counter()
+    wb.evaluate_log()
+    add("Setup", time.perf_counter() - t)
+    wb.set_kernel_timing(True)
+    pairs = 0
+    for step in range(a.n):
+        t = time.perf_counter()
+        wb.run(1, a.s, a.tau, mode, recompute_every=0 if a.fp64 else 1)
+        add("Diffusion", time.perf_counter() - t)
+        t = time.perf_counter()
+        # the reference discards the ratios (NonLocalPP.hpp:115-140); they stay on the device
+        _, cnt = wb.nlpp_evaluate(a.x, pp.randomize(), max_pairs=8, ratios=False)
+        add("Pseudopotential", time.perf_counter() - t)
+        pairs += int(cnt.sum())
+    add("Total", time.perf_counter() - t0)
+    for name, (ms, n) in wb.kernel_timing().items():
+        if n:
+            add(f"Diffusion::{name} (device)", ms * 1e-3, int(n))
+
+    print("================================== ")
+    print(timer_table(timers))
+    print()
+    print("========== Throughput ============ ")
+    print()
+    tp = throughput(a.w, sysd.nel, timers)
+    print(f"Total throughput ( N_walkers * N_elec^3 / Total time ) = {tp['total']:.6e}")
+    print(f"Diffusion throughput ( N_walkers * N_elec^3 / Diffusion time ) = {tp['diffusion']:.6e}")
+    print(f"Pseudopotential throughput ( N_walkers * N_elec^2 / Pseudopotential time ) = {tp['pseudopotential']:.6e}")
+    print(f"walker-moves/s {a.w * sysd.nel * a.n / timers['Diffusion'][0]:.4e}; "
+          f"acceptance {wb.accepts().sum() / (a.w * sysd.nel * a.n):.3f}; "
+          f"{pairs // a.n} electron-ion pairs per step")
+    na, nb, nc = a.tiling
+    path = f"{a.info_dir}/info_{na}_{nb}_{nc}.xml"
+    info_xml(path, sysd.nel, a.n, timers)
+    print(f"wrote {path}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

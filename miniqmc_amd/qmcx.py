@@ -324,17 +324,18 @@ class WalkerBatch:
         _check(load().qmcx_flex_evaluate_ratios(self.h, iel, vpos.shape[1], vpos.reshape(-1), out.ctypes.data))
         return out
 
-    def nlpp_evaluate(self, rmax, sphere, max_pairs=4, copy_out=True):
+    def nlpp_evaluate(self, rmax, sphere, max_pairs=4, copy_out=True, ratios=True):
         """NonLocalPP::evaluate for every electron of every walker on the GPU: ratios
-        [nw][N][max_pairs][12] and npairs [nw][N] (None, None without copy_out)."""
+        [nw][N][max_pairs][12] and npairs [nw][N] (None, None without copy_out; None, npairs
+        without ratios)."""
         sphere = _c(sphere).reshape(12, 3)
         if not copy_out:
             _check(load().qmcx_nlpp_evaluate(self.h, float(rmax), sphere.reshape(-1), max_pairs, None, None))
             return None, None
-        out = np.zeros((self.nw, self.N, max_pairs, 12))
+        out = np.zeros((self.nw, self.N, max_pairs, 12)) if ratios else None
         cnt = np.zeros((self.nw, self.N), dtype=np.int32)
-        _check(load().qmcx_nlpp_evaluate(self.h, float(rmax), sphere.reshape(-1), max_pairs, out.ctypes.data,
-                                         cnt.ctypes.data))
+        _check(load().qmcx_nlpp_evaluate(self.h, float(rmax), sphere.reshape(-1), max_pairs,
+                                         out.ctypes.data if ratios else None, cnt.ctypes.data))
         return out, cnt
 
     def set_kernel_timing(self, on=True):

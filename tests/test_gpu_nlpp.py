@@ -206,3 +206,13 @@ def test_nlpp_skewed_cell_vs_oracle(prec, tol):
                 checked += 1
     _say(f"  {count} flex ratios (worst {worst:.2e}), {checked} pairs")
     assert checked > 0
+
+
+@pytest.mark.parametrize("extra", [["-k", "4"], ["-k", "1", "--fp64", "--metropolis"]])
+def test_driver_runs(tmp_path, extra):
+    """The miniqmc-style driver end to end on a small system: sweeps, NonLocalPP, report, info XML."""
+    from miniqmc_amd import driver
+    _say("driver", extra)
+    rc = driver.main(["--system", "nio8-cubic", "-w", "2", "-n", "2", "--info-dir", str(tmp_path), *extra])
+    assert rc == 0
+    assert (tmp_path / "info_1_1_1.xml").exists()
