@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1316,6 +1317,8 @@ int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* sphere, int ma
                        int* npairs) {
   if (!b || !sphere || !(rmax > 0) || max_pairs < 1 || max_pairs > kMaxNlppPairs) return fail(QMCX_ERR_ARG, "bad argument");
   const size_t WN = (size_t)b->W * b->sys->N;
+  // work-list entries and knot units are int indices on the device
+  if (WN * max_pairs * kNlppKnots >= (size_t)INT_MAX) return fail(QMCX_ERR_ARG, "walkers x electrons x max_pairs too large");
   if (max_pairs > b->nl_cap) {
     HIPCHK(hipStreamSynchronize(b->stream));
     if (b->nl_out) hipFree(b->nl_out);
