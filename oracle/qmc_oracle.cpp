@@ -25,8 +25,12 @@
 //   * DiracDeterminant                        QMCWaveFunctions/DiracDeterminant.cpp:80-280
 //   * WaveFunction                            QMCWaveFunctions/WaveFunction.cpp:178-279
 //   * VMC sweep                               Drivers/miniqmc.cpp:422-467
+//   * WaveFunction::evaluateRatios (virtual moves of NonLocalPP)
+//                                             QMCWaveFunctions/WaveFunction.cpp:281-297,
+//                                             DiracDeterminant.cpp:216-224, SPOSet.h:72-83,
+//                                             TwoBodyJastrow.h:103-107, OneBodyJastrow.h:136-140
 //
-// Every function here is pinned by tests/test_oracle_*.py against the
+// Every function here is pinned by tests/test_oracle.py against the
 // reference's own KATs (test_dirac_matrix.cpp, test_dirac_det.cpp,
 // test_particle.cpp) or against closed forms / brute force; see the header of
 // each test for which check pins which function.
