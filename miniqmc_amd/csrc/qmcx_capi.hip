@@ -31,6 +31,20 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(QMCX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Device buffer freed when it goes out of scope, so an early return on a HIP error leaks nothing
+struct DevBuf {
+  void* p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes > 0 ? bytes : 1); }
+  template <typename P>
+  P* as() const { return (P*)p; }
+};
+
 // ------------------------------------------------------------------ lattice (host)
 void inv3(const double a[9], double o[9]) {
   const double c00 = a[4] * a[8] - a[5] * a[7];
@@ -165,6 +179,7 @@ struct qmcx_system_s {
   Cell<float>* d_cell_f = nullptr;
   Cell<double>* d_cell_d = nullptr;
   double* d_spoG = nullptr;
+  hipStream_t stream = nullptr;   // the stand-alone kernels (qmcx_spline_vgh, qmcx_distance_rows)
 
   template <typename T>
   SysP<T> params() const {
@@ -765,6 +780,7 @@ int qmcx_system_create(qmcx_system_t* out, const double* lattice9, int nions, co
     HIPCHK(hipMemcpy(s->d_cell_d, &cd, sizeof(cd), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&s->d_spoG, sizeof(double) * 9));
   }
+  HIPCHK(hipStreamCreate(&s->stream));
   *out = s;
   return 0;
 }
@@ -780,6 +796,7 @@ int qmcx_system_destroy(qmcx_system_t s) {
   hipFree(s->d_cell_f);
   hipFree(s->d_cell_d);
   hipFree(s->d_spoG);
+  if (s->stream) hipStreamDestroy(s->stream);
   delete s;
   return 0;
 }
@@ -847,79 +864,75 @@ int qmcx_spline_vgh(qmcx_system_t s, int precision, int npts, const double* r, d
   if (!s || !r || !out || npts <= 0) return fail(QMCX_ERR_ARG, "bad argument");
   const bool f32 = precision == QMCX_FP32;
   if ((f32 && !s->d_coef_f) || (!f32 && !s->d_coef_d)) return fail(QMCX_ERR_STATE, "no spline table of that precision");
-  double *dr = nullptr, *dout = nullptr;
-  HIPCHK(hipMalloc(&dr, sizeof(double) * 3 * npts));
-  HIPCHK(hipMalloc(&dout, sizeof(double) * 10 * (size_t)npts * s->norb));
-  HIPCHK(hipMemcpy(dr, r, sizeof(double) * 3 * npts, hipMemcpyHostToDevice));
+  const size_t nout = 10 * (size_t)npts * s->norb;
+  DevBuf dr, dout;
+  HIPCHK(dr.alloc(sizeof(double) * 3 * npts));
+  HIPCHK(dout.alloc(sizeof(double) * nout));
+  HIPCHK(hipMemcpyAsync(dr.p, r, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, s->stream));
   if (f32) {
     const int slices = spline_slices_for<float>(s, 1024);
     const int nt = std::max(64, (s->npad / 4 * slices + 63) / 64 * 64);
-    hipLaunchKernelGGL(k_spline_vgh<float>, dim3(npts), dim3(nt), sizeof(float) * slices * 10 * s->npad, 0,
-                       s->params<float>(), dr, dout, s->norb, slices);
+    hipLaunchKernelGGL(k_spline_vgh<float>, dim3(npts), dim3(nt), sizeof(float) * slices * 10 * s->npad, s->stream,
+                       s->params<float>(), dr.as<double>(), dout.as<double>(), s->norb, slices);
   } else {
     const int slices = spline_slices_for<double>(s, 1024);
     const int nt = std::max(64, (s->npad / 2 * slices + 63) / 64 * 64);
-    hipLaunchKernelGGL(k_spline_vgh<double>, dim3(npts), dim3(nt), sizeof(double) * slices * 10 * s->npad, 0,
-                       s->params<double>(), dr, dout, s->norb, slices);
+    hipLaunchKernelGGL(k_spline_vgh<double>, dim3(npts), dim3(nt), sizeof(double) * slices * 10 * s->npad, s->stream,
+                       s->params<double>(), dr.as<double>(), dout.as<double>(), s->norb, slices);
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpy(out, dout, sizeof(double) * 10 * (size_t)npts * s->norb, hipMemcpyDeviceToHost));
-  hipFree(dr);
-  hipFree(dout);
+  HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(double) * nout, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
 }
 
 int qmcx_distance_rows(qmcx_system_t s, int precision, int npts, const double* src, const int* flip_ind, int nt,
                        const double* targets, double* r, double* dr) {
   if (!s || !src || !flip_ind || !targets || !r || !dr || npts <= 0 || nt <= 0) return fail(QMCX_ERR_ARG, "bad argument");
-  double *d_src, *d_tgt, *d_r, *d_dr;
-  int* d_flip;
   const size_t np = (size_t)npts * nt;
-  HIPCHK(hipMalloc(&d_src, sizeof(double) * 3 * npts));
-  HIPCHK(hipMalloc(&d_tgt, sizeof(double) * 3 * nt));
-  HIPCHK(hipMalloc(&d_r, sizeof(double) * np));
-  HIPCHK(hipMalloc(&d_dr, sizeof(double) * 3 * np));
-  HIPCHK(hipMalloc(&d_flip, sizeof(int) * npts));
-  HIPCHK(hipMemcpy(d_src, src, sizeof(double) * 3 * npts, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(d_tgt, targets, sizeof(double) * 3 * nt, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(d_flip, flip_ind, sizeof(int) * npts, hipMemcpyHostToDevice));
+  if ((np + 255) / 256 > (size_t)INT_MAX) return fail(QMCX_ERR_ARG, "npts x nt too large");
+  DevBuf d_src, d_tgt, d_r, d_dr, d_flip;
+  HIPCHK(d_src.alloc(sizeof(double) * 3 * npts));
+  HIPCHK(d_tgt.alloc(sizeof(double) * 3 * nt));
+  HIPCHK(d_r.alloc(sizeof(double) * np));
+  HIPCHK(d_dr.alloc(sizeof(double) * 3 * np));
+  HIPCHK(d_flip.alloc(sizeof(int) * npts));
+  HIPCHK(hipMemcpyAsync(d_src.p, src, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(d_tgt.p, targets, sizeof(double) * 3 * nt, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(d_flip.p, flip_ind, sizeof(int) * npts, hipMemcpyHostToDevice, s->stream));
   const int blocks = (int)((np + 255) / 256);
   if (precision == QMCX_FP32)
-    hipLaunchKernelGGL(k_distance_rows<float>, dim3(blocks), dim3(256), 0, 0, make_cell<float>(s->lattice), npts, d_src,
-                       d_flip, nt, d_tgt, d_r, d_dr);
+    hipLaunchKernelGGL(k_distance_rows<float>, dim3(blocks), dim3(256), 0, s->stream, make_cell<float>(s->lattice),
+                       npts, d_src.as<double>(), d_flip.as<int>(), nt, d_tgt.as<double>(), d_r.as<double>(),
+                       d_dr.as<double>());
   else
-    hipLaunchKernelGGL(k_distance_rows<double>, dim3(blocks), dim3(256), 0, 0, make_cell<double>(s->lattice), npts,
-                       d_src, d_flip, nt, d_tgt, d_r, d_dr);
+    hipLaunchKernelGGL(k_distance_rows<double>, dim3(blocks), dim3(256), 0, s->stream, make_cell<double>(s->lattice),
+                       npts, d_src.as<double>(), d_flip.as<int>(), nt, d_tgt.as<double>(), d_r.as<double>(),
+                       d_dr.as<double>());
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpy(r, d_r, sizeof(double) * np, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(dr, d_dr, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
-  hipFree(d_src);
-  hipFree(d_tgt);
-  hipFree(d_r);
-  hipFree(d_dr);
-  hipFree(d_flip);
+  HIPCHK(hipMemcpyAsync(r, d_r.p, sizeof(double) * np, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(dr, d_dr.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
 }
 
 int qmcx_invert_transpose(int n, int nmat, const double* a, double* psiM, double* logdet, double* phase) {
   if (n <= 0 || n > 1024 || nmat <= 0 || !a || !psiM || !logdet || !phase) return fail(QMCX_ERR_ARG, "bad argument");
   const size_t nn = (size_t)n * n * nmat;
-  double *d_a, *d_p, *d_ws, *d_ld, *d_ph;
-  HIPCHK(hipMalloc(&d_a, sizeof(double) * nn));
-  HIPCHK(hipMalloc(&d_p, sizeof(double) * nn));
-  HIPCHK(hipMalloc(&d_ws, sizeof(double) * 2 * nn));
-  HIPCHK(hipMalloc(&d_ld, sizeof(double) * nmat));
-  HIPCHK(hipMalloc(&d_ph, sizeof(double) * nmat));
-  HIPCHK(hipMemcpy(d_a, a, sizeof(double) * nn, hipMemcpyHostToDevice));
-  HIPCHK((launch_gj_invert<double, double>(n, nmat, 0, d_a, d_p, d_ws, d_ld, d_ph, 1)));
-  HIPCHK(hipMemcpy(psiM, d_p, sizeof(double) * nn, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(logdet, d_ld, sizeof(double) * nmat, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(phase, d_ph, sizeof(double) * nmat, hipMemcpyDeviceToHost));
-  hipFree(d_a);
-  hipFree(d_p);
-  hipFree(d_ws);
-  hipFree(d_ld);
-  hipFree(d_ph);
+  DevBuf d_a, d_p, d_ws, d_ld, d_ph;
+  HIPCHK(d_a.alloc(sizeof(double) * nn));
+  HIPCHK(d_p.alloc(sizeof(double) * nn));
+  HIPCHK(d_ws.alloc(sizeof(double) * 2 * nn));
+  HIPCHK(d_ld.alloc(sizeof(double) * nmat));
+  HIPCHK(d_ph.alloc(sizeof(double) * nmat));
+  hipStream_t st = hipStreamPerThread;
+  HIPCHK(hipMemcpyAsync(d_a.p, a, sizeof(double) * nn, hipMemcpyHostToDevice, st));
+  HIPCHK((launch_gj_invert<double, double>(n, nmat, st, d_a.as<double>(), d_p.as<double>(), d_ws.as<double>(),
+                                          d_ld.as<double>(), d_ph.as<double>(), 1)));
+  HIPCHK(hipMemcpyAsync(psiM, d_p.p, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(logdet, d_ld.p, sizeof(double) * nmat, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(phase, d_ph.p, sizeof(double) * nmat, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   return 0;
 }
 
@@ -1250,15 +1263,17 @@ int qmcx_flex_ratio_grad(qmcx_batch_t b, int iel, const double* displ, double* r
   HIPCHK(hipMemcpyAsync(b->flex_displ, displ, sizeof(double) * 3 * b->W, hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipMemcpyAsync(b->flex_decide, none.data(), sizeof(int) * b->W, hipMemcpyHostToDevice, b->stream));
   if (int rc = flex_launch(b, iel, false)) return rc;
-  std::vector<double> rec(9);
+  // the electron's [W] slice of the trace ([W][N][9]) in one strided copy
+  std::vector<double> rec(9 * (size_t)b->W);
+  HIPCHK(hipMemcpy2DAsync(rec.data(), sizeof(double) * 9, b->trace + (size_t)iel * 9, sizeof(double) * 9 * b->sys->N,
+                          sizeof(double) * 9, b->W, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   for (int w = 0; w < b->W; ++w) {
-    HIPCHK(hipMemcpy(rec.data(), b->trace + ((size_t)w * b->sys->N + iel) * 9, sizeof(double) * 9,
-                     hipMemcpyDeviceToHost));
-    if (ratio) ratio[w] = rec[0];
+    const double* q = rec.data() + 9 * (size_t)w;
+    if (ratio) ratio[w] = q[0];
     for (int d = 0; d < 3; ++d) {
-      if (grad_now) grad_now[3 * w + d] = rec[3 + d];
-      if (grad_new) grad_new[3 * w + d] = rec[6 + d];
+      if (grad_now) grad_now[3 * w + d] = q[3 + d];
+      if (grad_new) grad_new[3 * w + d] = q[6 + d];
     }
   }
   b->flex_iel = iel;

@@ -1242,9 +1242,9 @@ __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<
 template <typename T>
 __global__ void k_distance_rows(Cell<T> C, int npts, const double* src, const int* flip_ind, int nt,
                                 const double* tgt, double* r, double* dr) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= npts * nt) return;
-  const int p = idx / nt, t = idx - p * nt;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)npts * nt) return;
+  const int p = (int)(idx / nt), t = (int)(idx - (size_t)p * nt);
   const T flip = t < flip_ind[p] ? T(1) : T(-1);
   T dx, dy, dz;
   const T rr = min_image(C, (T)(tgt[3 * t] - src[3 * p]), (T)(tgt[3 * t + 1] - src[3 * p + 1]),

@@ -65,6 +65,7 @@ def lib():
         L.orc_evaluate_log.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p]
         L.orc_evaluate_log.restype = C.c_double
         L.orc_evaluate_ratios.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, _dp, _dp]
+        L.orc_evaluate_log_parts.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -194,6 +195,14 @@ class OracleSystem:
         G, L = np.zeros((N, 3)), np.zeros(N)
         v = lib().orc_evaluate_log(self.h, _c(R).reshape(-1), G.ctypes.data, L.ctypes.data)
         return v, G, L
+
+    def evaluate_log_parts(self, R):
+        """evaluateLog per component: logs[4], G[4][N][3], L[4][N] for the up and down
+        determinants (SPO-lattice fractional gradients, h_xx), J1 and J2 (Cartesian)."""
+        N = self.sys.nel
+        logs, G, L = np.zeros(4), np.zeros((4, N, 3)), np.zeros((4, N))
+        lib().orc_evaluate_log_parts(self.h, _c(R).reshape(-1), logs, G.reshape(-1), L.reshape(-1))
+        return logs, G, L
 
     def evaluate_ratios(self, R, iel, vpos):
         """WaveFunction::evaluateRatios of electron iel at vpos[nvp][3] for configuration R[N][3]."""

@@ -1316,6 +1316,49 @@ void orc_evaluate_ratios(void* h, const double* R, int iel, int nvp, const doubl
   }
 }
 
+// evaluateLog split by component (the WaveFunctionComponent list of WaveFunction.cpp:178-195):
+// part 0 = up determinant, 1 = down determinant, 2 = J1, 3 = J2.  logs[4]; G[4][N][3]; L[4][N].
+// The determinant parts carry the reference's SPO convention (gradients in the SPO lattice's
+// fractional coordinates, d2psi = h_xx); the Jastrow parts are Cartesian.
+void orc_evaluate_log_parts(void* h, const double* R, double* logs, double* G, double* L) {
+  const System& s = ((OrcSystem*)h)->sys;
+  Walker w;
+  walker_init(w, s, R, 1);
+  evaluate_log(w);
+  const int N = w.N;
+  std::fill(G, G + 4 * 3 * (size_t)N, 0.0);
+  std::fill(L, L + 4 * (size_t)N, 0.0);
+  for (int d = 0; d < 2; ++d) {
+    const Determinant& D = w.det[d];
+    const int n = D.n;
+    logs[d] = D.log_value;
+    for (int i = 0; i < n; ++i) {
+      const int iat = D.first + i;
+      double gx = 0, gy = 0, gz = 0, lap = 0;
+      for (int j = 0; j < n; ++j) {
+        const double a = D.psiM[(size_t)i * n + j];
+        gx += a * D.dpsiM[3 * ((size_t)i * n + j) + 0];
+        gy += a * D.dpsiM[3 * ((size_t)i * n + j) + 1];
+        gz += a * D.dpsiM[3 * ((size_t)i * n + j) + 2];
+        lap += a * D.d2psiM[(size_t)i * n + j];
+      }
+      double* g = G + ((size_t)d * N + iat) * 3;
+      g[0] = gx; g[1] = gy; g[2] = gz;
+      L[(size_t)d * N + iat] = lap - (gx * gx + gy * gy + gz * gz);
+    }
+  }
+  logs[2] = w.j1_log;
+  logs[3] = w.j2_log;
+  for (int i = 0; i < N; ++i) {
+    double* g1 = G + ((size_t)2 * N + i) * 3;
+    double* g2 = G + ((size_t)3 * N + i) * 3;
+    g1[0] = w.Gx[i]; g1[1] = w.Gy[i]; g1[2] = w.Gz[i];
+    g2[0] = w.dUx[i]; g2[1] = w.dUy[i]; g2[2] = w.dUz[i];
+    L[(size_t)2 * N + i] = -w.Lap[i];
+    L[(size_t)3 * N + i] = w.d2Uat[i];
+  }
+}
+
 double orc_evaluate_log(void* h, const double* R, double* G, double* L) {
   const System& s = ((OrcSystem*)h)->sys;
   Walker w;

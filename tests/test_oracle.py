@@ -13,6 +13,8 @@ closed forms and brute force.  Which check pins which oracle function:
                                  first/second derivatives); constant coefficients -> v = c, derivatives 0
   SPO gradient convention        finite differences in Cartesian space times the cell length
   whole sweep                    incremental log|Psi|, G, L after sweeps = fresh evaluateLog
+  evaluateLog G and L            central differences of each component's log value (J1, J2 Cartesian;
+                                 determinants in the SPO lattice's fractional coordinates with h_xx)
 """
 import json
 import os
@@ -383,3 +385,64 @@ def test_evaluate_ratios_equal_first_move_ratio():
         v = R0[w, 0] + np.sqrt(tau) * np.asarray(g)
         rat = orc.evaluate_ratios(R0[w], 0, v)[0]
         assert abs(rat - res["trace"][w, 0, 0, 0]) < 1e-10 * max(1.0, abs(rat))
+
+
+# ------------------------------------------------------------------ G and L by finite differences
+@pytest.mark.parametrize("name", ["nio8-cubic", "tiny"])
+def test_gradient_laplacian_finite_differences(name):
+    """evaluateLog's G and L of every component against central differences of the same
+    component's log value (no shared code with the analytic sums: only log values are
+    differenced).  J1 and J2 are Cartesian: G = grad log J, L = laplacian of log J.  The
+    determinants keep the reference's SPO convention (einspline_spo.hpp:204-223): gradients in
+    the SPO lattice's fractional coordinates u (a step h along cell vector k moves u_k by h) and
+    d2psi = h_xx, so L_det = d2 log|det|/du_x^2 + g_x^2 - |g|^2 (DiracDeterminant.cpp:158-187).
+    A dropped 2 du/r term, a sign slip or a wrong axis moves L or G by O(0.1-1)."""
+    sysd = systems.make_system(name)
+    orc = po.OracleSystem(sysd)
+    R = systems.initial_positions(sysd, 1, seed=17)[0]
+    logs, G, L = orc.evaluate_log_parts(R)
+    N, nup = sysd.nel, sysd.nup
+    rng = np.random.default_rng(3)
+    electrons = sorted(set(rng.choice(nup, 3, replace=False)) | set(nup + rng.choice(N - nup, 3, replace=False)))
+
+    def parts_at(i, step):
+        Rp = R.copy()
+        Rp[i] += step
+        return orc.evaluate_log_parts(Rp)[0]
+
+    hg, hl = 1e-5, 5e-4
+    for i in electrons:
+        det = 0 if i < nup else 1
+        # Jastrow parts, Cartesian
+        lap = np.zeros(2)
+        for d in range(3):
+            e = np.zeros(3)
+            e[d] = 1.0
+            gp, gm = parts_at(i, hg * e), parts_at(i, -hg * e)
+            lp, lm = parts_at(i, hl * e), parts_at(i, -hl * e)
+            for k, part in enumerate((2, 3)):
+                fd_g = (gp[part] - gm[part]) / (2 * hg)
+                assert abs(fd_g - G[part, i, d]) < 1e-5 * (1 + abs(fd_g)), (name, i, part, d, fd_g, G[part, i, d])
+                lap[k] += (lp[part] - 2 * logs[part] + lm[part]) / hl**2
+        for k, part in enumerate((2, 3)):
+            assert abs(lap[k] - L[part, i]) < 1e-4 * (1 + abs(lap[k])), (name, i, part, lap[k], L[part, i])
+        # determinant part, fractional coordinates of the SPO lattice
+        fd = np.zeros(3)
+        for d in range(3):
+            step = sysd.spo_lattice[d]
+            fd[d] = (parts_at(i, hg * step)[det] - parts_at(i, -hg * step)[det]) / (2 * hg)
+        g = G[det, i]
+        assert np.all(np.abs(fd - g) < 1e-5 * (1 + np.abs(fd))), (name, i, fd, g)
+        step = sysd.spo_lattice[0]
+
+        def d2(h):
+            return (parts_at(i, h * step)[det] - 2 * logs[det] + parts_at(i, -h * step)[det]) / h**2
+
+        # Richardson extrapolation of the second difference (O(h^4)): the spline's second
+        # derivative is large near a node of the determinant
+        d2x = (4 * d2(2e-4) - d2(4e-4)) / 3
+        want = d2x + g[0] ** 2 - g @ g
+        assert abs(want - L[det, i]) < 1e-4 * (1 + abs(want)), (name, i, want, L[det, i])
+        # electrons of the other spin do not enter this determinant
+        other = 1 - det
+        assert abs(parts_at(i, hg * step)[other] - logs[other]) < 1e-12 * (1 + abs(logs[other]))
