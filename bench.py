@@ -195,13 +195,13 @@ def measure_nlpp(dev, sysd, R0, prec, steps, seed, tau, rmax=1.7):
     wb.evaluate_log()
     wb.run(1, seed, tau, qmcx.ACCEPT_REFERENCE)
     pp = nlpp.NonLocalPP(sysd, rmax=rmax, seed=seed)
-    sphere = pp.randomize()
-    _, cnt = wb.nlpp_evaluate(rmax, sphere, max_pairs=8)
+    spheres = [pp.randomize_walkers(W) for _ in range(steps + 1)]   # one rotation per walker and evaluation
+    _, cnt = wb.nlpp_evaluate(rmax, spheres[0], max_pairs=8)
     ratios = int(cnt.sum()) * nlpp.NKNOTS
-    wb.nlpp_evaluate(rmax, sphere, max_pairs=8, copy_out=False)
+    wb.nlpp_evaluate(rmax, spheres[0], max_pairs=8, copy_out=False)
     t0 = time.perf_counter()
-    for _ in range(steps):
-        wb.nlpp_evaluate(rmax, sphere, max_pairs=8, copy_out=False)
+    for st in range(steps):
+        wb.nlpp_evaluate(rmax, spheres[st + 1], max_pairs=8, copy_out=False)
     dt = (time.perf_counter() - t0) / steps
     s = 4 if prec == qmcx.FP32 else 8
     # per ratio: the value-only 4x4x4 stencil of every orbital, the inverse row, the other

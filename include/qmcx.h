@@ -140,11 +140,15 @@ int qmcx_flex_evaluate_gl(qmcx_batch_t b, double* log_out);
 int qmcx_flex_evaluate_ratios(qmcx_batch_t b, int iel, int nvp, const double* vpos, double* ratios);
 /* NonLocalPP::evaluate (src/Drivers/NonLocalPP.hpp:115-140; driver src/Drivers/miniqmc.cpp:470-495) for every
  * electron of every walker: for each ion within rmax of the electron (minimum image, ions in index order) the twelve
- * ratios at dist * sphere[k] + displ + r_e, with sphere[12][3] the rotated quadrature knots of this evaluation.
+ * ratios at dist * spheres[w][k] + displ + r_e, with spheres[nw][12][3] each walker's rotated quadrature knots of
+ * this evaluation (the reference rotates per mover and evaluation, NonLocalPP.hpp:100-113, miniqmc.cpp:470).
  * ratios[nw][N][max_pairs][12] and npairs[nw][N] (either may be NULL; unused slots are left as they were);
  * 1 <= max_pairs <= 64; QMCX_ERR_ARG when an electron has more than max_pairs ions inside rmax. */
-int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* sphere, int max_pairs, double* ratios,
+int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* spheres, int max_pairs, double* ratios,
                        int* npairs);
+/* Tuning knob of qmcx_nlpp_evaluate: knots of one electron-ion pair per work unit (1, 2, 4 or 6; 0 = chosen by
+ * orbital count and precision).  QMCX_ERR_ARG at evaluation when the unit's value rows do not fit a workgroup. */
+int qmcx_batch_set_nlpp_unit(qmcx_batch_t b, int knots_per_unit);
 
 /* Per-move record of the most recent sweep, [nw][N][9]:
  * ratio, det ratio, accepted, grad_now[3], grad_new[3].  enable must be set before the sweep. */

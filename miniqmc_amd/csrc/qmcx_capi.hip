@@ -234,6 +234,8 @@ struct qmcx_batch_s {
   double* nl_near = nullptr;      // [W][N][nl_cap][4] listed pairs: distance, displacement
   int* nl_items = nullptr;        // [W*N*nl_cap] work list of listed pairs, then its length
   int nl_cap = 0;
+  double* nl_sph = nullptr;       // [W][12][3] each walker's rotated quadrature knots
+  int nl_unit = 0;                // knots per NonLocalPP work unit (0: by size and precision)
   hipStream_t stream = nullptr;
   // FP64 recompute stream: a determinant's refresh runs there, on the CUs the move
   // kernel leaves idle, while the other determinant's electrons move
@@ -691,7 +693,7 @@ static int launch_virtual_ratios(qmcx_batch_t b, int iel, int nvp) {
 }
 
 template <typename T, int P, int PER>
-static int launch_nlpp_p(qmcx_batch_t b, double rmax, const NlppKnots& K, int maxp) {
+static int launch_nlpp_p(qmcx_batch_t b, double rmax, int maxp) {
   const qmcx_system_s* s = b->sys;
   const int WN = b->W * s->N;
   // nl_items: work list [WN*maxp], its length, per-key counts and cursors [M*kNlppRadialBins each],
@@ -714,24 +716,46 @@ static int launch_nlpp_p(qmcx_batch_t b, double rmax, const NlppKnots& K, int ma
   // workgroups walk the work list (P knots of a pair per unit); its length is only known on the device
   const int grid = (std::max(1, std::min(WN * maxp * (kNlppKnots / P), kNlppWorkgroups)) + 7) / 8 * 8;
   hipLaunchKernelGGL((k_nlpp_ratios<T, P, PER>), dim3(grid), dim3(nt), lds, b->stream, s->params<T>(),
-                     b->params<T>(), K, maxp, b->nl_near, b->nl_items, count, b->nl_out);
+                     b->params<T>(), b->nl_sph, maxp, b->nl_near, b->nl_items, count, b->nl_out);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-// Work-unit shape.  At NiO-64 (384 orbitals, 128 walkers, 33k electron-ion pairs) FP32 runs
-// fastest with one knot per unit and two stencil slices per lane group (7.1 ms per evaluation
-// against 8.9 ms for six knots per unit, 9.7 ms for two knots with two slices); FP64 with the
-// most knots whose value rows fit (four: 13.4 ms against 16.3 ms for one knot, two slices).
+// Work-unit shape: P knots of a pair per unit, PER of the 16 stencil columns per lane.  With the
+// shared sphere of an earlier form, NiO-64 (384 orbitals, 128 walkers, 33k electron-ion pairs) ran
+// fastest in FP32 at one knot per unit with two stencil slices, and in FP64 with the most knots
+// whose value rows fit (four).  `unit` (qmcx_batch_set_nlpp_unit) forces P.
 template <typename T>
-static int launch_nlpp(qmcx_batch_t b, double rmax, const NlppKnots& K, int maxp) {
+static int launch_nlpp(qmcx_batch_t b, double rmax, int maxp) {
   const int nq = b->sys->npad / (16 / (int)sizeof(T));
-  if (sizeof(T) == 4 && 2 * nq <= kNlppThreads) return launch_nlpp_p<T, 1, 8>(b, rmax, K, maxp);
-  if (6 * nq <= kNlppThreads) return launch_nlpp_p<T, 6, 16>(b, rmax, K, maxp);
-  if (4 * nq <= kNlppThreads) return launch_nlpp_p<T, 4, 16>(b, rmax, K, maxp);
-  if (2 * nq <= kNlppThreads) return launch_nlpp_p<T, 2, 16>(b, rmax, K, maxp);
-  if (nq <= kNlppThreads) return launch_nlpp_p<T, 1, 16>(b, rmax, K, maxp);
-  return fail(QMCX_ERR_ARG, "spline row does not fit a workgroup");
+  const int unit = b->nl_unit;
+  auto fits = [&](int p, int per) { return p * (16 / per) * nq <= kNlppThreads; };
+  if (unit == 0) {
+    if (sizeof(T) == 4 && fits(1, 8)) return launch_nlpp_p<T, 1, 8>(b, rmax, maxp);
+    if (fits(6, 16)) return launch_nlpp_p<T, 6, 16>(b, rmax, maxp);
+    if (fits(4, 16)) return launch_nlpp_p<T, 4, 16>(b, rmax, maxp);
+    if (fits(2, 16)) return launch_nlpp_p<T, 2, 16>(b, rmax, maxp);
+    if (fits(1, 16)) return launch_nlpp_p<T, 1, 16>(b, rmax, maxp);
+    return fail(QMCX_ERR_ARG, "spline row does not fit a workgroup");
+  }
+  switch (unit) {
+    case 1:
+      if (fits(1, 8)) return launch_nlpp_p<T, 1, 8>(b, rmax, maxp);
+      if (fits(1, 16)) return launch_nlpp_p<T, 1, 16>(b, rmax, maxp);
+      break;
+    case 2:
+      if (fits(2, 16)) return launch_nlpp_p<T, 2, 16>(b, rmax, maxp);
+      break;
+    case 4:
+      if (fits(4, 16)) return launch_nlpp_p<T, 4, 16>(b, rmax, maxp);
+      break;
+    case 6:
+      if (fits(6, 16)) return launch_nlpp_p<T, 6, 16>(b, rmax, maxp);
+      break;
+    default:
+      break;
+  }
+  return fail(QMCX_ERR_ARG, "NonLocalPP work unit of " + std::to_string(unit) + " knots does not fit a workgroup");
 }
 
 extern "C" {
@@ -1042,6 +1066,7 @@ int qmcx_batch_destroy(qmcx_batch_t b) {
   if (b->nl_cnt) hipFree(b->nl_cnt);
   if (b->nl_near) hipFree(b->nl_near);
   if (b->nl_items) hipFree(b->nl_items);
+  if (b->nl_sph) hipFree(b->nl_sph);
   if (b->stream) hipStreamDestroy(b->stream);
   if (b->aux) hipStreamDestroy(b->aux);
   if (b->pre) hipStreamDestroy(b->pre);
@@ -1328,9 +1353,17 @@ int qmcx_flex_evaluate_ratios(qmcx_batch_t b, int iel, int nvp, const double* vp
   return 0;
 }
 
-int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* sphere, int max_pairs, double* ratios,
+int qmcx_batch_set_nlpp_unit(qmcx_batch_t b, int knots_per_unit) {
+  if (!b || !(knots_per_unit == 0 || knots_per_unit == 1 || knots_per_unit == 2 || knots_per_unit == 4 ||
+              knots_per_unit == 6))
+    return fail(QMCX_ERR_ARG, "knots per NonLocalPP work unit must be 0 (automatic), 1, 2, 4 or 6");
+  b->nl_unit = knots_per_unit;
+  return 0;
+}
+
+int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* spheres, int max_pairs, double* ratios,
                        int* npairs) {
-  if (!b || !sphere || !(rmax > 0) || max_pairs < 1 || max_pairs > kMaxNlppPairs) return fail(QMCX_ERR_ARG, "bad argument");
+  if (!b || !spheres || !(rmax > 0) || max_pairs < 1 || max_pairs > kMaxNlppPairs) return fail(QMCX_ERR_ARG, "bad argument");
   const size_t WN = (size_t)b->W * b->sys->N;
   // work-list entries and knot units are int indices on the device
   if (WN * max_pairs * kNlppKnots >= (size_t)INT_MAX) return fail(QMCX_ERR_ARG, "walkers x electrons x max_pairs too large");
@@ -1352,10 +1385,10 @@ int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* sphere, int ma
                      sizeof(int) * (2 * WN * max_pairs + 1 + 2 * (size_t)b->sys->nions * kNlppRadialBins)));
     b->nl_cap = max_pairs;
   }
-  NlppKnots K;
-  for (int k = 0; k < kNlppKnots; ++k)
-    for (int d = 0; d < 3; ++d) K.p[k][d] = sphere[3 * k + d];
-  const int rc = b->prec == QMCX_FP32 ? launch_nlpp<float>(b, rmax, K, max_pairs) : launch_nlpp<double>(b, rmax, K, max_pairs);
+  if (!b->nl_sph) HIPCHK(hipMalloc(&b->nl_sph, sizeof(double) * 3 * kNlppKnots * (size_t)b->W));
+  HIPCHK(hipMemcpyAsync(b->nl_sph, spheres, sizeof(double) * 3 * kNlppKnots * (size_t)b->W, hipMemcpyHostToDevice,
+                        b->stream));
+  const int rc = b->prec == QMCX_FP32 ? launch_nlpp<float>(b, rmax, max_pairs) : launch_nlpp<double>(b, rmax, max_pairs);
   if (rc) return rc;
   std::vector<int> cnt(WN);
   HIPCHK(hipMemcpyAsync(cnt.data(), b->nl_cnt, sizeof(int) * WN, hipMemcpyDeviceToHost, b->stream));

@@ -1056,11 +1056,11 @@ __global__ __launch_bounds__(1024) void k_virtual_ratios(SysP<T> S, BatchP<T> B,
 // near[(w*N + e)*maxp + p][4] = dist, displacement and pkey[...] = ion * kNlppRadialBins + the
 // distance's bin in [0, rmax), counts them in npairs[w][e] (the full count, also beyond maxp) and
 // per key in icnt[key].  k_nlpp_offsets and k_nlpp_scatter order the listed pairs by key into the
-// work list items[] (count[0] entries).  k_nlpp_ratios walks knot-major over that list: one
-// sphere serves the whole call, so knot k of every pair around one ion at a similar distance sits
-// at nearly the same point, and consecutive work units reuse its stencil rows in L2.  The knots
-// are placed at dist * sphere[k] + displ + r_e (NonLocalPP.hpp:133);
-// ratios[w][e][maxp][kNlppKnots].
+// work list items[] (count[0] entries).  k_nlpp_ratios walks knot-major over that list, each
+// XCD over a contiguous run of ion keys.  Every walker has its own randomly rotated knots
+// (spheres[w][12][3], one rotation per walker and evaluation as the reference's per-mover
+// NonLocalPP::randomize, NonLocalPP.hpp:100-113, miniqmc.cpp:470); knot k of a pair sits at
+// dist * spheres[w][k] + displ + r_e (NonLocalPP.hpp:133); ratios[w][e][maxp][kNlppKnots].
 constexpr int kNlppKnots = 12;
 constexpr int kNlppRadialBins = 16;
 struct NlppKnots {
@@ -1145,7 +1145,7 @@ __global__ __launch_bounds__(256) void k_nlpp_scatter(int WN, int maxp, const in
 // electron's position once for all P knots.
 constexpr int kNlppThreads = 768;
 template <typename T, int P, int PER>
-__global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<T> B, NlppKnots K, int maxp,
+__global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<T> B, const double* spheres, int maxp,
                                                               const double* near, const int* items, const int* count,
                                                               double* ratios) {
   constexpr int SL = 16 / PER, KG = kNlppKnots / P;
@@ -1170,6 +1170,7 @@ __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<
     const int idx = item / maxp;
     const int w = idx / N, e = idx - w * N;
     const double* R = B.R + (size_t)w * 3 * N;
+    const double* sp = spheres + (size_t)w * 3 * kNlppKnots;   // this walker's knots
     const double* q = near + (size_t)item * 4;
     const double d = q[0];
     const double ox = R[e] + q[1], oy = R[N + e] + q[2], oz = R[2 * N + e] + q[3];
@@ -1182,7 +1183,7 @@ __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<
     {
       if (g < P) {
         const int k = k0 + g;
-        spline_gather<T, 1, PER>(S, S.spoG, d * K.p[k][0] + ox, d * K.p[k][1] + oy, d * K.p[k][2] + oz,
+        spline_gather<T, 1, PER>(S, S.spoG, d * sp[3 * k] + ox, d * sp[3 * k + 1] + oy, d * sp[3 * k + 2] + oz,
                                  part + (size_t)g * SL * S.npad, tid - g * L);
       }
       double v[3 * P];
@@ -1195,7 +1196,7 @@ __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<
         const double ix = S.ions[ci], iy = S.ions[M + ci], iz = S.ions[2 * M + ci];
 #pragma unroll
         for (int gg = 0; gg < P; ++gg) {
-          const double px = d * K.p[k0 + gg][0] + ox, py = d * K.p[k0 + gg][1] + oy, pz = d * K.p[k0 + gg][2] + oz;
+          const double px = d * sp[3 * (k0 + gg)] + ox, py = d * sp[3 * (k0 + gg) + 1] + oy, pz = d * sp[3 * (k0 + gg) + 2] + oz;
           const T r = min_image(C, (T)(ix - px), (T)(iy - py), (T)(iz - pz), T(-1), dx, dy, dz);
           if (r < f.rcut) {
             functor_vgl(f, r, u, du, d2u);
@@ -1209,7 +1210,7 @@ __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<
         const double rx = R[j], ry = R[N + j], rz = R[2 * N + j];
 #pragma unroll
         for (int gg = 0; gg < P; ++gg) {
-          const double px = d * K.p[k0 + gg][0] + ox, py = d * K.p[k0 + gg][1] + oy, pz = d * K.p[k0 + gg][2] + oz;
+          const double px = d * sp[3 * (k0 + gg)] + ox, py = d * sp[3 * (k0 + gg) + 1] + oy, pz = d * sp[3 * (k0 + gg) + 2] + oz;
           const T r = min_image(C, (T)(rx - px), (T)(ry - py), (T)(rz - pz), T(-1), dx, dy, dz);
           if (r < f.rcut) {
             functor_vgl(f, r, u, du, d2u);

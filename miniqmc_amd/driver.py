@@ -14,7 +14,7 @@ Differences from the reference, by design:
     walker thread, and the device stages come from HIP events (move, flush, recompute, evaluateGL,
     proposals);
   * acceptance: the reference's rule u < r with r = 0.5 (``-r`` only accepts 0.5), or Metropolis;
-  * one random sphere per pseudopotential evaluation for the whole batch;
+  * one random sphere per walker and pseudopotential evaluation (each mover rotates its own);
   * systems: the tilings with a prepared workload, -g "1 1 1" (NiO-32) and -g "1 1 2" (NiO-64).
 """
 from __future__ import annotations
@@ -133,7 +133,7 @@ def main(argv=None) -> int:
         add("Diffusion", time.perf_counter() - t)
         t = time.perf_counter()
         # the reference discards the ratios (NonLocalPP.hpp:115-140); they stay on the device
-        _, cnt = wb.nlpp_evaluate(a.x, pp.randomize(), max_pairs=8, ratios=False)
+        _, cnt = wb.nlpp_evaluate(a.x, pp.randomize_walkers(a.w), max_pairs=8, ratios=False)
         add("Pseudopotential", time.perf_counter() - t)
         pairs += int(cnt.sum())
     add("Total", time.perf_counter() - t0)

@@ -6,8 +6,9 @@ asks the wave function for the twelve virtual-move ratios (NonLocalPP::evaluate,
 NonLocalPP.hpp:115-140; driver miniqmc.cpp:470-495).  Here one call per electron covers every
 walker: the ratios run on the GPU through ``qmcx_flex_evaluate_ratios``, and walkers with fewer
 near ions than the batch's largest count are padded with the electron's own position, whose
-ratio is one up to rounding and is dropped.  ``NonLocalPP.evaluate`` is this host loop; the
-batch's ``nlpp_evaluate`` does the same work in two device launches.
+ratio is one up to rounding and is dropped.  Every walker rotates its own knots for each
+evaluation.  ``NonLocalPP.evaluate`` is this host loop; the batch's ``nlpp_evaluate`` does the same
+work in a few device launches.
 """
 from __future__ import annotations
 
@@ -68,19 +69,26 @@ class NonLocalPP:
         self.rng = np.random.default_rng(seed)
 
     def randomize(self) -> np.ndarray:
-        """Rotated knots [12][3] for one evaluation (one random sphere per call, as the driver)."""
+        """Rotated knots [12][3] from one random rotation."""
         return self.knots @ random_rotation(self.rng.random(3)).T
 
-    def virtual_positions(self, R: np.ndarray, jel: int, sphere: np.ndarray):
-        """For electron jel of every walker (R [nw][N][3]): vpos [nw][npair*12][3] and the number of
-        real (electron, ion) pairs per walker; padding points sit at the electron itself."""
+    def randomize_walkers(self, nw: int) -> np.ndarray:
+        """Rotated knots [nw][12][3]: one rotation per walker for one evaluation, as each of the
+        reference's movers rotates its own quadrature (miniqmc.cpp:470, NonLocalPP.hpp:100-113)."""
+        return np.stack([self.randomize() for _ in range(nw)])
+
+    def virtual_positions(self, R: np.ndarray, jel: int, spheres: np.ndarray):
+        """For electron jel of every walker (R [nw][N][3]), with walker w's knots spheres[w] (or one
+        [12][3] set for all): vpos [nw][npair*12][3] and the number of real (electron, ion) pairs per
+        walker; padding points sit at the electron itself."""
         nw = R.shape[0]
+        spheres = np.broadcast_to(np.asarray(spheres).reshape(-1, NKNOTS, 3), (nw, NKNOTS, 3))
         rows = []
         for w in range(nw):
             disp, dist = electron_ion_displacements(self.sys.lattice, self.sys.ion_pos, R[w, jel])
             near = np.nonzero(dist < self.rmax)[0]
             # virtualPos[k] = dist * rOnSphere[k] + displ + R[jel]  (NonLocalPP.hpp:133)
-            rows.append([dist[c] * sphere + disp[c] + R[w, jel] for c in near])
+            rows.append([dist[c] * spheres[w] + disp[c] + R[w, jel] for c in near])
         npair = np.array([len(r) for r in rows])
         width = max(1, int(npair.max())) * NKNOTS
         vpos = np.repeat(R[:, jel][:, None, :], width, axis=1).copy()
@@ -92,9 +100,9 @@ class NonLocalPP:
     def evaluate(self, wb, R: np.ndarray | None = None):
         """NonLocalPP::evaluate (NonLocalPP.hpp:115-140) for every walker of ``wb``.  Returns, per
         walker, the list of (electron, ratios[12]) for every electron-ion pair inside rmax, and the
-        random sphere used."""
+        walkers' rotated knots [nw][12][3]."""
         R = wb.positions() if R is None else R
-        sphere = self.randomize()
+        sphere = self.randomize_walkers(R.shape[0])
         out = [[] for _ in range(R.shape[0])]
         for jel in range(self.sys.nel):
             vpos, npair = self.virtual_positions(R, jel, sphere)

@@ -65,6 +65,7 @@ _SIGS = {
     "qmcx_flex_evaluate_gl": [_vp, _vp],
     "qmcx_flex_evaluate_ratios": [_vp, _i, _i, _dp, _vp],
     "qmcx_nlpp_evaluate": [_vp, C.c_double, _dp, _i, _vp, _vp],
+    "qmcx_batch_set_nlpp_unit": [_vp, _i],
     "qmcx_du_create": [C.POINTER(_vp), _i, _i, _i],
     "qmcx_du_destroy": [_vp],
     "qmcx_du_invert_transpose": [_vp, _dp, _vp, _vp, _vp],
@@ -324,11 +325,16 @@ class WalkerBatch:
         _check(load().qmcx_flex_evaluate_ratios(self.h, iel, vpos.shape[1], vpos.reshape(-1), out.ctypes.data))
         return out
 
-    def nlpp_evaluate(self, rmax, sphere, max_pairs=4, copy_out=True, ratios=True):
-        """NonLocalPP::evaluate for every electron of every walker on the GPU: ratios
+    def set_nlpp_unit(self, knots_per_unit):
+        """Knots per NonLocalPP work unit (1, 2, 4, 6; 0 = automatic)."""
+        _check(load().qmcx_batch_set_nlpp_unit(self.h, knots_per_unit))
+
+    def nlpp_evaluate(self, rmax, spheres, max_pairs=4, copy_out=True, ratios=True):
+        """NonLocalPP::evaluate for every electron of every walker on the GPU, each walker with its
+        own knots spheres[nw][12][3] (one [12][3] set is given to every walker): ratios
         [nw][N][max_pairs][12] and npairs [nw][N] (None, None without copy_out; None, npairs
         without ratios)."""
-        sphere = _c(sphere).reshape(12, 3)
+        sphere = _c(np.broadcast_to(_c(spheres).reshape(-1, 12, 3), (self.nw, 12, 3)))
         if not copy_out:
             _check(load().qmcx_nlpp_evaluate(self.h, float(rmax), sphere.reshape(-1), max_pairs, None, None))
             return None, None

@@ -66,6 +66,7 @@ def lib():
         L.orc_evaluate_log.restype = C.c_double
         L.orc_evaluate_ratios.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, _dp, _dp]
         L.orc_evaluate_log_parts.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_evaluate_ratios_many.argtypes = [C.c_void_p, _dp, C.c_int, _ip, C.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -209,6 +210,16 @@ class OracleSystem:
         vpos = _c(vpos).reshape(-1, 3)
         out = np.zeros(len(vpos))
         lib().orc_evaluate_ratios(self.h, _c(R).reshape(-1), iel, len(vpos), vpos.reshape(-1), out)
+        return out
+
+    def evaluate_ratios_many(self, R, iels, vpos):
+        """evaluate_ratios for several electrons of configuration R with one evaluateLog:
+        electron iels[q] at vpos[q][nvp][3] -> ratios[q][nvp]."""
+        iels = _c(iels, np.int32).reshape(-1)
+        vpos = _c(vpos).reshape(len(iels), -1, 3)
+        out = np.zeros((len(iels), vpos.shape[1]))
+        lib().orc_evaluate_ratios_many(self.h, _c(R).reshape(-1), len(iels), iels, vpos.shape[1], vpos.reshape(-1),
+                                       out.reshape(-1))
         return out
 
     def run_vmc(self, R, nsweeps, seed, tau, mode=1, delay=1, walker0=0, trace=False, nthreads=0):

@@ -1287,11 +1287,32 @@ double orc_run_vmc(void* h, int nw, int walker0, double* R, int nsweeps, uint64_
 //   J1           exp(Vat[iel] - sum_I u(|r_I - vpos|))      (OneBodyJastrow.h:136-140)
 //   J2           exp(Uat[iel] - sum_{j != iel} u(|r_j - vpos|)) (TwoBodyJastrow.h:103-107)
 // The virtual particle set's tables are BA rows from the point (VirtualParticleSet.cpp:35-56).
+static void evaluate_ratios_at(Walker& w, int iel, int nvp, const double* vpos, double* ratios);
+
 void orc_evaluate_ratios(void* h, const double* R, int iel, int nvp, const double* vpos, double* ratios) {
   const System& s = ((OrcSystem*)h)->sys;
   Walker w;
   walker_init(w, s, R, 1);
   evaluate_log(w);
+  evaluate_ratios_at(w, iel, nvp, vpos, ratios);
+}
+
+// the same for several electrons of one configuration with one evaluateLog: electron iels[q]
+// at vpos[q][nvp][3] -> ratios[q][nvp]
+void orc_evaluate_ratios_many(void* h, const double* R, int nq, const int* iels, int nvp, const double* vpos,
+                              double* ratios) {
+  const System& s = ((OrcSystem*)h)->sys;
+  Walker w;
+  walker_init(w, s, R, 1);
+  evaluate_log(w);
+  for (int q = 0; q < nq; ++q)
+    evaluate_ratios_at(w, iels[q], nvp, vpos + 3 * (size_t)q * nvp, ratios + (size_t)q * nvp);
+}
+
+}  // extern "C"
+
+static void evaluate_ratios_at(Walker& w, int iel, int nvp, const double* vpos, double* ratios) {
+  const System& s = *w.sys;
   Determinant& D = det_of(w, iel);
   const int n = D.n, row = iel - D.first, N = w.N;
   std::vector<double> psi(n), dpsi(3 * (size_t)n), d2psi(n), scratch;
@@ -1315,6 +1336,8 @@ void orc_evaluate_ratios(void* h, const double* R, int iel, int nvp, const doubl
     ratios[k] = det_ratio * std::exp((w.Vat[iel] - v1) + (w.Uat[iel] - v2));
   }
 }
+
+extern "C" {
 
 // evaluateLog split by component (the WaveFunctionComponent list of WaveFunction.cpp:178-195):
 // part 0 = up determinant, 1 = down determinant, 2 = J1, 3 = J2.  logs[4]; G[4][N][3]; L[4][N].

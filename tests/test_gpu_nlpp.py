@@ -20,7 +20,7 @@ def _say(*a):
 
 
 def _check_all(sysd, orc, wb, R, pp, tol, electrons):
-    sphere = pp.randomize()
+    sphere = pp.randomize_walkers(R.shape[0])
     worst, count = 0.0, 0
     for jel in electrons:
         vpos, npair = pp.virtual_positions(R, jel, sphere)
@@ -95,7 +95,7 @@ def test_nlpp_evaluate_pairs():
         for jel in range(sysd.nel):
             disp, dist = nlpp.electron_ion_displacements(sysd.lattice, sysd.ion_pos, R[w, jel])
             for c in np.nonzero(dist < 1.7)[0]:
-                expect.append((jel, dist[c] * sphere + disp[c] + R[w, jel]))
+                expect.append((jel, dist[c] * sphere[w] + disp[c] + R[w, jel]))
         assert len(out[w]) == len(expect) > 0
         for (jel, rat), (jel2, v) in zip(out[w], expect):
             assert jel == jel2
@@ -121,13 +121,45 @@ def _expected_pairs(sysd, R, jel, rmax, sphere):
     return [dist[c] * sphere + disp[c] + R[jel] for c in near], border
 
 
+def _check_device_evaluate(sysd, wb, rmax, spheres, tol, stride, max_pairs=4, walkers=None):
+    """qmcx_nlpp_evaluate against the oracle: each electron's ion list inside rmax (ion order) and,
+    for every stride-th electron, its 12 ratios per pair at its walker's own knots."""
+    rat, cnt = wb.nlpp_evaluate(rmax, spheres, max_pairs=max_pairs)
+    R = wb.positions()
+    orc = po.OracleSystem(sysd)
+    checked = 0
+    for w in (range(wb.nw) if walkers is None else walkers):
+        iels, vps, slots = [], [], []
+        for jel in range(sysd.nel):
+            exp_v, border = _expected_pairs(sysd, R[w], jel, rmax, spheres[w])
+            if border:
+                continue
+            assert cnt[w, jel] == len(exp_v), (w, jel)
+            if jel % stride or not exp_v:
+                continue
+            for p, v in enumerate(exp_v):
+                iels.append(jel)
+                vps.append(v)
+                slots.append((jel, p))
+        if not iels:
+            continue
+        ref = orc.evaluate_ratios_many(R[w], iels, np.array(vps))
+        for (jel, p), rr in zip(slots, ref):
+            err = np.abs(rat[w, jel, p] - rr) / np.maximum(np.abs(rr), 1.0)
+            assert err.max() < tol, (w, jel, p, err.max())
+            checked += 1
+    _say(f"  pairs {int(cnt.sum())}, checked {checked}")
+    assert checked > 0
+    return rat, cnt
+
+
 @pytest.mark.parametrize("name,prec,delay,tol,stride", [
     ("nio8-cubic", qmcx.FP64, 4, 1e-10, 1),
     ("nio8-cubic", qmcx.FP32, 4, 2e-3, 1),
     ("nio32", qmcx.FP32, 32, 2e-3, 23),
 ])
 def test_nlpp_device_evaluate_vs_oracle(name, prec, delay, tol, stride):
-    """qmcx_nlpp_evaluate: every electron's ion list inside rmax (ion order) and its 12 ratios per pair."""
+    """qmcx_nlpp_evaluate with one random rotation per walker (miniqmc.cpp:470)."""
     sysd = systems.make_system(name)
     _say("nlpp device evaluate", name, prec, delay)
     nw, rmax = 3, 1.7
@@ -137,26 +169,38 @@ def test_nlpp_device_evaluate_vs_oracle(name, prec, delay, tol, stride):
     wb.evaluate_log()
     wb.run(1, 9, 0.3, qmcx.ACCEPT_METROPOLIS, recompute_every=1)
     pp = nlpp.NonLocalPP(sysd, rmax=rmax, seed=8)
-    sphere = pp.randomize()
-    rat, cnt = wb.nlpp_evaluate(rmax, sphere, max_pairs=4)
-    R = wb.positions()
-    orc = po.OracleSystem(sysd)
-    checked = 0
-    for w in range(nw):
-        for jel in range(sysd.nel):
-            exp_v, border = _expected_pairs(sysd, R[w], jel, rmax, sphere)
-            if border:
-                continue
-            assert cnt[w, jel] == len(exp_v), (w, jel)
-            if jel % stride or not exp_v:
-                continue
-            for p, v in enumerate(exp_v):
-                ref = orc.evaluate_ratios(R[w], jel, v)
-                err = np.abs(rat[w, jel, p] - ref) / np.maximum(np.abs(ref), 1.0)
-                assert err.max() < tol, (w, jel, p, err.max())
-                checked += 1
-    _say(f"  pairs {int(cnt.sum())}, checked {checked}")
-    assert checked > 0
+    spheres = pp.randomize_walkers(nw)
+    assert not np.allclose(spheres[0], spheres[1])
+    _check_device_evaluate(sysd, wb, rmax, spheres, tol, stride)
+
+
+@pytest.mark.parametrize("prec", [qmcx.FP64, qmcx.FP32])
+@pytest.mark.parametrize("unit", [1, 2, 4, 6])
+def test_nlpp_work_unit_shapes_vs_oracle(prec, unit):
+    """Every work-unit shape of k_nlpp_ratios (P knots of a pair per unit), forced through the
+    tuning knob, against the oracle (NiO-8, 2 walkers)."""
+    sysd = systems.make_system("nio8-cubic")
+    _say("nlpp work unit", unit, prec)
+    dev = qmcx.System(sysd)
+    wb = qmcx.WalkerBatch(dev, 2, delay_rank=1, precision=prec)
+    wb.set_nlpp_unit(unit)
+    wb.set_positions(systems.initial_positions(sysd, 2, seed=21))
+    wb.evaluate_log()
+    spheres = nlpp.NonLocalPP(sysd, rmax=1.7, seed=unit).randomize_walkers(2)
+    _check_device_evaluate(sysd, wb, 1.7, spheres, 1e-10 if prec == qmcx.FP64 else 2e-3, 1)
+
+
+def test_nlpp_nio64_fp64_vs_oracle():
+    """The automatic shape at NiO-64 in FP64 (four knots per unit, 384 orbitals), one walker of two
+    checked against the oracle on every 16th electron."""
+    sysd = systems.make_system("nio64")
+    _say("nlpp nio64 fp64")
+    dev = qmcx.System(sysd)
+    wb = qmcx.WalkerBatch(dev, 2, delay_rank=1, precision=qmcx.FP64)
+    wb.set_positions(systems.initial_positions(sysd, 2, seed=5))
+    wb.evaluate_log()
+    spheres = nlpp.NonLocalPP(sysd, rmax=1.7, seed=2).randomize_walkers(2)
+    _check_device_evaluate(sysd, wb, 1.7, spheres, 1e-10, 16, max_pairs=8, walkers=[1])
 
 
 def test_nlpp_device_evaluate_overflow_is_an_error():
@@ -190,13 +234,13 @@ def test_nlpp_skewed_cell_vs_oracle(prec, tol):
     pp = nlpp.NonLocalPP(sysd, rmax=rmax, seed=6)
     worst, count = _check_all(sysd, orc, wb, wb.positions(), pp, tol, range(sysd.nel))
     assert count > 0
-    sphere = pp.randomize()
+    sphere = pp.randomize_walkers(nw)
     rat, cnt = wb.nlpp_evaluate(rmax, sphere, max_pairs=8)
     R = wb.positions()
     checked = 0
     for w in range(nw):
         for jel in range(sysd.nel):
-            exp_v, border = _expected_pairs(sysd, R[w], jel, rmax, sphere)
+            exp_v, border = _expected_pairs(sysd, R[w], jel, rmax, sphere[w])
             if border:
                 continue
             assert cnt[w, jel] == len(exp_v)
