@@ -155,6 +155,15 @@ int qmcx_batch_set_nlpp_unit(qmcx_batch_t b, int knots_per_unit);
 int qmcx_set_trace(qmcx_batch_t b, int enable);
 int qmcx_get_trace(qmcx_batch_t b, double* trace);
 
+/* Identical accept/reject decisions in the FP32 form under Metropolis acceptance (no reference
+ * counterpart: the reference is FP64 throughout).  A move whose uniform u satisfies
+ * |u - |R|^2| <= band |R|^2 for the FP32 ratio R is decided on its FP64 ratio: FP64 Slater rows with
+ * the determinant ratio by iterative refinement from the FP32 inverse, J1/J2 from scratch.
+ * band in [0, 1]; 0 switches it off; default 1e-2.  FP64 batches ignore it.
+ * qmcx_get_arbitrations: per-walker count of such decisions since the batch was created. */
+int qmcx_batch_set_accept_band(qmcx_batch_t b, double band);
+int qmcx_get_arbitrations(qmcx_batch_t b, long long* narb);
+
 /* Outputs of the last evaluateGL: G[nw][N][3], L[nw][N], log[nw]; per-walker accept counts. */
 int qmcx_get_gl(qmcx_batch_t b, double* G, double* L, double* log_out);
 int qmcx_get_accepts(qmcx_batch_t b, long long* naccept);

@@ -198,6 +198,8 @@ struct qmcx_system_s {
     S.xs = (long long)(ng[1] + 3) * S.ys;
     S.ions = d_ions;
     S.species = d_species;
+    S.fn64 = d_fnd;
+    S.cell64 = d_cell_d;
     if constexpr (sizeof(T) == 4) {
       S.coefs = d_coef_f;
       S.fn = d_fnf;
@@ -218,11 +220,16 @@ struct qmcx_system_s {
   }
 };
 
+// Default near-tie band of the FP32 form's Metropolis decisions: a move whose uniform lies within
+// this fraction of |ratio|^2 from |ratio|^2 is decided on the FP64 ratio.
+constexpr double kDefaultAcceptBand = 1e-2;
+
 struct qmcx_batch_s {
   qmcx_system_s* sys = nullptr;
   int W = 0, K = 1, prec = QMCX_FP64, walker0 = 0, slices = 0, nt = 0;
   int nch = 1;   // delayed-update state slots per walker and determinant: one per chunk
   int flush_kind = 0;   // 0: matrix cores when possible, 1: vector FMA, 2: matrix cores, U rows from HBM
+  double band = kDefaultAcceptBand;   // FP32 Metropolis: near-tie band of the FP64 re-evaluation
   double* flex_displ = nullptr;   // [W][3] proposals of the pending flex move
   int* flex_decide = nullptr;     // [W]
   int flex_iel = -1;
@@ -261,6 +268,8 @@ struct qmcx_batch_s {
   void* Vbuf = nullptr;
   void* Binv = nullptr;
   void* spo = nullptr;   // [W][2][n][5][npad] SPO rows at the sweep's proposals
+  double* A64 = nullptr;   // FP32 form: [W][2][n][n] FP64 Slater rows (the FP64 refresh inverts these)
+  double* spo64 = nullptr; // FP32 form: [W][2][n][npad] FP64 values at the sweep's proposals
   double* prop = nullptr; // [W][2][n][kProp] proposal records of the sweep
   int* rows = nullptr;
   int* ndelay = nullptr;
@@ -268,6 +277,7 @@ struct qmcx_batch_s {
   double* phase = nullptr;
   double* jlog = nullptr;
   unsigned long long* nacc = nullptr;
+  unsigned long long* narb = nullptr;
   double* trace = nullptr;
   int* sweep = nullptr;
   double* G = nullptr;
@@ -341,6 +351,9 @@ struct qmcx_batch_s {
     B.nacc = nacc;
     B.trace = trace_on ? trace : nullptr;
     B.sweep = sweep;
+    B.A64 = sizeof(T) == 4 ? A64 : nullptr;
+    B.spo64 = sizeof(T) == 4 ? spo64 : nullptr;
+    B.narb = narb;
     return B;
   }
   template <typename P>
@@ -410,7 +423,10 @@ int launch_proposals(qmcx_batch_t b, hipStream_t st, int det, int e0, int ne, ui
   const int nq = s->npad / (16 / (int)sizeof(T));
   const int slices = proposal_slices<T>(s, b->slices);
   const int nt = std::max(64, (nq * slices + 63) / 64 * 64);
-  const size_t lds = std::max(sizeof(T) * (size_t)slices * 5 * s->npad, sizeof(double) * 16 * 5);
+  size_t lds = std::max(sizeof(T) * (size_t)slices * 5 * s->npad, sizeof(double) * 16 * 5);
+  if (sizeof(T) == 4)   // and the FP64 values of the FP32 form
+    lds = std::max(lds, ((sizeof(T) * (size_t)slices * 5 * s->npad + 15) & ~(size_t)15) +
+                            sizeof(double) * (size_t)slices * s->npad);
   // about 1024 workgroups (four per CU), each looping over its electrons: the gather's burst of
   // HBM traffic stays below what slows the latency-bound move kernel beside it
   const int gx = std::max(1, std::min(ne, kPropWorkgroups / b->W));
@@ -436,7 +452,8 @@ int launch_spo_rows(qmcx_batch_t b) {
   const int nq = s->npad / vw;
   const int slices = spline_slices_for<T>(s, 1024);
   const int nt = std::max(64, (nq * slices + 63) / 64 * 64);
-  const size_t lds = sizeof(T) * ((size_t)slices * 5 * s->npad + 5 * (size_t)s->n);
+  size_t lds = sizeof(T) * ((size_t)slices * 5 * s->npad + 5 * (size_t)s->n);
+  if (sizeof(T) == 4) lds = ((lds + 15) & ~(size_t)15) + sizeof(double) * (size_t)slices * s->npad;   // FP64 values
   hipLaunchKernelGGL(k_spo_rows<T>, dim3(s->N, b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(),
                      slices);
   HIPCHK(hipGetLastError());
@@ -447,7 +464,10 @@ template <typename T>
 int launch_lu_batch(qmcx_batch_t b) {
   const qmcx_system_s* s = b->sys;
   const int n = s->n;
-  HIPCHK((launch_gj_invert<T, T>(n, b->W * 2, b->stream, (const T*)b->A, (T*)b->Ainv, b->lu_ws, b->logdet, b->phase, 1)));
+  if (sizeof(T) == 4)   // the FP32 form inverts its FP64 Slater rows
+    HIPCHK((launch_gj_invert<double, T>(n, b->W * 2, b->stream, b->A64, (T*)b->Ainv, b->lu_ws, b->logdet, b->phase, 1)));
+  else
+    HIPCHK((launch_gj_invert<T, T>(n, b->W * 2, b->stream, (const T*)b->A, (T*)b->Ainv, b->lu_ws, b->logdet, b->phase, 1)));
   return 0;
 }
 
@@ -528,6 +548,7 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
   a.spo = b->spo;
   a.prop = b->prop;
   a.decide = nullptr;
+  a.band = b->band;
   const int nchunks = (n + chunk - 1) / chunk;
   a.det = det;
   a.chunk_len = chunk;
@@ -573,12 +594,15 @@ int refresh_det(qmcx_batch_t b, int det, bool into_fresh) {
   const int n = b->sys->n;
   const size_t nn = (size_t)n * n;
   if (b->prof_record(2, b->aux)) return QMCX_ERR_HIP;
-  if (into_fresh)
-    HIPCHK((launch_gj_invert<T, T>(n, b->W, b->aux, (const T*)b->A + det * nn, (T*)b->Ainv + det * nn,
-                                   b->lu_ws + det * 2 * nn, b->ld_fresh, b->ph_fresh, 1, 2)));
+  double* ld = into_fresh ? b->ld_fresh : b->logdet + det;
+  double* ph = into_fresh ? b->ph_fresh : b->phase + det;
+  const int lds = into_fresh ? 1 : 2;
+  if (sizeof(T) == 4)   // the FP32 form inverts its FP64 Slater rows
+    HIPCHK((launch_gj_invert<double, T>(n, b->W, b->aux, b->A64 + det * nn, (T*)b->Ainv + det * nn,
+                                        b->lu_ws + det * 2 * nn, ld, ph, lds, 2)));
   else
     HIPCHK((launch_gj_invert<T, T>(n, b->W, b->aux, (const T*)b->A + det * nn, (T*)b->Ainv + det * nn,
-                                   b->lu_ws + det * 2 * nn, b->logdet + det, b->phase + det, 2, 2)));
+                                   b->lu_ws + det * 2 * nn, ld, ph, lds, 2)));
   if (b->prof_close(b->aux)) return QMCX_ERR_HIP;
   return 0;
 }
@@ -1020,6 +1044,7 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
   rc |= b->alloc(b->phase, W * 2);
   rc |= b->alloc(b->jlog, W * 2);
   rc |= b->alloc(b->nacc, W);
+  rc |= b->alloc(b->narb, W);
   rc |= b->alloc(b->sweep, 2);
   rc |= b->alloc(b->G, W * N * 3);
   rc |= b->alloc(b->L, W * N);
@@ -1027,6 +1052,10 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
   rc |= b->alloc(b->lu_ws, W * 2 * 2 * n * n);
   rc |= b->alloc(p8, es * W * 2 * n * 5 * s->npad);
   b->spo = p8;
+  if (precision == QMCX_FP32) {
+    rc |= b->alloc(b->A64, W * 2 * n * n);
+    rc |= b->alloc(b->spo64, W * 2 * n * (size_t)s->npad);
+  }
   rc |= b->alloc(b->prop, W * 2 * n * kProp);
   rc |= b->alloc(b->ld_fresh, W);
   rc |= b->alloc(b->ph_fresh, W);
@@ -1217,6 +1246,21 @@ int qmcx_get_accepts(qmcx_batch_t b, long long* naccept) {
   std::vector<unsigned long long> tmp(b->W);
   HIPCHK(hipMemcpy(tmp.data(), b->nacc, sizeof(unsigned long long) * b->W, hipMemcpyDeviceToHost));
   for (int i = 0; i < b->W; ++i) naccept[i] = (long long)tmp[i];
+  return 0;
+}
+
+int qmcx_batch_set_accept_band(qmcx_batch_t b, double band) {
+  if (!b || !(band >= 0.0) || band > 1.0) return fail(QMCX_ERR_ARG, "band must be in [0, 1]");
+  b->band = band;
+  return 0;
+}
+
+int qmcx_get_arbitrations(qmcx_batch_t b, long long* narb) {
+  if (!b || !narb) return fail(QMCX_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  std::vector<unsigned long long> tmp(b->W);
+  HIPCHK(hipMemcpy(tmp.data(), b->narb, sizeof(unsigned long long) * b->W, hipMemcpyDeviceToHost));
+  for (int i = 0; i < b->W; ++i) narb[i] = (long long)tmp[i];
   return 0;
 }
 

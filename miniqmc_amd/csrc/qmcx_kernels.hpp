@@ -29,6 +29,8 @@ struct SysP {
   const double* ions;         // [3][M]
   const int* species;         // [M]
   const Functor<T>* fn;       // [kMaxSpecies] J1 then [2] J2
+  const Functor<double>* fn64;  // the same functors in FP64 (FP64 arbitration of the FP32 form)
+  const Cell<double>* cell64;   // the minimum-image cell in FP64
 };
 
 template <typename T>
@@ -54,6 +56,11 @@ struct BatchP {
   unsigned long long* nacc;
   double* trace;   // [W][N][9] or null
   int* sweep;      // [0] global sweep index, [1] index of the first sweep of the current run
+  // FP32 form only (null in FP64): the Slater rows summed in FP64 from the same coefficients,
+  // A64 [W][2][n][n] and at the sweep's proposals spo64 [W][2][n][npad]
+  double* A64;
+  double* spo64;
+  unsigned long long* narb;   // [W] Metropolis decisions settled by the FP64 re-evaluation
 };
 
 // Proposal record of one electron, [W][2][n][kProp] doubles written by k_spo_proposals:
@@ -77,6 +84,9 @@ struct MoveArgs {
   // accepted, else from HBM.
   int chunk, chunk_len, nchunks, inflush, flush_u;
   const int* decide;     // [W] caller accept decisions (flex_acceptrestoreMove) or null
+  // FP32 form, Metropolis: a move whose uniform u lies within band * ratio^2 of ratio^2 is
+  // decided on its FP64 ratio (0: off)
+  double band;
   const void* spo;       // [W][2][n][5][npad] T: SPO rows at the proposals (k_spo_proposals)
   const double* prop;    // [W][2][n][kProp] proposal records (k_spo_proposals)
 };
@@ -101,9 +111,11 @@ __device__ inline void knot_weight(int i, T t, T& w, T& dw, T& d2w) {
   d2w = (T(6) * p3 * t + T(2) * p2) * s6;
 }
 
-template <typename T, int NO, int PER>
+// V64: the value is also summed in FP64 (weights from the FP64 fraction, the same loaded
+// coefficients) into part64 [S][npad]: the FP64 Slater rows of the FP32 form.
+template <typename T, int NO, int PER, bool V64 = false>
 __device__ inline void spline_gather(const SysP<T>& S, const double* spoG, double x, double y, double z, T* part,
-                                     int tid) {
+                                     int tid, double* part64 = nullptr) {
   using VT = typename V16<T>::type;
   constexpr int VW = V16<T>::n;
   constexpr int SL = 16 / PER;
@@ -125,6 +137,13 @@ __device__ inline void spline_gather(const SysP<T>& S, const double* spoG, doubl
   for (int o = 0; o < NO; ++o)
 #pragma unroll
     for (int e = 0; e < VW; ++e) acc[o][e] = 0;
+  double c64[4], acc64[VW];
+  if constexpr (V64) {
+    double dc64[4], d2c64[4];
+    bspline_weights<double>(tt[2], c64, dc64, d2c64);
+#pragma unroll
+    for (int e = 0; e < VW; ++e) acc64[e] = 0.0;
+  }
 #pragma unroll
   for (int g0 = 0; g0 < PER; g0 += G) {
   VT v4[G][4];
@@ -143,11 +162,20 @@ __device__ inline void spline_gather(const SysP<T>& S, const double* spoG, doubl
     knot_weight<T>(p & 3, ty, bj, dbj, d2bj);
     const T w00 = ai * bj, w10 = dai * bj, w01 = ai * dbj, w20 = d2ai * bj;
     const T w11 = dai * dbj, w02 = ai * d2bj;
+    double w00d = 0.0;
+    if constexpr (V64) {
+      double a64, da64, d2a64, b64, db64, d2b64;
+      knot_weight<double>(p >> 2, tt[0], a64, da64, d2a64);
+      knot_weight<double>(p & 3, tt[1], b64, db64, d2b64);
+      w00d = a64 * b64;
+    }
 #pragma unroll
     for (int e = 0; e < VW; ++e) {
       const T q0 = v4[pp][0][e], q1 = v4[pp][1][e], q2 = v4[pp][2][e], q3 = v4[pp][3][e];
       const T s0 = c[0] * q0 + c[1] * q1 + c[2] * q2 + c[3] * q3;
       acc[0][e] += w00 * s0;
+      if constexpr (V64)
+        acc64[e] += w00d * (c64[0] * (double)q0 + c64[1] * (double)q1 + c64[2] * (double)q2 + c64[3] * (double)q3);
       if constexpr (NO >= 5) {
         const T s1 = dc[0] * q0 + dc[1] * q1 + dc[2] * q2 + dc[3] * q3;
         acc[1][e] += w10 * s0;
@@ -170,16 +198,30 @@ __device__ inline void spline_gather(const SysP<T>& S, const double* spoG, doubl
   for (int o = 0; o < NO; ++o)
 #pragma unroll
     for (int e = 0; e < VW; ++e) part[((size_t)s * NO + o) * S.npad + q * VW + e] = acc[o][e];
+  if constexpr (V64)
+#pragma unroll
+    for (int e = 0; e < VW; ++e) part64[(size_t)s * S.npad + q * VW + e] = acc64[e];
 }
 
-template <typename T, int NO>
-__device__ inline void spline_gather_rt(const SysP<T>& S, double x, double y, double z, int slices, T* part) {
+template <typename T, int NO, bool V64 = false>
+__device__ inline void spline_gather_rt(const SysP<T>& S, double x, double y, double z, int slices, T* part,
+                                        double* part64 = nullptr) {
   switch (slices) {
-    case 16: spline_gather<T, NO, 1>(S, S.spoG, x, y, z, part, threadIdx.x); break;
-    case 8: spline_gather<T, NO, 2>(S, S.spoG, x, y, z, part, threadIdx.x); break;
-    case 4: spline_gather<T, NO, 4>(S, S.spoG, x, y, z, part, threadIdx.x); break;
-    case 2: spline_gather<T, NO, 8>(S, S.spoG, x, y, z, part, threadIdx.x); break;
-    default: spline_gather<T, NO, 16>(S, S.spoG, x, y, z, part, threadIdx.x); break;
+    case 16: spline_gather<T, NO, 1, V64>(S, S.spoG, x, y, z, part, threadIdx.x, part64); break;
+    case 8: spline_gather<T, NO, 2, V64>(S, S.spoG, x, y, z, part, threadIdx.x, part64); break;
+    case 4: spline_gather<T, NO, 4, V64>(S, S.spoG, x, y, z, part, threadIdx.x, part64); break;
+    case 2: spline_gather<T, NO, 8, V64>(S, S.spoG, x, y, z, part, threadIdx.x, part64); break;
+    default: spline_gather<T, NO, 16, V64>(S, S.spoG, x, y, z, part, threadIdx.x, part64); break;
+  }
+}
+
+// the FP64 value row: slice partials summed in order, out[j] for j < norb
+template <typename T>
+__device__ inline void spline_finish64(const SysP<T>& S, int slices, const double* part64, double* out, int norb) {
+  for (int j = threadIdx.x; j < norb; j += blockDim.x) {
+    double sum = 0;
+    for (int s = 0; s < slices; ++s) sum += part64[(size_t)s * S.npad + j];
+    out[j] = sum;
   }
 }
 
@@ -875,14 +917,21 @@ __global__ __launch_bounds__(1024) void k_spo_rows(SysP<T> S, BatchP<T> B, int s
   extern __shared__ __align__(16) unsigned char smem[];
   T* part = (T*)smem;
   T* out = part + (size_t)slices * 5 * S.npad;  // [5][n]
+  double* part64 = (double*)(smem + (((sizeof(T) * ((size_t)slices * 5 * S.npad + 5 * (size_t)S.n)) + 15) & ~(size_t)15));
   const int e = blockIdx.x, w = blockIdx.y, N = S.N, n = S.n;
   const double* R = B.R + (size_t)w * 3 * N;
-  spline_gather_rt<T, 5>(S, R[e], R[N + e], R[2 * N + e], slices, part);
-  __syncthreads();
-  spline_finish<T, 5, T>(S, slices, part, out, n, n);
-  __syncthreads();
   const int det = e < n ? 0 : 1, row = e - det * n;
   const size_t moff = ((size_t)w * 2 + det) * n * n;
+  if (B.A64 != nullptr) {
+    spline_gather_rt<T, 5, true>(S, R[e], R[N + e], R[2 * N + e], slices, part, part64);
+    __syncthreads();
+    spline_finish64<T>(S, slices, part64, B.A64 + moff + (size_t)row * n, n);
+  } else {
+    spline_gather_rt<T, 5>(S, R[e], R[N + e], R[2 * N + e], slices, part);
+    __syncthreads();
+  }
+  spline_finish<T, 5, T>(S, slices, part, out, n, n);
+  __syncthreads();
   T* Arow = B.A + moff + (size_t)row * n;
   T* dp = B.dpsiM + ((size_t)w * 2 + det) * (size_t)n * 3 * n + (size_t)row * 3 * n;
   T* d2 = B.d2psiM + moff + (size_t)row * n;
@@ -927,8 +976,15 @@ __global__ __launch_bounds__(1024) void k_spo_proposals(SysP<T> S, BatchP<T> B, 
     d[2] = sqrt_tau * nr[2];
   }
   const double x = R[iel] + d[0], y = R[N + iel] + d[1], z = R[2 * N + iel] + d[2];
-  spline_gather_rt<T, 5>(S, x, y, z, slices, part);
-  __syncthreads();
+  if (B.spo64 != nullptr) {
+    double* part64 = (double*)(smem + ((sizeof(T) * (size_t)slices * 5 * S.npad + 15) & ~(size_t)15));
+    spline_gather_rt<T, 5, true>(S, x, y, z, slices, part, part64);
+    __syncthreads();
+    spline_finish64<T>(S, slices, part64, B.spo64 + (((size_t)w * 2 + det) * n + el) * S.npad, n);
+  } else {
+    spline_gather_rt<T, 5>(S, x, y, z, slices, part);
+    __syncthreads();
+  }
   T* out = spo + (((size_t)w * 2 + det) * n + el) * 5 * S.npad;
   spline_finish<T, 5, T>(S, slices, part, out, S.npad, n);
   __syncthreads();

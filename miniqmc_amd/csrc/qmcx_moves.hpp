@@ -32,8 +32,12 @@ __host__ __device__ inline size_t flush_own_lds(int n, int K) {
 }
 
 struct MoveLds {
-  unsigned oR, oRed, oInv, oPsi, oBinv, oP, oPart, oRows, oAl, oJ1n, oFn, oCell, oProp, oJ1o, oJt, oJ2, oAc, oQ, oAcc, total;
+  unsigned oR, oRed, oInv, oPsi, oBinv, oP, oPart, oRows, oAl, oJ1n, oFn, oCell, oProp, oJ1o, oJt, oJ2, oAc, oQ, oAcc,
+      oArb, total;
 };
+// LDS of the FP64 re-evaluation of a near-tie decision (FP32 form, K > 1): three n-vectors and
+// 2K + 32 reduction slots
+__host__ __device__ inline size_t arb_lds(int n, int K) { return sizeof(double) * (3 * (size_t)n + 2 * (size_t)K + 32); }
 __host__ __device__ inline unsigned lds_take(unsigned& o, size_t bytes) {
   unsigned r = o;
   o = (unsigned)((o + bytes + 15) & ~size_t(15));
@@ -70,9 +74,12 @@ __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E) {
   L.oJt = lds_take(o, sizeof(T) * 5 * N);
   // the in-kernel flush (FP32) reuses the regions from oQ on, grown to its size when the
   // workgroup's LDS allows
+  L.oArb = o;   // during the moves the region past oJt belongs to the FP64 re-evaluation
   if (cache && sizeof(T) == 4) {
     const size_t need = L.oQ + flush_own_lds(n, K);
     if (need > o && need <= kMoveLdsMax) o = (unsigned)need;
+    const size_t arb = L.oArb + arb_lds(n, K);
+    if (arb > o) o = (unsigned)arb;
   }
   L.total = o;
   return L;
@@ -396,6 +403,114 @@ __device__ __attribute__((noinline)) void flush_own(int n, int K, int KP, int c,
   __syncthreads();   // the next chunk reads these rows
 }
 
+// ---- FP64 re-evaluation of a near-tie Metropolis decision (FP32 form, K > 1).
+// The determinant ratio of the move, psiV64 . x with A64 x = e_row (A64: the FP64 Slater rows
+// at the current positions, the chunk's accepted rows taken from their proposals' FP64 values),
+// by iterative refinement from the FP32 path's inverse row x0:
+//   s = e_row - A64 x,   x += X s,
+// where X is the walker's current FP32 inverse: the chunk-start inverse Ainv (psiM layout,
+// X s = sum_i s_i Ainv[i]) with the chunk's delayed Woodbury terms (DelayedUpdate.h:176-193):
+//   X s = sigma + sum_d t_d V_d,  sigma = sum_i s_i Ainv[i],
+//   t = -Binv^T q,  q_e = sigma . U_e - s[rows_e]
+// (U_e the new FP32 Slater rows, V_d the stale inverse rows, Binv the capacitance inverse).  It
+// stops when the ratio's last correction is below 1e-14 of it.  xv holds x0 on entry; xv, sv, tv
+// are n doubles of LDS, red 2K + 32.  Every lane returns the ratio.
+constexpr int kArbMaxIter = 8;
+__device__ __attribute__((noinline)) double arb_det_ratio(int n, int npad, int row, int r0, int c, int KP, int sa,
+                                                          const float* Ainv, const double* A64, const double* s64c,
+                                                          const float* spoc, const float* sAc, const double* sBinv,
+                                                          const int* sRows, const int* sAl, const double* psiv64,
+                                                          double* xv, double* sv, double* tv, double* red, int K) {
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
+  double* q = red;           // [K]
+  double* t = red + K;       // [K]
+  double* bs = red + 2 * K;  // block_sum scratch
+  double rho = 0.0;
+  for (int it = 0; it < kArbMaxIter; ++it) {
+    // residual, a wave per row
+    for (int i = wid; i < n; i += nw) {
+      const double* ar = A64 + (size_t)i * n;
+      const int ei = i - r0;
+      for (int d = 0; d < c; ++d)
+        if (sAl[d] == ei) ar = s64c + (size_t)ei * npad;
+      double acc = 0.0;
+      for (int j = lane; j < n; j += 64) acc += ar[j] * xv[j];
+      acc = wave_sum(acc);
+      if (lane == 0) sv[i] = (i == row ? 1.0 : 0.0) - acc;
+    }
+    __syncthreads();
+    // sigma = sum_i s_i Ainv[i]
+    if (tid < n) {
+      double sg = 0.0;
+#pragma unroll 8
+      for (int i = 0; i < n; ++i) sg += sv[i] * (double)Ainv[(size_t)i * n + tid];
+      tv[tid] = sg;
+    }
+    __syncthreads();
+    for (int e = wid; e < c; e += nw) {
+      const float* u = spoc + (size_t)(sRows[e] - r0) * 5 * npad;
+      double acc = 0.0;
+      for (int j = lane; j < n; j += 64) acc += tv[j] * (double)u[j];
+      acc = wave_sum(acc);
+      if (lane == 0) q[e] = acc - sv[sRows[e]];
+    }
+    __syncthreads();
+    for (int d = tid; d < c; d += nt) {
+      double acc = 0.0;
+      for (int e = 0; e < c; ++e) acc += q[e] * sBinv[e * KP + d];
+      t[d] = -acc;
+    }
+    __syncthreads();
+    double v[2] = {0.0, 0.0};
+    if (tid < n) {
+      double dl = tv[tid];
+      for (int d = 0; d < c; ++d) dl += t[d] * (double)sAc[(size_t)(sRows[d] - r0) * sa + tid];
+      const double xn = xv[tid] + dl;
+      xv[tid] = xn;
+      const double pv = psiv64[tid];
+      v[0] = pv * dl;
+      v[1] = pv * xn;
+    }
+    block_sum<double, 2>(v, bs);
+    rho = v[1];
+    if (fabs(v[0]) <= 1e-14 * fabs(rho)) break;
+  }
+  return rho;
+}
+
+// J1 and J2 ratio of moving electron iel from (x0, y0, z0) to (nx, ny, nz) in FP64 from scratch
+// (OneBodyJastrow.h:259-268, TwoBodyJastrow.h:294-304): exp(sum_I u(old) - u(new) + sum_j ...).
+// Every lane returns it; red: 64 doubles of LDS.
+template <typename T>
+__device__ __attribute__((noinline)) double arb_jastrow_ratio(const SysP<T>& S, const double* sR, int iel, double x0,
+                                                              double y0, double z0, double nx, double ny, double nz,
+                                                              double* red) {
+  const int tid = threadIdx.x, nt = blockDim.x, N = S.N, n = S.n, M = S.M;
+  const Cell<double>& C = *S.cell64;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  double dx, dy, dz;
+  for (int ci = tid; ci < M; ci += nt) {
+    const Functor<double>& f = S.fn64[S.species[ci]];
+    if (f.ncoef == 0) continue;
+    const double ix = S.ions[ci], iy = S.ions[M + ci], iz = S.ions[2 * M + ci];
+    const double ro = min_image(C, ix - x0, iy - y0, iz - z0, -1.0, dx, dy, dz);
+    const double rn = min_image(C, ix - nx, iy - ny, iz - nz, -1.0, dx, dy, dz);
+    if (ro < f.rcut) v[0] += functor_v(f, ro);
+    if (rn < f.rcut) v[1] += functor_v(f, rn);
+  }
+  const bool up = iel < n;
+  for (int j = tid; j < N; j += nt) {
+    if (j == iel) continue;
+    const Functor<double>& f = S.fn64[kMaxSpecies + (((j < n) == up) ? 0 : 1)];
+    const double ro = min_image(C, sR[j] - x0, sR[N + j] - y0, sR[2 * N + j] - z0, -1.0, dx, dy, dz);
+    const double rn = min_image(C, sR[j] - nx, sR[N + j] - ny, sR[2 * N + j] - nz, -1.0, dx, dy, dz);
+    if (ro < f.rcut) v[2] += functor_v(f, ro);
+    if (rn < f.rcut) v[3] += functor_v(f, rn);
+  }
+  block_sum<double, 4>(v, red);
+  return exp((v[0] - v[1]) + (v[2] - v[3]));
+}
+
 // KD: delayed updates (K > 1: inverse-row cache, Woodbury bordering) or Sherman-Morrison (K = 1)
 template <typename T, int JPT, bool KD>
 __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs a) {
@@ -425,6 +540,9 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   T* sAc = (T*)(smem + L.oAc);
   const int sa = ac_stride<T>(n);   // its row stride
   double* sQ = (double*)(smem + L.oQ);
+  // FP64 re-evaluation of near ties (FP32 form, Metropolis): the move's inverse row in FP64
+  const bool arb_on = sizeof(T) == 4 && KD && a.band > 0.0 && a.mode == 1 && a.decide == nullptr && n <= nt;
+  double* sArbX = (double*)(smem + L.oArb);
 #ifdef QMCX_PHASE_TIMING
   unsigned long long tick_ = wall_clock64();
   const unsigned long long wstart_ = tick_;
@@ -452,7 +570,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   // sums over the launch kept by thread 0 in LDS: log|det| (logd, and the running product pdet
   // folded into it before it leaves the double range), phase, J1 and J2 log changes, accepts
   double* sAcc = (double*)(smem + L.oAcc);
-  enum { aLogd, aPdet, aPh, aJl1, aJl2, aNacc };
+  enum { aLogd, aPdet, aPh, aJl1, aJl2, aNacc, aNarb };
   if (tid == 0) {
     sAcc[aLogd] = 0;
     sAcc[aPdet] = 1;
@@ -460,6 +578,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     sAcc[aJl1] = 0;
     sAcc[aJl2] = 0;
     sAcc[aNacc] = 0;
+    sAcc[aNarb] = 0;
   }
   const Cell<T>& C = *sCell;   // read from LDS where used (a register copy costs spills)
   const int jr = tid < n ? tid : n - 1;
@@ -580,6 +699,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       }
       vinv = (T)v;
       sInv[tid] = vinv;
+      if (arb_on) sArbX[tid] = v;
       gd0 = (double)vinv * (double)dp0;
       gd1 = (double)vinv * (double)dp1;
       gd2 = (double)vinv * (double)dp2;
@@ -658,10 +778,25 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     QMCX_TICK(3)
 
     // ---- phase 4: acceptance and updates
-    const double rdet = red[3];
-    const double ratio = rdet * exp((double)(Vat_old - (T)pr[kPropJ1]) + (double)(Uat_old - (T)red[7]));
+    double rdet = red[3];
+    double ratio = rdet * exp((double)(Vat_old - (T)pr[kPropJ1]) + (double)(Uat_old - (T)red[7]));
     const double uacc = a.decide == nullptr ? pr[kPropU] : flex_u;
-    const bool accept = a.mode == 0 ? (uacc < 0.5) : (uacc < ratio * ratio);
+    bool accept = a.mode == 0 ? (uacc < 0.5) : (uacc < ratio * ratio);
+    if constexpr (sizeof(T) == 4 && KD) {
+      // a uniform within the band around ratio^2: the decision is taken on the FP64 ratio
+      if (arb_on && fabs(uacc - ratio * ratio) <= a.band * ratio * ratio) {
+        double* sv = sArbX + n;
+        double* tv = sv + n;
+        double* ared = tv + n;
+        const size_t pro = ((size_t)w * 2 + a.det) * n + r0;   // the chunk's first proposal
+        rdet = arb_det_ratio(n, S.npad, row, r0, c, KP, sa, (const float*)Ainv, B.A64 + moff,
+                             B.spo64 + pro * S.npad, (const float*)spo, (const float*)sAc, sBinv, sRows, sAl,
+                             B.spo64 + (pro + e) * S.npad, sArbX, sv, tv, ared, K);
+        ratio = rdet * arb_jastrow_ratio<T>(S, sR, iel, x0, y0, z0, nx, ny, nz, ared);
+        accept = uacc < ratio * ratio;
+        if (tid == 0) sAcc[aNarb] += 1.0;
+      }
+    }
     if (B.trace != nullptr && tid == 0) {
       double* t = B.trace + ((size_t)w * N + iel) * 9;
       t[0] = ratio;
@@ -913,6 +1048,15 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       }
     }
   }
+  // the FP32 form's FP64 Slater rows of the accepted moves, from their proposals' FP64 values
+  if (B.A64 != nullptr) {
+    const double* s64 = B.spo64 + (((size_t)w * 2 + a.det) * n + r0) * S.npad;
+    double* A64 = B.A64 + moff;
+    for (int i = tid; i < ca * n; i += nt) {
+      const int d = i / n, j = i - d * n;
+      A64[(size_t)(r0 + sAl[d]) * n + j] = s64[(size_t)sAl[d] * S.npad + j];
+    }
+  }
   // the chunk's delayed-update state for a flush launch (none when the kernel applies it)
   if (tid == 0) B.ndelay[slot] = KD && !a.inflush ? c : 0;
   if (KD && !a.inflush) {
@@ -946,6 +1090,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     B.jlog[w * 2 + 0] += sAcc[aJl1];
     B.jlog[w * 2 + 1] += sAcc[aJl2];
     B.nacc[w] += (unsigned long long)sAcc[aNacc];
+    if (B.narb != nullptr) B.narb[w] += (unsigned long long)sAcc[aNarb];
   }
   QMCX_TICK_FLUSH()
 }

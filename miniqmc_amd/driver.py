@@ -9,10 +9,21 @@ At the end it prints a flat timer table and the reference's three throughput lin
 (miniqmc.cpp:509-523), and writes info_<na>_<nb>_<nc>.xml with the same elements
 (resources/hardware, timing, particles, run/driver; miniqmc.cpp:525-551).
 
+Timers carry the reference's names (miniqmc.cpp:137-147, WaveFunction.cpp:38-41) and its flat-profile
+line format (TimerManagerClass::print_flat, NewTimer.cpp:243-270).  Host wall clock: Total, Setup,
+Initialization (evaluateLog from scratch), Diffusion, Pseudopotential.  Device time of the sweep's
+stages from HIP events, under the reference scope each one does the work of:
+  * "New Gradient"      the SPO evaluation at every proposal (most of ratioGrad's time in the
+                        reference) and the J1 terms there;
+  * "Update"            the fused move kernel: evalGrad and ratioGrad's dot products, the J2 rows,
+                        acceptMove and the delayed-update bookkeeping ("Current Gradient", "Set
+                        active", "Make move" and "Accept move" have no separate kernel);
+  * "Complete Updates"  delayed-update flush launches and the FP64 inverse refresh;
+  * "Kinetic Energy"    evaluateGL.
+
 Differences from the reference, by design:
   * every walker runs in one batch on the GPU, so there is one timer per stage instead of one per
-    walker thread, and the device stages come from HIP events (move, flush, recompute, evaluateGL,
-    proposals);
+    walker thread;
   * acceptance: the reference's rule u < r with r = 0.5 (``-r`` only accepts 0.5), or Metropolis;
   * one random sphere per walker and pseudopotential evaluation (each mover rotates its own);
   * systems: the tilings with a prepared workload, -g "1 1 1" (NiO-32) and -g "1 1 2" (NiO-64).
@@ -61,13 +72,19 @@ def parse_args(argv=None):
     return a
 
 
+# device stages of qmcx_run_sweeps -> the reference timer whose work each one does
+STAGE_TIMERS = {"proposals": "New Gradient", "move": "Update", "flush": "Complete Updates",
+                "recompute": "Complete Updates", "gl": "Kinetic Energy"}
+
+
 def timer_table(timers: dict) -> str:
-    """Flat timer table: name, total seconds, calls, seconds per call (TimerManager::print_flat,
-    Utilities/NewTimer.cpp)."""
-    lines = [f"{'Timer':<40s}  {'Inclusive_time':>14s}  {'Calls':>8s}  {'Time_per_call':>14s}"]
+    """Flat profile in the reference's line format (TimerManagerClass::print_flat,
+    Utilities/NewTimer.cpp:243-270): name, seconds, calls, seconds per call, seconds per thread
+    (one host thread drives the batch)."""
+    lines = ["", "Flat profile"]
     for name in sorted(timers):
         t, n = timers[name]
-        lines.append(f"{name:<40s}  {t:14.4f}  {n:8d}  {t / max(n, 1):14.9f}")
+        lines.append("%-40s  %9.4f  %13d  %16.9f  %12.6f TIMER" % (name, t, n, t / (n + 2.220446049250313e-16), t))
     return "\n".join(lines)
 
 
@@ -105,11 +122,13 @@ def info_xml(path: str, nel: int, nsteps: int, timers: dict) -> None:
 
 def main(argv=None) -> int:
     a = parse_args(argv)
+    t_setup = time.perf_counter()
     sysd = systems.make_system(a.system, seed=a.s)
     prec = qmcx.FP64 if a.fp64 else qmcx.FP32
     dev = qmcx.System(sysd, precisions=(prec,))
     wb = qmcx.WalkerBatch(dev, a.w, delay_rank=a.k, precision=prec)
     wb.set_positions(systems.initial_positions(sysd, a.w, seed=a.s))
+    t_setup = time.perf_counter() - t_setup
     mode = qmcx.ACCEPT_METROPOLIS if a.metropolis else qmcx.ACCEPT_REFERENCE
     pp = nlpp.NonLocalPP(sysd, rmax=a.x, seed=a.s)
     print(f"miniqmc_amd: {sysd.name}, {sysd.nel} electrons, {sysd.norb} orbitals per determinant, {a.w} walkers, "
@@ -121,10 +140,12 @@ def main(argv=None) -> int:
         tt, nn = timers.get(name, (0.0, 0))
         timers[name] = (tt + t, nn + n)
 
+    add("Setup", t_setup)
+
     t0 = time.perf_counter()
     t = time.perf_counter()
     wb.evaluate_log()
-    add("Setup", time.perf_counter() - t)
+    add("Initialization", time.perf_counter() - t)
     wb.set_kernel_timing(True)
     pairs = 0
     for step in range(a.n):
@@ -137,9 +158,9 @@ def main(argv=None) -> int:
         add("Pseudopotential", time.perf_counter() - t)
         pairs += int(cnt.sum())
     add("Total", time.perf_counter() - t0)
-    for name, (ms, n) in wb.kernel_timing().items():
+    for stage, (ms, n) in wb.kernel_timing().items():
         if n:
-            add(f"Diffusion::{name} (device)", ms * 1e-3, int(n))
+            add(STAGE_TIMERS[stage], ms * 1e-3, int(n))
 
     print("================================== ")
     print(timer_table(timers))

@@ -56,6 +56,8 @@ _SIGS = {
     "qmcx_get_trace": [_vp, _dp],
     "qmcx_get_gl": [_vp, _vp, _vp, _vp],
     "qmcx_get_accepts": [_vp, _lp],
+    "qmcx_batch_set_accept_band": [_vp, C.c_double],
+    "qmcx_get_arbitrations": [_vp, _lp],
     "qmcx_get_inverse": [_vp, _i, _i, _dp],
     "qmcx_last_sweep_ms": [_vp, C.POINTER(C.c_double)],
     "qmcx_set_kernel_timing": [_vp, _i],
@@ -291,6 +293,15 @@ class WalkerBatch:
     def accepts(self):
         a = np.zeros(self.nw, dtype=np.int64)
         _check(load().qmcx_get_accepts(self.h, a))
+        return a
+
+    def set_accept_band(self, band):
+        """FP32 Metropolis: decide moves with |u - R^2| <= band R^2 on the FP64 ratio (0: off)."""
+        _check(load().qmcx_batch_set_accept_band(self.h, float(band)))
+
+    def arbitrations(self):
+        a = np.zeros(self.nw, dtype=np.int64)
+        _check(load().qmcx_get_arbitrations(self.h, a))
         return a
 
     def inverse(self, walker, det):

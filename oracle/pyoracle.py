@@ -60,7 +60,7 @@ def lib():
         L.orc_spo_evaluate.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
         L.orc_run_vmc.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_uint64, C.c_double, C.c_int,
                                   C.c_int, _dp, _dp, C.c_void_p, C.c_void_p, C.c_void_p,
-                                  np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), C.c_int]
+                                  np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), C.c_int, C.c_void_p]
         L.orc_run_vmc.restype = C.c_double
         L.orc_evaluate_log.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p]
         L.orc_evaluate_log.restype = C.c_double
@@ -222,15 +222,20 @@ class OracleSystem:
                                        out.reshape(-1))
         return out
 
-    def run_vmc(self, R, nsweeps, seed, tau, mode=1, delay=1, walker0=0, trace=False, nthreads=0):
+    def run_vmc(self, R, nsweeps, seed, tau, mode=1, delay=1, walker0=0, trace=False, nthreads=0,
+                sensitivity=False):
+        """sensitivity (delay 1): per move, the relative sensitivity of the determinant ratio to
+        relative perturbations of the Slater matrix and the new row, [nw][nsweeps][N]."""
         R = _c(R).copy()
         nw, N = R.shape[0], self.sys.nel
         log_init = np.zeros(nw)
         log_out = np.zeros((nw, nsweeps))
         tr = np.zeros((nw, nsweeps, N, 9)) if trace else None
+        sens = np.zeros((nw, nsweeps, N)) if sensitivity else None
         G, L = np.zeros((nw, N, 3)), np.zeros((nw, N))
         nacc = np.zeros(nw, dtype=np.int64)
         secs = lib().orc_run_vmc(self.h, nw, walker0, R.reshape(-1), nsweeps, seed, tau, mode, delay, log_init,
                                  log_out, tr.ctypes.data if trace else None, G.ctypes.data, L.ctypes.data, nacc,
-                                 nthreads)
-        return dict(R=R, log_init=log_init, log=log_out, trace=tr, G=G, L=L, naccept=nacc, seconds=secs)
+                                 nthreads, sens.ctypes.data if sensitivity else None)
+        return dict(R=R, log_init=log_init, log=log_out, trace=tr, G=G, L=L, naccept=nacc, seconds=secs,
+                    sensitivity=sens)

@@ -640,6 +640,7 @@ struct DelayedUpdate {
 struct Determinant {
   int first = 0, n = 0;
   std::vector<double> psiM, dpsiM, d2psiM;  // psiM n x n ; dpsiM n x n x 3 ; d2psiM n x n
+  std::vector<double> row_norm2;            // |A row i|^2 of the Slater matrix (sensitivity only)
   std::vector<double> psiV, dpsiV, d2psiV, inv_row;
   double log_value = 0.0, phase = 0.0, cur_ratio = 1.0;
   int inv_row_id = -1;
@@ -784,6 +785,9 @@ static double evaluate_log(Walker& w) {
       double r[3] = {w.Rx[iat], w.Ry[iat], w.Rz[iat]};
       spo_eval(s, r, &A[(size_t)i * n], &D.dpsiM[3 * (size_t)i * n], &D.d2psiM[(size_t)i * n], scratch);
     }
+    D.row_norm2.assign(n, 0.0);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) D.row_norm2[i] += A[(size_t)i * n + j] * A[(size_t)i * n + j];
     D.log_value = invert_transpose(n, A.data(), D.psiM.data(), D.phase);
     D.eng.c = 0;
     D.inv_row_id = -1;
@@ -958,6 +962,28 @@ static double ratio_grad(Walker& w, int iat, double g[3], double* det_ratio_out,
   return ratio;
 }
 
+// Sensitivity of the move's determinant ratio rho = psiV . c (c = psiM[row], the inverse's column
+// of the row) to relative perturbations of size eps of the Slater matrix A and of psiV:
+// |d rho| <= eps (|r| |A|_F + |psiV|) |c|, r = psiM psiV the ratios of putting psiV in each row.
+// Returned relative to |rho|.  Needs the current inverse (delay 1).
+static double move_sensitivity(Walker& w, int iat) {
+  Determinant& D = det_of(w, iat);
+  const int row = iat - D.first, n = D.n;
+  double r2 = 0, c2 = 0, p2 = 0, a2 = 0;
+  for (int i = 0; i < n; ++i) {
+    const double* pm = &D.psiM[(size_t)i * n];
+    double ri = 0;
+    for (int j = 0; j < n; ++j) ri += pm[j] * D.psiV[j];
+    r2 += ri * ri;
+    a2 += D.row_norm2[i];
+  }
+  for (int j = 0; j < n; ++j) {
+    c2 += D.psiM[(size_t)row * n + j] * D.psiM[(size_t)row * n + j];
+    p2 += D.psiV[j] * D.psiV[j];
+  }
+  return (std::sqrt(r2 * a2) + std::sqrt(p2)) * std::sqrt(c2) / std::fabs(D.cur_ratio);
+}
+
 // WaveFunction::acceptMove + ParticleSet::acceptMove
 static void accept_move(Walker& w, int iat) {
   Determinant& D = det_of(w, iat);
@@ -967,6 +993,11 @@ static void accept_move(Walker& w, int iat) {
   D.log_value += std::log(std::fabs(D.cur_ratio));
   D.eng.accept_row(D.psiM.data(), row, D.psiV.data());
   D.inv_row_id = -1;
+  if (!D.row_norm2.empty()) {
+    double nv = 0;
+    for (int j = 0; j < n; ++j) nv += D.psiV[j] * D.psiV[j];
+    D.row_norm2[row] = nv;
+  }
   std::copy(D.dpsiV.begin(), D.dpsiV.end(), D.dpsiM.begin() + 3 * (size_t)row * n);
   std::copy(D.d2psiV.begin(), D.d2psiV.end(), D.d2psiM.begin() + (size_t)row * n);
   D.cur_ratio = 1.0;
@@ -1213,9 +1244,10 @@ void orc_spo_evaluate(void* h, int npts, const double* r, double* psi, double* d
 //   G, L     optional [nw][N][3], [nw][N] after the last sweep
 //   naccept  [nw]
 // Returns the wall time spent in the sweeps (seconds, excluding evaluateLog).
+//   sens     [nw][nsweeps][N] or null: move_sensitivity of every move (delay 1 only)
 double orc_run_vmc(void* h, int nw, int walker0, double* R, int nsweeps, uint64_t seed, double tau, int mode,
                    int delay, double* log_init, double* log_out, double* trace, double* G, double* L,
-                   long long* naccept, int nthreads) {
+                   long long* naccept, int nthreads, double* sens) {
   const System& s = ((OrcSystem*)h)->sys;
   const int N = s.nel;
   std::vector<Walker> ws(nw);
@@ -1248,6 +1280,7 @@ double orc_run_vmc(void* h, int nw, int walker0, double* R, int nsweeps, uint64_
         make_move(w, iel, displ);
         double gnew[3], dratio;
         const double ratio = ratio_grad(w, iel, gnew, &dratio, scratch);
+        if (sens) sens[((size_t)iw * nsweeps + sw) * N + iel] = delay == 1 ? move_sensitivity(w, iel) : 0.0;
         const bool ok = (mode == 0) ? (uacc < 0.5) : (uacc < ratio * ratio);
         if (ok) {
           accept_move(w, iel);

@@ -29,10 +29,14 @@ def test_throughput_formulas():
 
 
 def test_timer_table_and_info_xml(tmp_path):
-    timers = {"Total": (2.0, 1), "Diffusion": (1.5, 3)}
+    timers = {"Total": (2.0, 1), "Diffusion": (1.5, 3), "New Gradient": (0.5, 6)}
     table = driver.timer_table(timers).splitlines()
-    assert table[0].split()[0] == "Timer" and len(table) == 3
-    assert table[1].split()[:3] == ["Diffusion", "1.5000", "3"]
+    # the reference's flat profile: header, then name (40 columns), time, calls, per call, per thread
+    assert table[1] == "Flat profile" and len(table) == 5
+    assert table[2][:40].strip() == "Diffusion" and table[2].split()[1:3] == ["1.5000", "3"]
+    assert table[3][:40].strip() == "New Gradient" and table[3].endswith(" TIMER")
+    assert set(driver.STAGE_TIMERS.values()) <= {"New Gradient", "Update", "Complete Updates", "Kinetic Energy"}
+    timers = {"Total": (2.0, 1), "Diffusion": (1.5, 3)}
     path = tmp_path / "info_1_1_2.xml"
     driver.info_xml(str(path), 768, 3, timers)
     root = ET.parse(path).getroot()

@@ -1,0 +1,124 @@
+"""The headline configuration, NiO-64 (768 e-, 384 orbitals, 64^3 FP32 table), against the FP64
+oracle: 2 walkers, one sweep, both acceptance rules (the reference driver's u < 0.5,
+miniqmc.cpp:445, and Metropolis), the batched step of miniqmc_sync_move.cpp:459-513.
+
+FP64 delay 1 must reproduce the oracle move by move (ratio 1e-9, gradients 1e-7, log|Psi| 1e-11).
+
+FP32 delay 32 tolerances are scaled to the conditioning of each move.  The oracle reports, per
+move, s_m = (|r| |A|_F + |psiV|) |c| / |rho|: the first-order bound on the relative change of the
+determinant ratio rho = psiV . c under relative perturbations eps of the Slater matrix A and the
+new row psiV (c = the inverse's column of the moving row, r = psiM psiV).  FP32 evaluation and FP32
+Woodbury updates perturb at eps ~ a few 2^-24, and an inverse carries what it picked up at earlier
+moves until the FP64 refresh, so move m is held to
+    |rho32 - rho64| / |rho64| <= 4 * 2^-24 * max(s_1 .. s_m)   (and <= 1e-3 for s up to 1e3).
+Gradients are held to the same relative bound times (1 + |g|).  log|Psi| is compared twice: before
+the FP64 refresh (the accumulated log of the accepted FP32 ratios, within the sum of the accepted
+moves' bounds plus 1e-4 for the FP32 Jastrow sums) and after it (FP64 LU of the FP64 Slater rows,
+FP32 Jastrow sums: 1e-7 relative).  Under Metropolis the accept sequence must be identical: near ties are decided on
+the FP64 ratio (qmcx_batch_set_accept_band).
+"""
+import numpy as np
+import pytest
+
+from miniqmc_amd import qmcx, systems
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+EPS32 = 2.0 ** -24
+SEED, TAU, NW = 5, 0.3, 2
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def nio64():
+    sysd = systems.make_system("nio64")
+    return sysd, qmcx.System(sysd), po.OracleSystem(sysd)
+
+
+@pytest.fixture(scope="module")
+def oracle_runs(nio64):
+    sysd, _, orc = nio64
+    R0 = systems.initial_positions(sysd, NW, seed=SEED)
+    out = {}
+    for mode in (qmcx.ACCEPT_REFERENCE, qmcx.ACCEPT_METROPOLIS):
+        _say("oracle NiO-64 sweep, accept mode", mode)
+        out[mode] = orc.run_vmc(R0, 1, SEED, TAU, mode=mode, delay=1, trace=True, sensitivity=True)
+    return R0, out
+
+
+def _gpu_sweep(dev, R0, prec, delay, mode, recompute):
+    wb = qmcx.WalkerBatch(dev, NW, delay_rank=delay, precision=prec)
+    wb.set_positions(R0)
+    log0 = wb.evaluate_log()
+    wb.set_trace(True)
+    hist = wb.run(1, SEED, TAU, mode, recompute_every=recompute)
+    G, L, lv = wb.gl()
+    return dict(log0=log0, trace=wb.trace(), log=hist[0], G=G, L=L, R=wb.positions(), acc=wb.accepts(),
+                arb=wb.arbitrations())
+
+
+@pytest.mark.parametrize("mode", [qmcx.ACCEPT_REFERENCE, qmcx.ACCEPT_METROPOLIS])
+def test_nio64_fp64_delay1_vs_oracle(nio64, oracle_runs, mode):
+    sysd, dev, _ = nio64
+    R0, runs = oracle_runs
+    ref = runs[mode]
+    _say("NiO-64 FP64 delay 1, accept mode", mode)
+    got = _gpu_sweep(dev, R0, qmcx.FP64, 1, mode, 0)
+    np.testing.assert_allclose(got["log0"], ref["log_init"], rtol=1e-11)
+    tr, rt = got["trace"], ref["trace"][:, 0]
+    assert np.array_equal(tr[:, :, 2], rt[:, :, 2])
+    rel = np.abs(tr[:, :, 0] - rt[:, :, 0]) / np.maximum(np.abs(rt[:, :, 0]), 1.0)
+    _say(f"  ratio err max {rel.max():.2e}")
+    assert rel.max() < 1e-9
+    gerr = np.abs(tr[:, :, 3:9] - rt[:, :, 3:9]) / (1 + np.abs(rt[:, :, 3:9]))
+    assert gerr.max() < 1e-7, gerr.max()
+    np.testing.assert_allclose(got["log"], ref["log"][:, 0], rtol=1e-11)
+    np.testing.assert_allclose(got["R"], ref["R"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(got["G"], ref["G"], rtol=1e-8, atol=1e-8)
+    np.testing.assert_allclose(got["L"], ref["L"], rtol=1e-8, atol=1e-6)
+    assert np.array_equal(got["acc"], ref["naccept"])
+
+
+@pytest.mark.parametrize("mode", [qmcx.ACCEPT_REFERENCE, qmcx.ACCEPT_METROPOLIS])
+def test_nio64_fp32_delay32_vs_oracle(nio64, oracle_runs, mode):
+    sysd, dev, _ = nio64
+    R0, runs = oracle_runs
+    ref = runs[mode]
+    rt = ref["trace"][:, 0]
+    sens = ref["sensitivity"][:, 0]
+    N, n = sysd.nel, sysd.nup
+    # running maximum of the sensitivity within each determinant's moves (the inverse is rebuilt
+    # from the FP64 Slater rows before the sweep)
+    smax = np.concatenate([np.maximum.accumulate(sens[:, :n], axis=1),
+                           np.maximum.accumulate(sens[:, n:], axis=1)], axis=1)
+    bound = 4 * EPS32 * smax
+    for recompute in (0, 1):
+        _say("NiO-64 FP32 delay 32, accept mode", mode, "refresh after the sweep", bool(recompute))
+        got = _gpu_sweep(dev, R0, qmcx.FP32, 32, mode, recompute)
+        tr = got["trace"]
+        assert np.array_equal(tr[:, :, 2], rt[:, :, 2]), "accept/reject sequence differs"
+        rel = np.abs(tr[:, :, 0] - rt[:, :, 0]) / np.abs(rt[:, :, 0])
+        q = rel / bound
+        w, m = np.unravel_index(np.argmax(q), q.shape)
+        _say(f"  ratio err max {rel.max():.2e}, median {np.median(rel):.2e}; worst err/bound {q.max():.3f} "
+             f"(walker {w}, electron {m}, err {rel[w, m]:.2e}, sensitivity {sens[w, m]:.2e}); "
+             f"arbitrations {got['arb'].tolist()}")
+        assert np.all(rel <= np.maximum(bound, 1e-12)), (w, m, rel[w, m], bound[w, m])
+        assert rel[smax <= 1e3].max(initial=0.0) < 1e-3
+        g = rt[:, :, 3:9]
+        gerr = np.abs(tr[:, :, 3:9] - g) / (1 + np.abs(g))
+        assert np.all(gerr <= np.maximum(bound, 1e-6)[:, :, None] * 10), gerr.max()
+        if recompute:
+            lerr = np.abs(got["log"] - ref["log"][:, 0]) / np.abs(ref["log"][:, 0])
+            _say(f"  log|Psi| after the FP64 refresh: rel err {lerr.max():.2e}")
+            assert lerr.max() < 1e-7
+        else:
+            acc = rt[:, :, 2] > 0
+            allowed = (bound * acc).sum(axis=1) + 1e-4
+            dlog = np.abs(got["log"] - ref["log"][:, 0])
+            _say(f"  log|Psi| before any refresh: |dlog| {dlog.tolist()}, allowed {allowed.tolist()}")
+            assert np.all(dlog <= allowed)
