@@ -36,19 +36,19 @@ def log(*a):
 
 
 def path_bytes_per_move(sysd, k, acc, s=4):
-    """Algorithmic HBM bytes per walker-move of the whole path (every kernel of a sweep).
-
-    Spline stencil 64*Np*s, positions 3(N+M)s, Jastrow state a*2*5*N*s, and the
-    determinant row work s*n*(1+2c) + a*2*n*s with c the mean number of delayed
-    rows seen by a move: moves inside a chunk of k electrons see on average
-    a*(k-1)/2 delayed rows (the flush's 2*n^2*s/k per move is not counted).
+    """Algorithmic HBM bytes per walker-move of the whole path (every kernel of a sweep), the
+    compulsory-traffic model of the path: spline stencil 64*Np*s, positions 3(N+M)s, Jastrow
+    state a*2*5*N*s, and the determinant s*n*(1 + 2c) + a*2*n*s + a*2*n^2*s/k with c = k/2 the
+    mean delay count a move sees (0 for k = 1): the inverse row and the U/V panels, the U/V
+    row writes of an accept, and one read and write of the inverse per flush (per accept with
+    Sherman-Morrison).  NiO-64 FP32, k = 32, a = 0.5: 194,304 B.
     """
     n, N, M, Np = sysd.nup, sysd.nel, sysd.nions, sysd.npad
-    cbar = acc * (k - 1) / 2.0 if k > 1 else 0.0
+    cbar = k / 2.0 if k > 1 else 0.0
     b_spline = 64 * Np * s
     b_pos = 3 * (N + M) * s
     b_j = acc * 2 * 5 * N * s
-    b_det = s * n * (1 + 2 * cbar) + acc * 2 * n * s
+    b_det = s * n * (1 + 2 * cbar) + acc * 2 * n * s + acc * 2 * n * n * s / k
     return b_spline + b_pos + b_j + b_det, dict(spline=b_spline, pos=b_pos, jastrow=b_j, det=b_det, cbar=cbar)
 
 
@@ -118,29 +118,67 @@ def measure_leg(dev, sysd, R0, k, tile, mode, steps, warmup, fp64_steps, err_ste
         b.set_positions(R0)
         b.evaluate_log()
         b.set_trace(True)
+    # the same start without the FP64 refresh: log|Psi| accumulated from the FP32 ratios alone
+    bnr = qmcx.WalkerBatch(dev, W, delay_rank=k, precision=qmcx.FP32, tile=tile)
+    bnr.set_positions(R0)
+    bnr.evaluate_log()
+    hnr = bnr.run(1, seed, tau, mode, recompute_every=0)
+    bnr.close()
     flipped = np.zeros(W, dtype=bool)
     flips = 0
-    ratio_err = []
+    ratio_err, grad_err, chunk_acc = [], [], []
     lerr_sweeps = []
-    for _ in range(err_steps):
+    n = sysd.nup
+    for sw in range(err_steps):
         h32 = b32.run(1, seed, tau, mode, recompute_every=recompute)
         h64 = b64.run(1, seed, tau, mode)
         t32, t64 = b32.trace(), b64.trace()
         f = t32[:, :, 2] != t64[:, :, 2]
         flips += int((f.any(axis=1) & ~flipped).sum())   # first divergence of a walker's chain
         clean_moves = ~(flipped[:, None] | (np.cumsum(f, axis=1) > 0))
-        rel = np.abs(t32[:, :, 0] - t64[:, :, 0]) / np.maximum(np.abs(t64[:, :, 0]), 1.0)
+        rel = np.abs(t32[:, :, 0] - t64[:, :, 0]) / np.abs(t64[:, :, 0])
+        g64 = t64[:, :, 3:9]
+        gerr = (np.abs(t32[:, :, 3:9] - g64) / (1.0 + np.abs(g64))).max(axis=2)
+        # accepted moves in each move's delay-rank chunk (k electrons of one determinant)
+        cid = (np.arange(N) // n) * ((n + k - 1) // k) + (np.arange(N) % n) // k
+        acc = np.stack([np.bincount(cid, weights=t64[w_, :, 2], minlength=cid.max() + 1) for w_ in range(W)])
+        cacc = acc[:, cid]
         ratio_err.append(rel[clean_moves])
+        grad_err.append(gerr[clean_moves])
+        chunk_acc.append(cacc[clean_moves])
         flipped |= f.any(axis=1)
         lerr_sweeps.append(np.abs(h32[0] - h64[0]) / np.abs(h64[0]))
-    acc_rate = float(t64[:, :, 2].mean())
+        if sw == 0:
+            same = ~f.any(axis=1)
+            lerr_norefresh = (np.abs(hnr[0] - h64[0]) / np.abs(h64[0]))[same]
     clean = ~flipped
     lerr = np.array(lerr_sweeps)[:, clean] if clean.any() else np.full((1, 1), np.nan)
-    ratio_err = np.concatenate(ratio_err)
+    ratio_err, grad_err, chunk_acc = np.concatenate(ratio_err), np.concatenate(grad_err), np.concatenate(chunk_acc)
     for b in (b32, b64):
         b.set_trace(False)
+
+    def pct(x):
+        return {"p50": float(np.percentile(x, 50)), "p99": float(np.percentile(x, 99)),
+                "p99.9": float(np.percentile(x, 99.9)), "max": float(x.max()), "moves": int(x.size)}
+
+    bins = {}
+    for name, lo, hi in (("accepts<=16", 0, 16), ("accepts17-24", 17, 24), ("accepts>24", 25, k)):
+        sel = (chunk_acc >= lo) & (chunk_acc <= hi)
+        if sel.any():
+            bins[name] = pct(ratio_err[sel])
+    errors = {"ratio_rel": pct(ratio_err), "grad_rel": pct(grad_err), "ratio_rel_by_chunk_accepts": bins,
+              "log_psi_rel_before_refresh": {"max": float(lerr_norefresh.max()), "mean": float(lerr_norefresh.mean()),
+                                             "walkers": int(lerr_norefresh.size), "sweeps": 1},
+              "note": "FP32 delay-rank form vs the FP64 delay-1 GPU path move by move on walkers whose accept "
+                      "sequence is still identical; ratio error |r32 - r64| / |r64|, gradient error "
+                      "|g32 - g64| / (1 + |g64|) (largest component); the tail is set by moves whose Slater "
+                      "matrix is ill-conditioned (the GPU parity tests bound each move by the oracle's "
+                      "conditioning)"}
     log(f"error leg ({err_steps} sweeps): walkers with identical accept sequences {int(clean.sum())}/{W}, "
-        f"flips {flips}; max rel |dlog| on those {np.nanmax(lerr):.2e}; ratio err max {ratio_err.max():.2e}")
+        f"flips {flips}; max rel |dlog| on those {np.nanmax(lerr):.2e} (before the refresh "
+        f"{lerr_norefresh.max():.2e}); ratio err p50 {errors['ratio_rel']['p50']:.1e} p99 "
+        f"{errors['ratio_rel']['p99']:.1e} max {ratio_err.max():.1e}; by chunk accepts " +
+        ", ".join(f"{b_}: max {v['max']:.1e} ({v['moves']} moves)" for b_, v in bins.items()))
 
     # ---- headline: batched FP32 (FP64 LU recompute every `recompute` sweeps inside the timed region)
     b32.run(warmup, seed, tau, mode, recompute_every=recompute)
@@ -181,7 +219,8 @@ def measure_leg(dev, sysd, R0, k, tile, mode, steps, warmup, fp64_steps, err_ste
         (f", one walker {value64_1:.4e} moves/s" if one_walker else ""))
     b32.close()
     b64.close()
-    return dict(flips=flips, clean=clean, lerr=lerr, ratio_err=ratio_err, wall=wall, kt=kt, moves=moves, value=value,
+    return dict(flips=flips, clean=clean, lerr=lerr, ratio_err=ratio_err, errors=errors, wall=wall, kt=kt, moves=moves,
+                value=value,
                 acc_timed=acc_timed, stages=stages, wall64=wall64, value64=value64, value64_1=value64_1)
 
 
@@ -314,7 +353,7 @@ def main():
                  "steps": args.other_steps, "acceptance": O["acc_timed"],
                  "fp64_delay1_value": O["value64"], "speedup_vs_fp64_delay1": O["value"] / O["value64"],
                  "accept_flips": O["flips"], "walkers_identical_sequence": int(O["clean"].sum()),
-                 "log_psi_err_max_rel": float(np.nanmax(O["lerr"]))}
+                 "log_psi_err_max_rel": float(np.nanmax(O["lerr"])), "errors": O["errors"]}
 
     # ---- rooflines.  The dominant kernel (most of the sweep's time) is the fused move kernel:
     # one workgroup per walker walks its electrons in order, a latency-bound chain that moves
@@ -329,15 +368,21 @@ def main():
     nch = -(-sysd.nup // k) if k > 1 else 0
     inflush = k > 1 and kt["flush"][1] == 0
     flushes = (nch - (0.5 if args.recompute == 1 else 0.0)) if inflush else 0.0
-    bpm = move_kernel_bytes_per_move(sysd, k, acc_timed, per_walker, flushes)
+    bpm_impl = move_kernel_bytes_per_move(sysd, k, acc_timed, per_walker, flushes)
+    # the move kernel's share of the path's algorithmic bytes: everything but the spline stencil,
+    # which the proposals kernel reads
+    bpm = bpm_path - parts["spline"]
     launch_ms = mv_ms / max(mv_n, 1)
     achieved = bpm * W * per_walker / (launch_ms * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(sysd, k, W * per_walker)
     roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=traffic_src,
                     kernel="k_moves<float>", bytes_per_move=round(bpm, 1),
+                    bytes_per_move_kernel_count=round(bpm_impl, 1),
                     moves_per_launch=W * per_walker, flushes_per_launch=flushes,
-                    avg_launch_ms=round(launch_ms, 4))
+                    avg_launch_ms=round(launch_ms, 4),
+                    note="bytes_per_move: the path's algorithmic bytes per move less the spline stencil; "
+                         "bytes_per_move_kernel_count: the kernel's own rows read and written per move")
     pr_ms, pr_n = kt["proposals"]
     pr_launch = pr_ms / max(pr_n, 1)
     pr_units = W * sysd.nup
@@ -411,6 +456,7 @@ def main():
                         "vs": "FP64 delay-1 GPU path (parity-pinned to the FP64 oracle), same start and streams; "
                               "walkers whose accept sequence stayed identical"},
         "ratio_err_max_rel": float(ratio_err.max()),
+        "errors": L["errors"],
         "accept_flips": flips,
         "walkers_identical_sequence": int(clean.sum()),
         "acceptance": acc_timed,

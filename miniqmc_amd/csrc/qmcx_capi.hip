@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,29 @@ struct DevBuf {
   template <typename P>
   P* as() const { return (P*)p; }
 };
+
+// The library's three HIP streams on a device: work of a batch (moves, evaluateGL, stand-alone
+// kernels), the FP64 refresh, and the proposals' SPO rows.  They are created once per device,
+// on the first system created there, and kept until the process ends; every system, batch and
+// DU engine on the device shares them.  The runtime gives each stream a hardware queue while it
+// has queues free and then shares the least-used one (HIP's default limit is four per process,
+// one of them the null stream's): three streams made once get three queues, whereas streams made
+// per batch could land two of a batch's side streams on one queue, where they run one after the
+// other instead of beside the moves.
+constexpr int kStreamDevices = 64;
+std::mutex g_stream_mu;
+hipStream_t g_streams[kStreamDevices][3] = {};
+
+int device_streams(hipStream_t out[3]) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kStreamDevices) return fail(QMCX_ERR_ARG, "device index beyond the stream table");
+  std::lock_guard<std::mutex> lock(g_stream_mu);
+  for (int i = 0; i < 3; ++i)
+    if (g_streams[dev][i] == nullptr) HIPCHK(hipStreamCreate(&g_streams[dev][i]));
+  for (int i = 0; i < 3; ++i) out[i] = g_streams[dev][i];
+  return 0;
+}
 
 // ------------------------------------------------------------------ lattice (host)
 void inv3(const double a[9], double o[9]) {
@@ -389,17 +413,23 @@ int proposal_slices(const qmcx_system_s* s, int want) {
   return nq * 2 <= 1024 ? 2 : 1;
 }
 
-template <typename T, int JPT, bool KD>
+template <typename T, int JPT, bool KD, bool ARB>
 int launch_moves_kd(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
   const qmcx_system_s* s = b->sys;
-  hipLaunchKernelGGL((k_moves<T, JPT, KD>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(), a);
+  hipLaunchKernelGGL((k_moves<T, JPT, KD, ARB>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(),
+                     a);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
+// the near-tie FP64 re-evaluation exists for FP32 delayed updates under Metropolis acceptance
 template <typename T, int JPT>
 int launch_moves_jpt(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
-  return b->K > 1 ? launch_moves_kd<T, JPT, true>(b, nt, lds, a) : launch_moves_kd<T, JPT, false>(b, nt, lds, a);
+  if (b->K == 1) return launch_moves_kd<T, JPT, false, false>(b, nt, lds, a);
+  if constexpr (sizeof(T) == 4)
+    if (a.mode == QMCX_ACCEPT_METROPOLIS && a.band > 0.0 && a.decide == nullptr)
+      return launch_moves_kd<T, JPT, true, true>(b, nt, lds, a);
+  return launch_moves_kd<T, JPT, true, false>(b, nt, lds, a);
 }
 
 template <typename T>
@@ -828,7 +858,11 @@ int qmcx_system_create(qmcx_system_t* out, const double* lattice9, int nions, co
     HIPCHK(hipMemcpy(s->d_cell_d, &cd, sizeof(cd), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&s->d_spoG, sizeof(double) * 9));
   }
-  HIPCHK(hipStreamCreate(&s->stream));
+  {
+    hipStream_t st[3];
+    if (int rc = device_streams(st)) return rc;
+    s->stream = st[0];
+  }
   *out = s;
   return 0;
 }
@@ -844,8 +878,7 @@ int qmcx_system_destroy(qmcx_system_t s) {
   hipFree(s->d_cell_f);
   hipFree(s->d_cell_d);
   hipFree(s->d_spoG);
-  if (s->stream) hipStreamDestroy(s->stream);
-  delete s;
+  delete s;   // its stream is the device's shared one
   return 0;
 }
 
@@ -1063,9 +1096,16 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
     qmcx_batch_destroy(b);
     return QMCX_ERR_NOMEM;
   }
-  HIPCHK(hipStreamCreate(&b->stream));
-  HIPCHK(hipStreamCreate(&b->aux));
-  HIPCHK(hipStreamCreate(&b->pre));
+  {
+    hipStream_t st[3];
+    if (int rc = device_streams(st)) {
+      qmcx_batch_destroy(b);
+      return rc;
+    }
+    b->stream = st[0];
+    b->aux = st[1];
+    b->pre = st[2];
+  }
   for (int d = 0; d < 2; ++d) HIPCHK(hipEventCreateWithFlags(&b->ev_pre[d], hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&b->ev_start, hipEventDisableTiming));
   for (int d = 0; d < 2; ++d) {
@@ -1096,9 +1136,7 @@ int qmcx_batch_destroy(qmcx_batch_t b) {
   if (b->nl_near) hipFree(b->nl_near);
   if (b->nl_items) hipFree(b->nl_items);
   if (b->nl_sph) hipFree(b->nl_sph);
-  if (b->stream) hipStreamDestroy(b->stream);
-  if (b->aux) hipStreamDestroy(b->aux);
-  if (b->pre) hipStreamDestroy(b->pre);
+  // the streams are the device's shared ones and stay
   for (int d = 0; d < 2; ++d)
     if (b->ev_pre[d]) hipEventDestroy(b->ev_pre[d]);
   if (b->ev_start) hipEventDestroy(b->ev_start);
@@ -1597,7 +1635,15 @@ int qmcx_du_create(qmcx_du_t* out, int norb, int delay, int precision) {
     delete h;
     return QMCX_ERR_NOMEM;
   }
-  HIPCHK(hipStreamCreate(&h->stream));
+  {
+    hipStream_t st[3];
+    if (int rc2 = device_streams(st)) {
+      for (void* p : h->allocs) hipFree(p);
+      delete h;
+      return rc2;
+    }
+    h->stream = st[0];
+  }
   *out = h;
   return 0;
 }
@@ -1606,8 +1652,7 @@ int qmcx_du_destroy(qmcx_du_t h) {
   if (!h) return 0;
   if (h->stream) hipStreamSynchronize(h->stream);
   for (void* p : h->allocs) hipFree(p);
-  if (h->stream) hipStreamDestroy(h->stream);
-  delete h;
+  delete h;   // its stream is the device's shared one
   return 0;
 }
 

@@ -416,7 +416,7 @@ __device__ __attribute__((noinline)) void flush_own(int n, int K, int KP, int c,
 // stops when the ratio's last correction is below 1e-14 of it.  xv holds x0 on entry; xv, sv, tv
 // are n doubles of LDS, red 2K + 32.  Every lane returns the ratio.
 constexpr int kArbMaxIter = 8;
-__device__ __attribute__((noinline)) double arb_det_ratio(int n, int npad, int row, int r0, int c, int KP, int sa,
+__device__ inline double arb_det_ratio(int n, int npad, int row, int r0, int c, int KP, int sa,
                                                           const float* Ainv, const double* A64, const double* s64c,
                                                           const float* spoc, const float* sAc, const double* sBinv,
                                                           const int* sRows, const int* sAl, const double* psiv64,
@@ -482,7 +482,7 @@ __device__ __attribute__((noinline)) double arb_det_ratio(int n, int npad, int r
 // (OneBodyJastrow.h:259-268, TwoBodyJastrow.h:294-304): exp(sum_I u(old) - u(new) + sum_j ...).
 // Every lane returns it; red: 64 doubles of LDS.
 template <typename T>
-__device__ __attribute__((noinline)) double arb_jastrow_ratio(const SysP<T>& S, const double* sR, int iel, double x0,
+__device__ inline double arb_jastrow_ratio(const SysP<T>& S, const double* sR, int iel, double x0,
                                                               double y0, double z0, double nx, double ny, double nz,
                                                               double* red) {
   const int tid = threadIdx.x, nt = blockDim.x, N = S.N, n = S.n, M = S.M;
@@ -511,8 +511,10 @@ __device__ __attribute__((noinline)) double arb_jastrow_ratio(const SysP<T>& S, 
   return exp((v[0] - v[1]) + (v[2] - v[3]));
 }
 
-// KD: delayed updates (K > 1: inverse-row cache, Woodbury bordering) or Sherman-Morrison (K = 1)
-template <typename T, int JPT, bool KD>
+// KD: delayed updates (K > 1: inverse-row cache, Woodbury bordering) or Sherman-Morrison (K = 1).
+// ARB: the FP64 re-evaluation of near-tie Metropolis decisions is compiled in (FP32, K > 1); only
+// that instantiation carries its code, whose registers would otherwise spill in every move.
+template <typename T, int JPT, bool KD, bool ARB>
 __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
@@ -541,7 +543,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   const int sa = ac_stride<T>(n);   // its row stride
   double* sQ = (double*)(smem + L.oQ);
   // FP64 re-evaluation of near ties (FP32 form, Metropolis): the move's inverse row in FP64
-  const bool arb_on = sizeof(T) == 4 && KD && a.band > 0.0 && a.mode == 1 && a.decide == nullptr && n <= nt;
+  const bool arb_on = ARB && sizeof(T) == 4 && KD && a.band > 0.0 && a.mode == 1 && a.decide == nullptr && n <= nt;
   double* sArbX = (double*)(smem + L.oArb);
 #ifdef QMCX_PHASE_TIMING
   unsigned long long tick_ = wall_clock64();
@@ -782,7 +784,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     double ratio = rdet * exp((double)(Vat_old - (T)pr[kPropJ1]) + (double)(Uat_old - (T)red[7]));
     const double uacc = a.decide == nullptr ? pr[kPropU] : flex_u;
     bool accept = a.mode == 0 ? (uacc < 0.5) : (uacc < ratio * ratio);
-    if constexpr (sizeof(T) == 4 && KD) {
+    if constexpr (ARB && sizeof(T) == 4 && KD) {
       // a uniform within the band around ratio^2: the decision is taken on the FP64 ratio
       if (arb_on && fabs(uacc - ratio * ratio) <= a.band * ratio * ratio) {
         double* sv = sArbX + n;

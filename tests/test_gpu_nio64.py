@@ -2,7 +2,9 @@
 oracle: 2 walkers, one sweep, both acceptance rules (the reference driver's u < 0.5,
 miniqmc.cpp:445, and Metropolis), the batched step of miniqmc_sync_move.cpp:459-513.
 
-FP64 delay 1 must reproduce the oracle move by move (ratio 1e-9, gradients 1e-7, log|Psi| 1e-11).
+FP64 delay 1 must reproduce the oracle move by move: the same accept sequence and positions, ratios
+within 16 * 2^-53 * max(s_1 .. s_m) (the sensitivity s below; at least 1e-12), gradients within ten
+times that relative to 1 + |g|, log|Psi| within 1e-11 relative.
 
 FP32 delay 32 tolerances are scaled to the conditioning of each move.  The oracle reports, per
 move, s_m = (|r| |A|_F + |psiV|) |c| / |rho|: the first-order bound on the relative change of the
@@ -26,6 +28,7 @@ from oracle import pyoracle as po
 pytestmark = pytest.mark.gpu
 
 EPS32 = 2.0 ** -24
+EPS64 = 2.0 ** -53
 SEED, TAU, NW = 5, 0.3, 2
 
 
@@ -50,6 +53,12 @@ def oracle_runs(nio64):
     return R0, out
 
 
+def _running_sensitivity(sens, n):
+    """Running maximum of the oracle's per-move sensitivity within each determinant's moves."""
+    return np.concatenate([np.maximum.accumulate(sens[:, :n], axis=1),
+                           np.maximum.accumulate(sens[:, n:], axis=1)], axis=1)
+
+
 def _gpu_sweep(dev, R0, prec, delay, mode, recompute):
     wb = qmcx.WalkerBatch(dev, NW, delay_rank=delay, precision=prec)
     wb.set_positions(R0)
@@ -71,11 +80,18 @@ def test_nio64_fp64_delay1_vs_oracle(nio64, oracle_runs, mode):
     np.testing.assert_allclose(got["log0"], ref["log_init"], rtol=1e-11)
     tr, rt = got["trace"], ref["trace"][:, 0]
     assert np.array_equal(tr[:, :, 2], rt[:, :, 2])
-    rel = np.abs(tr[:, :, 0] - rt[:, :, 0]) / np.maximum(np.abs(rt[:, :, 0]), 1.0)
-    _say(f"  ratio err max {rel.max():.2e}")
-    assert rel.max() < 1e-9
+    rel = np.abs(tr[:, :, 0] - rt[:, :, 0]) / np.abs(rt[:, :, 0])
+    # FP64 rounding in both evaluations and in the Sherman-Morrison updates since the sweep began,
+    # scaled like the FP32 bound: 16 * 2^-53 * max(s_1 .. s_m), at least 1e-12
+    bound = np.maximum(16 * EPS64 * _running_sensitivity(ref["sensitivity"][:, 0], sysd.nup), 1e-12)
+    q = rel / bound
+    w, m = np.unravel_index(np.argmax(q), q.shape)
+    _say(f"  ratio err max {rel.max():.2e}, median {np.median(rel):.2e}; worst err/bound {q.max():.3f} "
+         f"(walker {w}, electron {m}, err {rel[w, m]:.2e}, bound {bound[w, m]:.2e})")
+    assert np.all(rel <= bound), (w, m, rel[w, m], bound[w, m])
     gerr = np.abs(tr[:, :, 3:9] - rt[:, :, 3:9]) / (1 + np.abs(rt[:, :, 3:9]))
-    assert gerr.max() < 1e-7, gerr.max()
+    _say(f"  gradient err max {gerr.max():.2e}")
+    assert np.all(gerr <= np.maximum(10 * bound, 1e-9)[:, :, None]), gerr.max()
     np.testing.assert_allclose(got["log"], ref["log"][:, 0], rtol=1e-11)
     np.testing.assert_allclose(got["R"], ref["R"], rtol=0, atol=1e-12)
     np.testing.assert_allclose(got["G"], ref["G"], rtol=1e-8, atol=1e-8)
@@ -93,8 +109,7 @@ def test_nio64_fp32_delay32_vs_oracle(nio64, oracle_runs, mode):
     N, n = sysd.nel, sysd.nup
     # running maximum of the sensitivity within each determinant's moves (the inverse is rebuilt
     # from the FP64 Slater rows before the sweep)
-    smax = np.concatenate([np.maximum.accumulate(sens[:, :n], axis=1),
-                           np.maximum.accumulate(sens[:, n:], axis=1)], axis=1)
+    smax = _running_sensitivity(sens, n)
     bound = 4 * EPS32 * smax
     for recompute in (0, 1):
         _say("NiO-64 FP32 delay 32, accept mode", mode, "refresh after the sweep", bool(recompute))
