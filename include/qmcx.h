@@ -84,6 +84,17 @@ int qmcx_distance_rows(qmcx_system_t sys, int precision, int npts, const double*
  * logdet[nmat] = log|det a|, phase[nmat] in {0, pi}. */
 int qmcx_invert_transpose(int n, int nmat, const double* a, double* psiM, double* logdet, double* phase);
 
+/* qmcx_invert_transpose on caller-owned DEVICE buffers that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same kernel and results as qmcx_invert_transpose.
+ * d_a[nmat][n][n] -> d_psiM[nmat][n][n], d_logdet[nmat], d_phase[nmat]; asynchronous on `stream` (a
+ * hipStream_t; NULL = the null stream).  The elimination workspace is a 32 MiB device array owned by
+ * the library, one per GPU, held until the process ends; it takes floor(2^21 / n^2) matrices per launch
+ * and larger batches run as several launches in order.  n <= 1024, else QMCX_ERR_ARG.  Calls that
+ * overlap on one GPU (other threads or streams) share the array and race. */
+int qmcx_invert_transpose_noalloc(int n, int nmat, const double* d_a, double* d_psiM, double* d_logdet,
+                                  double* d_phase, void* stream);
+
 /* ---- walker batch -----------------------------------------------------------
  * Device-resident walker state for nwalkers walkers (ParticleSet + WaveFunction
  * per Mover, src/Drivers/Mover.hpp:46-64; DiracDeterminant<DU_TYPE>,

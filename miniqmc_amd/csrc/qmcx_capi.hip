@@ -729,6 +729,34 @@ int settle_refresh(qmcx_batch_t b) {
   return 0;
 }
 
+// ---- allocation-free FP64 inversion (qmcx_invert_transpose_noalloc): its Gauss-Jordan
+// workspace is this library-owned device array, 32 MiB, which every GPU the library runs on
+// holds from the code object's load until the process ends.  Host code never names the array
+// in a launch or copy (on the host the name is only a stand-in): it uses the device address
+// that hipGetSymbolAddress gives for the current device, looked up once per device.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+// largest order it takes: one matrix needs 2 n^2 doubles of it
+constexpr int kMaxNoallocOrder = 1024;
+static_assert(sizeof(double) * 2 * kMaxNoallocOrder * kMaxNoallocOrder <= kScratchBytes, "scratch too small");
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchDevices] = {};
+
+int scratch_of_current_device(double** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) return fail(QMCX_ERR_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -1014,6 +1042,27 @@ int qmcx_invert_transpose(int n, int nmat, const double* a, double* psiM, double
   HIPCHK(hipMemcpyAsync(logdet, d_ld.p, sizeof(double) * nmat, hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(phase, d_ph.p, sizeof(double) * nmat, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int qmcx_invert_transpose_noalloc(int n, int nmat, const double* d_a, double* d_psiM, double* d_logdet,
+                                  double* d_phase, void* stream) {
+  if (n <= 0 || nmat <= 0 || !d_a || !d_psiM || !d_logdet || !d_phase) return fail(QMCX_ERR_ARG, "bad argument");
+  if (n > kMaxNoallocOrder)
+    return fail(QMCX_ERR_ARG, "qmcx_invert_transpose_noalloc: n = " + std::to_string(n) + " exceeds " +
+                                  std::to_string(kMaxNoallocOrder) + " (the scratch holds 16 n^2 bytes per matrix)");
+  // matrices per launch: each takes 2 n^2 doubles of the scratch; launches on one stream run in
+  // order, so each reuses the whole array
+  const size_t per = kScratchBytes / (sizeof(double) * 2 * (size_t)n * n);
+  double* ws = nullptr;
+  if (int rc = scratch_of_current_device(&ws)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t nn = (size_t)n * n;
+  for (int m0 = 0; m0 < nmat; m0 += (int)per) {
+    const int m = (int)std::min<size_t>(per, (size_t)(nmat - m0));
+    HIPCHK((launch_gj_invert<double, double>(n, m, st, d_a + m0 * nn, d_psiM + m0 * nn, ws, d_logdet + m0,
+                                            d_phase + m0, 1)));
+  }
   return 0;
 }
 

@@ -43,6 +43,7 @@ _SIGS = {
     "qmcx_spline_vgh": [_vp, _i, _i, _dp, _dp],
     "qmcx_distance_rows": [_vp, _i, _i, _dp, _ip, _i, _dp, _dp, _dp],
     "qmcx_invert_transpose": [_i, _i, _dp, _dp, _dp, _dp],
+    "qmcx_invert_transpose_noalloc": [_i, _i, _vp, _vp, _vp, _vp, _vp],
     "qmcx_batch_create": [C.POINTER(_vp), _vp, _i, _i, _i],
     "qmcx_batch_destroy": [_vp],
     "qmcx_batch_set_positions": [_vp, _dp],
