@@ -28,6 +28,8 @@ from miniqmc_amd import nlpp, qmcx, systems  # noqa: E402
 from miniqmc_amd import autotune  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# Metropolis legs: the FP32 form decides moves with |u - R^2| <= 1e-2 R^2 on the FP64 ratio
+ACCEPT_BAND = 1e-2
 METRIC = "walker-moves/s at NiO-64 on 1 MI355X, batched FP32 vs FP64 delay-1; log|Psi| err"
 
 
@@ -114,12 +116,15 @@ def measure_leg(dev, sysd, R0, k, tile, mode, steps, warmup, fp64_steps, err_ste
     # the two forms' ratio^2); log|Psi| is compared on the walkers without a flip.
     b32 = qmcx.WalkerBatch(dev, W, delay_rank=k, precision=qmcx.FP32, tile=tile)
     b64 = qmcx.WalkerBatch(dev, W, delay_rank=1, precision=qmcx.FP64)
+    band = ACCEPT_BAND if mode == qmcx.ACCEPT_METROPOLIS else 0.0
+    b32.set_accept_band(band)
     for b in (b32, b64):
         b.set_positions(R0)
         b.evaluate_log()
         b.set_trace(True)
     # the same start without the FP64 refresh: log|Psi| accumulated from the FP32 ratios alone
     bnr = qmcx.WalkerBatch(dev, W, delay_rank=k, precision=qmcx.FP32, tile=tile)
+    bnr.set_accept_band(band)
     bnr.set_positions(R0)
     bnr.evaluate_log()
     hnr = bnr.run(1, seed, tau, mode, recompute_every=0)
@@ -217,9 +222,10 @@ def measure_leg(dev, sysd, R0, k, tile, mode, steps, warmup, fp64_steps, err_ste
         b1.close()
     log(f"FP64 k=1: batched {value64:.4e} moves/s ({1e3 * wall64 / fp64_steps:.2f} ms/sweep)" +
         (f", one walker {value64_1:.4e} moves/s" if one_walker else ""))
+    narb = int(b32.arbitrations().sum())
     b32.close()
     b64.close()
-    return dict(flips=flips, clean=clean, lerr=lerr, ratio_err=ratio_err, errors=errors, wall=wall, kt=kt, moves=moves,
+    return dict(band=band, arbitrations=narb, flips=flips, clean=clean, lerr=lerr, ratio_err=ratio_err, errors=errors, wall=wall, kt=kt, moves=moves,
                 value=value,
                 acc_timed=acc_timed, stages=stages, wall64=wall64, value64=value64, value64_1=value64_1)
 
@@ -353,7 +359,8 @@ def main():
                  "steps": args.other_steps, "acceptance": O["acc_timed"],
                  "fp64_delay1_value": O["value64"], "speedup_vs_fp64_delay1": O["value"] / O["value64"],
                  "accept_flips": O["flips"], "walkers_identical_sequence": int(O["clean"].sum()),
-                 "log_psi_err_max_rel": float(np.nanmax(O["lerr"])), "errors": O["errors"]}
+                 "log_psi_err_max_rel": float(np.nanmax(O["lerr"])), "errors": O["errors"],
+                 "accept_band": O["band"], "fp64_arbitrations": O["arbitrations"]}
 
     # ---- rooflines.  The dominant kernel (most of the sweep's time) is the fused move kernel:
     # one workgroup per walker walks its electrons in order, a latency-bound chain that moves

@@ -244,16 +244,17 @@ struct qmcx_system_s {
   }
 };
 
-// Default near-tie band of the FP32 form's Metropolis decisions: a move whose uniform lies within
-// this fraction of |ratio|^2 from |ratio|^2 is decided on the FP64 ratio.
-constexpr double kDefaultAcceptBand = 1e-2;
 
 struct qmcx_batch_s {
   qmcx_system_s* sys = nullptr;
   int W = 0, K = 1, prec = QMCX_FP64, walker0 = 0, slices = 0, nt = 0;
   int nch = 1;   // delayed-update state slots per walker and determinant: one per chunk
   int flush_kind = 0;   // 0: matrix cores when possible, 1: vector FMA, 2: matrix cores, U rows from HBM
-  double band = kDefaultAcceptBand;   // FP32 Metropolis: near-tie band of the FP64 re-evaluation
+  // FP32 Metropolis: near-tie band of the FP64 re-evaluation (0: off).  While it is on the batch
+  // keeps FP64 Slater rows (A64, spo64), which the re-evaluation and the FP64 refresh read;
+  // rows64_stale until an evaluateLog has filled them
+  double band = 0.0;
+  bool rows64 = false, rows64_stale = false;
   double* flex_displ = nullptr;   // [W][3] proposals of the pending flex move
   int* flex_decide = nullptr;     // [W]
   int flex_iel = -1;
@@ -375,8 +376,8 @@ struct qmcx_batch_s {
     B.nacc = nacc;
     B.trace = trace_on ? trace : nullptr;
     B.sweep = sweep;
-    B.A64 = sizeof(T) == 4 ? A64 : nullptr;
-    B.spo64 = sizeof(T) == 4 ? spo64 : nullptr;
+    B.A64 = sizeof(T) == 4 && rows64 ? A64 : nullptr;
+    B.spo64 = sizeof(T) == 4 && rows64 ? spo64 : nullptr;
     B.narb = narb;
     return B;
   }
@@ -494,7 +495,7 @@ template <typename T>
 int launch_lu_batch(qmcx_batch_t b) {
   const qmcx_system_s* s = b->sys;
   const int n = s->n;
-  if (sizeof(T) == 4)   // the FP32 form inverts its FP64 Slater rows
+  if (sizeof(T) == 4 && b->rows64)   // the FP32 form with FP64 Slater rows inverts those
     HIPCHK((launch_gj_invert<double, T>(n, b->W * 2, b->stream, b->A64, (T*)b->Ainv, b->lu_ws, b->logdet, b->phase, 1)));
   else
     HIPCHK((launch_gj_invert<T, T>(n, b->W * 2, b->stream, (const T*)b->A, (T*)b->Ainv, b->lu_ws, b->logdet, b->phase, 1)));
@@ -578,7 +579,7 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
   a.spo = b->spo;
   a.prop = b->prop;
   a.decide = nullptr;
-  a.band = b->band;
+  a.band = b->rows64 ? b->band : 0.0;
   const int nchunks = (n + chunk - 1) / chunk;
   a.det = det;
   a.chunk_len = chunk;
@@ -627,7 +628,7 @@ int refresh_det(qmcx_batch_t b, int det, bool into_fresh) {
   double* ld = into_fresh ? b->ld_fresh : b->logdet + det;
   double* ph = into_fresh ? b->ph_fresh : b->phase + det;
   const int lds = into_fresh ? 1 : 2;
-  if (sizeof(T) == 4)   // the FP32 form inverts its FP64 Slater rows
+  if (sizeof(T) == 4 && b->rows64)   // the FP32 form with FP64 Slater rows inverts those
     HIPCHK((launch_gj_invert<double, T>(n, b->W, b->aux, b->A64 + det * nn, (T*)b->Ainv + det * nn,
                                         b->lu_ws + det * 2 * nn, ld, ph, lds, 2)));
   else
@@ -1249,6 +1250,7 @@ int qmcx_evaluate_log(qmcx_batch_t b, double* log_out) {
   if (!b) return fail(QMCX_ERR_ARG, "null batch");
   int rc = b->prec == QMCX_FP32 ? evaluate_log_impl<float>(b) : evaluate_log_impl<double>(b);
   if (rc) return rc;
+  b->rows64_stale = false;   // the FP64 Slater rows are filled with the SPO rows when kept
   HIPCHK(hipStreamSynchronize(b->stream));
   if (log_out) HIPCHK(hipMemcpy(log_out, b->logv, sizeof(double) * b->W, hipMemcpyDeviceToHost));
   return 0;
@@ -1273,6 +1275,8 @@ int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int 
   if (!b || nsweeps < 0 || !(tau > 0)) return fail(QMCX_ERR_ARG, "bad argument");
   if (accept_mode != QMCX_ACCEPT_REFERENCE && accept_mode != QMCX_ACCEPT_METROPOLIS)
     return fail(QMCX_ERR_ARG, "unknown accept mode");
+  if (b->rows64 && b->rows64_stale)
+    return fail(QMCX_ERR_STATE, "the accept band was switched on after qmcx_evaluate_log: call it again");
   if (nsweeps > b->hist_cap) {
     if (b->hist) hipFree(b->hist);
     b->hist = nullptr;
@@ -1339,6 +1343,9 @@ int qmcx_get_accepts(qmcx_batch_t b, long long* naccept) {
 int qmcx_batch_set_accept_band(qmcx_batch_t b, double band) {
   if (!b || !(band >= 0.0) || band > 1.0) return fail(QMCX_ERR_ARG, "band must be in [0, 1]");
   b->band = band;
+  const bool on = b->prec == QMCX_FP32 && band > 0.0;
+  if (on && !b->rows64) b->rows64_stale = true;
+  b->rows64 = on;
   return 0;
 }
 

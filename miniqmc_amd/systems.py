@@ -51,6 +51,34 @@ NIO_J1_PARAMS = (
 J2_CUSP = 0.0    # nio.hpp:104,121
 J1_CUSP = -0.25  # nio.hpp:149,166
 
+# nio.hpp:170-244 (buildJeeI) — the reference's three-body (eeI) PolynomialFunctor3D parameters,
+# shape (N_eI, N_ee) = (3, 3), 26 free parameters each, keyed by (ion species, spin pair) as the
+# reference's addFunc(ion species, 0, 0 | 1) assigns them; cutoff min(Wigner-Seitz radius, 4.8261684030)
+NIO_J3_SHAPE = (3, 3)
+NIO_J3_RCUT_MAX = 4.8261684030
+NIO_J3_PARAMS = {
+    (0, 0): [-0.003356164484, 0.002412623253, 0.01653623839, 0.0008341346169, -0.002808360734, 0.000710697475,
+             0.01076942152, 0.0009228283355, 0.01576022161, -0.003585259096, 0.003323106938, -0.02282975998,
+             -0.002246144403, -0.007196992871, -0.00404316239, 0.001465337212, 0.02026982926, -0.03528735393,
+             0.04594087928, -0.008776410679, -0.001552528476, -0.005554407743, 0.001858594451, 0.002001634408,
+             0.0009302256139, -0.0006304447229],
+    (0, 1): [-0.006861627197, 0.003278047306, 0.03324006545, 0.003097361067, -0.004710623571, 9.652180317e-06,
+             0.02212708787, -0.003718893286, 0.03390124932, -0.00710566395, 0.008807743592, -0.04281661568,
+             -0.008463011294, -0.01269994613, -0.002005229447, 0.002186590944, 0.03350196472, -0.05677253817,
+             0.07810604648, -0.009629896208, -0.006372643712, -0.01056861605, 0.002485188615, 0.008392442289,
+             1.073423014e-05, -0.0004812466328],
+    (1, 0): [-0.003775082438, -0.00169971229, 0.02162925441, 0.005674020544, -0.0008296047161, 0.00128057705,
+             0.005487203215, 0.001637322446, 0.02976838198, -0.0003207100945, 0.01143855436, -0.05336741304,
+             -0.00732359381, -0.01556942626, 0.0001149478453, 0.001838601199, 0.02570154203, -0.0675325214,
+             0.1080671614, -0.01258358969, 0.001839834045, -0.02422400426, 0.005154953014, 0.003510582598,
+             0.007464427016, -0.002454817757],
+    (1, 1): [-0.009590393593, 0.002498010871, 0.04225872633, 0.00460311261, -0.01071033503, 0.001253155062,
+             0.02934351285, -0.01823794726, 0.07224890393, -0.01020046849, 0.006310807929, -0.05655009412,
+             -0.0363775247, 0.002062411388, 0.02037856173, 0.003372676617, 0.03915277249, -0.02680556816,
+             0.08648136635, 0.01499088063, -0.02231984329, -0.02399792096, 0.001105720128, 0.02196005181,
+             0.003162638982, -0.00119645772],
+}
+
 
 @dataclasses.dataclass
 class Functor:
@@ -58,6 +86,17 @@ class Functor:
     rcut: float
     cusp: float
     params: np.ndarray  # float64 [nparams]
+
+
+@dataclasses.dataclass
+class Functor3D:
+    """A three-body (eeI) polynomial functor of one ion species and spin pair (0 same, 1 opposite)."""
+    species: int
+    pair: int
+    neI: int
+    nee: int
+    rcut: float
+    params: np.ndarray  # float64, the free parameters
 
 
 @dataclasses.dataclass
@@ -76,6 +115,7 @@ class System:
     coefs: np.ndarray          # float32 [(nx+3)*(ny+3)*(nz+3)*npad]
     j1: list                   # [nspecies] Functor
     j2: list                   # [uu, ud] Functor
+    j3: list = dataclasses.field(default_factory=list)   # Functor3D per (ion species, spin pair); empty: no J3
 
     @property
     def nel(self) -> int:
@@ -156,13 +196,22 @@ def spline_coefficients(ng, norb: int, npad: int, seed: int) -> np.ndarray:
     return full.reshape(-1)
 
 
-def make_system(name: str, seed: int = 11, j_mode: str = "random") -> System:
+def nio_j3(lattice: np.ndarray) -> list:
+    """The reference's eeI Jastrow (buildJeeI, nio.hpp:170-244, called with the electrons' Wigner-Seitz
+    radius, WaveFunction.cpp:100,137)."""
+    rcut = min(wigner_seitz_radius(lattice), NIO_J3_RCUT_MAX)
+    neI, nee = NIO_J3_SHAPE
+    return [Functor3D(sp, pair, neI, nee, rcut, np.array(p, dtype=float)) for (sp, pair), p in NIO_J3_PARAMS.items()]
+
+
+def make_system(name: str, seed: int = 11, j_mode: str = "random", j3: bool = False) -> System:
     """Build one of the named workloads.
 
     name: 'nio8-cubic' (96 e-), 'nio16' (192 e-), 'nio32' (384 e-), 'nio64' (768 e-),
           or 'tiny' (small test system).
     j_mode: 'random' — 8 B-spline parameters U(0,0.5) (last 0), rcut 3.9;
             'nio'    — the reference's own nio.hpp parameter sets and cutoffs.
+    j3:     add the reference's three-body eeI Jastrow (miniqmc's -j option).
     """
     if name == "nio8-cubic":
         a = 7.9
@@ -225,7 +274,8 @@ def make_system(name: str, seed: int = 11, j_mode: str = "random") -> System:
     else:
         raise ValueError(j_mode)
     coefs = spline_coefficients(ng, norb, npad, seed)
-    return System(name, lattice, ion_pos, species, 2, nup, ndn, ng, norb, npad, spo_lattice, coefs, j1, j2)
+    return System(name, lattice, ion_pos, species, 2, nup, ndn, ng, norb, npad, spo_lattice, coefs, j1, j2,
+                  nio_j3(lattice) if j3 else [])
 
 
 def initial_positions(system: System, nwalkers: int, seed: int = 11, walker0: int = 0) -> np.ndarray:

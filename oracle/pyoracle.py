@@ -67,6 +67,10 @@ def lib():
         L.orc_evaluate_ratios.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, _dp, _dp]
         L.orc_evaluate_log_parts.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_evaluate_ratios_many.argtypes = [C.c_void_p, _dp, C.c_int, _ip, C.c_int, _dp, _dp]
+        L.orc_system_set_j3.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int]
+        L.orc_system_set_j3.restype = C.c_int
+        L.orc_poly3d.argtypes = [C.c_int, C.c_int, C.c_double, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_poly3d.restype = C.c_int
         _lib = L
     return _lib
 
@@ -127,6 +131,20 @@ def spline_vgh_unit(ng, norb, npad, coefs, u):
     return out
 
 
+def poly3d(neI, nee, rcut, params, r12, r1, r2):
+    """The eeI polynomial functor alone: gamma[(neI+1),(neI+1),(nee+1)], and at the points
+    (r12, r1, r2): out[npts][9] = F, F_12/r12, F_1/r1, F_2/r2, F_12,12, F_1,1, F_2,2,
+    F_12,1/(r12 r1), F_12,2/(r12 r2) (zero where r1 or r2 reaches L), and the value alone."""
+    r12, r1, r2 = (_c(x).reshape(-1) for x in (r12, r1, r2))
+    params = _c(params)
+    gamma = np.zeros((neI + 1) * (neI + 1) * (nee + 1))
+    out, v = np.zeros((len(r12), 9)), np.zeros(len(r12))
+    rc = lib().orc_poly3d(neI, nee, rcut, params, len(params), len(r12), r12, r1, r2, gamma, out.reshape(-1), v)
+    if rc < 0:
+        raise ValueError(f"shape ({neI}, {nee}) does not take {len(params)} parameters")
+    return gamma.reshape(neI + 1, neI + 1, nee + 1), out, v
+
+
 def invert_transpose(A):
     A = _c(A)
     n = A.shape[0]
@@ -178,6 +196,9 @@ class OracleSystem:
             L.orc_system_set_j2(self.h, p, f.nparams, f.rcut, f.cusp, _c(f.params))
         L.orc_system_set_spline(self.h, _c(sysd.ng, np.int32), sysd.norb, sysd.npad, sysd.coefs,
                                 _c(sysd.spo_lattice).reshape(-1), spline_prec)
+        for f in getattr(sysd, "j3", []) or []:
+            if L.orc_system_set_j3(self.h, f.species, f.pair, f.neI, f.nee, f.rcut, _c(f.params), len(f.params)) < 0:
+                raise ValueError("J3 functor shape and parameter count disagree")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -198,10 +219,11 @@ class OracleSystem:
         return v, G, L
 
     def evaluate_log_parts(self, R):
-        """evaluateLog per component: logs[4], G[4][N][3], L[4][N] for the up and down
-        determinants (SPO-lattice fractional gradients, h_xx), J1 and J2 (Cartesian)."""
+        """evaluateLog per component: logs[5], G[5][N][3], L[5][N] for the up and down
+        determinants (SPO-lattice fractional gradients, h_xx), J1, J2 and J3 (Cartesian; zero
+        without a J3)."""
         N = self.sys.nel
-        logs, G, L = np.zeros(4), np.zeros((4, N, 3)), np.zeros((4, N))
+        logs, G, L = np.zeros(5), np.zeros((5, N, 3)), np.zeros((5, N))
         lib().orc_evaluate_log_parts(self.h, _c(R).reshape(-1), logs, G.reshape(-1), L.reshape(-1))
         return logs, G, L
 

@@ -20,6 +20,9 @@
 //   * BsplineFunctor                          QMCWaveFunctions/Jastrow/BsplineFunctor.h:93-339
 //   * OneBodyJastrow                          QMCWaveFunctions/Jastrow/OneBodyJastrow.h:110-283
 //   * TwoBodyJastrow                          QMCWaveFunctions/Jastrow/TwoBodyJastrow.h:251-430
+//   * ThreeBodyJastrow (eeI) with PolynomialFunctor3D
+//                                             QMCWaveFunctions/Jastrow/ThreeBodyJastrow.h:30-677,
+//                                             Jastrow/PolynomialFunctor3D.h:34-560
 //   * DiracMatrix::invert_transpose, computeLogDet QMCWaveFunctions/DiracMatrix.h:26-121
 //   * DelayedUpdate getInvRow/acceptRow/updateInvMat QMCWaveFunctions/DelayedUpdate.h:96-231
 //   * DiracDeterminant                        QMCWaveFunctions/DiracDeterminant.cpp:80-280
@@ -327,6 +330,169 @@ static inline double functor_v(const Functor& f, double r) {
   double v = 0;
   for (int m = 0; m < 4; ++m) v += c[m] * (BS_V[m][0] * t3 + BS_V[m][1] * t2 + BS_V[m][2] * t + BS_V[m][3]);
   return v;
+}
+
+// ---------------------------------------------------------------- three-body (eeI) polynomial functor
+// PolynomialFunctor3D (Jastrow/PolynomialFunctor3D.h:34-560): with L = rcut/2 and C = 3,
+//   F(r12, r1I, r2I) = sum_{l,m <= neI, n <= nee} gamma(l,m,n) r1I^l r2I^m r12^n
+//                      * ((r1I - L)(r2I - L))^C               for r1I < L and r2I < L, else 0,
+// gamma symmetric in (l, m).  The free parameters fill gamma after the linear constraints are
+// eliminated (resize :65-172, reset_gamma :180-260):
+//   e-e no cusp, k = 0..2 neI:       sum_{l+m=k} gamma(l,m,1) = 0
+//   e-I no cusp, k = 0..neI+nee:     sum_{m+n=k} C gamma(0,m,n) - L gamma(1,m,n) = 0
+// by row reduction of the constraint matrix over the unknowns (m <= l), scanning columns in
+// order with the largest pivot of each column; the columns left without a pivot take the
+// parameters in order, and each pivot column is solved from its row.
+struct Poly3D {
+  bool present = false;
+  int neI = 0, nee = 0;
+  double L = 0.0;
+  std::vector<double> g;   // gamma[l][m][n], (neI+1) x (neI+1) x (nee+1)
+  inline double gamma(int l, int m, int n) const { return g[((size_t)l * (neI + 1) + m) * (nee + 1) + n]; }
+};
+constexpr int kPolyC = 3;
+
+// returns the number of free parameters the shape takes, or -1 when nparams differs
+static int poly3d_setup(Poly3D& f, int neI, int nee, double rcut, const double* params, int nparams) {
+  const double L = 0.5 * rcut;
+  // unknowns: (l, m, n) with m <= l, numbered for m, then l >= m, then n
+  std::vector<int> idx((size_t)(neI + 1) * (neI + 1) * (nee + 1), -1);
+  auto at = [&](int l, int m, int n) -> int& { return idx[((size_t)l * (neI + 1) + m) * (nee + 1) + n]; };
+  int nu = 0;
+  for (int m = 0; m <= neI; ++m)
+    for (int l = m; l <= neI; ++l)
+      for (int n = 0; n <= nee; ++n) at(l, m, n) = at(m, l, n) = nu++;
+  const int nc = (2 * neI + 1) + (neI + nee + 1);
+  if (nparams != nu - nc) return -1;
+  std::vector<double> A((size_t)nc * nu, 0.0);
+  auto a = [&](int r, int c) -> double& { return A[(size_t)r * nu + c]; };
+  for (int k = 0; k <= 2 * neI; ++k)
+    for (int m = 0; m <= k; ++m) {
+      const int l = k - m;
+      if (l > neI || m > neI || l < m) continue;
+      a(k, at(l, m, 1)) = l > m ? 2.0 : 1.0;   // gamma(l,m,1) and gamma(m,l,1) are one unknown
+    }
+  const int r0 = 2 * neI + 1;
+  for (int k = 0; k <= neI + nee; ++k) {
+    // (m, n) with m + n = k: C gamma(0,m,n) - L gamma(1,m,n)
+    for (int m = 0; m <= std::min(k, neI); ++m) {
+      const int n = k - m;
+      if (n > nee) continue;
+      a(r0 + k, at(0, m, n)) = (double)kPolyC;
+      a(r0 + k, at(1, m, n)) = -L;
+    }
+  }
+  // reduced row echelon form, pivot columns in scan order
+  std::vector<int> pivot_col(nc, -1);
+  std::vector<char> free_col(nu, 1);
+  int col = -1;
+  for (int r = 0; r < nc; ++r) {
+    int best = r;
+    double bmax = 0.0;
+    do {
+      ++col;
+      best = r;
+      bmax = std::fabs(a(r, col));
+      for (int q = r + 1; q < nc; ++q)
+        if (std::fabs(a(q, col)) > bmax) { bmax = std::fabs(a(q, col)); best = q; }
+    } while (bmax < 1e-6);
+    for (int c = 0; c < nu; ++c) std::swap(a(r, c), a(best, c));
+    const double inv = 1.0 / a(r, col);
+    for (int c = 0; c < nu; ++c) a(r, c) *= inv;
+    for (int q = 0; q < nc; ++q) {
+      if (q == r) continue;
+      const double v = a(q, col);
+      if (v != 0.0)
+        for (int c = 0; c < nu; ++c) a(q, c) -= v * a(r, c);
+    }
+    pivot_col[r] = col;
+    free_col[col] = 0;
+  }
+  std::vector<double> gv(nu, 0.0);
+  int p = 0;
+  for (int c = 0; c < nu; ++c)
+    if (free_col[c]) gv[c] = params[p++];
+  for (int r = 0; r < nc; ++r) {
+    const int c0 = pivot_col[r];
+    double v = 0.0;
+    for (int c = 0; c < nu; ++c)
+      if (c != c0) v -= a(r, c) * gv[c];
+    gv[c0] = v;
+  }
+  f.present = true;
+  f.neI = neI;
+  f.nee = nee;
+  f.L = L;
+  f.g.assign((size_t)(neI + 1) * (neI + 1) * (nee + 1), 0.0);
+  for (int l = 0; l <= neI; ++l)
+    for (int m = 0; m <= neI; ++m)
+      for (int n = 0; n <= nee; ++n) f.g[((size_t)l * (neI + 1) + m) * (nee + 1) + n] = gv[at(l, m, n)];
+  return nu - nc;
+}
+
+// Value and derivatives of F at (r12, r1, r2) (r1 < L, r2 < L), in the layout of
+// evaluateVGL (PolynomialFunctor3D.h:445-556): out = {F, F_12/r12, F_1/r1, F_2/r2, F_12,12,
+// F_1,1, F_2,2, F_12,1/(r12 r1), F_12,2/(r12 r2)}.  The polynomial P and its partials are summed
+// directly, then the cutoff factor B^C, B = (r1 - L)(r2 - L), enters by the product rule.
+static void poly3d_vgl(const Poly3D& f, double r12, double r1, double r2, double out[9]) {
+  const int NE = f.neI, NN = f.nee;
+  double pw1[8], pw2[8], pw0[8];
+  pw1[0] = pw2[0] = pw0[0] = 1.0;
+  for (int i = 1; i <= std::max(NE, NN); ++i) {
+    pw1[i] = pw1[i - 1] * r1;
+    pw2[i] = pw2[i - 1] * r2;
+    pw0[i] = pw0[i - 1] * r12;
+  }
+  auto pw = [](const double* p, int e) { return e < 0 ? 0.0 : p[e]; };
+  double P = 0, P0 = 0, P1 = 0, P2 = 0, P00 = 0, P11 = 0, P22 = 0, P01 = 0, P02 = 0;
+  for (int l = 0; l <= NE; ++l)
+    for (int m = 0; m <= NE; ++m)
+      for (int n = 0; n <= NN; ++n) {
+        const double g = f.gamma(l, m, n);
+        if (g == 0.0) continue;
+        P += g * pw1[l] * pw2[m] * pw0[n];
+        P0 += g * pw1[l] * pw2[m] * n * pw(pw0, n - 1);
+        P1 += g * l * pw(pw1, l - 1) * pw2[m] * pw0[n];
+        P2 += g * pw1[l] * m * pw(pw2, m - 1) * pw0[n];
+        P00 += g * pw1[l] * pw2[m] * n * (n - 1) * pw(pw0, n - 2);
+        P11 += g * l * (l - 1) * pw(pw1, l - 2) * pw2[m] * pw0[n];
+        P22 += g * pw1[l] * m * (m - 1) * pw(pw2, m - 2) * pw0[n];
+        P01 += g * l * pw(pw1, l - 1) * pw2[m] * n * pw(pw0, n - 1);
+        P02 += g * pw1[l] * m * pw(pw2, m - 1) * n * pw(pw0, n - 1);
+      }
+  const double a1 = r1 - f.L, a2 = r2 - f.L, B = a1 * a2;
+  const double BC = std::pow(B, kPolyC), BC1 = std::pow(B, kPolyC - 1), BC2 = std::pow(B, kPolyC - 2);
+  const double C = (double)kPolyC;
+  const double F = P * BC;
+  const double F0 = P0 * BC, F00 = P00 * BC;
+  const double F1 = P1 * BC + P * C * BC1 * a2;
+  const double F2 = P2 * BC + P * C * BC1 * a1;
+  const double F11 = P11 * BC + 2.0 * P1 * C * BC1 * a2 + P * C * (C - 1.0) * BC2 * a2 * a2;
+  const double F22 = P22 * BC + 2.0 * P2 * C * BC1 * a1 + P * C * (C - 1.0) * BC2 * a1 * a1;
+  const double F01 = P01 * BC + P0 * C * BC1 * a2;
+  const double F02 = P02 * BC + P0 * C * BC1 * a1;
+  out[0] = F;
+  out[1] = F0 / r12;
+  out[2] = F1 / r1;
+  out[3] = F2 / r2;
+  out[4] = F00;
+  out[5] = F11;
+  out[6] = F22;
+  out[7] = F01 / (r12 * r1);
+  out[8] = F02 / (r12 * r2);
+}
+
+static double poly3d_v(const Poly3D& f, double r12, double r1, double r2) {
+  double P = 0;
+  double p1 = 1.0;
+  for (int l = 0; l <= f.neI; ++l, p1 *= r1) {
+    double p2 = 1.0;
+    for (int m = 0; m <= f.neI; ++m, p2 *= r2) {
+      double p0 = 1.0;
+      for (int n = 0; n <= f.nee; ++n, p0 *= r12) P += f.gamma(l, m, n) * p1 * p2 * p0;
+    }
+  }
+  return P * std::pow((r1 - f.L) * (r2 - f.L), kPolyC);
 }
 
 // ---------------------------------------------------------------- 3D multi B-spline
@@ -656,6 +822,8 @@ struct System {
   std::vector<int> species_first, species_last;
   std::vector<Functor> j1;   // per ion species
   Functor j2[2];             // [0] same spin, [1] opposite spin
+  std::vector<Poly3D> j3;    // eeI functor per ion species and spin pair: [species * 2 + (same ? 0 : 1)]
+  bool has_j3 = false;
   Spline spo;
 };
 
@@ -677,6 +845,12 @@ struct Walker {
   std::vector<double> U1, dU1, d2U1;
   double curAt = 0.0, curLap = 0.0, curGrad[3] = {0, 0, 0}, j1_log = 0.0;
   Determinant det[2];
+  // J3 (eeI): Uat, dUat, d2Uat of ThreeBodyJastrow.h:50-58, the electron-ion rows of every
+  // electron (the eI table, [N][M]) and the moving electron's per-electron terms
+  std::vector<double> U3, dU3x, dU3y, dU3z, d2U3;
+  std::vector<double> eiTr, eiTx, eiTy, eiTz;
+  std::vector<double> nUk, ndUx, ndUy, ndUz, nd2Uk, oUk, odUx, odUy, odUz, od2Uk;
+  double cur_U3 = 0.0, cur_dU3[3] = {0, 0, 0}, cur_d2U3 = 0.0, j3_log = 0.0;
   double log_value = 0.0;
   std::vector<double> G, L;  // per electron (3N, N)
 };
@@ -715,6 +889,69 @@ static void j1_u3(const Walker& w, const double* dist, double* u, double* du, do
   }
 }
 
+// ThreeBodyJastrow::computeU3 with computeU3_engine (ThreeBodyJastrow.h:441-625) for electron jel
+// with electron-ion row (rjI, displacement ion - electron) and electron-electron row (rjk,
+// displacement k - jel): over the ions I with r_jI < L_I and the electrons k != jel (k < jel
+// with `triangle`) with r_kI < L_I, F(r_jk, r_jI, r_kI) of (ion species, spin pair):
+//   Uj += F;  dUj += F_jI/r_jI d_jI + F_jk/r_jk d_jk  (d = displacements);
+//   d2Uj -= F_jk,jk + F_jI,jI + 2 (F_jk/r_jk + F_jI/r_jI) + 2 F_jk,jI/(r_jk r_jI) d_jk.d_jI;
+//   Uk[k] += F;  dUk[k] += F_kI/r_kI d_kI - F_jk/r_jk d_jk;
+//   d2Uk[k] -= F_jk,jk + F_kI,kI + 2 (F_jk/r_jk + F_kI/r_kI) - 2 F_jk,kI/(r_jk r_kI) d_kI.d_jk.
+// The other electrons' ion rows come from the walker's eI table.
+static void j3_u3(const Walker& w, int jel, const double* rjI, const double* xjI, const double* yjI,
+                  const double* zjI, const double* rjk, const double* xjk, const double* yjk, const double* zjk,
+                  bool triangle, double& Uj, double dUj[3], double& d2Uj, double* Uk, double* dUx, double* dUy,
+                  double* dUz, double* d2Uk) {
+  const System& s = *w.sys;
+  const int N = w.N, M = s.nions;
+  const int kmax = triangle ? jel : N;
+  Uj = d2Uj = 0.0;
+  dUj[0] = dUj[1] = dUj[2] = 0.0;
+  for (int k = 0; k < kmax; ++k) Uk[k] = dUx[k] = dUy[k] = dUz[k] = d2Uk[k] = 0.0;
+  const bool jup = jel < s.nup;
+  for (int I = 0; I < M; ++I) {
+    const int sp = s.ion_species[I];
+    const Poly3D& f0 = s.j3[2 * sp];
+    if (!f0.present || !(rjI[I] < f0.L)) continue;
+    for (int k = 0; k < kmax; ++k) {
+      const size_t kI = (size_t)k * M + I;
+      if (k == jel || !(w.eiTr[kI] < f0.L)) continue;
+      const Poly3D& f = s.j3[2 * sp + (((k < s.nup) == jup) ? 0 : 1)];
+      double o[9];
+      poly3d_vgl(f, rjk[k], rjI[I], w.eiTr[kI], o);
+      const double jk[3] = {xjk[k], yjk[k], zjk[k]}, jI[3] = {xjI[I], yjI[I], zjI[I]};
+      const double kIv[3] = {w.eiTx[kI], w.eiTy[kI], w.eiTz[kI]};
+      Uj += o[0];
+      d2Uj -= o[4] + o[5] + 2.0 * (o[1] + o[2]) + 2.0 * o[7] * dot3(jk, jI);
+      for (int d = 0; d < 3; ++d) dUj[d] += o[2] * jI[d] + o[1] * jk[d];
+      Uk[k] += o[0];
+      dUx[k] += o[3] * kIv[0] - o[1] * jk[0];
+      dUy[k] += o[3] * kIv[1] - o[1] * jk[1];
+      dUz[k] += o[3] * kIv[2] - o[1] * jk[2];
+      d2Uk[k] -= o[4] + o[6] + 2.0 * (o[1] + o[3]) - 2.0 * o[8] * dot3(kIv, jk);
+    }
+  }
+}
+
+// ThreeBodyJastrow::computeU (ThreeBodyJastrow.h:403-439): value only, for electron jel's rows
+static double j3_v(const Walker& w, int jel, const double* rjI, const double* rjk) {
+  const System& s = *w.sys;
+  const int N = w.N, M = s.nions;
+  const bool jup = jel < s.nup;
+  double Uj = 0.0;
+  for (int I = 0; I < M; ++I) {
+    const int sp = s.ion_species[I];
+    const Poly3D& f0 = s.j3[2 * sp];
+    if (!f0.present || !(rjI[I] < f0.L)) continue;
+    for (int k = 0; k < N; ++k) {
+      const size_t kI = (size_t)k * M + I;
+      if (k == jel || !(w.eiTr[kI] < f0.L)) continue;
+      Uj += poly3d_v(s.j3[2 * sp + (((k < s.nup) == jup) ? 0 : 1)], rjk[k], rjI[I], w.eiTr[kI]);
+    }
+  }
+  return Uj;
+}
+
 // SPO evaluation at Cartesian r: psi, dpsi (fractional-coordinate gradient),
 // d2psi = h_xx (einspline_spo.hpp:204-223)
 static void spo_eval(const System& s, const double r[3], double* psi, double* dpsi, double* d2psi,
@@ -749,6 +986,12 @@ static void walker_init(Walker& w, const System& s, const double* R, int delay) 
     v->assign(M, 0.0);
   w.G.assign(3 * (size_t)N, 0.0);
   w.L.assign(N, 0.0);
+  if (s.has_j3) {
+    for (auto* v : {&w.U3, &w.dU3x, &w.dU3y, &w.dU3z, &w.d2U3, &w.nUk, &w.ndUx, &w.ndUy, &w.ndUz, &w.nd2Uk, &w.oUk,
+                    &w.odUx, &w.odUy, &w.odUz, &w.od2Uk})
+      v->assign(N, 0.0);
+    for (auto* v : {&w.eiTr, &w.eiTx, &w.eiTy, &w.eiTz}) v->assign((size_t)N * M, 0.0);
+  }
   for (int d = 0; d < 2; ++d) {
     Determinant& D = w.det[d];
     D.first = d == 0 ? 0 : s.nup;
@@ -867,7 +1110,44 @@ static double evaluate_log(Walker& w) {
     w.G[3 * i + 2] += w.Gz[i] + w.dUz[i];
     w.L[i] += -w.Lap[i] + w.d2Uat[i];
   }
-  w.log_value = logv + w.j1_log + w.j2_log;
+  w.j3_log = 0.0;
+  if (s.has_j3) {
+    // ThreeBodyJastrow::recompute (ThreeBodyJastrow.h:363-401): the eI table, then for every
+    // electron its terms against the electrons before it, both sides accumulated
+    const int M = s.nions;
+    for (int i = 0; i < N; ++i) {
+      double pos[3] = {w.Rx[i], w.Ry[i], w.Rz[i]};
+      const size_t o = (size_t)i * M;
+      ei_row(w, pos, &w.eiTr[o], &w.eiTx[o], &w.eiTy[o], &w.eiTz[o]);
+    }
+    std::vector<double> r(N), dx(N), dy(N), dz(N);
+    for (int i = 0; i < N; ++i) {
+      double pos[3] = {w.Rx[i], w.Ry[i], w.Rz[i]};
+      aa_row(w, i, pos, r.data(), dx.data(), dy.data(), dz.data());
+      const size_t o = (size_t)i * M;
+      double Uj, dUj[3], d2Uj;
+      j3_u3(w, i, &w.eiTr[o], &w.eiTx[o], &w.eiTy[o], &w.eiTz[o], r.data(), dx.data(), dy.data(), dz.data(), true,
+            Uj, dUj, d2Uj, w.nUk.data(), w.ndUx.data(), w.ndUy.data(), w.ndUz.data(), w.nd2Uk.data());
+      w.U3[i] = Uj;
+      w.dU3x[i] = dUj[0]; w.dU3y[i] = dUj[1]; w.dU3z[i] = dUj[2];
+      w.d2U3[i] = d2Uj;
+      for (int k = 0; k < i; ++k) {
+        w.U3[k] += w.nUk[k];
+        w.d2U3[k] += w.nd2Uk[k];
+        w.dU3x[k] += w.ndUx[k]; w.dU3y[k] += w.ndUy[k]; w.dU3z[k] += w.ndUz[k];
+      }
+    }
+    double sum = 0;
+    for (int i = 0; i < N; ++i) {
+      sum += w.U3[i];
+      w.G[3 * i] += w.dU3x[i];
+      w.G[3 * i + 1] += w.dU3y[i];
+      w.G[3 * i + 2] += w.dU3z[i];
+      w.L[i] += w.d2U3[i];
+    }
+    w.j3_log = -0.5 * sum;
+  }
+  w.log_value = logv + w.j1_log + w.j2_log + w.j3_log;
   return w.log_value;
 }
 
@@ -894,6 +1174,11 @@ static void eval_grad(Walker& w, int iat, double g[3]) {
   g[0] = gx + w.Gx[iat] + w.dUx[iat];
   g[1] = gy + w.Gy[iat] + w.dUy[iat];
   g[2] = gz + w.Gz[iat] + w.dUz[iat];
+  if (w.sys->has_j3) {   // ThreeBodyJastrow::evalGrad (ThreeBodyJastrow.h:275)
+    g[0] += w.dU3x[iat];
+    g[1] += w.dU3y[iat];
+    g[2] += w.dU3z[iat];
+  }
 }
 
 // ParticleSet::makeMove
@@ -958,6 +1243,14 @@ static double ratio_grad(Walker& w, int iat, double g[3], double* det_ratio_out,
     w.cur_Uat = su;
     g[0] += ax; g[1] += ay; g[2] += az;
     ratio *= std::exp(w.Uat[iat] - su);
+  }
+  // J3 (ThreeBodyJastrow::ratioGrad, ThreeBodyJastrow.h:277-300): the proposal's rows
+  if (s.has_j3) {
+    j3_u3(w, iat, w.ei_rn.data(), w.ei_dxn.data(), w.ei_dyn.data(), w.ei_dzn.data(), w.dist_new.data(),
+          w.dxn.data(), w.dyn.data(), w.dzn.data(), false, w.cur_U3, w.cur_dU3, w.cur_d2U3, w.nUk.data(),
+          w.ndUx.data(), w.ndUy.data(), w.ndUz.data(), w.nd2Uk.data());
+    g[0] += w.cur_dU3[0]; g[1] += w.cur_dU3[1]; g[2] += w.cur_dU3[2];
+    ratio *= std::exp(w.U3[iat] - w.cur_U3);
   }
   return ratio;
 }
@@ -1026,6 +1319,31 @@ static void accept_move(Walker& w, int iat) {
   w.Uat[iat] = w.cur_Uat;
   w.dUx[iat] = cgx; w.dUy[iat] = cgy; w.dUz[iat] = cgz;
   w.d2Uat[iat] = cur_d2;
+  // J3 (ThreeBodyJastrow::acceptMove, ThreeBodyJastrow.h:302-361): the terms at the old position
+  // from its rows, every electron's sums moved by the difference, the eI row replaced
+  if (w.sys->has_j3) {
+    const int M = w.sys->nions;
+    const size_t o = (size_t)iat * M;
+    double Uold, dUold[3], d2old;
+    j3_u3(w, iat, &w.eiTr[o], &w.eiTx[o], &w.eiTy[o], &w.eiTz[o], w.dist_old.data(), w.dxo.data(), w.dyo.data(),
+          w.dzo.data(), false, Uold, dUold, d2old, w.oUk.data(), w.odUx.data(), w.odUy.data(), w.odUz.data(),
+          w.od2Uk.data());
+    for (int k = 0; k < w.N; ++k) {
+      w.U3[k] += w.nUk[k] - w.oUk[k];
+      w.d2U3[k] += w.nd2Uk[k] - w.od2Uk[k];
+      w.dU3x[k] += w.ndUx[k] - w.odUx[k];
+      w.dU3y[k] += w.ndUy[k] - w.odUy[k];
+      w.dU3z[k] += w.ndUz[k] - w.odUz[k];
+    }
+    w.j3_log += Uold - w.cur_U3;
+    w.U3[iat] = w.cur_U3;
+    w.dU3x[iat] = w.cur_dU3[0]; w.dU3y[iat] = w.cur_dU3[1]; w.dU3z[iat] = w.cur_dU3[2];
+    w.d2U3[iat] = w.cur_d2U3;
+    for (int c = 0; c < M; ++c) {
+      w.eiTr[o + c] = w.ei_rn[c];
+      w.eiTx[o + c] = w.ei_dxn[c]; w.eiTy[o + c] = w.ei_dyn[c]; w.eiTz[o + c] = w.ei_dzn[c];
+    }
+  }
   // positions
   w.Rx[iat] = w.active_pos[0]; w.Ry[iat] = w.active_pos[1]; w.Rz[iat] = w.active_pos[2];
   w.active = -1;
@@ -1075,7 +1393,19 @@ static double evaluate_gl(Walker& w) {
   }
   w.j1_log = -v1;
   w.j2_log = -0.5 * u2;
-  w.log_value = logv + w.j1_log + w.j2_log;
+  w.j3_log = 0.0;
+  if (w.sys->has_j3) {   // ThreeBodyJastrow::evaluateGL (ThreeBodyJastrow.h:627-645)
+    double u3 = 0;
+    for (int i = 0; i < N; ++i) {
+      u3 += w.U3[i];
+      w.G[3 * i] += w.dU3x[i];
+      w.G[3 * i + 1] += w.dU3y[i];
+      w.G[3 * i + 2] += w.dU3z[i];
+      w.L[i] += w.d2U3[i];
+    }
+    w.j3_log = -0.5 * u3;
+  }
+  w.log_value = logv + w.j1_log + w.j2_log + w.j3_log;
   return w.log_value;
 }
 
@@ -1219,6 +1549,43 @@ void orc_system_set_spline(void* h, const int* ng, int norb, int npad, const flo
 }
 
 double orc_system_ws_radius(void* h) { return ((OrcSystem*)h)->sys.lat.ws_radius; }
+
+// ThreeBodyJastrow::addFunc (ThreeBodyJastrow.h:118-152) with a PolynomialFunctor3D of shape
+// (neI, nee), cutoff rcut and free parameters params[nparams] (nio.hpp:170-260): the functor of
+// ion species `species` and spin pair `pair` (0 = same, 1 = opposite).  Returns the number of
+// free parameters of the shape, or -1 when nparams differs (the functor is then not set).
+int orc_system_set_j3(void* h, int species, int pair, int neI, int nee, double rcut, const double* params,
+                      int nparams) {
+  System& s = ((OrcSystem*)h)->sys;
+  if ((int)s.j3.size() < 2 * s.nspecies) s.j3.assign(2 * s.nspecies, Poly3D());
+  Poly3D f;
+  const int r = poly3d_setup(f, neI, nee, rcut, params, nparams);
+  if (r < 0) return r;
+  s.j3[2 * species + pair] = f;
+  s.has_j3 = true;
+  return r;
+}
+
+// The functor alone: gamma[(neI+1)^2 (nee+1)] and, at npts points (r12, r1, r2), out[npts][9] as
+// poly3d_vgl (F zero with its derivatives where r1 >= L or r2 >= L) and the value alone v[npts].
+int orc_poly3d(int neI, int nee, double rcut, const double* params, int nparams, int npts, const double* r12,
+               const double* r1, const double* r2, double* gamma, double* out, double* v) {
+  Poly3D f;
+  const int r = poly3d_setup(f, neI, nee, rcut, params, nparams);
+  if (r < 0) return r;
+  std::copy(f.g.begin(), f.g.end(), gamma);
+  for (int p = 0; p < npts; ++p) {
+    double* o = out + 9 * (size_t)p;
+    if (!(r1[p] < f.L) || !(r2[p] < f.L)) {
+      std::fill(o, o + 9, 0.0);
+      v[p] = 0.0;
+      continue;
+    }
+    poly3d_vgl(f, r12[p], r1[p], r2[p], o);
+    v[p] = poly3d_v(f, r12[p], r1[p], r2[p]);
+  }
+  return r;
+}
 
 // SPO evaluation at Cartesian points (through toUnit_floor with the SPO lattice):
 // psi[npts][norb], dpsi[npts][norb][3], d2psi[npts][norb]
@@ -1366,14 +1733,21 @@ static void evaluate_ratios_at(Walker& w, int iel, int nvp, const double* vpos, 
       const Functor& f = j2f(s, iel, j);
       if (j != iel && r[j] < f.rcut) v2 += functor_v(f, r[j]);
     }
-    ratios[k] = det_ratio * std::exp((w.Vat[iel] - v1) + (w.Uat[iel] - v2));
+    double lj = (w.Vat[iel] - v1) + (w.Uat[iel] - v2);
+    if (s.has_j3) {   // ThreeBodyJastrow::evaluateRatios (ThreeBodyJastrow.h:262-273)
+      std::vector<double> rI(s.nions), ax(s.nions), ay(s.nions), az(s.nions);
+      ei_row(w, p, rI.data(), ax.data(), ay.data(), az.data());
+      lj += w.U3[iel] - j3_v(w, iel, rI.data(), r.data());
+    }
+    ratios[k] = det_ratio * std::exp(lj);
   }
 }
 
 extern "C" {
 
 // evaluateLog split by component (the WaveFunctionComponent list of WaveFunction.cpp:178-195):
-// part 0 = up determinant, 1 = down determinant, 2 = J1, 3 = J2.  logs[4]; G[4][N][3]; L[4][N].
+// part 0 = up determinant, 1 = down determinant, 2 = J1, 3 = J2, 4 = J3 (zero without one).
+// logs[5]; G[5][N][3]; L[5][N].
 // The determinant parts carry the reference's SPO convention (gradients in the SPO lattice's
 // fractional coordinates, d2psi = h_xx); the Jastrow parts are Cartesian.
 void orc_evaluate_log_parts(void* h, const double* R, double* logs, double* G, double* L) {
@@ -1382,8 +1756,8 @@ void orc_evaluate_log_parts(void* h, const double* R, double* logs, double* G, d
   walker_init(w, s, R, 1);
   evaluate_log(w);
   const int N = w.N;
-  std::fill(G, G + 4 * 3 * (size_t)N, 0.0);
-  std::fill(L, L + 4 * (size_t)N, 0.0);
+  std::fill(G, G + 5 * 3 * (size_t)N, 0.0);
+  std::fill(L, L + 5 * (size_t)N, 0.0);
   for (int d = 0; d < 2; ++d) {
     const Determinant& D = w.det[d];
     const int n = D.n;
@@ -1405,6 +1779,7 @@ void orc_evaluate_log_parts(void* h, const double* R, double* logs, double* G, d
   }
   logs[2] = w.j1_log;
   logs[3] = w.j2_log;
+  logs[4] = w.j3_log;
   for (int i = 0; i < N; ++i) {
     double* g1 = G + ((size_t)2 * N + i) * 3;
     double* g2 = G + ((size_t)3 * N + i) * 3;
@@ -1412,6 +1787,11 @@ void orc_evaluate_log_parts(void* h, const double* R, double* logs, double* G, d
     g2[0] = w.dUx[i]; g2[1] = w.dUy[i]; g2[2] = w.dUz[i];
     L[(size_t)2 * N + i] = -w.Lap[i];
     L[(size_t)3 * N + i] = w.d2Uat[i];
+    if (s.has_j3) {
+      double* g3 = G + ((size_t)4 * N + i) * 3;
+      g3[0] = w.dU3x[i]; g3[1] = w.dU3y[i]; g3[2] = w.dU3z[i];
+      L[(size_t)4 * N + i] = w.d2U3[i];
+    }
   }
 }
 

@@ -16,8 +16,8 @@ moves until the FP64 refresh, so move m is held to
 Gradients are held to the same relative bound times (1 + |g|).  log|Psi| is compared twice: before
 the FP64 refresh (the accumulated log of the accepted FP32 ratios, within the sum of the accepted
 moves' bounds plus 1e-4 for the FP32 Jastrow sums) and after it (FP64 LU of the FP64 Slater rows,
-FP32 Jastrow sums: 1e-7 relative).  Under Metropolis the accept sequence must be identical: near ties are decided on
-the FP64 ratio (qmcx_batch_set_accept_band).
+FP32 Jastrow sums: 1e-7 relative).  Under Metropolis the accept sequence must be identical: the FP32 batch
+decides near ties (|u - R^2| <= 1e-2 R^2) on the FP64 ratio (qmcx_batch_set_accept_band).
 """
 import numpy as np
 import pytest
@@ -30,6 +30,7 @@ pytestmark = pytest.mark.gpu
 EPS32 = 2.0 ** -24
 EPS64 = 2.0 ** -53
 SEED, TAU, NW = 5, 0.3, 2
+BAND = 1e-2   # near-tie band of the FP64 re-evaluation under Metropolis
 
 
 def _say(*a):
@@ -61,6 +62,8 @@ def _running_sensitivity(sens, n):
 
 def _gpu_sweep(dev, R0, prec, delay, mode, recompute):
     wb = qmcx.WalkerBatch(dev, NW, delay_rank=delay, precision=prec)
+    if prec == qmcx.FP32 and mode == qmcx.ACCEPT_METROPOLIS:
+        wb.set_accept_band(BAND)
     wb.set_positions(R0)
     log0 = wb.evaluate_log()
     wb.set_trace(True)

@@ -15,6 +15,7 @@ closed forms and brute force.  Which check pins which oracle function:
   whole sweep                    incremental log|Psi|, G, L after sweeps = fresh evaluateLog
   evaluateLog G and L            central differences of each component's log value (J1, J2 Cartesian;
                                  determinants in the SPO lattice's fractional coordinates with h_xx)
+  three-body (eeI) Jastrow       tests/test_oracle_j3.py
 """
 import json
 import os
