@@ -60,6 +60,18 @@ int qmcx_system_set_j1(qmcx_system_t sys, int species, int nparams, double rcut,
  * (TwoBodyJastrow::addFunc, TwoBodyJastrow.h:204-240). */
 int qmcx_system_set_j2(qmcx_system_t sys, int pair, int nparams, double rcut, double cusp, const double* params);
 
+/* Three-body electron-electron-ion Jastrow (ThreeBodyJastrow::addFunc, src/QMCWaveFunctions/Jastrow/
+ * ThreeBodyJastrow.h:118-152, with PolynomialFunctor3D::resize/reset_gamma, PolynomialFunctor3D.h:65-260;
+ * miniqmc's -j option, src/Input/nio.hpp:170-244): the functor of ion species `species` and spin pair
+ * `pair` (0 = same, 1 = opposite), polynomial degrees neI, nee in [1, 3], cutoff rcut (electrons count
+ * within rcut/2 of the ion), params[nparams] the free parameters left by the no-cusp constraints
+ * (26 for (3, 3); QMCX_ERR_ARG names the count when it differs).  All functors of one species take one
+ * rcut; at most 256 ions.  Set it before qmcx_batch_create: a batch made earlier returns QMCX_ERR_STATE.
+ * With a three-body Jastrow, qmcx_flex_evaluate_ratios and qmcx_nlpp_evaluate return QMCX_ERR_ARG, and
+ * an electron position may have at most 6 ions within the cutoff (else QMCX_ERR_STATE). */
+int qmcx_system_set_j3(qmcx_system_t sys, int species, int pair, int neI, int nee, double rcut, const double* params,
+                       int nparams);
+
 /* Spline table upload: coefs is [ng0+3][ng1+3][ng2+3][npad] FP32, orbitals
  * innermost (BsplineAllocator::allocateMultiBspline layout,
  * src/Numerics/Spline2/BsplineAllocator.hpp:84-136).  norb must equal nup.

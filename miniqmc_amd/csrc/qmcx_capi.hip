@@ -195,6 +195,16 @@ struct qmcx_system_s {
   Functor<double>* d_fnd = nullptr;
   Functor<float>* d_fnf = nullptr;
   bool fn_dirty = true;
+  // three-body Jastrow: functor of (ion species, spin pair) at [2 * species + pair] in both
+  // precisions, the cutoff radius each species was given (0: none yet), ion-mask words
+  Poly3<double> j3d[2 * kMaxSpecies];
+  Poly3<float> j3f[2 * kMaxSpecies];
+  double j3rcut[kMaxSpecies] = {0, 0, 0, 0};
+  Poly3<double>* d_j3d = nullptr;
+  Poly3<float>* d_j3f = nullptr;
+  bool has_j3 = false, j3_dirty = false;
+  int mw = 0;
+  int nbatches = 0;   // live batches of this system
   int ng[3] = {0, 0, 0};
   int norb = 0, npad = 0;
   double spoG[9];
@@ -224,18 +234,26 @@ struct qmcx_system_s {
     S.species = d_species;
     S.fn64 = d_fnd;
     S.cell64 = d_cell_d;
+    S.mw = has_j3 ? mw : 0;
     if constexpr (sizeof(T) == 4) {
       S.coefs = d_coef_f;
       S.fn = d_fnf;
       S.cell = d_cell_f;
+      S.j3 = has_j3 ? d_j3f : nullptr;
     } else {
       S.coefs = d_coef_d;
       S.fn = d_fnd;
       S.cell = d_cell_d;
+      S.j3 = has_j3 ? d_j3d : nullptr;
     }
     return S;
   }
   int sync_functors() {
+    if (j3_dirty) {
+      HIPCHK(hipMemcpy(d_j3d, j3d, sizeof(j3d), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d_j3f, j3f, sizeof(j3f), hipMemcpyHostToDevice));
+      j3_dirty = false;
+    }
     if (!fn_dirty) return 0;
     HIPCHK(hipMemcpy(d_fnd, fnd, sizeof(fnd), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_fnf, fnf, sizeof(fnf), hipMemcpyHostToDevice));
@@ -293,6 +311,15 @@ struct qmcx_batch_s {
   void* Vbuf = nullptr;
   void* Binv = nullptr;
   void* spo = nullptr;   // [W][2][n][5][npad] SPO rows at the sweep's proposals
+  // three-body Jastrow state (systems with one): sums [W][5][N], ion masks [W][N][mw], near-ion
+  // records of the current positions [W][N][kJ3Rec] and of the proposals [W][2][n][kJ3Rec]
+  void* J3 = nullptr;
+  unsigned long long* j3mask = nullptr;
+  void* j3near = nullptr;
+  void* j3prop = nullptr;
+  bool j3 = false;
+  bool counted = false;   // counted in its system's live batches
+  int* flags = nullptr;   // [1] kernel error bits
   double* A64 = nullptr;   // FP32 form: [W][2][n][n] FP64 Slater rows (the FP64 refresh inverts these)
   double* spo64 = nullptr; // FP32 form: [W][2][n][npad] FP64 values at the sweep's proposals
   double* prop = nullptr; // [W][2][n][kProp] proposal records of the sweep
@@ -379,6 +406,11 @@ struct qmcx_batch_s {
     B.A64 = sizeof(T) == 4 && rows64 ? A64 : nullptr;
     B.spo64 = sizeof(T) == 4 && rows64 ? spo64 : nullptr;
     B.narb = narb;
+    B.J3 = (T*)J3;
+    B.j3mask = j3mask;
+    B.j3near = (T*)j3near;
+    B.j3prop = (T*)j3prop;
+    B.flags = flags;
     return B;
   }
   template <typename P>
@@ -414,23 +446,27 @@ int proposal_slices(const qmcx_system_s* s, int want) {
   return nq * 2 <= 1024 ? 2 : 1;
 }
 
-template <typename T, int JPT, bool KD, bool ARB>
+template <typename T, int JPT, bool KD, bool ARB, bool J3>
 int launch_moves_kd(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
   const qmcx_system_s* s = b->sys;
-  hipLaunchKernelGGL((k_moves<T, JPT, KD, ARB>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(), b->params<T>(),
-                     a);
+  hipLaunchKernelGGL((k_moves<T, JPT, KD, ARB, J3>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(),
+                     b->params<T>(), a);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-// the near-tie FP64 re-evaluation exists for FP32 delayed updates under Metropolis acceptance
+// the near-tie FP64 re-evaluation exists for FP32 delayed updates under Metropolis acceptance and
+// a wave function without a three-body Jastrow
 template <typename T, int JPT>
 int launch_moves_jpt(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
-  if (b->K == 1) return launch_moves_kd<T, JPT, false, false>(b, nt, lds, a);
+  if (b->j3)
+    return b->K == 1 ? launch_moves_kd<T, JPT, false, false, true>(b, nt, lds, a)
+                     : launch_moves_kd<T, JPT, true, false, true>(b, nt, lds, a);
+  if (b->K == 1) return launch_moves_kd<T, JPT, false, false, false>(b, nt, lds, a);
   if constexpr (sizeof(T) == 4)
     if (a.mode == QMCX_ACCEPT_METROPOLIS && a.band > 0.0 && a.decide == nullptr)
-      return launch_moves_kd<T, JPT, true, true>(b, nt, lds, a);
-  return launch_moves_kd<T, JPT, true, false>(b, nt, lds, a);
+      return launch_moves_kd<T, JPT, true, true, false>(b, nt, lds, a);
+  return launch_moves_kd<T, JPT, true, false, false>(b, nt, lds, a);
 }
 
 template <typename T>
@@ -519,9 +555,8 @@ int evaluate_log_impl(qmcx_batch_t b) {
   const qmcx_system_s* s = b->sys;
   if (launch_spo_rows<T>(b)) return QMCX_ERR_HIP;
   if (launch_lu_batch<T>(b)) return QMCX_ERR_HIP;
-  // positions, reduction scratch, functors and the minimum-image cell (k_jastrow_init's layout)
-  const size_t lds =
-      sizeof(double) * (3 * (size_t)s->N + 32) + sizeof(Functor<T>) * (kMaxSpecies + 2) + sizeof(Cell<T>);
+  // positions, reduction scratch, functors, the minimum-image cell and the J3 ion masks
+  const size_t lds = jastrow_init_lds<T>(s->N, s->has_j3 ? s->mw : 0);
   hipLaunchKernelGGL(k_jastrow_init<T>, dim3(b->W), dim3(1024), lds, b->stream, s->params<T>(), b->params<T>());
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemsetAsync(b->ndelay, 0, sizeof(int) * b->W * 2 * b->nch, b->stream));
@@ -535,9 +570,10 @@ bool use_mfma_flush(const qmcx_batch_t b) {
   return sizeof(T) == 4 && flush_mfma_fits(b->sys->n) && b->flush_kind != 1;
 }
 
-// Electrons per move-kernel launch: the delay rank (one flush per chunk), or up to 64 with
-// Sherman-Morrison updates (K = 1), which bounds the chunk's LDS records.
-static int move_chunk(int n, int K) { return K == 1 ? std::min(n, 64) : K; }
+// Electrons per chunk of a move launch: the delay rank (one flush per chunk), or up to 64 with
+// Sherman-Morrison updates (K = 1; 16 with a three-body Jastrow), which bounds the chunk's LDS
+// records.
+static int move_chunk(int n, int K, bool j3) { return K == 1 ? std::min(n, j3 ? 16 : 64) : K; }
 
 // updateInvMat of the delayed rows of state slots [j0, j1) of determinant det, in order
 template <typename T>
@@ -571,8 +607,8 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
   int nt = 0;
   if (choose_shape(s, nt)) return QMCX_ERR_ARG;
   const int n = s->n;
-  const int chunk = move_chunk(n, b->K);
-  const MoveLds L = move_lds<T>(s->N, s->nions, n, b->K, chunk);
+  const int chunk = move_chunk(n, b->K, b->j3);
+  const MoveLds L = move_lds<T>(s->N, s->nions, n, b->K, chunk, b->j3 ? s->mw : 0);
   MoveArgs a;
   a.mode = mode;
   a.slices = 0;
@@ -761,6 +797,25 @@ int scratch_of_current_device(double** out) {
 }  // namespace
 
 // ====================================================================== C ABI
+// A batch keeps three-body Jastrow state only when its system had one at its creation.
+static int check_j3(const qmcx_batch_t b) {
+  if (b->sys->has_j3 != b->j3)
+    return fail(QMCX_ERR_STATE, "the three-body Jastrow was set after the batch was created: create it again");
+  return 0;
+}
+
+// The kernels' error bits since the last check (the call that reads them has synchronized).
+static int check_flags(qmcx_batch_t b) {
+  int f = 0;
+  HIPCHK(hipMemcpy(&f, b->flags, sizeof(int), hipMemcpyDeviceToHost));
+  if (f == 0) return 0;
+  HIPCHK(hipMemset(b->flags, 0, sizeof(int)));
+  if (f & kFlagJ3Near)
+    return fail(QMCX_ERR_STATE, "an electron position has more than " + std::to_string(kJ3Near) +
+                                    " ions within their three-body Jastrow cutoff");
+  return fail(QMCX_ERR_STATE, "more than " + std::to_string(kJ3Slots) + " three-body Jastrow pairs in one wave of a move");
+}
+
 template <typename T>
 static int launch_virtual_ratios(qmcx_batch_t b, int iel, int nvp) {
   const qmcx_system_s* s = b->sys;
@@ -877,6 +932,10 @@ int qmcx_system_create(qmcx_system_t* out, const double* lattice9, int nions, co
   HIPCHK(hipMalloc(&s->d_species, sizeof(int) * nions));
   HIPCHK(hipMemcpy(s->d_species, ion_species, sizeof(int) * nions, hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&s->d_fnd, sizeof(s->fnd)));
+  HIPCHK(hipMalloc(&s->d_j3d, sizeof(s->j3d)));
+  HIPCHK(hipMalloc(&s->d_j3f, sizeof(s->j3f)));
+  std::memset(s->j3d, 0, sizeof(s->j3d));
+  std::memset(s->j3f, 0, sizeof(s->j3f));
   HIPCHK(hipMalloc(&s->d_fnf, sizeof(s->fnf)));
   {
     const Cell<float> cf = make_cell<float>(lattice9);
@@ -901,6 +960,8 @@ int qmcx_system_destroy(qmcx_system_t s) {
   hipFree(s->d_ions);
   hipFree(s->d_species);
   hipFree(s->d_fnd);
+  hipFree(s->d_j3d);
+  hipFree(s->d_j3f);
   hipFree(s->d_fnf);
   hipFree(s->d_coef_f);
   hipFree(s->d_coef_d);
@@ -918,6 +979,110 @@ static int set_functor(qmcx_system_t s, int slot, int nparams, double rcut, doub
   s->fnd[slot] = make_functor<double>(nparams, rcut, cusp, params);
   s->fnf[slot] = make_functor<float>(nparams, rcut, cusp, params);
   s->fn_dirty = true;
+  return 0;
+}
+
+// PolynomialFunctor3D::resize + reset_gamma (PolynomialFunctor3D.h:65-260): the unknowns gamma(l,m,n),
+// m <= l (gamma is symmetric in l, m), numbered m-major then l then n; the constraints
+//   e-e no cusp, k = 0..2 neI:    sum_{l+m=k} gamma(l,m,1) = 0          (each unordered l, m once)
+//   e-I no cusp, k = 0..neI+nee:  sum_{m+n=k} 3 gamma(0,m,n) - L gamma(1,m,n) = 0
+// reduced to row echelon form scanning the unknowns in order (largest pivot of each column,
+// columns below 1e-6 skipped); the unknowns without a pivot take the parameters in order and the
+// pivot unknowns follow from their rows.  g[(l*4 + m)*4 + n], zero past the shape.
+static int poly3_gamma(int neI, int nee, double rcut, const double* params, int nparams, double g[kJ3G]) {
+  const double L = 0.5 * rcut;
+  const int D = kJ3Deg + 1;
+  std::vector<int> uid(D * D * D, -1);
+  auto U = [&](int l, int m, int n) -> int& { return uid[(l * D + m) * D + n]; };
+  int nu = 0;
+  for (int m = 0; m <= neI; ++m)
+    for (int l = m; l <= neI; ++l)
+      for (int n = 0; n <= nee; ++n) U(l, m, n) = U(m, l, n) = nu++;
+  const int nee_rows = 2 * neI + 1, nc = nee_rows + neI + nee + 1;
+  if (nparams != nu - nc) return fail(QMCX_ERR_ARG, "a (" + std::to_string(neI) + ", " + std::to_string(nee) +
+                                                         ") three-body functor takes " + std::to_string(nu - nc) +
+                                                         " parameters");
+  std::vector<double> A((size_t)nc * nu, 0.0);
+  for (int k = 0; k <= 2 * neI; ++k)
+    for (int l = k; l >= 0; --l) {
+      const int m = k - l;
+      if (l <= neI && m <= l) A[(size_t)k * nu + U(l, m, 1)] = l > m ? 2.0 : 1.0;
+    }
+  for (int k = 0; k <= neI + nee; ++k)
+    for (int n = 0; n <= std::min(k, nee); ++n) {
+      const int m = k - n;
+      if (m > neI) continue;
+      A[(size_t)(nee_rows + k) * nu + U(0, m, n)] = 3.0;
+      A[(size_t)(nee_rows + k) * nu + U(1, m, n)] = -L;
+    }
+  std::vector<int> pc(nc);
+  std::vector<char> is_pivot(nu, 0);
+  int col = 0;
+  for (int r = 0; r < nc; ++r) {
+    int best = -1;
+    for (; col < nu; ++col) {
+      double bmax = 0.0;
+      for (int q = r; q < nc; ++q) {
+        const double v = std::fabs(A[(size_t)q * nu + col]);
+        if (v > bmax) { bmax = v; best = q; }
+      }
+      if (bmax >= 1e-6) break;
+    }
+    if (col == nu) return fail(QMCX_ERR_ARG, "three-body functor constraints are rank deficient");
+    for (int c = 0; c < nu; ++c) std::swap(A[(size_t)r * nu + c], A[(size_t)best * nu + c]);
+    const double inv = 1.0 / A[(size_t)r * nu + col];
+    for (int c = 0; c < nu; ++c) A[(size_t)r * nu + c] *= inv;
+    for (int q = 0; q < nc; ++q) {
+      const double f = A[(size_t)q * nu + col];
+      if (q == r || f == 0.0) continue;
+      for (int c = 0; c < nu; ++c) A[(size_t)q * nu + c] -= f * A[(size_t)r * nu + c];
+    }
+    pc[r] = col;
+    is_pivot[col] = 1;
+    ++col;
+  }
+  std::vector<double> x(nu, 0.0);
+  for (int c = 0, p = 0; c < nu; ++c)
+    if (!is_pivot[c]) x[c] = params[p++];
+  for (int r = 0; r < nc; ++r) {
+    double v = 0.0;
+    for (int c = 0; c < nu; ++c)
+      if (c != pc[r]) v -= A[(size_t)r * nu + c] * x[c];
+    x[pc[r]] = v;
+  }
+  for (int l = 0; l < D; ++l)
+    for (int m = 0; m < D; ++m)
+      for (int n = 0; n < D; ++n)
+        g[(l * D + m) * D + n] = (l <= neI && m <= neI && n <= nee) ? x[U(l, m, n)] : 0.0;
+  return 0;
+}
+
+int qmcx_system_set_j3(qmcx_system_t s, int species, int pair, int neI, int nee, double rcut, const double* params,
+                       int nparams) {
+  if (!s || !params) return fail(QMCX_ERR_ARG, "null argument");
+  if (species < 0 || species >= s->nspecies) return fail(QMCX_ERR_ARG, "species out of range");
+  if (pair < 0 || pair > 1) return fail(QMCX_ERR_ARG, "pair must be 0 (same spin) or 1 (opposite spin)");
+  if (neI < 1 || nee < 1 || neI > kJ3Deg || nee > kJ3Deg)
+    return fail(QMCX_ERR_ARG, "three-body functor degrees must be in [1, " + std::to_string(kJ3Deg) + "]");
+  if (!(rcut > 0)) return fail(QMCX_ERR_ARG, "rcut must be positive");
+  if (s->j3rcut[species] != 0.0 && s->j3rcut[species] != rcut)
+    return fail(QMCX_ERR_ARG, "the three-body functors of one ion species need one cutoff (ThreeBodyJastrow.h:196-218)");
+  if (s->nions > kJ3MaxIons) return fail(QMCX_ERR_ARG, "a three-body Jastrow takes at most 256 ions");
+  if (s->nbatches > 0) return fail(QMCX_ERR_STATE, "set the three-body Jastrow before creating batches of the system");
+  double g[kJ3G];
+  if (int rc = poly3_gamma(neI, nee, rcut, params, nparams, g)) return rc;
+  Poly3<double>& fd = s->j3d[2 * species + pair];
+  Poly3<float>& ff = s->j3f[2 * species + pair];
+  fd.L = 0.5 * rcut;
+  ff.L = (float)fd.L;
+  for (int i = 0; i < kJ3G; ++i) {
+    fd.g[i] = g[i];
+    ff.g[i] = (float)g[i];
+  }
+  s->j3rcut[species] = rcut;
+  s->has_j3 = true;
+  s->j3_dirty = true;
+  s->mw = (s->nions + 63) / 64;
   return 0;
 }
 
@@ -1092,9 +1257,10 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
       delete b;
       return rc;
     }
-    const int chunk = move_chunk(s->n, delay_rank);
-    const size_t lds = precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, chunk).total
-                                              : move_lds<double>(s->N, s->nions, s->n, delay_rank, chunk).total;
+    const int chunk = move_chunk(s->n, delay_rank, s->has_j3);
+    const int mw = s->has_j3 ? s->mw : 0;
+    const size_t lds = precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, chunk, mw).total
+                                              : move_lds<double>(s->N, s->nions, s->n, delay_rank, chunk, mw).total;
     if (lds > kMoveLdsMax) {
       delete b;
       return fail(QMCX_ERR_ARG, "walker state (" + std::to_string(lds / 1024) +
@@ -1125,7 +1291,18 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
   rc |= b->alloc(b->ndelay, S);
   rc |= b->alloc(b->logdet, W * 2);
   rc |= b->alloc(b->phase, W * 2);
-  rc |= b->alloc(b->jlog, W * 2);
+  rc |= b->alloc(b->jlog, W * 3);
+  rc |= b->alloc(b->flags, 1);
+  if (s->has_j3) {
+    b->j3 = true;
+    rc |= b->alloc(p8, es * W * 5 * N);
+    b->J3 = p8;
+    rc |= b->alloc(b->j3mask, W * N * (size_t)s->mw);
+    rc |= b->alloc(p8, es * W * N * kJ3Rec);
+    b->j3near = p8;
+    rc |= b->alloc(p8, es * W * 2 * n * kJ3Rec);
+    b->j3prop = p8;
+  }
   rc |= b->alloc(b->nacc, W);
   rc |= b->alloc(b->narb, W);
   rc |= b->alloc(b->sweep, 2);
@@ -1165,12 +1342,15 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
   HIPCHK(hipEventCreateWithFlags(&b->ev_gl, hipEventDisableTiming));
   HIPCHK(hipEventCreate(&b->ev0));
   HIPCHK(hipEventCreate(&b->ev1));
+  s->nbatches += 1;
+  b->counted = true;
   *out = b;
   return 0;
 }
 
 int qmcx_batch_destroy(qmcx_batch_t b) {
   if (!b) return 0;
+  if (b->counted) b->sys->nbatches -= 1;
   if (b->stream) hipStreamSynchronize(b->stream);
   if (b->aux) hipStreamSynchronize(b->aux);
   if (b->pre) hipStreamSynchronize(b->pre);
@@ -1248,10 +1428,14 @@ int qmcx_batch_set_tile(qmcx_batch_t b, int spline_slices) {
 
 int qmcx_evaluate_log(qmcx_batch_t b, double* log_out) {
   if (!b) return fail(QMCX_ERR_ARG, "null batch");
+  if (int rc = check_j3(b)) return rc;
+  if (int rc = b->sys->sync_functors()) return rc;
   int rc = b->prec == QMCX_FP32 ? evaluate_log_impl<float>(b) : evaluate_log_impl<double>(b);
   if (rc) return rc;
   b->rows64_stale = false;   // the FP64 Slater rows are filled with the SPO rows when kept
   HIPCHK(hipStreamSynchronize(b->stream));
+  if (b->j3)
+    if (int rc2 = check_flags(b)) return rc2;
   if (log_out) HIPCHK(hipMemcpy(log_out, b->logv, sizeof(double) * b->W, hipMemcpyDeviceToHost));
   return 0;
 }
@@ -1277,6 +1461,7 @@ int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int 
     return fail(QMCX_ERR_ARG, "unknown accept mode");
   if (b->rows64 && b->rows64_stale)
     return fail(QMCX_ERR_STATE, "the accept band was switched on after qmcx_evaluate_log: call it again");
+  if (int rc = check_j3(b)) return rc;
   if (nsweeps > b->hist_cap) {
     if (b->hist) hipFree(b->hist);
     b->hist = nullptr;
@@ -1316,6 +1501,8 @@ int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int 
   HIPCHK(hipEventSynchronize(b->ev1));
   HIPCHK(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
   if (b->prof_collect()) return QMCX_ERR_HIP;
+  if (b->j3)
+    if (int rc = check_flags(b)) return rc;
   if (log_out && nsweeps > 0)
     HIPCHK(hipMemcpy(log_out, b->hist, sizeof(double) * (size_t)nsweeps * b->W, hipMemcpyDeviceToHost));
   return 0;
@@ -1342,6 +1529,8 @@ int qmcx_get_accepts(qmcx_batch_t b, long long* naccept) {
 
 int qmcx_batch_set_accept_band(qmcx_batch_t b, double band) {
   if (!b || !(band >= 0.0) || band > 1.0) return fail(QMCX_ERR_ARG, "band must be in [0, 1]");
+  if (band > 0.0 && b->j3)
+    return fail(QMCX_ERR_ARG, "the near-tie FP64 re-evaluation does not take a three-body Jastrow");
   b->band = band;
   const bool on = b->prec == QMCX_FP32 && band > 0.0;
   if (on && !b->rows64) b->rows64_stale = true;
@@ -1398,10 +1587,10 @@ static int flex_launch(qmcx_batch_t b, int iel, bool apply) {
   const bool trace_was = b->trace_on;
   b->trace_on = true;
   if (f32) {
-    const MoveLds L = move_lds<float>(s->N, s->nions, s->n, b->K, 1);
+    const MoveLds L = move_lds<float>(s->N, s->nions, s->n, b->K, 1, b->j3 ? s->mw : 0);
     rc = launch_moves<float>(b, nt, L.total, a);
   } else {
-    const MoveLds L = move_lds<double>(s->N, s->nions, s->n, b->K, 1);
+    const MoveLds L = move_lds<double>(s->N, s->nions, s->n, b->K, 1, b->j3 ? s->mw : 0);
     rc = launch_moves<double>(b, nt, L.total, a);
   }
   b->trace_on = trace_was;
@@ -1417,6 +1606,7 @@ static int flex_launch(qmcx_batch_t b, int iel, bool apply) {
 int qmcx_flex_ratio_grad(qmcx_batch_t b, int iel, const double* displ, double* ratio, double* grad_now,
                          double* grad_new) {
   if (!b || !displ || iel < 0 || iel >= b->sys->N) return fail(QMCX_ERR_ARG, "bad argument");
+  if (int rc = check_j3(b)) return rc;
   if (!b->flex_displ) {
     HIPCHK(hipMalloc(&b->flex_displ, sizeof(double) * 3 * b->W));
     HIPCHK(hipMalloc(&b->flex_decide, sizeof(int) * b->W));
@@ -1471,6 +1661,7 @@ int qmcx_flex_evaluate_gl(qmcx_batch_t b, double* log_out) {
 int qmcx_flex_evaluate_ratios(qmcx_batch_t b, int iel, int nvp, const double* vpos, double* ratios) {
   if (!b || !vpos || !ratios || iel < 0 || iel >= b->sys->N || nvp < 1 || nvp > kMaxVirtualPoints)
     return fail(QMCX_ERR_ARG, "bad argument");
+  if (b->j3) return fail(QMCX_ERR_ARG, "virtual-move ratios with a three-body Jastrow are not built");
   if (nvp > b->vr_cap) {
     HIPCHK(hipStreamSynchronize(b->stream));
     if (b->vr_pos) hipFree(b->vr_pos);
@@ -1502,6 +1693,7 @@ int qmcx_batch_set_nlpp_unit(qmcx_batch_t b, int knots_per_unit) {
 int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* spheres, int max_pairs, double* ratios,
                        int* npairs) {
   if (!b || !spheres || !(rmax > 0) || max_pairs < 1 || max_pairs > kMaxNlppPairs) return fail(QMCX_ERR_ARG, "bad argument");
+  if (b->j3) return fail(QMCX_ERR_ARG, "NonLocalPP ratios with a three-body Jastrow are not built");
   const size_t WN = (size_t)b->W * b->sys->N;
   // work-list entries and knot units are int indices on the device
   if (WN * max_pairs * kNlppKnots >= (size_t)INT_MAX) return fail(QMCX_ERR_ARG, "walkers x electrons x max_pairs too large");

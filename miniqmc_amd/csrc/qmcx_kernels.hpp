@@ -11,9 +11,10 @@
 //   Vbuf    T      [W][K][n]       stale inverse rows of the delayed electrons
 //   Binv    T      [W][K][K]       inverse of the delayed capacitance matrix
 //   rows    int    [W][K]          delayed rows;  ndelay int [W]
-//   logdet, phase, jlog double [W][2]
+//   logdet, phase double [W][2]; jlog double [W][3] (J1, J2, J3)
 #pragma once
 #include "qmcx_device.hpp"
+#include "qmcx_j3.hpp"
 
 namespace qmcx {
 
@@ -31,6 +32,10 @@ struct SysP {
   const Functor<T>* fn;       // [kMaxSpecies] J1 then [2] J2
   const Functor<double>* fn64;  // the same functors in FP64 (FP64 arbitration of the FP32 form)
   const Cell<double>* cell64;   // the minimum-image cell in FP64
+  // three-body Jastrow: functor of (ion species, spin pair) at [2 * species + pair] (L = 0 for a
+  // species without one), or null; mw = 64-bit words of an electron's ion mask, ceil(M / 64)
+  const Poly3<T>* j3;
+  int mw;
 };
 
 template <typename T>
@@ -61,7 +66,19 @@ struct BatchP {
   double* A64;
   double* spo64;
   unsigned long long* narb;   // [W] Metropolis decisions settled by the FP64 re-evaluation
+  // three-body Jastrow (with S.j3): sums J3 [W][5][N] (U, grad U xyz, lap U), ion masks j3mask
+  // [W][N][mw] (bit I: the electron is within L_I of ion I), near-ion records of the current
+  // positions j3near [W][N][kJ3Rec] and of the sweep's proposals j3prop [W][2][n][kJ3Rec]
+  T* J3;
+  unsigned long long* j3mask;
+  T* j3near;
+  T* j3prop;
+  int* flags;      // [1] error bits set by kernels (kFlag*)
 };
+
+// kernel error bits in BatchP::flags
+constexpr int kFlagJ3Near = 1;    // an electron position with more than kJ3Near ions inside the J3 cutoff
+constexpr int kFlagJ3Pairs = 2;   // a wave's J3 pairs of one move phase beyond its slots
 
 // Proposal record of one electron, [W][2][n][kProp] doubles written by k_spo_proposals:
 // J1 at the proposal (sum u, sum (d2u + 2 du/r), gradient), the displacement, the
@@ -1025,6 +1042,37 @@ __global__ __launch_bounds__(1024) void k_spo_proposals(SysP<T> S, BatchP<T> B, 
     const int k = threadIdx.x - 5;
     prop[(((size_t)w * 2 + det) * n + el) * kProp + kPropDispl + k] = k < 3 ? d[k] : u;
   }
+  if (S.j3 != nullptr && wid == 0) {
+    // the ions within their three-body cutoff of the proposal, in ion order (ballot compaction
+    // over 64 ions at a time): count, then (ion, r, ion - proposal)
+    T* rec = B.j3prop + (((size_t)w * 2 + det) * n + el) * kJ3Rec;
+    int cnt = 0;
+    for (int c0 = 0; c0 < S.M; c0 += 64) {
+      const int ci = c0 + lane;
+      bool in = false;
+      T r = 0, dx = 0, dy = 0, dz = 0;
+      if (ci < S.M) {
+        r = min_image(C, (T)(S.ions[ci] - x), (T)(S.ions[S.M + ci] - y), (T)(S.ions[2 * S.M + ci] - z), T(-1), dx,
+                      dy, dz);
+        in = r < S.j3[2 * S.species[ci]].L;
+      }
+      const unsigned long long bal = __ballot(in);
+      const int pos = cnt + __popcll(bal & ((1ull << lane) - 1ull));
+      if (in && pos < kJ3Near) {
+        T* e = rec + 1 + 5 * pos;
+        e[0] = (T)ci;
+        e[1] = r;
+        e[2] = dx;
+        e[3] = dy;
+        e[4] = dz;
+      }
+      cnt += __popcll(bal);
+    }
+    if (lane == 0) {
+      rec[0] = (T)cnt;
+      if (cnt > kJ3Near) atomicOr(B.flags, kFlagJ3Near);
+    }
+  }
   __syncthreads();   // the next electron's gather reuses the LDS
   }
 }
@@ -1314,15 +1362,29 @@ __global__ void k_distance_rows(Cell<T> C, int npts, const double* src, const in
 
 // ------------------------------------------------------------------ Jastrow from scratch
 // J1 per electron over the ions and J2 per electron over its full row; the
-// sums equal the lower-triangle recompute of TwoBodyJastrow.h:356-400.
+// sums equal the lower-triangle recompute of TwoBodyJastrow.h:356-400.  With a
+// three-body Jastrow (ThreeBodyJastrow::recompute, ThreeBodyJastrow.h:363-401): every
+// electron's near-ion record and ion mask, then every electron's sums over its pairs
+// (each pair is evaluated by both of its electrons, so each writes only its own sums).
+// LDS: positions [3N], reduction [48], functors, cell, then the ion masks [N][mw].
+template <typename T>
+__host__ __device__ inline size_t jastrow_init_lds(int N, int mw) {
+  return sizeof(double) * (3 * (size_t)N + 48) + sizeof(Functor<T>) * (kMaxSpecies + 2) + sizeof(Cell<T>) + 16 +
+         sizeof(unsigned long long) * (size_t)N * mw;
+}
 template <typename T>
 __global__ __launch_bounds__(1024) void k_jastrow_init(SysP<T> S, BatchP<T> B) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x, N = S.N, n = S.n, M = S.M;
   double* sR = (double*)smem;
   double* sRed = sR + 3 * N;
-  Functor<T>* sFn = (Functor<T>*)(sRed + 32);
+  Functor<T>* sFn = (Functor<T>*)(sRed + 48);
   Cell<T>* sCell = (Cell<T>*)(sFn + kMaxSpecies + 2);
+  unsigned long long* sMask =
+      (unsigned long long*)(smem + ((sizeof(double) * (3 * (size_t)N + 48) + sizeof(Functor<T>) * (kMaxSpecies + 2) +
+                                      sizeof(Cell<T>) + 15) & ~(size_t)15));
+  const bool j3 = S.j3 != nullptr;
+  const int mw = S.mw;
   if (tid == 0) *sCell = *S.cell;
   for (int i = tid; i < 3 * N; i += nt) sR[i] = B.R[(size_t)w * 3 * N + i];
   {
@@ -1332,17 +1394,21 @@ __global__ __launch_bounds__(1024) void k_jastrow_init(SysP<T> S, BatchP<T> B) {
   __syncthreads();
   T* J2 = B.J2 + (size_t)w * 5 * N;
   T* J1 = B.J1 + (size_t)w * 5 * N;
-  double vsum = 0, usum = 0;
+  double vsum = 0, usum = 0, tsum = 0;
   for (int i = tid; i < N; i += nt) {
     const double x = sR[i], y = sR[N + i], z = sR[2 * N + i];
     T v = 0, gx = 0, gy = 0, gz = 0, lap = 0;
+    T* rec = j3 ? B.j3near + ((size_t)w * N + i) * kJ3Rec : nullptr;
+    unsigned long long mk[kJ3MaxIons / 64] = {0, 0, 0, 0};
+    int cnt = 0;
     for (int ci = 0; ci < M; ++ci) {
       const Functor<T>& f = sFn[S.species[ci]];
-      if (f.ncoef == 0) continue;
+      const T L3 = j3 ? S.j3[2 * S.species[ci]].L : T(0);
+      if (f.ncoef == 0 && !(L3 > T(0))) continue;
       T dx, dy, dz;
       const T r = min_image(*sCell, (T)(S.ions[ci] - x), (T)(S.ions[M + ci] - y), (T)(S.ions[2 * M + ci] - z), T(-1),
                             dx, dy, dz);
-      if (r < f.rcut) {
+      if (f.ncoef > 0 && r < f.rcut) {
         T u, du, d2u;
         functor_vgl(f, r, u, du, d2u);
         v += u;
@@ -1350,6 +1416,26 @@ __global__ __launch_bounds__(1024) void k_jastrow_init(SysP<T> S, BatchP<T> B) {
         gx += du * dx;
         gy += du * dy;
         gz += du * dz;
+      }
+      if (r < L3) {
+        if (cnt < kJ3Near) {
+          T* e = rec + 1 + 5 * cnt;
+          e[0] = (T)ci;
+          e[1] = r;
+          e[2] = dx;
+          e[3] = dy;
+          e[4] = dz;
+        }
+        ++cnt;
+        mk[ci >> 6] |= 1ull << (ci & 63);
+      }
+    }
+    if (j3) {
+      rec[0] = (T)cnt;
+      if (cnt > kJ3Near) atomicOr(B.flags, kFlagJ3Near);
+      for (int q = 0; q < mw; ++q) {
+        sMask[(size_t)i * mw + q] = mk[q];
+        B.j3mask[((size_t)w * N + i) * mw + q] = mk[q];
       }
     }
     J1[i] = v;
@@ -1383,11 +1469,46 @@ __global__ __launch_bounds__(1024) void k_jastrow_init(SysP<T> S, BatchP<T> B) {
     J2[4 * N + i] = -l2;
     usum += (double)U;
   }
-  double red[2] = {vsum, usum};
-  block_sum<double, 2>(red, sRed);
+  if (j3) {
+    __syncthreads();   // every electron's mask and near-ion record are in place
+    T* J3 = B.J3 + (size_t)w * 5 * N;
+    for (int j = tid; j < N; j += nt) {
+      const double x = sR[j], y = sR[N + j], z = sR[2 * N + j];
+      const T* rec = B.j3near + ((size_t)w * N + j) * kJ3Rec;
+      const int cnt = min((int)rec[0], kJ3Near);
+      const bool jup = j < n;
+      T acc[5] = {0, 0, 0, 0, 0};
+      for (int q = 0; q < cnt; ++q) {
+        const T* e = rec + 1 + 5 * q;
+        const int I = (int)e[0];
+        const T rjI = e[1], jI[3] = {e[2], e[3], e[4]};
+        const unsigned long long bit = 1ull << (I & 63);
+        const int wd = I >> 6, sp = S.species[I];
+        for (int k = 0; k < N; ++k) {
+          if (k == j || !(sMask[(size_t)k * mw + wd] & bit)) continue;
+          T kI[3], jk[3];
+          const T rkI = min_image(*sCell, (T)(S.ions[I] - sR[k]), (T)(S.ions[M + I] - sR[N + k]),
+                                  (T)(S.ions[2 * M + I] - sR[2 * N + k]), T(-1), kI[0], kI[1], kI[2]);
+          const T rjk = min_image(*sCell, (T)(sR[k] - x), (T)(sR[N + k] - y), (T)(sR[2 * N + k] - z),
+                                  k < j ? T(1) : T(-1), jk[0], jk[1], jk[2]);
+          T o[9], js[5], ks[5];
+          poly3_vgl(S.j3[2 * sp + (((k < n) == jup) ? 0 : 1)], rjk, rjI, rkI, o);
+          j3_pair_terms(o, jk, jI, kI, js, ks);
+#pragma unroll
+          for (int c = 0; c < 5; ++c) acc[c] += js[c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 5; ++c) J3[(size_t)c * N + j] = acc[c];
+      tsum += (double)acc[0];
+    }
+  }
+  double red[3] = {vsum, usum, tsum};
+  block_sum<double, 3>(red, sRed);
   if (tid == 0) {
-    B.jlog[w * 2 + 0] = -red[0];
-    B.jlog[w * 2 + 1] = -0.5 * red[1];
+    B.jlog[w * 3 + 0] = -red[0];
+    B.jlog[w * 3 + 1] = -0.5 * red[1];
+    B.jlog[w * 3 + 2] = -0.5 * red[2];
   }
 }
 
@@ -1430,11 +1551,12 @@ __global__ __launch_bounds__(256) void k_det_gl(SysP<T> S, BatchP<T> B, double* 
 template <typename T>
 __global__ __launch_bounds__(256) void k_finish_gl(SysP<T> S, BatchP<T> B, double* G, double* Lp, double* logv,
                                                    double* hist, int hist_cap) {
-  __shared__ double sRed[32];
+  __shared__ double sRed[48];
   const int w = blockIdx.x, N = S.N;
   const T* J2 = B.J2 + (size_t)w * 5 * N;
   const T* J1 = B.J1 + (size_t)w * 5 * N;
-  double vs = 0, us = 0;
+  const T* J3 = S.j3 != nullptr ? B.J3 + (size_t)w * 5 * N : nullptr;
+  double vs = 0, us = 0, ts = 0;
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
     const size_t o = (size_t)w * N + i;
     G[3 * o] += (double)J1[N + i] + (double)J2[N + i];
@@ -1443,13 +1565,21 @@ __global__ __launch_bounds__(256) void k_finish_gl(SysP<T> S, BatchP<T> B, doubl
     Lp[o] += -(double)J1[4 * N + i] + (double)J2[4 * N + i];
     vs += (double)J1[i];
     us += (double)J2[i];
+    if (J3 != nullptr) {   // ThreeBodyJastrow::evaluateGL (ThreeBodyJastrow.h:627-645)
+      G[3 * o] += (double)J3[N + i];
+      G[3 * o + 1] += (double)J3[2 * N + i];
+      G[3 * o + 2] += (double)J3[3 * N + i];
+      Lp[o] += (double)J3[4 * N + i];
+      ts += (double)J3[i];
+    }
   }
-  double red[2] = {vs, us};
-  block_sum<double, 2>(red, sRed);
+  double red[3] = {vs, us, ts};
+  block_sum<double, 3>(red, sRed);
   if (threadIdx.x == 0) {
-    B.jlog[w * 2 + 0] = -red[0];
-    B.jlog[w * 2 + 1] = -0.5 * red[1];
-    const double v = B.logdet[w * 2] + B.logdet[w * 2 + 1] - red[0] - 0.5 * red[1];
+    B.jlog[w * 3 + 0] = -red[0];
+    B.jlog[w * 3 + 1] = -0.5 * red[1];
+    B.jlog[w * 3 + 2] = -0.5 * red[2];
+    const double v = B.logdet[w * 2] + B.logdet[w * 2 + 1] - red[0] - 0.5 * red[1] - 0.5 * red[2];
     logv[w] = v;
     if (hist != nullptr) {
       const int slot = B.sweep[0] - B.sweep[1];

@@ -33,8 +33,15 @@ __host__ __device__ inline size_t flush_own_lds(int n, int K) {
 
 struct MoveLds {
   unsigned oR, oRed, oInv, oPsi, oBinv, oP, oPart, oRows, oAl, oJ1n, oFn, oCell, oProp, oJ1o, oJt, oJ2, oAc, oQ, oAcc,
-      oArb, total;
+      oArb, oJ3, oJ3m, oJ3np, oJ3ne, oJ3k, oJ3v, oJ3r, total;
 };
+// per-wave pair slots of one J3 move phase: codes [waves][kJ3Slots] and k-side terms
+// [waves][5][kJ3Slots]
+constexpr int kJ3Slots = 32;
+template <typename T>
+__host__ __device__ inline size_t j3_slots_lds(int waves) {
+  return (size_t)waves * kJ3Slots * (sizeof(int) + 5 * sizeof(T));
+}
 // LDS of the FP64 re-evaluation of a near-tie decision (FP32 form, K > 1): three n-vectors and
 // 2K + 32 reduction slots
 __host__ __device__ inline size_t arb_lds(int n, int K) { return sizeof(double) * (3 * (size_t)n + 2 * (size_t)K + 32); }
@@ -44,9 +51,11 @@ __host__ __device__ inline unsigned lds_take(unsigned& o, size_t bytes) {
   return r;
 }
 // E = electrons per chunk; the inverse-row cache is used when K > 1 (M, the ion count, is
-// not needed: the J1 terms come with the proposals)
+// not needed: the J1 terms come with the proposals).  mw > 0: a three-body Jastrow with ion
+// masks of mw words (its sums, masks and the chunk's near-ion records stay for the launch; its
+// pair slots share the region past the J2 pair terms)
 template <typename T>
-__host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E) {
+__host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E, int mw = 0) {
   (void)M;
   MoveLds L;
   unsigned o = 0;
@@ -67,6 +76,10 @@ __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E) {
   L.oProp = lds_take(o, sizeof(double) * kProp * E);        // the chunk's proposal records
   L.oJ1o = lds_take(o, sizeof(T) * 4 * E);                  // J1 value and gradient of its electrons
   L.oJ2 = lds_take(o, sizeof(T) * 5 * N);
+  L.oJ3 = lds_take(o, mw > 0 ? sizeof(T) * 5 * N : 0);
+  L.oJ3m = lds_take(o, sizeof(unsigned long long) * (size_t)N * mw);
+  L.oJ3np = lds_take(o, mw > 0 ? sizeof(T) * kJ3Rec * E : 0);
+  L.oJ3ne = lds_take(o, mw > 0 ? sizeof(T) * kJ3Rec * E : 0);
   L.oAc = lds_take(o, cache ? sizeof(T) * (size_t)E * ac_stride<T>(n) : 0);
   L.oQ = lds_take(o, cache ? sizeof(double) * (size_t)K * E : 0);
   // J2 pair terms at the current proposal; with the region before it, the scratch of the
@@ -74,7 +87,15 @@ __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E) {
   L.oJt = lds_take(o, sizeof(T) * 5 * N);
   // the in-kernel flush (FP32) reuses the regions from oQ on, grown to its size when the
   // workgroup's LDS allows
-  L.oArb = o;   // during the moves the region past oJt belongs to the FP64 re-evaluation
+  L.oArb = o;   // during the moves the region past oJt belongs to the FP64 re-evaluation or the J3 pairs
+  {
+    unsigned p = o;
+    const int waves = kMoveThreads / 64;
+    L.oJ3k = lds_take(p, mw > 0 ? 2 * j3_slots_lds<T>(waves) : 0);   // new and old phase slots
+    L.oJ3v = L.oJ3k + (unsigned)(2 * sizeof(int) * waves * kJ3Slots);
+    L.oJ3r = lds_take(p, mw > 0 ? sizeof(double) * 16 * 5 : 0);    // j-side wave partials
+    if (mw > 0) o = p;
+  }
   if (cache && sizeof(T) == 4) {
     const size_t need = L.oQ + flush_own_lds(n, K);
     if (need > o && need <= kMoveLdsMax) o = (unsigned)need;
@@ -511,10 +532,79 @@ __device__ inline double arb_jastrow_ratio(const SysP<T>& S, const double* sR, i
   return exp((v[0] - v[1]) + (v[2] - v[3]));
 }
 
+// ---- three-body Jastrow of one move phase (ThreeBodyJastrow::computeU3 with computeU3_engine,
+// ThreeBodyJastrow.h:441-625) for the moving electron iel at position p with near-ion record rec.
+// Every wave takes the pairs (k, ion q of rec) of the electrons k its own lanes hold (k = tid +
+// qq nt) whose ion mask has the ion, compacts them into its kJ3Slots slots (slot = ballot rank,
+// in (q, qq, lane) order), and lane r evaluates the functor for slot r: its j-side terms are
+// returned in js (zero for lanes without a slot), the k-side terms go to the slot's row of sv
+// [5][kJ3Slots] beside its code k | q << 16 in sk.  Returns the wave's slot count (more pairs
+// than slots raise kFlagJ3Pairs).  Deterministic: the slot order depends only on the
+// configuration.
+template <typename T, int JPT>
+__device__ inline int j3_phase(const SysP<T>& S, const Cell<T>& C, const double* sR, const unsigned long long* sMask,
+                               const T* rec, int iel, double px, double py, double pz, int tid, int nt, int lane,
+                               int* sk, T* sv, T js[5], int* flags) {
+  const int N = S.N, n = S.n, M = S.M, mw = S.mw;
+  const int cnt = (int)rec[0];
+  const int nI = cnt < kJ3Near ? cnt : kJ3Near;
+  int ns = 0;
+  for (int q = 0; q < nI; ++q) {
+    const int I = (int)rec[1 + 5 * q];
+    const unsigned long long bit = 1ull << (I & 63);
+    const int wd = I >> 6;
+#pragma unroll
+    for (int qq = 0; qq < JPT; ++qq) {
+      const int k = tid + qq * nt;
+      const bool in = k < N && k != iel && (sMask[(size_t)k * mw + wd] & bit) != 0ull;
+      const unsigned long long bal = __ballot(in);
+      const int pos = ns + __popcll(bal & ((1ull << lane) - 1ull));
+      if (in && pos < kJ3Slots) sk[pos] = k | (q << 16);
+      ns += __popcll(bal);
+    }
+  }
+  if (lane == 0 && (ns > kJ3Slots || cnt > kJ3Near)) atomicOr(flags, ns > kJ3Slots ? kFlagJ3Pairs : kFlagJ3Near);
+  ns = ns < kJ3Slots ? ns : kJ3Slots;
+  wave_lds_fence();   // the slots are read by other lanes of the wave
+#pragma unroll
+  for (int c = 0; c < 5; ++c) js[c] = T(0);
+  if (lane < ns) {
+    const int code = sk[lane], k = code & 0xffff, q = code >> 16;
+    const T* e = rec + 1 + 5 * q;
+    const int I = (int)e[0];
+    const T jI[3] = {e[2], e[3], e[4]};
+    T kI[3], jk[3], o[9], ks[5];
+    const T rkI = min_image(C, (T)(S.ions[I] - sR[k]), (T)(S.ions[M + I] - sR[N + k]),
+                            (T)(S.ions[2 * M + I] - sR[2 * N + k]), T(-1), kI[0], kI[1], kI[2]);
+    const T rjk = min_image(C, (T)(sR[k] - px), (T)(sR[N + k] - py), (T)(sR[2 * N + k] - pz),
+                            k < iel ? T(1) : T(-1), jk[0], jk[1], jk[2]);
+    poly3_vgl(S.j3[2 * S.species[I] + (((k < n) == (iel < n)) ? 0 : 1)], rjk, e[1], rkI, o);
+    j3_pair_terms(o, jk, jI, kI, js, ks);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) sv[c * kJ3Slots + lane] = ks[c];
+  }
+  return ns;
+}
+
+// Adds sign times the k-side terms of the wave's ns slots to the sums sJ3 [5][N], one near ion
+// at a time (an electron appears at most once per ion), so the order of additions is fixed.
+template <typename T>
+__device__ inline void j3_apply(T* sJ3, int N, const int* sk, const T* sv, int ns, int nI, T sign, int lane) {
+  for (int q = 0; q < nI; ++q) {
+    if (lane < ns && (sk[lane] >> 16) == q) {
+      const int k = sk[lane] & 0xffff;
+#pragma unroll
+      for (int c = 0; c < 5; ++c) sJ3[(size_t)c * N + k] += sign * sv[c * kJ3Slots + lane];
+    }
+    wave_lds_fence();
+  }
+}
+
 // KD: delayed updates (K > 1: inverse-row cache, Woodbury bordering) or Sherman-Morrison (K = 1).
 // ARB: the FP64 re-evaluation of near-tie Metropolis decisions is compiled in (FP32, K > 1); only
 // that instantiation carries its code, whose registers would otherwise spill in every move.
-template <typename T, int JPT, bool KD, bool ARB>
+// J3: the three-body Jastrow is compiled in (its ratio, gradient and update in every move).
+template <typename T, int JPT, bool KD, bool ARB, bool J3>
 __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
@@ -522,7 +612,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   const int N = S.N, n = S.n, K = KD ? B.K : 1;
   const int KP = K + 1;   // LDS row stride of Binv: a column walk down its rows is free of bank conflicts
   const int Emax = a.chunk_len;   // electrons per chunk (the LDS layout is sized for it)
-  const MoveLds L = move_lds<T>(N, S.M, n, K, Emax);
+  const MoveLds L = move_lds<T>(N, S.M, n, K, Emax, J3 ? S.mw : 0);
   double* sR = (double*)(smem + L.oR);
   double* sRed = (double*)(smem + L.oRed);
   T* sInv = (T*)(smem + L.oInv);
@@ -542,6 +632,19 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   T* sAc = (T*)(smem + L.oAc);
   const int sa = ac_stride<T>(n);   // its row stride
   double* sQ = (double*)(smem + L.oQ);
+  // three-body Jastrow: sums [5][N], ion masks [N][mw], near-ion records of the chunk's
+  // proposals and current positions [E][kJ3Rec], this wave's pair slots of the new and the old
+  // phase, the j-side wave partials [16][5]
+  constexpr int kWaves = kMoveThreads / 64;
+  T* sJ3 = (T*)(smem + L.oJ3);
+  unsigned long long* sMask = (unsigned long long*)(smem + L.oJ3m);
+  T* sJ3np = (T*)(smem + L.oJ3np);
+  T* sJ3ne = (T*)(smem + L.oJ3ne);
+  int* sJk = (int*)(smem + L.oJ3k) + wid * kJ3Slots;
+  int* sJkO = (int*)(smem + L.oJ3k) + (kWaves + wid) * kJ3Slots;
+  T* sJv = (T*)(smem + L.oJ3v) + (size_t)wid * 5 * kJ3Slots;
+  T* sJvO = (T*)(smem + L.oJ3v) + (size_t)(kWaves + wid) * 5 * kJ3Slots;
+  double* sJr = (double*)(smem + L.oJ3r);
   // FP64 re-evaluation of near ties (FP32 form, Metropolis): the move's inverse row in FP64
   const bool arb_on = ARB && sizeof(T) == 4 && KD && a.band > 0.0 && a.mode == 1 && a.decide == nullptr && n <= nt;
   double* sArbX = (double*)(smem + L.oArb);
@@ -567,12 +670,18 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
                            {sCell, S.cell, (unsigned)sizeof(Cell<T>)}};
     stage_lds(cp);
   }
+  if constexpr (J3) {
+    const T* g3 = B.J3 + (size_t)w * 5 * N;
+    for (int i = tid; i < 5 * N; i += nt) sJ3[i] = g3[i];
+    const unsigned long long* gm = B.j3mask + (size_t)w * N * S.mw;
+    for (int i = tid; i < N * S.mw; i += nt) sMask[i] = gm[i];
+  }
   // a caller's accept decision (the flex stage calls) replaces the uniform: -inf accepts, +inf rejects
   const double flex_u = a.decide == nullptr ? 0.0 : (a.decide[w] != 0 ? -INFINITY : INFINITY);
   // sums over the launch kept by thread 0 in LDS: log|det| (logd, and the running product pdet
   // folded into it before it leaves the double range), phase, J1 and J2 log changes, accepts
   double* sAcc = (double*)(smem + L.oAcc);
-  enum { aLogd, aPdet, aPh, aJl1, aJl2, aNacc, aNarb };
+  enum { aLogd, aPdet, aPh, aJl1, aJl2, aNacc, aNarb, aJl3 };
   if (tid == 0) {
     sAcc[aLogd] = 0;
     sAcc[aPdet] = 1;
@@ -581,6 +690,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     sAcc[aJl2] = 0;
     sAcc[aNacc] = 0;
     sAcc[aNarb] = 0;
+    sAcc[aJl3] = 0;
   }
   const Cell<T>& C = *sCell;   // read from LDS where used (a register copy costs spills)
   const int jr = tid < n ? tid : n - 1;
@@ -611,6 +721,15 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
                            {sJ1o + 3 * E, J1 + 3 * N + cb, je}};
     stage_lds(cp);
   }
+  if constexpr (J3) {
+    // the near-ion records of the chunk's proposals and of its electrons' current positions
+    const T* gp = B.j3prop + (((size_t)w * 2 + a.det) * n + r0) * kJ3Rec;
+    const T* ge = B.j3near + ((size_t)w * N + cb) * kJ3Rec;
+    for (int i = tid; i < E * kJ3Rec; i += nt) {
+      sJ3np[i] = gp[i];
+      sJ3ne[i] = ge[i];
+    }
+  }
   QMCX_TICK(16)
   QMCX_TICK(17)
   int c = 0, ca = 0;   // delayed rows; accepted moves of the chunk
@@ -639,6 +758,13 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     // displacement, uniform
     const T Vat_old = sJ1o[e], g1x = sJ1o[E + e], g1y = sJ1o[2 * E + e], g1z = sJ1o[3 * E + e];
     const T Uat_old = sJ2[iel], g2x = sJ2[N + iel], g2y = sJ2[2 * N + iel], g2z = sJ2[3 * N + iel];
+    T u3o = 0, g3x = 0, g3y = 0, g3z = 0;
+    if constexpr (J3) {
+      u3o = sJ3[iel];
+      g3x = sJ3[N + iel];
+      g3y = sJ3[2 * N + iel];
+      g3z = sJ3[3 * N + iel];
+    }
     const T dp0 = ndp[0], dp1 = ndp[1], dp2 = ndp[2];
     T p5[5];
 #pragma unroll
@@ -772,16 +898,34 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       }
       QMCX_TICK(7)
     }
+    int ns3 = 0;   // this wave's J3 pair slots at the proposal
+    if constexpr (J3) {
+      T js[5];
+      ns3 = j3_phase<T, JPT>(S, C, sR, sMask, sJ3np + (size_t)e * kJ3Rec, iel, nx, ny, nz, tid, nt, lane, sJk, sJv,
+                             js, B.flags);
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        const double t = wave_sum((double)js[c]);
+        if (lane == 0) sJr[wid * 5 + c] = t;
+      }
+    }
     lds_sync();
     QMCX_TICK(2)
 
     double red[kRed];
     combine_partials(sRed, nw, red);
+    // J3 at the proposal: U, grad U, lap U of the moving electron (the wave partials in order)
+    double cur3[5] = {0, 0, 0, 0, 0};
+    if constexpr (J3)
+      for (int q = 0; q < nw; ++q)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) cur3[c] += sJr[q * 5 + c];
     QMCX_TICK(3)
 
     // ---- phase 4: acceptance and updates
     double rdet = red[3];
-    double ratio = rdet * exp((double)(Vat_old - (T)pr[kPropJ1]) + (double)(Uat_old - (T)red[7]));
+    double ratio = rdet * exp((double)(Vat_old - (T)pr[kPropJ1]) + (double)(Uat_old - (T)red[7]) +
+                              (J3 ? (double)(u3o - (T)cur3[0]) : 0.0));
     const double uacc = a.decide == nullptr ? pr[kPropU] : flex_u;
     bool accept = a.mode == 0 ? (uacc < 0.5) : (uacc < ratio * ratio);
     if constexpr (ARB && sizeof(T) == 4 && KD) {
@@ -804,12 +948,12 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
       t[0] = ratio;
       t[1] = rdet;
       t[2] = accept ? 1.0 : 0.0;
-      t[3] = red[0] + (double)g1x + (double)g2x;
-      t[4] = red[1] + (double)g1y + (double)g2y;
-      t[5] = red[2] + (double)g1z + (double)g2z;
-      t[6] = red[4] / rdet + pr[kPropJ1 + 2] + red[8];
-      t[7] = red[5] / rdet + pr[kPropJ1 + 3] + red[9];
-      t[8] = red[6] / rdet + pr[kPropJ1 + 4] + red[10];
+      t[3] = red[0] + (double)g1x + (double)g2x + (double)g3x;
+      t[4] = red[1] + (double)g1y + (double)g2y + (double)g3y;
+      t[5] = red[2] + (double)g1z + (double)g2z + (double)g3z;
+      t[6] = red[4] / rdet + pr[kPropJ1 + 2] + red[8] + cur3[1];
+      t[7] = red[5] / rdet + pr[kPropJ1 + 3] + red[9] + cur3[2];
+      t[8] = red[6] / rdet + pr[kPropJ1 + 4] + red[10] + cur3[3];
     }
     if (accept) {
       if (tid == 0) {
@@ -824,6 +968,7 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
         sAcc[aPh] += rdet > 0.0 ? 0.0 : M_PI;
         sAcc[aJl1] += (double)(Vat_old - (T)pr[kPropJ1]);
         sAcc[aJl2] += (double)(Uat_old - (T)red[7]);
+        if constexpr (J3) sAcc[aJl3] += (double)(u3o - (T)cur3[0]);
         sAcc[aNacc] += 1.0;
         // the new J1 state of the electron (OneBodyJastrow.h:271-283), written at the chunk's end
         sAl[ca] = e;
@@ -970,6 +1115,31 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
           sR[2 * N + iel] = nz;
         }
       }
+      if constexpr (J3) {
+        // J3 (ThreeBodyJastrow::acceptMove, ThreeBodyJastrow.h:302-361): the pair terms at the old
+        // position, then every other electron's sums moved by new - old (each wave applies its
+        // own electrons' slots), the moving electron's sums set to phase 2's and its ion mask to
+        // the proposal's near ions
+        T jso[5];
+        const T* reco = sJ3ne + (size_t)e * kJ3Rec;
+        const T* recn = sJ3np + (size_t)e * kJ3Rec;
+        const int ns3o = j3_phase<T, JPT>(S, C, sR, sMask, reco, iel, x0, y0, z0, tid, nt, lane, sJkO, sJvO, jso,
+                                          B.flags);
+        const int nIn = min((int)recn[0], kJ3Near), nIo = min((int)reco[0], kJ3Near);
+        j3_apply<T>(sJ3, N, sJk, sJv, ns3, nIn, T(1), lane);
+        j3_apply<T>(sJ3, N, sJkO, sJvO, ns3o, nIo, T(-1), lane);
+        if (tid == 0)
+#pragma unroll
+          for (int c2 = 0; c2 < 5; ++c2) sJ3[(size_t)c2 * N + iel] = (T)cur3[c2];
+        if (tid < S.mw) {
+          unsigned long long m = 0;
+          for (int q = 0; q < nIn; ++q) {
+            const int I = (int)recn[1 + 5 * q];
+            if ((I >> 6) == tid) m |= 1ull << (I & 63);
+          }
+          sMask[(size_t)iel * S.mw + tid] = m;
+        }
+      }
       QMCX_TICK(15)
     }
     // with K = 1 the next move reads rows of Ainv that other waves just wrote
@@ -981,6 +1151,11 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   // ---- chunk end: the accepted rows' Slater matrix, gradient and Laplacian rows
   // (DiracDeterminant.cpp:135-141) copied from the proposals' SPO rows, the stale inverse rows
   // for the flush, the new J1 state
+  if constexpr (J3)   // the accepted electrons' near-ion records are their proposals'
+    for (int i = tid; i < ca * kJ3Rec; i += nt) {
+      const int d = i / kJ3Rec, q = i - d * kJ3Rec;
+      B.j3near[((size_t)w * N + cb + sAl[d]) * kJ3Rec + q] = sJ3np[(size_t)sAl[d] * kJ3Rec + q];
+    }
   for (int d = tid; d < ca; d += nt) {
     const int ie = cb + sAl[d];
     J1[ie] = sJ1n[d * 5 + 0];
@@ -1086,11 +1261,18 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
   // ---- walker state back to HBM
   for (int i = tid; i < 3 * N; i += nt) B.R[(size_t)w * 3 * N + i] = sR[i];
   for (int i = tid; i < 5 * N; i += nt) J2[i] = sJ2[i];
+  if constexpr (J3) {
+    T* g3 = B.J3 + (size_t)w * 5 * N;
+    for (int i = tid; i < 5 * N; i += nt) g3[i] = sJ3[i];
+    unsigned long long* gm = B.j3mask + (size_t)w * N * S.mw;
+    for (int i = tid; i < N * S.mw; i += nt) gm[i] = sMask[i];
+    if (tid == 0) B.jlog[w * 3 + 2] += sAcc[aJl3];
+  }
   if (tid == 0) {
     B.logdet[w * 2 + a.det] += sAcc[aLogd] + log(sAcc[aPdet]);
     B.phase[w * 2 + a.det] += sAcc[aPh];
-    B.jlog[w * 2 + 0] += sAcc[aJl1];
-    B.jlog[w * 2 + 1] += sAcc[aJl2];
+    B.jlog[w * 3 + 0] += sAcc[aJl1];
+    B.jlog[w * 3 + 1] += sAcc[aJl2];
     B.nacc[w] += (unsigned long long)sAcc[aNacc];
     if (B.narb != nullptr) B.narb[w] += (unsigned long long)sAcc[aNarb];
   }

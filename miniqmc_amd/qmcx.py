@@ -39,6 +39,7 @@ _SIGS = {
     "qmcx_system_destroy": [_vp],
     "qmcx_system_set_j1": [_vp, _i, _i, C.c_double, C.c_double, _dp],
     "qmcx_system_set_j2": [_vp, _i, _i, C.c_double, C.c_double, _dp],
+    "qmcx_system_set_j3": [_vp, _i, _i, _i, _i, C.c_double, _dp, _i],
     "qmcx_system_set_spline": [_vp, _ip, _i, _i, _fp, _dp, _i],
     "qmcx_spline_vgh": [_vp, _i, _i, _dp, _dp],
     "qmcx_distance_rows": [_vp, _i, _i, _dp, _ip, _i, _dp, _dp, _dp],
@@ -185,7 +186,7 @@ class DelayedUpdate:
 
 
 class System:
-    """Device-resident system description (lattice, ions, J1/J2 functors, SPO table)."""
+    """Device-resident system description (lattice, ions, J1/J2 (and J3) functors, SPO table)."""
 
     def __init__(self, sysd, precisions=(FP32, FP64)):
         L = load()
@@ -199,6 +200,8 @@ class System:
             _check(L.qmcx_system_set_j1(h, s, f.nparams, f.rcut, f.cusp, _c(f.params)))
         for p, f in enumerate(sysd.j2):
             _check(L.qmcx_system_set_j2(h, p, f.nparams, f.rcut, f.cusp, _c(f.params)))
+        for f in getattr(sysd, "j3", []) or []:   # ThreeBodyJastrow::addFunc
+            _check(L.qmcx_system_set_j3(h, f.species, f.pair, f.neI, f.nee, f.rcut, _c(f.params), len(f.params)))
         mask = (1 if FP32 in precisions else 0) | (2 if FP64 in precisions else 0)
         _check(L.qmcx_system_set_spline(h, _c(sysd.ng, np.int32), sysd.norb, sysd.npad,
                                         np.ascontiguousarray(sysd.coefs, dtype=np.float32),
