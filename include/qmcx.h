@@ -140,6 +140,15 @@ int qmcx_evaluate_log(qmcx_batch_t b, double* log_out);
 int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int accept_mode, int recompute_every,
                     double* log_out);
 
+/* Graph replay of qmcx_run_sweeps (no reference counterpart; the reference issues every kernel from the
+ * host): with enable != 0 a run is captured into a HIP graph on its first call (the three streams' launches
+ * and event dependencies of all its sweeps) and replayed by later calls with the same nsweeps, seed, tau,
+ * acceptance rule and refresh cadence; other parameters capture a new graph.  The sweep numbers of the
+ * random streams and of the log history come from device counters, so a replay continues the chains
+ * exactly as the launches would.  Changing a batch setting (trace, flush kind, tile, band, walker offset)
+ * drops the graph.  Not with kernel timing on. */
+int qmcx_batch_set_graph(qmcx_batch_t b, int enable);
+
 /* ---- flex_* stage calls (the batched boundary of src/Drivers/miniqmc_sync_move.cpp:459-513) ----
  * One electron of every walker per call.
  * qmcx_flex_ratio_grad: ParticleSet::flex_setActive + WaveFunction::flex_evalGrad + ParticleSet::flex_makeMove

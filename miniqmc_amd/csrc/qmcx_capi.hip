@@ -263,6 +263,18 @@ struct qmcx_system_s {
 };
 
 
+// the parameters a captured run depends on
+struct GraphKey {
+  int nsweeps;
+  uint64_t seed;
+  double tau;
+  int mode, recompute_every;
+  bool operator==(const GraphKey& o) const {
+    return nsweeps == o.nsweeps && seed == o.seed && tau == o.tau && mode == o.mode &&
+           recompute_every == o.recompute_every;
+  }
+};
+
 struct qmcx_batch_s {
   qmcx_system_s* sys = nullptr;
   int W = 0, K = 1, prec = QMCX_FP64, walker0 = 0, slices = 0, nt = 0;
@@ -319,6 +331,15 @@ struct qmcx_batch_s {
   void* j3prop = nullptr;
   bool j3 = false;
   bool counted = false;   // counted in its system's live batches
+  // graph replay of whole runs (qmcx_batch_set_graph): the instantiated graph of the last run
+  // and the run parameters it was captured with
+  bool graph_on = false;
+  hipGraphExec_t gexec = nullptr;
+  GraphKey gkey{};
+  void drop_graph() {
+    if (gexec) hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+  }
   int* flags = nullptr;   // [1] kernel error bits
   double* A64 = nullptr;   // FP32 form: [W][2][n][n] FP64 Slater rows (the FP64 refresh inverts these)
   double* spo64 = nullptr; // FP32 form: [W][2][n][npad] FP64 values at the sweep's proposals
@@ -484,7 +505,7 @@ constexpr int kMaxNlppPairs = 64;
 constexpr int kNlppWorkgroups = 8192;
 
 template <typename T>
-int launch_proposals(qmcx_batch_t b, hipStream_t st, int det, int e0, int ne, uint64_t seed, int sweep, double tau,
+int launch_proposals(qmcx_batch_t b, hipStream_t st, int det, int e0, int ne, uint64_t seed, double tau,
                      const double* displ) {
   const qmcx_system_s* s = b->sys;
   const int nq = s->npad / (16 / (int)sizeof(T));
@@ -498,7 +519,7 @@ int launch_proposals(qmcx_batch_t b, hipStream_t st, int det, int e0, int ne, ui
   // HBM traffic stays below what slows the latency-bound move kernel beside it
   const int gx = std::max(1, std::min(ne, kPropWorkgroups / b->W));
   hipLaunchKernelGGL(k_spo_proposals<T>, dim3(gx, b->W), dim3(nt), lds, st, s->params<T>(), b->params<T>(), det, e0,
-                     ne, seed, sweep, std::sqrt(tau), b->walker0, displ, (T*)b->spo, b->prop, slices);
+                     ne, seed, b->sweep, std::sqrt(tau), b->walker0, displ, (T*)b->spo, b->prop, slices);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -693,22 +714,26 @@ int hold_stream(hipStream_t st, int us) {
   return 0;
 }
 
-// SPO rows of determinant det's proposals in sweep `sweep` on the proposals stream, after
-// the event `after` (positions final), announced by ev_pre[det]; `hold` when a move launch is
-// released by the same event
+// SPO rows of determinant det's proposals on the proposals stream, after the event `after`
+// (positions final), announced by ev_pre[det]; `hold` when a move launch is released by the same
+// event.  Their sweep number (the random streams' counter) is the device counter sweep[2], which
+// only this stream reads and advances, after the second determinant's proposals.
 template <typename T>
-int issue_proposals(qmcx_batch_t b, int det, hipEvent_t after, uint64_t seed, int sweep, double tau,
-                    bool hold = false) {
+int issue_proposals(qmcx_batch_t b, int det, hipEvent_t after, uint64_t seed, double tau, bool hold = false) {
   if (after) HIPCHK(hipStreamWaitEvent(b->pre, after, 0));
   if (hold && hold_stream(b->pre, det == 0 ? QMCX_HOLD_PROP0_US : QMCX_HOLD_PROP1_US)) return QMCX_ERR_HIP;
   if (b->prof_record(4, b->pre)) return QMCX_ERR_HIP;
-  if (int rc = launch_proposals<T>(b, b->pre, det, 0, b->sys->n, seed, sweep, tau, nullptr)) return rc;
+  if (int rc = launch_proposals<T>(b, b->pre, det, 0, b->sys->n, seed, tau, nullptr)) return rc;
   if (b->prof_close(b->pre)) return QMCX_ERR_HIP;
+  if (det == 1) {
+    hipLaunchKernelGGL(k_count, dim3(1), dim3(64), 0, b->pre, b->sweep + 2);
+    HIPCHK(hipGetLastError());
+  }
   HIPCHK(hipEventRecord(b->ev_pre[det], b->pre));
   return 0;
 }
 
-// One sweep (absolute sweep number `sweep`; `slot` in this run).  A determinant's moves wait
+// One sweep.  A determinant's moves wait
 // for its proposals' SPO rows; as soon as they are done, the next sweep's rows for that
 // determinant are evaluated on the proposals stream (unless this is the run's last sweep).
 // With `recompute`, each determinant's inverse and log|det| are refreshed in FP64 on the
@@ -718,12 +743,12 @@ int issue_proposals(qmcx_batch_t b, int det, hipEvent_t after, uint64_t seed, in
 // corrects this sweep's recorded log|Psi| (k_refresh_det).  The next sweep's
 // second-determinant moves wait for it.
 template <typename T>
-int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompute, int slot, int sweep, bool last) {
+int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompute, bool last) {
   HIPCHK(hipStreamWaitEvent(b->stream, b->ev_pre[0], 0));
   if (int rc = moves_det<T>(b, 0, mode, !recompute)) return rc;
   HIPCHK(hipEventRecord(b->ev_done[0], b->stream));
   if (!last)
-    if (int rc = issue_proposals<T>(b, 0, b->ev_done[0], seed, sweep + 1, tau, true)) return rc;
+    if (int rc = issue_proposals<T>(b, 0, b->ev_done[0], seed, tau, true)) return rc;
   if (recompute) {
     HIPCHK(hipStreamWaitEvent(b->aux, b->ev_done[0], 0));
     if (hold_stream(b->aux, QMCX_HOLD_REFRESH_US)) return QMCX_ERR_HIP;
@@ -738,7 +763,7 @@ int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompu
   if (int rc = moves_det<T>(b, 1, mode, true)) return rc;
   HIPCHK(hipEventRecord(b->ev_done[1], b->stream));
   if (!last)
-    if (int rc = issue_proposals<T>(b, 1, b->ev_done[1], seed, sweep + 1, tau, true)) return rc;
+    if (int rc = issue_proposals<T>(b, 1, b->ev_done[1], seed, tau, true)) return rc;
   if (recompute) HIPCHK(hipStreamWaitEvent(b->stream, b->ev_fresh[0], 0));
   if (b->prof_record(3)) return QMCX_ERR_HIP;
   if (int rc = launch_gl<T>(b, true)) return rc;
@@ -749,7 +774,7 @@ int sweep_impl(qmcx_batch_t b, uint64_t seed, double tau, int mode, bool recompu
     if (hold_stream(b->aux, QMCX_HOLD_REFRESH_US)) return QMCX_ERR_HIP;
     if (int rc = refresh_det<T>(b, 1, true)) return rc;
     hipLaunchKernelGGL(k_refresh_det, dim3((b->W + 63) / 64), dim3(64), 0, b->aux, b->W, 1, b->ld_fresh, b->ph_fresh,
-                       b->logdet, b->phase, b->logv, b->hist, slot);
+                       b->logdet, b->phase, b->logv, b->hist, b->hist_cap, b->sweep);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(b->ev_fresh[1], b->aux));
     b->pending_dn = true;
@@ -795,6 +820,31 @@ int scratch_of_current_device(double** out) {
 }
 
 }  // namespace
+
+// The launches of one run of nsweeps sweeps, on the batch's three streams: the first sweep's
+// proposals, the sweeps, and the wait for the last refresh.  Nothing here synchronizes with the
+// host or clears memory, so the same sequence can be captured into a graph.
+static int enqueue_run(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int accept_mode, int recompute_every) {
+  const bool f32 = b->prec == QMCX_FP32;
+  // the first sweep's proposals, after whatever the batch stream did before this run
+  HIPCHK(hipEventRecord(b->ev_start, b->stream));
+  for (int det = 0; det < 2; ++det) {
+    int rc = f32 ? issue_proposals<float>(b, det, b->ev_start, seed, tau)
+                 : issue_proposals<double>(b, det, b->ev_start, seed, tau);
+    if (rc) return rc;
+  }
+  for (int sw = 0; sw < nsweeps; ++sw) {
+    const bool rec = recompute_every > 0 && (sw + 1) % recompute_every == 0;
+    const bool last = sw + 1 == nsweeps;
+    int rc = f32 ? sweep_impl<float>(b, seed, tau, accept_mode, rec, last)
+                 : sweep_impl<double>(b, seed, tau, accept_mode, rec, last);
+    if (rc) return rc;
+    if (b->prof_on && b->prof_used > 4096 && b->prof_collect()) return QMCX_ERR_HIP;
+    hipLaunchKernelGGL(k_count, dim3(1), dim3(64), 0, b->stream, b->sweep);
+    HIPCHK(hipGetLastError());
+  }
+  return settle_refresh(b);
+}
 
 // ====================================================================== C ABI
 // A batch keeps three-body Jastrow state only when its system had one at its creation.
@@ -1305,7 +1355,7 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
   }
   rc |= b->alloc(b->nacc, W);
   rc |= b->alloc(b->narb, W);
-  rc |= b->alloc(b->sweep, 2);
+  rc |= b->alloc(b->sweep, 4);
   rc |= b->alloc(b->G, W * N * 3);
   rc |= b->alloc(b->L, W * N);
   rc |= b->alloc(b->logv, W);
@@ -1351,6 +1401,7 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
 int qmcx_batch_destroy(qmcx_batch_t b) {
   if (!b) return 0;
   if (b->counted) b->sys->nbatches -= 1;
+  b->drop_graph();
   if (b->stream) hipStreamSynchronize(b->stream);
   if (b->aux) hipStreamSynchronize(b->aux);
   if (b->pre) hipStreamSynchronize(b->pre);
@@ -1407,6 +1458,7 @@ int qmcx_batch_get_positions(qmcx_batch_t b, double* R) {
 
 int qmcx_batch_set_walker_offset(qmcx_batch_t b, int walker0) {
   if (!b || walker0 < 0) return fail(QMCX_ERR_ARG, "bad argument");
+  b->drop_graph();   // a captured run holds the old setting
   b->walker0 = walker0;
   return 0;
 }
@@ -1414,12 +1466,14 @@ int qmcx_batch_set_walker_offset(qmcx_batch_t b, int walker0) {
 int qmcx_batch_set_flush(qmcx_batch_t b, int kind) {
   if (!b || kind < 0 || kind > 2)
     return fail(QMCX_ERR_ARG, "flush kind must be 0 (matrix cores), 1 (vector) or 2 (matrix cores, U rows from HBM)");
+  b->drop_graph();   // a captured run holds the old setting
   b->flush_kind = kind;
   return 0;
 }
 
 int qmcx_batch_set_tile(qmcx_batch_t b, int spline_slices) {
   if (!b || spline_slices < 0 || spline_slices > 16 || (spline_slices && 16 % spline_slices)) return fail(QMCX_ERR_ARG, "bad tile");
+  b->drop_graph();   // a captured run holds the old setting
   const int nq = b->sys->npad / (b->prec == QMCX_FP32 ? 4 : 2);
   if (nq * spline_slices > 1024) return fail(QMCX_ERR_ARG, "spline tile does not fit a workgroup");
   b->slices = spline_slices;
@@ -1442,6 +1496,7 @@ int qmcx_evaluate_log(qmcx_batch_t b, double* log_out) {
 
 int qmcx_set_trace(qmcx_batch_t b, int enable) {
   if (!b) return fail(QMCX_ERR_ARG, "null batch");
+  b->drop_graph();   // a captured run holds the old setting
   if (enable && !b->trace) HIPCHK(hipMalloc(&b->trace, sizeof(double) * 9 * (size_t)b->W * b->sys->N));
   b->trace_on = enable != 0;
   return 0;
@@ -1462,41 +1517,49 @@ int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int 
   if (b->rows64 && b->rows64_stale)
     return fail(QMCX_ERR_STATE, "the accept band was switched on after qmcx_evaluate_log: call it again");
   if (int rc = check_j3(b)) return rc;
+  if (b->graph_on && b->prof_on) return fail(QMCX_ERR_STATE, "kernel timing and graph replay exclude each other");
   if (nsweeps > b->hist_cap) {
     if (b->hist) hipFree(b->hist);
     b->hist = nullptr;
     HIPCHK(hipMalloc(&b->hist, sizeof(double) * (size_t)nsweeps * b->W));
     b->hist_cap = nsweeps;
+    b->drop_graph();   // the graph's kernels hold the old history buffer
   }
-  // mark the first sweep of this run so evaluateGL writes its history from slot 0; cur[0] is
-  // the absolute number of this run's first sweep
-  int cur[2];
+  // this run's counters: evaluateGL writes its history from slot 0 (sweep[1] = the first sweep),
+  // the first proposals belong to that sweep (sweep[2])
+  int cur[4];
   HIPCHK(hipStreamSynchronize(b->stream));
   HIPCHK(hipMemcpy(cur, b->sweep, sizeof(cur), hipMemcpyDeviceToHost));
   cur[1] = cur[0];
+  cur[2] = cur[0];
   HIPCHK(hipMemcpy(b->sweep, cur, sizeof(cur), hipMemcpyHostToDevice));
-  const bool f32 = b->prec == QMCX_FP32;
   HIPCHK(hipEventRecord(b->ev0, b->stream));
   if (nsweeps > 0) {
-    // the first sweep's proposals, after whatever the batch stream did before this run
-    HIPCHK(hipEventRecord(b->ev_start, b->stream));
-    for (int det = 0; det < 2; ++det) {
-      int rc = f32 ? issue_proposals<float>(b, det, b->ev_start, seed, cur[0], tau)
-                   : issue_proposals<double>(b, det, b->ev_start, seed, cur[0], tau);
-      if (rc) return rc;
+    const GraphKey key{nsweeps, seed, tau, accept_mode, recompute_every};
+    if (b->graph_on) {
+      // the whole run as one graph, captured on its first use and replayed while the run's
+      // parameters stay the same (the sweep numbers come from the device counters)
+      if (!b->gexec || !(b->gkey == key)) {
+        b->drop_graph();
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_run(b, nsweeps, seed, tau, accept_mode, recompute_every);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc) {
+          if (g) hipGraphDestroy(g);
+          return rc;
+        }
+        HIPCHK(e);
+        const hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        HIPCHK(ei);
+        b->gkey = key;
+      }
+      HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    } else if (int rc = enqueue_run(b, nsweeps, seed, tau, accept_mode, recompute_every)) {
+      return rc;
     }
   }
-  for (int sw = 0; sw < nsweeps; ++sw) {
-    const bool rec = recompute_every > 0 && (sw + 1) % recompute_every == 0;
-    const bool last = sw + 1 == nsweeps;
-    int rc = f32 ? sweep_impl<float>(b, seed, tau, accept_mode, rec, sw, cur[0] + sw, last)
-                 : sweep_impl<double>(b, seed, tau, accept_mode, rec, sw, cur[0] + sw, last);
-    if (rc) return rc;
-    if (b->prof_on && b->prof_used > 4096 && b->prof_collect()) return QMCX_ERR_HIP;
-    hipLaunchKernelGGL(k_next_sweep, dim3(1), dim3(64), 0, b->stream, b->sweep);
-    HIPCHK(hipGetLastError());
-  }
-  if (int rc = settle_refresh(b)) return rc;
   HIPCHK(hipEventRecord(b->ev1, b->stream));
   HIPCHK(hipEventSynchronize(b->ev1));
   HIPCHK(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
@@ -1505,6 +1568,13 @@ int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int 
     if (int rc = check_flags(b)) return rc;
   if (log_out && nsweeps > 0)
     HIPCHK(hipMemcpy(log_out, b->hist, sizeof(double) * (size_t)nsweeps * b->W, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int qmcx_batch_set_graph(qmcx_batch_t b, int enable) {
+  if (!b) return fail(QMCX_ERR_ARG, "null batch");
+  b->graph_on = enable != 0;
+  if (!b->graph_on) b->drop_graph();
   return 0;
 }
 
@@ -1529,6 +1599,7 @@ int qmcx_get_accepts(qmcx_batch_t b, long long* naccept) {
 
 int qmcx_batch_set_accept_band(qmcx_batch_t b, double band) {
   if (!b || !(band >= 0.0) || band > 1.0) return fail(QMCX_ERR_ARG, "band must be in [0, 1]");
+  b->drop_graph();   // a captured run holds the old setting
   if (band > 0.0 && b->j3)
     return fail(QMCX_ERR_ARG, "the near-tie FP64 re-evaluation does not take a three-body Jastrow");
   b->band = band;
@@ -1569,8 +1640,8 @@ static int flex_launch(qmcx_batch_t b, int iel, bool apply) {
   int rc = choose_shape(s, nt);
   if (rc) return rc;
   const int det = iel < s->n ? 0 : 1;
-  rc = f32 ? launch_proposals<float>(b, b->stream, det, iel - det * s->n, 1, 0, 0, 1.0, b->flex_displ)
-           : launch_proposals<double>(b, b->stream, det, iel - det * s->n, 1, 0, 0, 1.0, b->flex_displ);
+  rc = f32 ? launch_proposals<float>(b, b->stream, det, iel - det * s->n, 1, 0, 1.0, b->flex_displ)
+           : launch_proposals<double>(b, b->stream, det, iel - det * s->n, 1, 0, 1.0, b->flex_displ);
   if (rc) return rc;
   MoveArgs a;
   std::memset(&a, 0, sizeof(a));

@@ -60,7 +60,10 @@ struct BatchP {
   double* jlog;
   unsigned long long* nacc;
   double* trace;   // [W][N][9] or null
-  int* sweep;      // [0] global sweep index, [1] index of the first sweep of the current run
+  // device sweep counters: [0] the sweep the moves are in (advanced on the batch stream after
+  // evaluateGL), [1] the first sweep of the current run, [2] the sweep of the proposals being
+  // evaluated (advanced on the proposals stream), [3] the history slot of the latest evaluateGL
+  int* sweep;
   // FP32 form only (null in FP64): the Slater rows summed in FP64 from the same coefficients,
   // A64 [W][2][n][n] and at the sweep's proposals spo64 [W][2][n][npad]
   double* A64;
@@ -970,11 +973,12 @@ __global__ __launch_bounds__(1024) void k_spo_rows(SysP<T> S, BatchP<T> B, int s
 // caller's (displ[w][3]).
 template <typename T>
 __global__ __launch_bounds__(1024) void k_spo_proposals(SysP<T> S, BatchP<T> B, int det, int e0, int ne, uint64_t seed,
-                                                         int sweep, double sqrt_tau, int walker0, const double* displ,
-                                                         T* spo, double* prop, int slices) {
+                                                         const int* sweep_ctr, double sqrt_tau, int walker0,
+                                                         const double* displ, T* spo, double* prop, int slices) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* part = (T*)smem;
   const int w = blockIdx.y, N = S.N, n = S.n;
+  const int sweep = displ != nullptr ? 0 : sweep_ctr[2];   // the random streams' sweep counter
   // a workgroup takes electrons e0 + blockIdx.x, + gridDim.x, ... (fewer workgroups than
   // electrons bound the gather's share of HBM beside the move kernel)
   for (int el = e0 + blockIdx.x; el < e0 + ne; el += gridDim.x) {
@@ -1584,6 +1588,7 @@ __global__ __launch_bounds__(256) void k_finish_gl(SysP<T> S, BatchP<T> B, doubl
     if (hist != nullptr) {
       const int slot = B.sweep[0] - B.sweep[1];
       if (slot >= 0 && slot < hist_cap) hist[(size_t)slot * B.W + w] = v;
+      if (w == 0) B.sweep[3] = slot;
     }
   }
 }
@@ -1599,22 +1604,25 @@ __global__ void k_hold(unsigned long long ticks) {
   }
 }
 
-__global__ void k_next_sweep(int* sweep) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) sweep[0] += 1;
+__global__ void k_count(int* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += 1;
 }
 
 // The second determinant's FP64 refresh lands after that sweep's evaluateGL (it runs beside
 // the next sweep's first-determinant moves): swap in the fresh log|det| and phase and move the
 // sweep's recorded log|Psi| by the same difference.
+// The sweep's history slot is the one its evaluateGL wrote (sweep[3]): this kernel runs after
+// that evaluateGL and before the next one.
 __global__ void k_refresh_det(int W, int det, const double* ld_fresh, const double* ph_fresh, double* logdet,
-                              double* phase, double* logv, double* hist, int slot) {
+                              double* phase, double* logv, double* hist, int hist_cap, const int* sweep) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= W) return;
   const double delta = ld_fresh[w] - logdet[w * 2 + det];
   logdet[w * 2 + det] = ld_fresh[w];
   phase[w * 2 + det] = ph_fresh[w];
   logv[w] += delta;
-  if (hist != nullptr) hist[(size_t)slot * W + w] += delta;
+  const int slot = sweep[3];
+  if (hist != nullptr && slot >= 0 && slot < hist_cap) hist[(size_t)slot * W + w] += delta;
 }
 
 }  // namespace qmcx

@@ -52,6 +52,7 @@ _SIGS = {
     "qmcx_batch_set_walker_offset": [_vp, _i],
     "qmcx_batch_set_tile": [_vp, _i],
     "qmcx_batch_set_flush": [_vp, _i],
+    "qmcx_batch_set_graph": [_vp, _i],
     "qmcx_evaluate_log": [_vp, _vp],
     "qmcx_run_sweeps": [_vp, _i, C.c_uint64, C.c_double, _i, _i, _vp],
     "qmcx_set_trace": [_vp, _i],
@@ -298,6 +299,10 @@ class WalkerBatch:
         a = np.zeros(self.nw, dtype=np.int64)
         _check(load().qmcx_get_accepts(self.h, a))
         return a
+
+    def set_graph(self, enable=True):
+        """Capture run_sweeps into a HIP graph on first use and replay it (qmcx_batch_set_graph)."""
+        _check(load().qmcx_batch_set_graph(self.h, int(bool(enable))))
 
     def set_accept_band(self, band):
         """FP32 Metropolis: decide moves with |u - R^2| <= band R^2 on the FP64 ratio (0: off)."""
