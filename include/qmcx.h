@@ -20,6 +20,7 @@ extern "C" {
 
 typedef struct qmcx_system_s* qmcx_system_t;
 typedef struct qmcx_batch_s* qmcx_batch_t;
+typedef struct qmcx_group_s* qmcx_group_t;
 
 enum qmcx_status {
   QMCX_OK = 0,
@@ -240,6 +241,24 @@ int qmcx_du_delay_count(qmcx_du_t h, int* count);
 
 /* Device time of the last qmcx_run_sweeps (HIP events on the batch stream), milliseconds. */
 int qmcx_last_sweep_ms(qmcx_batch_t b, double* ms);
+
+/* ---- mixed-size groups (no reference counterpart: the reference advances one system's walkers per
+ * process; miniqmc_sync_move.cpp:459-513 is the batched step this generalizes) ----------------------
+ * Walkers of up to 4 batches, each of its own system (its own spline table, Jastrows and size) and delay
+ * rank, advanced together: every move stage and every proposals stage is one launch over all the
+ * batches' walkers (the workgroup size and LDS are the largest the batches need); the FP64 refresh runs
+ * one launch per batch (matrix size), evaluateGL per batch.  Batches: FP32, delay rank >= 2 with the
+ * delayed updates applied inside the move kernel (n a multiple of 16, rank <= 32), no three-body Jastrow,
+ * no near-tie band, one device.  The batches stay usable on their own; each keeps its own state,
+ * random streams (its walker offset) and log history.
+ * qmcx_group_run_sweeps: as qmcx_run_sweeps for every batch at once; log_out [nsweeps][sum of nw] (the
+ * batches' walkers in group order, may be NULL).  qmcx_group_set_graph: as qmcx_batch_set_graph. */
+int qmcx_group_create(qmcx_group_t* out, const qmcx_batch_t* batches, int nbatches);
+int qmcx_group_destroy(qmcx_group_t g);
+int qmcx_group_set_graph(qmcx_group_t g, int enable);
+int qmcx_group_run_sweeps(qmcx_group_t g, int nsweeps, uint64_t seed, double tau, int accept_mode,
+                          int recompute_every, double* log_out);
+int qmcx_group_last_ms(qmcx_group_t g, double* ms);
 
 #ifdef __cplusplus
 }

@@ -336,9 +336,11 @@ struct qmcx_batch_s {
   bool graph_on = false;
   hipGraphExec_t gexec = nullptr;
   GraphKey gkey{};
+  unsigned epoch = 0;   // counts setting changes (a group's graph compares it)
   void drop_graph() {
     if (gexec) hipGraphExecDestroy(gexec);
     gexec = nullptr;
+    ++epoch;
   }
   int* flags = nullptr;   // [1] kernel error bits
   double* A64 = nullptr;   // FP32 form: [W][2][n][n] FP64 Slater rows (the FP64 refresh inverts these)
@@ -845,6 +847,168 @@ static int enqueue_run(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, i
   }
   return settle_refresh(b);
 }
+
+// ====================================================================== mixed-size groups
+// Walkers of several batches (systems of different sizes, FP32, delayed updates) advanced together:
+// one grouped launch per move stage and per proposals stage covers every batch's walkers; the FP64
+// refresh runs per batch (one launch per matrix size); evaluateGL and the sweep counters per batch.
+struct qmcx_group_s {
+  std::vector<qmcx_batch_t> b;
+  int Wtot = 0, jpt = 1, nt = 256, pnt = 64, pgx = 1;
+  size_t lds = 0, plds = 0;
+  hipStream_t stream = nullptr, aux = nullptr, pre = nullptr;
+  hipEvent_t ev_pre[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr}, ev_fresh[2] = {nullptr, nullptr};
+  hipEvent_t ev_gl = nullptr, ev_start = nullptr, ev0 = nullptr, ev1 = nullptr;
+  bool pending_dn = false;
+  float last_ms = 0.f;
+  bool graph_on = false;
+  hipGraphExec_t gexec = nullptr;
+  GraphKey gkey{};
+  std::vector<unsigned> epochs;   // the batches' setting epochs at capture
+  void drop_graph() {
+    if (gexec) hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+  }
+  bool graph_current(const GraphKey& key) const {
+    if (!gexec || !(gkey == key) || epochs.size() != b.size()) return false;
+    for (size_t i = 0; i < b.size(); ++i)
+      if (epochs[i] != b[i]->epoch) return false;
+    return true;
+  }
+};
+
+namespace {
+
+// the moves of determinant det of every batch in one launch; each batch's chunks are flushed by
+// the kernel, the last one only when `flush`
+int group_moves(qmcx_group_s* g, int det, int mode, bool flush) {
+  MoveGroup<float> G;
+  std::memset(&G, 0, sizeof(G));
+  G.ng = (int)g->b.size();
+  int first = 0;
+  for (int i = 0; i < G.ng; ++i) {
+    qmcx_batch_t b = g->b[i];
+    const qmcx_system_s* s = b->sys;
+    G.first[i] = first;
+    first += b->W;
+    G.S[i] = s->params<float>();
+    G.B[i] = b->params<float>();
+    MoveArgs& a = G.a[i];
+    a.mode = mode;
+    a.spo = b->spo;
+    a.prop = b->prop;
+    a.det = det;
+    a.chunk_len = b->K;
+    a.e_begin = det * s->n;
+    a.e_end = det * s->n + s->n;
+    a.chunk = 0;
+    a.nchunks = (s->n + b->K - 1) / b->K;
+    a.inflush = flush ? 2 : 1;
+    a.flush_u = b->flush_kind == 2 ? 0 : kFlushOwnU;
+  }
+  G.first[G.ng] = first;
+  if (g->jpt == 1)
+    hipLaunchKernelGGL((k_moves_group<float, 1>), dim3(g->Wtot), dim3(g->nt), g->lds, g->stream, G);
+  else
+    hipLaunchKernelGGL((k_moves_group<float, 2>), dim3(g->Wtot), dim3(g->nt), g->lds, g->stream, G);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// the proposals of determinant det of every batch, on the proposals stream after `after`
+int group_proposals(qmcx_group_s* g, int det, hipEvent_t after, uint64_t seed, double tau, bool hold) {
+  if (after) HIPCHK(hipStreamWaitEvent(g->pre, after, 0));
+  if (hold && hold_stream(g->pre, det == 0 ? QMCX_HOLD_PROP0_US : QMCX_HOLD_PROP1_US)) return QMCX_ERR_HIP;
+  PropGroup<float> P;
+  std::memset(&P, 0, sizeof(P));
+  P.ng = (int)g->b.size();
+  int first = 0;
+  for (int i = 0; i < P.ng; ++i) {
+    qmcx_batch_t b = g->b[i];
+    P.first[i] = first;
+    first += b->W;
+    P.S[i] = b->sys->params<float>();
+    P.B[i] = b->params<float>();
+    P.spo[i] = (float*)b->spo;
+    P.prop[i] = b->prop;
+    P.walker0[i] = b->walker0;
+    P.slices[i] = proposal_slices<float>(b->sys, b->slices);
+  }
+  P.first[P.ng] = first;
+  hipLaunchKernelGGL(k_spo_proposals_group<float>, dim3(g->pgx, g->Wtot), dim3(g->pnt), g->plds, g->pre, P, det, seed,
+                     std::sqrt(tau));
+  HIPCHK(hipGetLastError());
+  if (det == 1)
+    for (qmcx_batch_t b : g->b) {
+      hipLaunchKernelGGL(k_count, dim3(1), dim3(64), 0, g->pre, b->sweep + 2);
+      HIPCHK(hipGetLastError());
+    }
+  HIPCHK(hipEventRecord(g->ev_pre[det], g->pre));
+  return 0;
+}
+
+// one sweep of the group, the DAG of sweep_impl with grouped move and proposals launches
+int group_sweep(qmcx_group_s* g, uint64_t seed, double tau, int mode, bool recompute, bool last) {
+  HIPCHK(hipStreamWaitEvent(g->stream, g->ev_pre[0], 0));
+  if (int rc = group_moves(g, 0, mode, !recompute)) return rc;
+  HIPCHK(hipEventRecord(g->ev_done[0], g->stream));
+  if (!last)
+    if (int rc = group_proposals(g, 0, g->ev_done[0], seed, tau, true)) return rc;
+  if (recompute) {
+    HIPCHK(hipStreamWaitEvent(g->aux, g->ev_done[0], 0));
+    if (hold_stream(g->aux, QMCX_HOLD_REFRESH_US)) return QMCX_ERR_HIP;
+    for (qmcx_batch_t b : g->b)
+      if (int rc = refresh_det<float>(b, 0, false)) return rc;
+    HIPCHK(hipEventRecord(g->ev_fresh[0], g->aux));
+  }
+  if (g->pending_dn) {
+    HIPCHK(hipStreamWaitEvent(g->stream, g->ev_fresh[1], 0));
+    g->pending_dn = false;
+  }
+  HIPCHK(hipStreamWaitEvent(g->stream, g->ev_pre[1], 0));
+  if (int rc = group_moves(g, 1, mode, true)) return rc;
+  HIPCHK(hipEventRecord(g->ev_done[1], g->stream));
+  if (!last)
+    if (int rc = group_proposals(g, 1, g->ev_done[1], seed, tau, true)) return rc;
+  if (recompute) HIPCHK(hipStreamWaitEvent(g->stream, g->ev_fresh[0], 0));
+  for (qmcx_batch_t b : g->b)
+    if (int rc = launch_gl<float>(b, true)) return rc;
+  if (recompute) {
+    HIPCHK(hipEventRecord(g->ev_gl, g->stream));
+    HIPCHK(hipStreamWaitEvent(g->aux, g->ev_gl, 0));
+    if (hold_stream(g->aux, QMCX_HOLD_REFRESH_US)) return QMCX_ERR_HIP;
+    for (qmcx_batch_t b : g->b) {
+      if (int rc = refresh_det<float>(b, 1, true)) return rc;
+      hipLaunchKernelGGL(k_refresh_det, dim3((b->W + 63) / 64), dim3(64), 0, g->aux, b->W, 1, b->ld_fresh, b->ph_fresh,
+                         b->logdet, b->phase, b->logv, b->hist, b->hist_cap, b->sweep);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(g->ev_fresh[1], g->aux));
+    g->pending_dn = true;
+  }
+  for (qmcx_batch_t b : g->b) {
+    hipLaunchKernelGGL(k_count, dim3(1), dim3(64), 0, g->stream, b->sweep);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+int enqueue_group_run(qmcx_group_s* g, int nsweeps, uint64_t seed, double tau, int mode, int recompute_every) {
+  HIPCHK(hipEventRecord(g->ev_start, g->stream));
+  for (int det = 0; det < 2; ++det)
+    if (int rc = group_proposals(g, det, g->ev_start, seed, tau, false)) return rc;
+  for (int sw = 0; sw < nsweeps; ++sw) {
+    const bool rec = recompute_every > 0 && (sw + 1) % recompute_every == 0;
+    if (int rc = group_sweep(g, seed, tau, mode, rec, sw + 1 == nsweeps)) return rc;
+  }
+  if (g->pending_dn) {
+    HIPCHK(hipStreamWaitEvent(g->stream, g->ev_fresh[1], 0));
+    g->pending_dn = false;
+  }
+  return 0;
+}
+
+}  // namespace
 
 // ====================================================================== C ABI
 // A batch keeps three-body Jastrow state only when its system had one at its creation.
@@ -2069,6 +2233,172 @@ int qmcx_du_update_inv_mat(qmcx_du_t h) {
 int qmcx_du_delay_count(qmcx_du_t h, int* count) {
   if (!h || !count) return fail(QMCX_ERR_ARG, "null argument");
   *count = h->count;
+  return 0;
+}
+
+}  // extern "C"
+
+// ====================================================================== mixed-size groups (C ABI)
+extern "C" {
+
+int qmcx_group_create(qmcx_group_t* out, const qmcx_batch_t* batches, int nbatches) {
+  if (!out || !batches || nbatches < 1 || nbatches > kMaxGroups)
+    return fail(QMCX_ERR_ARG, "a group takes 1 to " + std::to_string(kMaxGroups) + " batches");
+  auto* g = new qmcx_group_s;
+  int nt = 256, need_jpt2 = 0;
+  size_t lds = 0, plds = 0;
+  int pnt = 64, nmax = 1;
+  for (int i = 0; i < nbatches; ++i) {
+    qmcx_batch_t b = batches[i];
+    const qmcx_system_s* s = b ? b->sys : nullptr;
+    std::string why;
+    if (!b) why = "null batch";
+    else if (b->prec != QMCX_FP32) why = "FP32 batches only";
+    else if (b->K < 2) why = "delay rank 2 or more";
+    else if (b->j3) why = "no three-body Jastrow";
+    else if (b->rows64) why = "no near-tie band";
+    else if (!use_mfma_flush<float>(b) || !flush_own_fits<float>(s->N, s->n, b->K, b->K))
+      why = "each batch's delayed updates must be applied inside the move kernel (n a multiple of 16, delay <= 32)";
+    else if (b->sys->dev != batches[0]->sys->dev) why = "batches of one device";
+    for (int j = 0; j < i && why.empty(); ++j)
+      if (batches[j] == b) why = "a batch appears twice";
+    if (!why.empty()) {
+      delete g;
+      return fail(QMCX_ERR_ARG, "group batch " + std::to_string(i) + ": " + why);
+    }
+    int bnt = 0;
+    if (int rc = choose_shape(s, bnt)) {
+      delete g;
+      return rc;
+    }
+    nt = std::max(nt, bnt);
+    g->b.push_back(b);
+    g->Wtot += b->W;
+    nmax = std::max(nmax, s->n);
+  }
+  for (qmcx_batch_t b : g->b) {
+    const qmcx_system_s* s = b->sys;
+    if (s->N > nt) need_jpt2 = 1;
+    lds = std::max(lds, (size_t)move_lds<float>(s->N, s->nions, s->n, b->K, b->K).total);
+    const int nq = s->npad / 4, slices = proposal_slices<float>(s, b->slices);
+    pnt = std::max(pnt, (nq * slices + 63) / 64 * 64);
+    plds = std::max(plds, std::max(sizeof(float) * (size_t)slices * 5 * s->npad, sizeof(double) * 16 * 5));
+  }
+  g->nt = nt;
+  g->jpt = need_jpt2 ? 2 : 1;
+  g->lds = lds;
+  g->pnt = pnt;
+  g->plds = plds;
+  g->pgx = std::max(1, std::min(nmax, kPropWorkgroups / g->Wtot));
+  if (lds > kMoveLdsMax || pnt > 1024) {
+    delete g;
+    return fail(QMCX_ERR_ARG, "group workgroup does not fit");
+  }
+  g->stream = g->b[0]->stream;
+  g->aux = g->b[0]->aux;
+  g->pre = g->b[0]->pre;
+  for (int d = 0; d < 2; ++d) {
+    HIPCHK(hipEventCreateWithFlags(&g->ev_pre[d], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&g->ev_done[d], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&g->ev_fresh[d], hipEventDisableTiming));
+  }
+  HIPCHK(hipEventCreateWithFlags(&g->ev_gl, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&g->ev_start, hipEventDisableTiming));
+  HIPCHK(hipEventCreate(&g->ev0));
+  HIPCHK(hipEventCreate(&g->ev1));
+  *out = g;
+  return 0;
+}
+
+int qmcx_group_destroy(qmcx_group_t g) {
+  if (!g) return 0;
+  if (g->stream) hipStreamSynchronize(g->stream);
+  g->drop_graph();
+  for (hipEvent_t e : {g->ev_pre[0], g->ev_pre[1], g->ev_done[0], g->ev_done[1], g->ev_fresh[0], g->ev_fresh[1],
+                       g->ev_gl, g->ev_start, g->ev0, g->ev1})
+    if (e) hipEventDestroy(e);
+  delete g;
+  return 0;
+}
+
+int qmcx_group_set_graph(qmcx_group_t g, int enable) {
+  if (!g) return fail(QMCX_ERR_ARG, "null group");
+  g->graph_on = enable != 0;
+  if (!g->graph_on) g->drop_graph();
+  return 0;
+}
+
+int qmcx_group_run_sweeps(qmcx_group_t g, int nsweeps, uint64_t seed, double tau, int accept_mode, int recompute_every,
+                          double* log_out) {
+  if (!g || nsweeps < 0 || !(tau > 0)) return fail(QMCX_ERR_ARG, "bad argument");
+  if (accept_mode != QMCX_ACCEPT_REFERENCE && accept_mode != QMCX_ACCEPT_METROPOLIS)
+    return fail(QMCX_ERR_ARG, "unknown accept mode");
+  HIPCHK(hipStreamSynchronize(g->stream));
+  for (qmcx_batch_t b : g->b) {
+    if (b->rows64 || b->prof_on) return fail(QMCX_ERR_STATE, "a group batch took the near-tie band or kernel timing");
+    if (int rc = check_j3(b)) return rc;
+    if (nsweeps > b->hist_cap) {
+      if (b->hist) hipFree(b->hist);
+      b->hist = nullptr;
+      HIPCHK(hipMalloc(&b->hist, sizeof(double) * (size_t)nsweeps * b->W));
+      b->hist_cap = nsweeps;
+      b->drop_graph();
+      g->drop_graph();   // the group's graph holds the old history buffer
+    }
+    int cur[4];
+    HIPCHK(hipMemcpy(cur, b->sweep, sizeof(cur), hipMemcpyDeviceToHost));
+    cur[1] = cur[0];
+    cur[2] = cur[0];
+    HIPCHK(hipMemcpy(b->sweep, cur, sizeof(cur), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipEventRecord(g->ev0, g->stream));
+  if (nsweeps > 0) {
+    const GraphKey key{nsweeps, seed, tau, accept_mode, recompute_every};
+    if (g->graph_on) {
+      if (!g->graph_current(key)) {
+        g->drop_graph();
+        HIPCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_group_run(g, nsweeps, seed, tau, accept_mode, recompute_every);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(g->stream, &gr);
+        if (rc) {
+          if (gr) hipGraphDestroy(gr);
+          return rc;
+        }
+        HIPCHK(e);
+        const hipError_t ei = hipGraphInstantiate(&g->gexec, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        HIPCHK(ei);
+        g->gkey = key;
+        g->epochs.clear();
+        for (qmcx_batch_t b : g->b) g->epochs.push_back(b->epoch);
+      }
+      HIPCHK(hipGraphLaunch(g->gexec, g->stream));
+    } else if (int rc = enqueue_group_run(g, nsweeps, seed, tau, accept_mode, recompute_every)) {
+      return rc;
+    }
+  }
+  HIPCHK(hipEventRecord(g->ev1, g->stream));
+  HIPCHK(hipEventSynchronize(g->ev1));
+  HIPCHK(hipEventElapsedTime(&g->last_ms, g->ev0, g->ev1));
+  if (log_out && nsweeps > 0) {
+    // [nsweeps][all walkers], the batches' walkers in group order
+    std::vector<double> h;
+    size_t off = 0;
+    for (qmcx_batch_t b : g->b) {
+      h.resize((size_t)nsweeps * b->W);
+      HIPCHK(hipMemcpy(h.data(), b->hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+      for (int sw = 0; sw < nsweeps; ++sw)
+        std::memcpy(log_out + (size_t)sw * g->Wtot + off, h.data() + (size_t)sw * b->W, sizeof(double) * b->W);
+      off += b->W;
+    }
+  }
+  return 0;
+}
+
+int qmcx_group_last_ms(qmcx_group_t g, double* ms) {
+  if (!g || !ms) return fail(QMCX_ERR_ARG, "null argument");
+  *ms = g->last_ms;
   return 0;
 }
 

@@ -972,12 +972,12 @@ __global__ __launch_bounds__(1024) void k_spo_rows(SysP<T> S, BatchP<T> B, int s
 // move kernel's: sqrt(tau) times the walker's Philox normals of (sweep, electron), or the
 // caller's (displ[w][3]).
 template <typename T>
-__global__ __launch_bounds__(1024) void k_spo_proposals(SysP<T> S, BatchP<T> B, int det, int e0, int ne, uint64_t seed,
-                                                         const int* sweep_ctr, double sqrt_tau, int walker0,
-                                                         const double* displ, T* spo, double* prop, int slices) {
+__device__ __forceinline__ void spo_proposals_body(const SysP<T>& S, const BatchP<T>& B, int det, int e0, int ne,
+                                                   uint64_t seed, const int* sweep_ctr, double sqrt_tau, int walker0,
+                                                   const double* displ, T* spo, double* prop, int slices, const int w) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* part = (T*)smem;
-  const int w = blockIdx.y, N = S.N, n = S.n;
+  const int N = S.N, n = S.n;
   const int sweep = displ != nullptr ? 0 : sweep_ctr[2];   // the random streams' sweep counter
   // a workgroup takes electrons e0 + blockIdx.x, + gridDim.x, ... (fewer workgroups than
   // electrons bound the gather's share of HBM beside the move kernel)
@@ -1079,6 +1079,35 @@ __global__ __launch_bounds__(1024) void k_spo_proposals(SysP<T> S, BatchP<T> B, 
   }
   __syncthreads();   // the next electron's gather reuses the LDS
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_spo_proposals(SysP<T> S, BatchP<T> B, int det, int e0, int ne, uint64_t seed,
+                                                         const int* sweep_ctr, double sqrt_tau, int walker0,
+                                                         const double* displ, T* spo, double* prop, int slices) {
+  spo_proposals_body<T>(S, B, det, e0, ne, seed, sweep_ctr, sqrt_tau, walker0, displ, spo, prop, slices, blockIdx.y);
+}
+
+// The proposals of one determinant of every walker of several batches in one launch: batch g owns
+// the grid rows [first[g], first[g+1]); the workgroup size and LDS are the largest any batch needs.
+constexpr int kMaxPropGroups = 4;
+template <typename T>
+struct PropGroup {
+  int ng;
+  int first[kMaxPropGroups + 1];
+  SysP<T> S[kMaxPropGroups];
+  BatchP<T> B[kMaxPropGroups];
+  T* spo[kMaxPropGroups];
+  double* prop[kMaxPropGroups];
+  int walker0[kMaxPropGroups], slices[kMaxPropGroups];
+};
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_spo_proposals_group(PropGroup<T> G, int det, uint64_t seed, double sqrt_tau) {
+  int g = 0;
+  while (g + 1 < G.ng && (int)blockIdx.y >= G.first[g + 1]) ++g;
+  spo_proposals_body<T>(G.S[g], G.B[g], det, 0, G.S[g].n, seed, G.B[g].sweep, sqrt_tau, G.walker0[g], nullptr,
+                        G.spo[g], G.prop[g], G.slices[g], (int)blockIdx.y - G.first[g]);
 }
 
 // stand-alone VGH at arbitrary points: out[p][10][norb] (FP64)

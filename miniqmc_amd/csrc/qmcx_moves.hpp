@@ -604,10 +604,11 @@ __device__ inline void j3_apply(T* sJ3, int N, const int* sk, const T* sv, int n
 // ARB: the FP64 re-evaluation of near-tie Metropolis decisions is compiled in (FP32, K > 1); only
 // that instantiation carries its code, whose registers would otherwise spill in every move.
 // J3: the three-body Jastrow is compiled in (its ratio, gradient and update in every move).
+// The workgroup moves walker w of batch (S, B) as directed by a.
 template <typename T, int JPT, bool KD, bool ARB, bool J3>
-__global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs a) {
+__device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B, const MoveArgs& a, const int w) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int w = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
   const int N = S.N, n = S.n, K = KD ? B.K : 1;
   const int KP = K + 1;   // LDS row stride of Binv: a column walk down its rows is free of bank conflicts
@@ -1277,6 +1278,32 @@ __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, 
     if (B.narb != nullptr) B.narb[w] += (unsigned long long)sAcc[aNarb];
   }
   QMCX_TICK_FLUSH()
+}
+
+template <typename T, int JPT, bool KD, bool ARB, bool J3>
+__global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs a) {
+  moves_body<T, JPT, KD, ARB, J3>(S, B, a, blockIdx.x);
+}
+
+// ---- walkers of several batches (systems of different sizes) in one launch: batch g owns the
+// workgroups [first[g], first[g+1]); the workgroup size and LDS are the largest any batch needs
+// (a batch of a smaller system leaves the extra lanes idle).  Delayed updates with the flush
+// inside the kernel (FP32), each batch at its own delay rank.
+constexpr int kMaxGroups = 4;
+template <typename T>
+struct MoveGroup {
+  int ng;
+  int first[kMaxGroups + 1];
+  SysP<T> S[kMaxGroups];
+  BatchP<T> B[kMaxGroups];
+  MoveArgs a[kMaxGroups];
+};
+
+template <typename T, int JPT>
+__global__ __launch_bounds__(kMoveThreads) void k_moves_group(MoveGroup<T> G) {
+  int g = 0;
+  while (g + 1 < G.ng && (int)blockIdx.x >= G.first[g + 1]) ++g;
+  moves_body<T, JPT, true, false, false>(G.S[g], G.B[g], G.a[g], (int)blockIdx.x - G.first[g]);
 }
 
 }  // namespace qmcx

@@ -81,6 +81,11 @@ _SIGS = {
     "qmcx_du_update_inv_mat": [_vp],
     "qmcx_du_delay_count": [_vp, C.POINTER(C.c_int)],
     "qmcx_get_kernel_timing": [_vp, _dp, _lp],
+    "qmcx_group_create": [C.POINTER(_vp), C.POINTER(_vp), _i],
+    "qmcx_group_destroy": [_vp],
+    "qmcx_group_set_graph": [_vp, _i],
+    "qmcx_group_run_sweeps": [_vp, _i, C.c_uint64, C.c_double, _i, _i, _vp],
+    "qmcx_group_last_ms": [_vp, C.POINTER(C.c_double)],
 }
 
 
@@ -379,4 +384,43 @@ class WalkerBatch:
     def last_sweep_ms(self):
         ms = C.c_double(0)
         _check(load().qmcx_last_sweep_ms(self.h, C.byref(ms)))
+        return ms.value
+
+
+class Group:
+    """Walkers of several batches (systems of different sizes) advanced together: one launch per move
+    stage and per proposals stage over all of them (qmcx_group_*)."""
+
+    def __init__(self, batches):
+        self.batches = list(batches)
+        arr = (_vp * len(self.batches))(*[b.h for b in self.batches])
+        h = _vp()
+        _check(load().qmcx_group_create(C.byref(h), arr, len(self.batches)))
+        self.h = h
+        self.nw = sum(b.nw for b in self.batches)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().qmcx_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_graph(self, enable=True):
+        _check(load().qmcx_group_set_graph(self.h, int(bool(enable))))
+
+    def run(self, nsweeps, seed, tau, mode=ACCEPT_METROPOLIS, recompute_every=0):
+        """Returns the log|Psi| history per batch: a list of [nsweeps][nw_b] arrays."""
+        hist = np.zeros((max(nsweeps, 1), self.nw))
+        _check(load().qmcx_group_run_sweeps(self.h, nsweeps, seed, tau, mode, recompute_every, hist.ctypes.data))
+        out, o = [], 0
+        for b in self.batches:
+            out.append(hist[:nsweeps, o:o + b.nw].copy())
+            o += b.nw
+        return out
+
+    def last_ms(self):
+        ms = C.c_double(0)
+        _check(load().qmcx_group_last_ms(self.h, C.byref(ms)))
         return ms.value
