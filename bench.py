@@ -203,6 +203,16 @@ def measure_leg(dev, sysd, R0, k, tile, mode, steps, warmup, fp64_steps, err_ste
               for st, (ms, n) in kt.items()}
     log(f"FP32 k={k} ({'u < 0.5' if mode == qmcx.ACCEPT_REFERENCE else 'Metropolis'}): {value:.4e} moves/s, {1e3 * wall / steps:.2f} ms/sweep; stages " +
         ", ".join(f"{st} {v['ms_per_step']:.2f} ms/sweep" for st, v in stages.items()))
+    # the same sweeps replayed from a captured HIP graph (one capture, then a timed replay)
+    b32.set_graph(True)
+    b32.run(steps, seed, tau, mode, recompute_every=recompute)
+    t0 = time.perf_counter()
+    b32.run(steps, seed, tau, mode, recompute_every=recompute)
+    wall_g = time.perf_counter() - t0
+    b32.set_graph(False)
+    graph = {"value": moves / wall_g, "ms_per_step": 1e3 * wall_g / steps, "steps": steps,
+             "note": "the run captured once into a HIP graph (qmcx_batch_set_graph) and replayed"}
+    log(f"  graph replay: {graph['value']:.4e} moves/s, {graph['ms_per_step']:.2f} ms/sweep")
 
     # ---- FP64 delay-1 GPU path: batched (same walkers, continuing its own chain) and one walker alone
     b64.run(1, seed, tau, mode)
@@ -225,7 +235,7 @@ def measure_leg(dev, sysd, R0, k, tile, mode, steps, warmup, fp64_steps, err_ste
     narb = int(b32.arbitrations().sum())
     b32.close()
     b64.close()
-    return dict(band=band, arbitrations=narb, flips=flips, clean=clean, lerr=lerr, ratio_err=ratio_err, errors=errors, wall=wall, kt=kt, moves=moves,
+    return dict(graph=graph, band=band, arbitrations=narb, flips=flips, clean=clean, lerr=lerr, ratio_err=ratio_err, errors=errors, wall=wall, kt=kt, moves=moves,
                 value=value,
                 acc_timed=acc_timed, stages=stages, wall64=wall64, value64=value64, value64_1=value64_1)
 
@@ -457,6 +467,7 @@ def main():
                         "form": "batched, 1 workgroup per walker, in-kernel Sherman-Morrison",
                         "one_walker_value": value64_1},
         "speedup_vs_fp64_delay1": value / value64,
+        "graph_replay": L["graph"],
         "other_accept_rule": other,
         "log_psi_err": {"max_rel": float(np.nanmax(lerr)), "mean_rel": float(np.nanmean(lerr)),
                         "walkers_compared": int(clean.sum()), "sweeps": args.err_steps,
