@@ -6,14 +6,15 @@ few sweeps with HIP events (FP64 refresh every sweep, as the FP32 form runs);
 the fastest pair wins.  The delay rank trades the per-move cost of applying
 pending rows (grows with k) against the per-flush cost of rewriting the
 inverse (shrinks as 1/k); S sets how many lanes of the proposals kernel share
-one point's 4x4x4 stencil (0: the library's choice).
+one point's 4x4x4 stencil (S = 1, the library's default, keeps the partial sums in
+registers; S > 1 reduces them through LDS).
 """
 from __future__ import annotations
 
 from . import qmcx
 
 DELAYS = (8, 16, 32, 64)
-TILES = (0, 1, 4)
+TILES = (1, 2, 4)
 
 
 def tune(dev: qmcx.System, nwalkers: int, R0, seed: int = 11, tau: float = 0.3, sweeps: int = 2,

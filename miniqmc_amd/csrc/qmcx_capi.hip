@@ -459,14 +459,14 @@ int choose_shape(const qmcx_system_s* s, int& nt) {
   return 0;
 }
 
-// spline slices (16/slices stencil columns per lane) of the proposals kernel.  Two by default:
-// small blocks (a few waves, slices x 5 x npad of LDS) that fit beside the move kernel's and
-// the FP64 refresh's workgroups on their CUs, so the proposals use bandwidth those leave idle.
+// spline slices (16/slices stencil columns per lane) of the proposals kernel.  One by default:
+// each lane sums all 64 stencil points of its orbital quad and stores its outputs from
+// registers, so the workgroups need no LDS for partials and fit beside the move kernel's and
+// the FP64 refresh's workgroups on their CUs, using bandwidth those leave idle.
 template <typename T>
 int proposal_slices(const qmcx_system_s* s, int want) {
-  const int nq = s->npad / (16 / (int)sizeof(T));
-  if (want > 0) return want;
-  return nq * 2 <= 1024 ? 2 : 1;
+  (void)s;
+  return want > 0 ? want : 1;
 }
 
 template <typename T, int JPT, bool KD, bool ARB, bool J3>
@@ -513,15 +513,36 @@ int launch_proposals(qmcx_batch_t b, hipStream_t st, int det, int e0, int ne, ui
   const int nq = s->npad / (16 / (int)sizeof(T));
   const int slices = proposal_slices<T>(s, b->slices);
   const int nt = std::max(64, (nq * slices + 63) / 64 * 64);
-  size_t lds = std::max(sizeof(T) * (size_t)slices * 5 * s->npad, sizeof(double) * 16 * 5);
-  if (sizeof(T) == 4)   // and the FP64 values of the FP32 form
+  if (nt > 1024) return fail(QMCX_ERR_ARG, "the orbital table is too wide for one proposals workgroup");
+  const bool v64 = sizeof(T) == 4 && b->rows64;   // BatchP::spo64 is set exactly then
+  // LDS: the J1 wave sums, the slice partials (none with one slice: its lanes store from
+  // registers) and the FP64 values of the FP32 form's band
+  size_t lds = sizeof(double) * 16 * 5;
+  if (slices != 1 || v64) lds = std::max(lds, sizeof(T) * (size_t)slices * 5 * s->npad);
+  if (v64)
     lds = std::max(lds, ((sizeof(T) * (size_t)slices * 5 * s->npad + 15) & ~(size_t)15) +
                             sizeof(double) * (size_t)slices * s->npad);
   // about 1024 workgroups (four per CU), each looping over its electrons: the gather's burst of
   // HBM traffic stays below what slows the latency-bound move kernel beside it
   const int gx = std::max(1, std::min(ne, kPropWorkgroups / b->W));
-  hipLaunchKernelGGL(k_spo_proposals<T>, dim3(gx, b->W), dim3(nt), lds, st, s->params<T>(), b->params<T>(), det, e0,
-                     ne, seed, b->sweep, std::sqrt(tau), b->walker0, displ, (T*)b->spo, b->prop, slices);
+  auto kern = [&](auto kslc, auto kv64) {
+    hipLaunchKernelGGL((k_spo_proposals<T, decltype(kslc)::value, decltype(kv64)::value>), dim3(gx, b->W), dim3(nt),
+                       lds, st, s->params<T>(), b->params<T>(), det, e0, ne, seed, b->sweep, std::sqrt(tau),
+                       b->walker0, displ, (T*)b->spo, b->prop, slices);
+  };
+  auto by_slices = [&](auto kv64) {
+    switch (slices) {
+      case 1: kern(std::integral_constant<int, 1>{}, kv64); break;
+      case 2: kern(std::integral_constant<int, 2>{}, kv64); break;
+      case 4: kern(std::integral_constant<int, 4>{}, kv64); break;
+      case 8: kern(std::integral_constant<int, 8>{}, kv64); break;
+      default: kern(std::integral_constant<int, 16>{}, kv64); break;
+    }
+  };
+  if (v64)
+    by_slices(std::true_type{});
+  else
+    by_slices(std::false_type{});
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -133,31 +133,43 @@ __device__ inline void knot_weight(int i, T t, T& w, T& dw, T& d2w) {
 
 // V64: the value is also summed in FP64 (weights from the FP64 fraction, the same loaded
 // coefficients) into part64 [S][npad]: the FP64 Slater rows of the FP32 form.
-template <typename T, int NO, int PER, bool V64 = false>
-__device__ inline void spline_gather(const SysP<T>& S, const double* spoG, double x, double y, double z, T* part,
-                                     int tid, double* part64 = nullptr) {
+// The partial sums of lane tid (orbital quad q, stencil slice s) in registers: acc[o][e] for
+// orbital q*VW + e, and with V64 the FP64 values acc64.  False for the lanes past the slices.
+// UNI (one slice only): the point is the same on every lane of the wave, so its grid cell moves
+// to scalar registers and every load takes a scalar address plus the lane's orbital offset.
+template <typename T, int NO, int PER, bool V64, bool UNI = false>
+__device__ inline bool spline_gather_regs(const SysP<T>& S, const double* spoG, double x, double y, double z, int tid,
+                                          T (&acc)[NO][V16<T>::n], double (&acc64)[V16<T>::n], int& q, int& s) {
   using VT = typename V16<T>::type;
   constexpr int VW = V16<T>::n;
   constexpr int SL = 16 / PER;
   // stencil columns with loads in flight per batch: up to four in FP32, two in FP64
   constexpr int G = sizeof(T) == 8 ? (PER > 2 ? 2 : PER) : (PER > 4 ? 4 : PER);
   const int nq = S.npad / VW;
-  const int q = tid % nq, s = tid / nq;
-  if (s >= SL) return;
+  q = tid % nq;
+  s = tid / nq;
+  if (s >= SL) return false;
+  static_assert(!UNI || SL == 1, "a wave-uniform point with one slice only");
+  if constexpr (UNI) s = 0;
   int ix[3];
   double tt[3];
   spline_locate(spoG, S.ng, x, y, z, ix, tt);
-  const T* base0 = S.coefs + (size_t)ix[0] * S.xs + (size_t)ix[1] * S.ys + (size_t)ix[2] * S.zs + (size_t)q * VW;
-  const size_t zs = S.zs / VW;
+  if constexpr (UNI) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ix[j] = __builtin_amdgcn_readfirstlane(ix[j]);
+  }
+  // base0 (the cell's first coefficient) is wave-uniform with UNI; the lane's quad is added at the load
+  const T* base0 = S.coefs + (size_t)ix[0] * S.xs + (size_t)ix[1] * S.ys + (size_t)ix[2] * S.zs;
+  const unsigned lane_off = (unsigned)q * VW;
+  const size_t zs = S.zs;
   T c[4], dc[4], d2c[4];
   bspline_weights<T>((T)tt[2], c, dc, d2c);
   const T tx = (T)tt[0], ty = (T)tt[1];
-  T acc[NO][VW];
 #pragma unroll
   for (int o = 0; o < NO; ++o)
 #pragma unroll
     for (int e = 0; e < VW; ++e) acc[o][e] = 0;
-  double c64[4], acc64[VW];
+  double c64[4];
   if constexpr (V64) {
     double dc64[4], d2c64[4];
     bspline_weights<double>(tt[2], c64, dc64, d2c64);
@@ -170,9 +182,10 @@ __device__ inline void spline_gather(const SysP<T>& S, const double* spoG, doubl
 #pragma unroll
   for (int pp = 0; pp < G; ++pp) {
     const int p = s * PER + g0 + pp;
-    const VT* src = reinterpret_cast<const VT*>(base0 + (size_t)(p >> 2) * S.xs + (size_t)(p & 3) * S.ys);
+    const T* src = base0 + (size_t)(p >> 2) * S.xs + (size_t)(p & 3) * S.ys;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v4[pp][k] = __builtin_nontemporal_load(src + k * zs);
+    for (int k = 0; k < 4; ++k)
+      v4[pp][k] = __builtin_nontemporal_load(reinterpret_cast<const VT*>(src + k * zs + lane_off));
   }
 #pragma unroll
   for (int pp = 0; pp < G; ++pp) {
@@ -214,10 +227,26 @@ __device__ inline void spline_gather(const SysP<T>& S, const double* spoG, doubl
     }
   }
   }
+  return true;
+}
+
+template <typename T, int NO, int PER, bool V64 = false>
+__device__ inline void spline_gather(const SysP<T>& S, const double* spoG, double x, double y, double z, T* part,
+                                     int tid, double* part64 = nullptr) {
+  using VT = typename V16<T>::type;
+  constexpr int VW = V16<T>::n;
+  T acc[NO][VW];
+  double acc64[VW];
+  int q, s;
+  if (!spline_gather_regs<T, NO, PER, V64>(S, spoG, x, y, z, tid, acc, acc64, q, s)) return;
+  // one 16-byte LDS store per output (npad is a multiple of VW)
 #pragma unroll
-  for (int o = 0; o < NO; ++o)
+  for (int o = 0; o < NO; ++o) {
+    VT v;
 #pragma unroll
-    for (int e = 0; e < VW; ++e) part[((size_t)s * NO + o) * S.npad + q * VW + e] = acc[o][e];
+    for (int e = 0; e < VW; ++e) v[e] = acc[o][e];
+    *reinterpret_cast<VT*>(part + ((size_t)s * NO + o) * S.npad + q * VW) = v;
+  }
   if constexpr (V64)
 #pragma unroll
     for (int e = 0; e < VW; ++e) part64[(size_t)s * S.npad + q * VW + e] = acc64[e];
@@ -263,6 +292,29 @@ __device__ inline void spline_finish(const SysP<T>& S, int slices, const T* part
     T sum = 0;
     for (int s = 0; s < slices; ++s) sum += part[((size_t)s * NO + o) * S.npad + j];
     out[(size_t)o * ostride + j] = (TO)(sum * vgh_scale<T>(o, gx, gy, gz));
+  }
+}
+
+// One slice (each lane sums all 64 stencil points of its orbital quad): the scaled outputs go from
+// registers to out[o*npad + j] with 16-byte stores, no LDS.  The padded orbitals (zero
+// coefficients) are written too.  Equal to spline_gather + spline_finish with one slice.  The
+// point must be the same on every lane of the workgroup.
+template <typename T, int NO>
+__device__ inline void spline_gather_direct(const SysP<T>& S, double x, double y, double z, T* out) {
+  using VT = typename V16<T>::type;
+  constexpr int VW = V16<T>::n;
+  T acc[NO][VW];
+  double acc64[VW];
+  int q, s;
+  if (!spline_gather_regs<T, NO, 16, false, true>(S, S.spoG, x, y, z, threadIdx.x, acc, acc64, q, s)) return;
+  const T gx = (T)S.ng[0], gy = (T)S.ng[1], gz = (T)S.ng[2];
+#pragma unroll
+  for (int o = 0; o < NO; ++o) {
+    const T f = vgh_scale<T>(o, gx, gy, gz);
+    VT v;
+#pragma unroll
+    for (int e = 0; e < VW; ++e) v[e] = (T(0) + acc[o][e]) * f;
+    *reinterpret_cast<VT*>(out + (size_t)o * S.npad + q * VW) = v;
   }
 }
 
@@ -971,7 +1023,20 @@ __global__ __launch_bounds__(1024) void k_spo_rows(SysP<T> S, BatchP<T> B, int s
 // the spline's HBM traffic out of the move kernel's accept chain.  The displacement is the
 // move kernel's: sqrt(tau) times the walker's Philox normals of (sweep, electron), or the
 // caller's (displ[w][3]).
-template <typename T>
+// the gather of one point with the slice count fixed at compile time (SLC > 0) or chosen at run
+// time (SLC == 0): a kernel that carries one variant allocates registers for that one alone
+template <typename T, int NO, int SLC, bool V64 = false>
+__device__ inline void spline_gather_sl(const SysP<T>& S, double x, double y, double z, int slices, T* part,
+                                        double* part64 = nullptr) {
+  if constexpr (SLC == 0)
+    spline_gather_rt<T, NO, V64>(S, x, y, z, slices, part, part64);
+  else
+    spline_gather<T, NO, 16 / SLC, V64>(S, S.spoG, x, y, z, part, threadIdx.x, part64);
+}
+
+// SLC: slices fixed at compile time (0: the run-time argument); V64: the FP32 form's FP64 value
+// rows are written as well (only while the batch keeps them, B.spo64 != nullptr)
+template <typename T, int SLC, bool V64>
 __device__ __forceinline__ void spo_proposals_body(const SysP<T>& S, const BatchP<T>& B, int det, int e0, int ne,
                                                    uint64_t seed, const int* sweep_ctr, double sqrt_tau, int walker0,
                                                    const double* displ, T* spo, double* prop, int slices, const int w) {
@@ -997,18 +1062,22 @@ __device__ __forceinline__ void spo_proposals_body(const SysP<T>& S, const Batch
     d[2] = sqrt_tau * nr[2];
   }
   const double x = R[iel] + d[0], y = R[N + iel] + d[1], z = R[2 * N + iel] + d[2];
-  if (B.spo64 != nullptr) {
+  if constexpr (V64) {
     double* part64 = (double*)(smem + ((sizeof(T) * (size_t)slices * 5 * S.npad + 15) & ~(size_t)15));
-    spline_gather_rt<T, 5, true>(S, x, y, z, slices, part, part64);
+    spline_gather_sl<T, 5, SLC, true>(S, x, y, z, slices, part, part64);
     __syncthreads();
     spline_finish64<T>(S, slices, part64, B.spo64 + (((size_t)w * 2 + det) * n + el) * S.npad, n);
-  } else {
-    spline_gather_rt<T, 5>(S, x, y, z, slices, part);
+  } else if constexpr (SLC != 1) {
+    spline_gather_sl<T, 5, SLC>(S, x, y, z, slices, part);
     __syncthreads();
   }
   T* out = spo + (((size_t)w * 2 + det) * n + el) * 5 * S.npad;
-  spline_finish<T, 5, T>(S, slices, part, out, S.npad, n);
-  __syncthreads();
+  if constexpr (SLC == 1 && !V64) {
+    spline_gather_direct<T, 5>(S, x, y, z, out);
+  } else {
+    spline_finish<T, 5, T>(S, slices, part, out, S.npad, n);
+    __syncthreads();
+  }
   // J1 at the proposal (OneBodyJastrow::ratioGrad, OneBodyJastrow.h:259-268): it depends on the
   // proposal and the ions only, so it is evaluated here with the SPO row
   const Cell<T> C = *S.cell;
@@ -1081,11 +1150,12 @@ __device__ __forceinline__ void spo_proposals_body(const SysP<T>& S, const Batch
   }
 }
 
-template <typename T>
+template <typename T, int SLC, bool V64>
 __global__ __launch_bounds__(1024) void k_spo_proposals(SysP<T> S, BatchP<T> B, int det, int e0, int ne, uint64_t seed,
                                                          const int* sweep_ctr, double sqrt_tau, int walker0,
                                                          const double* displ, T* spo, double* prop, int slices) {
-  spo_proposals_body<T>(S, B, det, e0, ne, seed, sweep_ctr, sqrt_tau, walker0, displ, spo, prop, slices, blockIdx.y);
+  spo_proposals_body<T, SLC, V64>(S, B, det, e0, ne, seed, sweep_ctr, sqrt_tau, walker0, displ, spo, prop, slices,
+                                  blockIdx.y);
 }
 
 // The proposals of one determinant of every walker of several batches in one launch: batch g owns
@@ -1106,8 +1176,8 @@ template <typename T>
 __global__ __launch_bounds__(1024) void k_spo_proposals_group(PropGroup<T> G, int det, uint64_t seed, double sqrt_tau) {
   int g = 0;
   while (g + 1 < G.ng && (int)blockIdx.y >= G.first[g + 1]) ++g;
-  spo_proposals_body<T>(G.S[g], G.B[g], det, 0, G.S[g].n, seed, G.B[g].sweep, sqrt_tau, G.walker0[g], nullptr,
-                        G.spo[g], G.prop[g], G.slices[g], (int)blockIdx.y - G.first[g]);
+  spo_proposals_body<T, 0, false>(G.S[g], G.B[g], det, 0, G.S[g].n, seed, G.B[g].sweep, sqrt_tau, G.walker0[g],
+                                  nullptr, G.spo[g], G.prop[g], G.slices[g], (int)blockIdx.y - G.first[g]);
 }
 
 // stand-alone VGH at arbitrary points: out[p][10][norb] (FP64)

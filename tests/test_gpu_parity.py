@@ -92,10 +92,11 @@ def test_invert_transpose_batched_vs_oracle(n):
         assert np.abs(psiM[m] - ref).max() < 1e-9 * np.abs(ref).max()
 
 
-def _compare_run(sysd, dev, orc, nw, nsweeps, delay, prec, mode, tau, tol_ratio, tol_log, seed=11, flush_kind=0):
+def _compare_run(sysd, dev, orc, nw, nsweeps, delay, prec, mode, tau, tol_ratio, tol_log, seed=11, flush_kind=0,
+                 tile=0):
     R0 = systems.initial_positions(sysd, nw, seed=seed)
     ref = orc.run_vmc(R0, nsweeps, seed, tau, mode=mode, delay=delay, trace=True)
-    wb = qmcx.WalkerBatch(dev, nw, delay_rank=delay, precision=prec)
+    wb = qmcx.WalkerBatch(dev, nw, delay_rank=delay, precision=prec, tile=tile)
     wb.set_flush(flush_kind)
     wb.set_positions(R0)
     log0 = wb.evaluate_log()
@@ -123,6 +124,21 @@ def _compare_run(sysd, dev, orc, nw, nsweeps, delay, prec, mode, tau, tol_ratio,
         np.testing.assert_allclose(hist[0], ref["log"][:, s], rtol=tol_log)
     G, L, lv = wb.gl()
     return dict(flips=flips, ref=ref, G=G, L=L, log=lv, acc=wb.accepts(), R=wb.positions())
+
+
+@pytest.mark.parametrize("prec", [qmcx.FP64, qmcx.FP32])
+@pytest.mark.parametrize("tile", [1, 2, 4, 8, 16])
+def test_sweep_spline_tiles_vs_oracle(cubic, cubic_dev, cubic_orc, prec, tile):
+    """Every stencil split of the proposals kernel (1 slice: partial sums kept in registers and
+    stored directly; 2-16 slices: reduced through LDS): FP64 ratios within 1e-9, FP32 within 2e-3
+    of max(|r|, 1), identical accept sequences."""
+    _say("sweep tile", tile, "precision", prec)
+    fp64 = prec == qmcx.FP64
+    out = _compare_run(cubic, cubic_dev, cubic_orc, nw=2, nsweeps=1, delay=1 if fp64 else 8, prec=prec,
+                       mode=qmcx.ACCEPT_METROPOLIS, tau=0.5, tol_ratio=1e-9 if fp64 else 2e-3,
+                       tol_log=1e-11 if fp64 else 1e-6, tile=tile)
+    assert out["flips"] == 0
+    assert np.array_equal(out["acc"], out["ref"]["naccept"])
 
 
 @pytest.mark.parametrize("delay", [1, 4, 16])
