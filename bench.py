@@ -278,6 +278,84 @@ def measure_nlpp(dev, sysd, R0, prec, steps, seed, tau, rmax=1.7):
                                  "the algorithmic bytes"}}
 
 
+MIXED = (("nio16", 64), ("nio32", 64), ("nio64", 32))
+
+
+def measure_mixed(dev64, sys64, mode, steps, err_steps, seed, tau):
+    """The uneven mixed batch: 64 NiO-16 + 64 NiO-32 + 32 NiO-64 walkers, each system with its own
+    seeded spline table and Jastrows, advanced together by one grouped move launch and one grouped
+    proposals launch per stage (qmcx_group_*), FP64 refresh every sweep of FP64 Slater rows
+    (qmcx_batch_set_fp64_rows: the refresh carries no FP32 rounding of ill-conditioned matrices,
+    which the reference driver's u < 0.5 rule visits at NiO-32).  Per size: the autotuned
+    delay rank, moves/s (that size's moves over the group's wall time), log|Psi| error vs the FP64
+    delay-1 GPU path from the same start and streams.  Beside it the same batches run one after
+    the other, and the group replayed from a captured HIP graph."""
+    parts, off = [], 0
+    for name, nw in MIXED:
+        sysd = sys64 if name == sys64.name else systems.make_system(name, seed=seed)
+        dev = dev64 if name == sys64.name else qmcx.System(sysd, precisions=(qmcx.FP32, qmcx.FP64))
+        R0 = systems.initial_positions(sysd, nw, seed=seed, walker0=off)
+        k, tile, tune = autotune.tune(dev, nw, R0, seed=seed, tau=tau, mode=mode, delays=(16, 32), tiles=(1,))
+        parts.append(dict(name=name, sysd=sysd, dev=dev, nw=nw, R0=R0, k=k, off=off, tune=tune))
+        off += nw
+
+    def batches(prec=qmcx.FP32):
+        out = []
+        for p in parts:
+            b = qmcx.WalkerBatch(p["dev"], p["nw"], delay_rank=p["k"] if prec == qmcx.FP32 else 1, precision=prec,
+                                 walker0=p["off"])
+            b.set_fp64_rows(True)
+            b.set_positions(p["R0"])
+            b.evaluate_log()
+            out.append(b)
+        return out
+
+    # error leg: the group vs FP64 delay-1 batches, same start, same streams
+    grp_b, ref_b = batches(), batches(qmcx.FP64)
+    g = qmcx.Group(grp_b)
+    hg = g.run(err_steps, seed, tau, mode, recompute_every=1)
+    errs = []
+    for i, p in enumerate(parts):
+        hr = ref_b[i].run(err_steps, seed, tau, mode)
+        same = grp_b[i].accepts() == ref_b[i].accepts()
+        e = np.abs(hg[i] - hr) / np.abs(hr)
+        errs.append(dict(max_rel=float(e[:, same].max()) if same.any() else None,
+                         walkers_same_accepts=int(same.sum())))
+        ref_b[i].close()
+    # timed: the group launched, then replayed from a graph, then the batches one after the other
+    g.run(1, seed, tau, mode, recompute_every=1)
+    t0 = time.perf_counter()
+    g.run(steps, seed, tau, mode, recompute_every=1)
+    wall = time.perf_counter() - t0
+    g.set_graph(True)
+    g.run(steps, seed, tau, mode, recompute_every=1)
+    t0 = time.perf_counter()
+    g.run(steps, seed, tau, mode, recompute_every=1)
+    wall_g = time.perf_counter() - t0
+    g.close()
+    t0 = time.perf_counter()
+    for b in grp_b:
+        b.run(steps, seed, tau, mode, recompute_every=1)
+    wall_sep = time.perf_counter() - t0
+    for b in grp_b:
+        b.close()
+    total = sum(p["nw"] * p["sysd"].nel for p in parts) * steps
+    sizes = {}
+    for i, p in enumerate(parts):
+        mv = p["nw"] * p["sysd"].nel * steps
+        sizes[p["name"]] = {"walkers": p["nw"], "electrons": p["sysd"].nel, "delay_rank": p["k"],
+                            "moves_per_s": mv / wall, "log_psi_err_vs_fp64": errs[i], "autotune": p["tune"]}
+        log(f"mixed {p['name']}: {p['nw']} walkers, delay {p['k']}, {mv / wall:.4e} moves/s in the group, "
+            f"log err vs FP64 {errs[i]['max_rel']:.2e} ({errs[i]['walkers_same_accepts']}/{p['nw']} same accepts)")
+    log(f"mixed batch: {total / wall:.4e} moves/s grouped ({1e3 * wall / steps:.2f} ms/sweep), graph replay "
+        f"{total / wall_g:.4e}, the batches one after the other {total / wall_sep:.4e}")
+    return {"value": total / wall, "unit": "walker-moves/s", "ms_per_step": 1e3 * wall / steps, "steps": steps,
+            "graph_replay_value": total / wall_g, "separate_launches_value": total / wall_sep, "sizes": sizes,
+            "note": "one grouped move launch and one grouped proposals launch per determinant advance every "
+                    "walker of the three systems; FP64 refresh every sweep per batch (FP64 Slater rows); log|Psi| after "
+                    f"{err_steps} sweeps vs the FP64 delay-1 GPU path on walkers with the same accept counts"}
+
+
 def measure_evaluate_log(dev, sysd, R0, prec, steps):
     """WaveFunction::evaluateLog from scratch for the whole batch (SPO rows at every electron, FP64 LU,
     J1/J2 over all pairs, evaluateGL): walkers per second."""
@@ -313,6 +391,8 @@ def main():
                     help="FP32 form: FP64 LU inverse recompute every this many sweeps (0: never)")
     ap.add_argument("--err-steps", type=int, default=2, help="sweeps of the FP32-vs-FP64 error leg")
     ap.add_argument("--nlpp-steps", type=int, default=5, help="timed NonLocalPP evaluations per precision (0: skip)")
+    ap.add_argument("--mixed-steps", type=int, default=10,
+                    help="timed sweeps of the mixed NiO-16/32/64 batch in one grouped launch (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device five times (a kernel of known HBM traffic "
@@ -430,6 +510,10 @@ def main():
             elog[r["dtype"]] = r
             log(f"evaluateLog ({r['dtype']}): {r['ms_per_batch']:.2f} ms for {W} walkers")
 
+    mixed = None
+    if args.mixed_steps > 0:
+        mixed = measure_mixed(dev, sysd, mode, args.mixed_steps, args.err_steps, args.seed, args.tau)
+
     cpu = None
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
@@ -484,6 +568,7 @@ def main():
         "autotune": tune,
         "nlpp_virtual_ratios": nl or None,
         "evaluate_log": elog or None,
+        "mixed_batch": mixed,
     }
     print(json.dumps(out))
 

@@ -200,6 +200,13 @@ int qmcx_get_trace(qmcx_batch_t b, double* trace);
 int qmcx_batch_set_accept_band(qmcx_batch_t b, double band);
 int qmcx_get_arbitrations(qmcx_batch_t b, long long* narb);
 
+/* FP64 Slater rows in the FP32 form without the band (no reference counterpart): the proposals also
+ * sum the FP64 spline values, accepted moves store them, and the FP64 refresh inverts those, so
+ * log|Psi| after a refresh carries no FP32 rounding of the matrix, however ill-conditioned.  As with
+ * the band, the next qmcx_evaluate_log fills them before the next qmcx_run_sweeps.  enable 0 (the
+ * default) or 1; FP64 batches ignore it; not with a three-body Jastrow. */
+int qmcx_batch_set_fp64_rows(qmcx_batch_t b, int enable);
+
 /* Outputs of the last evaluateGL: G[nw][N][3], L[nw][N], log[nw]; per-walker accept counts. */
 int qmcx_get_gl(qmcx_batch_t b, double* G, double* L, double* log_out);
 int qmcx_get_accepts(qmcx_batch_t b, long long* naccept);
@@ -249,7 +256,8 @@ int qmcx_last_sweep_ms(qmcx_batch_t b, double* ms);
  * batches' walkers (the workgroup size and LDS are the largest the batches need); the FP64 refresh runs
  * one launch per batch (matrix size), evaluateGL per batch.  Batches: FP32, delay rank >= 2 with the
  * delayed updates applied inside the move kernel (n a multiple of 16, rank <= 32), no three-body Jastrow,
- * no near-tie band, one device.  The batches stay usable on their own; each keeps its own state,
+ * no near-tie band, all with FP64 Slater rows (qmcx_batch_set_fp64_rows) or none, one device.  The
+ * batches stay usable on their own; each keeps its own state,
  * random streams (its walker offset) and log history.
  * qmcx_group_run_sweeps: as qmcx_run_sweeps for every batch at once; log_out [nsweeps][sum of nw] (the
  * batches' walkers in group order, may be NULL).  qmcx_group_set_graph: as qmcx_batch_set_graph. */

@@ -285,6 +285,7 @@ struct qmcx_batch_s {
   // rows64_stale until an evaluateLog has filled them
   double band = 0.0;
   bool rows64 = false, rows64_stale = false;
+  bool fp64_rows = false;   // FP64 Slater rows asked for on their own (rows64 also follows the band)
   double* flex_displ = nullptr;   // [W][3] proposals of the pending flex move
   int* flex_decide = nullptr;     // [W]
   int flex_iel = -1;
@@ -515,11 +516,11 @@ int launch_proposals(qmcx_batch_t b, hipStream_t st, int det, int e0, int ne, ui
   const int nt = std::max(64, (nq * slices + 63) / 64 * 64);
   if (nt > 1024) return fail(QMCX_ERR_ARG, "the orbital table is too wide for one proposals workgroup");
   const bool v64 = sizeof(T) == 4 && b->rows64;   // BatchP::spo64 is set exactly then
-  // LDS: the J1 wave sums, the slice partials (none with one slice: its lanes store from
-  // registers) and the FP64 values of the FP32 form's band
+  // LDS: the J1 wave sums, the slice partials and the FP64 values of the FP32 form's FP64 rows
+  // (neither with one slice: its lanes store from registers)
   size_t lds = sizeof(double) * 16 * 5;
-  if (slices != 1 || v64) lds = std::max(lds, sizeof(T) * (size_t)slices * 5 * s->npad);
-  if (v64)
+  if (slices != 1) lds = std::max(lds, sizeof(T) * (size_t)slices * 5 * s->npad);
+  if (v64 && slices != 1)
     lds = std::max(lds, ((sizeof(T) * (size_t)slices * 5 * s->npad + 15) & ~(size_t)15) +
                             sizeof(double) * (size_t)slices * s->npad);
   // about 1024 workgroups (four per CU), each looping over its electrons: the gather's burst of
@@ -956,8 +957,12 @@ int group_proposals(qmcx_group_s* g, int det, hipEvent_t after, uint64_t seed, d
     P.slices[i] = proposal_slices<float>(b->sys, b->slices);
   }
   P.first[P.ng] = first;
-  hipLaunchKernelGGL(k_spo_proposals_group<float>, dim3(g->pgx, g->Wtot), dim3(g->pnt), g->plds, g->pre, P, det, seed,
-                     std::sqrt(tau));
+  if (g->b[0]->rows64)   // every batch of the group agrees (checked at each run)
+    hipLaunchKernelGGL((k_spo_proposals_group<float, true>), dim3(g->pgx, g->Wtot), dim3(g->pnt), g->plds, g->pre, P,
+                       det, seed, std::sqrt(tau));
+  else
+    hipLaunchKernelGGL((k_spo_proposals_group<float, false>), dim3(g->pgx, g->Wtot), dim3(g->pnt), g->plds, g->pre, P,
+                       det, seed, std::sqrt(tau));
   HIPCHK(hipGetLastError());
   if (det == 1)
     for (qmcx_batch_t b : g->b) {
@@ -1788,7 +1793,18 @@ int qmcx_batch_set_accept_band(qmcx_batch_t b, double band) {
   if (band > 0.0 && b->j3)
     return fail(QMCX_ERR_ARG, "the near-tie FP64 re-evaluation does not take a three-body Jastrow");
   b->band = band;
-  const bool on = b->prec == QMCX_FP32 && band > 0.0;
+  const bool on = b->prec == QMCX_FP32 && (band > 0.0 || b->fp64_rows);
+  if (on && !b->rows64) b->rows64_stale = true;
+  b->rows64 = on;
+  return 0;
+}
+
+int qmcx_batch_set_fp64_rows(qmcx_batch_t b, int enable) {
+  if (!b) return fail(QMCX_ERR_ARG, "null batch");
+  if (enable && b->j3) return fail(QMCX_ERR_ARG, "FP64 Slater rows do not take a three-body Jastrow");
+  b->drop_graph();   // a captured run holds the old setting
+  b->fp64_rows = enable != 0;
+  const bool on = b->prec == QMCX_FP32 && (b->band > 0.0 || b->fp64_rows);
   if (on && !b->rows64) b->rows64_stale = true;
   b->rows64 = on;
   return 0;
@@ -2277,7 +2293,8 @@ int qmcx_group_create(qmcx_group_t* out, const qmcx_batch_t* batches, int nbatch
     else if (b->prec != QMCX_FP32) why = "FP32 batches only";
     else if (b->K < 2) why = "delay rank 2 or more";
     else if (b->j3) why = "no three-body Jastrow";
-    else if (b->rows64) why = "no near-tie band";
+    else if (b->band > 0.0) why = "no near-tie band";
+    else if (b->rows64 != batches[0]->rows64) why = "all batches with FP64 Slater rows or none";
     else if (!use_mfma_flush<float>(b) || !flush_own_fits<float>(s->N, s->n, b->K, b->K))
       why = "each batch's delayed updates must be applied inside the move kernel (n a multiple of 16, delay <= 32)";
     else if (b->sys->dev != batches[0]->sys->dev) why = "batches of one device";
@@ -2303,7 +2320,10 @@ int qmcx_group_create(qmcx_group_t* out, const qmcx_batch_t* batches, int nbatch
     lds = std::max(lds, (size_t)move_lds<float>(s->N, s->nions, s->n, b->K, b->K).total);
     const int nq = s->npad / 4, slices = proposal_slices<float>(s, b->slices);
     pnt = std::max(pnt, (nq * slices + 63) / 64 * 64);
-    plds = std::max(plds, std::max(sizeof(float) * (size_t)slices * 5 * s->npad, sizeof(double) * 16 * 5));
+    // slice partials, then the FP64 values when the batches keep FP64 Slater rows
+    plds = std::max(plds, std::max(((sizeof(float) * (size_t)slices * 5 * s->npad + 15) & ~(size_t)15) +
+                                       sizeof(double) * (size_t)slices * s->npad,
+                                   sizeof(double) * 16 * 5));
   }
   g->nt = nt;
   g->jpt = need_jpt2 ? 2 : 1;
@@ -2356,7 +2376,12 @@ int qmcx_group_run_sweeps(qmcx_group_t g, int nsweeps, uint64_t seed, double tau
     return fail(QMCX_ERR_ARG, "unknown accept mode");
   HIPCHK(hipStreamSynchronize(g->stream));
   for (qmcx_batch_t b : g->b) {
-    if (b->rows64 || b->prof_on) return fail(QMCX_ERR_STATE, "a group batch took the near-tie band or kernel timing");
+    if (b->band > 0.0 || b->prof_on)
+      return fail(QMCX_ERR_STATE, "a group batch took the near-tie band or kernel timing");
+    if (b->rows64 != g->b[0]->rows64)
+      return fail(QMCX_ERR_STATE, "the group's batches must all keep FP64 Slater rows or none");
+    if (b->rows64 && b->rows64_stale)
+      return fail(QMCX_ERR_STATE, "FP64 Slater rows were switched on after qmcx_evaluate_log: call it again");
     if (int rc = check_j3(b)) return rc;
     if (nsweeps > b->hist_cap) {
       if (b->hist) hipFree(b->hist);

@@ -143,8 +143,9 @@ __device__ inline bool spline_gather_regs(const SysP<T>& S, const double* spoG, 
   using VT = typename V16<T>::type;
   constexpr int VW = V16<T>::n;
   constexpr int SL = 16 / PER;
-  // stencil columns with loads in flight per batch: up to four in FP32, two in FP64
-  constexpr int G = sizeof(T) == 8 ? (PER > 2 ? 2 : PER) : (PER > 4 ? 4 : PER);
+  // stencil columns with loads in flight per batch: up to four in FP32 (two when the FP64 values
+  // are summed as well), two in FP64
+  constexpr int G = (sizeof(T) == 8 || V64) ? (PER > 2 ? 2 : PER) : (PER > 4 ? 4 : PER);
   const int nq = S.npad / VW;
   q = tid % nq;
   s = tid / nq;
@@ -299,14 +300,25 @@ __device__ inline void spline_finish(const SysP<T>& S, int slices, const T* part
 // registers to out[o*npad + j] with 16-byte stores, no LDS.  The padded orbitals (zero
 // coefficients) are written too.  Equal to spline_gather + spline_finish with one slice.  The
 // point must be the same on every lane of the workgroup.
-template <typename T, int NO>
-__device__ inline void spline_gather_direct(const SysP<T>& S, double x, double y, double z, T* out) {
+// With V64 the FP64 values go to out64[j] as well.
+template <typename T, int NO, bool V64 = false>
+__device__ inline void spline_gather_direct(const SysP<T>& S, double x, double y, double z, T* out,
+                                            double* out64 = nullptr) {
   using VT = typename V16<T>::type;
   constexpr int VW = V16<T>::n;
   T acc[NO][VW];
   double acc64[VW];
   int q, s;
-  if (!spline_gather_regs<T, NO, 16, false, true>(S, S.spoG, x, y, z, threadIdx.x, acc, acc64, q, s)) return;
+  if (!spline_gather_regs<T, NO, 16, V64, true>(S, S.spoG, x, y, z, threadIdx.x, acc, acc64, q, s)) return;
+  if constexpr (V64) {
+#pragma unroll
+    for (int e = 0; e < VW; e += 2) {
+      f64x2 v;
+      v[0] = 0.0 + acc64[e];
+      v[1] = 0.0 + acc64[e + 1];
+      *reinterpret_cast<f64x2*>(out64 + q * VW + e) = v;
+    }
+  }
   const T gx = (T)S.ng[0], gy = (T)S.ng[1], gz = (T)S.ng[2];
 #pragma unroll
   for (int o = 0; o < NO; ++o) {
@@ -1062,18 +1074,20 @@ __device__ __forceinline__ void spo_proposals_body(const SysP<T>& S, const Batch
     d[2] = sqrt_tau * nr[2];
   }
   const double x = R[iel] + d[0], y = R[N + iel] + d[1], z = R[2 * N + iel] + d[2];
-  if constexpr (V64) {
+  if constexpr (SLC == 1) {
+  } else if constexpr (V64) {
     double* part64 = (double*)(smem + ((sizeof(T) * (size_t)slices * 5 * S.npad + 15) & ~(size_t)15));
     spline_gather_sl<T, 5, SLC, true>(S, x, y, z, slices, part, part64);
     __syncthreads();
     spline_finish64<T>(S, slices, part64, B.spo64 + (((size_t)w * 2 + det) * n + el) * S.npad, n);
-  } else if constexpr (SLC != 1) {
+  } else {
     spline_gather_sl<T, 5, SLC>(S, x, y, z, slices, part);
     __syncthreads();
   }
   T* out = spo + (((size_t)w * 2 + det) * n + el) * 5 * S.npad;
-  if constexpr (SLC == 1 && !V64) {
-    spline_gather_direct<T, 5>(S, x, y, z, out);
+  if constexpr (SLC == 1) {
+    spline_gather_direct<T, 5, V64>(S, x, y, z, out,
+                                    V64 ? B.spo64 + (((size_t)w * 2 + det) * n + el) * S.npad : nullptr);
   } else {
     spline_finish<T, 5, T>(S, slices, part, out, S.npad, n);
     __syncthreads();
@@ -1172,12 +1186,12 @@ struct PropGroup {
   int walker0[kMaxPropGroups], slices[kMaxPropGroups];
 };
 
-template <typename T>
+template <typename T, bool V64>
 __global__ __launch_bounds__(1024) void k_spo_proposals_group(PropGroup<T> G, int det, uint64_t seed, double sqrt_tau) {
   int g = 0;
   while (g + 1 < G.ng && (int)blockIdx.y >= G.first[g + 1]) ++g;
-  spo_proposals_body<T, 0, false>(G.S[g], G.B[g], det, 0, G.S[g].n, seed, G.B[g].sweep, sqrt_tau, G.walker0[g],
-                                  nullptr, G.spo[g], G.prop[g], G.slices[g], (int)blockIdx.y - G.first[g]);
+  spo_proposals_body<T, 0, V64>(G.S[g], G.B[g], det, 0, G.S[g].n, seed, G.B[g].sweep, sqrt_tau, G.walker0[g],
+                                nullptr, G.spo[g], G.prop[g], G.slices[g], (int)blockIdx.y - G.first[g]);
 }
 
 // stand-alone VGH at arbitrary points: out[p][10][norb] (FP64)

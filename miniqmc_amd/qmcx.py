@@ -60,6 +60,7 @@ _SIGS = {
     "qmcx_get_gl": [_vp, _vp, _vp, _vp],
     "qmcx_get_accepts": [_vp, _lp],
     "qmcx_batch_set_accept_band": [_vp, C.c_double],
+    "qmcx_batch_set_fp64_rows": [_vp, C.c_int],
     "qmcx_get_arbitrations": [_vp, _lp],
     "qmcx_get_inverse": [_vp, _i, _i, _dp],
     "qmcx_last_sweep_ms": [_vp, C.POINTER(C.c_double)],
@@ -312,6 +313,10 @@ class WalkerBatch:
     def set_accept_band(self, band):
         """FP32 Metropolis: decide moves with |u - R^2| <= band R^2 on the FP64 ratio (0: off)."""
         _check(load().qmcx_batch_set_accept_band(self.h, float(band)))
+
+    def set_fp64_rows(self, enable=True):
+        """FP32 form: keep FP64 Slater rows for the FP64 refresh (evaluate_log fills them)."""
+        _check(load().qmcx_batch_set_fp64_rows(self.h, int(bool(enable))))
 
     def arbitrations(self):
         a = np.zeros(self.nw, dtype=np.int64)

@@ -99,3 +99,27 @@ def test_group_rejects_unsupported_batches(mixed):
         qmcx.Group([qmcx.WalkerBatch(fp64_sys, 1, delay_rank=4, precision=qmcx.FP64)])
     with pytest.raises(qmcx.QmcxError, match="delay rank"):
         qmcx.Group([qmcx.WalkerBatch(dev, 1, delay_rank=1, precision=qmcx.FP32)])
+
+
+def test_group_fp64_rows_vs_oracle(mixed):
+    """Batches keeping FP64 Slater rows (the grouped proposals also sum the FP64 values): log|Psi|
+    after each sweep's FP64 refresh within 1e-6 of the FP64 oracle (the determinant's part carries
+    no FP32 rounding of the matrix; the Jastrow sums stay FP32); the oracle's chains."""
+    grp_b, R0s = _batches(mixed)
+    for b in grp_b:
+        b.set_fp64_rows(True)
+        b.evaluate_log()
+    g = qmcx.Group(grp_b)
+    hg = g.run(2, SEED, TAU, qmcx.ACCEPT_REFERENCE, recompute_every=1)
+    off = 0
+    for i, (name, sysd, dev, k, nw) in enumerate(mixed):
+        ref = po.OracleSystem(sysd).run_vmc(R0s[i], 2, SEED, TAU, mode=qmcx.ACCEPT_REFERENCE, delay=1, walker0=off)
+        err = np.abs(hg[i] - ref["log"].T) / np.abs(ref["log"].T)
+        _say(f"  {name} with FP64 rows: log err vs the FP64 oracle {err.max():.2e}")
+        assert err.max() < 1e-6
+        np.testing.assert_allclose(grp_b[i].positions(), ref["R"], rtol=0, atol=1e-11)
+        off += nw
+    # all or none
+    grp_b[0].set_fp64_rows(False)
+    with pytest.raises(qmcx.QmcxError, match="FP64 Slater rows"):
+        g.run(1, SEED, TAU, qmcx.ACCEPT_REFERENCE, recompute_every=1)
