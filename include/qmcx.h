@@ -126,6 +126,16 @@ int qmcx_batch_set_tile(qmcx_batch_t b, int spline_slices);
  * 0 = FP32 matrix cores (v_mfma_f32_16x16x4f32) when the batch is FP32 and n % 16 == 0, 1 = vector FMA,
  * 2 = matrix cores with the new Slater rows read from HBM rather than staged in LDS. */
 int qmcx_batch_set_flush(qmcx_batch_t b, int kind);
+/* Form of the delayed updates (DelayedUpdate.h:96-231; no reference counterpart for the choice):
+ * 0 = automatic: a chunk's stale inverse rows, the capacitance inverse and the products U_d . Ainv[row]
+ * stay in LDS (the in-kernel flush when it fits), or the streamed form when the delay rank's rows do not
+ * fit the LDS (e.g. k = 64 and 128 at 384 orbitals); 1 = streamed: the stale rows are read from the
+ * inverse in HBM, p = U . Ainv[row] is formed at each move as getInvRow does, the capacitance inverse and
+ * the row list live in the chunk's delayed-update state, and a flush launch applies each chunk.  The flex_*
+ * calls always use the streamed form with k > 1 and keep each walker's accepted rows pending across calls
+ * until it has k of them or qmcx_flex_complete_updates.  get reports the form the batch's sweeps use. */
+int qmcx_batch_set_delay_form(qmcx_batch_t b, int form);
+int qmcx_batch_get_delay_form(qmcx_batch_t b, int* form);
 
 /* WaveFunction::evaluateLog (src/QMCWaveFunctions/WaveFunction.cpp:178-195): SPO for all
  * electrons, FP64 LU, J1/J2 from scratch.  log_out[nw] (may be NULL). */
@@ -157,7 +167,11 @@ int qmcx_batch_set_graph(qmcx_batch_t b, int enable);
  *   displ[nw][3] -> ratio[nw], grad_now[nw][3], grad_new[nw][3] (outputs may be NULL); no state changes.
  * qmcx_flex_accept_restore: WaveFunction::flex_acceptrestoreMove + ParticleSet::acceptMove/rejectMove
  *   (WaveFunction.cpp:418-485, miniqmc_sync_move.cpp:496-505) for the same electron, accepted[nw] != 0.
- * qmcx_flex_complete_updates: WaveFunction::flex_completeUpdates (WaveFunction.cpp:486-499).
+ * qmcx_flex_complete_updates: WaveFunction::flex_completeUpdates (WaveFunction.cpp:486-499): the delayed rows
+ *   that flex moves left pending reach the inverse (DiracDeterminant.cpp:294-321).  With delay rank k > 1 an
+ *   accepted flex move is a pending delayed row of its walker until the walker has k of them, as in
+ *   DelayedUpdate::acceptRow (DelayedUpdate.h:151-152); calls that read the inverse (evaluate_gl, ratios,
+ *   NonLocalPP, run_sweeps, get_inverse) apply pending rows first.
  * qmcx_flex_evaluate_gl: WaveFunction::flex_evaluateGL; log_out[nw] (G, L via qmcx_get_gl). */
 int qmcx_flex_ratio_grad(qmcx_batch_t b, int iel, const double* displ, double* ratio, double* grad_now,
                          double* grad_new);

@@ -280,6 +280,11 @@ struct qmcx_batch_s {
   int W = 0, K = 1, prec = QMCX_FP64, walker0 = 0, slices = 0, nt = 0;
   int nch = 1;   // delayed-update state slots per walker and determinant: one per chunk
   int flush_kind = 0;   // 0: matrix cores when possible, 1: vector FMA, 2: matrix cores, U rows from HBM
+  // streamed form of the delayed updates (stale rows and capacitance inverse in HBM, a flush launch
+  // per chunk): needed when the LDS-resident form does not fit the delay rank (st_need), or asked
+  // for (st_ask); flex_pending: flex_* moves left delayed rows pending in state slot 0
+  bool st_need = false, st_ask = false, flex_pending = false;
+  bool streamed() const { return K > 1 && (st_need || st_ask); }
   // FP32 Metropolis: near-tie band of the FP64 re-evaluation (0: off).  While it is on the batch
   // keeps FP64 Slater rows (A64, spo64), which the re-evaluation and the FP64 refresh read;
   // rows64_stale until an evaluateLog has filled them
@@ -470,32 +475,34 @@ int proposal_slices(const qmcx_system_s* s, int want) {
   return want > 0 ? want : 1;
 }
 
-template <typename T, int JPT, bool KD, bool ARB, bool J3>
+template <typename T, int JPT, bool KD, bool ARB, bool J3, bool ST>
 int launch_moves_kd(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
   const qmcx_system_s* s = b->sys;
-  hipLaunchKernelGGL((k_moves<T, JPT, KD, ARB, J3>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(),
+  hipLaunchKernelGGL((k_moves<T, JPT, KD, ARB, J3, ST>), dim3(b->W), dim3(nt), lds, b->stream, s->params<T>(),
                      b->params<T>(), a);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
 // the near-tie FP64 re-evaluation exists for FP32 delayed updates under Metropolis acceptance and
-// a wave function without a three-body Jastrow
-template <typename T, int JPT>
+// a wave function without a three-body Jastrow; ST: the streamed form of the delayed updates
+template <typename T, int JPT, bool ST>
 int launch_moves_jpt(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
   if (b->j3)
-    return b->K == 1 ? launch_moves_kd<T, JPT, false, false, true>(b, nt, lds, a)
-                     : launch_moves_kd<T, JPT, true, false, true>(b, nt, lds, a);
-  if (b->K == 1) return launch_moves_kd<T, JPT, false, false, false>(b, nt, lds, a);
+    return b->K == 1 ? launch_moves_kd<T, JPT, false, false, true, false>(b, nt, lds, a)
+                     : launch_moves_kd<T, JPT, true, false, true, ST>(b, nt, lds, a);
+  if (b->K == 1) return launch_moves_kd<T, JPT, false, false, false, false>(b, nt, lds, a);
   if constexpr (sizeof(T) == 4)
     if (a.mode == QMCX_ACCEPT_METROPOLIS && a.band > 0.0 && a.decide == nullptr)
-      return launch_moves_kd<T, JPT, true, true, false>(b, nt, lds, a);
-  return launch_moves_kd<T, JPT, true, false, false>(b, nt, lds, a);
+      return launch_moves_kd<T, JPT, true, true, false, ST>(b, nt, lds, a);
+  return launch_moves_kd<T, JPT, true, false, false, ST>(b, nt, lds, a);
 }
 
 template <typename T>
-int launch_moves(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
-  return (b->sys->N <= nt) ? launch_moves_jpt<T, 1>(b, nt, lds, a) : launch_moves_jpt<T, 2>(b, nt, lds, a);
+int launch_moves(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a, bool st) {
+  if (st)
+    return (b->sys->N <= nt) ? launch_moves_jpt<T, 1, true>(b, nt, lds, a) : launch_moves_jpt<T, 2, true>(b, nt, lds, a);
+  return (b->sys->N <= nt) ? launch_moves_jpt<T, 1, false>(b, nt, lds, a) : launch_moves_jpt<T, 2, false>(b, nt, lds, a);
 }
 
 // SPO rows at the proposals of electrons det*n + [e0, e0 + ne) of every walker
@@ -620,15 +627,20 @@ bool use_mfma_flush(const qmcx_batch_t b) {
 // records.
 static int move_chunk(int n, int K, bool j3) { return K == 1 ? std::min(n, j3 ? 16 : 64) : K; }
 
-// updateInvMat of the delayed rows of state slots [j0, j1) of determinant det, in order
+// updateInvMat of the delayed rows of state slots [j0, j1) of determinant det, in order; thr > 1:
+// only the slots holding at least thr rows
 template <typename T>
-int launch_flush(qmcx_batch_t b, hipStream_t st, int det, int j0, int j1) {
+int launch_flush(qmcx_batch_t b, hipStream_t st, int det, int j0, int j1, int thr = 1) {
   const qmcx_system_s* s = b->sys;
-  const FlushRows FR{0, s->n};
+  const FlushRows FR{0, s->n, thr};
   if (use_mfma_flush<T>(b)) {
     const size_t lds = flush_mfma_lds(s->n, b->K);
-    hipLaunchKernelGGL(k_flush_mfma, dim3((s->n + kFlushBM - 1) / kFlushBM, b->W), dim3(64 * kFlushWaves), lds, st,
-                       s->params<float>(), b->params<float>(), det, j0, j1, FR);
+    if (flush_t_tiles(b->K) == 1)
+      hipLaunchKernelGGL(k_flush_mfma<true>, dim3((s->n + kFlushBM - 1) / kFlushBM, b->W), dim3(64 * kFlushWaves), lds,
+                         st, s->params<float>(), b->params<float>(), det, j0, j1, FR);
+    else
+      hipLaunchKernelGGL(k_flush_mfma<false>, dim3((s->n + kFlushBM - 1) / kFlushBM, b->W), dim3(64 * kFlushWaves),
+                         lds, st, s->params<float>(), b->params<float>(), det, j0, j1, FR);
     HIPCHK(hipGetLastError());
   } else {
     const size_t lds = sizeof(T) * 2 * kFlushRows * b->K + sizeof(int) * b->K;
@@ -653,8 +665,10 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
   if (choose_shape(s, nt)) return QMCX_ERR_ARG;
   const int n = s->n;
   const int chunk = move_chunk(n, b->K, b->j3);
-  const MoveLds L = move_lds<T>(s->N, s->nions, n, b->K, chunk, b->j3 ? s->mw : 0);
+  const bool st = b->streamed();
+  const MoveLds L = move_lds<T>(s->N, s->nions, n, b->K, chunk, b->j3 ? s->mw : 0, st);
   MoveArgs a;
+  std::memset(&a, 0, sizeof(a));
   a.mode = mode;
   a.slices = 0;
   a.spo = b->spo;
@@ -668,7 +682,7 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
   a.flush_u = 0;
   // one launch for all the chunks when no flush launch is needed between them: Sherman-Morrison
   // moves, or each chunk's updates applied by the move kernel itself
-  const bool inflush = b->K > 1 && use_mfma_flush<T>(b) && flush_own_fits<T>(s->N, n, b->K, chunk);
+  const bool inflush = b->K > 1 && !st && use_mfma_flush<T>(b) && flush_own_fits<T>(s->N, n, b->K, chunk);
   if (b->K == 1 || inflush) {
     a.e_begin = det * n;
     a.e_end = det * n + n;
@@ -677,7 +691,7 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
     a.inflush = inflush ? (flush ? 2 : 1) : 0;
     a.flush_u = b->flush_kind == 2 ? 0 : kFlushOwnU;
     if (b->prof_record(0)) return QMCX_ERR_HIP;
-    if (launch_moves<T>(b, nt, L.total, a)) return QMCX_ERR_HIP;
+    if (launch_moves<T>(b, nt, L.total, a, false)) return QMCX_ERR_HIP;
     if (b->prof_close()) return QMCX_ERR_HIP;
     return 0;
   }
@@ -688,7 +702,7 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
     a.e_end = det * n + std::min(n, e + chunk);
     a.chunk = ci;
     if (b->prof_record(0)) return QMCX_ERR_HIP;
-    if (launch_moves<T>(b, nt, L.total, a)) return QMCX_ERR_HIP;
+    if (launch_moves<T>(b, nt, L.total, a, st)) return QMCX_ERR_HIP;
     if (b->prof_close()) return QMCX_ERR_HIP;
     if (b->K > 1 && (ci + 1 < nchunks || flush)) {
       if (b->prof_record(1)) return QMCX_ERR_HIP;
@@ -696,6 +710,26 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
       if (b->prof_close()) return QMCX_ERR_HIP;
     }
   }
+  return 0;
+}
+
+// The delayed rows flex_* moves left pending (state slot 0 of each determinant) reach the inverse
+// (DiracDeterminant::completeUpdates, DiracDeterminant.cpp:294-321), or, with thr > 1, only those of
+// the walkers whose pending rows reached thr; the applied slots are emptied.
+template <typename T>
+int flush_pending_t(qmcx_batch_t b, int thr) {
+  for (int det = 0; det < 2; ++det) {
+    if (int rc = launch_flush<T>(b, b->stream, det, 0, 1, thr)) return rc;
+    hipLaunchKernelGGL(k_reset_delay<T>, dim3((b->W + 255) / 256), dim3(256), 0, b->stream, b->params<T>(), det, thr);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+int flush_pending(qmcx_batch_t b, int thr = 1) {
+  if (!b->flex_pending) return 0;
+  const int rc = b->prec == QMCX_FP32 ? flush_pending_t<float>(b, thr) : flush_pending_t<double>(b, thr);
+  if (rc) return rc;
+  if (thr <= 1) b->flex_pending = false;
   return 0;
 }
 
@@ -1499,8 +1533,16 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
     }
     const int chunk = move_chunk(s->n, delay_rank, s->has_j3);
     const int mw = s->has_j3 ? s->mw : 0;
-    const size_t lds = precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, chunk, mw).total
-                                              : move_lds<double>(s->N, s->nions, s->n, delay_rank, chunk, mw).total;
+    auto lds_of = [&](bool st) {
+      return precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, chunk, mw, st).total
+                                    : move_lds<double>(s->N, s->nions, s->n, delay_rank, chunk, mw, st).total;
+    };
+    size_t lds = lds_of(false);
+    if (lds > kMoveLdsMax && delay_rank > 1) {
+      // the delay rank's inverse-row cache does not fit: the streamed form of the delayed updates
+      b->st_need = true;
+      lds = lds_of(true);
+    }
     if (lds > kMoveLdsMax) {
       delete b;
       return fail(QMCX_ERR_ARG, "walker state (" + std::to_string(lds / 1024) +
@@ -1631,6 +1673,7 @@ int qmcx_batch_set_positions(qmcx_batch_t b, const double* R) {
     for (int i = 0; i < N; ++i)
       for (int d = 0; d < 3; ++d) soa[((size_t)w * 3 + d) * N + i] = R[((size_t)w * N + i) * 3 + d];
   HIPCHK(hipMemcpy(b->R, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
+  b->flex_pending = false;   // the state is rebuilt by the next evaluateLog; pending rows are void
   return 0;
 }
 
@@ -1650,6 +1693,21 @@ int qmcx_batch_set_walker_offset(qmcx_batch_t b, int walker0) {
   if (!b || walker0 < 0) return fail(QMCX_ERR_ARG, "bad argument");
   b->drop_graph();   // a captured run holds the old setting
   b->walker0 = walker0;
+  return 0;
+}
+
+int qmcx_batch_set_delay_form(qmcx_batch_t b, int form) {
+  if (!b || form < 0 || form > 1)
+    return fail(QMCX_ERR_ARG, "delay form must be 0 (automatic) or 1 (streamed)");
+  if (int rc = flush_pending(b)) return rc;
+  b->drop_graph();   // a captured run holds the old setting
+  b->st_ask = form == 1;
+  return 0;
+}
+
+int qmcx_batch_get_delay_form(qmcx_batch_t b, int* form) {
+  if (!b || !form) return fail(QMCX_ERR_ARG, "null argument");
+  *form = b->streamed() ? 1 : 0;
   return 0;
 }
 
@@ -1676,6 +1734,7 @@ int qmcx_evaluate_log(qmcx_batch_t b, double* log_out) {
   if (int rc = b->sys->sync_functors()) return rc;
   int rc = b->prec == QMCX_FP32 ? evaluate_log_impl<float>(b) : evaluate_log_impl<double>(b);
   if (rc) return rc;
+  b->flex_pending = false;   // the inverse is rebuilt from scratch: pending delayed rows are void
   b->rows64_stale = false;   // the FP64 Slater rows are filled with the SPO rows when kept
   HIPCHK(hipStreamSynchronize(b->stream));
   if (b->j3)
@@ -1708,6 +1767,7 @@ int qmcx_run_sweeps(qmcx_batch_t b, int nsweeps, uint64_t seed, double tau, int 
     return fail(QMCX_ERR_STATE, "the accept band was switched on after qmcx_evaluate_log: call it again");
   if (int rc = check_j3(b)) return rc;
   if (b->graph_on && b->prof_on) return fail(QMCX_ERR_STATE, "kernel timing and graph replay exclude each other");
+  if (int rc = flush_pending(b)) return rc;
   if (nsweeps > b->hist_cap) {
     if (b->hist) hipFree(b->hist);
     b->hist = nullptr;
@@ -1821,6 +1881,7 @@ int qmcx_get_arbitrations(qmcx_batch_t b, long long* narb) {
 
 int qmcx_get_inverse(qmcx_batch_t b, int walker, int det, double* psiM) {
   if (!b || !psiM || walker < 0 || walker >= b->W || det < 0 || det > 1) return fail(QMCX_ERR_ARG, "bad argument");
+  if (int rc = flush_pending(b)) return rc;
   HIPCHK(hipStreamSynchronize(b->stream));
   const size_t n = b->sys->n, nn = n * n, off = ((size_t)walker * 2 + det) * nn;
   if (b->prec == QMCX_FP32) {
@@ -1856,21 +1917,26 @@ static int flex_launch(qmcx_batch_t b, int iel, bool apply) {
   a.mode = QMCX_ACCEPT_METROPOLIS;
   a.slices = 0;
   a.decide = b->flex_decide;
+  // delay rank > 1: the streamed form keeps each walker's accepted rows pending in state slot 0
+  // of the determinant (DelayedUpdate::acceptRow, DelayedUpdate.h:124-153) until the walker's
+  // count reaches the delay rank (flushed after the accept) or completeUpdates
+  const bool st = b->K > 1;
+  a.resume = b->flex_pending ? 1 : 0;
   const bool trace_was = b->trace_on;
   b->trace_on = true;
   if (f32) {
-    const MoveLds L = move_lds<float>(s->N, s->nions, s->n, b->K, 1, b->j3 ? s->mw : 0);
-    rc = launch_moves<float>(b, nt, L.total, a);
+    const MoveLds L = move_lds<float>(s->N, s->nions, s->n, b->K, 1, b->j3 ? s->mw : 0, st);
+    rc = launch_moves<float>(b, nt, L.total, a, st);
   } else {
-    const MoveLds L = move_lds<double>(s->N, s->nions, s->n, b->K, 1, b->j3 ? s->mw : 0);
-    rc = launch_moves<double>(b, nt, L.total, a);
+    const MoveLds L = move_lds<double>(s->N, s->nions, s->n, b->K, 1, b->j3 ? s->mw : 0, st);
+    rc = launch_moves<double>(b, nt, L.total, a, st);
   }
   b->trace_on = trace_was;
   if (rc) return rc;
-  if (apply && b->K > 1) {
-    // every flex move is its own chunk: its (at most one) delayed row is applied now
-    rc = f32 ? launch_flush<float>(b, b->stream, a.det, 0, 1) : launch_flush<double>(b, b->stream, a.det, 0, 1);
-    if (rc) return rc;
+  if (st) {
+    b->flex_pending = true;
+    if (apply)
+      if (int rc2 = flush_pending(b, b->K)) return rc2;
   }
   return 0;
 }
@@ -1917,12 +1983,14 @@ int qmcx_flex_accept_restore(qmcx_batch_t b, int iel, const int* accepted) {
 
 int qmcx_flex_complete_updates(qmcx_batch_t b) {
   if (!b) return fail(QMCX_ERR_ARG, "null batch");
-  HIPCHK(hipStreamSynchronize(b->stream));   // flex moves leave no pending delayed rows
+  if (int rc = flush_pending(b)) return rc;
+  HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
 }
 
 int qmcx_flex_evaluate_gl(qmcx_batch_t b, double* log_out) {
   if (!b) return fail(QMCX_ERR_ARG, "null batch");
+  if (int rc = flush_pending(b)) return rc;   // evaluateGL reads the up-to-date inverse
   int rc = b->prec == QMCX_FP32 ? launch_gl<float>(b, false) : launch_gl<double>(b, false);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -1934,6 +2002,7 @@ int qmcx_flex_evaluate_ratios(qmcx_batch_t b, int iel, int nvp, const double* vp
   if (!b || !vpos || !ratios || iel < 0 || iel >= b->sys->N || nvp < 1 || nvp > kMaxVirtualPoints)
     return fail(QMCX_ERR_ARG, "bad argument");
   if (b->j3) return fail(QMCX_ERR_ARG, "virtual-move ratios with a three-body Jastrow are not built");
+  if (int rc = flush_pending(b)) return rc;   // the ratios read the up-to-date inverse rows
   if (nvp > b->vr_cap) {
     HIPCHK(hipStreamSynchronize(b->stream));
     if (b->vr_pos) hipFree(b->vr_pos);
@@ -1966,6 +2035,7 @@ int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* spheres, int m
                        int* npairs) {
   if (!b || !spheres || !(rmax > 0) || max_pairs < 1 || max_pairs > kMaxNlppPairs) return fail(QMCX_ERR_ARG, "bad argument");
   if (b->j3) return fail(QMCX_ERR_ARG, "NonLocalPP ratios with a three-body Jastrow are not built");
+  if (int rc = flush_pending(b)) return rc;   // the ratios read the up-to-date inverse rows
   const size_t WN = (size_t)b->W * b->sys->N;
   // work-list entries and knot units are int indices on the device
   if (WN * max_pairs * kNlppKnots >= (size_t)INT_MAX) return fail(QMCX_ERR_ARG, "walkers x electrons x max_pairs too large");
@@ -2107,8 +2177,12 @@ int du_flush(qmcx_du_t h) {
   if (h->count == 0) return 0;
   if (sizeof(T) == 4 && flush_mfma_fits(h->n)) {
     const size_t lds = flush_mfma_lds(h->n, h->K);
-    hipLaunchKernelGGL(k_flush_mfma, dim3((h->n + kFlushBM - 1) / kFlushBM, 1), dim3(64 * kFlushWaves), lds, h->stream,
-                       h->sys<float>(), h->params<float>(), 0, 0, 1, FlushRows{0, h->n});
+    if (flush_t_tiles(h->K) == 1)
+      hipLaunchKernelGGL(k_flush_mfma<true>, dim3((h->n + kFlushBM - 1) / kFlushBM, 1), dim3(64 * kFlushWaves), lds,
+                         h->stream, h->sys<float>(), h->params<float>(), 0, 0, 1, FlushRows{0, h->n});
+    else
+      hipLaunchKernelGGL(k_flush_mfma<false>, dim3((h->n + kFlushBM - 1) / kFlushBM, 1), dim3(64 * kFlushWaves), lds,
+                         h->stream, h->sys<float>(), h->params<float>(), 0, 0, 1, FlushRows{0, h->n});
   } else {
     const size_t lds = sizeof(T) * 2 * kFlushRows * h->K + sizeof(int) * h->K;
     hipLaunchKernelGGL(k_flush_generic<T>, dim3((h->n + kFlushRows - 1) / kFlushRows, 1), dim3(256), lds, h->stream,
@@ -2295,7 +2369,7 @@ int qmcx_group_create(qmcx_group_t* out, const qmcx_batch_t* batches, int nbatch
     else if (b->j3) why = "no three-body Jastrow";
     else if (b->band > 0.0) why = "no near-tie band";
     else if (b->rows64 != batches[0]->rows64) why = "all batches with FP64 Slater rows or none";
-    else if (!use_mfma_flush<float>(b) || !flush_own_fits<float>(s->N, s->n, b->K, b->K))
+    else if (b->streamed() || !use_mfma_flush<float>(b) || !flush_own_fits<float>(s->N, s->n, b->K, b->K))
       why = "each batch's delayed updates must be applied inside the move kernel (n a multiple of 16, delay <= 32)";
     else if (b->sys->dev != batches[0]->sys->dev) why = "batches of one device";
     for (int j = 0; j < i && why.empty(); ++j)
@@ -2383,6 +2457,7 @@ int qmcx_group_run_sweeps(qmcx_group_t g, int nsweeps, uint64_t seed, double tau
     if (b->rows64 && b->rows64_stale)
       return fail(QMCX_ERR_STATE, "FP64 Slater rows were switched on after qmcx_evaluate_log: call it again");
     if (int rc = check_j3(b)) return rc;
+    if (int rc = flush_pending(b)) return rc;
     if (nsweeps > b->hist_cap) {
       if (b->hist) hipFree(b->hist);
       b->hist = nullptr;

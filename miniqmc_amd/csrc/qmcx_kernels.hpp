@@ -103,6 +103,9 @@ struct MoveArgs {
   // flush takes the new Slater rows from LDS when at most flush_u moves of the chunk were
   // accepted, else from HBM.
   int chunk, chunk_len, nchunks, inflush, flush_u;
+  // streamed delayed updates: the launch continues from the delayed rows pending in its state slot
+  // (flex_* calls keep them pending until the delay rank is reached or completeUpdates)
+  int resume;
   const int* decide;     // [W] caller accept decisions (flex_acceptrestoreMove) or null
   // FP32 form, Metropolis: a move whose uniform u lies within band * ratio^2 of ratio^2 is
   // decided on its FP64 ratio (0: off)
@@ -341,19 +344,31 @@ namespace qmcx {
 //   T  = Ainv U^T - E          (U rows = new Slater rows, E_{m,d} = [m == rows_d])
 //   T' = T Binv
 //   Ainv -= T' V               (V rows = stale inverse rows)
-// A launch covers rows [lo, hi).
+// A launch covers rows [lo, hi) and applies the slots holding at least thr delayed rows (thr > 1:
+// the walkers whose pending rows reached the delay rank, DelayedUpdate.h:151-152).
 struct FlushRows {
   int lo, hi;
+  int thr = 1;
   __device__ bool writes(int m) const { return m < hi; }
 };
 constexpr int kFlushRows = 16;
+// After a flush launch over state slot 0 of determinant det with threshold thr: the applied slots
+// hold no delayed rows (a thread per walker).
+template <typename T>
+__global__ void k_reset_delay(BatchP<T> B, int det, int thr) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= B.W) return;
+  int* nd = B.ndelay + state_slot(B, w, det, 0);
+  const int c = *nd;
+  if (c > 0 && c >= thr) *nd = 0;
+}
 // one state slot j per launch
 template <typename T>
 __global__ __launch_bounds__(256) void k_flush_generic(SysP<T> S, BatchP<T> B, int det, int j, FlushRows FR) {
   const int w = blockIdx.y, m0 = FR.lo + blockIdx.x * kFlushRows;
   const size_t sl = state_slot(B, w, det, j);
   const int c = B.ndelay[sl];
-  if (c == 0) return;
+  if (c == 0 || c < FR.thr) return;
   const int n = S.n, K = B.K, tid = threadIdx.x, nt = blockDim.x;
   extern __shared__ __align__(16) unsigned char smem[];
   T* sT = (T*)smem;                     // [FB][K]
@@ -423,16 +438,23 @@ constexpr int kFlushBinvLds = 64;   // Binv staged in LDS up to this delay rank
 constexpr int kFlushVLds = 32;      // V and U staged in LDS up to this delay rank
 __host__ __device__ inline bool flush_mfma_fits(int n) { return n % 16 == 0 && n <= 16 * kFlushTiles; }
 __host__ __device__ inline int flush_v_stride(int n) { return n + 16; }   // LDS row stride of V
-// LDS of k_flush_mfma: per wave T and T2 (16 x cp floats each), Binv (K x K doubles when
-// K <= kFlushBinvLds), V and U (cp x (n + 16) floats each when K <= kFlushVLds), the row list
+// LDS of k_flush_mfma: per wave T and T2 (16 x cp floats each; past kFlushBinvLds T2 is written
+// over T), Binv (K x K doubles when K <= kFlushBinvLds), V and U (cp x (n + 16) floats each when
+// K <= kFlushVLds), the row list
+__host__ __device__ inline int flush_t_tiles(int K) { return K <= kFlushBinvLds ? 2 : 1; }
 __host__ __device__ inline size_t flush_mfma_lds(int n, int K) {
   const int cp = (K + 15) & ~15;
   const size_t kb = K <= kFlushBinvLds ? (size_t)(K * K + 1) & ~(size_t)1 : 0;   // keeps V 16-byte aligned
   const size_t vb = K <= kFlushVLds ? (size_t)2 * cp * flush_v_stride(n) : 0;
-  return sizeof(float) * kFlushWaves * 2 * 16 * cp + sizeof(double) * kb + sizeof(float) * vb + sizeof(int) * K;
+  return sizeof(float) * kFlushWaves * flush_t_tiles(K) * 16 * cp + sizeof(double) * kb + sizeof(float) * vb +
+         sizeof(int) * K;
 }
+// the largest delay rank whose T2 is held in registers while it is written over T
+constexpr int kFlushT2Regs = 128;
 
 // State slots [j0, j1) of determinant det are applied in order to the rows held in registers.
+// OVER (delay ranks past kFlushBinvLds): T2 is written over T.
+template <bool OVER>
 __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, BatchP<float> B, int det, int j0,
                                                                   int j1, FlushRows FR) {
   const int w = blockIdx.y;
@@ -441,12 +463,20 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
   const int nct = n / 16;                  // column tiles (n % 16 == 0, nct <= kFlushTiles)
   const int r0 = FR.lo + blockIdx.x * kFlushBM + wid * 16;   // this wave's first row
   const int KC = (K + 15) & ~15;           // LDS sizing uses the delay rank, not c
+  // nothing to apply for this walker: no Ainv traffic
+  bool any = false;
+  for (int j = j0; j < j1; ++j) {
+    const int cj = B.ndelay[state_slot(B, w, det, j)];
+    any = any || (cj > 0 && cj >= FR.thr);
+  }
+  if (!any) return;
   extern __shared__ __align__(16) unsigned char smem[];
-  float* sT = (float*)smem + wid * 2 * 16 * KC;       // [16][cp]   T = Ainv U^T - E
-  float* sT2 = sT + 16 * KC;                           // [16][cp]  -T Binv
+  const int tt = OVER ? 1 : 2;
+  float* sT = (float*)smem + wid * tt * 16 * KC;      // [16][cp]   T = Ainv U^T - E
+  float* sT2 = sT + (tt - 1) * 16 * KC;                // [16][cp]  -T Binv (over T when tt = 1)
   const bool binv_lds = K <= kFlushBinvLds;
   const bool v_lds = K <= kFlushVLds;
-  double* sB = (double*)((float*)smem + kFlushWaves * 2 * 16 * KC);        // [c][c]
+  double* sB = (double*)((float*)smem + kFlushWaves * tt * 16 * KC);       // [c][c]
   float* sV = (float*)(sB + (binv_lds ? (K * K + 1) & ~1 : 0));            // [cp][n + 16]
   float* sU = sV + (v_lds ? (size_t)KC * flush_v_stride(n) : 0);           // [cp][n + 16]
   int* sRows = (int*)(sU + (v_lds ? (size_t)KC * flush_v_stride(n) : 0));
@@ -482,7 +512,7 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
   for (int j = j0; j < j1; ++j) {
     const size_t sl = state_slot(B, w, det, j);
     const int c = B.ndelay[sl];
-    if (c == 0) continue;
+    if (c == 0 || c < FR.thr) continue;
     changed = true;
     const int cp = (c + 15) & ~15;         // delayed rows padded to a 16-wide tile
     const float* Vb = B.Vbuf + sl * K * n;
@@ -543,6 +573,30 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
     wave_lds_fence();   // T and T2 are this wave's own
     FL_TICK(26)
     // ---- T2 = -(T Binv), FP64 products of the small capacitance inverse
+    if constexpr (OVER) {
+      // over T: every lane's entries into registers first (cp <= kFlushT2Regs)
+      float t2v[kFlushT2Regs / 4];
+#pragma unroll
+      for (int q = 0; q < kFlushT2Regs / 4; ++q) {
+        const int idx = lane + 64 * q;
+        double sum = 0;
+        if (idx < 16 * cp) {
+          const int i = idx / cp, dd = idx - i * cp;
+          if (dd < c)
+            for (int e = 0; e < c; ++e) sum += (double)sT[i * KC + e] * Bi[e * K + dd];
+        }
+        t2v[q] = (float)(-sum);
+      }
+      wave_lds_fence();
+#pragma unroll
+      for (int q = 0; q < kFlushT2Regs / 4; ++q) {
+        const int idx = lane + 64 * q;
+        if (idx < 16 * cp) {
+          const int i = idx / cp, dd = idx - i * cp;
+          sT[i * KC + dd] = t2v[q];
+        }
+      }
+    } else
     for (int idx = lane; idx < 16 * cp; idx += 64) {
       const int i = idx / cp, dd = idx - i * cp;
       double sum = 0;

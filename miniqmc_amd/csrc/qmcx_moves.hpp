@@ -33,7 +33,8 @@ __host__ __device__ inline size_t flush_own_lds(int n, int K) {
 
 struct MoveLds {
   unsigned oR, oRed, oInv, oPsi, oBinv, oP, oPart, oRows, oAl, oJ1n, oFn, oCell, oProp, oJ1o, oJt, oJ2, oAc, oQ, oAcc,
-      oArb, oJ3, oJ3m, oJ3np, oJ3ne, oJ3k, oJ3v, oJ3r, total;
+      oArb, oJ3, oJ3m, oJ3np, oJ3ne, oJ3k, oJ3v, oJ3r, oSt, total;
+  bool binv_lds;   // streamed form: the capacitance inverse is in LDS (else in its HBM state slot)
 };
 // per-wave pair slots of one J3 move phase: codes [waves][kJ3Slots] and k-side terms
 // [waves][5][kJ3Slots]
@@ -53,20 +54,23 @@ __host__ __device__ inline unsigned lds_take(unsigned& o, size_t bytes) {
 // E = electrons per chunk; the inverse-row cache is used when K > 1 (M, the ion count, is
 // not needed: the J1 terms come with the proposals).  mw > 0: a three-body Jastrow with ion
 // masks of mw words (its sums, masks and the chunk's near-ion records stay for the launch; its
-// pair slots share the region past the J2 pair terms)
+// pair slots share the region past the J2 pair terms).  st: the streamed form of the delayed
+// updates (no inverse-row cache, no products of accepted rows with later rows; the capacitance
+// inverse in LDS with row stride K when binv_lds, else in HBM; p, y and reduction partials in oSt)
 template <typename T>
-__host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E, int mw = 0) {
-  (void)M;
+__host__ __device__ inline MoveLds move_lds_at(int N, int n, int K, int E, int mw, bool st, bool binv_lds) {
   MoveLds L;
   unsigned o = 0;
-  const bool cache = K > 1;
+  const bool cache = K > 1 && !st;
+  L.binv_lds = st && binv_lds;
   L.oR = lds_take(o, sizeof(double) * 3 * N);
   L.oRed = lds_take(o, sizeof(double) * kRed * 16);
   L.oInv = lds_take(o, sizeof(T) * n);
   L.oPsi = lds_take(o, sizeof(T) * 5 * n);
-  L.oBinv = lds_take(o, cache ? sizeof(double) * K * (K + 1) : 0);   // row stride K + 1
+  L.oBinv = lds_take(o, cache ? sizeof(double) * K * (K + 1) : (L.binv_lds ? sizeof(double) * K * K : 0));
+  L.oSt = lds_take(o, st ? sizeof(double) * (2 * (size_t)K + kMoveThreads) : 0);
   L.oP = lds_take(o, sizeof(double) * (K + 4));
-  L.oPart = lds_take(o, cache ? 0 : sizeof(double) * n);   // Sherman-Morrison scratch
+  L.oPart = lds_take(o, K > 1 ? 0 : sizeof(double) * n);   // Sherman-Morrison scratch
   L.oRows = lds_take(o, sizeof(int) * K);
   L.oAl = lds_take(o, sizeof(int) * E);                    // accepted moves of the chunk
   L.oJ1n = lds_take(o, sizeof(T) * 5 * E);                 // their new J1 state
@@ -99,11 +103,19 @@ __host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E, i
   if (cache && sizeof(T) == 4) {
     const size_t need = L.oQ + flush_own_lds(n, K);
     if (need > o && need <= kMoveLdsMax) o = (unsigned)need;
+  }
+  if (K > 1 && sizeof(T) == 4) {
     const size_t arb = L.oArb + arb_lds(n, K);
     if (arb > o) o = (unsigned)arb;
   }
   L.total = o;
   return L;
+}
+template <typename T>
+__host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E, int mw = 0, bool st = false) {
+  (void)M;
+  const MoveLds L = move_lds_at<T>(N, n, K, E, mw, st, st);
+  return st && L.total > kMoveLdsMax ? move_lds_at<T>(N, n, K, E, mw, st, false) : L;
 }
 
 // Sum the wave partials red[v*16 + w] (w < nw); every lane of every wave gets
@@ -604,16 +616,24 @@ __device__ inline void j3_apply(T* sJ3, int N, const int* sk, const T* sv, int n
 // ARB: the FP64 re-evaluation of near-tie Metropolis decisions is compiled in (FP32, K > 1); only
 // that instantiation carries its code, whose registers would otherwise spill in every move.
 // J3: the three-body Jastrow is compiled in (its ratio, gradient and update in every move).
+// ST (with KD): the streamed form of the delayed updates, for delay ranks whose inverse-row cache
+// does not fit the LDS and for delayed rows kept pending across launches (a.resume): the stale
+// inverse rows are read from Ainv in HBM (unchanged until the flush launch), p_d = U_d . Ainv[row]
+// is formed at each move (DelayedUpdate.h:108-113, U_d the new Slater row of delayed row d), and
+// the capacitance inverse, the row list and the count live in the chunk's state slot (Binv in LDS
+// while the chunk runs when it fits); each accepted row's stale inverse row goes to the slot's V
+// rows at once.  A flush launch applies the slot.
 // The workgroup moves walker w of batch (S, B) as directed by a.
-template <typename T, int JPT, bool KD, bool ARB, bool J3>
+template <typename T, int JPT, bool KD, bool ARB, bool J3, bool ST = false>
 __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B, const MoveArgs& a, const int w) {
+  static_assert(!ST || KD, "the streamed form is a form of the delayed updates");
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
   const int N = S.N, n = S.n, K = KD ? B.K : 1;
   const int KP = K + 1;   // LDS row stride of Binv: a column walk down its rows is free of bank conflicts
   const int Emax = a.chunk_len;   // electrons per chunk (the LDS layout is sized for it)
-  const MoveLds L = move_lds<T>(N, S.M, n, K, Emax, J3 ? S.mw : 0);
+  const MoveLds L = move_lds<T>(N, S.M, n, K, Emax, J3 ? S.mw : 0, ST);
   double* sR = (double*)(smem + L.oR);
   double* sRed = (double*)(smem + L.oRed);
   T* sInv = (T*)(smem + L.oInv);
@@ -633,6 +653,10 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
   T* sAc = (T*)(smem + L.oAc);
   const int sa = ac_stride<T>(n);   // its row stride
   double* sQ = (double*)(smem + L.oQ);
+  // streamed form: p_d of the move [K], y = Binv^T p [K], reduction partials [kMoveThreads]
+  double* sSp = (double*)(smem + L.oSt);
+  double* sSy = sSp + K;
+  double* sSg = sSy + K;
   // three-body Jastrow: sums [5][N], ion masks [N][mw], near-ion records of the chunk's
   // proposals and current positions [E][kJ3Rec], this wave's pair slots of the new and the old
   // phase, the j-side wave partials [16][5]
@@ -713,7 +737,7 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
   // round of loads)
   {
     const unsigned je = (unsigned)(sizeof(T) * E);
-    const LdsCopy cp[6] = {{sAc, Ainv + (size_t)r0 * n, KD ? (unsigned)(sizeof(T) * E * n) : 0u,
+    const LdsCopy cp[6] = {{sAc, Ainv + (size_t)r0 * n, KD && !ST ? (unsigned)(sizeof(T) * E * n) : 0u,
                             (unsigned)(sizeof(T) * n), (unsigned)(sizeof(T) * sa)},
                            {sProp, prop, (unsigned)(sizeof(double) * kProp * E)},
                            {sJ1o, J1 + cb, je},
@@ -734,6 +758,23 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
   QMCX_TICK(16)
   QMCX_TICK(17)
   int c = 0, ca = 0;   // delayed rows; accepted moves of the chunk
+  // streamed form: the slot's capacitance inverse (row stride K, in LDS while the chunk runs when
+  // it fits) and the c0 delayed rows pending from earlier launches (a.resume)
+  double* const gBinv = B.Binv + slot * K * K;
+  double* const binv = ST && L.binv_lds ? sBinv : gBinv;
+  int c0 = 0;
+  if constexpr (ST) {
+    if (a.resume) {
+      c0 = B.ndelay[slot];
+      for (int i = tid; i < c0; i += nt) sRows[i] = B.rows[slot * K + i];
+      if (L.binv_lds)
+        for (int i = tid; i < c0 * c0; i += nt) {
+          const int s2 = i / c0, r = i - s2 * c0;
+          sBinv[(size_t)s2 * K + r] = gBinv[(size_t)s2 * K + r];
+        }
+    }
+    c = c0;
+  }
   __syncthreads();
   // Rows a move reads from HBM that no earlier move of the chunk changes: the moving electron's
   // dpsiM row and the SPO row of its proposal.  Move e+1's are loaded at the start of move e
@@ -779,7 +820,33 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
     T inv_sm = 0;
     if constexpr (!KD) inv_sm = Ainv[(size_t)row * n + jr];   // changed by the previous accept
     const T* prow = spo + (size_t)e * 5 * S.npad;
-    if (KD && c > 0) {
+    if constexpr (ST) {
+      if (c > 0) {
+        // p_d = U_d . Ainv[row] for the delayed rows, a wave per row (U_d: the chunk's proposal
+        // row, or for a row pending from an earlier launch the Slater row written there)
+        const T* ar = Ainv + (size_t)row * n;
+        for (int d = wid; d < c; d += nw) {
+          const T* u = d < c0 ? Amat + (size_t)sRows[d] * n : spo + (size_t)(sRows[d] - r0) * 5 * S.npad;
+          double s = 0;
+          for (int j = lane; j < n; j += 64) s += (double)u[j] * (double)ar[j];
+          s = wave_sum(s);
+          if (lane == 0) sSp[d] = s;
+        }
+        lds_sync();
+        // y_r = sum_q Binv[q][r] p_q: G groups of 128 lanes take every G-th q
+        const int G = nt >> 7, r = tid & 127, gq = tid >> 7;
+        double s = 0;
+        if (r < c)
+          for (int q = gq; q < c; q += G) s += binv[(size_t)q * K + r] * sSp[q];
+        sSg[gq * 128 + r] = s;
+        lds_sync();
+        if (tid < c) {
+          double y = 0;
+          for (int g = 0; g < G; ++g) y += sSg[g * 128 + tid];
+          sSy[tid] = y;
+        }
+      }
+    } else if (KD && c > 0) {
       // y = Binv^T p with p_d = U_d . Ainv[row] formed when row d was accepted; parked in row c of Binv
       for (int r = tid; r < c; r += nt) {
         double s = 0;
@@ -807,7 +874,20 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
     T vinv = 0;
     if (tid < n) {
       double v;
-      if (KD) {
+      if constexpr (ST) {
+        // the stale row minus the delayed rows' terms, eight HBM rows in flight
+        const T* ac = Ainv + tid;
+        v = (double)ac[(size_t)row * n];
+        int d = 0;
+        for (; d + 8 <= c; d += 8) {
+          T a8[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) a8[k] = ac[(size_t)sRows[d + k] * n];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) v -= (double)a8[k] * sSy[d + k];
+        }
+        for (; d < c; ++d) v -= (double)ac[(size_t)sRows[d] * n] * sSy[d];
+      } else if (KD) {
         v = (double)sAc[(size_t)e * sa + tid];
         const double* y = sBinv + c * KP;
         // four delayed rows' LDS reads in flight at a time; subtraction order unchanged
@@ -841,9 +921,11 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
     }
     for (int j = tid + nt; j < n; j += nt) {
       // only when n exceeds the workgroup (not the case for the shipped systems)
-      double v = KD ? (double)sAc[(size_t)e * sa + j] : (double)Ainv[(size_t)row * n + j];
+      double v = KD && !ST ? (double)sAc[(size_t)e * sa + j] : (double)Ainv[(size_t)row * n + j];
       if (KD)
-        for (int d = 0; d < c; ++d) v -= (double)sAc[(size_t)(sRows[d] - r0) * sa + j] * sBinv[c * KP + d];
+        for (int d = 0; d < c; ++d)
+          v -= ST ? (double)Ainv[(size_t)sRows[d] * n + j] * sSy[d]
+                  : (double)sAc[(size_t)(sRows[d] - r0) * sa + j] * sBinv[c * KP + d];
       sInv[j] = (T)v;
       const T* dp = dpsiM + (size_t)row * 3 * n;
       gd0 += v * (double)dp[j];
@@ -931,14 +1013,16 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
     bool accept = a.mode == 0 ? (uacc < 0.5) : (uacc < ratio * ratio);
     if constexpr (ARB && sizeof(T) == 4 && KD) {
       // a uniform within the band around ratio^2: the decision is taken on the FP64 ratio
-      if (arb_on && fabs(uacc - ratio * ratio) <= a.band * ratio * ratio) {
+      if (arb_on && c0 == 0 && fabs(uacc - ratio * ratio) <= a.band * ratio * ratio) {
         double* sv = sArbX + n;
         double* tv = sv + n;
         double* ared = tv + n;
         const size_t pro = ((size_t)w * 2 + a.det) * n + r0;   // the chunk's first proposal
-        rdet = arb_det_ratio(n, S.npad, row, r0, c, KP, sa, (const float*)Ainv, B.A64 + moff,
-                             B.spo64 + pro * S.npad, (const float*)spo, (const float*)sAc, sBinv, sRows, sAl,
-                             B.spo64 + (pro + e) * S.npad, sArbX, sv, tv, ared, K);
+        // the stale rows: the LDS cache, or (streamed) Ainv's rows of the chunk in HBM
+        rdet = arb_det_ratio(n, S.npad, row, r0, c, ST ? K : KP, ST ? n : sa, (const float*)Ainv, B.A64 + moff,
+                             B.spo64 + pro * S.npad, (const float*)spo,
+                             ST ? (const float*)Ainv + (size_t)r0 * n : (const float*)sAc, ST ? binv : sBinv, sRows,
+                             sAl, B.spo64 + (pro + e) * S.npad, sArbX, sv, tv, ared, K);
         ratio = rdet * arb_jastrow_ratio<T>(S, sR, iel, x0, y0, z0, nx, ny, nz, ared);
         accept = uacc < ratio * ratio;
         if (tid == 0) sAcc[aNarb] += 1.0;
@@ -1013,8 +1097,10 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
         // chunk's later rows r', U_c . Ainv[r'] used by their getInvRow; four dot products per
         // wave share the psiV loads.  Vector t2: delayed row t2 (t2 < c), this row (t2 == c),
         // the chunk's row e + t2 - c (t2 > c).
-        const int nd = c + 1 + (E - 1 - e);
+        // Streamed: p_d only (V_d the stale rows in HBM; this move's is row `row`).
+        const int nd = ST ? c + 1 : c + 1 + (E - 1 - e);
         auto dvec = [&](int t2) -> const T* {
+          if constexpr (ST) return Ainv + (size_t)(t2 < c ? sRows[t2] : row) * n;
           const int ee = t2 < c ? sRows[t2] - r0 : e + (t2 - c);
           return sAc + (size_t)ee * sa;
         };
@@ -1060,7 +1146,35 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
         }
         lds_sync();
         QMCX_TICK(13)
-        if (wid == nw - 1) {
+        if constexpr (ST) {
+          // the capacitance inverse grows by one row and column with every wave: acc_s = Binv[s] . p
+          // (s < c) and y . p (s = c), a wave per row; then Binv[s][r] -= y_r yi acc_s,
+          // Binv[s][c] = yi acc_s, Binv[c][r] = -yi y_r, Binv[c][c] = yi, yi = 1 / (y . p - p_c)
+          for (int s2 = wid; s2 <= c; s2 += nw) {
+            const double* br = s2 < c ? binv + (size_t)s2 * K : sSy;
+            double t = 0;
+            for (int r = lane; r < c; r += 64) t += br[r] * sP[r];
+            t = wave_sum(t);
+            if (lane == 0) sSg[s2] = t;
+          }
+          // the move's stale inverse row becomes the slot's V row c
+          for (int j = tid; j < n; j += nt) Vb[(size_t)c * n + j] = Ainv[(size_t)row * n + j];
+          __syncthreads();
+          const double yi = 1.0 / (sSg[c] - sP[c]);
+          for (int i = tid; i < c * c; i += nt) {
+            const int s2 = i / c, r = i - s2 * c;
+            binv[(size_t)s2 * K + r] -= sSy[r] * (yi * sSg[s2]);
+          }
+          for (int i = tid; i < c; i += nt) {
+            binv[(size_t)i * K + c] = yi * sSg[i];
+            binv[(size_t)c * K + i] = -sSy[i] * yi;
+          }
+          if (tid == 0) {
+            binv[(size_t)c * K + c] = yi;
+            sRows[c] = row;
+          }
+          __syncthreads();
+        } else if (wid == nw - 1) {
           // the capacitance inverse grows by one row and column (FP64, the last wave, which
           // has the least J2 work): z = row c (parked by phase 1), y = z.p - p_c,
           // Binv[s][c] = (Binv[s].p) / y, Binv[s][r] -= z_r Binv[s][c], row c = -z / y
@@ -1165,7 +1279,7 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
     J1[3 * N + ie] = sJ1n[d * 5 + 3];
     J1[4 * N + ie] = sJ1n[d * 5 + 4];
   }
-  if (KD && !a.inflush)
+  if (KD && !ST && !a.inflush)
     for (int i = tid; i < ca * n; i += nt) {
       const int d = i / n, j = i - d * n;
       Vb[(size_t)d * n + j] = sAc[(size_t)sAl[d] * sa + j];
@@ -1237,10 +1351,15 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
   }
   // the chunk's delayed-update state for a flush launch (none when the kernel applies it)
   if (tid == 0) B.ndelay[slot] = KD && !a.inflush ? c : 0;
-  if (KD && !a.inflush) {
+  if (KD && !ST && !a.inflush)
     for (int i = tid; i < c * K; i += nt) B.Binv[slot * K * K + i] = sBinv[(i / K) * KP + i % K];
+  if (ST && L.binv_lds)
+    for (int i = tid; i < c * c; i += nt) {
+      const int s2 = i / c, r = i - s2 * c;
+      gBinv[(size_t)s2 * K + r] = sBinv[(size_t)s2 * K + r];
+    }
+  if (KD && !a.inflush)
     for (int i = tid; i < c; i += nt) B.rows[slot * K + i] = sRows[i];
-  }
   QMCX_TICK(22)
   if constexpr (KD && sizeof(T) == 4) {
     if (a.inflush && c > 0 && (ci + 1 < a.nchunks || a.inflush == 2)) {
@@ -1280,9 +1399,9 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
   QMCX_TICK_FLUSH()
 }
 
-template <typename T, int JPT, bool KD, bool ARB, bool J3>
+template <typename T, int JPT, bool KD, bool ARB, bool J3, bool ST>
 __global__ __launch_bounds__(kMoveThreads) void k_moves(SysP<T> S, BatchP<T> B, MoveArgs a) {
-  moves_body<T, JPT, KD, ARB, J3>(S, B, a, blockIdx.x);
+  moves_body<T, JPT, KD, ARB, J3, ST>(S, B, a, blockIdx.x);
 }
 
 // ---- walkers of several batches (systems of different sizes) in one launch: batch g owns the

@@ -52,6 +52,8 @@ _SIGS = {
     "qmcx_batch_set_walker_offset": [_vp, _i],
     "qmcx_batch_set_tile": [_vp, _i],
     "qmcx_batch_set_flush": [_vp, _i],
+    "qmcx_batch_set_delay_form": [_vp, _i],
+    "qmcx_batch_get_delay_form": [_vp, C.POINTER(C.c_int)],
     "qmcx_batch_set_graph": [_vp, _i],
     "qmcx_evaluate_log": [_vp, _vp],
     "qmcx_run_sweeps": [_vp, _i, C.c_uint64, C.c_double, _i, _i, _vp],
@@ -268,6 +270,15 @@ class WalkerBatch:
     def set_flush(self, kind):
         """0: matrix-core (MFMA) flush when possible, 1: vector-FMA flush."""
         _check(load().qmcx_batch_set_flush(self.h, kind))
+
+    def set_delay_form(self, form):
+        """0: automatic (LDS-resident delayed rows when they fit), 1: the streamed form."""
+        _check(load().qmcx_batch_set_delay_form(self.h, form))
+
+    def delay_form(self):
+        f = C.c_int(0)
+        _check(load().qmcx_batch_get_delay_form(self.h, C.byref(f)))
+        return f.value
 
     def set_positions(self, R):
         R = _c(R).reshape(self.nw, self.N, 3)

@@ -23,8 +23,13 @@ def _say(*a):
     print(*a, flush=True)
 
 
-@pytest.mark.parametrize("prec,delay,tol", [(qmcx.FP64, 1, 1e-10), (qmcx.FP64, 4, 1e-10), (qmcx.FP32, 4, 2e-3)])
+@pytest.mark.parametrize("prec,delay,tol", [(qmcx.FP64, 1, 1e-10), (qmcx.FP64, 4, 1e-10), (qmcx.FP64, 48, 1e-10),
+                                            (qmcx.FP32, 4, 2e-3), (qmcx.FP32, 16, 2e-3)])
 def test_flex_loop_vs_oracle(cubic, prec, delay, tol):
+    """With delay rank k > 1 an accepted flex move stays a pending delayed row of its walker until the
+    walker has k of them (DelayedUpdate.h:151-152) or flex_completeUpdates; later ratios read the
+    pending rows through getInvRow (k = 48: nothing is flushed before completeUpdates in the first
+    determinant's 48 moves)."""
     _say("flex loop", prec, delay)
     nw, seed, tau = 3, 11, 0.5
     R0 = systems.initial_positions(cubic, nw, seed=seed)

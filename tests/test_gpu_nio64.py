@@ -70,7 +70,7 @@ def _gpu_sweep(dev, R0, prec, delay, mode, recompute):
     hist = wb.run(1, SEED, TAU, mode, recompute_every=recompute)
     G, L, lv = wb.gl()
     return dict(log0=log0, trace=wb.trace(), log=hist[0], G=G, L=L, R=wb.positions(), acc=wb.accepts(),
-                arb=wb.arbitrations())
+                arb=wb.arbitrations(), form=wb.delay_form())
 
 
 @pytest.mark.parametrize("mode", [qmcx.ACCEPT_REFERENCE, qmcx.ACCEPT_METROPOLIS])
@@ -102,8 +102,12 @@ def test_nio64_fp64_delay1_vs_oracle(nio64, oracle_runs, mode):
     assert np.array_equal(got["acc"], ref["naccept"])
 
 
+@pytest.mark.parametrize("delay", [32, 64, 128])
 @pytest.mark.parametrize("mode", [qmcx.ACCEPT_REFERENCE, qmcx.ACCEPT_METROPOLIS])
-def test_nio64_fp32_delay32_vs_oracle(nio64, oracle_runs, mode):
+def test_nio64_fp32_delayed_vs_oracle(nio64, oracle_runs, mode, delay):
+    """Delay ranks 32 (stale rows in LDS, the flush inside the move kernel) and 64, 128 (the streamed
+    form: stale rows and capacitance inverse in HBM, a flush launch per chunk), the ranks the autotuner
+    chooses from."""
     sysd, dev, _ = nio64
     R0, runs = oracle_runs
     ref = runs[mode]
@@ -115,8 +119,9 @@ def test_nio64_fp32_delay32_vs_oracle(nio64, oracle_runs, mode):
     smax = _running_sensitivity(sens, n)
     bound = 4 * EPS32 * smax
     for recompute in (0, 1):
-        _say("NiO-64 FP32 delay 32, accept mode", mode, "refresh after the sweep", bool(recompute))
-        got = _gpu_sweep(dev, R0, qmcx.FP32, 32, mode, recompute)
+        _say("NiO-64 FP32 delay", delay, "accept mode", mode, "refresh after the sweep", bool(recompute))
+        got = _gpu_sweep(dev, R0, qmcx.FP32, delay, mode, recompute)
+        assert got["form"] == (0 if delay <= 32 else 1)
         tr = got["trace"]
         assert np.array_equal(tr[:, :, 2], rt[:, :, 2]), "accept/reject sequence differs"
         rel = np.abs(tr[:, :, 0] - rt[:, :, 0]) / np.abs(rt[:, :, 0])

@@ -93,11 +93,15 @@ def test_invert_transpose_batched_vs_oracle(n):
 
 
 def _compare_run(sysd, dev, orc, nw, nsweeps, delay, prec, mode, tau, tol_ratio, tol_log, seed=11, flush_kind=0,
-                 tile=0):
+                 tile=0, delay_form=0, band=0.0):
     R0 = systems.initial_positions(sysd, nw, seed=seed)
     ref = orc.run_vmc(R0, nsweeps, seed, tau, mode=mode, delay=delay, trace=True)
     wb = qmcx.WalkerBatch(dev, nw, delay_rank=delay, precision=prec, tile=tile)
     wb.set_flush(flush_kind)
+    wb.set_delay_form(delay_form)
+    assert wb.delay_form() == delay_form
+    if band > 0.0:
+        wb.set_accept_band(band)
     wb.set_positions(R0)
     log0 = wb.evaluate_log()
     np.testing.assert_allclose(log0, ref["log_init"], rtol=tol_log)
@@ -167,6 +171,30 @@ def test_sweep_fp32_vs_fp64_oracle(cubic, cubic_dev, cubic_orc, delay, kind):
     out = _compare_run(cubic, cubic_dev, cubic_orc, nw=3, nsweeps=2, delay=delay, prec=qmcx.FP32,
                        mode=qmcx.ACCEPT_METROPOLIS, tau=0.5, tol_ratio=2e-3, tol_log=1e-6, flush_kind=kind)
     assert out["flips"] == 0
+
+
+@pytest.mark.parametrize("prec,delay,kind,band", [(qmcx.FP64, 4, 0, 0.0), (qmcx.FP64, 16, 0, 0.0),
+                                                  (qmcx.FP32, 8, 0, 0.0), (qmcx.FP32, 32, 0, 1e-2),
+                                                  (qmcx.FP32, 48, 1, 0.0)])
+def test_sweep_streamed_vs_oracle(cubic, cubic_dev, cubic_orc, prec, delay, kind, band):
+    """The streamed form of the delayed updates (the form of delay ranks whose stale rows do not fit
+    the LDS): stale inverse rows read from HBM, p = U . Ainv[row] formed at each move, the capacitance
+    inverse in the chunk's state slot, a flush launch per chunk (matrix cores, or vector FMA with kind
+    1).  The same bar as the LDS-resident form: FP64 ratios within 1e-9 and log|Psi| within 1e-11 of
+    the oracle with identical accept sequences, positions, G and L; FP32 ratios within 2e-3 and
+    log|Psi| within 1e-6, identical accept sequences (delay 32 with the near-tie band)."""
+    _say("sweep streamed form, precision", prec, "delay", delay, "flush kind", kind, "band", band)
+    fp64 = prec == qmcx.FP64
+    out = _compare_run(cubic, cubic_dev, cubic_orc, nw=3, nsweeps=2, delay=delay, prec=prec,
+                       mode=qmcx.ACCEPT_METROPOLIS, tau=0.5, tol_ratio=1e-9 if fp64 else 2e-3,
+                       tol_log=1e-11 if fp64 else 1e-6, flush_kind=kind, delay_form=1, band=band)
+    assert out["flips"] == 0
+    ref = out["ref"]
+    assert np.array_equal(out["acc"], ref["naccept"])
+    if fp64:
+        np.testing.assert_allclose(out["R"], ref["R"], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(out["G"], ref["G"], rtol=1e-8, atol=1e-8)
+        np.testing.assert_allclose(out["L"], ref["L"], rtol=1e-8, atol=1e-7)
 
 
 @pytest.mark.parametrize("delay,kind", [(32, 0), (32, 2), (16, 0)])
