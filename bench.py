@@ -28,8 +28,9 @@ from miniqmc_amd import nlpp, qmcx, systems  # noqa: E402
 from miniqmc_amd import autotune  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# Metropolis legs: the FP32 form decides moves with |u - R^2| <= 1e-2 R^2 on the FP64 ratio
-ACCEPT_BAND = 1e-2
+# Metropolis legs: the FP32 form decides moves with |u - R^2| <= 2e-3 R^2 on the FP64 ratio (wider
+# bands measured 1e-3 .. 3e-1: the walkers that still part are the ill-conditioned ones, at any band)
+ACCEPT_BAND = 2e-3
 METRIC = "walker-moves/s at NiO-64 on 1 MI355X, batched FP32 vs FP64 delay-1; log|Psi| err"
 
 

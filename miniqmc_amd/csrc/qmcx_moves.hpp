@@ -43,9 +43,12 @@ template <typename T>
 __host__ __device__ inline size_t j3_slots_lds(int waves) {
   return (size_t)waves * kJ3Slots * (sizeof(int) + 5 * sizeof(T));
 }
-// LDS of the FP64 re-evaluation of a near-tie decision (FP32 form, K > 1): three n-vectors and
-// 2K + 32 reduction slots
-__host__ __device__ inline size_t arb_lds(int n, int K) { return sizeof(double) * (3 * (size_t)n + 2 * (size_t)K + 32); }
+// LDS of the FP64 re-evaluation of a near-tie decision (FP32 form, K > 1): three n-vectors, 2K + 32
+// reduction slots and kArbGroups n-vectors of partial sums
+constexpr int kArbGroups = 5;
+__host__ __device__ inline size_t arb_lds(int n, int K) {
+  return sizeof(double) * ((3 + kArbGroups) * (size_t)n + 2 * (size_t)K + 32);
+}
 __host__ __device__ inline unsigned lds_take(unsigned& o, size_t bytes) {
   unsigned r = o;
   o = (unsigned)((o + bytes + 15) & ~size_t(15));
@@ -446,36 +449,77 @@ __device__ __attribute__((noinline)) void flush_own(int n, int K, int KP, int c,
 //   X s = sigma + sum_d t_d V_d,  sigma = sum_i s_i Ainv[i],
 //   t = -Binv^T q,  q_e = sigma . U_e - s[rows_e]
 // (U_e the new FP32 Slater rows, V_d the stale inverse rows, Binv the capacitance inverse).  It
-// stops when the ratio's last correction is below 1e-14 of it.  xv holds x0 on entry; xv, sv, tv
-// are n doubles of LDS, red 2K + 32.  Every lane returns the ratio.
+// stops when the ratio's last correction is below 1e-14 of it, when the corrections stop shrinking
+// (the FP64 rounding floor of an ill-conditioned row), or (j2 > 0) when the error left, bounded
+// from the last two corrections as a geometric tail, can no longer move (rho^2 j2) across the
+// uniform u.  xv holds x0 on entry; xv, sv, tv are n doubles of LDS, red 2K + 32, part kArbGroups
+// n.  Every lane returns the ratio.
 constexpr int kArbMaxIter = 8;
 __device__ inline double arb_det_ratio(int n, int npad, int row, int r0, int c, int KP, int sa,
                                                           const float* Ainv, const double* A64, const double* s64c,
                                                           const float* spoc, const float* sAc, const double* sBinv,
                                                           const int* sRows, const int* sAl, const double* psiv64,
-                                                          double* xv, double* sv, double* tv, double* red, int K) {
+                                                          double* xv, double* sv, double* tv, double* red, int K,
+                                                          double* part, double u, double j2) {
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
   double* q = red;           // [K]
   double* t = red + K;       // [K]
   double* bs = red + 2 * K;  // block_sum scratch
-  double rho = 0.0;
+  // the chunk's accepted electrons (their FP64 Slater rows are their proposals'), a bit each
+  unsigned long long m0 = 0ull, m1 = 0ull;
+  for (int d = 0; d < c; ++d) {
+    const int ei = sAl[d];
+    if (ei < 64) m0 |= 1ull << ei;
+    else m1 |= 1ull << (ei - 64);
+  }
+  double rho = 0.0, prev = INFINITY;
   for (int it = 0; it < kArbMaxIter; ++it) {
-    // residual, a wave per row
-    for (int i = wid; i < n; i += nw) {
-      const double* ar = A64 + (size_t)i * n;
-      const int ei = i - r0;
-      for (int d = 0; d < c; ++d)
-        if (sAl[d] == ei) ar = s64c + (size_t)ei * npad;
-      double acc = 0.0;
-      for (int j = lane; j < n; j += 64) acc += ar[j] * xv[j];
-      acc = wave_sum(acc);
-      if (lane == 0) sv[i] = (i == row ? 1.0 : 0.0) - acc;
+    // residual, four rows per wave at a time (their loads in flight together)
+    for (int i0 = wid * 4; i0 < n; i0 += nw * 4) {
+      const double* ar[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = min(i0 + r, n - 1), ei = i - r0;
+        const bool acc = ei >= 0 && ei < 128 && (((ei < 64 ? m0 : m1) >> (ei & 63)) & 1ull) != 0ull;
+        ar[r] = acc ? s64c + (size_t)ei * npad : A64 + (size_t)i * n;
+      }
+      double a4[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int j = lane; j < n; j += 64) {
+        const double xj = xv[j];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a4[r] += ar[r][j] * xj;
+      }
+      wave_sums_rows<4>(a4);
+      const int i = i0 + (lane >> 4);   // row r of the wave holds the sum of row i0 + r
+      if ((lane & 15) == 0 && i < n) sv[i] = (i == row ? 1.0 : 0.0) - a4[0];
     }
     __syncthreads();
-    // sigma = sum_i s_i Ainv[i]
-    if (tid < n) {
-      double sg = 0.0;
+    // sigma = sum_i s_i Ainv[i]: a lane takes four columns (16-byte loads) and G groups of lanes
+    // split the rows, their partial sums added in group order
+    const int nq = n >> 2;
+    if ((n & 3) == 0 && nq <= nt) {
+      const int G = min(nt / nq, kArbGroups), gq = tid / nq, cq = tid - gq * nq;
+      if (gq < G) {
+        double s4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll 8
+        for (int i = gq; i < n; i += G) {
+          const f32x4_t a = *reinterpret_cast<const f32x4_t*>(Ainv + (size_t)i * n + 4 * cq);
+          const double si = sv[i];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) s4[r] += si * (double)a[r];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part[(size_t)gq * n + 4 * cq + r] = s4[r];
+      }
+      __syncthreads();
+      if (tid < n) {
+        double sg = 0.0;
+        for (int g = 0; g < G; ++g) sg += part[(size_t)g * n + tid];
+        tv[tid] = sg;
+      }
+    } else if (tid < n) {
+      double sg = 0.0;
+#pragma unroll 16
       for (int i = 0; i < n; ++i) sg += sv[i] * (double)Ainv[(size_t)i * n + tid];
       tv[tid] = sg;
     }
@@ -506,7 +550,14 @@ __device__ inline double arb_det_ratio(int n, int npad, int row, int r0, int c, 
     }
     block_sum<double, 2>(v, bs);
     rho = v[1];
-    if (fabs(v[0]) <= 1e-14 * fabs(rho)) break;
+    const double dr = fabs(v[0]);
+    if (dr <= 1e-14 * fabs(rho) || dr >= 0.5 * prev) break;
+    if (j2 > 0.0 && it > 0) {
+      const double qc = dr / prev;                 // contraction of the last step
+      const double rem = dr * qc / (1.0 - qc);     // geometric bound on the error left
+      if (4.0 * fabs(rho) * rem * j2 < fabs(u - rho * rho * j2)) break;
+    }
+    prev = dr;
   }
   return rho;
 }
@@ -1013,17 +1064,20 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
     bool accept = a.mode == 0 ? (uacc < 0.5) : (uacc < ratio * ratio);
     if constexpr (ARB && sizeof(T) == 4 && KD) {
       // a uniform within the band around ratio^2: the decision is taken on the FP64 ratio
-      if (arb_on && c0 == 0 && fabs(uacc - ratio * ratio) <= a.band * ratio * ratio) {
+      // a rare branch: marked cold so that the register allocator keeps its spills inside it
+      if (__builtin_expect(arb_on && c0 == 0 && fabs(uacc - ratio * ratio) <= a.band * ratio * ratio, 0)) {
         double* sv = sArbX + n;
         double* tv = sv + n;
         double* ared = tv + n;
         const size_t pro = ((size_t)w * 2 + a.det) * n + r0;   // the chunk's first proposal
         // the stale rows: the LDS cache, or (streamed) Ainv's rows of the chunk in HBM
+        const double jr = arb_jastrow_ratio<T>(S, sR, iel, x0, y0, z0, nx, ny, nz, ared);
         rdet = arb_det_ratio(n, S.npad, row, r0, c, ST ? K : KP, ST ? n : sa, (const float*)Ainv, B.A64 + moff,
                              B.spo64 + pro * S.npad, (const float*)spo,
                              ST ? (const float*)Ainv + (size_t)r0 * n : (const float*)sAc, ST ? binv : sBinv, sRows,
-                             sAl, B.spo64 + (pro + e) * S.npad, sArbX, sv, tv, ared, K);
-        ratio = rdet * arb_jastrow_ratio<T>(S, sR, iel, x0, y0, z0, nx, ny, nz, ared);
+                             sAl, B.spo64 + (pro + e) * S.npad, sArbX, sv, tv, ared, K, ared + 2 * K + 32, uacc,
+                             jr * jr);
+        ratio = rdet * jr;
         accept = uacc < ratio * ratio;
         if (tid == 0) sAcc[aNarb] += 1.0;
       }

@@ -145,3 +145,43 @@ def test_nio64_fp32_delayed_vs_oracle(nio64, oracle_runs, mode, delay):
             dlog = np.abs(got["log"] - ref["log"][:, 0])
             _say(f"  log|Psi| before any refresh: |dlog| {dlog.tolist()}, allowed {allowed.tolist()}")
             assert np.all(dlog <= allowed)
+
+
+def test_nio64_metropolis_flips_explained_by_conditioning(nio64):
+    """The headline batch (128 walkers, delay 32, FP64 refresh every sweep) under Metropolis with the
+    bench's near-tie band: a walker's FP32 accept sequence may part from the FP64 one only at a move
+    where the FP32 ratio error is within the conditioning bound 4 * 2^-24 * max(s_1 .. s_m) (the
+    oracle's sensitivity, running over the determinant's moves) and that bound exceeds the band, i.e.
+    where no FP32 evaluation can resolve the decision.  At most a few walkers may part."""
+    sysd, dev, orc = nio64
+    W, seed, tau, band = 128, 11, 0.3, 2e-3
+    M = qmcx.ACCEPT_METROPOLIS
+    R0 = systems.initial_positions(sysd, W, seed=seed)
+    b64 = qmcx.WalkerBatch(dev, W, delay_rank=1, precision=qmcx.FP64)
+    b32 = qmcx.WalkerBatch(dev, W, delay_rank=32, precision=qmcx.FP32)
+    b32.set_accept_band(band)
+    for b in (b64, b32):
+        b.set_positions(R0)
+        b.evaluate_log()
+        b.set_trace(True)
+    b64.run(1, seed, tau, M)
+    b32.run(1, seed, tau, M, recompute_every=1)
+    t64, t32 = b64.trace(), b32.trace()
+    parted = np.where((t32[:, :, 2] != t64[:, :, 2]).any(axis=1))[0]
+    _say(f"walkers parted from the FP64 path in one sweep: {parted.tolist()}; arbitrations "
+         f"{int(b32.arbitrations().sum())}")
+    assert len(parted) <= 8
+    n = sysd.nup
+    for w in parted:
+        ref = orc.run_vmc(R0[w:w + 1], 1, seed, tau, mode=M, delay=1, walker0=int(w), trace=True, sensitivity=True)
+        rt, s = ref["trace"][0, 0], ref["sensitivity"][0, 0]
+        m = int(np.argmax(t32[w, :, 2] != rt[:, 2]))
+        det = slice(0, n) if m < n else slice(n, 2 * n)
+        smax = np.maximum.accumulate(s[det])[m % n]
+        bound = 4 * EPS32 * smax
+        err = abs(t32[w, m, 0] - rt[m, 0]) / abs(rt[m, 0])
+        _say(f"  walker {w}, move {m}: FP32 ratio {t32[w, m, 0]:.6f}, FP64 {rt[m, 0]:.6f}, rel err {err:.2e}, "
+             f"running sensitivity {smax:.2e}, bound {bound:.2e}")
+        assert np.array_equal(t64[w, :, 2], rt[:, 2])   # the FP64 GPU path is the oracle's chain
+        assert err <= bound
+        assert bound > band
