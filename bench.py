@@ -357,6 +357,61 @@ def measure_mixed(dev64, sys64, mode, steps, err_steps, seed, tau):
                     f"{err_steps} sweeps vs the FP64 delay-1 GPU path on walkers with the same accept counts"}
 
 
+def measure_nio32(mode, steps, err_steps, fp64_steps, seed, tau, walkers=256, delay=32, rows=16):
+    """The NiO-32 batched precision form: 256 walkers, FP32 at delay rank 32, with the FP64 inverse
+    recomputed either every sweep or every 16 accepted rows of a walker's determinant (no per-sweep
+    refresh), both inverting FP64 Slater rows.  Per cadence: walker-moves/s launched and replayed
+    from a captured HIP graph, and
+    log|Psi| after err_steps sweeps vs the FP64 delay-1 GPU path from the same start and streams;
+    beside them the FP64 delay-1 path's own rate."""
+    sysd = systems.make_system("nio32", seed=seed)
+    dev = qmcx.System(sysd, precisions=(qmcx.FP32, qmcx.FP64))
+    R0 = systems.initial_positions(sysd, walkers, seed=seed)
+    moves = walkers * sysd.nel * steps
+    b64 = qmcx.WalkerBatch(dev, walkers, delay_rank=1, precision=qmcx.FP64)
+    b64.set_positions(R0)
+    b64.evaluate_log()
+    h64 = b64.run(err_steps, seed, tau, mode)
+    a64 = b64.accepts()
+    t0 = time.perf_counter()
+    b64.run(fp64_steps, seed, tau, mode)
+    value64 = walkers * sysd.nel * fp64_steps / (time.perf_counter() - t0)
+    b64.close()
+    out = {"walkers": walkers, "electrons": sysd.nel, "delay_rank": delay, "steps": steps,
+           "fp64_delay1_value": value64, "cadences": {}}
+    for name, nrows, every in (("refresh_every_sweep", 0, 1), (f"recompute_every_{rows}_accepted_rows", rows, 0)):
+        b = qmcx.WalkerBatch(dev, walkers, delay_rank=delay, precision=qmcx.FP32)
+        b.set_recompute_rows(nrows)
+        b.set_fp64_rows(True)   # the FP64 inversions take FP64 Slater rows (no FP32 rounding of A)
+        b.set_positions(R0)
+        b.evaluate_log()
+        h = b.run(err_steps, seed, tau, mode, recompute_every=every)
+        same = b.accepts() == a64
+        err = (np.abs(h - h64) / np.abs(h64))[:, same]
+        b.run(1, seed, tau, mode, recompute_every=every)
+        t0 = time.perf_counter()
+        b.run(steps, seed, tau, mode, recompute_every=every)
+        wall = time.perf_counter() - t0
+        b.set_graph(True)
+        b.run(steps, seed, tau, mode, recompute_every=every)
+        t0 = time.perf_counter()
+        b.run(steps, seed, tau, mode, recompute_every=every)
+        wall_g = time.perf_counter() - t0
+        b.close()
+        out["cadences"][name] = {"value": moves / wall, "ms_per_step": 1e3 * wall / steps,
+                                 "graph_replay_value": moves / wall_g,
+                                 "speedup_vs_fp64_delay1": moves / wall / value64,
+                                 "log_psi_err_vs_fp64": {"max_rel": float(err.max()) if err.size else None,
+                                                         "mean_rel": float(err.mean()) if err.size else None,
+                                                         "walkers_same_accepts": int(same.sum()),
+                                                         "sweeps": err_steps}}
+        log(f"nio32 {walkers} walkers, delay {delay}, {name}: {moves / wall:.4e} moves/s ({1e3 * wall / steps:.2f} "
+            f"ms/sweep), graph replay {moves / wall_g:.4e}; log err vs FP64 after {err_steps} sweeps "
+            f"{out['cadences'][name]['log_psi_err_vs_fp64']['max_rel']:.2e}")
+    log(f"nio32 FP64 delay 1: {value64:.4e} moves/s")
+    return out
+
+
 def measure_evaluate_log(dev, sysd, R0, prec, steps):
     """WaveFunction::evaluateLog from scratch for the whole batch (SPO rows at every electron, FP64 LU,
     J1/J2 over all pairs, evaluateGL): walkers per second."""
@@ -394,6 +449,8 @@ def main():
     ap.add_argument("--nlpp-steps", type=int, default=5, help="timed NonLocalPP evaluations per precision (0: skip)")
     ap.add_argument("--mixed-steps", type=int, default=10,
                     help="timed sweeps of the mixed NiO-16/32/64 batch in one grouped launch (0: skip)")
+    ap.add_argument("--nio32-steps", type=int, default=10,
+                    help="timed sweeps of the NiO-32 batched leg (256 walkers, two recompute cadences; 0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device five times (a kernel of known HBM traffic "
@@ -515,6 +572,10 @@ def main():
     if args.mixed_steps > 0:
         mixed = measure_mixed(dev, sysd, mode, args.mixed_steps, args.err_steps, args.seed, args.tau)
 
+    nio32 = None
+    if args.nio32_steps > 0:
+        nio32 = measure_nio32(mode, args.nio32_steps, args.err_steps, 2, args.seed, args.tau)
+
     cpu = None
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
@@ -570,6 +631,7 @@ def main():
         "nlpp_virtual_ratios": nl or None,
         "evaluate_log": elog or None,
         "mixed_batch": mixed,
+        "nio32_batched": nio32,
     }
     print(json.dumps(out))
 

@@ -135,6 +135,15 @@ int qmcx_batch_set_flush(qmcx_batch_t b, int kind);
  * calls always use the streamed form with k > 1 and keep each walker's accepted rows pending across calls
  * until it has k of them or qmcx_flex_complete_updates.  get reports the form the batch's sweeps use. */
 int qmcx_batch_set_delay_form(qmcx_batch_t b, int form);
+/* FP64 recompute cadence (no reference counterpart; the FP32 form's "FP64 LU inverse recomputation every
+ * rows accepted rows per walker"): with rows > 0 each walker's inverse of a determinant is rebuilt in FP64
+ * from its Slater rows (DiracMatrix::invert_transpose, DiracMatrix.h:93-121) after every `rows` accepted
+ * moves of that determinant, and its log|det| and phase are replaced by the FP64 values; the walker's
+ * pending delayed rows are dropped there rather than flushed.  The count runs across sweeps and restarts at
+ * every FP64 inversion (evaluateLog, the per-sweep refresh of qmcx_run_sweeps).  0 = off.  Needs
+ * Sherman-Morrison moves or the in-kernel flush (FP32, n a multiple of 16 up to 384, k <= 32) and no
+ * three-body Jastrow. */
+int qmcx_batch_set_recompute_rows(qmcx_batch_t b, int rows);
 int qmcx_batch_get_delay_form(qmcx_batch_t b, int* form);
 
 /* WaveFunction::evaluateLog (src/QMCWaveFunctions/WaveFunction.cpp:178-195): SPO for all

@@ -285,6 +285,12 @@ struct qmcx_batch_s {
   // for (st_ask); flex_pending: flex_* moves left delayed rows pending in state slot 0
   bool st_need = false, st_ask = false, flex_pending = false;
   bool streamed() const { return K > 1 && (st_need || st_ask); }
+  // FP64 recompute every `cadence` accepted rows of a walker's determinant (0: off): per walker and
+  // determinant its position in the sweep and accepted rows since the last recompute, cad [W][2][2];
+  // the walkers a round stopped for their recompute, cad_flag [W]
+  int cadence = 0;
+  int* cad = nullptr;
+  int* cad_flag = nullptr;
   // FP32 Metropolis: near-tie band of the FP64 re-evaluation (0: off).  While it is on the batch
   // keeps FP64 Slater rows (A64, spo64), which the re-evaluation and the FP64 refresh read;
   // rows64_stale until an evaluateLog has filled them
@@ -653,6 +659,9 @@ int launch_flush(qmcx_batch_t b, hipStream_t st, int det, int j0, int j1, int th
   return 0;
 }
 
+template <typename T>
+int invert_det(qmcx_batch_t b, int det, bool into_fresh, hipStream_t st, const int* only);
+
 // The moves of one determinant's electrons in one sweep.  With K > 1 each chunk's delayed
 // updates (recorded in its state slot) reach the whole inverse before the next chunk: inside
 // the move kernel when its LDS holds the flush (FP32, n a multiple of 16 up to 384), else by a
@@ -683,6 +692,34 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
   // one launch for all the chunks when no flush launch is needed between them: Sherman-Morrison
   // moves, or each chunk's updates applied by the move kernel itself
   const bool inflush = b->K > 1 && !st && use_mfma_flush<T>(b) && flush_own_fits<T>(s->N, n, b->K, chunk);
+  if (b->cadence > 0) {
+    // FP64 recompute every b->cadence accepted rows of a walker: rounds of one move launch (each
+    // walker from its own position until its cadence or the determinant's end) and one FP64
+    // inversion of the walkers that stopped; a walker stops at most n / cadence times
+    if (!(b->K == 1 || inflush) || b->j3)
+      return fail(QMCX_ERR_STATE, "the recompute cadence needs Sherman-Morrison moves or delayed updates applied "
+                                  "inside the move kernel, and no three-body Jastrow");
+    hipLaunchKernelGGL(k_cadence_begin, dim3((b->W + 255) / 256), dim3(256), 0, b->stream, b->cad, b->W, det);
+    HIPCHK(hipGetLastError());
+    a.e_begin = det * n;
+    a.e_end = det * n + n;
+    a.chunk = 0;
+    a.nchunks = n;   // bounded by the determinant's end
+    a.inflush = inflush ? (flush ? 2 : 1) : 0;
+    a.flush_u = b->flush_kind == 2 ? 0 : kFlushOwnU;
+    a.cadence = b->cadence;
+    a.cad = b->cad;
+    a.cad_flag = b->cad_flag;
+    for (int round = 0; round <= n / b->cadence; ++round) {
+      if (b->prof_record(0)) return QMCX_ERR_HIP;
+      if (launch_moves<T>(b, nt, L.total, a, false)) return QMCX_ERR_HIP;
+      if (b->prof_close()) return QMCX_ERR_HIP;
+      if (b->prof_record(2)) return QMCX_ERR_HIP;
+      if (int rc = invert_det<T>(b, det, false, b->stream, b->cad_flag)) return rc;
+      if (b->prof_close()) return QMCX_ERR_HIP;
+    }
+    return 0;
+  }
   if (b->K == 1 || inflush) {
     a.e_begin = det * n;
     a.e_end = det * n + n;
@@ -733,23 +770,36 @@ int flush_pending(qmcx_batch_t b, int thr = 1) {
   return 0;
 }
 
-// FP64 refresh of one determinant of every walker on the recompute stream: the inverse in
-// place; log|det| and phase into the batch arrays (out = logdet) or into ld_fresh / ph_fresh
+// FP64 inverse of one determinant of every walker (only: of the walkers whose flag is set) from
+// its Slater rows on stream st: the inverse in place; log|det| and phase into the batch arrays
+// (into_fresh false) or into ld_fresh / ph_fresh
 template <typename T>
-int refresh_det(qmcx_batch_t b, int det, bool into_fresh) {
+int invert_det(qmcx_batch_t b, int det, bool into_fresh, hipStream_t st, const int* only) {
   const int n = b->sys->n;
   const size_t nn = (size_t)n * n;
-  if (b->prof_record(2, b->aux)) return QMCX_ERR_HIP;
   double* ld = into_fresh ? b->ld_fresh : b->logdet + det;
   double* ph = into_fresh ? b->ph_fresh : b->phase + det;
   const int lds = into_fresh ? 1 : 2;
   if (sizeof(T) == 4 && b->rows64)   // the FP32 form with FP64 Slater rows inverts those
-    HIPCHK((launch_gj_invert<double, T>(n, b->W, b->aux, b->A64 + det * nn, (T*)b->Ainv + det * nn,
-                                        b->lu_ws + det * 2 * nn, ld, ph, lds, 2)));
+    HIPCHK((launch_gj_invert<double, T>(n, b->W, st, b->A64 + det * nn, (T*)b->Ainv + det * nn,
+                                        b->lu_ws + det * 2 * nn, ld, ph, lds, 2, only)));
   else
-    HIPCHK((launch_gj_invert<T, T>(n, b->W, b->aux, (const T*)b->A + det * nn, (T*)b->Ainv + det * nn,
-                                   b->lu_ws + det * 2 * nn, ld, ph, lds, 2)));
+    HIPCHK((launch_gj_invert<T, T>(n, b->W, st, (const T*)b->A + det * nn, (T*)b->Ainv + det * nn,
+                                   b->lu_ws + det * 2 * nn, ld, ph, lds, 2, only)));
+  return 0;
+}
+
+// FP64 refresh of one determinant of every walker on the recompute stream (it also restarts the
+// count of the recompute cadence)
+template <typename T>
+int refresh_det(qmcx_batch_t b, int det, bool into_fresh) {
+  if (b->prof_record(2, b->aux)) return QMCX_ERR_HIP;
+  if (int rc = invert_det<T>(b, det, into_fresh, b->aux, nullptr)) return rc;
   if (b->prof_close(b->aux)) return QMCX_ERR_HIP;
+  if (b->cad) {
+    hipLaunchKernelGGL(k_cadence_reset, dim3((b->W + 255) / 256), dim3(256), 0, b->aux, b->cad, b->W, det);
+    HIPCHK(hipGetLastError());
+  }
   return 0;
 }
 
@@ -1642,6 +1692,8 @@ int qmcx_batch_destroy(qmcx_batch_t b) {
   if (b->hist) hipFree(b->hist);
   if (b->flex_displ) hipFree(b->flex_displ);
   if (b->flex_decide) hipFree(b->flex_decide);
+  if (b->cad) hipFree(b->cad);
+  if (b->cad_flag) hipFree(b->cad_flag);
   if (b->vr_pos) hipFree(b->vr_pos);
   if (b->vr_out) hipFree(b->vr_out);
   if (b->nl_out) hipFree(b->nl_out);
@@ -1696,6 +1748,22 @@ int qmcx_batch_set_walker_offset(qmcx_batch_t b, int walker0) {
   return 0;
 }
 
+int qmcx_batch_set_recompute_rows(qmcx_batch_t b, int rows) {
+  if (!b || rows < 0) return fail(QMCX_ERR_ARG, "bad argument");
+  if (rows > 0 && b->j3) return fail(QMCX_ERR_ARG, "the recompute cadence is not built with a three-body Jastrow");
+  if (rows > 0 && !b->cad) {
+    HIPCHK(hipMalloc(&b->cad, sizeof(int) * 4 * (size_t)b->W));
+    HIPCHK(hipMalloc(&b->cad_flag, sizeof(int) * (size_t)b->W));
+    hipLaunchKernelGGL(k_cadence_clear, dim3((4 * b->W + 255) / 256), dim3(256), 0, b->stream, b->cad, 4 * b->W);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_cadence_clear, dim3((b->W + 255) / 256), dim3(256), 0, b->stream, b->cad_flag, b->W);
+    HIPCHK(hipGetLastError());
+  }
+  b->drop_graph();   // a captured run holds the old setting
+  b->cadence = rows;
+  return 0;
+}
+
 int qmcx_batch_set_delay_form(qmcx_batch_t b, int form) {
   if (!b || form < 0 || form > 1)
     return fail(QMCX_ERR_ARG, "delay form must be 0 (automatic) or 1 (streamed)");
@@ -1735,6 +1803,10 @@ int qmcx_evaluate_log(qmcx_batch_t b, double* log_out) {
   int rc = b->prec == QMCX_FP32 ? evaluate_log_impl<float>(b) : evaluate_log_impl<double>(b);
   if (rc) return rc;
   b->flex_pending = false;   // the inverse is rebuilt from scratch: pending delayed rows are void
+  if (b->cad) {
+    hipLaunchKernelGGL(k_cadence_reset, dim3((b->W + 255) / 256), dim3(256), 0, b->stream, b->cad, b->W, -1);
+    HIPCHK(hipGetLastError());
+  }
   b->rows64_stale = false;   // the FP64 Slater rows are filled with the SPO rows when kept
   HIPCHK(hipStreamSynchronize(b->stream));
   if (b->j3)

@@ -106,6 +106,12 @@ struct MoveArgs {
   // streamed delayed updates: the launch continues from the delayed rows pending in its state slot
   // (flex_* calls keep them pending until the delay rank is reached or completeUpdates)
   int resume;
+  // FP64 recompute cadence (cadence > 0 accepted rows): a walker's moves of the determinant start
+  // at its own position cad[w][det][0] and stop after the move that brings its accepted rows since
+  // the last recompute (cad[w][det][1]) to cadence; it then sets cad_flag[w] for the recompute
+  int cadence;
+  int* cad;
+  int* cad_flag;
   const int* decide;     // [W] caller accept decisions (flex_acceptrestoreMove) or null
   // FP32 form, Metropolis: a move whose uniform u lies within band * ratio^2 of ratio^2 is
   // decided on its FP64 ratio (0: off)
@@ -361,6 +367,22 @@ __global__ void k_reset_delay(BatchP<T> B, int det, int thr) {
   int* nd = B.ndelay + state_slot(B, w, det, 0);
   const int c = *nd;
   if (c > 0 && c >= thr) *nd = 0;
+}
+// FP64 recompute cadence: a determinant's sweep starts every walker at its first electron
+__global__ void k_cadence_begin(int* cad, int W, int det) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < W) cad[((size_t)w * 2 + det) * 2] = 0;
+}
+// an FP64 inversion of determinant det of every walker: no accepted rows since (det < 0: both)
+__global__ void k_cadence_reset(int* cad, int W, int det) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  for (int d = 0; d < 2; ++d)
+    if (det < 0 || d == det) cad[((size_t)w * 2 + d) * 2 + 1] = 0;
+}
+__global__ void k_cadence_clear(int* p, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0;
 }
 // one state slot j per launch
 template <typename T>
@@ -693,8 +715,9 @@ __host__ __device__ inline GJLds gj_lds(int n) { return gj_lds_b(n, gj_panel(n))
 
 template <typename TI, typename TO, int B, int NT>
 __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, double* ws, double* logdet,
-                                                          double* phase, int ld_stride, int mat_stride) {
+                                                          double* phase, int ld_stride, int mat_stride, const int* only) {
   const int mat = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  if (only != nullptr && only[mat] == 0) return;   // only the matrices whose flag is set
   const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
   const int b = B, ps = b + 1;
   const GJLds L = gj_lds(n);
@@ -1030,20 +1053,21 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
 // nmat matrices at A, A + mat_stride n^2, ...; log|det| and phase of matrix m at [m * ld_stride]
 template <typename TI, typename TO>
 inline hipError_t launch_gj_invert(int n, int nmat, hipStream_t st, const TI* A, TO* psiM, double* ws,
-                                   double* logdet, double* phase, int ld_stride, int mat_stride = 1) {
+                                   double* logdet, double* phase, int ld_stride, int mat_stride = 1,
+                                   const int* only = nullptr) {
   // one thread per row of the panel: 512 threads up to n = 448 (32 registers pairs of panel row),
   // 1024 threads beyond
   const size_t lds = gj_lds(n).total;
   const int b = gj_panel(n);
   if (b == 32)
     hipLaunchKernelGGL((k_gj_invert<TI, TO, 32, 512>), dim3(nmat), dim3(512), lds, st, n, A, psiM, ws, logdet, phase,
-                       ld_stride, mat_stride);
+                       ld_stride, mat_stride, only);
   else if (b == 16)
     hipLaunchKernelGGL((k_gj_invert<TI, TO, 16, 1024>), dim3(nmat), dim3(1024), lds, st, n, A, psiM, ws, logdet,
-                       phase, ld_stride, mat_stride);
+                       phase, ld_stride, mat_stride, only);
   else
     hipLaunchKernelGGL((k_gj_invert<TI, TO, 8, 1024>), dim3(nmat), dim3(1024), lds, st, n, A, psiM, ws, logdet, phase,
-                       ld_stride, mat_stride);
+                       ld_stride, mat_stride, only);
   return hipGetLastError();
 }
 

@@ -770,10 +770,21 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
   }
   const Cell<T>& C = *sCell;   // read from LDS where used (a register copy costs spills)
   const int jr = tid < n ? tid : n - 1;
+  // FP64 recompute cadence: this walker's first electron and its accepted rows since the last
+  // recompute; it stops after the move that brings them to a.cadence
+  int cbn = a.e_begin, since = 0;
+  bool stopped = false;
+  int* cadw = nullptr;
+  if (a.cadence > 0) {
+    cadw = a.cad + ((size_t)w * 2 + a.det) * 2;
+    cbn += cadw[0];
+    since = cadw[1];
+  }
 
-  for (int ci = 0; ci < a.nchunks; ++ci) {
-  const int cb = a.e_begin + ci * Emax;     // the chunk's first electron
+  for (int ci = 0; ci < a.nchunks && cbn < a.e_end && !stopped; ++ci) {
+  const int cb = cbn;                       // the chunk's first electron
   const int E = min(Emax, a.e_end - cb);
+  int Eend = E;                             // moves made (fewer when the cadence stops the walker)
   const int r0 = cb - a.det * n;
   const int chunk = a.chunk + ci;           // index of the chunk in its determinant's sweep
   // this chunk's delayed-update state slot
@@ -842,7 +853,7 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
   };
   prefetch(0);
 
-  for (int e = 0; e < E; ++e) {
+  for (int e = 0; e < Eend; ++e) {
     const int iel = cb + e, row = r0 + e;
     QMCX_TICK(0)
     // ---- phase 1: issue the next move's global loads, then getInvRow's small algebra
@@ -1314,6 +1325,10 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
     // with K = 1 the next move reads rows of Ainv that other waves just wrote
     if (!KD) __syncthreads();
     else lds_sync();
+    if (a.cadence > 0 && accept && ++since >= a.cadence) {
+      stopped = true;   // the recompute follows this launch
+      Eend = e + 1;
+    }
     QMCX_TICK(4)
   }
 
@@ -1416,7 +1431,8 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
     for (int i = tid; i < c; i += nt) B.rows[slot * K + i] = sRows[i];
   QMCX_TICK(22)
   if constexpr (KD && sizeof(T) == 4) {
-    if (a.inflush && c > 0 && (ci + 1 < a.nchunks || a.inflush == 2)) {
+    // a stopped walker's inverse is rebuilt by the recompute instead
+    if (a.inflush && c > 0 && !stopped && (cb + E < a.e_end || a.inflush == 2)) {
       // the chunk's delayed updates reach this walker's whole inverse before the next chunk
       __syncthreads();
       float* sU = (float*)(smem + L.oQ);
@@ -1430,7 +1446,13 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
     }
   }
   QMCX_TICK(23)
+  cbn = cb + Eend;
   }   // chunks
+  if (a.cadence > 0 && tid == 0) {
+    cadw[0] = cbn - a.e_begin;
+    cadw[1] = stopped ? 0 : since;
+    a.cad_flag[w] = stopped ? 1 : 0;
+  }
 
   // ---- walker state back to HBM
   for (int i = tid; i < 3 * N; i += nt) B.R[(size_t)w * 3 * N + i] = sR[i];

@@ -53,6 +53,7 @@ _SIGS = {
     "qmcx_batch_set_tile": [_vp, _i],
     "qmcx_batch_set_flush": [_vp, _i],
     "qmcx_batch_set_delay_form": [_vp, _i],
+    "qmcx_batch_set_recompute_rows": [_vp, _i],
     "qmcx_batch_get_delay_form": [_vp, C.POINTER(C.c_int)],
     "qmcx_batch_set_graph": [_vp, _i],
     "qmcx_evaluate_log": [_vp, _vp],
@@ -274,6 +275,10 @@ class WalkerBatch:
     def set_delay_form(self, form):
         """0: automatic (LDS-resident delayed rows when they fit), 1: the streamed form."""
         _check(load().qmcx_batch_set_delay_form(self.h, form))
+
+    def set_recompute_rows(self, rows):
+        """FP64 recompute of a walker's inverse after every `rows` accepted rows of a determinant (0: off)."""
+        _check(load().qmcx_batch_set_recompute_rows(self.h, rows))
 
     def delay_form(self):
         f = C.c_int(0)

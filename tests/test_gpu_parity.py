@@ -93,7 +93,7 @@ def test_invert_transpose_batched_vs_oracle(n):
 
 
 def _compare_run(sysd, dev, orc, nw, nsweeps, delay, prec, mode, tau, tol_ratio, tol_log, seed=11, flush_kind=0,
-                 tile=0, delay_form=0, band=0.0):
+                 tile=0, delay_form=0, band=0.0, recompute_rows=0):
     R0 = systems.initial_positions(sysd, nw, seed=seed)
     ref = orc.run_vmc(R0, nsweeps, seed, tau, mode=mode, delay=delay, trace=True)
     wb = qmcx.WalkerBatch(dev, nw, delay_rank=delay, precision=prec, tile=tile)
@@ -102,6 +102,8 @@ def _compare_run(sysd, dev, orc, nw, nsweeps, delay, prec, mode, tau, tol_ratio,
     assert wb.delay_form() == delay_form
     if band > 0.0:
         wb.set_accept_band(band)
+    if recompute_rows:
+        wb.set_recompute_rows(recompute_rows)
     wb.set_positions(R0)
     log0 = wb.evaluate_log()
     np.testing.assert_allclose(log0, ref["log_init"], rtol=tol_log)
@@ -195,6 +197,49 @@ def test_sweep_streamed_vs_oracle(cubic, cubic_dev, cubic_orc, prec, delay, kind
         np.testing.assert_allclose(out["R"], ref["R"], rtol=0, atol=1e-12)
         np.testing.assert_allclose(out["G"], ref["G"], rtol=1e-8, atol=1e-8)
         np.testing.assert_allclose(out["L"], ref["L"], rtol=1e-8, atol=1e-7)
+
+
+@pytest.mark.parametrize("prec,delay,rows", [(qmcx.FP64, 1, 4), (qmcx.FP64, 1, 16), (qmcx.FP32, 1, 8),
+                                             (qmcx.FP32, 8, 4), (qmcx.FP32, 16, 16), (qmcx.FP32, 32, 5)])
+def test_sweep_recompute_cadence_vs_oracle(cubic, cubic_dev, cubic_orc, prec, delay, rows):
+    """FP64 recompute of a walker's inverse after every `rows` accepted moves of a determinant (each
+    walker stops there, its inverse and log|det| are rebuilt in FP64, and it continues from its own
+    electron in the next round), no per-sweep refresh.  FP64: the oracle's sweep to 1e-9 in the ratios
+    and 1e-11 in log|Psi|, identical accepts, positions, G and L (the resumed positions, Jastrow sums
+    and log|det| replacement are exact).  FP32: ratios within 2e-3, log|Psi| within 1e-6."""
+    _say("sweep recompute cadence", prec, "delay", delay, "rows", rows)
+    fp64 = prec == qmcx.FP64
+    out = _compare_run(cubic, cubic_dev, cubic_orc, nw=3, nsweeps=2, delay=delay, prec=prec,
+                       mode=qmcx.ACCEPT_METROPOLIS, tau=0.5, tol_ratio=1e-9 if fp64 else 2e-3,
+                       tol_log=1e-11 if fp64 else 1e-6, recompute_rows=rows)
+    assert out["flips"] == 0
+    ref = out["ref"]
+    assert np.array_equal(out["acc"], ref["naccept"])
+    np.testing.assert_allclose(out["R"], ref["R"], rtol=0, atol=1e-12 if fp64 else 1e-5)
+    if fp64:
+        np.testing.assert_allclose(out["G"], ref["G"], rtol=1e-8, atol=1e-8)
+        np.testing.assert_allclose(out["L"], ref["L"], rtol=1e-8, atol=1e-7)
+
+
+def test_nio32_recompute_cadence_reference_rule(nio32):
+    """NiO-32 batched FP32 at delay 32 with the FP64 recompute every 16 accepted rows and no per-sweep
+    refresh (the reference driver's u < 0.5): log|Psi| within 1e-7 of the FP64 oracle after each of two
+    sweeps, identical accepts."""
+    sysd, dev, orc = nio32
+    R0 = systems.initial_positions(sysd, 2, seed=5)
+    ref = orc.run_vmc(R0, 2, 5, 0.3, mode=qmcx.ACCEPT_REFERENCE, delay=1, trace=True)
+    wb = qmcx.WalkerBatch(dev, 2, delay_rank=32, precision=qmcx.FP32)
+    wb.set_recompute_rows(16)
+    wb.set_positions(R0)
+    wb.evaluate_log()
+    wb.set_trace(True)
+    for s in range(2):
+        hist = wb.run(1, 5, 0.3, qmcx.ACCEPT_REFERENCE, recompute_every=0)
+        assert np.array_equal(wb.trace()[:, :, 2], ref["trace"][:, s][:, :, 2])
+        lerr = np.abs(hist[0] - ref["log"][:, s]) / np.abs(ref["log"][:, s])
+        _say(f"  sweep {s}: max |dlog|/|log| {lerr.max():.3e}")
+        assert lerr.max() < 1e-7
+    assert np.array_equal(wb.accepts(), ref["naccept"])
 
 
 @pytest.mark.parametrize("delay,kind", [(32, 0), (32, 2), (16, 0)])
