@@ -2441,6 +2441,7 @@ int qmcx_group_create(qmcx_group_t* out, const qmcx_batch_t* batches, int nbatch
     else if (b->j3) why = "no three-body Jastrow";
     else if (b->band > 0.0) why = "no near-tie band";
     else if (b->rows64 != batches[0]->rows64) why = "all batches with FP64 Slater rows or none";
+    else if (b->cadence > 0) why = "no recompute cadence";
     else if (b->streamed() || !use_mfma_flush<float>(b) || !flush_own_fits<float>(s->N, s->n, b->K, b->K))
       why = "each batch's delayed updates must be applied inside the move kernel (n a multiple of 16, delay <= 32)";
     else if (b->sys->dev != batches[0]->sys->dev) why = "batches of one device";
@@ -2529,6 +2530,8 @@ int qmcx_group_run_sweeps(qmcx_group_t g, int nsweeps, uint64_t seed, double tau
     if (b->rows64 && b->rows64_stale)
       return fail(QMCX_ERR_STATE, "FP64 Slater rows were switched on after qmcx_evaluate_log: call it again");
     if (int rc = check_j3(b)) return rc;
+    if (b->cadence > 0 || b->streamed())
+      return fail(QMCX_ERR_STATE, "a group batch took the recompute cadence or the streamed delay form");
     if (int rc = flush_pending(b)) return rc;
     if (nsweeps > b->hist_cap) {
       if (b->hist) hipFree(b->hist);
