@@ -2174,33 +2174,6 @@ int qmcx_last_sweep_ms(qmcx_batch_t b, double* ms) {
 
 }  // extern "C"
 
-#ifdef QMCX_PHASE_TIMING
-// Profiling builds only: per-phase wall_clock64 ticks (walker / matrix 0, lane 0); slots 0-7
-// and 12-22 the move kernel's phases, 8-11 the FP64 inverse's transpose, panels, updates and output,
-// 24-29 the flush's row loads, staging, products and stores.
-extern "C" int qmcx_debug_walker_ticks(double* out, int reset) {
-  static unsigned long long t[128 * 256];
-  HIPCHK(hipDeviceSynchronize());
-  HIPCHK(hipMemcpyFromSymbol(t, HIP_SYMBOL(qmcx_walker_ticks), sizeof(t)));
-  for (int i = 0; i < 128 * 256; ++i) out[i] = (double)t[i];
-  if (reset) {
-    static unsigned long long z[128 * 256];
-    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(qmcx_walker_ticks), z, sizeof(z)));
-  }
-  return 0;
-}
-extern "C" int qmcx_debug_phase_ticks(double* out32, int reset) {
-  unsigned long long t[qmcx::kPhaseSlots];
-  HIPCHK(hipDeviceSynchronize());
-  HIPCHK(hipMemcpyFromSymbol(t, HIP_SYMBOL(qmcx_phase_ticks), sizeof(t)));
-  for (int i = 0; i < qmcx::kPhaseSlots; ++i) out32[i] = (double)t[i];
-  if (reset) {
-    unsigned long long z[qmcx::kPhaseSlots] = {};
-    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(qmcx_phase_ticks), z, sizeof(z)));
-  }
-  return 0;
-}
-#endif
 
 // ====================================================================== delayed-update engine (DU_TYPE)
 struct qmcx_du_s {

@@ -506,18 +506,6 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
   float* Ainv = B.Ainv + moff;
   const float* Amat = B.A + moff;
   const int vs = flush_v_stride(n);
-#ifdef QMCX_PHASE_TIMING
-  unsigned long long ftick_ = wall_clock64();
-  unsigned long long facc_[6] = {0, 0, 0, 0, 0, 0};
-#define FL_TICK(slot)                                      \
-  if (w == 0 && blockIdx.x == 0 && tid == 0) {             \
-    const unsigned long long now_ = wall_clock64();        \
-    facc_[(slot) - 24] += now_ - ftick_;                   \
-    ftick_ = now_;                                         \
-  }
-#else
-#define FL_TICK(slot)
-#endif
   // this wave's rows into registers (row clamped into the range, zeroed after)
   const int row = r0 + li;
   const bool rok = row < FR.hi;
@@ -530,7 +518,6 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
   for (int ct = 0; ct < kFlushTiles; ++ct)
     if (!rok) a[ct] = f32x4_t{0.f, 0.f, 0.f, 0.f};
   bool changed = false;
-  FL_TICK(24)
   for (int j = j0; j < j1; ++j) {
     const size_t sl = state_slot(B, w, det, j);
     const int c = B.ndelay[sl];
@@ -561,7 +548,6 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
       }
     }
     __syncthreads();
-    FL_TICK(25)
     // ---- GEMM 1: T^T tile by tile of 16 delayed rows; two accumulators split the orbitals.
     // U rows from LDS (staged) or HBM, in separate loops so that each is its own kind of load.
     auto gemm1 = [&](const float* ur, int dt, int d) {
@@ -593,7 +579,6 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
       else gemm1(Amat + (size_t)(d < c ? sRows[d] : sRows[0]) * n + 4 * g, dt, d);
     }
     wave_lds_fence();   // T and T2 are this wave's own
-    FL_TICK(26)
     // ---- T2 = -(T Binv), FP64 products of the small capacitance inverse
     if constexpr (OVER) {
       // over T: every lane's entries into registers first (cp <= kFlushT2Regs)
@@ -633,7 +618,6 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
       sT2[i * KC + dd] = (float)(-sum);
     }
     wave_lds_fence();
-    FL_TICK(27)
     // ---- GEMM 2: rows^T += V^T T2^T over the delayed rows, four at a time
     for (int d0 = 0; d0 < cp; d0 += 4) {
       const int dk = d0 + g;
@@ -648,7 +632,6 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
         }
       }
     }
-    FL_TICK(28)
   }
   if (changed && FR.writes(row)) {
     float* aw = Ainv + (size_t)row * n + 4 * g;
@@ -656,12 +639,6 @@ __global__ __launch_bounds__(64 * kFlushWaves) void k_flush_mfma(SysP<float> S, 
     for (int ct = 0; ct < kFlushTiles; ++ct)
       if (ct < nct) *reinterpret_cast<f32x4_t*>(aw + ct * 16) = a[ct];
   }
-  FL_TICK(29)
-#ifdef QMCX_PHASE_TIMING
-  if (w == 0 && blockIdx.x == 0 && tid == 0)
-    for (int i = 0; i < 6; ++i) qmcx_phase_ticks[24 + i] += facc_[i];
-#endif
-#undef FL_TICK
 }
 
 // ------------------------------------------------------------------ FP64 inverse (recompute)
@@ -736,17 +713,6 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
   const size_t mo = (size_t)mat * mat_stride;   // matrix slot (every mat_stride-th matrix)
   double* X = ws + mo * 2 * n * n;
   const TI* a = A + mo * n * n;
-#ifdef QMCX_PHASE_TIMING
-  unsigned long long gtick_ = wall_clock64();
-#define GJ_TICK(slot)                                      \
-  if (mat == 0 && tid == 0) {                              \
-    const unsigned long long now_ = wall_clock64();        \
-    qmcx_phase_ticks[slot] += now_ - gtick_;               \
-    gtick_ = now_;                                         \
-  }
-#else
-#define GJ_TICK(slot)
-#endif
 
   // X = A^T, b rows of A at a time through LDS (the panel buffer is free yet;
   // b (n + 1) <= n (b + 1) doubles)
@@ -780,7 +746,6 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
   double pmant = 1.0;
   int pexp = 0;
   int sign = 1;
-  GJ_TICK(8)
   for (int k0 = 0; k0 < n; k0 += b) {
     const int bb = min(b, n - k0);
     // ---- panel: columns [k0, k0 + bb); thread i holds row i in registers
@@ -875,7 +840,6 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
       for (int c = 0; c < B; ++c)
         if (c < bb) P[tid * ps + c] = pr[c];
     __syncthreads();
-    GJ_TICK(9)
     // ---- panel back to X; row sources of the panel's swaps; rows the swaps displaced
     for (int e = tid; e < n * bb; e += nt) {
       const int i = e / bb, j = e % bb;
@@ -1005,7 +969,6 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
       }
       __syncthreads();
     }
-    GJ_TICK(10)
   }
   // ---- undo the pivoting: columns swapped back in reverse order, then out in TO
   if (tid == 0) {
@@ -1045,8 +1008,6 @@ __global__ __launch_bounds__(NT) void k_gj_invert(int n, const TI* A, TO* psiM, 
     }
     __syncthreads();
   }
-  GJ_TICK(11)
-#undef GJ_TICK
 }
 
 // host side: the panel width is a template argument (the panel row lives in registers)

@@ -148,28 +148,6 @@ __device__ inline void j2_pair(const Cell<T>& C, const Functor<T>& f, double xj,
 }
 
 
-#ifdef QMCX_PHASE_TIMING
-// wall_clock64 ticks per phase, taken by lane 0 of workgroup 0 into registers and added to
-// this table when the kernel ends (a global read-modify-write per tick would time itself)
-constexpr int kPhaseSlots = 32;
-__device__ unsigned long long qmcx_phase_ticks[kPhaseSlots];
-// per-walker duration of each chunk launch: [det * 64 + chunk][walker < 256]
-__device__ unsigned long long qmcx_walker_ticks[128 * 256];
-#define QMCX_TICK(slot)                                                  \
-  if (w == 0 && tid == 0) {                                              \
-    const unsigned long long now_ = wall_clock64();                      \
-    tacc_[slot] += now_ - tick_;                                         \
-    tick_ = now_;                                                        \
-  }
-#define QMCX_TICK_FLUSH()                                                \
-  if (w == 0 && tid == 0)                                                \
-    for (int i_ = 0; i_ < kPhaseSlots; ++i_) qmcx_phase_ticks[i_] += tacc_[i_]; \
-  if (tid == 0 && w < 256 && a.chunk < 64)                               \
-    qmcx_walker_ticks[(a.det * 64 + a.chunk) * 256 + w] += wall_clock64() - wstart_;
-#else
-#define QMCX_TICK(slot)
-#define QMCX_TICK_FLUSH()
-#endif
 
 // J2 pair terms of the items j = tid + q*nt against the electron iel at (x, y, z); items
 // past N and j == iel give zeros.  Identical arithmetic to j2_pair for every valid item; the
@@ -282,17 +260,6 @@ __device__ __attribute__((noinline)) void flush_own_tile(int n, int K, int KP, i
   const int li = lane & 15, g = lane >> 4;
   const int KC = (K + 15) & ~15, c4 = (c + 3) & ~3, cp = (c + 15) & ~15, nct = n >> 4, us = n + 4;
   float* sT = scratch + (size_t)wid * 16 * KC;
-#ifdef QMCX_PHASE_TIMING
-  unsigned long long ft_ = wall_clock64(), fa_[6] = {};
-#define FT_TICK(i)                                        \
-  if (blockIdx.x == 0 && tid == 0) {                      \
-    const unsigned long long now_ = wall_clock64();       \
-    fa_[i] += now_ - ft_;                                 \
-    ft_ = now_;                                           \
-  }
-#else
-#define FT_TICK(i)
-#endif
   for (int tile = wid; tile < nct; tile += nw) {
     const int r0 = tile * 16;
     float* ar = Ainv + (size_t)(r0 + li) * n + 4 * g;
@@ -300,7 +267,6 @@ __device__ __attribute__((noinline)) void flush_own_tile(int n, int K, int KP, i
 #pragma unroll
     for (int q = 0; q < kFlushOwnTiles; ++q)
       a[q] = *reinterpret_cast<const f32x4_t*>(ar + min(q, nct - 1) * 16);
-    FT_TICK(0)
     // ---- GEMM 1: T^T = U rows^T over all orbitals, 16 delayed rows per pass, two accumulators
     for (int dt = 0; dt < cp; dt += 16) {
       const int d = dt + li;
@@ -329,9 +295,7 @@ __device__ __attribute__((noinline)) void flush_own_tile(int n, int K, int KP, i
       }
     }
     wave_lds_fence();
-    FT_TICK(2)
     flush_own_t2_mfma(c, cp, KC, KP, sBinv, sT);
-    FT_TICK(3)
     // ---- GEMM 2: rows^T += V^T T2^T, four delayed rows per step; V rows from LDS
     for (int d0 = 0; d0 < c4; d0 += 4) {
       const int dk = d0 + g;
@@ -344,20 +308,12 @@ __device__ __attribute__((noinline)) void flush_own_tile(int n, int K, int KP, i
           a[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(v, bv, a[q], 0, 0, 0);
         }
     }
-    FT_TICK(4)
 #pragma unroll
     for (int q = 0; q < kFlushOwnTiles; ++q)
       if (q < nct) *reinterpret_cast<f32x4_t*>(ar + q * 16) = a[q];
     wave_lds_fence();   // the next tile's T overwrites this one's T2
-    FT_TICK(5)
   }
   __syncthreads();   // the next chunk reads these rows
-#ifdef QMCX_PHASE_TIMING
-  FT_TICK(1)
-  if (blockIdx.x == 0 && tid == 0)
-    for (int i = 0; i < 6; ++i) qmcx_phase_ticks[24 + i] += fa_[i];
-#endif
-#undef FT_TICK
 }
 
 // The same update when more than kFlushOwnU moves of the chunk were accepted: a tile is held
@@ -724,11 +680,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
   // FP64 re-evaluation of near ties (FP32 form, Metropolis): the move's inverse row in FP64
   const bool arb_on = ARB && sizeof(T) == 4 && KD && a.band > 0.0 && a.mode == 1 && a.decide == nullptr && n <= nt;
   double* sArbX = (double*)(smem + L.oArb);
-#ifdef QMCX_PHASE_TIMING
-  unsigned long long tick_ = wall_clock64();
-  const unsigned long long wstart_ = tick_;
-  unsigned long long tacc_[kPhaseSlots] = {};
-#endif
   const size_t moff = ((size_t)w * 2 + a.det) * n * n;
   T* Ainv = B.Ainv + moff;
   T* Amat = B.A + moff;
@@ -817,8 +768,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
       sJ3ne[i] = ge[i];
     }
   }
-  QMCX_TICK(16)
-  QMCX_TICK(17)
   int c = 0, ca = 0;   // delayed rows; accepted moves of the chunk
   // streamed form: the slot's capacitance inverse (row stride K, in LDS while the chunk runs when
   // it fits) and the c0 delayed rows pending from earlier launches (a.resume)
@@ -855,7 +804,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
 
   for (int e = 0; e < Eend; ++e) {
     const int iel = cb + e, row = r0 + e;
-    QMCX_TICK(0)
     // ---- phase 1: issue the next move's global loads, then getInvRow's small algebra
     // stored state of the moving electron (J1, J2), its dpsiM row (loaded one move ahead) and
     // its proposal record: J1 at the proposal (sum u, sum (d2u + 2 du/r), gradient),
@@ -927,7 +875,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
       }
     }
     lds_sync();
-    QMCX_TICK(1)
 
     // ---- phase 2: invRow, evalGrad of the determinant and ratioGrad_compute against the
     // proposal's SPO row (einspline_spo.hpp:204-223, DiracDeterminant.cpp:103-124), J2 at
@@ -1005,7 +952,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
       d2 += vt * (double)o5[2];
       d3 += vt * (double)o5[3];
     }
-    QMCX_TICK(5)
     {
       // pair terms at the proposal: u, du dx, du dy, du dz, d2u + 2 du, parked in LDS for the
       // J2 update if the move is accepted
@@ -1029,7 +975,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
           sJt[4 * N + j] = t4;
         }
       }
-      QMCX_TICK(6)
       // slots: gnow 0-2, det ratio 3, its gradient 4-6, J2 7-11; after the row-wise wave sums
       // v[i] holds slot i + 3r in the lanes of row r, which lane 16r + i stores
       double v[kRed] = {gd0, gd1, gd2, d0, d1, d2, d3, r7, r8, r9, r10, r11};
@@ -1041,7 +986,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
         for (int i = 1; i < kRed / 4; ++i) x = k == i ? v[i] : x;
         sRed[(k + (lane >> 4) * (kRed / 4)) * 16 + wid] = x;
       }
-      QMCX_TICK(7)
     }
     int ns3 = 0;   // this wave's J3 pair slots at the proposal
     if constexpr (J3) {
@@ -1055,7 +999,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
       }
     }
     lds_sync();
-    QMCX_TICK(2)
 
     double red[kRed];
     combine_partials(sRed, nw, red);
@@ -1065,7 +1008,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
       for (int q = 0; q < nw; ++q)
 #pragma unroll
         for (int c = 0; c < 5; ++c) cur3[c] += sJr[q * 5 + c];
-    QMCX_TICK(3)
 
     // ---- phase 4: acceptance and updates
     double rdet = red[3];
@@ -1129,7 +1071,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
         sJ1n[ca * 5 + 4] = (T)pr[kPropJ1 + 1];
       }
       ++ca;
-      QMCX_TICK(12)
       if (!KD) {
         // Fahy/Sherman-Morrison (DelayedUpdate.h:166-173):
         // Ainv[m] -= ((Ainv[m].psiV - [m == row]) / ratio) * Ainv_old[row]
@@ -1210,7 +1151,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
           }
         }
         lds_sync();
-        QMCX_TICK(13)
         if constexpr (ST) {
           // the capacitance inverse grows by one row and column with every wave: acc_s = Binv[s] . p
           // (s < c) and y . p (s = c), a wave per row; then Binv[s][r] -= y_r yi acc_s,
@@ -1269,7 +1209,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
           }
         }
         ++c;
-        QMCX_TICK(14)
       }
       // J2 (TwoBodyJastrow.h:307-353): the pair terms at the old position are recomputed here,
       // those at the accepted one are phase 2's
@@ -1320,7 +1259,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
           sMask[(size_t)iel * S.mw + tid] = m;
         }
       }
-      QMCX_TICK(15)
     }
     // with K = 1 the next move reads rows of Ainv that other waves just wrote
     if (!KD) __syncthreads();
@@ -1329,7 +1267,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
       stopped = true;   // the recompute follows this launch
       Eend = e + 1;
     }
-    QMCX_TICK(4)
   }
 
   // ---- chunk end: the accepted rows' Slater matrix, gradient and Laplacian rows
@@ -1429,7 +1366,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
     }
   if (KD && !a.inflush)
     for (int i = tid; i < c; i += nt) B.rows[slot * K + i] = sRows[i];
-  QMCX_TICK(22)
   if constexpr (KD && sizeof(T) == 4) {
     // a stopped walker's inverse is rebuilt by the recompute instead
     if (a.inflush && c > 0 && !stopped && (cb + E < a.e_end || a.inflush == 2)) {
@@ -1445,7 +1381,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
         flush_own(n, K, KP, c, sRows, sAl, sBinv, (const float*)sAc, sa, (const float*)Amat, (float*)Ainv, scratch);
     }
   }
-  QMCX_TICK(23)
   cbn = cb + Eend;
   }   // chunks
   if (a.cadence > 0 && tid == 0) {
@@ -1472,7 +1407,6 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
     B.nacc[w] += (unsigned long long)sAcc[aNacc];
     if (B.narb != nullptr) B.narb[w] += (unsigned long long)sAcc[aNarb];
   }
-  QMCX_TICK_FLUSH()
 }
 
 template <typename T, int JPT, bool KD, bool ARB, bool J3, bool ST>
