@@ -18,9 +18,13 @@ def _say(*a):
     print(*a, flush=True)
 
 
-@pytest.mark.parametrize("name,prec,delay,recompute", [("nio8-cubic", qmcx.FP32, 8, 1), ("nio8-cubic", qmcx.FP64, 1, 0),
-                                                       ("nio32", qmcx.FP32, 32, 1)])
-def test_graph_replays_equal_launches(name, prec, delay, recompute):
+@pytest.mark.parametrize("name,prec,delay,recompute,form,rows", [
+    ("nio8-cubic", qmcx.FP32, 8, 1, 0, 0), ("nio8-cubic", qmcx.FP64, 1, 0, 0, 0), ("nio32", qmcx.FP32, 32, 1, 0, 0),
+    # the streamed delayed-update form (move and flush launches per chunk) and the FP64 recompute
+    # cadence (rounds of move launches and inversions of the stopped walkers, device-side positions)
+    ("nio8-cubic", qmcx.FP32, 16, 1, 1, 0), ("nio8-cubic", qmcx.FP64, 8, 0, 1, 0),
+    ("nio8-cubic", qmcx.FP32, 8, 0, 0, 6), ("nio32", qmcx.FP32, 32, 0, 0, 16)])
+def test_graph_replays_equal_launches(name, prec, delay, recompute, form, rows):
     sysd = systems.make_system(name)
     dev = qmcx.System(sysd)
     nw, seed, tau, ns = 4, 11, 0.3, 3
@@ -29,10 +33,12 @@ def test_graph_replays_equal_launches(name, prec, delay, recompute):
     graph = qmcx.WalkerBatch(dev, nw, delay_rank=delay, precision=prec)
     graph.set_graph(True)
     for b in (eager, graph):
+        b.set_delay_form(form)
+        b.set_recompute_rows(rows)
         b.set_positions(R0)
         b.evaluate_log()
     for call in range(3):
-        _say(f"{name} prec {prec} delay {delay}: run {call}")
+        _say(f"{name} prec {prec} delay {delay} form {form} rows {rows}: run {call}")
         he = eager.run(ns, seed, tau, qmcx.ACCEPT_METROPOLIS, recompute_every=recompute)
         hg = graph.run(ns, seed, tau, qmcx.ACCEPT_METROPOLIS, recompute_every=recompute)
         np.testing.assert_array_equal(hg, he)
@@ -45,7 +51,7 @@ def test_graph_replays_equal_launches(name, prec, delay, recompute):
         # eager work on the graph batch between replays
         np.testing.assert_allclose(graph.evaluate_log(), lg, rtol=1e-5)
         eager.evaluate_log()
-    if prec == qmcx.FP64:
+    if prec == qmcx.FP64 and delay == 1:
         # and the chain is the oracle's: 9 sweeps from R0
         ref = po.OracleSystem(sysd).run_vmc(R0, 3 * ns, seed, tau, mode=qmcx.ACCEPT_METROPOLIS, delay=1)
         np.testing.assert_allclose(graph.positions(), ref["R"], rtol=0, atol=1e-11)
