@@ -412,6 +412,28 @@ def measure_nio32(mode, steps, err_steps, fp64_steps, seed, tau, walkers=256, de
     return out
 
 
+def measure_single_walker(seed, tau, mode):
+    """The two one-walker configurations: the plain FP64 delay-1 drop-in (Sherman-Morrison, one
+    workgroup) on the CPU-runnable NiO-8 cubic cell (96 e-, 5 sweeps, tau 0.5) and on NiO-32 (384 e-, 20
+    sweeps, tau 0.3): walker-moves/s over the sweeps."""
+    out = {}
+    for name, nsw, t in (("nio8-cubic", 5, 0.5), ("nio32", 20, tau)):
+        sysd = systems.make_system(name, seed=seed)
+        dev = qmcx.System(sysd, precisions=(qmcx.FP64,))
+        b = qmcx.WalkerBatch(dev, 1, delay_rank=1, precision=qmcx.FP64)
+        b.set_positions(systems.initial_positions(sysd, 1, seed=seed))
+        b.evaluate_log()
+        b.run(1, seed, t, mode)
+        t0 = time.perf_counter()
+        b.run(nsw, seed, t, mode)
+        wall = time.perf_counter() - t0
+        b.close()
+        out[name] = {"value": sysd.nel * nsw / wall, "unit": "walker-moves/s", "sweeps": nsw, "tau": t,
+                     "ms_per_step": 1e3 * wall / nsw, "form": "FP64, delay rank 1, one walker"}
+        log(f"{name} one walker, FP64 delay 1: {sysd.nel * nsw / wall:.4e} moves/s ({1e3 * wall / nsw:.2f} ms/sweep)")
+    return out
+
+
 def measure_evaluate_log(dev, sysd, R0, prec, steps):
     """WaveFunction::evaluateLog from scratch for the whole batch (SPO rows at every electron, FP64 LU,
     J1/J2 over all pairs, evaluateGL): walkers per second."""
@@ -575,6 +597,7 @@ def main():
     nio32 = None
     if args.nio32_steps > 0:
         nio32 = measure_nio32(mode, args.nio32_steps, args.err_steps, 2, args.seed, args.tau)
+        nio32["one_walker_configs"] = measure_single_walker(args.seed, args.tau, mode)
 
     cpu = None
     if not args.no_cpu:
