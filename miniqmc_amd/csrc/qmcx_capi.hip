@@ -177,7 +177,15 @@ Functor<T> make_functor(int np, double rcut, double cusp, const double* p) {
   f.rcut = (T)rcut;
   f.drinv = (T)(1.0 / dr);
   f.ncoef = np + 4;
-  for (int i = 0; i < np + 4; ++i) f.coef[i] = (T)c[i];
+  // interval i (r / dr in [i, i + 1), i <= np): the power form of the cubic B-spline basis over
+  // c_i .. c_i+3 (BsplineFunctor.h:240-300 evaluates the same spline)
+  for (int i = 0; i <= np; ++i) {
+    const double c0 = c[i], c1 = c[i + 1], c2 = c[i + 2], c3 = c[i + 3];
+    f.poly[i][0] = (T)((c0 + 4.0 * c1 + c2) / 6.0);
+    f.poly[i][1] = (T)(0.5 * (c2 - c0));
+    f.poly[i][2] = (T)(0.5 * (c0 - 2.0 * c1 + c2));
+    f.poly[i][3] = (T)((-c0 + 3.0 * c1 - 3.0 * c2 + c3) / 6.0);
+  }
   return f;
 }
 

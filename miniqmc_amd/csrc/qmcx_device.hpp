@@ -238,31 +238,40 @@ __device__ inline void block_sum(T (&v)[NV], T* scratch) {
 }
 
 // ------------------------------------------------------------------ radial functor
-// Uniform cubic B-spline u(r) with cutoff (BsplineFunctor.h:93-339):
-// coef[0] = p1 - 2 dr cusp, coef[1..] = p0, p1, ..., three trailing zeros.
+// Uniform cubic B-spline u(r) with cutoff (BsplineFunctor.h:93-339): spline coefficients
+// c[0] = p1 - 2 dr cusp, c[1..] = p0, p1, ..., three trailing zeros.  Kept as a cubic per knot
+// interval: u = ((a3 t + a2) t + a1) t + a0 with t the fraction in interval i and (a0 .. a3) =
+// poly[i], the power form of the basis over c_i .. c_i+3 computed on the host in FP64; one 16-byte
+// (FP32) read per evaluation.
 template <typename T>
 struct Functor {
   T rcut, drinv;
   int ncoef;
   int pad;
-  T coef[kMaxFunctorCoefs];
+  alignas(16) T poly[kMaxFunctorCoefs - 3][4];
 };
+
+template <typename T>
+__device__ inline void functor_poly(const Functor<T>& f, T r, T& t, T& a0, T& a1, T& a2, T& a3) {
+  const T x = r * f.drinv;
+  const int i = (int)x;
+  t = x - (T)i;
+  if constexpr (sizeof(T) == 4) {
+    const float4 p = *reinterpret_cast<const float4*>(f.poly[i]);
+    a0 = p.x, a1 = p.y, a2 = p.z, a3 = p.w;
+  } else {
+    a0 = f.poly[i][0], a1 = f.poly[i][1], a2 = f.poly[i][2], a3 = f.poly[i][3];
+  }
+}
 
 // u, (du/dr)/r and d2u/dr2; caller guarantees r < rcut.
 template <typename T>
 __device__ inline void functor_vgl(const Functor<T>& f, T r, T& u, T& du, T& d2u) {
-  const T x = r * f.drinv;
-  const int i = (int)x;
-  const T t = x - (T)i;
-  const T t2 = t * t, t3 = t2 * t;
-  const T c0 = f.coef[i], c1 = f.coef[i + 1], c2 = f.coef[i + 2], c3 = f.coef[i + 3];
-  const T s6 = T(1) / T(6);
-  // power-form rows of the cubic B-spline basis
-  u = c0 * ((-t3 + T(3) * t2 - T(3) * t + T(1)) * s6) + c1 * ((T(3) * t3 - T(6) * t2 + T(4)) * s6) +
-      c2 * ((-T(3) * t3 + T(3) * t2 + T(3) * t + T(1)) * s6) + c3 * (t3 * s6);
-  const T d = c0 * (T(-0.5) * t2 + t - T(0.5)) + c1 * (T(1.5) * t2 - T(2) * t) +
-              c2 * (T(-1.5) * t2 + t + T(0.5)) + c3 * (T(0.5) * t2);
-  const T dd = c0 * (T(1) - t) + c1 * (T(3) * t - T(2)) + c2 * (T(1) - T(3) * t) + c3 * t;
+  T t, a0, a1, a2, a3;
+  functor_poly(f, r, t, a0, a1, a2, a3);
+  u = ((a3 * t + a2) * t + a1) * t + a0;
+  const T d = (T(3) * a3 * t + T(2) * a2) * t + a1;   // du/dt
+  const T dd = T(6) * a3 * t + T(2) * a2;              // d2u/dt2
   if constexpr (sizeof(T) == 4)
     du = f.drinv * d * __builtin_amdgcn_rcpf(r);   // v_rcp_f32 (1 ulp) in the FP32 form
   else
@@ -272,14 +281,9 @@ __device__ inline void functor_vgl(const Functor<T>& f, T r, T& u, T& du, T& d2u
 
 template <typename T>
 __device__ inline T functor_v(const Functor<T>& f, T r) {
-  const T x = r * f.drinv;
-  const int i = (int)x;
-  const T t = x - (T)i;
-  const T t2 = t * t, t3 = t2 * t;
-  const T s6 = T(1) / T(6);
-  return f.coef[i] * ((-t3 + T(3) * t2 - T(3) * t + T(1)) * s6) +
-         f.coef[i + 1] * ((T(3) * t3 - T(6) * t2 + T(4)) * s6) +
-         f.coef[i + 2] * ((-T(3) * t3 + T(3) * t2 + T(3) * t + T(1)) * s6) + f.coef[i + 3] * (t3 * s6);
+  T t, a0, a1, a2, a3;
+  functor_poly(f, r, t, a0, a1, a2, a3);
+  return ((a3 * t + a2) * t + a1) * t + a0;
 }
 
 // ------------------------------------------------------------------ minimum image
