@@ -5,7 +5,9 @@
 // quantity.  The reference implementation here is the FP64 oracle (oracle/qmc_oracle.cpp, linked as
 // liboracle.so).  The system is a NiO-8-like cubic cell built here from a seeded generator (4 "Ni",
 // 4 "O", 96 electrons, 48 orbitals on a 24^3 periodic B-spline table, J1 per species, J2 per spin
-// pair).  Exit status 0 when every error is within its stated tolerance.
+// pair, and in the last cases a three-body eeI Jastrow per ion species and spin pair, the components
+// check_wfc takes with -f Det, J1, J2, J3).  Exit status 0 when every error is within its stated
+// tolerance.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -20,6 +22,8 @@ void* orc_system_create(const double* lattice9, int nions, const double* ion_pos
 void orc_system_destroy(void* h);
 void orc_system_set_j1(void* h, int species, int nparams, double rcut, double cusp, const double* params);
 void orc_system_set_j2(void* h, int pair, int nparams, double rcut, double cusp, const double* params);
+int orc_system_set_j3(void* h, int species, int pair, int neI, int nee, double rcut, const double* params,
+                      int nparams);
 void orc_system_set_spline(void* h, const int* ng, int norb, int npad, const float* coefs, const double* spo_lattice9,
                            int prec);
 void orc_spo_evaluate(void* h, int npts, const double* r, double* psi, double* dpsi, double* d2psi);
@@ -77,6 +81,10 @@ void report(const char* what, double err, double tol) {
 }
 
 constexpr double kA = 7.9;
+// three-body (eeI) Jastrow: the reference's functor shape (3, 3) with its 26 free parameters, cutoff the
+// cell's Wigner-Seitz radius (ThreeBodyJastrow.h:196-218, PolynomialFunctor3D.h:65-260)
+constexpr int kJ3NeI = 3, kJ3Nee = 3, kJ3Params = 26;
+constexpr double kJ3Rcut = 0.5 * kA;
 constexpr int kNup = 48, kN = 96, kNorb = 48, kNpad = 48, kG = 24;
 
 struct SystemData {
@@ -85,6 +93,7 @@ struct SystemData {
   std::vector<int> species;
   std::vector<float> coefs;
   std::vector<double> j1[2], j2[2];
+  std::vector<double> j3[2][2];   // three-body functor parameters per (ion species, spin pair)
 };
 
 SystemData make_data(Rng& rng) {
@@ -110,12 +119,15 @@ SystemData make_data(Rng& rng) {
     for (int p = 0; p < 8; ++p) d.j1[s].push_back(p == 7 ? 0.0 : 0.5 * rng.uniform());
     for (int p = 0; p < 8; ++p) d.j2[s].push_back(p == 7 ? 0.0 : 0.5 * rng.uniform());
   }
+  for (int s = 0; s < 2; ++s)
+    for (int pr = 0; pr < 2; ++pr)
+      for (int p = 0; p < kJ3Params; ++p) d.j3[s][pr].push_back(0.04 * (rng.uniform() - 0.5));
   return d;
 }
 
 const double kRcut = 3.9, kCusp2[2] = {-0.25, -0.5};
 
-void setup_qmcx(qmcx_system_t& sys, const SystemData& d, bool jastrows) {
+void setup_qmcx(qmcx_system_t& sys, const SystemData& d, bool jastrows, bool j3 = false) {
   qmcx::check(qmcx_system_create(&sys, d.lattice, 8, d.ions.data(), d.species.data(), 2, kNup, kNup),
               "qmcx_system_create");
   const int ng[3] = {kG, kG, kG};
@@ -125,9 +137,14 @@ void setup_qmcx(qmcx_system_t& sys, const SystemData& d, bool jastrows) {
       qmcx::check(qmcx_system_set_j1(sys, s, 8, kRcut, 0.0, d.j1[s].data()), "qmcx_system_set_j1");
       qmcx::check(qmcx_system_set_j2(sys, s, 8, kRcut, kCusp2[s], d.j2[s].data()), "qmcx_system_set_j2");
     }
+  if (j3)
+    for (int s = 0; s < 2; ++s)
+      for (int pr = 0; pr < 2; ++pr)
+        qmcx::check(qmcx_system_set_j3(sys, s, pr, kJ3NeI, kJ3Nee, kJ3Rcut, d.j3[s][pr].data(), kJ3Params),
+                    "qmcx_system_set_j3");
 }
 
-void* setup_oracle(const SystemData& d, bool jastrows) {
+void* setup_oracle(const SystemData& d, bool jastrows, bool j3 = false) {
   void* h = orc_system_create(d.lattice, 8, d.ions.data(), d.species.data(), 2, kNup, kNup);
   const int ng[3] = {kG, kG, kG};
   orc_system_set_spline(h, ng, kNorb, kNpad, d.coefs.data(), d.lattice, 8);
@@ -136,6 +153,11 @@ void* setup_oracle(const SystemData& d, bool jastrows) {
       orc_system_set_j1(h, s, 8, kRcut, 0.0, d.j1[s].data());
       orc_system_set_j2(h, s, 8, kRcut, kCusp2[s], d.j2[s].data());
     }
+  if (j3)
+    for (int s = 0; s < 2; ++s)
+      for (int pr = 0; pr < 2; ++pr)
+        if (orc_system_set_j3(h, s, pr, kJ3NeI, kJ3Nee, kJ3Rcut, d.j3[s][pr].data(), kJ3Params) < 0)
+          throw std::runtime_error("orc_system_set_j3");
   return h;
 }
 
@@ -374,11 +396,13 @@ void check_wfc(qmcx_system_t sys, void* orc, const char* label, int prec, int de
 int main() {
   Rng rng{2024};
   const SystemData d = make_data(rng);
-  qmcx_system_t sys = nullptr, sys_det = nullptr;
+  qmcx_system_t sys = nullptr, sys_det = nullptr, sys_j3 = nullptr;
   setup_qmcx(sys, d, true);
   setup_qmcx(sys_det, d, false);
+  setup_qmcx(sys_j3, d, true, true);
   void* orc = setup_oracle(d, true);
   void* orc_det = setup_oracle(d, false);
+  void* orc_j3 = setup_oracle(d, true, true);
   try {
     check_spo<double>(sys, orc, rng, 1e-12);
     check_spo<float>(sys, orc, rng, 2e-5);
@@ -388,14 +412,19 @@ int main() {
     check_wfc(sys, orc, "Det+J1+J2 FP64 delay 1", QMCX_FP64, 1, 1e-9, 1e-11);
     check_wfc(sys, orc, "Det+J1+J2 FP64 delay 4", QMCX_FP64, 4, 1e-9, 1e-11);
     check_wfc(sys, orc, "Det+J1+J2 FP32 delay 8", QMCX_FP32, 8, 2e-3, 1e-6);
+    check_wfc(sys_j3, orc_j3, "Det+J1+J2+J3 FP64 delay 1", QMCX_FP64, 1, 1e-9, 1e-11);
+    check_wfc(sys_j3, orc_j3, "Det+J1+J2+J3 FP64 delay 4", QMCX_FP64, 4, 1e-9, 1e-11);
+    check_wfc(sys_j3, orc_j3, "Det+J1+J2+J3 FP32 delay 8", QMCX_FP32, 8, 2e-3, 1e-6);
   } catch (const std::exception& e) {
     std::printf("error: %s\n", e.what());
     ++failures;
   }
   orc_system_destroy(orc);
   orc_system_destroy(orc_det);
+  orc_system_destroy(orc_j3);
   qmcx_system_destroy(sys);
   qmcx_system_destroy(sys_det);
+  qmcx_system_destroy(sys_j3);
   if (failures) {
     std::printf("%d checks FAILED\n", failures);
     return 1;

@@ -117,26 +117,29 @@ def test_j3_sweep_fp32_vs_oracle(cubic_j3, delay):
     _run_compare(sysd, dev, orc, 3, 2, delay, qmcx.FP32, qmcx.ACCEPT_METROPOLIS, 0.5, 11, 2e-3, 1e-6)
 
 
-def test_j3_flex_loop_vs_oracle(cubic_j3):
-    """The flex_* stage calls with caller displacements retrace the oracle's sweep with J3."""
+@pytest.mark.parametrize("prec,delay,tol,tol_log", [(qmcx.FP64, 1, 1e-9, 1e-11), (qmcx.FP64, 4, 1e-9, 1e-11),
+                                                     (qmcx.FP32, 4, 2e-3, 1e-6)])
+def test_j3_flex_loop_vs_oracle(cubic_j3, prec, delay, tol, tol_log):
+    """The flex_* stage calls with caller displacements retrace the oracle's sweep with J3 (delay rank 4:
+    accepted rows pending across the calls in the streamed form)."""
     sysd, dev, orc = cubic_j3
     nw, seed, tau = 2, 11, 0.5
     R0 = systems.initial_positions(sysd, nw, seed=seed)
     ref = orc.run_vmc(R0, 1, seed, tau, mode=qmcx.ACCEPT_METROPOLIS, delay=1, trace=True)
-    wb = qmcx.WalkerBatch(dev, nw, delay_rank=1, precision=qmcx.FP64)
+    wb = qmcx.WalkerBatch(dev, nw, delay_rank=delay, precision=prec)
     wb.set_positions(R0)
     wb.evaluate_log()
     for iel in range(sysd.nel):
         displ = np.array([np.sqrt(tau) * np.asarray(po.move_randoms(seed, w, 0, iel)[0]) for w in range(nw)])
         ratio, gnow, gnew = wb.flex_ratio_grad(iel, displ)
         rt = ref["trace"][:, 0, iel]
-        assert (np.abs(ratio - rt[:, 0]) / np.maximum(np.abs(rt[:, 0]), 1.0)).max() < 1e-9, iel
-        assert (np.abs(gnew - rt[:, 6:9]) / (1 + np.abs(rt[:, 6:9]))).max() < 1e-7, iel
-        assert (np.abs(gnow - rt[:, 3:6]) / (1 + np.abs(rt[:, 3:6]))).max() < 1e-7, iel
+        assert (np.abs(ratio - rt[:, 0]) / np.maximum(np.abs(rt[:, 0]), 1.0)).max() < tol, iel
+        assert (np.abs(gnew - rt[:, 6:9]) / (1 + np.abs(rt[:, 6:9]))).max() < 100 * tol, iel
+        assert (np.abs(gnow - rt[:, 3:6]) / (1 + np.abs(rt[:, 3:6]))).max() < 100 * tol, iel
         wb.flex_accept_restore(iel, rt[:, 2].astype(np.int32))
     wb.flex_complete_updates()
     log = wb.flex_evaluate_gl()
-    np.testing.assert_allclose(log, ref["log"][:, 0], rtol=1e-11)
+    np.testing.assert_allclose(log, ref["log"][:, 0], rtol=tol_log)
 
 
 def test_j3_nio64_fp32_vs_oracle():
