@@ -76,3 +76,26 @@ def test_flex_state_errors(cubic):
         wb.flex_accept_restore(0, np.ones(2, np.int32))   # no pending ratio_grad
     with pytest.raises(qmcx.QmcxError):
         wb.flex_ratio_grad(cubic.nel, np.zeros((2, 3)))     # electron out of range
+
+
+def test_flex_pending_rows_applied_before_other_calls(cubic):
+    """Delayed rows left pending by flex moves (delay rank 16, a dozen accepts) reach the inverse before
+    any call that reads it: the inverse read back, and a following sweep, equal those of a batch whose
+    pending rows were applied by flex_completeUpdates first (the same kernels, so bit for bit)."""
+    nw, seed, tau = 2, 5, 0.5
+    R0 = systems.initial_positions(cubic, nw, seed=seed)
+    dev = qmcx.System(cubic)
+    a = qmcx.WalkerBatch(dev, nw, delay_rank=16, precision=qmcx.FP64)
+    b = qmcx.WalkerBatch(dev, nw, delay_rank=16, precision=qmcx.FP64)
+    for wb in (a, b):
+        wb.set_positions(R0)
+        wb.evaluate_log()
+        for iel in range(12):
+            displ = np.array([np.sqrt(tau) * np.asarray(po.move_randoms(seed, w, 0, iel)[0]) for w in range(nw)])
+            wb.flex_ratio_grad(iel, displ)
+            wb.flex_accept_restore(iel, np.ones(nw, np.int32))
+    b.flex_complete_updates()
+    for w in range(nw):
+        np.testing.assert_array_equal(a.inverse(w, 0), b.inverse(w, 0))
+    np.testing.assert_array_equal(a.run(1, seed, tau, qmcx.ACCEPT_METROPOLIS), b.run(1, seed, tau, qmcx.ACCEPT_METROPOLIS))
+    np.testing.assert_array_equal(a.positions(), b.positions())
