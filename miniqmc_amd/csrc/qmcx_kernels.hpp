@@ -1270,16 +1270,13 @@ __device__ inline double virtual_ratio(const SysP<T>& S, const BatchP<T>& B, con
     for (int sl = 0; sl < slices; ++sl) s += part[(size_t)sl * S.npad + j];
     v[0] += (double)s * (double)arow[j];
   }
-  T dx, dy, dz, u, du, d2u;
+  T dx, dy, dz;   // values only
   for (int ci = threadIdx.x; ci < M; ci += blockDim.x) {
     const Functor<T>& f = S.fn[S.species[ci]];
     if (f.ncoef == 0) continue;
     const T r = min_image(C, (T)(S.ions[ci] - x), (T)(S.ions[M + ci] - y), (T)(S.ions[2 * M + ci] - z), T(-1), dx,
                           dy, dz);
-    if (r < f.rcut) {
-      functor_vgl(f, r, u, du, d2u);
-      v[1] += (double)u;
-    }
+    if (r < f.rcut) v[1] += (double)functor_v(f, r);
   }
   const double* R = B.R + (size_t)w * 3 * N;
   const bool up = iel < n;
@@ -1287,10 +1284,7 @@ __device__ inline double virtual_ratio(const SysP<T>& S, const BatchP<T>& B, con
     if (j == iel) continue;
     const Functor<T>& f = S.fn[kMaxSpecies + (((j < n) == up) ? 0 : 1)];
     const T r = min_image(C, (T)(R[j] - x), (T)(R[N + j] - y), (T)(R[2 * N + j] - z), T(-1), dx, dy, dz);
-    if (r < f.rcut) {
-      functor_vgl(f, r, u, du, d2u);
-      v[2] += (double)u;
-    }
+    if (r < f.rcut) v[2] += (double)functor_v(f, r);
   }
   block_sum<double, 3>(v, red);   // ends with a barrier: part and red are free again
   const size_t o = (size_t)w * 5 * N + iel;
@@ -1449,7 +1443,7 @@ __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<
       double v[3 * P];
 #pragma unroll
       for (int i = 0; i < 3 * P; ++i) v[i] = 0.0;
-      T dx, dy, dz, u, du, d2u;
+      T dx, dy, dz;   // the values only: each knot needs u, not its derivatives
       for (int ci = tid; ci < M; ci += nt) {
         const Functor<T>& f = S.fn[S.species[ci]];
         if (f.ncoef == 0) continue;
@@ -1458,10 +1452,7 @@ __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<
         for (int gg = 0; gg < P; ++gg) {
           const double px = d * sp[3 * (k0 + gg)] + ox, py = d * sp[3 * (k0 + gg) + 1] + oy, pz = d * sp[3 * (k0 + gg) + 2] + oz;
           const T r = min_image(C, (T)(ix - px), (T)(iy - py), (T)(iz - pz), T(-1), dx, dy, dz);
-          if (r < f.rcut) {
-            functor_vgl(f, r, u, du, d2u);
-            v[P + gg] += (double)u;
-          }
+          if (r < f.rcut) v[P + gg] += (double)functor_v(f, r);
         }
       }
       for (int j = tid; j < N; j += nt) {
@@ -1472,10 +1463,7 @@ __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<
         for (int gg = 0; gg < P; ++gg) {
           const double px = d * sp[3 * (k0 + gg)] + ox, py = d * sp[3 * (k0 + gg) + 1] + oy, pz = d * sp[3 * (k0 + gg) + 2] + oz;
           const T r = min_image(C, (T)(rx - px), (T)(ry - py), (T)(rz - pz), T(-1), dx, dy, dz);
-          if (r < f.rcut) {
-            functor_vgl(f, r, u, du, d2u);
-            v[2 * P + gg] += (double)u;
-          }
+          if (r < f.rcut) v[2 * P + gg] += (double)functor_v(f, r);
         }
       }
       __syncthreads();   // the P value rows are in LDS
