@@ -78,12 +78,16 @@ def test_invert_transpose_kat():
     assert np.abs(psiM[0] - b).max() < 1e-9
 
 
-@pytest.mark.parametrize("n", [4, 8, 24, 96])
-def test_invert_transpose_batched_vs_oracle(n):
-    """Regression (n < 32): the LDS row that a pivot swap parks must hold a whole panel row."""
-    _say("LU batched n =", n)
+@pytest.mark.parametrize("n", [4, 8, 24, 96, 384, 600])
+@pytest.mark.parametrize("ties", [False, True])
+def test_invert_transpose_batched_vs_oracle(n, ties):
+    """Regression (n < 32): the LDS row that a pivot swap parks must hold a whole panel row.  384 is
+    NiO-64's determinant (32-column panels), 600 takes 16-column panels; with ties the entries are
+    small integers, so most pivot searches meet equal magnitudes in several rows and waves (the
+    first such row must win, as idamax picks it) and the matrices are still almost surely regular."""
+    _say("LU batched n =", n, "ties", ties)
     rng = np.random.default_rng(7)
-    a = rng.uniform(-1, 1, size=(5, n, n))
+    a = rng.integers(-3, 4, size=(5, n, n)).astype(np.float64) if ties else rng.uniform(-1, 1, size=(5, n, n))
     psiM, ld, ph = qmcx.invert_transpose(a)
     for m in range(5):
         ref, rld, rph = po.invert_transpose(a[m])
