@@ -10,7 +10,10 @@ At the end it prints a flat timer table and the reference's three throughput lin
 (resources/hardware, timing, particles, run/driver; miniqmc.cpp:525-551).
 
 Timers carry the reference's names (miniqmc.cpp:137-147, WaveFunction.cpp:38-41) and its flat-profile
-line format (TimerManagerClass::print_flat, NewTimer.cpp:243-270).  Host wall clock: Total, Setup,
+line format (TimerManagerClass::print_flat, NewTimer.cpp:243-270), or with --stack-timers its stack
+profile (print_stack, NewTimer.cpp:281-331: the nesting of the reference's scopes, inclusive and
+exclusive time); info_*.xml holds the nested <timing> tree either way (output_timing,
+NewTimer.cpp:333-395).  Host wall clock: Total, Setup,
 Initialization (evaluateLog from scratch), Diffusion, Pseudopotential.  Device time of the sweep's
 stages from HIP events, under the reference scope each one does the work of:
   * "New Gradient"      the SPO evaluation at every proposal (most of ratioGrad's time in the
@@ -54,6 +57,8 @@ def parse_args(argv=None):
     ap.add_argument("--fp64", action="store_true", help="FP64 batch (default FP32 with the FP64 refresh)")
     ap.add_argument("--system", default=None, help="a named workload instead of -g (e.g. nio8-cubic)")
     ap.add_argument("--info-dir", default=".", help="where info_<na>_<nb>_<nc>.xml is written")
+    ap.add_argument("--stack-timers", action="store_true",
+                    help="print the stack timer profile (the reference built with USE_STACK_TIMERS)")
     a = ap.parse_args(argv)
     try:
         a.tiling = tuple(int(t) for t in a.g.split())
@@ -88,6 +93,52 @@ def timer_table(timers: dict) -> str:
     return "\n".join(lines)
 
 
+# The reference's timer scopes as they nest at run time (miniqmc.cpp:330-500; WaveFunction.cpp's
+# "Complete Updates" and "Kinetic Energy" run inside Diffusion): name -> parent
+TIMER_PARENT = {"Setup": None, "Total": None, "Initialization": "Total", "Diffusion": "Total",
+                "Current Gradient": "Diffusion", "New Gradient": "Diffusion", "Update": "Diffusion",
+                "Complete Updates": "Diffusion", "Kinetic Energy": "Diffusion", "Pseudopotential": "Total",
+                "Value": "Pseudopotential"}
+
+
+def timer_stack(timers: dict) -> list:
+    """Depth-first rows (level, name, inclusive, exclusive, calls) of the timers that ran, in the
+    order the reference's scopes open (collate_stack_profile's order).  A timer outside the known
+    scopes sits at the top level.  Exclusive time is the inclusive time less the children's; the
+    device stages under Diffusion run on three streams at once, so their sum can exceed Diffusion's
+    wall time, and the exclusive time is then 0."""
+    order = list(TIMER_PARENT) + sorted(k for k in timers if k not in TIMER_PARENT)
+    parent = {k: (TIMER_PARENT.get(k) if TIMER_PARENT.get(k) in timers else None) for k in timers}
+    rows = []
+
+    def visit(name, level):
+        t, n = timers[name]
+        kids = [k for k in order if k in timers and parent[k] == name]
+        rows.append((level, name, t, max(0.0, t - sum(timers[k][0] for k in kids)), n))
+        for k in kids:
+            visit(k, level + 1)
+
+    for k in order:
+        if k in timers and parent[k] is None:
+            visit(k, 0)
+    return rows
+
+
+def stack_table(timers: dict) -> str:
+    """Stack profile in the reference's layout (TimerManagerClass::print_stack, NewTimer.cpp:281-331):
+    names indented two spaces per level and padded to the longest, then inclusive time, exclusive
+    time, calls and time per call."""
+    rows = timer_stack(timers)
+    width = max([len("Timer")] + [2 * lv + len(name) for lv, name, _, _, _ in rows])
+    lines = ["Stack timer profile in seconds",
+             "%s  %-9s  %-9s  %-10s  %-13s" % ("Timer".ljust(width), "Inclusive_time", "Exclusive_time", "Calls",
+                                               "Time_per_call")]
+    for lv, name, t, tx, n in rows:
+        lines.append("%s  %9.4f  %9.4f  %13d  %16.9f" % ((" " * (2 * lv) + name).ljust(width), t, tx, n,
+                                                       t / (n + 2.220446049250313e-16)))
+    return "\n".join(lines)
+
+
 def throughput(nw: int, nel: int, timers: dict) -> dict:
     """The reference's three throughput figures (miniqmc.cpp:514-522)."""
     out = {}
@@ -98,16 +149,25 @@ def throughput(nw: int, nel: int, timers: dict) -> dict:
 
 
 def info_xml(path: str, nel: int, nsteps: int, timers: dict) -> None:
-    """info_<na>_<nb>_<nc>.xml with the reference's element layout (miniqmc.cpp:525-551)."""
+    """info_<na>_<nb>_<nc>.xml with the reference's element layout (miniqmc.cpp:525-551); <timing> is
+    the timer tree of TimerManagerClass::output_timing (NewTimer.cpp:333-395): a timer's children
+    under its <includes>."""
     res = ET.Element("resources")
     ET.SubElement(res, "hardware")
     timing = ET.SubElement(res, "timing")
-    for name in sorted(timers):
-        t, n = timers[name]
-        tm = ET.SubElement(timing, "timer")
+    ET.SubElement(timing, "max_stack_level_exceeded").text = "no"
+    ET.SubElement(timing, "max_timers_exceeded").text = "no"
+    holders = [timing]   # holders[level]: where a timer of that level goes
+    rows = timer_stack(timers)
+    for i, (lv, name, t, tx, n) in enumerate(rows):
+        del holders[lv + 1:]
+        tm = ET.SubElement(holders[lv], "timer")
         ET.SubElement(tm, "name").text = name
         ET.SubElement(tm, "time_incl").text = f"{t:.6f}"
+        ET.SubElement(tm, "time_excl").text = f"{tx:.6f}"
         ET.SubElement(tm, "calls").text = str(n)
+        if i + 1 < len(rows) and rows[i + 1][0] > lv:
+            holders.append(ET.SubElement(tm, "includes"))
     parts = ET.SubElement(res, "particles")
     p = ET.SubElement(parts, "particle")
     ET.SubElement(p, "name").text = "e"
@@ -163,7 +223,7 @@ def main(argv=None) -> int:
             add(STAGE_TIMERS[stage], ms * 1e-3, int(n))
 
     print("================================== ")
-    print(timer_table(timers))
+    print(stack_table(timers) if a.stack_timers else timer_table(timers))
     print()
     print("========== Throughput ============ ")
     print()

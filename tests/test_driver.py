@@ -45,4 +45,41 @@ def test_timer_table_and_info_xml(tmp_path):
     assert root.find("particles/particle/size").text == "768"
     assert root.find("run/driver/name").text == "miniqmc"
     assert root.find("run/driver/steps").text == "3"
-    assert sorted(t.find("name").text for t in root.find("timing")) == ["Diffusion", "Total"]
+    timing = root.find("timing")
+    assert timing.find("max_stack_level_exceeded").text == "no"
+    assert [t.find("name").text for t in timing.findall("timer")] == ["Total"]
+    assert [t.find("name").text for t in timing.findall("timer/includes/timer")] == ["Diffusion"]
+
+
+def test_stack_profile_nesting_and_xml_tree(tmp_path):
+    """The reference's stack profile (NewTimer.cpp:281-331) and <timing> tree (NewTimer.cpp:333-395):
+    scopes nested as they open in miniqmc.cpp:330-500, exclusive = inclusive less the children's."""
+    timers = {"Setup": (1.0, 1), "Total": (10.0, 1), "Initialization": (0.5, 1), "Diffusion": (6.0, 5),
+              "New Gradient": (2.0, 10), "Update": (3.0, 10), "Kinetic Energy": (0.25, 10),
+              "Pseudopotential": (2.5, 5)}
+    rows = driver.timer_stack(timers)
+    assert [(lv, name) for lv, name, _, _, _ in rows] == [
+        (0, "Setup"), (0, "Total"), (1, "Initialization"), (1, "Diffusion"), (2, "New Gradient"), (2, "Update"),
+        (2, "Kinetic Energy"), (1, "Pseudopotential")]
+    excl = {name: tx for _, name, _, tx, _ in rows}
+    assert excl["Total"] == pytest.approx(10.0 - 0.5 - 6.0 - 2.5)
+    assert excl["Diffusion"] == pytest.approx(6.0 - 2.0 - 3.0 - 0.25)
+    assert excl["Update"] == pytest.approx(3.0)
+    # device stages that overlap on several streams can sum past their parent: exclusive time 0
+    over = dict(timers, Update=(5.0, 10))
+    assert {n: tx for _, n, _, tx, _ in driver.timer_stack(over)}["Diffusion"] == 0.0
+    lines = driver.stack_table(timers).splitlines()
+    assert lines[0] == "Stack timer profile in seconds"
+    assert lines[1].split() == ["Timer", "Inclusive_time", "Exclusive_time", "Calls", "Time_per_call"]
+    assert lines[7].startswith("    Update") and lines[7].split()[1:4] == ["3.0000", "3.0000", "10"]
+    assert lines[4].startswith("  Initialization")
+    path = tmp_path / "info_1_1_1.xml"
+    driver.info_xml(str(path), 192, 5, timers)
+    timing = ET.parse(path).getroot().find("timing")
+    assert [t.find("name").text for t in timing.findall("timer")] == ["Setup", "Total"]
+    assert [t.find("name").text for t in timing.findall("timer/includes/timer")] == [
+        "Initialization", "Diffusion", "Pseudopotential"]
+    diff = timing.findall("timer/includes/timer")[1]
+    assert [t.find("name").text for t in diff.findall("includes/timer")] == ["New Gradient", "Update", "Kinetic Energy"]
+    assert float(diff.find("time_excl").text) == pytest.approx(0.75)
+    assert timing.find("timer").find("includes") is None   # Setup has no children
