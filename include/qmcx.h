@@ -68,8 +68,9 @@ int qmcx_system_set_j2(qmcx_system_t sys, int pair, int nparams, double rcut, do
  * within rcut/2 of the ion), params[nparams] the free parameters left by the no-cusp constraints
  * (26 for (3, 3); QMCX_ERR_ARG names the count when it differs).  All functors of one species take one
  * rcut; at most 256 ions.  Set it before qmcx_batch_create: a batch made earlier returns QMCX_ERR_STATE.
- * With a three-body Jastrow, qmcx_flex_evaluate_ratios and qmcx_nlpp_evaluate return QMCX_ERR_ARG, and
- * an electron position may have at most 6 ions within the cutoff (else QMCX_ERR_STATE). */
+ * With a three-body Jastrow, qmcx_flex_evaluate_ratios and qmcx_nlpp_evaluate include its factor
+ * (ThreeBodyJastrow::evaluateRatios, ThreeBodyJastrow.h:262-273), and an electron position may have at
+ * most 6 ions within the cutoff (else QMCX_ERR_STATE). */
 int qmcx_system_set_j3(qmcx_system_t sys, int species, int pair, int neI, int nee, double rcut, const double* params,
                        int nparams);
 

@@ -1185,8 +1185,12 @@ static int launch_nlpp_p(qmcx_batch_t b, double rmax, int maxp) {
   const size_t lds = sizeof(T) * (size_t)P * SL * s->npad + sizeof(double) * 3 * P * 16;
   // workgroups walk the work list (P knots of a pair per unit); its length is only known on the device
   const int grid = (std::max(1, std::min(WN * maxp * (kNlppKnots / P), kNlppWorkgroups)) + 7) / 8 * 8;
-  hipLaunchKernelGGL((k_nlpp_ratios<T, P, PER>), dim3(grid), dim3(nt), lds, b->stream, s->params<T>(),
-                     b->params<T>(), b->nl_sph, maxp, b->nl_near, b->nl_items, count, b->nl_out);
+  if (b->j3)
+    hipLaunchKernelGGL((k_nlpp_ratios<T, P, PER, true>), dim3(grid), dim3(nt), lds, b->stream, s->params<T>(),
+                       b->params<T>(), b->nl_sph, maxp, b->nl_near, b->nl_items, count, b->nl_out);
+  else
+    hipLaunchKernelGGL((k_nlpp_ratios<T, P, PER, false>), dim3(grid), dim3(nt), lds, b->stream, s->params<T>(),
+                       b->params<T>(), b->nl_sph, maxp, b->nl_near, b->nl_items, count, b->nl_out);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2081,7 +2085,7 @@ int qmcx_flex_evaluate_gl(qmcx_batch_t b, double* log_out) {
 int qmcx_flex_evaluate_ratios(qmcx_batch_t b, int iel, int nvp, const double* vpos, double* ratios) {
   if (!b || !vpos || !ratios || iel < 0 || iel >= b->sys->N || nvp < 1 || nvp > kMaxVirtualPoints)
     return fail(QMCX_ERR_ARG, "bad argument");
-  if (b->j3) return fail(QMCX_ERR_ARG, "virtual-move ratios with a three-body Jastrow are not built");
+  if (int rc = check_j3(b)) return rc;   // the kernel reads the batch's J3 state when the system has one
   if (int rc = flush_pending(b)) return rc;   // the ratios read the up-to-date inverse rows
   if (nvp > b->vr_cap) {
     HIPCHK(hipStreamSynchronize(b->stream));
@@ -2114,7 +2118,7 @@ int qmcx_batch_set_nlpp_unit(qmcx_batch_t b, int knots_per_unit) {
 int qmcx_nlpp_evaluate(qmcx_batch_t b, double rmax, const double* spheres, int max_pairs, double* ratios,
                        int* npairs) {
   if (!b || !spheres || !(rmax > 0) || max_pairs < 1 || max_pairs > kMaxNlppPairs) return fail(QMCX_ERR_ARG, "bad argument");
-  if (b->j3) return fail(QMCX_ERR_ARG, "NonLocalPP ratios with a three-body Jastrow are not built");
+  if (int rc = check_j3(b)) return rc;   // the kernel reads the batch's J3 state when the system has one
   if (int rc = flush_pending(b)) return rc;   // the ratios read the up-to-date inverse rows
   const size_t WN = (size_t)b->W * b->sys->N;
   // work-list entries and knot units are int indices on the device

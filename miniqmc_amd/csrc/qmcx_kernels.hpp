@@ -1251,9 +1251,44 @@ __global__ __launch_bounds__(1024) void k_spline_vgh(SysP<T> S, const double* r,
 //     (DiracDeterminant::evaluateRatios, DiracDeterminant.cpp:216-224; SPOSet::evaluateDetRatios,
 //     SPOSet.h:72-83; MultiBspline::evaluate_v, MultiBspline.hpp:34-63);
 //   J1: exp(Vat[iel] - sum over ions of u(|ion - point|))   (OneBodyJastrow.h:136-140);
-//   J2: exp(Uat[iel] - sum over j != iel of u(|r_j - point|)) (TwoBodyJastrow.h:103-107).
+//   J2: exp(Uat[iel] - sum over j != iel of u(|r_j - point|)) (TwoBodyJastrow.h:103-107);
+//   J3 (with a three-body Jastrow): exp(U3[iel] - U3 at the point)   (ThreeBodyJastrow.h:262-273).
 // One workgroup per (point, walker).  The inverse must carry no pending delayed rows, which
 // holds between the batch's calls.
+// Three-body Jastrow sum U of electron e of walker w placed at (x, y, z), the others at their
+// current positions (ThreeBodyJastrow::evaluateRatios' computeU, ThreeBodyJastrow.h:262-273):
+// over ions I with the point within L_I and electrons k != e within L_I of I (the batch's ion
+// masks of the current positions), F(r_xk, r_xI, r_kI) with the functor of I's species and the
+// (e, k) spin pair.  The lanes [tid, tid + nt, ...) take the electrons k; returns this lane's part.
+template <typename T>
+__device__ inline double j3_value_partial(const SysP<T>& S, const BatchP<T>& B, const Cell<T>& C, int w, int e,
+                                          double x, double y, double z, int tid, int nt) {
+  const int N = S.N, n = S.n, M = S.M, mw = S.mw;
+  const double* R = B.R + (size_t)w * 3 * N;
+  const unsigned long long* mk = B.j3mask + (size_t)w * N * mw;
+  const bool up = e < n;
+  double acc = 0.0;
+  for (int I = 0; I < M; ++I) {
+    const int sp = S.species[I];
+    const T L = S.j3[2 * sp].L;
+    if (!(L > T(0))) continue;   // the species has no three-body functor
+    const double ix = S.ions[I], iy = S.ions[M + I], iz = S.ions[2 * M + I];
+    T dx, dy, dz;
+    const T rxI = min_image(C, (T)(ix - x), (T)(iy - y), (T)(iz - z), T(-1), dx, dy, dz);
+    if (!(rxI < L)) continue;
+    const unsigned long long bit = 1ull << (I & 63);
+    const int wd = I >> 6;
+    for (int k = tid; k < N; k += nt) {
+      if (k == e || !(mk[(size_t)k * mw + wd] & bit)) continue;
+      const double kx = R[k], ky = R[N + k], kz = R[2 * N + k];
+      const T rkI = min_image(C, (T)(ix - kx), (T)(iy - ky), (T)(iz - kz), T(-1), dx, dy, dz);
+      const T rxk = min_image(C, (T)(kx - x), (T)(ky - y), (T)(kz - z), T(-1), dx, dy, dz);
+      acc += (double)poly3_v(S.j3[2 * sp + (((k < n) == up) ? 0 : 1)], rxk, rxI, rkI);
+    }
+  }
+  return acc;
+}
+
 // Ratio at one point for walker w, electron iel; the workgroup's lanes share the work and
 // every lane returns the ratio.  part: LDS [slices][npad] T; red: LDS 48 doubles.
 template <typename T>
@@ -1264,7 +1299,7 @@ __device__ inline double virtual_ratio(const SysP<T>& S, const BatchP<T>& B, con
   __syncthreads();
   const int det = iel < n ? 0 : 1, row = iel - det * n;
   const T* arow = B.Ainv + ((size_t)w * 2 + det) * n * n + (size_t)row * n;
-  double v[3] = {0.0, 0.0, 0.0};   // det ratio, J1 sum, J2 sum
+  double v[3] = {0.0, 0.0, 0.0};   // det ratio, J1 sum, J2 and J3 sum (both enter the exponent alike)
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     T s = 0;
     for (int sl = 0; sl < slices; ++sl) s += part[(size_t)sl * S.npad + j];
@@ -1286,9 +1321,11 @@ __device__ inline double virtual_ratio(const SysP<T>& S, const BatchP<T>& B, con
     const T r = min_image(C, (T)(R[j] - x), (T)(R[N + j] - y), (T)(R[2 * N + j] - z), T(-1), dx, dy, dz);
     if (r < f.rcut) v[2] += (double)functor_v(f, r);
   }
+  if (S.j3 != nullptr) v[2] += j3_value_partial<T>(S, B, C, w, iel, x, y, z, threadIdx.x, blockDim.x);
   block_sum<double, 3>(v, red);   // ends with a barrier: part and red are free again
   const size_t o = (size_t)w * 5 * N + iel;
-  return v[0] * exp(((double)B.J1[o] - v[1]) + ((double)B.J2[o] - v[2]));
+  const double u23 = (double)B.J2[o] + (S.j3 != nullptr ? (double)B.J3[o] : 0.0);
+  return v[0] * exp(((double)B.J1[o] - v[1]) + (u23 - v[2]));
 }
 
 template <typename T>
@@ -1396,9 +1433,10 @@ __global__ __launch_bounds__(256) void k_nlpp_scatter(int WN, int maxp, const in
 // A work unit is P of a pair's twelve knots.  Lanes [g*L, (g+1)*L), L = (16/PER)*nq, gather
 // knot g's value row (a lane: one 16-byte orbital group, PER of the 16 stencil columns), then
 // every lane takes a share of the P dot products and of the P Jastrow sums, reading each
-// electron's position once for all P knots.
+// electron's position once for all P knots.  J3: the system has a three-body Jastrow (its sums
+// at the knots join the J2 sums, as both enter the exponent alike).
 constexpr int kNlppThreads = 768;
-template <typename T, int P, int PER>
+template <typename T, int P, int PER, bool J3>
 __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<T> B, const double* spheres, int maxp,
                                                               const double* near, const int* items, const int* count,
                                                               double* ratios) {
@@ -1431,7 +1469,7 @@ __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<
     const int det = e < n ? 0 : 1, row = e - det * n;
     const T* arow = B.Ainv + ((size_t)w * 2 + det) * n * n + (size_t)row * n;
     const size_t o = (size_t)w * 5 * N + e;
-    const double j1o = (double)B.J1[o], j2o = (double)B.J2[o];
+    const double j1o = (double)B.J1[o], j2o = (double)B.J2[o] + (J3 ? (double)B.J3[o] : 0.0);
     const bool up = e < n;
     double* out = ratios + (size_t)item * kNlppKnots;
     {
@@ -1440,7 +1478,7 @@ __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<
         spline_gather<T, 1, PER>(S, S.spoG, d * sp[3 * k] + ox, d * sp[3 * k + 1] + oy, d * sp[3 * k + 2] + oz,
                                  part + (size_t)g * SL * S.npad, tid - g * L);
       }
-      double v[3 * P];
+      double v[3 * P];   // det ratios, J1 sums, J2 (+ J3) sums of the P knots
 #pragma unroll
       for (int i = 0; i < 3 * P; ++i) v[i] = 0.0;
       T dx, dy, dz;   // the values only: each knot needs u, not its derivatives
@@ -1466,6 +1504,12 @@ __global__ __launch_bounds__(kNlppThreads) void k_nlpp_ratios(SysP<T> S, BatchP<
           if (r < f.rcut) v[2 * P + gg] += (double)functor_v(f, r);
         }
       }
+      if constexpr (J3)
+#pragma unroll
+        for (int gg = 0; gg < P; ++gg) {
+          const double px = d * sp[3 * (k0 + gg)] + ox, py = d * sp[3 * (k0 + gg) + 1] + oy, pz = d * sp[3 * (k0 + gg) + 2] + oz;
+          v[2 * P + gg] += j3_value_partial<T>(S, B, C, w, e, px, py, pz, tid, nt);
+        }
       __syncthreads();   // the P value rows are in LDS
       for (int ii = tid; ii < P * n; ii += nt) {
         const int gi = ii / n, j = ii - gi * n;
