@@ -1,6 +1,6 @@
 """miniqmc-style driver over one walker batch (src/Drivers/miniqmc.cpp).
 
-    python -m miniqmc_amd.driver -g "1 1 2" -w 128 -n 5 -k 32 [-x 1.7] [-s 11] [--fp64]
+    python -m miniqmc_amd.driver -g "1 1 2" -w 128 -n 5 -k 32 [-j] [-N 1] [-x 1.7] [-s 11] [-t fine] [--fp64]
 
 Per step, as the reference's main loop (miniqmc.cpp:406-495): one VMC sweep of every electron of
 every walker ("Diffusion": evaluateLog's refresh, the moves, completeUpdates, evaluateGL), then
@@ -29,7 +29,12 @@ Differences from the reference, by design:
     walker thread;
   * acceptance: the reference's rule u < r with r = 0.5 (``-r`` only accepts 0.5), or Metropolis;
   * one random sphere per walker and pseudopotential evaluation (each mover rotates its own);
-  * systems: the tilings with a prepared workload, -g "1 1 1" (NiO-32) and -g "1 1 2" (NiO-64).
+  * systems: the tilings with a prepared workload, -g "1 1 1" (NiO-32) and -g "1 1 2" (NiO-64);
+  * options the reference has and this driver does not take: -a (orbital tiles of the spline: a
+    workgroup here evaluates all orbitals of a point), -m (mesh factor: the workloads' grids are
+    fixed), -b (the reference's scalar implementations);
+  * -t: every timer reported here is a coarse-level one in the reference too (miniqmc.cpp:312,
+    WaveFunction.cpp:38-41), so "coarse" and "fine" print the same set.
 """
 from __future__ import annotations
 
@@ -49,6 +54,11 @@ def parse_args(argv=None):
     ap.add_argument("-w", type=int, default=16, help="number of walkers")
     ap.add_argument("-n", type=int, default=5, help="number of MC steps")
     ap.add_argument("-k", type=int, default=32, help="delayed update rank")
+    ap.add_argument("-j", action="store_true", help="enable the three-body Jastrow")
+    ap.add_argument("-N", type=int, default=1, help="number of MC substeps (sweeps per step)")
+    ap.add_argument("-t", default="fine", help="timer level: coarse or fine")
+    ap.add_argument("-v", action="store_true", help="verbose output")
+    ap.add_argument("-V", action="store_true", help="print version information and exit")
     ap.add_argument("-x", type=float, default=1.7, help="Rmax of the pseudopotential")
     ap.add_argument("-r", type=float, default=0.5, help="acceptance threshold of the reference rule (0.5 only)")
     ap.add_argument("-s", type=int, default=11, help="random seed")
@@ -72,8 +82,10 @@ def parse_args(argv=None):
         a.system = TILINGS[a.tiling]
     if a.r != 0.5:
         ap.error("the reference acceptance rule runs with r = 0.5 only")
-    if a.w < 1 or a.n < 1 or a.k < 1:
-        ap.error("-w, -n and -k must be positive")
+    if a.w < 1 or a.n < 1 or a.k < 1 or a.N < 1:
+        ap.error("-w, -n, -N and -k must be positive")
+    if a.t not in ("coarse", "fine"):
+        ap.error(f"Timer level should be 'coarse' or 'fine', name given: {a.t}")
     return a
 
 
@@ -180,10 +192,16 @@ def info_xml(path: str, nel: int, nsteps: int, timers: dict) -> None:
     ET.ElementTree(res).write(path, xml_declaration=True, encoding="utf-8")
 
 
+VERSION = "miniqmc_amd 1.0 (MI355X, gfx950)"
+
+
 def main(argv=None) -> int:
     a = parse_args(argv)
+    if a.V:
+        print(VERSION)
+        return 0
     t_setup = time.perf_counter()
-    sysd = systems.make_system(a.system, seed=a.s)
+    sysd = systems.make_system(a.system, seed=a.s, j3=a.j)
     prec = qmcx.FP64 if a.fp64 else qmcx.FP32
     dev = qmcx.System(sysd, precisions=(prec,))
     wb = qmcx.WalkerBatch(dev, a.w, delay_rank=a.k, precision=prec)
@@ -192,7 +210,11 @@ def main(argv=None) -> int:
     mode = qmcx.ACCEPT_METROPOLIS if a.metropolis else qmcx.ACCEPT_REFERENCE
     pp = nlpp.NonLocalPP(sysd, rmax=a.x, seed=a.s)
     print(f"miniqmc_amd: {sysd.name}, {sysd.nel} electrons, {sysd.norb} orbitals per determinant, {a.w} walkers, "
-          f"{a.n} steps, delay rank {a.k}, {'FP64' if a.fp64 else 'FP32'}, Rmax {a.x}", flush=True)
+          f"{a.n} steps of {a.N} substeps, delay rank {a.k}, {'FP64' if a.fp64 else 'FP32'}, Rmax {a.x}"
+          f"{', three-body Jastrow' if a.j else ''}", flush=True)
+    if a.v:
+        print(f"  cell {sysd.lattice.tolist()}, {len(sysd.ion_pos)} ions, spline grid {tuple(sysd.ng)}, "
+              f"delay form {'streamed' if wb.delay_form() else 'LDS-resident'}, tau {a.tau}, seed {a.s}", flush=True)
 
     timers = {}
 
@@ -210,7 +232,7 @@ def main(argv=None) -> int:
     pairs = 0
     for step in range(a.n):
         t = time.perf_counter()
-        wb.run(1, a.s, a.tau, mode, recompute_every=0 if a.fp64 else 1)
+        wb.run(a.N, a.s, a.tau, mode, recompute_every=0 if a.fp64 else 1)
         add("Diffusion", time.perf_counter() - t)
         t = time.perf_counter()
         # the reference discards the ratios (NonLocalPP.hpp:115-140); they stay on the device
@@ -231,8 +253,8 @@ def main(argv=None) -> int:
     print(f"Total throughput ( N_walkers * N_elec^3 / Total time ) = {tp['total']:.6e}")
     print(f"Diffusion throughput ( N_walkers * N_elec^3 / Diffusion time ) = {tp['diffusion']:.6e}")
     print(f"Pseudopotential throughput ( N_walkers * N_elec^2 / Pseudopotential time ) = {tp['pseudopotential']:.6e}")
-    print(f"walker-moves/s {a.w * sysd.nel * a.n / timers['Diffusion'][0]:.4e}; "
-          f"acceptance {wb.accepts().sum() / (a.w * sysd.nel * a.n):.3f}; "
+    print(f"walker-moves/s {a.w * sysd.nel * a.n * a.N / timers['Diffusion'][0]:.4e}; "
+          f"acceptance {wb.accepts().sum() / (a.w * sysd.nel * a.n * a.N):.3f}; "
           f"{pairs // a.n} electron-ion pairs per step")
     na, nb, nc = a.tiling
     path = f"{a.info_dir}/info_{na}_{nb}_{nc}.xml"

@@ -12,9 +12,19 @@ def test_options_map_tilings_to_workloads():
     assert driver.parse_args(["--system", "nio8-cubic", "-g", "3 3 3"]).system == "nio8-cubic"
     a = driver.parse_args(["-w", "4", "-n", "3", "-k", "8", "-x", "2.0", "-s", "5"])
     assert (a.w, a.n, a.k, a.x, a.s) == (4, 3, 8, 2.0, 5)
+    # miniqmc.cpp:150-175 defaults: no three-body Jastrow, one substep, fine timers
+    assert (a.j, a.N, a.t) == (False, 1, "fine")
+    a = driver.parse_args(["-j", "-N", "2", "-t", "coarse", "-v"])
+    assert (a.j, a.N, a.t, a.v) == (True, 2, "coarse", True)
 
 
-@pytest.mark.parametrize("argv", [["-g", "2 2 2"], ["-g", "1 1"], ["-g", "a b c"], ["-r", "0.4"], ["-w", "0"]])
+def test_version_exits_before_any_work(capsys):
+    assert driver.main(["-V"]) == 0
+    assert "miniqmc_amd" in capsys.readouterr().out
+
+
+@pytest.mark.parametrize("argv", [["-g", "2 2 2"], ["-g", "1 1"], ["-g", "a b c"], ["-r", "0.4"], ["-w", "0"],
+                                  ["-N", "0"], ["-t", "medium"], ["-a", "16"], ["-m", "2"]])
 def test_options_reject_unsupported(argv):
     with pytest.raises(SystemExit):
         driver.parse_args(argv)

@@ -252,7 +252,8 @@ def test_nlpp_skewed_cell_vs_oracle(prec, tol):
     assert checked > 0
 
 
-@pytest.mark.parametrize("extra", [["-k", "4", "--stack-timers"], ["-k", "1", "--fp64", "--metropolis"]])
+@pytest.mark.parametrize("extra", [["-k", "4", "--stack-timers"], ["-k", "1", "--fp64", "--metropolis"],
+                                   ["-k", "4", "-j", "-N", "2", "-v"]])
 def test_driver_runs(tmp_path, extra):
     """The miniqmc-style driver end to end on a small system: sweeps, NonLocalPP, report, info XML."""
     from miniqmc_amd import driver
