@@ -142,8 +142,7 @@ int qmcx_batch_set_delay_form(qmcx_batch_t b, int form);
  * moves of that determinant, and its log|det| and phase are replaced by the FP64 values; the walker's
  * pending delayed rows are dropped there rather than flushed.  The count runs across sweeps and restarts at
  * every FP64 inversion (evaluateLog, the per-sweep refresh of qmcx_run_sweeps).  0 = off.  Needs
- * Sherman-Morrison moves or the in-kernel flush (FP32, n a multiple of 16 up to 384, k <= 32) and no
- * three-body Jastrow. */
+ * Sherman-Morrison moves or the in-kernel flush (FP32, n a multiple of 16 up to 384, k <= 32). */
 int qmcx_batch_set_recompute_rows(qmcx_batch_t b, int rows);
 int qmcx_batch_get_delay_form(qmcx_batch_t b, int* form);
 
@@ -228,7 +227,7 @@ int qmcx_get_arbitrations(qmcx_batch_t b, long long* narb);
  * sum the FP64 spline values, accepted moves store them, and the FP64 refresh inverts those, so
  * log|Psi| after a refresh carries no FP32 rounding of the matrix, however ill-conditioned.  As with
  * the band, the next qmcx_evaluate_log fills them before the next qmcx_run_sweeps.  enable 0 (the
- * default) or 1; FP64 batches ignore it; not with a three-body Jastrow. */
+ * default) or 1; FP64 batches ignore it. */
 int qmcx_batch_set_fp64_rows(qmcx_batch_t b, int enable);
 
 /* Outputs of the last evaluateGL: G[nw][N][3], L[nw][N], log[nw]; per-walker accept counts. */

@@ -704,9 +704,9 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
     // FP64 recompute every b->cadence accepted rows of a walker: rounds of one move launch (each
     // walker from its own position until its cadence or the determinant's end) and one FP64
     // inversion of the walkers that stopped; a walker stops at most n / cadence times
-    if (!(b->K == 1 || inflush) || b->j3)
+    if (!(b->K == 1 || inflush))
       return fail(QMCX_ERR_STATE, "the recompute cadence needs Sherman-Morrison moves or delayed updates applied "
-                                  "inside the move kernel, and no three-body Jastrow");
+                                  "inside the move kernel");
     hipLaunchKernelGGL(k_cadence_begin, dim3((b->W + 255) / 256), dim3(256), 0, b->stream, b->cad, b->W, det);
     HIPCHK(hipGetLastError());
     a.e_begin = det * n;
@@ -1762,7 +1762,6 @@ int qmcx_batch_set_walker_offset(qmcx_batch_t b, int walker0) {
 
 int qmcx_batch_set_recompute_rows(qmcx_batch_t b, int rows) {
   if (!b || rows < 0) return fail(QMCX_ERR_ARG, "bad argument");
-  if (rows > 0 && b->j3) return fail(QMCX_ERR_ARG, "the recompute cadence is not built with a three-body Jastrow");
   if (rows > 0 && !b->cad) {
     HIPCHK(hipMalloc(&b->cad, sizeof(int) * 4 * (size_t)b->W));
     HIPCHK(hipMalloc(&b->cad_flag, sizeof(int) * (size_t)b->W));
@@ -1945,7 +1944,6 @@ int qmcx_batch_set_accept_band(qmcx_batch_t b, double band) {
 
 int qmcx_batch_set_fp64_rows(qmcx_batch_t b, int enable) {
   if (!b) return fail(QMCX_ERR_ARG, "null batch");
-  if (enable && b->j3) return fail(QMCX_ERR_ARG, "FP64 Slater rows do not take a three-body Jastrow");
   b->drop_graph();   // a captured run holds the old setting
   b->fp64_rows = enable != 0;
   const bool on = b->prec == QMCX_FP32 && (b->band > 0.0 || b->fp64_rows);

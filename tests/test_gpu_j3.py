@@ -50,23 +50,27 @@ def test_j3_evaluate_log_vs_oracle(cubic_j3, prec, tol):
 EPS32 = 2.0 ** -24
 
 
-def _run_compare(sysd, dev, orc, nw, nsweeps, delay, prec, mode, tau, seed, tol_ratio, tol_log, cond=0.0):
+def _run_compare(sysd, dev, orc, nw, nsweeps, delay, prec, mode, tau, seed, tol_ratio, tol_log, cond=0.0, setup=None,
+                 recompute=0):
     """Sweeps of the device against the oracle's (same delay rank).  Ratios: |dr| / max(|r|, 1) <=
     tol_ratio, or, with cond > 0, |dr| / |r| <= max(cond 2^-24 max(s_1 .. s_m), tol_ratio) with s the
     oracle's conditioning of the move (its relative sensitivity to relative perturbations of the
     Slater matrix and the new row, from a delay-1 run whose chain is the same).  Gradients: 100
     tol_ratio (resp. ten times the bound) relative to 1 + |g|.  log|Psi| after each sweep (no FP64
-    refresh) within tol_log relative, plus with cond the summed bounds of the accepted moves."""
+    refresh, or after it with recompute) within tol_log relative, plus with cond the summed bounds of
+    the accepted moves.  setup(batch) runs before the positions are set."""
     R0 = systems.initial_positions(sysd, nw, seed=seed)
     ref = orc.run_vmc(R0, nsweeps, seed, tau, mode=mode, delay=delay, trace=True)
     sens = orc.run_vmc(R0, nsweeps, seed, tau, mode=mode, delay=1, trace=True, sensitivity=True) if cond else None
     wb = qmcx.WalkerBatch(dev, nw, delay_rank=delay, precision=prec)
+    if setup is not None:
+        setup(wb)
     wb.set_positions(R0)
     np.testing.assert_allclose(wb.evaluate_log(), ref["log_init"], rtol=tol_log)
     wb.set_trace(True)
     n = sysd.nup
     for s in range(nsweeps):
-        hist = wb.run(1, seed, tau, mode)
+        hist = wb.run(1, seed, tau, mode, recompute_every=recompute)
         tr, rt = wb.trace(), ref["trace"][:, s]
         assert np.array_equal(tr[:, :, 2], rt[:, :, 2]), ("accept sequence", s)
         gerr = (np.abs(tr[:, :, 3:9] - rt[:, :, 3:9]) / (1 + np.abs(rt[:, :, 3:9]))).max(axis=2)
@@ -154,6 +158,38 @@ def test_j3_nio64_fp32_vs_oracle():
     # the FP64 form's walker state with a three-body Jastrow exceeds the LDS at NiO-64
     with pytest.raises(qmcx.QmcxError, match="LDS"):
         qmcx.WalkerBatch(dev, 2, delay_rank=1, precision=qmcx.FP64)
+
+
+@pytest.mark.parametrize("prec,delay,rows", [(qmcx.FP64, 1, 4), (qmcx.FP64, 1, 16), (qmcx.FP32, 1, 5),
+                                             (qmcx.FP32, 8, 4), (qmcx.FP32, 16, 16)])
+def test_j3_recompute_cadence_vs_oracle(cubic_j3, prec, delay, rows):
+    """The FP64 recompute after every `rows` accepted moves of a walker's determinant (the walker
+    stops, its inverse is rebuilt, it resumes from its own electron; Sherman-Morrison moves, or the
+    FP32 form's delayed updates applied inside the move kernel) with the three-body Jastrow:
+    the oracle's sweeps, FP64 to 1e-8 in the ratios and 1e-11 in log|Psi| with identical positions,
+    G and L; FP32 to 2e-3 and 1e-6 (Metropolis acceptance)."""
+    sysd, dev, orc = cubic_j3
+    fp64 = prec == qmcx.FP64
+    _say("J3 recompute cadence", prec, "delay", delay, "rows", rows)
+    ref, G, L, wb = _run_compare(sysd, dev, orc, 3, 2, delay, prec, qmcx.ACCEPT_METROPOLIS, 0.5, 11,
+                                 1e-8 if fp64 else 2e-3, 1e-11 if fp64 else 1e-6,
+                                 setup=lambda b: b.set_recompute_rows(rows))
+    assert np.array_equal(wb.accepts(), ref["naccept"])
+    if fp64:
+        np.testing.assert_allclose(wb.positions(), ref["R"], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(G, ref["G"], rtol=1e-8, atol=1e-8)
+        np.testing.assert_allclose(L, ref["L"], rtol=1e-8, atol=1e-7)
+
+
+@pytest.mark.parametrize("delay", [1, 8])
+def test_j3_fp32_fp64_rows_refresh_vs_oracle(cubic_j3, delay):
+    """The FP32 form with FP64 Slater rows and the per-sweep FP64 refresh inverting them, with the
+    three-body Jastrow: ratios within 2e-3 of max(|r|, 1) and, after each sweep's refresh, log|Psi|
+    within 1e-7 of the oracle (FP64 determinant, FP32 Jastrow sums)."""
+    sysd, dev, orc = cubic_j3
+    _say("J3 FP32 with FP64 rows, delay", delay)
+    _run_compare(sysd, dev, orc, 3, 2, delay, qmcx.FP32, qmcx.ACCEPT_METROPOLIS, 0.5, 11, 2e-3, 1e-7,
+                 setup=lambda b: b.set_fp64_rows(True), recompute=1)
 
 
 def test_j3_unsupported_paths_raise(cubic_j3):
