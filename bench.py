@@ -412,6 +412,57 @@ def measure_nio32(mode, steps, err_steps, fp64_steps, seed, tau, walkers=256, de
     return out
 
 
+def measure_j3(mode, steps, err_steps, fp64_steps, seed, tau, walkers=256, delay=32, rmax=1.7):
+    """NiO-32 with the three-body Jastrow (the reference's -j; at NiO-64 the FP64 path's walker state
+    with the J3 pair slots exceeds the LDS), 256 walkers: the FP32 form at delay rank 32 with FP64
+    Slater rows and the per-sweep FP64 refresh, against the FP64 delay-1 GPU path from the same start
+    and streams: walker-moves/s of both, log|Psi| error and identical accept sequences, and one
+    NonLocalPP evaluation (FP32) with the J3 factor at every knot."""
+    sysd = systems.make_system("nio32", seed=seed, j3=True)
+    dev = qmcx.System(sysd, precisions=(qmcx.FP32, qmcx.FP64))
+    R0 = systems.initial_positions(sysd, walkers, seed=seed)
+    b64 = qmcx.WalkerBatch(dev, walkers, delay_rank=1, precision=qmcx.FP64)
+    b64.set_positions(R0)
+    b64.evaluate_log()
+    h64 = b64.run(err_steps, seed, tau, mode)
+    a64 = b64.accepts()
+    t0 = time.perf_counter()
+    b64.run(fp64_steps, seed, tau, mode)
+    value64 = walkers * sysd.nel * fp64_steps / (time.perf_counter() - t0)
+    b64.close()
+    b = qmcx.WalkerBatch(dev, walkers, delay_rank=delay, precision=qmcx.FP32)
+    b.set_fp64_rows(True)
+    b.set_positions(R0)
+    b.evaluate_log()
+    h = b.run(err_steps, seed, tau, mode, recompute_every=1)
+    same = b.accepts() == a64
+    err = (np.abs(h - h64) / np.abs(h64))[:, same]
+    b.run(1, seed, tau, mode, recompute_every=1)
+    t0 = time.perf_counter()
+    b.run(steps, seed, tau, mode, recompute_every=1)
+    wall = time.perf_counter() - t0
+    spheres = nlpp.NonLocalPP(sysd, rmax=rmax, seed=seed).randomize_walkers(walkers)
+    b.nlpp_evaluate(rmax, spheres, max_pairs=8, ratios=False)
+    t0 = time.perf_counter()
+    _, cnt = b.nlpp_evaluate(rmax, spheres, max_pairs=8, ratios=False)
+    nl_wall = time.perf_counter() - t0
+    form = b.delay_form()
+    b.close()
+    moves = walkers * sysd.nel * steps
+    nratios = int(np.minimum(cnt, 8).sum()) * 12
+    out = {"workload": f"nio32 with the three-body Jastrow (-j), {walkers} walkers, tau {tau}", "walkers": walkers,
+           "delay_rank": delay, "delay_form": "streamed" if form else "lds_resident", "steps": steps,
+           "value": moves / wall, "ms_per_step": 1e3 * wall / steps, "fp64_delay1_value": value64,
+           "speedup_vs_fp64_delay1": moves / wall / value64,
+           "log_psi_err_vs_fp64": {"max_rel": float(err.max()) if err.size else None,
+                                   "walkers_same_accepts": int(same.sum()), "sweeps": err_steps},
+           "nlpp_f32": {"ratios_per_s": nratios / nl_wall, "ms": 1e3 * nl_wall, "ratios": nratios}}
+    log(f"nio32 -j: FP32 k={delay} {moves / wall:.4e} moves/s ({1e3 * wall / steps:.2f} ms/sweep) vs FP64 delay 1 "
+        f"{value64:.4e}; log err {out['log_psi_err_vs_fp64']['max_rel']}; same accepts {int(same.sum())}/{walkers}; "
+        f"NonLocalPP {nratios / nl_wall:.4e} ratios/s")
+    return out
+
+
 def measure_single_walker(seed, tau, mode):
     """The two one-walker configurations: the plain FP64 delay-1 drop-in (Sherman-Morrison, one
     workgroup) on the CPU-runnable NiO-8 cubic cell (96 e-, 5 sweeps, tau 0.5) and on NiO-32 (384 e-, 20
@@ -473,6 +524,8 @@ def main():
                     help="timed sweeps of the mixed NiO-16/32/64 batch in one grouped launch (0: skip)")
     ap.add_argument("--nio32-steps", type=int, default=10,
                     help="timed sweeps of the NiO-32 batched leg (256 walkers, two recompute cadences; 0: skip)")
+    ap.add_argument("--j3-steps", type=int, default=5,
+                    help="timed sweeps of NiO-32 with the three-body Jastrow (-j; 0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device five times (a kernel of known HBM traffic "
@@ -599,6 +652,10 @@ def main():
         nio32 = measure_nio32(mode, args.nio32_steps, args.err_steps, 2, args.seed, args.tau)
         nio32["one_walker_configs"] = measure_single_walker(args.seed, args.tau, mode)
 
+    j3 = None
+    if args.j3_steps > 0:
+        j3 = measure_j3(mode, args.j3_steps, args.err_steps, 2, args.seed, args.tau)
+
     cpu = None
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
@@ -655,6 +712,7 @@ def main():
         "evaluate_log": elog or None,
         "mixed_batch": mixed,
         "nio32_batched": nio32,
+        "three_body_jastrow": j3,
     }
     print(json.dumps(out))
 
