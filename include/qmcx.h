@@ -214,7 +214,9 @@ int qmcx_get_trace(qmcx_batch_t b, double* trace);
 /* Identical accept/reject decisions in the FP32 form under Metropolis acceptance (no reference
  * counterpart: the reference is FP64 throughout).  With band > 0 a move whose uniform u satisfies
  * |u - |R|^2| <= band |R|^2 for the FP32 ratio R is decided on its FP64 ratio: FP64 Slater rows with
- * the determinant ratio by iterative refinement from the FP32 inverse, J1/J2 from scratch.  While the
+ * the determinant ratio by iterative refinement from the FP32 inverse, J1/J2 (and J3) from scratch.  With a
+ * three-body Jastrow the re-evaluation needs LDS beside its pair slots (QMCX_ERR_ARG names the need
+ * when the delay rank leaves too little).  While the
  * band is on, the batch also keeps the FP64 Slater rows (the FP64 refresh then inverts those instead of
  * the FP32 rows); they are filled by the next qmcx_evaluate_log, which must come before the next
  * qmcx_run_sweeps (else QMCX_ERR_STATE).  band in [0, 1]; 0 (the default) switches it off.  FP64

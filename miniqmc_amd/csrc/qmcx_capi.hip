@@ -243,6 +243,7 @@ struct qmcx_system_s {
     S.fn64 = d_fnd;
     S.cell64 = d_cell_d;
     S.mw = has_j3 ? mw : 0;
+    S.j3_64 = has_j3 ? d_j3d : nullptr;
     if constexpr (sizeof(T) == 4) {
       S.coefs = d_coef_f;
       S.fn = d_fnf;
@@ -498,18 +499,23 @@ int launch_moves_kd(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
   return 0;
 }
 
-// the near-tie FP64 re-evaluation exists for FP32 delayed updates under Metropolis acceptance and
-// a wave function without a three-body Jastrow; ST: the streamed form of the delayed updates
+// the near-tie FP64 re-evaluation exists for FP32 delayed updates under Metropolis acceptance
+// (the kernel with ARB); ST: the streamed form of the delayed updates
+template <typename T>
+bool arb_launch(const qmcx_batch_t b, int mode, double band, const void* decide) {
+  return sizeof(T) == 4 && b->K > 1 && mode == QMCX_ACCEPT_METROPOLIS && band > 0.0 && decide == nullptr;
+}
 template <typename T, int JPT, bool ST>
 int launch_moves_jpt(qmcx_batch_t b, int nt, size_t lds, const MoveArgs& a) {
-  if (b->j3)
-    return b->K == 1 ? launch_moves_kd<T, JPT, false, false, true, false>(b, nt, lds, a)
-                     : launch_moves_kd<T, JPT, true, false, true, ST>(b, nt, lds, a);
-  if (b->K == 1) return launch_moves_kd<T, JPT, false, false, false, false>(b, nt, lds, a);
+  if (b->K == 1)
+    return b->j3 ? launch_moves_kd<T, JPT, false, false, true, false>(b, nt, lds, a)
+                 : launch_moves_kd<T, JPT, false, false, false, false>(b, nt, lds, a);
   if constexpr (sizeof(T) == 4)
-    if (a.mode == QMCX_ACCEPT_METROPOLIS && a.band > 0.0 && a.decide == nullptr)
-      return launch_moves_kd<T, JPT, true, true, false, ST>(b, nt, lds, a);
-  return launch_moves_kd<T, JPT, true, false, false, ST>(b, nt, lds, a);
+    if (arb_launch<T>(b, a.mode, a.band, a.decide))
+      return b->j3 ? launch_moves_kd<T, JPT, true, true, true, ST>(b, nt, lds, a)
+                   : launch_moves_kd<T, JPT, true, true, false, ST>(b, nt, lds, a);
+  return b->j3 ? launch_moves_kd<T, JPT, true, false, true, ST>(b, nt, lds, a)
+               : launch_moves_kd<T, JPT, true, false, false, ST>(b, nt, lds, a);
 }
 
 template <typename T>
@@ -683,7 +689,10 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
   const int n = s->n;
   const int chunk = move_chunk(n, b->K, b->j3);
   const bool st = b->streamed();
-  const MoveLds L = move_lds<T>(s->N, s->nions, n, b->K, chunk, b->j3 ? s->mw : 0, st);
+  // the launch takes the FP64 re-evaluation's kernel (launch_moves_jpt) exactly when arb3 holds for
+  // a three-body Jastrow: its LDS layout then keeps the re-evaluation apart from the J3 pair slots
+  const bool arb3 = b->j3 && arb_launch<T>(b, mode, b->rows64 ? b->band : 0.0, nullptr);
+  const MoveLds L = move_lds<T>(s->N, s->nions, n, b->K, chunk, b->j3 ? s->mw : 0, st, arb3);
   MoveArgs a;
   std::memset(&a, 0, sizeof(a));
   a.mode = mode;
@@ -1933,8 +1942,15 @@ int qmcx_get_accepts(qmcx_batch_t b, long long* naccept) {
 int qmcx_batch_set_accept_band(qmcx_batch_t b, double band) {
   if (!b || !(band >= 0.0) || band > 1.0) return fail(QMCX_ERR_ARG, "band must be in [0, 1]");
   b->drop_graph();   // a captured run holds the old setting
-  if (band > 0.0 && b->j3)
-    return fail(QMCX_ERR_ARG, "the near-tie FP64 re-evaluation does not take a three-body Jastrow");
+  if (band > 0.0 && b->j3 && b->prec == QMCX_FP32 && b->K > 1) {
+    // with a three-body Jastrow the re-evaluation takes LDS of its own beside the J3 pair slots
+    const qmcx_system_s* s = b->sys;
+    const size_t lds = move_lds<float>(s->N, s->nions, s->n, b->K, move_chunk(s->n, b->K, true), s->mw,
+                                       b->streamed(), true).total;
+    if (lds > kMoveLdsMax)
+      return fail(QMCX_ERR_ARG, "the near-tie band with a three-body Jastrow needs " + std::to_string(lds / 1024) +
+                                    " KB of LDS at this delay rank; use a smaller one");
+  }
   b->band = band;
   const bool on = b->prec == QMCX_FP32 && (band > 0.0 || b->fp64_rows);
   if (on && !b->rows64) b->rows64_stale = true;

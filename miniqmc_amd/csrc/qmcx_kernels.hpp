@@ -35,6 +35,7 @@ struct SysP {
   // three-body Jastrow: functor of (ion species, spin pair) at [2 * species + pair] (L = 0 for a
   // species without one), or null; mw = 64-bit words of an electron's ion mask, ceil(M / 64)
   const Poly3<T>* j3;
+  const Poly3<double>* j3_64;   // the same functors in FP64 (FP64 re-evaluation of the FP32 form)
   int mw;
 };
 

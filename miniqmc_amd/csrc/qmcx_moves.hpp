@@ -59,9 +59,12 @@ __host__ __device__ inline unsigned lds_take(unsigned& o, size_t bytes) {
 // masks of mw words (its sums, masks and the chunk's near-ion records stay for the launch; its
 // pair slots share the region past the J2 pair terms).  st: the streamed form of the delayed
 // updates (no inverse-row cache, no products of accepted rows with later rows; the capacitance
-// inverse in LDS with row stride K when binv_lds, else in HBM; p, y and reduction partials in oSt)
+// inverse in LDS with row stride K when binv_lds, else in HBM; p, y and reduction partials in oSt).
+// arb3: the FP64 re-evaluation with a three-body Jastrow, whose pair slots stay live across the
+// re-evaluation: its region then follows theirs instead of sharing it.
 template <typename T>
-__host__ __device__ inline MoveLds move_lds_at(int N, int n, int K, int E, int mw, bool st, bool binv_lds) {
+__host__ __device__ inline MoveLds move_lds_at(int N, int n, int K, int E, int mw, bool st, bool binv_lds,
+                                               bool arb3 = false) {
   MoveLds L;
   unsigned o = 0;
   const bool cache = K > 1 && !st;
@@ -102,6 +105,7 @@ __host__ __device__ inline MoveLds move_lds_at(int N, int n, int K, int E, int m
     L.oJ3v = L.oJ3k + (unsigned)(2 * sizeof(int) * waves * kJ3Slots);
     L.oJ3r = lds_take(p, mw > 0 ? sizeof(double) * 16 * 5 : 0);    // j-side wave partials
     if (mw > 0) o = p;
+    if (mw > 0 && arb3) L.oArb = o;
   }
   if (cache && sizeof(T) == 4) {
     const size_t need = L.oQ + flush_own_lds(n, K);
@@ -115,10 +119,11 @@ __host__ __device__ inline MoveLds move_lds_at(int N, int n, int K, int E, int m
   return L;
 }
 template <typename T>
-__host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E, int mw = 0, bool st = false) {
+__host__ __device__ inline MoveLds move_lds(int N, int M, int n, int K, int E, int mw = 0, bool st = false,
+                                           bool arb3 = false) {
   (void)M;
-  const MoveLds L = move_lds_at<T>(N, n, K, E, mw, st, st);
-  return st && L.total > kMoveLdsMax ? move_lds_at<T>(N, n, K, E, mw, st, false) : L;
+  const MoveLds L = move_lds_at<T>(N, n, K, E, mw, st, st, arb3);
+  return st && L.total > kMoveLdsMax ? move_lds_at<T>(N, n, K, E, mw, st, false, arb3) : L;
 }
 
 // Sum the wave partials red[v*16 + w] (w < nw); every lane of every wave gets
@@ -524,10 +529,10 @@ __device__ inline double arb_det_ratio(int n, int npad, int row, int r0, int c, 
 template <typename T>
 __device__ inline double arb_jastrow_ratio(const SysP<T>& S, const double* sR, int iel, double x0,
                                                               double y0, double z0, double nx, double ny, double nz,
-                                                              double* red) {
+                                                              double* red, const unsigned long long* sMask = nullptr) {
   const int tid = threadIdx.x, nt = blockDim.x, N = S.N, n = S.n, M = S.M;
   const Cell<double>& C = *S.cell64;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  double v[4] = {0.0, 0.0, 0.0, 0.0};   // J1 old, new; J2 (+ J3) old, new
   double dx, dy, dz;
   for (int ci = tid; ci < M; ci += nt) {
     const Functor<double>& f = S.fn64[S.species[ci]];
@@ -546,6 +551,31 @@ __device__ inline double arb_jastrow_ratio(const SysP<T>& S, const double* sR, i
     const double rn = min_image(C, sR[j] - nx, sR[N + j] - ny, sR[2 * N + j] - nz, -1.0, dx, dy, dz);
     if (ro < f.rcut) v[2] += functor_v(f, ro);
     if (rn < f.rcut) v[3] += functor_v(f, rn);
+  }
+  if (sMask != nullptr) {
+    // three-body terms (ThreeBodyJastrow.h:277-300, values only) of the electron at its old and new
+    // positions in FP64: ions within L of the position, electrons k within L of the ion (the ion
+    // masks of the current positions)
+    const int mw = S.mw;
+    for (int I = 0; I < M; ++I) {
+      const int sp = S.species[I];
+      const double L = S.j3_64[2 * sp].L;
+      if (!(L > 0.0)) continue;
+      const double ix = S.ions[I], iy = S.ions[M + I], iz = S.ions[2 * M + I];
+      const double roI = min_image(C, ix - x0, iy - y0, iz - z0, -1.0, dx, dy, dz);
+      const double rnI = min_image(C, ix - nx, iy - ny, iz - nz, -1.0, dx, dy, dz);
+      if (!(roI < L) && !(rnI < L)) continue;
+      const unsigned long long bit = 1ull << (I & 63);
+      const int wd = I >> 6;
+      for (int k = tid; k < N; k += nt) {
+        if (k == iel || !(sMask[(size_t)k * mw + wd] & bit)) continue;
+        const double kx = sR[k], ky = sR[N + k], kz = sR[2 * N + k];
+        const double rkI = min_image(C, ix - kx, iy - ky, iz - kz, -1.0, dx, dy, dz);
+        const Poly3<double>& f = S.j3_64[2 * sp + (((k < n) == up) ? 0 : 1)];
+        if (roI < L) v[2] += poly3_v(f, min_image(C, kx - x0, ky - y0, kz - z0, -1.0, dx, dy, dz), roI, rkI);
+        if (rnI < L) v[3] += poly3_v(f, min_image(C, kx - nx, ky - ny, kz - nz, -1.0, dx, dy, dz), rnI, rkI);
+      }
+    }
   }
   block_sum<double, 4>(v, red);
   return exp((v[0] - v[1]) + (v[2] - v[3]));
@@ -640,7 +670,7 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
   const int N = S.N, n = S.n, K = KD ? B.K : 1;
   const int KP = K + 1;   // LDS row stride of Binv: a column walk down its rows is free of bank conflicts
   const int Emax = a.chunk_len;   // electrons per chunk (the LDS layout is sized for it)
-  const MoveLds L = move_lds<T>(N, S.M, n, K, Emax, J3 ? S.mw : 0, ST);
+  const MoveLds L = move_lds<T>(N, S.M, n, K, Emax, J3 ? S.mw : 0, ST, ARB && J3);
   double* sR = (double*)(smem + L.oR);
   double* sRed = (double*)(smem + L.oRed);
   T* sInv = (T*)(smem + L.oInv);
@@ -1024,7 +1054,7 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
         double* ared = tv + n;
         const size_t pro = ((size_t)w * 2 + a.det) * n + r0;   // the chunk's first proposal
         // the stale rows: the LDS cache, or (streamed) Ainv's rows of the chunk in HBM
-        const double jr = arb_jastrow_ratio<T>(S, sR, iel, x0, y0, z0, nx, ny, nz, ared);
+        const double jr = arb_jastrow_ratio<T>(S, sR, iel, x0, y0, z0, nx, ny, nz, ared, J3 ? sMask : nullptr);
         rdet = arb_det_ratio(n, S.npad, row, r0, c, ST ? K : KP, ST ? n : sa, (const float*)Ainv, B.A64 + moff,
                              B.spo64 + pro * S.npad, (const float*)spo,
                              ST ? (const float*)Ainv + (size_t)r0 * n : (const float*)sAc, ST ? binv : sBinv, sRows,
