@@ -222,3 +222,76 @@ def test_j3_nio64_near_tie_band_vs_oracle():
                                  cond=8.0, setup=lambda b: b.set_accept_band(1e-2))
     _say(f"  re-evaluated moves {int(wb.arbitrations().sum())}")
     assert wb.delay_form() == 0
+
+
+@pytest.mark.parametrize("prec,delay,tol", [(qmcx.FP64, 1, 1e-10), (qmcx.FP64, 4, 1e-10), (qmcx.FP32, 4, 2e-3)])
+def test_j3_virtual_ratios_vs_oracle(cubic_j3, prec, delay, tol):
+    """WaveFunction::evaluateRatios with the three-body Jastrow (ThreeBodyJastrow.h:262-273) through
+    qmcx_flex_evaluate_ratios at the NonLocalPP quadrature points of every electron of 2 walkers,
+    after a Metropolis sweep (the ion masks and sums then come from the move kernel's accepts), against
+    the oracle.  Ratios within tol of max(|r|, 1)."""
+    from miniqmc_amd import nlpp
+    sysd, dev, orc = cubic_j3
+    _say("J3 virtual ratios, precision", prec, "delay", delay)
+    nw = 2
+    wb = qmcx.WalkerBatch(dev, nw, delay_rank=delay, precision=prec)
+    wb.set_positions(systems.initial_positions(sysd, nw, seed=4))
+    wb.evaluate_log()
+    wb.run(1, 6, 0.3, qmcx.ACCEPT_METROPOLIS, recompute_every=1)
+    R = wb.positions()
+    pp = nlpp.NonLocalPP(sysd, rmax=1.7, seed=3)
+    sphere = pp.randomize_walkers(nw)
+    worst, count, moved = 0.0, 0, 0.0
+    for jel in range(sysd.nel):
+        vpos, npair = pp.virtual_positions(R, jel, sphere)
+        if npair.max() == 0:
+            continue
+        got = wb.flex_evaluate_ratios(jel, vpos)
+        for w in range(nw):
+            ref = orc.evaluate_ratios(R[w], jel, vpos[w])
+            err = np.abs(got[w] - ref) / np.maximum(np.abs(ref), 1.0)
+            worst = max(worst, float(err.max()))
+            count += len(ref)
+            assert err.max() < tol, (jel, w, err.max(), got[w][:4], ref[:4])
+    _say(f"  {count} ratios, worst {worst:.2e}")
+    assert count > 100
+
+
+@pytest.mark.parametrize("prec,tol", [(qmcx.FP64, 1e-10), (qmcx.FP32, 2e-3)])
+def test_j3_nlpp_evaluate_vs_oracle(cubic_j3, prec, tol):
+    """qmcx_nlpp_evaluate with the three-body Jastrow (the reference's -j runs NonLocalPP::evaluate
+    with it, miniqmc.cpp:470-495): every work-unit shape, one rotation per walker, against the
+    oracle's evaluateRatios of each pair's knots.  Ratios within tol of max(|r|, 1)."""
+    from miniqmc_amd import nlpp
+    sysd, dev, orc = cubic_j3
+    nw, rmax = 2, 1.7
+    wb = qmcx.WalkerBatch(dev, nw, delay_rank=1, precision=prec)
+    wb.set_positions(systems.initial_positions(sysd, nw, seed=9))
+    wb.evaluate_log()
+    wb.run(1, 2, 0.3, qmcx.ACCEPT_METROPOLIS, recompute_every=1)
+    R = wb.positions()
+    for unit in (1, 2, 4, 6):
+        _say("J3 NonLocalPP, precision", prec, "knots per unit", unit)
+        wb.set_nlpp_unit(unit)
+        spheres = nlpp.NonLocalPP(sysd, rmax=rmax, seed=unit).randomize_walkers(nw)
+        rat, cnt = wb.nlpp_evaluate(rmax, spheres, max_pairs=4)
+        checked = 0
+        for w in range(nw):
+            iels, vps, slots = [], [], []
+            for jel in range(sysd.nel):
+                disp, dist = nlpp.electron_ion_displacements(sysd.lattice, sysd.ion_pos, R[w, jel])
+                if np.any(np.abs(dist - rmax) < 1e-5):
+                    continue
+                near = np.nonzero(dist < rmax)[0]
+                assert cnt[w, jel] == len(near)
+                for p, c in enumerate(near[:4]):
+                    iels.append(jel)
+                    vps.append(dist[c] * spheres[w] + disp[c] + R[w, jel])
+                    slots.append((jel, p))
+            ref = orc.evaluate_ratios_many(R[w], iels, np.array(vps))
+            for (jel, p), rr in zip(slots, ref):
+                err = np.abs(rat[w, jel, p] - rr) / np.maximum(np.abs(rr), 1.0)
+                assert err.max() < tol, (unit, w, jel, p, err.max())
+                checked += 1
+        _say(f"  {checked} pairs checked")
+        assert checked > 10
