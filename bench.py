@@ -413,8 +413,8 @@ def measure_nio32(mode, steps, err_steps, fp64_steps, seed, tau, walkers=256, de
 
 
 def measure_j3(mode, steps, err_steps, fp64_steps, seed, tau, walkers=256, delay=32, rmax=1.7):
-    """NiO-32 with the three-body Jastrow (the reference's -j), 256 walkers: the FP32 form at delay rank 32
-    (at NiO-64 the J3 pair slots leave room for k <= 16 only) with FP64
+    """NiO-32 with the three-body Jastrow (the reference's -j; at NiO-64 the FP64 path's walker state
+    with the J3 pair slots exceeds the LDS), 256 walkers: the FP32 form at delay rank 32 with FP64
     Slater rows and the per-sweep FP64 refresh, against the FP64 delay-1 GPU path from the same start
     and streams: walker-moves/s of both, log|Psi| error and identical accept sequences, and one
     NonLocalPP evaluation (FP32) with the J3 factor at every knot."""

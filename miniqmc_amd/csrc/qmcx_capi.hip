@@ -643,11 +643,9 @@ bool use_mfma_flush(const qmcx_batch_t b) {
 }
 
 // Electrons per chunk of a move launch: the delay rank (one flush per chunk), or up to 64 with
-// Sherman-Morrison updates (K = 1; 16 with a three-body Jastrow, 8 in FP64), which bounds the chunk's LDS
+// Sherman-Morrison updates (K = 1; 16 with a three-body Jastrow), which bounds the chunk's LDS
 // records.
-static int move_chunk(int n, int K, bool j3, bool fp64 = false) {
-  return K == 1 ? std::min(n, j3 ? (fp64 ? 8 : 16) : 64) : K;
-}
+static int move_chunk(int n, int K, bool j3) { return K == 1 ? std::min(n, j3 ? 16 : 64) : K; }
 
 // updateInvMat of the delayed rows of state slots [j0, j1) of determinant det, in order; thr > 1:
 // only the slots holding at least thr rows
@@ -689,7 +687,7 @@ int moves_det(qmcx_batch_t b, int det, int mode, bool flush) {
   int nt = 0;
   if (choose_shape(s, nt)) return QMCX_ERR_ARG;
   const int n = s->n;
-  const int chunk = move_chunk(n, b->K, b->j3, sizeof(T) == 8);
+  const int chunk = move_chunk(n, b->K, b->j3);
   const bool st = b->streamed();
   // the launch takes the FP64 re-evaluation's kernel (launch_moves_jpt) exactly when arb3 holds for
   // a three-body Jastrow: its LDS layout then keeps the re-evaluation apart from the J3 pair slots
@@ -1156,7 +1154,7 @@ static int check_flags(qmcx_batch_t b) {
   if (f & kFlagJ3Near)
     return fail(QMCX_ERR_STATE, "an electron position has more than " + std::to_string(kJ3Near) +
                                     " ions within their three-body Jastrow cutoff");
-  return fail(QMCX_ERR_STATE, "more than " + std::to_string(b->prec == QMCX_FP32 ? j3_slots<float>() : j3_slots<double>()) + " three-body Jastrow pairs in one wave of a move");
+  return fail(QMCX_ERR_STATE, "more than " + std::to_string(kJ3Slots) + " three-body Jastrow pairs in one wave of a move");
 }
 
 template <typename T>
@@ -1604,7 +1602,7 @@ int qmcx_batch_create(qmcx_batch_t* out, qmcx_system_t s, int nwalkers, int dela
       delete b;
       return rc;
     }
-    const int chunk = move_chunk(s->n, delay_rank, s->has_j3, precision == QMCX_FP64);
+    const int chunk = move_chunk(s->n, delay_rank, s->has_j3);
     const int mw = s->has_j3 ? s->mw : 0;
     auto lds_of = [&](bool st) {
       return precision == QMCX_FP32 ? move_lds<float>(s->N, s->nions, s->n, delay_rank, chunk, mw, st).total

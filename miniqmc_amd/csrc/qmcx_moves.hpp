@@ -36,13 +36,12 @@ struct MoveLds {
       oArb, oJ3, oJ3m, oJ3np, oJ3ne, oJ3k, oJ3v, oJ3r, oSt, total;
   bool binv_lds;   // streamed form: the capacitance inverse is in LDS (else in its HBM state slot)
 };
-// per-wave pair slots of one J3 move phase: codes [waves][j3_slots] and k-side terms
-// [waves][5][j3_slots]; 32 in FP32, 16 in FP64 (whose walker state needs the LDS elsewhere)
-template <typename T>
-__host__ __device__ constexpr int j3_slots() { return sizeof(T) == 4 ? 32 : 16; }
+// per-wave pair slots of one J3 move phase: codes [waves][kJ3Slots] and k-side terms
+// [waves][5][kJ3Slots]
+constexpr int kJ3Slots = 32;
 template <typename T>
 __host__ __device__ inline size_t j3_slots_lds(int waves) {
-  return (size_t)waves * j3_slots<T>() * (sizeof(int) + 5 * sizeof(T));
+  return (size_t)waves * kJ3Slots * (sizeof(int) + 5 * sizeof(T));
 }
 // LDS of the FP64 re-evaluation of a near-tie decision (FP32 form, K > 1): three n-vectors, 2K + 32
 // reduction slots and kArbGroups n-vectors of partial sums
@@ -103,7 +102,7 @@ __host__ __device__ inline MoveLds move_lds_at(int N, int n, int K, int E, int m
     unsigned p = o;
     const int waves = kMoveThreads / 64;
     L.oJ3k = lds_take(p, mw > 0 ? 2 * j3_slots_lds<T>(waves) : 0);   // new and old phase slots
-    L.oJ3v = L.oJ3k + (unsigned)(2 * sizeof(int) * waves * j3_slots<T>());
+    L.oJ3v = L.oJ3k + (unsigned)(2 * sizeof(int) * waves * kJ3Slots);
     L.oJ3r = lds_take(p, mw > 0 ? sizeof(double) * 16 * 5 : 0);    // j-side wave partials
     if (mw > 0) o = p;
     if (mw > 0 && arb3) L.oArb = o;
@@ -585,10 +584,10 @@ __device__ inline double arb_jastrow_ratio(const SysP<T>& S, const double* sR, i
 // ---- three-body Jastrow of one move phase (ThreeBodyJastrow::computeU3 with computeU3_engine,
 // ThreeBodyJastrow.h:441-625) for the moving electron iel at position p with near-ion record rec.
 // Every wave takes the pairs (k, ion q of rec) of the electrons k its own lanes hold (k = tid +
-// qq nt) whose ion mask has the ion, compacts them into its j3_slots slots (slot = ballot rank,
+// qq nt) whose ion mask has the ion, compacts them into its kJ3Slots slots (slot = ballot rank,
 // in (q, qq, lane) order), and lane r evaluates the functor for slot r: its j-side terms are
 // returned in js (zero for lanes without a slot), the k-side terms go to the slot's row of sv
-// [5][j3_slots] beside its code k | q << 16 in sk.  Returns the wave's slot count (more pairs
+// [5][kJ3Slots] beside its code k | q << 16 in sk.  Returns the wave's slot count (more pairs
 // than slots raise kFlagJ3Pairs).  Deterministic: the slot order depends only on the
 // configuration.
 template <typename T, int JPT>
@@ -609,13 +608,12 @@ __device__ inline int j3_phase(const SysP<T>& S, const Cell<T>& C, const double*
       const bool in = k < N && k != iel && (sMask[(size_t)k * mw + wd] & bit) != 0ull;
       const unsigned long long bal = __ballot(in);
       const int pos = ns + __popcll(bal & ((1ull << lane) - 1ull));
-      if (in && pos < j3_slots<T>()) sk[pos] = k | (q << 16);
+      if (in && pos < kJ3Slots) sk[pos] = k | (q << 16);
       ns += __popcll(bal);
     }
   }
-  constexpr int SL = j3_slots<T>();
-  if (lane == 0 && (ns > SL || cnt > kJ3Near)) atomicOr(flags, ns > SL ? kFlagJ3Pairs : kFlagJ3Near);
-  ns = ns < SL ? ns : SL;
+  if (lane == 0 && (ns > kJ3Slots || cnt > kJ3Near)) atomicOr(flags, ns > kJ3Slots ? kFlagJ3Pairs : kFlagJ3Near);
+  ns = ns < kJ3Slots ? ns : kJ3Slots;
   wave_lds_fence();   // the slots are read by other lanes of the wave
 #pragma unroll
   for (int c = 0; c < 5; ++c) js[c] = T(0);
@@ -632,7 +630,7 @@ __device__ inline int j3_phase(const SysP<T>& S, const Cell<T>& C, const double*
     poly3_vgl(S.j3[2 * S.species[I] + (((k < n) == (iel < n)) ? 0 : 1)], rjk, e[1], rkI, o);
     j3_pair_terms(o, jk, jI, kI, js, ks);
 #pragma unroll
-    for (int c = 0; c < 5; ++c) sv[c * j3_slots<T>() + lane] = ks[c];
+    for (int c = 0; c < 5; ++c) sv[c * kJ3Slots + lane] = ks[c];
   }
   return ns;
 }
@@ -645,7 +643,7 @@ __device__ inline void j3_apply(T* sJ3, int N, const int* sk, const T* sv, int n
     if (lane < ns && (sk[lane] >> 16) == q) {
       const int k = sk[lane] & 0xffff;
 #pragma unroll
-      for (int c = 0; c < 5; ++c) sJ3[(size_t)c * N + k] += sign * sv[c * j3_slots<T>() + lane];
+      for (int c = 0; c < 5; ++c) sJ3[(size_t)c * N + k] += sign * sv[c * kJ3Slots + lane];
     }
     wave_lds_fence();
   }
@@ -704,10 +702,10 @@ __device__ __forceinline__ void moves_body(const SysP<T>& S, const BatchP<T>& B,
   unsigned long long* sMask = (unsigned long long*)(smem + L.oJ3m);
   T* sJ3np = (T*)(smem + L.oJ3np);
   T* sJ3ne = (T*)(smem + L.oJ3ne);
-  int* sJk = (int*)(smem + L.oJ3k) + wid * j3_slots<T>();
-  int* sJkO = (int*)(smem + L.oJ3k) + (kWaves + wid) * j3_slots<T>();
-  T* sJv = (T*)(smem + L.oJ3v) + (size_t)wid * 5 * j3_slots<T>();
-  T* sJvO = (T*)(smem + L.oJ3v) + (size_t)(kWaves + wid) * 5 * j3_slots<T>();
+  int* sJk = (int*)(smem + L.oJ3k) + wid * kJ3Slots;
+  int* sJkO = (int*)(smem + L.oJ3k) + (kWaves + wid) * kJ3Slots;
+  T* sJv = (T*)(smem + L.oJ3v) + (size_t)wid * 5 * kJ3Slots;
+  T* sJvO = (T*)(smem + L.oJ3v) + (size_t)(kWaves + wid) * 5 * kJ3Slots;
   double* sJr = (double*)(smem + L.oJ3r);
   // FP64 re-evaluation of near ties (FP32 form, Metropolis): the move's inverse row in FP64
   const bool arb_on = ARB && sizeof(T) == 4 && KD && a.band > 0.0 && a.mode == 1 && a.decide == nullptr && n <= nt;
